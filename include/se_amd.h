@@ -1,0 +1,195 @@
+/*
+ * se_amd.h -- C ABI of the MI355X (gfx950) native spectral-element hot path.
+ *
+ * Drop-in boundary for davelee2804/SE's matrix-free Helmholtz/Poisson CG solve
+ * and semi-Lagrangian (SL) advection step.  Every entry point states the
+ * reference interface it replaces as  [ref: file:line].  Plain C types only:
+ * pointers, sizes, int status codes.  A context owns one HIP stream and all
+ * device memory it allocates; a context is used by one host thread at a time.
+ *
+ * Pointer conventions
+ *   *_dev entry points take device pointers that are already resident in HBM
+ *   and enqueue work on the context stream; plain entry points take host
+ *   pointers, stage through the context and return after completion.
+ *   Field arrays are node-major: field node v = J*nVerts[0] + I holds
+ *   nDofs consecutive doubles (the reference's Field::vals[v][dof]).
+ *   Vectors of unknowns use the reference's numbering (BCs::fieldToVecMap).
+ *
+ * Status codes: 0 = success, negative = error (text in se_last_error()).
+ */
+#ifndef SE_AMD_H
+#define SE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SE_OK = 0,
+    SE_ERR_ARG = -1,         /* invalid argument */
+    SE_ERR_HIP = -2,         /* HIP runtime failure */
+    SE_ERR_UNSUPPORTED = -3, /* configuration the reference rejects or leaves undefined */
+    SE_ERR_NOMEM = -4,       /* allocation failed / over a memory budget */
+    SE_ERR_STATE = -5        /* call out of order */
+};
+
+/* Laplacian Jacobian convention.  LITERAL reproduces the reference, which
+ * overwrites detJ with 1 in Laplacian::AssembleElement [ref:
+ * src/base/Operator.cpp:72]; PHYSICAL keeps the true detJ. */
+enum { SE_DETJ_LITERAL = 0, SE_DETJ_PHYSICAL = 1 };
+
+/* Solve precision.  FP64: plain FP64 Jacobi-PCG.  FP32_IR / BF16_IR: inner
+ * Jacobi-PCG in FP32 (resp. BF16 storage, FP32 arithmetic on MFMA) inside an
+ * FP64 iterative-refinement loop whose residual, norms and update are FP64. */
+enum { SE_PREC_FP64 = 0, SE_PREC_FP32_IR = 1, SE_PREC_BF16_IR = 2 };
+
+/* Solve status */
+enum {
+    SE_SOLVE_CONVERGED = 0, /* true ||b - A x||_2 <= rtol ||b||_2 */
+    SE_SOLVE_MAXIT = 1,     /* iteration limit reached */
+    SE_SOLVE_BREAKDOWN = 2, /* p.Ap <= 0: operator or preconditioner indefinite */
+    SE_SOLVE_NONFINITE = 3  /* NaN / Inf in the data */
+};
+
+typedef struct se_ctx se_ctx;
+typedef struct se_mesh se_mesh;
+typedef struct se_op se_op;
+
+const char* se_last_error(void);
+int se_version(void);
+
+/* ---- context -------------------------------------------------------------- */
+/* [ref: none -- the reference is a serial CPU code; PetscInitialize is the
+ *  closest analogue, models/qg-channel/BarotropicChannel.cpp:74] */
+int se_ctx_create(int device, se_ctx** out);
+int se_ctx_destroy(se_ctx* ctx);
+int se_ctx_synchronize(se_ctx* ctx);
+void* se_ctx_stream(se_ctx* ctx); /* the context's hipStream_t */
+
+/* ---- mesh and global numbering --------------------------------------------
+ * [ref: Mesh::Mesh src/base/Mesh.cpp:17-84 with elType "legendre";
+ *       BCs::BCs (scalar) src/base/BCs.cpp:14-173]
+ * dirichlet = {bottom, top, left, right}.  Builds vertex coordinates, the
+ * field<->unknown maps and the exact per-element Jacobian tables on device. */
+int se_mesh_create(se_ctx* ctx, const int nEls[2], int N, const double min[2], const double max[2],
+                   const int periodic[2], const int dirichlet[4], se_mesh** out);
+int se_mesh_destroy(se_mesh* mesh);
+
+typedef struct {
+    int N;
+    int nEls[2];
+    int nVerts[2];
+    int nV;        /* field nodes incl. periodic duplicates (Mesh::nVertsTotal) */
+    int nElsTotal;
+    int n;         /* unknowns (vecSize) */
+    int size0;     /* BCs::size[0] */
+} se_mesh_info;
+int se_mesh_get_info(const se_mesh* mesh, se_mesh_info* out);
+/* [ref: Legendre::Legendre src/base/Legendre.cpp:15-95] abcissa/weights N+1, dCij (N+1)^2 */
+int se_mesh_gll(const se_mesh* mesh, double* abcissa, double* weights, double* dCij);
+/* [ref: BCs::fieldToVecMap / vecToFieldMap src/base/BCs.h:8-9, Mesh::ElNodes
+ *  src/base/Mesh.cpp:138-152]  host outputs (nV, n, nElsTotal*(N+1)^2); any may be NULL */
+int se_mesh_maps(const se_mesh* mesh, int32_t* fieldToVec, int32_t* vecToField, int32_t* elNodes);
+/* [ref: Mesh::verts src/base/Mesh.cpp:63-73]  host output nV x 2 */
+int se_mesh_verts(const se_mesh* mesh, double* verts);
+/* device pointers to the resident maps / coordinates (owned by the mesh) */
+const int32_t* se_mesh_dev_field_to_vec(const se_mesh* mesh);
+const double* se_mesh_dev_verts(const se_mesh* mesh);
+
+/* ---- field <-> unknown vector ---------------------------------------------
+ * [ref: Vector::UpdateVec src/base/Vector.cpp:63-81] */
+int se_field_to_vec_dev(se_mesh* mesh, const double* d_field, double* d_vec);
+/* [ref: Vector::UpdateField src/base/Vector.cpp:40-61 + Field::PeriodicUpdate
+ *  src/base/Field.cpp:232-252]  Dirichlet nodes of d_field are left untouched. */
+int se_vec_to_field_dev(se_mesh* mesh, const double* d_vec, double* d_field);
+
+/* ---- Helmholtz / Poisson operator -----------------------------------------
+ * A = c_mass*M + c_lap*K~ on the unknowns, matrix-free.
+ * [ref: HelmholtzEqn::HelmholtzEqn src/linsys/HelmholtzEqn.cpp:21-82 with
+ *       c_mass = c, c_lap = nu*dt;  PoissonEqn::PoissonEqn
+ *       src/linsys/PoissonEqn.cpp:30-80 with c_mass = 0, c_lap = c;
+ *       element blocks Laplacian::AssembleElement Operator.cpp:63-84,
+ *       MassMatrix::AssembleElement Operator.cpp:157-177] */
+int se_helm_create(se_mesh* mesh, double c_mass, double c_lap, int detj_mode, se_op** out);
+int se_helm_destroy(se_op* op);
+/* y = A x   [ref: MatMult on the matrix built by Matrix::Assemble
+ *            src/base/Matrix.cpp:47-133] */
+int se_helm_apply(se_op* op, const double* x, double* y);
+int se_helm_apply_dev(se_op* op, const double* d_x, double* d_y);
+/* [ref: Matrix::InverseDiagonal src/base/Matrix.cpp:162-184]  host output n */
+int se_helm_inv_diag(se_op* op, double* dinv);
+/* Dirichlet lifting from the solution field's boundary values
+ * [ref: Matrix::Assemble lifting src/base/Matrix.cpp:101-124, run by
+ *  HelmholtzEqn::Assemble src/linsys/HelmholtzEqn.cpp:91-103]; host nV in */
+int se_helm_set_dirichlet(se_op* op, const double* sol_field);
+int se_helm_set_dirichlet_dev(se_op* op, const double* d_sol_field);
+int se_helm_lift(se_op* op, double* b); /* host output n */
+/* f = M*rhs_field + b  [ref: Vector::Assemble of FieldRHS src/base/Vector.cpp:89-121,
+ *  RHSOp.cpp:31-50, then VecAXPY HelmholtzEqn.cpp:111-118] */
+int se_helm_rhs(se_op* op, const double* rhs_field, double* f);
+int se_helm_rhs_dev(se_op* op, const double* d_rhs_field, double* d_f);
+
+typedef struct {
+    double rtol;        /* relative true-residual target (default 1e-10) */
+    int maxit;          /* iteration limit, inner iterations counted (default 100000) */
+    int precision;      /* SE_PREC_* (default FP64) */
+    int null_space;     /* 1: constant null space (periodic Poisson) */
+    int check_every;    /* iterations per host convergence check (default 16) */
+    double inner_rtol;  /* IR inner tolerance (0 = precision default) */
+    int max_outer;      /* IR outer step limit (default 50) */
+    double* history;    /* optional host array, maxit+1 recursive relative residuals (FP64 only) */
+} se_solve_opts;
+
+typedef struct {
+    int status;               /* SE_SOLVE_* */
+    int iters;                /* CG iterations (inner iterations for IR) */
+    int outer_iters;          /* IR outer steps (1 for FP64) */
+    double true_rel_residual; /* FP64 ||b - A x|| / ||b|| at exit */
+    double seconds;           /* device time of the solve */
+} se_solve_info;
+
+void se_solve_opts_default(se_solve_opts* opts);
+/* Jacobi-PCG on A x = f, x0 = 0  [ref: KSPSolve with -ksp_type cg -pc_type
+ *  jacobi, src/linsys/HelmholtzEqn.cpp:120-131] (sign-normalised: a
+ *  negative-definite A is solved as -A x = -f) */
+int se_helm_pcg_dev(se_op* op, const double* d_f, double* d_x, const se_solve_opts* opts, se_solve_info* info);
+int se_helm_pcg(se_op* op, const double* f, double* x, const se_solve_opts* opts, se_solve_info* info);
+/* full HelmholtzEqn::Solve: f = M rhs + b, solve, UpdateField into sol_field
+ * [ref: HelmholtzEqn::Solve src/linsys/HelmholtzEqn.cpp:105-136,
+ *       PoissonEqn::Solve src/linsys/PoissonEqn.cpp:102-134] */
+int se_helm_solve(se_op* op, const double* rhs_field, double* sol_field, const se_solve_opts* opts,
+                  se_solve_info* info);
+int se_helm_solve_dev(se_op* op, const double* d_rhs_field, double* d_sol_field, const se_solve_opts* opts,
+                      se_solve_info* info);
+
+/* ---- semi-Lagrangian advection ---------------------------------------------
+ * [ref: Advector::Advect / FirstOrder src/base/Advector.cpp:108-120,181-201,
+ *  IntegrateRK2 :136-148, PeriodicUpdate :122-134, Mesh::GetEl Mesh.cpp:106-137,
+ *  Field::InterpGlobal Field.cpp:57-76, Legendre::ShapeFuncsAtCoord
+ *  Legendre.cpp:145-169, bicubic Interpolate (N < 3) Advector.cpp:248-337]
+ * q, qSL: nV x nDofs (nDofs 1..3); vel: nV x 2; dep_el (nullable): nV. */
+int se_sl_advect(se_mesh* mesh, const double* q, int nDofs, const double* vel, double dt, double* qSL,
+                 int32_t* dep_el);
+int se_sl_advect_dev(se_mesh* mesh, const double* d_q, int nDofs, const double* d_vel, double dt,
+                     double* d_qSL, int32_t* d_dep_el);
+/* [ref: Advector::SecondOrder + ExtrapolateVelocities Advector.cpp:203-246,339-349] */
+int se_sl_advect2(se_mesh* mesh, const double* q, const double* qPrev, int nDofs, const double* vel,
+                  const double* velPrev, double dt, double* qSL);
+int se_sl_advect2_dev(se_mesh* mesh, const double* d_q, const double* d_qPrev, int nDofs, const double* d_vel,
+                      const double* d_velPrev, double dt, double* d_qSL);
+/* u = -dpsi/dy, v = dpsi/dx at every vertex  [ref: QuasiGeostrophicEqn::GenVel
+ *  src/qg/QuasiGeostrophicEqn.cpp:207-223 via Field::InterpDerivsGlobal
+ *  Field.cpp:96-118, Mesh::ShapeFuncDerivsAtCoord Mesh.cpp:252-285] */
+int se_genvel(se_mesh* mesh, const double* psi, double* vel);
+int se_genvel_dev(se_mesh* mesh, const double* d_psi, double* d_vel);
+/* [ref: CalcTimeStep src/base/Utils.cpp:23-40] */
+int se_calc_timestep(se_mesh* mesh, const double* vel, double courant, double* dt);
+int se_calc_timestep_dev(se_mesh* mesh, const double* d_vel, double courant, double* dt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SE_AMD_H */

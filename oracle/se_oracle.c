@@ -1,0 +1,1250 @@
+/*
+ * se_oracle.c -- CPU oracle (TEST INFRASTRUCTURE ONLY, never shipped or called
+ * by the product path).  Plain-C restatement of the reference SE hot path.
+ * Compile with -ffp-contract=off so that the operation order written here is
+ * the operation order executed (the reference is built by g++ for baseline
+ * x86-64, which has no FMA).
+ *
+ * Parity status per function (pins are in tests/test_oracle*.py):
+ *   orc_gll, mesh verts/maps, GetEl, GlobalToLocal, ShapeFuncsAtCoord,
+ *   InterpGlobal, DetJac, element Laplacian/Mass/FieldRHS, advect (1st/2nd),
+ *   GenVel, CalcTimeStep      -- pinned bit-exact against golden vectors from
+ *                                the compiled reference (tests/golden) plus
+ *                                closed forms.
+ *   CSR assembly, lifting     -- pinned against the element blocks above and
+ *                                the dense restriction A_full*u_D (closed form).
+ *   PCG, solve                -- PETSc KSPSolve is not buildable here: the
+ *                                algorithm is the build's definition (Jacobi-
+ *                                PCG, x0 = 0, true residual stop); pinned by
+ *                                direct dense solves and manufactured solutions.
+ *   QG steps                  -- compositions of the above; the diagonal mass
+ *                                solve of SolveQ is taken as exact division
+ *                                (parity unpinned for that one ulp).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "se_oracle.h"
+
+/* ------------------------------------------------------------------------- */
+/* a1: Legendre polynomials by the Jacobi three-term recurrence              */
+/* ------------------------------------------------------------------------- */
+
+/* Jacobi::Gen, Jacobi.cpp:21-36.  c has n+1 entries and starts zeroed. */
+static void jac_gen(double* c, int n, double a, const double* m1, int n1,
+                    const double* m2, int n2) {
+    double c1 = 2 * n * (2 * a + n) * (2 * a + 2 * n - 2);
+    double c2 = (2 * a + 2 * n - 1) * (2 * a + 2 * n) * (2 * a + 2 * n - 2);
+    double c3 = 2 * (a + n - 1) * (a + n - 1) * (2 * a + 2 * n);
+    int i;
+    c2 /= c1;
+    c3 /= c1;
+    for (i = 0; i <= n1; i++) c[i + 1] += c2 * m1[i];
+    for (i = 0; i <= n2; i++) c[i] -= c3 * m2[i];
+}
+
+/* Jacobi::Eval, Jacobi.cpp:38-46 (parity-definite: every other term). */
+double orc_leg_eval(const double* c, int n, double x) {
+    double p = 0.0;
+    int i;
+    for (i = n; i >= 0; i -= 2) p += c[i] * pow(x, i);
+    return p;
+}
+
+/* Jacobi::EvalDeriv, Jacobi.cpp:58-66 */
+double orc_leg_eval_deriv(const double* c, int n, double x) {
+    double dp = 0.0;
+    int i;
+    for (i = n; i >= 1; i -= 2) dp += c[i] * i * pow(x, i - 1);
+    return dp;
+}
+
+/* Jacobi::Eval2ndDeriv, Jacobi.cpp:68-76 */
+double orc_leg_eval_2nd(const double* c, int n, double x) {
+    double d2p = 0.0;
+    int i;
+    for (i = n; i >= 2; i -= 2) d2p += c[i] * i * (i - 1) * pow(x, i - 2);
+    return d2p;
+}
+
+/* RootFinder, Element.cpp:54-66 (Newton on P_N', tolerance 1e-12).  The
+ * iteration cap only guards against a non-converging start; it never
+ * triggers for the orders used here. */
+static double root_finder(double x0, const double* P, int N) {
+    double x = x0, xp;
+    int it;
+    for (it = 0; it < 100000; it++) {
+        xp = x;
+        x = xp - orc_leg_eval_deriv(P, N, xp) / orc_leg_eval_2nd(P, N, xp);
+        if (fabs(x - xp) < 1.0e-12) return x;
+    }
+    return x;
+}
+
+/* AlreadyFound, Element.cpp:68-77 */
+static int already_found(int n, const double* roots, double root) {
+    int i;
+    for (i = 0; i < n; i++)
+        if (fabs(roots[i] - root) < 1.0e-6) return 1;
+    return 0;
+}
+
+/* SortEntries, Element.cpp:79-106 (selection of the running minimum). */
+static void sort_entries(double* vals, int n) {
+    int i, j, minInd = 999999999;
+    int* done = (int*)calloc((size_t)n, sizeof(int));
+    double* tmp = (double*)malloc(sizeof(double) * (size_t)n);
+    double minVal;
+    for (i = 0; i < n; i++) {
+        minVal = 1.0e+99;
+        for (j = 0; j < n; j++) {
+            if (!done[j] && vals[j] < minVal) {
+                minVal = vals[j];
+                minInd = j;
+            }
+        }
+        tmp[i] = minVal;
+        done[minInd] = 1;
+    }
+    for (i = 0; i < n; i++) vals[i] = tmp[i];
+    free(tmp);
+    free(done);
+}
+
+/* Legendre::Legendre, Legendre.cpp:15-95.  Outputs abcissa[N+1],
+ * weights[N+1], dCij[(N+1)^2] (i = point, j = basis) and the N+1
+ * monomial coefficients of P_N.  Returns 0 on success. */
+int orc_gll(int N, double* abcissa, double* weights, double* dCij, double* pn) {
+    int i, j, endLoop;
+    double x0;
+    double** polys;
+    double* P;
+    if (N < 2) return -1; /* Legendre.cpp:58 reads polys[N-2] */
+    polys = (double**)malloc(sizeof(double*) * (size_t)(N + 2));
+    for (i = 0; i <= N + 1; i++) polys[i] = (double*)calloc((size_t)(i + 1), sizeof(double));
+    polys[0][0] = 1.0;
+    polys[1][1] = 1.0;
+    for (i = 2; i <= N + 1; i++) jac_gen(polys[i], i, 0.0, polys[i - 1], i - 1, polys[i - 2], i - 2);
+
+    abcissa[0] = -1.0;
+    endLoop = ((N + 1) % 2 == 1) ? N / 2 : (N + 1) / 2;
+    for (i = 1; i < endLoop; i++) {
+        x0 = abcissa[i - 1];
+        do {
+            x0 += 1.0e-3;
+            abcissa[i] = root_finder(x0, polys[N], N);
+        } while (already_found(i, abcissa, abcissa[i]) || abcissa[i] > 0.0 - 1.0e-3);
+        abcissa[N - i] = -abcissa[i];
+    }
+    abcissa[N] = 1.0;
+    if ((N + 1) % 2 == 1) abcissa[(N + 1) / 2] = 0.0;
+    sort_entries(abcissa, N + 1);
+
+    weights[0] = 2.0 / ((N + 1) * N);
+    for (i = 1; i < N; i++)
+        weights[i] = 2.0 / ((N + 1) * N * orc_leg_eval(polys[N], N, abcissa[i]) *
+                            orc_leg_eval(polys[N], N, abcissa[i]));
+    weights[N] = 2.0 / ((N + 1) * N);
+
+    /* the cardinal-function polynomial P is regenerated from polys[N-1],
+     * polys[N-2] exactly as polys[N] was (Legendre.cpp:71-72) */
+    P = (double*)calloc((size_t)(N + 1), sizeof(double));
+    jac_gen(P, N, 0.0, polys[N - 1], N - 1, polys[N - 2], N - 2);
+    for (i = 0; i <= N; i++) {
+        for (j = 0; j <= N; j++) {
+            double v;
+            if (i == j && i == 0) v = -0.25 * N * (N + 1);
+            else if (i == j && i == N) v = +0.25 * N * (N + 1);
+            else if (i == j) v = 0.0;
+            else v = orc_leg_eval(P, N, abcissa[i]) / (orc_leg_eval(P, N, abcissa[j]) * (abcissa[i] - abcissa[j]));
+            if (dCij) dCij[i * (N + 1) + j] = v;
+        }
+    }
+    if (pn) memcpy(pn, P, sizeof(double) * (size_t)(N + 1));
+    free(P);
+    for (i = 0; i <= N + 1; i++) free(polys[i]);
+    free(polys);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* a2/a3: mesh and global numbering                                          */
+/* ------------------------------------------------------------------------- */
+
+struct orc_mesh {
+    int N, nl;                 /* order, (N+1)^2 */
+    int nEls[2], nVerts[2], nV, nElsTotal;
+    int periodic[2];
+    int bc[4];                 /* effective bottom, top, left, right after BCs.cpp:39-44 */
+    double min[2], max[2], dx[2];
+    double *abcissa, *weights, *dCij, *pn;
+    double* verts;             /* nV x 2 */
+    int *f2v, *v2f;
+    int n, size0;
+};
+
+/* BCs::IsBCNode, BCs.cpp:349-367 */
+static int is_bc_node(const orc_mesh* m, int node) {
+    int t0 = node % m->nVerts[0], t1 = node / m->nVerts[0];
+    if (m->bc[0] && t1 == 0) return 1;
+    if (m->bc[1] && t1 == m->nVerts[1] - 1) return 1;
+    if (m->bc[2] && t0 == 0) return 1;
+    if (m->bc[3] && t0 == m->nVerts[0] - 1) return 1;
+    return 0;
+}
+
+/* BCs::BCs (scalar), BCs.cpp:14-173.  Returns 0, or -1 where the reference
+ * exits (inconsistent bcs), or -2 for the combinations the reference leaves
+ * unmapped (y-periodic only; x-periodic with a bottom wall only). */
+static int build_bcs(orc_mesh* m) {
+    int nv0 = m->nVerts[0], nv1 = m->nVerts[1], nV = m->nV;
+    int fieldNode, vecNode = 0, t0, t1;
+    int* bcb = &m->bc[0]; int* bct = &m->bc[1]; int* bcl = &m->bc[2]; int* bcr = &m->bc[3];
+    int size0 = 0;
+    if ((*bcb || *bct) && m->periodic[1]) return -1;
+    if ((*bcl || *bcr) && m->periodic[0]) return -1;
+    if (m->periodic[0]) *bcr = 1;
+    if (m->periodic[1]) *bct = 1;
+    if (*bcb) size0 += nv0;
+    if (*bct || m->periodic[1]) size0 += nv0;
+    if (*bcl) size0 += nv1;
+    if (*bcr || m->periodic[0]) size0 += nv1;
+    if (*bcb && *bcl) size0--;
+    if (*bcb && *bcr) size0--;
+    if (*bct && *bcl) size0--;
+    if (*bct && *bcr) size0--;
+    m->size0 = size0;
+    m->n = nV - size0;
+    m->v2f = (int*)malloc(sizeof(int) * (size_t)(m->n > 0 ? m->n : 1));
+    m->f2v = (int*)malloc(sizeof(int) * (size_t)nV);
+    if ((!m->periodic[0] && !m->periodic[1]) || (m->periodic[0] && m->periodic[1])) {
+        for (fieldNode = 0; fieldNode < nV; fieldNode++) {
+            t0 = fieldNode % nv0; t1 = fieldNode / nv0;
+            if (!is_bc_node(m, fieldNode)) {
+                m->v2f[vecNode] = fieldNode;
+                m->f2v[fieldNode] = vecNode++;
+            } else if (m->periodic[0] && m->periodic[1] && fieldNode == nV - 1) {
+                m->f2v[fieldNode] = 0;
+            } else if (m->periodic[0] && t0 == nv0 - 1) {
+                m->f2v[fieldNode] = t1 * (nv0 - 1);
+            } else if (m->periodic[1] && t1 == nv1 - 1) {
+                m->f2v[fieldNode] = t0;
+            } else {
+                m->f2v[fieldNode] = -1;
+            }
+        }
+    } else if (m->periodic[0] && (*bct && *bcb)) {
+        for (fieldNode = 0; fieldNode < nV; fieldNode++) {
+            t0 = fieldNode % nv0; t1 = fieldNode / nv0;
+            if (!is_bc_node(m, fieldNode)) {
+                m->v2f[vecNode] = fieldNode;
+                m->f2v[fieldNode] = vecNode++;
+            } else if (t1 == 0 || t1 == nv1 - 1) {
+                m->f2v[fieldNode] = -1;
+            } else if (t0 == nv0 - 1) {
+                m->f2v[fieldNode] = (t1 - 1) * (nv0 - 1);
+            } else {
+                return -1;
+            }
+        }
+    } else if (m->periodic[0] && *bct) {
+        for (fieldNode = 0; fieldNode < nV; fieldNode++) {
+            t0 = fieldNode % nv0; t1 = fieldNode / nv0;
+            if (!is_bc_node(m, fieldNode)) {
+                m->v2f[vecNode] = fieldNode;
+                m->f2v[fieldNode] = vecNode++;
+            } else if (t1 == nv1 - 1) {
+                m->f2v[fieldNode] = -1;
+            } else if (t0 == nv0 - 1) {
+                m->f2v[fieldNode] = t1 * (nv0 - 1);
+            } else {
+                return -1;
+            }
+        }
+    } else if (m->periodic[0]) {
+        for (fieldNode = 0; fieldNode < nV; fieldNode++) {
+            t0 = fieldNode % nv0; t1 = fieldNode / nv0;
+            if (!is_bc_node(m, fieldNode)) {
+                m->v2f[vecNode] = fieldNode;
+                m->f2v[fieldNode] = vecNode++;
+            } else if (t0 == nv0 - 1) {
+                m->f2v[fieldNode] = t1 * (nv0 - 1);
+            } else {
+                return -1;
+            }
+        }
+    } else {
+        return -2; /* y-periodic only: no branch of BCs.cpp:91-172 assigns the map */
+    }
+    return 0;
+}
+
+/* Mesh::Mesh, Mesh.cpp:17-73 (legendre elements) + BCs */
+orc_mesh* orc_mesh_create(const int* nEls, int N, const double* min, const double* max,
+                          const int* periodic, const int* bc, int* status) {
+    orc_mesh* m = (orc_mesh*)calloc(1, sizeof(orc_mesh));
+    int d, v, rc;
+    m->N = N;
+    m->nl = (N + 1) * (N + 1);
+    m->abcissa = (double*)malloc(sizeof(double) * (size_t)(N + 1));
+    m->weights = (double*)malloc(sizeof(double) * (size_t)(N + 1));
+    m->dCij = (double*)malloc(sizeof(double) * (size_t)m->nl);
+    m->pn = (double*)malloc(sizeof(double) * (size_t)(N + 1));
+    if (orc_gll(N, m->abcissa, m->weights, m->dCij, m->pn) != 0) {
+        if (status) *status = -3;
+        orc_mesh_destroy(m);
+        return NULL;
+    }
+    for (d = 0; d < 2; d++) {
+        m->nEls[d] = nEls[d];
+        m->nVerts[d] = nEls[d] * N + 1;
+        m->min[d] = min[d];
+        m->max[d] = max[d];
+        m->periodic[d] = periodic[d] ? 1 : 0;
+        m->dx[d] = (max[d] - min[d]) / nEls[d];
+    }
+    for (d = 0; d < 4; d++) m->bc[d] = bc[d] ? 1 : 0;
+    m->nElsTotal = m->nEls[0] * m->nEls[1];
+    m->nV = m->nVerts[0] * m->nVerts[1];
+    m->verts = (double*)malloc(sizeof(double) * 2 * (size_t)m->nV);
+    for (v = 0; v < m->nV; v++) {
+        int t0 = v % m->nVerts[0], t1 = v / m->nVerts[0];
+        int ei = t0 / N, ej = t1 / N;
+        double l0 = m->abcissa[t0 % N], l1 = m->abcissa[t1 % N];
+        m->verts[2 * v + 0] = m->min[0] + ei * m->dx[0] + ((l0 + 1.0) / 2.0) * m->dx[0];
+        m->verts[2 * v + 1] = m->min[1] + ej * m->dx[1] + ((l1 + 1.0) / 2.0) * m->dx[1];
+    }
+    rc = build_bcs(m);
+    if (status) *status = rc;
+    if (rc != 0) {
+        orc_mesh_destroy(m);
+        return NULL;
+    }
+    return m;
+}
+
+void orc_mesh_destroy(orc_mesh* m) {
+    if (!m) return;
+    free(m->abcissa); free(m->weights); free(m->dCij); free(m->pn);
+    free(m->verts); free(m->f2v); free(m->v2f);
+    free(m);
+}
+
+void orc_mesh_sizes(const orc_mesh* m, int* out) {
+    out[0] = m->nVerts[0]; out[1] = m->nVerts[1]; out[2] = m->nV; out[3] = m->nElsTotal;
+    out[4] = m->n; out[5] = m->size0; out[6] = m->N;
+}
+
+void orc_mesh_verts(const orc_mesh* m, double* verts) {
+    memcpy(verts, m->verts, sizeof(double) * 2 * (size_t)m->nV);
+}
+
+void orc_mesh_maps(const orc_mesh* m, int* f2v, int* v2f) {
+    if (f2v) memcpy(f2v, m->f2v, sizeof(int) * (size_t)m->nV);
+    if (v2f) memcpy(v2f, m->v2f, sizeof(int) * (size_t)m->n);
+}
+
+void orc_mesh_gll(const orc_mesh* m, double* abcissa, double* weights, double* dCij) {
+    if (abcissa) memcpy(abcissa, m->abcissa, sizeof(double) * (size_t)(m->N + 1));
+    if (weights) memcpy(weights, m->weights, sizeof(double) * (size_t)(m->N + 1));
+    if (dCij) memcpy(dCij, m->dCij, sizeof(double) * (size_t)m->nl);
+}
+
+/* Mesh::ElNodes, Mesh.cpp:138-152 */
+void orc_el_nodes(const orc_mesh* m, int el, int* nodes) {
+    int N = m->N, ii, jj, k = 0;
+    int t0 = N * (el % m->nEls[0]), t1 = N * (el / m->nEls[0]);
+    int n0 = t1 * m->nVerts[0] + t0;
+    for (jj = 0; jj < N + 1; jj++)
+        for (ii = 0; ii < N + 1; ii++) nodes[k++] = n0 + jj * m->nVerts[0] + ii;
+}
+
+#define NUMAPPROX(var, val) ((var) >= (val) - 1.0e-12 && (var) <= (val) + 1.0e-12)
+
+/* Mesh::GetEl, Mesh.cpp:106-137 */
+void orc_get_el(const orc_mesh* m, const double* coord, int* el) {
+    int topo[2], d;
+    double out, frac, integer;
+    for (d = 0; d < 2; d++) {
+        if (NUMAPPROX(coord[d] - m->max[d], 0.0)) {
+            topo[d] = m->nEls[d] - 1;
+        } else {
+            out = (coord[d] - m->min[d]) / m->dx[d];
+            frac = modf(out, &integer);
+            topo[d] = (int)integer;
+            if (topo[d] > 0 && NUMAPPROX(frac, 0.0)) topo[d]--;
+        }
+    }
+    *el = topo[1] * m->nEls[0] + topo[0];
+}
+
+/* Mesh::GlobalToLocal, Mesh.cpp:163-177 */
+void orc_global_to_local(const orc_mesh* m, const double* g, int* el, double* l) {
+    int d;
+    orc_get_el(m, g, el);
+    l[0] = (g[0] - m->min[0] - (*el % m->nEls[0]) * m->dx[0]) * (2.0 / m->dx[0]) - 1.0;
+    l[1] = (g[1] - m->min[1] - (*el / m->nEls[0]) * m->dx[1]) * (2.0 / m->dx[1]) - 1.0;
+    for (d = 0; d < 2; d++) {
+        if (l[d] < -1.0) l[d] = -1;
+        if (l[d] > +1.0) l[d] = +1;
+    }
+}
+
+/* Legendre::IsRoot, Legendre.cpp:291-305 (EPS 1e-8) */
+static int is_root(const orc_mesh* m, double x, int* i) {
+    int k;
+    for (k = 0; k <= m->N; k++) {
+        if (fabs(x - m->abcissa[k]) < 1.0e-8) {
+            *i = k;
+            return 1;
+        }
+    }
+    *i = -1;
+    return 0;
+}
+
+/* Legendre::ShapeFuncsAtCoord, Legendre.cpp:145-169 */
+void orc_shape_funcs_at_coord(const orc_mesh* m, const double* coord, double* Ni) {
+    int N = m->N, pt_j, x_i, y_i, x_j, y_j;
+    double Nx, Ny;
+    for (pt_j = 0; pt_j < m->nl; pt_j++) {
+        x_j = pt_j % (N + 1);
+        y_j = pt_j / (N + 1);
+        if (is_root(m, coord[0], &x_i)) {
+            Nx = (x_i == x_j) ? 1.0 : 0.0;
+        } else {
+            Nx = -((1.0 - coord[0] * coord[0]) /
+                   (N * (N + 1) * orc_leg_eval(m->pn, N, m->abcissa[x_j]) * (coord[0] - m->abcissa[x_j]))) *
+                 orc_leg_eval_deriv(m->pn, N, coord[0]);
+        }
+        if (is_root(m, coord[1], &y_i)) {
+            Ny = (y_i == y_j) ? 1.0 : 0.0;
+        } else {
+            Ny = -((1.0 - coord[1] * coord[1]) /
+                   (N * (N + 1) * orc_leg_eval(m->pn, N, m->abcissa[y_j]) * (coord[1] - m->abcissa[y_j]))) *
+                 orc_leg_eval_deriv(m->pn, N, coord[1]);
+        }
+        Ni[pt_j] = Nx * Ny;
+    }
+}
+
+/* Field::InterpLocal, Field.cpp:57-68 (vals is nV x nDofs, node-major) */
+static void interp_local(const orc_mesh* m, const double* vals, int nDofs, int el,
+                         const double* l, double* out, double* Ni, int* nodes) {
+    int node, d;
+    orc_shape_funcs_at_coord(m, l, Ni);
+    orc_el_nodes(m, el, nodes);
+    for (d = 0; d < nDofs; d++) out[d] = 0.0;
+    for (node = 0; node < m->nl; node++)
+        for (d = 0; d < nDofs; d++) out[d] += Ni[node] * vals[(size_t)nodes[node] * nDofs + d];
+}
+
+/* Field::InterpGlobal, Field.cpp:70-76 */
+void orc_interp_global(const orc_mesh* m, const double* vals, int nDofs, const double* g, double* out) {
+    int el;
+    double l[2];
+    double* Ni = (double*)malloc(sizeof(double) * (size_t)m->nl);
+    int* nodes = (int*)malloc(sizeof(int) * (size_t)m->nl);
+    orc_global_to_local(m, g, &el, l);
+    interp_local(m, vals, nDofs, el, l, out, Ni, nodes);
+    free(Ni);
+    free(nodes);
+}
+
+/* Legendre::ShapeFuncDerivs, Legendre.cpp:171-184 (GNix is 2 x nl) */
+static void ref_shape_func_derivs(const orc_mesh* m, int pt, double* GNix) {
+    int N = m->N, pt_j;
+    int x_i = pt % (N + 1), y_i = pt / (N + 1);
+    for (pt_j = 0; pt_j < m->nl; pt_j++) {
+        int x_j = pt_j % (N + 1), y_j = pt_j / (N + 1);
+        GNix[pt_j] = (y_i == y_j) ? m->dCij[x_i * (N + 1) + x_j] : 0.0;
+        GNix[m->nl + pt_j] = (x_i == x_j) ? m->dCij[y_i * (N + 1) + y_j] : 0.0;
+    }
+}
+
+/* Jacobian from vertex coordinates, Mesh.cpp:183-198 / 217-250 */
+static void jacobian(const orc_mesh* m, int el, const double* GNix, double J[2][2], int* nodes) {
+    int node;
+    orc_el_nodes(m, el, nodes);
+    J[0][0] = J[0][1] = J[1][0] = J[1][1] = 0.0;
+    for (node = 0; node < m->nl; node++) {
+        const double* vx = &m->verts[2 * (size_t)nodes[node]];
+        J[0][0] += GNix[node] * vx[0];
+        J[0][1] += GNix[node] * vx[1];
+        J[1][0] += GNix[m->nl + node] * vx[0];
+        J[1][1] += GNix[m->nl + node] * vx[1];
+    }
+}
+
+/* Mesh::DetJac, Mesh.cpp:183-198 */
+double orc_det_jac(const orc_mesh* m, int el, int pt) {
+    double J[2][2];
+    double* GNix = (double*)malloc(sizeof(double) * 2 * (size_t)m->nl);
+    int* nodes = (int*)malloc(sizeof(int) * (size_t)m->nl);
+    ref_shape_func_derivs(m, pt, GNix);
+    jacobian(m, el, GNix, J, nodes);
+    free(GNix);
+    free(nodes);
+    return J[0][0] * J[1][1] - J[0][1] * J[1][0];
+}
+
+/* global derivatives from local ones through the inverted Jacobian,
+ * Mesh.cpp:227-249 */
+static void global_derivs(const orc_mesh* m, int el, const double* GNix, double* gGNix,
+                          double* detJac, int* nodes) {
+    double J[2][2], temp;
+    int d, dj, node;
+    jacobian(m, el, GNix, J, nodes);
+    *detJac = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    temp = J[0][0];
+    J[0][0] = J[1][1] / (*detJac);
+    J[1][1] = temp / (*detJac);
+    J[0][1] = -J[0][1] / (*detJac);
+    J[1][0] = -J[1][0] / (*detJac);
+    for (d = 0; d < 2; d++) {
+        for (node = 0; node < m->nl; node++) {
+            temp = 0.0;
+            for (dj = 0; dj < 2; dj++) temp += GNix[dj * m->nl + node] * J[d][dj];
+            gGNix[d * m->nl + node] = temp;
+        }
+    }
+}
+
+/* Mesh::ShapeFuncDerivs, Mesh.cpp:217-250 */
+void orc_shape_func_derivs(const orc_mesh* m, int el, int pt, double* gGNix, double* detJac) {
+    double* GNix = (double*)malloc(sizeof(double) * 2 * (size_t)m->nl);
+    int* nodes = (int*)malloc(sizeof(int) * (size_t)m->nl);
+    ref_shape_func_derivs(m, pt, GNix);
+    global_derivs(m, el, GNix, gGNix, detJac, nodes);
+    free(GNix);
+    free(nodes);
+}
+
+/* ------------------------------------------------------------------------- */
+/* a5-a7: element operators                                                  */
+/* ------------------------------------------------------------------------- */
+
+/* Laplacian::AssembleElement, Operator.cpp:63-84.  Adds into K (nl x nl). */
+void orc_elem_laplacian(const orc_mesh* m, int el, double c, int detj_mode, double* K) {
+    int nl = m->nl, pt, r, col;
+    double* gGNix = (double*)malloc(sizeof(double) * 2 * (size_t)nl);
+    for (pt = 0; pt < nl; pt++) {
+        int px = pt % (m->N + 1), py = pt / (m->N + 1);
+        double weight = m->weights[px] * m->weights[py];
+        double detJac, w;
+        orc_shape_func_derivs(m, el, pt, gGNix, &detJac);
+        if (detj_mode == 0) detJac = 1.0;
+        w = detJac * weight * c;
+        for (r = 0; r < nl; r++)
+            for (col = 0; col < nl; col++)
+                K[r * nl + col] += w * (gGNix[r] * gGNix[col] + gGNix[nl + r] * gGNix[nl + col]);
+    }
+    free(gGNix);
+}
+
+/* MassMatrix::AssembleElement, Operator.cpp:157-177 (GLL collocation: the
+ * shape functions at quadrature points are Kronecker deltas). */
+void orc_elem_mass(const orc_mesh* m, int el, double c, double* M) {
+    int nl = m->nl, pt, r, col;
+    for (pt = 0; pt < nl; pt++) {
+        int px = pt % (m->N + 1), py = pt / (m->N + 1);
+        double weight = m->weights[px] * m->weights[py];
+        double detJac = orc_det_jac(m, el, pt);
+        for (r = 0; r < nl; r++)
+            for (col = 0; col < nl; col++) {
+                double Nr = (r == pt) ? 1.0 : 0.0, Nc = (col == pt) ? 1.0 : 0.0;
+                M[r * nl + col] += detJac * weight * c * Nr * Nc;
+            }
+    }
+}
+
+/* FieldRHS::AssembleElement, RHSOp.cpp:31-50 (nDofs = 1).  Adds into rhs. */
+void orc_elem_field_rhs(const orc_mesh* m, int el, double c, const double* field, double* rhs) {
+    int nl = m->nl, N = m->N, pt, node;
+    double* Ni = (double*)malloc(sizeof(double) * (size_t)nl);
+    int* nodes = (int*)malloc(sizeof(int) * (size_t)nl);
+    for (pt = 0; pt < nl; pt++) {
+        int px = pt % (N + 1), py = pt / (N + 1);
+        double coord[2] = {m->abcissa[px], m->abcissa[py]};
+        double weight = m->weights[px] * m->weights[py];
+        double detJac = orc_det_jac(m, el, pt), f, w;
+        interp_local(m, field, 1, el, coord, &f, Ni, nodes);
+        w = detJac * weight * c;
+        for (node = 0; node < nl; node++) {
+            double Nn = (node == pt) ? 1.0 : 0.0;
+            rhs[node] += w * f * Nn;
+        }
+    }
+    free(Ni);
+    free(nodes);
+}
+
+/* ------------------------------------------------------------------------- */
+/* a8-a10: assembled operator (PETSc AIJ pattern of dense element blocks)    */
+/* ------------------------------------------------------------------------- */
+
+struct orc_helm {
+    const orc_mesh* m;
+    double c_mass, c_lap;
+    int detj_mode;
+    long nnz;
+    long* rowptr;
+    int* cols;
+    double* vals;
+    double* b;        /* Dirichlet lifting, HelmholtzEqn::Assemble */
+    double* dinv;     /* Matrix::InverseDiagonal */
+    int assembled;
+};
+
+orc_helm* orc_helm_create(const orc_mesh* m, double c_mass, double c_lap, int detj_mode) {
+    orc_helm* h = (orc_helm*)calloc(1, sizeof(orc_helm));
+    h->m = m;
+    h->c_mass = c_mass;
+    h->c_lap = c_lap;
+    h->detj_mode = detj_mode;
+    return h;
+}
+
+void orc_helm_destroy(orc_helm* h) {
+    if (!h) return;
+    free(h->rowptr); free(h->cols); free(h->vals); free(h->b); free(h->dinv);
+    free(h);
+}
+
+static int cmp_int(const void* a, const void* b) {
+    int x = *(const int*)a, y = *(const int*)b;
+    return (x > y) - (x < y);
+}
+
+/* element block: ops applied in the reference order {mass, laplacian}
+ * (HelmholtzEqn.cpp:69-81); PoissonEqn has the Laplacian only. */
+static void elem_block(const orc_helm* h, int el, double* Ae) {
+    memset(Ae, 0, sizeof(double) * (size_t)h->m->nl * (size_t)h->m->nl);
+    if (h->c_mass != 0.0) orc_elem_mass(h->m, el, h->c_mass, Ae);
+    orc_elem_laplacian(h->m, el, h->c_lap, h->detj_mode, Ae);
+}
+
+/* Matrix::Assemble, Matrix.cpp:47-133 + HelmholtzEqn::Assemble :91-103.
+ * The stored pattern is the union of dense element blocks (MatSetValues of the
+ * whole block, zeros included); values accumulate in element order. */
+void orc_helm_assemble(orc_helm* h, const double* sol_field) {
+    const orc_mesh* m = h->m;
+    int nl = m->nl, n = m->n, el, i, j;
+    int* nodes = (int*)malloc(sizeof(int) * (size_t)nl);
+    int* rn = (int*)malloc(sizeof(int) * (size_t)nl);
+    double* Ae = (double*)malloc(sizeof(double) * (size_t)nl * (size_t)nl);
+    double* rhs = (double*)malloc(sizeof(double) * (size_t)nl);
+    long* cnt = (long*)calloc((size_t)n + 1, sizeof(long));
+    long k;
+    int* tmp;
+    free(h->rowptr); free(h->cols); free(h->vals); free(h->b); free(h->dinv);
+    /* pattern: count, fill, sort, unique */
+    for (el = 0; el < m->nElsTotal; el++) {
+        int nv = 0;
+        orc_el_nodes(m, el, nodes);
+        for (i = 0; i < nl; i++) if (m->f2v[nodes[i]] >= 0) nv++;
+        for (i = 0; i < nl; i++) { int r = m->f2v[nodes[i]]; if (r >= 0) cnt[r + 1] += nv; }
+    }
+    for (i = 0; i < n; i++) cnt[i + 1] += cnt[i];
+    tmp = (int*)malloc(sizeof(int) * (size_t)(cnt[n] > 0 ? cnt[n] : 1));
+    {
+        long* pos = (long*)malloc(sizeof(long) * (size_t)(n + 1));
+        memcpy(pos, cnt, sizeof(long) * (size_t)(n + 1));
+        for (el = 0; el < m->nElsTotal; el++) {
+            orc_el_nodes(m, el, nodes);
+            for (i = 0; i < nl; i++) {
+                int r = m->f2v[nodes[i]];
+                if (r < 0) continue;
+                for (j = 0; j < nl; j++) {
+                    int cidx = m->f2v[nodes[j]];
+                    if (cidx >= 0) tmp[pos[r]++] = cidx;
+                }
+            }
+        }
+        free(pos);
+    }
+    h->rowptr = (long*)malloc(sizeof(long) * (size_t)(n + 1));
+    h->rowptr[0] = 0;
+    k = 0;
+    for (i = 0; i < n; i++) {
+        long s = cnt[i], e = cnt[i + 1], t;
+        qsort(tmp + s, (size_t)(e - s), sizeof(int), cmp_int);
+        for (t = s; t < e; t++)
+            if (t == s || tmp[t] != tmp[t - 1]) tmp[k++] = tmp[t];
+        h->rowptr[i + 1] = k;
+    }
+    h->nnz = k;
+    h->cols = (int*)malloc(sizeof(int) * (size_t)(k > 0 ? k : 1));
+    memcpy(h->cols, tmp, sizeof(int) * (size_t)k);
+    free(tmp);
+    free(cnt);
+    h->vals = (double*)calloc((size_t)(k > 0 ? k : 1), sizeof(double));
+    h->b = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    /* values (ADD_VALUES in element order) + lifting (Matrix.cpp:101-124) */
+    for (el = 0; el < m->nElsTotal; el++) {
+        int addToVec = 0;
+        elem_block(h, el, Ae);
+        orc_el_nodes(m, el, nodes);
+        for (i = 0; i < nl; i++) rn[i] = m->f2v[nodes[i]];
+        for (i = 0; i < nl; i++) {
+            int r = rn[i];
+            long s, e;
+            if (r < 0) continue;
+            s = h->rowptr[r];
+            e = h->rowptr[r + 1];
+            for (j = 0; j < nl; j++) {
+                int cidx = rn[j];
+                long lo = s, hi = e - 1;
+                if (cidx < 0) continue;
+                while (lo < hi) {
+                    long mid = (lo + hi) / 2;
+                    if (h->cols[mid] < cidx) lo = mid + 1; else hi = mid;
+                }
+                h->vals[lo] += Ae[i * nl + j];
+            }
+        }
+        memset(rhs, 0, sizeof(double) * (size_t)nl);
+        for (j = 0; j < nl; j++) {
+            if (rn[j] == -1) {
+                addToVec = 1;
+                for (i = 0; i < nl; i++) rhs[i] -= sol_field[nodes[j]] * Ae[i * nl + j];
+            }
+        }
+        if (addToVec)
+            for (i = 0; i < nl; i++)
+                if (rn[i] >= 0) h->b[rn[i]] += rhs[i];
+    }
+    /* inverse diagonal, Matrix.cpp:162-184 */
+    h->dinv = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    for (i = 0; i < n; i++) {
+        long t;
+        double d = 0.0;
+        for (t = h->rowptr[i]; t < h->rowptr[i + 1]; t++)
+            if (h->cols[t] == i) { d = h->vals[t]; break; }
+        h->dinv[i] = 1.0 / d;
+    }
+    h->assembled = 1;
+    free(nodes); free(rn); free(Ae); free(rhs);
+}
+
+long orc_helm_nnz(const orc_helm* h) { return h->nnz; }
+
+void orc_helm_csr(const orc_helm* h, long* rowptr, int* cols, double* vals) {
+    if (rowptr) memcpy(rowptr, h->rowptr, sizeof(long) * (size_t)(h->m->n + 1));
+    if (cols) memcpy(cols, h->cols, sizeof(int) * (size_t)h->nnz);
+    if (vals) memcpy(vals, h->vals, sizeof(double) * (size_t)h->nnz);
+}
+
+void orc_helm_lift(const orc_helm* h, double* b) { memcpy(b, h->b, sizeof(double) * (size_t)h->m->n); }
+void orc_helm_inv_diag(const orc_helm* h, double* d) { memcpy(d, h->dinv, sizeof(double) * (size_t)h->m->n); }
+
+/* MatMult_SeqAIJ: y_i = sum over the sorted row of a_ij x_j */
+void orc_helm_matvec(const orc_helm* h, const double* x, double* y) {
+    int i, n = h->m->n;
+    for (i = 0; i < n; i++) {
+        double s = 0.0;
+        long t;
+        for (t = h->rowptr[i]; t < h->rowptr[i + 1]; t++) s += h->vals[t] * x[h->cols[t]];
+        y[i] = s;
+    }
+}
+
+/* Vector::Assemble of FieldRHS(c = 1) (Vector.cpp:89-121) followed by
+ * VecAXPY(f, 1, b) (HelmholtzEqn.cpp:111-118). */
+void orc_helm_rhs(const orc_helm* h, const double* rhs_field, double* f) {
+    const orc_mesh* m = h->m;
+    int nl = m->nl, el, i, n = m->n;
+    int* nodes = (int*)malloc(sizeof(int) * (size_t)nl);
+    double* re = (double*)malloc(sizeof(double) * (size_t)nl);
+    memset(f, 0, sizeof(double) * (size_t)n);
+    for (el = 0; el < m->nElsTotal; el++) {
+        memset(re, 0, sizeof(double) * (size_t)nl);
+        orc_elem_field_rhs(m, el, 1.0, rhs_field, re);
+        orc_el_nodes(m, el, nodes);
+        for (i = 0; i < nl; i++) {
+            int r = m->f2v[nodes[i]];
+            if (r >= 0) f[r] += re[i];
+        }
+    }
+    for (i = 0; i < n; i++) f[i] = f[i] + 1.0 * h->b[i];
+    free(nodes);
+    free(re);
+}
+
+static double dot(const double* a, const double* b, int n) {
+    double s = 0.0;
+    int i;
+    for (i = 0; i < n; i++) s += a[i] * b[i];
+    return s;
+}
+
+static void remove_mean(double* v, int n) {
+    double s = 0.0;
+    int i;
+    for (i = 0; i < n; i++) s += v[i];
+    s /= n;
+    for (i = 0; i < n; i++) v[i] -= s;
+}
+
+/* Jacobi-preconditioned CG, the build's definition of the reference's
+ * KSPSolve(cg, jacobi) (parity of PETSc internals unpinned):
+ *   sign-normalised to an SPD system (the QG operators are negative definite),
+ *   x0 = 0, null space = constant vector (b and every preconditioned residual
+ *   projected to zero Euclidean mean), stop when the recursive residual
+ *   satisfies ||r|| <= rtol ||b|| AND the recomputed true residual does too
+ *   (otherwise the true residual replaces r and iteration continues). */
+int orc_helm_pcg(const orc_helm* h, const double* f, double* x, double rtol, int maxit,
+                 int null_space, int* iters, double* true_relres, double* hist) {
+    int n = h->m->n, i, k, status = 1;
+    double sgn = (n > 0 && h->dinv[0] < 0.0) ? -1.0 : 1.0;
+    double *bb = (double*)malloc(sizeof(double) * (size_t)n), *r = (double*)malloc(sizeof(double) * (size_t)n);
+    double *z = (double*)malloc(sizeof(double) * (size_t)n), *p = (double*)malloc(sizeof(double) * (size_t)n);
+    double *q = (double*)malloc(sizeof(double) * (size_t)n);
+    double bnorm, rho, rnorm, tnorm = 0.0;
+    for (i = 0; i < n; i++) {
+        if ((h->dinv[i] < 0.0) != (sgn < 0.0)) { status = 2; goto done_early; }
+        bb[i] = sgn * f[i];
+        x[i] = 0.0;
+    }
+    if (null_space) remove_mean(bb, n);
+    bnorm = sqrt(dot(bb, bb, n));
+    if (!isfinite(bnorm)) { status = 3; goto done_early; }
+    if (hist) hist[0] = 1.0;
+    if (bnorm == 0.0) { status = 0; k = 0; tnorm = 0.0; goto done; }
+    for (i = 0; i < n; i++) { r[i] = bb[i]; z[i] = sgn * h->dinv[i] * r[i]; }
+    if (null_space) remove_mean(z, n);
+    memcpy(p, z, sizeof(double) * (size_t)n);
+    rho = dot(r, z, n);
+    for (k = 1; k <= maxit; k++) {
+        double pq, alpha, rho_new, beta;
+        orc_helm_matvec(h, p, q);
+        for (i = 0; i < n; i++) q[i] *= sgn;
+        pq = dot(p, q, n);
+        if (!(pq > 0.0)) { status = isfinite(pq) ? 2 : 3; break; }
+        alpha = rho / pq;
+        for (i = 0; i < n; i++) { x[i] += alpha * p[i]; r[i] -= alpha * q[i]; }
+        rnorm = sqrt(dot(r, r, n));
+        if (hist) hist[k] = rnorm / bnorm;
+        if (!isfinite(rnorm)) { status = 3; break; }
+        if (rnorm <= rtol * bnorm) {
+            orc_helm_matvec(h, x, q);
+            for (i = 0; i < n; i++) q[i] = bb[i] - sgn * q[i];
+            if (null_space) remove_mean(q, n);
+            tnorm = sqrt(dot(q, q, n));
+            if (tnorm <= rtol * bnorm) { status = 0; break; }
+            memcpy(r, q, sizeof(double) * (size_t)n);
+        }
+        for (i = 0; i < n; i++) z[i] = sgn * h->dinv[i] * r[i];
+        if (null_space) remove_mean(z, n);
+        rho_new = dot(r, z, n);
+        beta = rho_new / rho;
+        rho = rho_new;
+        for (i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+    }
+    if (k > maxit) k = maxit;
+done:
+    if (status != 0 || tnorm == 0.0) {
+        orc_helm_matvec(h, x, q);
+        for (i = 0; i < n; i++) q[i] = bb[i] - sgn * q[i];
+        if (null_space) remove_mean(q, n);
+        tnorm = sqrt(dot(q, q, n));
+    }
+    if (iters) *iters = k;
+    if (true_relres) *true_relres = bnorm > 0.0 ? tnorm / bnorm : 0.0;
+    free(bb); free(r); free(z); free(p); free(q);
+    return status;
+done_early:
+    if (iters) *iters = 0;
+    if (true_relres) *true_relres = NAN;
+    free(bb); free(r); free(z); free(p); free(q);
+    return status;
+}
+
+/* Field::PeriodicUpdate, Field.cpp:232-252 */
+void orc_field_periodic_update(const orc_mesh* m, double* field, int nDofs) {
+    int nv0 = m->nVerts[0], nv1 = m->nVerts[1], i, d;
+    if (m->periodic[0])
+        for (i = 0; i < nv1; i++)
+            for (d = 0; d < nDofs; d++)
+                field[(size_t)(i * nv0 + nv0 - 1) * nDofs + d] = field[(size_t)(i * nv0) * nDofs + d];
+    if (m->periodic[1])
+        for (i = 0; i < nv0; i++)
+            for (d = 0; d < nDofs; d++)
+                field[(size_t)(nv0 * (nv1 - 1) + i) * nDofs + d] = field[(size_t)i * nDofs + d];
+    if (m->periodic[0] && m->periodic[1])
+        for (d = 0; d < nDofs; d++) field[(size_t)(nv0 * nv1 - 1) * nDofs + d] = field[d];
+}
+
+/* Vector::UpdateField, Vector.cpp:40-61 */
+void orc_update_field(const orc_mesh* m, const double* x, double* field) {
+    int i;
+    for (i = 0; i < m->n; i++) field[m->v2f[i]] = x[i];
+    if (m->periodic[0] || m->periodic[1]) orc_field_periodic_update(m, field, 1);
+}
+
+/* Vector::UpdateVec, Vector.cpp:63-81 */
+void orc_update_vec(const orc_mesh* m, const double* field, double* x) {
+    int v;
+    for (v = 0; v < m->nV; v++)
+        if (!is_bc_node(m, v)) x[m->f2v[v]] = field[v];
+}
+
+/* HelmholtzEqn::Solve / PoissonEqn::Solve (HelmholtzEqn.cpp:105-136) */
+int orc_helm_solve(orc_helm* h, const double* rhs_field, double* sol_field, double rtol,
+                   int maxit, int null_space, int* iters, double* true_relres) {
+    int n = h->m->n, st;
+    double* f = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double* x = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    if (!h->assembled) orc_helm_assemble(h, sol_field);
+    orc_helm_rhs(h, rhs_field, f);
+    st = orc_helm_pcg(h, f, x, rtol, maxit, null_space, iters, true_relres, NULL);
+    orc_update_field(h->m, x, sol_field);
+    free(f);
+    free(x);
+    return st;
+}
+
+/* Matrix-free form of c_mass*M + c_lap*K~ with ideal affine geometry
+ * (J = diag(dx/2, dy/2)): per element
+ *   A_e u = c_mass*detJ*(w (x) w).u + c_lap*s*[ax W_y (x) G_x + ay G_y (x) W_x] u,
+ * G = D^T W D, ax = (2/dx)^2, ay = (2/dy)^2, s = 1 (literal) or detJ (physical).
+ * Summed over elements through the numbering map (Dirichlet columns are zero). */
+void orc_helm_apply_ideal(const orc_mesh* m, double c_mass, double c_lap, int detj_mode,
+                          const double* x, double* y) {
+    int N = m->N, n1 = N + 1, nl = m->nl, el, i, j, k, q;
+    double* G = (double*)calloc((size_t)n1 * n1, sizeof(double));
+    double* u = (double*)malloc(sizeof(double) * (size_t)nl);
+    int* nodes = (int*)malloc(sizeof(int) * (size_t)nl);
+    double detJ = 0.25 * m->dx[0] * m->dx[1];
+    double ax = (2.0 / m->dx[0]) * (2.0 / m->dx[0]), ay = (2.0 / m->dx[1]) * (2.0 / m->dx[1]);
+    double s = detj_mode ? detJ : 1.0;
+    for (i = 0; i < n1; i++)
+        for (j = 0; j < n1; j++)
+            for (q = 0; q < n1; q++) G[i * n1 + j] += m->weights[q] * m->dCij[q * n1 + i] * m->dCij[q * n1 + j];
+    memset(y, 0, sizeof(double) * (size_t)m->n);
+    for (el = 0; el < m->nElsTotal; el++) {
+        orc_el_nodes(m, el, nodes);
+        for (i = 0; i < nl; i++) {
+            int g = m->f2v[nodes[i]];
+            u[i] = g >= 0 ? x[g] : 0.0;
+        }
+        for (j = 0; j < n1; j++)
+            for (i = 0; i < n1; i++) {
+                int g = m->f2v[nodes[j * n1 + i]];
+                double sx = 0.0, sy = 0.0, v;
+                if (g < 0) continue;
+                for (k = 0; k < n1; k++) {
+                    sx += G[i * n1 + k] * u[j * n1 + k];
+                    sy += G[j * n1 + k] * u[k * n1 + i];
+                }
+                v = c_mass * detJ * m->weights[i] * m->weights[j] * u[j * n1 + i] +
+                    c_lap * s * (ax * m->weights[j] * sx + ay * m->weights[i] * sy);
+                y[g] += v;
+            }
+    }
+    free(G);
+    free(u);
+    free(nodes);
+}
+
+/* ------------------------------------------------------------------------- */
+/* a11-a16: semi-Lagrangian advection                                        */
+/* ------------------------------------------------------------------------- */
+
+/* Advector::PeriodicUpdate, Advector.cpp:122-134 */
+static void coord_wrap(const orc_mesh* m, double* c) {
+    int d;
+    for (d = 0; d < 2; d++) {
+        if (c[d] < m->min[d]) c[d] = m->periodic[d] ? m->max[d] - m->min[d] + c[d] : m->min[d];
+        if (c[d] > m->max[d]) c[d] = m->periodic[d] ? m->min[d] - m->max[d] + c[d] : m->max[d];
+    }
+}
+
+/* Advector::InterpLagrange, Advector.cpp:248-275 (nDofs <= 3) */
+static void interp_lagrange(double x, const double* coords, int nDofs, double vals[4][3], double* psi) {
+    int node, d, o, oc, others[3];
+    double factor;
+    for (d = 0; d < nDofs; d++) psi[d] = 0.0;
+    for (node = 0; node < 4; node++) {
+        oc = 0;
+        for (o = 0; o < 4; o++) if (o != node) others[oc++] = o;
+        factor = 1.0;
+        for (o = 0; o < 3; o++) factor *= (x - coords[others[o]]) / (coords[node] - coords[others[o]]);
+        for (d = 0; d < nDofs; d++) psi[d] += vals[node][d] * factor;
+    }
+}
+
+/* Advector::Interpolate (bicubic, used when N < 3), Advector.cpp:278-337 */
+static void interp_bicubic(const orc_mesh* m, const double* vals, int nDofs, const double* coord, double* psi) {
+    int el, x0, y0, xi, yi, d, node;
+    int* nodes = (int*)malloc(sizeof(int) * (size_t)m->nl);
+    int idx[4][4];
+    double px[4], py[4], l[2];
+    double ptsX[4][3], ptsY[4][3];
+    double ddx = m->dx[0] / m->N, ddy = m->dx[1] / m->N;
+    orc_global_to_local(m, coord, &el, l);
+    orc_el_nodes(m, el, nodes);
+    node = nodes[0];
+    x0 = node % m->nVerts[0];
+    y0 = (node / m->nVerts[0]) % m->nVerts[1];
+    if (coord[0] <= m->verts[2 * (size_t)nodes[1] + 0]) x0--;
+    if (coord[1] <= m->verts[2 * (size_t)nodes[3] + 1]) y0--;
+    if (coord[0] <= m->min[0] + ddx) x0++;
+    else if (coord[0] >= m->max[0] - ddx) x0--;
+    if (coord[1] <= m->min[1] + ddy) y0++;
+    else if (coord[1] >= m->max[1] - ddy) y0--;
+    for (yi = 0; yi < 4; yi++)
+        for (xi = 0; xi < 4; xi++) idx[xi][yi] = x0 + xi + (y0 + yi) * m->nVerts[0];
+    for (xi = 0; xi < 4; xi++) px[xi] = m->verts[2 * (size_t)idx[xi][0] + 0];
+    for (yi = 0; yi < 4; yi++) py[yi] = m->verts[2 * (size_t)idx[0][yi] + 1];
+    for (yi = 0; yi < 4; yi++) {
+        for (xi = 0; xi < 4; xi++)
+            for (d = 0; d < nDofs; d++) ptsX[xi][d] = vals[(size_t)idx[xi][yi] * nDofs + d];
+        interp_lagrange(coord[0], px, nDofs, ptsX, ptsY[yi]);
+    }
+    interp_lagrange(coord[1], py, nDofs, ptsY, psi);
+    free(nodes);
+}
+
+/* Advector::IntegrateRK2, Advector.cpp:136-148 */
+static void integrate_rk2(const orc_mesh* m, double dt, const double* vel0, const double* velH,
+                          const double* origin, double* coord) {
+    double xh[2], v[2];
+    orc_interp_global(m, vel0, 2, origin, v);
+    xh[0] = origin[0] - 0.5 * dt * v[0];
+    xh[1] = origin[1] - 0.5 * dt * v[1];
+    coord_wrap(m, xh);
+    orc_interp_global(m, velH, 2, xh, v);
+    coord[0] = origin[0] - dt * v[0];
+    coord[1] = origin[1] - dt * v[1];
+    coord_wrap(m, coord);
+}
+
+/* Advector::FirstOrder, Advector.cpp:181-201 (q: nV x nDofs, vel: nV x 2).
+ * dep_el / dep_xy (nullable) receive the departure element and point. */
+void orc_advect_first(const orc_mesh* m, const double* q, int nDofs, const double* vel,
+                      double dt, double* qSL, int* dep_el, double* dep_xy) {
+    int v, d;
+    double c[2], val[3];
+    for (v = 0; v < m->nV; v++) {
+        integrate_rk2(m, dt, vel, vel, &m->verts[2 * (size_t)v], c);
+        if (m->N < 3) interp_bicubic(m, q, nDofs, c, val);
+        else orc_interp_global(m, q, nDofs, c, val);
+        for (d = 0; d < nDofs; d++) qSL[(size_t)v * nDofs + d] = val[d];
+        if (dep_el) orc_get_el(m, c, &dep_el[v]);
+        if (dep_xy) { dep_xy[2 * (size_t)v] = c[0]; dep_xy[2 * (size_t)v + 1] = c[1]; }
+    }
+    if (m->periodic[0] || m->periodic[1]) orc_field_periodic_update(m, qSL, nDofs);
+}
+
+/* Advector::ExtrapolateVelocities + SecondOrder, Advector.cpp:203-246,339-349 */
+void orc_advect_second(const orc_mesh* m, const double* q, const double* qPrev, int nDofs,
+                       const double* vel, const double* velPrev, double dt, double* qSL) {
+    size_t nv2 = 2 * (size_t)m->nV, i;
+    double* vp1 = (double*)malloc(sizeof(double) * nv2);
+    double* vph = (double*)malloc(sizeof(double) * nv2);
+    double* vmh = (double*)malloc(sizeof(double) * nv2);
+    int v, d;
+    for (i = 0; i < nv2; i++) {
+        vp1[i] = 2.0 * vel[i] - 1.0 * velPrev[i];
+        vph[i] = 1.5 * vel[i] - 0.5 * velPrev[i];
+        vmh[i] = 0.5 * vel[i] + 0.5 * velPrev[i];
+    }
+    for (v = 0; v < m->nV; v++) {
+        const double* o = &m->verts[2 * (size_t)v];
+        double x1[2], x2[2], xh[2], w[2], p1[3], p2[3];
+        orc_interp_global(m, vp1, 2, o, w);
+        xh[0] = o[0] - 0.5 * dt * w[0];
+        xh[1] = o[1] - 0.5 * dt * w[1];
+        coord_wrap(m, xh);
+        orc_interp_global(m, vph, 2, xh, w);
+        x1[0] = o[0] - dt * w[0];
+        x1[1] = o[1] - dt * w[1];
+        coord_wrap(m, x1);
+        orc_interp_global(m, vel, 2, x1, w);
+        xh[0] = x1[0] - 0.5 * dt * w[0];
+        xh[1] = x1[1] - 0.5 * dt * w[1];
+        coord_wrap(m, xh);
+        orc_interp_global(m, vmh, 2, xh, w);
+        x2[0] = x1[0] - dt * w[0];
+        x2[1] = x1[1] - dt * w[1];
+        coord_wrap(m, x2);
+        if (m->N < 3) {
+            interp_bicubic(m, q, nDofs, x1, p1);
+            interp_bicubic(m, qPrev, nDofs, x2, p2);
+        } else {
+            orc_interp_global(m, q, nDofs, x1, p1);
+            orc_interp_global(m, qPrev, nDofs, x2, p2);
+        }
+        for (d = 0; d < nDofs; d++) qSL[(size_t)v * nDofs + d] = 2.0 * p1[d] - 0.5 * p2[d];
+    }
+    if (m->periodic[0] || m->periodic[1]) orc_field_periodic_update(m, qSL, nDofs);
+    free(vp1); free(vph); free(vmh);
+}
+
+/* Legendre::ShapeFuncDerivsAtCoord, Legendre.cpp:187-223 */
+static void shape_func_derivs_at_coord(const orc_mesh* m, const double* coord, double* GNix) {
+    int N = m->N, nl = m->nl, pt_j, x_i, y_i, x_j, y_j;
+    double Nx, Ny, GNx, GNy, a, b, c;
+    for (pt_j = 0; pt_j < nl; pt_j++) {
+        x_j = pt_j % (N + 1);
+        y_j = pt_j / (N + 1);
+        if (is_root(m, coord[0], &x_i)) {
+            Nx = (x_i == x_j) ? 1.0 : 0.0;
+            GNx = m->dCij[x_i * (N + 1) + x_j];
+        } else {
+            Nx = -((1.0 - coord[0] * coord[0]) /
+                   (N * (N + 1) * orc_leg_eval(m->pn, N, m->abcissa[x_j]) * (coord[0] - m->abcissa[x_j]))) *
+                 orc_leg_eval_deriv(m->pn, N, coord[0]);
+            a = coord[0] * coord[0] + 1.0 - 2.0 * coord[0] * m->abcissa[x_j];
+            b = (1.0 - coord[0] * coord[0]) * (coord[0] - m->abcissa[x_j]);
+            c = N * (N + 1) * (coord[0] - m->abcissa[x_j]) * (coord[0] - m->abcissa[x_j]) *
+                orc_leg_eval(m->pn, N, m->abcissa[x_j]);
+            GNx = (a * orc_leg_eval_deriv(m->pn, N, coord[0]) - b * orc_leg_eval_2nd(m->pn, N, coord[0])) / c;
+        }
+        if (is_root(m, coord[1], &y_i)) {
+            Ny = (y_i == y_j) ? 1.0 : 0.0;
+            GNy = m->dCij[y_i * (N + 1) + y_j];
+        } else {
+            Ny = -((1.0 - coord[1] * coord[1]) /
+                   (N * (N + 1) * orc_leg_eval(m->pn, N, m->abcissa[y_j]) * (coord[1] - m->abcissa[y_j]))) *
+                 orc_leg_eval_deriv(m->pn, N, coord[1]);
+            a = coord[1] * coord[1] + 1.0 - 2.0 * coord[1] * m->abcissa[y_j];
+            b = (1.0 - coord[1] * coord[1]) * (coord[1] - m->abcissa[y_j]);
+            c = N * (N + 1) * (coord[1] - m->abcissa[y_j]) * (coord[1] - m->abcissa[y_j]) *
+                orc_leg_eval(m->pn, N, m->abcissa[y_j]);
+            GNy = (a * orc_leg_eval_deriv(m->pn, N, coord[1]) - b * orc_leg_eval_2nd(m->pn, N, coord[1])) / c;
+        }
+        GNix[pt_j] = GNx * Ny;
+        GNix[nl + pt_j] = Nx * GNy;
+    }
+}
+
+/* QuasiGeostrophicEqn::GenVel, QuasiGeostrophicEqn.cpp:207-223, through
+ * Field::InterpDerivsGlobal (Field.cpp:96-118) and
+ * Mesh::ShapeFuncDerivsAtCoord (Mesh.cpp:252-285).  vel is nV x 2. */
+void orc_genvel(const orc_mesh* m, const double* psi, double* vel) {
+    int nl = m->nl, v, node;
+    double* GNix = (double*)malloc(sizeof(double) * 2 * (size_t)nl);
+    double* gGNix = (double*)malloc(sizeof(double) * 2 * (size_t)nl);
+    int* nodes = (int*)malloc(sizeof(int) * (size_t)nl);
+    for (v = 0; v < m->nV; v++) {
+        int el;
+        double l[2], detJac, g0 = 0.0, g1 = 0.0;
+        orc_global_to_local(m, &m->verts[2 * (size_t)v], &el, l);
+        shape_func_derivs_at_coord(m, l, GNix);
+        global_derivs(m, el, GNix, gGNix, &detJac, nodes);
+        for (node = 0; node < nl; node++) {
+            g0 += gGNix[node] * psi[nodes[node]];
+            g1 += gGNix[nl + node] * psi[nodes[node]];
+        }
+        vel[2 * (size_t)v + 0] = -1.0 * g1;
+        vel[2 * (size_t)v + 1] = +1.0 * g0;
+    }
+    free(GNix);
+    free(gGNix);
+    free(nodes);
+}
+
+/* CalcTimeStep, Utils.cpp:23-40 */
+double orc_calc_timestep(const orc_mesh* m, const double* vel, double courant) {
+    double minSep = (m->dx[0] < m->dx[1]) ? m->dx[0] : m->dx[1];
+    double maxVelSq = 0.0;
+    int v;
+    for (v = 0; v < m->nV; v++) {
+        double s = vel[2 * (size_t)v] * vel[2 * (size_t)v] + vel[2 * (size_t)v + 1] * vel[2 * (size_t)v + 1];
+        if (s > maxVelSq) maxVelSq = s;
+    }
+    return courant * minSep / sqrt(maxVelSq);
+}
+
+/* ------------------------------------------------------------------------- */
+/* callers: QG steps                                                         */
+/* ------------------------------------------------------------------------- */
+
+/* QuasiGeostrophicEqn::FirstOrder, QuasiGeostrophicEqn.cpp:117-135, with
+ * nu = r = 0 and no topography: GenVel, GenPV, SL advection of the PV, the
+ * Helmholtz inversion he1 and the omega-hat update (:131-136) applied as the
+ * new state.  With the dissipation off, SolveQ (:138) makes alpha = q_SL on
+ * every unknown, so the reference's second inversion he2 (:141-147) repeats
+ * he1 on the same unknown-row RHS and is not repeated here.  Dirichlet values
+ * of psi are the wall data (lifting).  psi / omega are nV fields updated in
+ * place.  Returns the solve status. */
+int orc_qg_step(const orc_mesh* m, double F, double beta, double dt, int detj_mode,
+                double* psi, double* omega, double rtol, int maxit, int* iters) {
+    size_t nV = (size_t)m->nV, v;
+    double* vel = (double*)malloc(sizeof(double) * 2 * nV);
+    double* alpha = (double*)malloc(sizeof(double) * nV);
+    double* qsl = (double*)malloc(sizeof(double) * nV);
+    double* rhs = (double*)malloc(sizeof(double) * nV);
+    orc_helm* h = orc_helm_create(m, -F, -1.0 * 1.0, detj_mode);
+    double rr;
+    int st;
+    orc_genvel(m, psi, vel);
+    for (v = 0; v < nV; v++) alpha[v] = omega[v] - F * psi[v] + beta * m->verts[2 * v + 1];
+    orc_advect_first(m, alpha, 1, vel, dt, qsl, NULL, NULL);
+    for (v = 0; v < nV; v++) rhs[v] = qsl[v] - beta * m->verts[2 * v + 1];
+    orc_helm_assemble(h, psi);
+    st = orc_helm_solve(h, rhs, psi, rtol, maxit, 0, iters, &rr);
+    for (v = 0; v < nV; v++) omega[v] = qsl[v] + F * psi[v] - beta * m->verts[2 * v + 1];
+    orc_helm_destroy(h);
+    free(vel); free(alpha); free(qsl); free(rhs);
+    return st;
+}
+
+/* TwoLayerQGEqn::FirstOrder, TwoLayerQGEqn.cpp:174-243 steps 1-4 (advect both
+ * layers' PV, baroclinic Helmholtz for tau = psi1 - psi2 with lambda = F1+F2,
+ * barotropic Poisson for chi = psi1 + psi2), dissipation off; psi/omega are
+ * then updated from (tau, chi) as in steps 4/8 of the reference.  The second
+ * solve pair of the reference (which repeats the first after the layer-1
+ * dissipation solve) is not repeated. */
+int orc_qg2_step(const orc_mesh* m, double F1, double F2, double beta, double dt, int detj_mode,
+                 double* psi1, double* omega1, double* psi2, double* omega2,
+                 double rtol, int maxit, int* iters) {
+    size_t nV = (size_t)m->nV, v;
+    double* vel1 = (double*)malloc(sizeof(double) * 2 * nV);
+    double* vel2 = (double*)malloc(sizeof(double) * 2 * nV);
+    double* a1 = (double*)malloc(sizeof(double) * nV);
+    double* a2 = (double*)malloc(sizeof(double) * nV);
+    double* h1 = (double*)malloc(sizeof(double) * nV);
+    double* h2 = (double*)malloc(sizeof(double) * nV);
+    double* rhs = (double*)malloc(sizeof(double) * nV);
+    double* tau = (double*)malloc(sizeof(double) * nV);
+    double* chi = (double*)malloc(sizeof(double) * nV);
+    orc_helm* hh = orc_helm_create(m, -(F1 + F2), -1.0 * 1.0, detj_mode);
+    orc_helm* hp = orc_helm_create(m, 0.0, -1.0, detj_mode);
+    double rr;
+    int st, st2, it1 = 0, it2 = 0;
+    orc_genvel(m, psi1, vel1);
+    orc_genvel(m, psi2, vel2);
+    for (v = 0; v < nV; v++) {
+        double y = m->verts[2 * v + 1];
+        a1[v] = omega1[v] + F1 * (psi2[v] - psi1[v]) + beta * y;
+        a2[v] = omega2[v] + F2 * (psi1[v] - psi2[v]) + beta * y;
+        tau[v] = psi1[v] - psi2[v];
+        chi[v] = psi1[v] + psi2[v];
+    }
+    orc_advect_first(m, a1, 1, vel1, dt, h1, NULL, NULL);
+    orc_advect_first(m, a2, 1, vel2, dt, h2, NULL, NULL);
+    for (v = 0; v < nV; v++) rhs[v] = h1[v] - h2[v];
+    orc_helm_assemble(hh, tau);
+    st = orc_helm_solve(hh, rhs, tau, rtol, maxit, 0, &it1, &rr);
+    for (v = 0; v < nV; v++) {
+        double y = m->verts[2 * v + 1];
+        rhs[v] = h1[v] + h2[v] + (F1 - F2) * tau[v] - 2.0 * beta * y;
+    }
+    orc_helm_assemble(hp, chi);
+    st2 = orc_helm_solve(hp, rhs, chi, rtol, maxit, 0, &it2, &rr);
+    for (v = 0; v < nV; v++) {
+        double y = m->verts[2 * v + 1];
+        psi1[v] = 0.5 * (tau[v] + chi[v]);
+        psi2[v] = 0.5 * (chi[v] - tau[v]);
+        omega1[v] = h1[v] - F1 * (psi2[v] - psi1[v]) - beta * y;
+        omega2[v] = h2[v] - F2 * (psi1[v] - psi2[v]) - beta * y;
+    }
+    if (iters) *iters = it1 + it2;
+    orc_helm_destroy(hh);
+    orc_helm_destroy(hp);
+    free(vel1); free(vel2); free(a1); free(a2); free(h1); free(h2); free(rhs); free(tau); free(chi);
+    return st != 0 ? st : st2;
+}

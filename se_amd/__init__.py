@@ -1,0 +1,303 @@
+"""se_amd -- MI355X (gfx950) native spectral-element hot path of davelee2804/SE.
+
+Python view of the C ABI declared in include/se_amd.h (libse_amd.so, built
+in-tree by se_amd/build.py).  The compute path is the HIP library; this module
+only marshals numpy arrays (host entry points) or raw device pointers (the
+*_dev entry points, e.g. from torch tensors) and raises on every non-zero
+status.  There is no CPU fallback: without the built library every call fails.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libse_amd.so")
+
+DETJ_LITERAL, DETJ_PHYSICAL = 0, 1
+PREC_FP64, PREC_FP32_IR, PREC_BF16_IR = 0, 1, 2
+SOLVE_CONVERGED, SOLVE_MAXIT, SOLVE_BREAKDOWN, SOLVE_NONFINITE = 0, 1, 2, 3
+
+_D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_I = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lib = None
+
+
+class SEError(RuntimeError):
+    pass
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("maxit", C.c_int), ("precision", C.c_int), ("null_space", C.c_int),
+                ("check_every", C.c_int), ("inner_rtol", C.c_double), ("max_outer", C.c_int),
+                ("history", C.c_void_p)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("iters", C.c_int), ("outer_iters", C.c_int),
+                ("true_rel_residual", C.c_double), ("seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MeshInfo(C.Structure):
+    _fields_ = [("N", C.c_int), ("nEls", C.c_int * 2), ("nVerts", C.c_int * 2), ("nV", C.c_int),
+                ("nElsTotal", C.c_int), ("n", C.c_int), ("size0", C.c_int)]
+
+
+def lib():
+    """Load libse_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SEError("se_amd: %s is not built; run `python -m se_amd.build`" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        vp, i, d = C.c_void_p, C.c_int, C.c_double
+        pvp = C.POINTER(C.c_void_p)
+        sig = {
+            "se_last_error": (C.c_char_p, []),
+            "se_version": (i, []),
+            "se_ctx_create": (i, [i, pvp]),
+            "se_ctx_destroy": (i, [vp]),
+            "se_ctx_synchronize": (i, [vp]),
+            "se_ctx_stream": (vp, [vp]),
+            "se_mesh_create": (i, [vp, _I, i, _D, _D, _I, _I, pvp]),
+            "se_mesh_destroy": (i, [vp]),
+            "se_mesh_get_info": (i, [vp, C.POINTER(MeshInfo)]),
+            "se_mesh_gll": (i, [vp, _D, _D, _D]),
+            "se_mesh_maps": (i, [vp, vp, vp, vp]),
+            "se_mesh_verts": (i, [vp, _D]),
+            "se_mesh_dev_field_to_vec": (vp, [vp]),
+            "se_mesh_dev_verts": (vp, [vp]),
+            "se_field_to_vec_dev": (i, [vp, vp, vp]),
+            "se_vec_to_field_dev": (i, [vp, vp, vp]),
+            "se_helm_create": (i, [vp, d, d, i, pvp]),
+            "se_helm_destroy": (i, [vp]),
+            "se_helm_apply": (i, [vp, _D, _D]),
+            "se_helm_apply_dev": (i, [vp, vp, vp]),
+            "se_helm_inv_diag": (i, [vp, _D]),
+            "se_helm_set_dirichlet": (i, [vp, _D]),
+            "se_helm_set_dirichlet_dev": (i, [vp, vp]),
+            "se_helm_lift": (i, [vp, _D]),
+            "se_helm_rhs": (i, [vp, _D, _D]),
+            "se_helm_rhs_dev": (i, [vp, vp, vp]),
+            "se_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
+            "se_helm_pcg": (i, [vp, _D, _D, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
+            "se_helm_pcg_dev": (i, [vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
+            "se_helm_solve": (i, [vp, _D, _D, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
+            "se_helm_solve_dev": (i, [vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
+            "se_sl_advect": (i, [vp, _D, i, _D, d, _D, vp]),
+            "se_sl_advect_dev": (i, [vp, vp, i, vp, d, vp, vp]),
+            "se_sl_advect2": (i, [vp, _D, _D, i, _D, _D, d, _D]),
+            "se_sl_advect2_dev": (i, [vp, vp, vp, i, vp, vp, d, vp]),
+            "se_genvel": (i, [vp, _D, _D]),
+            "se_genvel_dev": (i, [vp, vp, vp]),
+            "se_calc_timestep": (i, [vp, _D, d, C.POINTER(d)]),
+            "se_calc_timestep_dev": (i, [vp, vp, d, C.POINTER(d)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().se_last_error().decode(errors="replace")
+        raise SEError("%s failed (%d): %s" % (what, rc, msg))
+
+
+def _d(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def solve_opts(rtol=1e-10, maxit=100000, precision=PREC_FP64, null_space=False, check_every=16,
+               inner_rtol=0.0, max_outer=50, history=None):
+    o = SolveOpts()
+    lib().se_solve_opts_default(C.byref(o))
+    o.rtol, o.maxit, o.precision, o.null_space = rtol, maxit, precision, int(bool(null_space))
+    o.check_every, o.inner_rtol, o.max_outer = check_every, inner_rtol, max_outer
+    o.history = history.ctypes.data if history is not None else None
+    return o
+
+
+class Context:
+    """One HIP stream + the device memory of everything created from it."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().se_ctx_create(device, C.byref(h)), "se_ctx_create")
+        self.h = h
+
+    @property
+    def stream(self):
+        return lib().se_ctx_stream(self.h)
+
+    def synchronize(self):
+        check(lib().se_ctx_synchronize(self.h), "se_ctx_synchronize")
+
+    def close(self):
+        if self.h:
+            lib().se_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Mesh:
+    """Mesh + BCs of the reference (legendre elements).  bc = (bottom, top, left, right)."""
+
+    def __init__(self, ctx, nEls, N, lo, hi, periodic, bc=(0, 0, 0, 0)):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(lib().se_mesh_create(ctx.h, np.array(nEls, dtype=np.int32), N, _d(lo), _d(hi),
+                                   np.array(periodic, dtype=np.int32), np.array(bc, dtype=np.int32),
+                                   C.byref(h)), "se_mesh_create")
+        self.h = h
+        info = MeshInfo()
+        check(lib().se_mesh_get_info(h, C.byref(info)), "se_mesh_get_info")
+        self.N, self.nV, self.n, self.nElsTotal, self.size0 = info.N, info.nV, info.n, info.nElsTotal, info.size0
+        self.nEls = (info.nEls[0], info.nEls[1])
+        self.nVerts = (info.nVerts[0], info.nVerts[1])
+        self.nl = (self.N + 1) ** 2
+        self.lo, self.hi, self.periodic, self.bc = tuple(lo), tuple(hi), tuple(periodic), tuple(bc)
+
+    def gll(self):
+        N = self.N
+        a, w, D = np.zeros(N + 1), np.zeros(N + 1), np.zeros((N + 1) ** 2)
+        check(lib().se_mesh_gll(self.h, a, w, D), "se_mesh_gll")
+        return a, w, D.reshape(N + 1, N + 1)
+
+    def maps(self, el_nodes=False):
+        f2v = np.zeros(self.nV, dtype=np.int32)
+        v2f = np.zeros(max(self.n, 1), dtype=np.int32)
+        en = np.zeros(self.nElsTotal * self.nl, dtype=np.int32) if el_nodes else None
+        check(lib().se_mesh_maps(self.h, f2v.ctypes.data, v2f.ctypes.data, en.ctypes.data if el_nodes else None),
+              "se_mesh_maps")
+        if el_nodes:
+            return f2v, v2f[: self.n], en.reshape(self.nElsTotal, self.nl)
+        return f2v, v2f[: self.n]
+
+    def verts(self):
+        v = np.zeros(2 * self.nV)
+        check(lib().se_mesh_verts(self.h, v), "se_mesh_verts")
+        return v.reshape(self.nV, 2)
+
+    def advect(self, q, vel, dt, nDofs=1, want_dep=True):
+        qsl = np.zeros(self.nV * nDofs)
+        dep = np.zeros(self.nV, dtype=np.int32)
+        check(lib().se_sl_advect(self.h, _d(q), nDofs, _d(vel), dt, qsl, dep.ctypes.data if want_dep else None),
+              "se_sl_advect")
+        return (qsl, dep) if want_dep else qsl
+
+    def advect2(self, q, qprev, vel, velprev, dt, nDofs=1):
+        qsl = np.zeros(self.nV * nDofs)
+        check(lib().se_sl_advect2(self.h, _d(q), _d(qprev), nDofs, _d(vel), _d(velprev), dt, qsl), "se_sl_advect2")
+        return qsl
+
+    def genvel(self, psi):
+        vel = np.zeros(2 * self.nV)
+        check(lib().se_genvel(self.h, _d(psi), vel), "se_genvel")
+        return vel.reshape(self.nV, 2)
+
+    def calc_timestep(self, vel, courant):
+        dt = C.c_double()
+        check(lib().se_calc_timestep(self.h, _d(vel), courant, C.byref(dt)), "se_calc_timestep")
+        return dt.value
+
+    def close(self):
+        if self.h:
+            lib().se_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Helm:
+    """Matrix-free c_mass*M + c_lap*K~ (HelmholtzEqn: c_mass=c, c_lap=nu*dt; PoissonEqn: 0, c)."""
+
+    def __init__(self, mesh, c_mass, c_lap, detj_mode=DETJ_LITERAL):
+        self.mesh = mesh
+        h = C.c_void_p()
+        check(lib().se_helm_create(mesh.h, c_mass, c_lap, detj_mode, C.byref(h)), "se_helm_create")
+        self.h = h
+
+    def apply(self, x):
+        y = np.zeros(self.mesh.n)
+        check(lib().se_helm_apply(self.h, _d(x), y), "se_helm_apply")
+        return y
+
+    def apply_dev(self, x_ptr, y_ptr):
+        check(lib().se_helm_apply_dev(self.h, x_ptr, y_ptr), "se_helm_apply_dev")
+
+    def inv_diag(self):
+        d = np.zeros(self.mesh.n)
+        check(lib().se_helm_inv_diag(self.h, d), "se_helm_inv_diag")
+        return d
+
+    def set_dirichlet(self, sol_field):
+        check(lib().se_helm_set_dirichlet(self.h, _d(sol_field)), "se_helm_set_dirichlet")
+
+    def lift(self):
+        b = np.zeros(self.mesh.n)
+        check(lib().se_helm_lift(self.h, b), "se_helm_lift")
+        return b
+
+    def rhs(self, rhs_field):
+        f = np.zeros(self.mesh.n)
+        check(lib().se_helm_rhs(self.h, _d(rhs_field), f), "se_helm_rhs")
+        return f
+
+    def pcg(self, f, **kw):
+        hist = None
+        if kw.pop("history", False):
+            hist = np.zeros(kw.get("maxit", 100000) + 1)
+        o = solve_opts(history=hist, **kw)
+        x = np.zeros(self.mesh.n)
+        info = SolveInfo()
+        check(lib().se_helm_pcg(self.h, _d(f), x, C.byref(o), C.byref(info)), "se_helm_pcg")
+        out = info.as_dict()
+        if hist is not None:
+            out["hist"] = hist[: info.iters + 1]
+        return x, out
+
+    def pcg_dev(self, f_ptr, x_ptr, **kw):
+        o = solve_opts(**kw)
+        info = SolveInfo()
+        check(lib().se_helm_pcg_dev(self.h, f_ptr, x_ptr, C.byref(o), C.byref(info)), "se_helm_pcg_dev")
+        return info.as_dict()
+
+    def solve(self, rhs_field, sol_field, **kw):
+        o = solve_opts(**kw)
+        sol = _d(sol_field).copy()
+        info = SolveInfo()
+        check(lib().se_helm_solve(self.h, _d(rhs_field), sol, C.byref(o), C.byref(info)), "se_helm_solve")
+        return sol, info.as_dict()
+
+    def solve_dev(self, rhs_ptr, sol_ptr, **kw):
+        o = solve_opts(**kw)
+        info = SolveInfo()
+        check(lib().se_helm_solve_dev(self.h, rhs_ptr, sol_ptr, C.byref(o), C.byref(info)), "se_helm_solve_dev")
+        return info.as_dict()
+
+    def close(self):
+        if self.h:
+            lib().se_helm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

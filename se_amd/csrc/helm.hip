@@ -1,0 +1,647 @@
+// helm.hip -- Helmholtz/Poisson operator objects, the device-resident FP64
+// Jacobi-PCG and the C ABI for them.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "helm_kernels.h"
+#include "se_common.h"
+#include "se_op.h"
+
+using namespace se;
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------
+// CG vector kernels (grid-stride; the last block of each reduction finalises
+// the recurrence scalars in fixed block order).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sum(int n, const double* __restrict__ a, double scale,
+                                             double* __restrict__ partials, CgState* __restrict__ st) {
+    double s = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += scale * a[i];
+    double v[1] = {s}, tot[1];
+    if (block_reduce_last<1>(v, partials, &st->counter, tot) && threadIdx.x == 0) st->mu = tot[0] / n;
+}
+
+// b = sgn*f (- mean), x = 0, r = b; first preconditioned residual
+__global__ __launch_bounds__(256) void k_cg_start(int n, double sgn, double rtol, const double* __restrict__ f,
+                                                  const double* __restrict__ dinv, double* __restrict__ b,
+                                                  double* __restrict__ x, double* __restrict__ r,
+                                                  double* __restrict__ partials, CgState* __restrict__ st) {
+    const double mu_b = st->null_space ? st->mu : 0.0;
+    double bb = 0.0, rz = 0.0, sz = 0.0, sr = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double bi = sgn * f[i] - mu_b;
+        b[i] = bi;
+        r[i] = bi;
+        x[i] = 0.0;
+        double z = dinv[i] * bi;
+        bb += bi * bi;
+        rz += bi * z;
+        sz += z;
+        sr += bi;
+    }
+    double v[4] = {bb, rz, sz, sr}, t[4];
+    if (block_reduce_last<4>(v, partials, &st->counter, t) && threadIdx.x == 0) {
+        st->bb = t[0];
+        st->tol2 = rtol * rtol * t[0];
+        double mu = st->null_space ? t[2] / n : 0.0;
+        st->mu = mu;
+        st->rho = t[1] - mu * t[3];
+        st->iter = 0;
+        st->first = 1;
+        st->rr = t[0];
+        bool finite = (t[0] - t[0] == 0.0);
+        st->stop = (t[0] == 0.0 || !finite) ? 1 : 0;
+        st->status = t[0] == 0.0 ? 0 : (finite ? CG_RUNNING : 3);
+        st->tt = -1.0;
+    }
+}
+
+// p = (D^-1 r - mu) + beta p   (first direction: p = D^-1 r - mu)
+__global__ __launch_bounds__(256) void k_cg_dir(int n, const double* __restrict__ dinv, const double* __restrict__ r,
+                                                double* __restrict__ p, CgState* __restrict__ st) {
+    if (st->stop) return;
+    const double beta = st->beta, mu = st->mu;
+    const bool first = st->first != 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double z = dinv[i] * r[i] - mu;
+        p[i] = first ? z : z + beta * p[i];
+    }
+}
+
+// x += alpha p, r -= alpha q, then r.r, r.z (z = D^-1 r) and the next beta
+__global__ __launch_bounds__(256) void k_cg_update(int n, const double* __restrict__ p, const double* __restrict__ q,
+                                                   const double* __restrict__ dinv, double* __restrict__ x,
+                                                   double* __restrict__ r, double* __restrict__ hist,
+                                                   double* __restrict__ partials, CgState* __restrict__ st) {
+    if (st->stop) return;
+    const double alpha = st->alpha;
+    double rr = 0.0, rz = 0.0, sz = 0.0, sr = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        x[i] += alpha * p[i];
+        double ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        double z = dinv[i] * ri;
+        rr += ri * ri;
+        rz += ri * z;
+        sz += z;
+        sr += ri;
+    }
+    double v[4] = {rr, rz, sz, sr}, t[4];
+    if (block_reduce_last<4>(v, partials, &st->counter, t) && threadIdx.x == 0) {
+        int it = st->iter + 1;
+        st->iter = it;
+        st->first = 0;
+        st->rr = t[0];
+        if (hist) hist[it] = sqrt(t[0] / st->bb);
+        if (!(t[0] - t[0] == 0.0)) {
+            st->stop = 1;
+            st->status = 3;
+        } else if (t[0] <= st->tol2) {
+            st->stop = 1;
+            st->status = CG_PENDING;
+        } else {
+            double mu = st->null_space ? t[2] / n : 0.0;
+            double rho_new = t[1] - mu * t[3];
+            st->mu = mu;
+            st->beta = rho_new / st->rho;
+            st->rho = rho_new;
+            if (it >= st->maxit) {
+                st->stop = 1;
+                st->status = 1;
+            }
+        }
+    }
+}
+
+// t = b - q (q = A x); r = t; ||t - mean(t)||^2 into st->tt
+__global__ __launch_bounds__(256) void k_true_resid(int n, const double* __restrict__ b, const double* __restrict__ q,
+                                                    double* __restrict__ r, double* __restrict__ partials,
+                                                    CgState* __restrict__ st) {
+    double s = 0.0, s2 = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double t = b[i] - q[i];
+        r[i] = t;
+        s += t;
+        s2 += t * t;
+    }
+    double v[2] = {s, s2}, t[2];
+    if (block_reduce_last<2>(v, partials, &st->counter, t) && threadIdx.x == 0) {
+        double mt = st->null_space ? t[0] / n : 0.0;
+        st->tt = st->null_space ? t[1] - t[0] * mt : t[1];
+        st->mu = mt;  // consumed by k_cg_restart
+    }
+}
+
+// continue from the true residual: r -= mean (null space), new rho and beta
+__global__ __launch_bounds__(256) void k_cg_restart(int n, const double* __restrict__ dinv, double* __restrict__ r,
+                                                    double* __restrict__ partials, CgState* __restrict__ st) {
+    const double mt = st->null_space ? st->mu : 0.0;
+    double rr = 0.0, rz = 0.0, sz = 0.0, sr = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double ri = r[i] - mt;
+        r[i] = ri;
+        double z = dinv[i] * ri;
+        rr += ri * ri;
+        rz += ri * z;
+        sz += z;
+        sr += ri;
+    }
+    double v[4] = {rr, rz, sz, sr}, t[4];
+    if (block_reduce_last<4>(v, partials, &st->counter, t) && threadIdx.x == 0) {
+        double mu = st->null_space ? t[2] / n : 0.0;
+        double rho_new = t[1] - mu * t[3];
+        st->rr = t[0];
+        st->mu = mu;
+        st->beta = rho_new / st->rho;
+        st->rho = rho_new;
+        st->stop = 0;
+        st->status = CG_RUNNING;
+    }
+}
+
+__global__ void k_scale_copy(int n, double s, const double* __restrict__ a, double* __restrict__ b) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = s * a[i];
+}
+
+// ---------------------------------------------------------------------------
+// tile-kernel dispatch over (N, tile, mode)
+// ---------------------------------------------------------------------------
+template <int N, int TX, int TY, int MODE, bool DOT>
+void launch_tile_t(const OpParams& P, const double* in, double* out, const double* aux, double* partials,
+                   CgState* st, hipStream_t s) {
+    int tiles = ((P.nEls0 + TX - 1) / TX) * ((P.nEls1 + TY - 1) / TY);
+    hipLaunchKernelGGL((k_helm_tile<N, TX, TY, MODE, DOT>), dim3(tiles), dim3(kBlock), 0, s, P, in, out, aux,
+                       partials, st);
+}
+
+template <int N, int MODE, bool DOT>
+void launch_tile_n(int tile, const OpParams& P, const double* in, double* out, const double* aux, double* partials,
+                   CgState* st, hipStream_t s) {
+    if (tile == 4) launch_tile_t<N, 4, 4, MODE, DOT>(P, in, out, aux, partials, st, s);
+    else launch_tile_t<N, 2, 2, MODE, DOT>(P, in, out, aux, partials, st, s);
+}
+
+template <int MODE, bool DOT>
+int launch_tile(int N, int tile, const OpParams& P, const double* in, double* out, const double* aux,
+                double* partials, CgState* st, hipStream_t s) {
+    switch (N) {
+        case 2: launch_tile_n<2, MODE, DOT>(tile, P, in, out, aux, partials, st, s); break;
+        case 3: launch_tile_n<3, MODE, DOT>(tile, P, in, out, aux, partials, st, s); break;
+        case 4: launch_tile_n<4, MODE, DOT>(tile, P, in, out, aux, partials, st, s); break;
+        case 5: launch_tile_n<5, MODE, DOT>(tile, P, in, out, aux, partials, st, s); break;
+        case 6: launch_tile_n<6, MODE, DOT>(tile, P, in, out, aux, partials, st, s); break;
+        case 7: launch_tile_n<7, MODE, DOT>(tile, P, in, out, aux, partials, st, s); break;
+        case 8: launch_tile_n<8, MODE, DOT>(tile, P, in, out, aux, partials, st, s); break;
+        default: return fail(SE_ERR_UNSUPPORTED, "operator kernels are built for orders 2..8");
+    }
+    return SE_OK;
+}
+
+int vec_blocks(int n) {
+    int b = (n + kBlock * 4 - 1) / (kBlock * 4);
+    return std::max(1, std::min(b, 2048));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// operator object
+// ---------------------------------------------------------------------------
+namespace se {
+
+OpParams op_params(const se_op* op, double scale) {
+    const se_mesh* m = op->mesh;
+    OpParams P;
+    P.nEls0 = m->nEls[0];
+    P.nEls1 = m->nEls[1];
+    P.nv0 = m->nVerts[0];
+    P.nv1 = m->nVerts[1];
+    P.per0 = m->periodic[0];
+    P.per1 = m->periodic[1];
+    P.detj = op->detj;
+    P.c_mass = op->c_mass;
+    P.c_lap = op->c_lap;
+    P.scale = scale;
+    P.f2v = m->d_f2v.as<int32_t>();
+    const int n1 = m->N + 1;
+    P.J00 = m->d_J00.as<double>();
+    P.J10 = m->d_J10.as<double>();
+    P.J11 = m->d_J11.as<double>();
+    P.J01 = m->d_J01.as<double>();
+    const double* basis = m->d_basis.as<double>();
+    P.w = basis + n1;
+    P.D = basis + 2 * n1;
+    return P;
+}
+
+int op_tile(const se_op* op) {
+    // 4x4 tiles when they still give every CU two workgroups, else 2x2
+    const se_mesh* m = op->mesh;
+    long t4 = (long)((m->nEls[0] + 3) / 4) * ((m->nEls[1] + 3) / 4);
+    return (t4 >= 2L * m->ctx->cu_count) ? 4 : 2;
+}
+
+int op_apply_dev(se_op* op, const double* d_x, double* d_y, double scale, bool dot) {
+    OpParams P = op_params(op, scale);
+    hipStream_t s = op->mesh->ctx->stream;
+    int rc = dot ? launch_tile<MODE_APPLY, true>(op->mesh->N, op->tile, P, d_x, d_y, nullptr,
+                                                 op->partials.as<double>(), op->state.as<CgState>(), s)
+                 : launch_tile<MODE_APPLY, false>(op->mesh->N, op->tile, P, d_x, d_y, nullptr,
+                                                  op->partials.as<double>(), op->state.as<CgState>(), s);
+    if (rc) return rc;
+    SE_HIP(hipGetLastError());
+    return SE_OK;
+}
+
+int op_ensure_work(se_op* op) {
+    const se_mesh* m = op->mesh;
+    size_t nb = sizeof(double) * (size_t)std::max(m->n, 1);
+    int tiles = ((m->nEls[0] + 1) / 2) * ((m->nEls[1] + 1) / 2);
+    size_t np = sizeof(double) * 4 * (size_t)std::max(tiles, 2048);
+    int rc = SE_OK;
+    if (!rc) rc = op->b.alloc(nb);
+    if (!rc) rc = op->x.alloc(nb);
+    if (!rc) rc = op->r.alloc(nb);
+    if (!rc) rc = op->p.alloc(nb);
+    if (!rc) rc = op->q.alloc(nb);
+    if (!rc) rc = op->f.alloc(nb);
+    if (!rc) rc = op->partials.alloc(np);
+    return rc;
+}
+
+}  // namespace se
+
+extern "C" void se_solve_opts_default(se_solve_opts* o) {
+    if (!o) return;
+    o->rtol = 1e-10;
+    o->maxit = 100000;
+    o->precision = SE_PREC_FP64;
+    o->null_space = 0;
+    o->check_every = 16;
+    o->inner_rtol = 0.0;
+    o->max_outer = 50;
+    o->history = nullptr;
+}
+
+extern "C" int se_helm_create(se_mesh* m, double c_mass, double c_lap, int detj_mode, se_op** out) {
+    SE_CHECK(m && out, SE_ERR_ARG, "se_helm_create: NULL argument");
+    SE_CHECK(detj_mode == SE_DETJ_LITERAL || detj_mode == SE_DETJ_PHYSICAL, SE_ERR_ARG, "se_helm_create: bad detj_mode");
+    SE_CHECK(m->N >= 2 && m->N <= 8, SE_ERR_UNSUPPORTED, "se_helm_create: operator kernels cover orders 2..8");
+    SE_HIP(hipSetDevice(m->ctx->device));
+    se_op* op = new se_op;
+    op->mesh = m;
+    op->c_mass = c_mass;
+    op->c_lap = c_lap;
+    op->detj = detj_mode;
+    op->tile = op_tile(op);
+    size_t nb = sizeof(double) * (size_t)std::max(m->n, 1);
+    int rc = op->dinv.alloc(nb);
+    if (!rc) rc = op->lift.alloc(nb);
+    if (!rc) rc = op->state.alloc(sizeof(CgState));
+    if (!rc) rc = op_ensure_work(op);
+    if (rc) {
+        se_helm_destroy(op);
+        return rc;
+    }
+    hipStream_t s = m->ctx->stream;
+    (void)hipMemsetAsync(op->state.ptr, 0, sizeof(CgState), s);
+    (void)hipMemsetAsync(op->lift.ptr, 0, nb, s);
+    // inverse diagonal of the assembled operator (Matrix::InverseDiagonal)
+    OpParams P = op_params(op, 1.0);
+    rc = launch_tile<MODE_DIAG, false>(m->N, 2, P, nullptr, op->dinv.as<double>(), nullptr, nullptr, nullptr, s);
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(s);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e != hipSuccess) rc = fail(SE_ERR_HIP, std::string("diagonal kernel: ") + hipGetErrorString(e));
+    }
+    if (rc) {
+        se_helm_destroy(op);
+        return rc;
+    }
+    // sign of the operator: all diagonal entries must share it (K6 of the
+    // reference QG operators: negative definite)
+    std::vector<double> d(m->n);
+    if (m->n > 0) (void)hipMemcpy(d.data(), op->dinv.ptr, nb, hipMemcpyDeviceToHost);
+    int pos = 0, neg = 0;
+    for (double v : d) {
+        if (v > 0) pos++;
+        else if (v < 0) neg++;
+    }
+    op->sign = (neg > 0 && pos == 0) ? -1 : 1;
+    op->definite = (pos == m->n) || (neg == m->n);
+    *out = op;
+    return SE_OK;
+}
+
+extern "C" int se_helm_destroy(se_op* op) {
+    if (!op) return SE_OK;
+    (void)hipSetDevice(op->mesh->ctx->device);
+    if (op->graph_exec) (void)hipGraphExecDestroy(op->graph_exec);
+    op->release_all();
+    delete op;
+    return SE_OK;
+}
+
+extern "C" int se_helm_apply_dev(se_op* op, const double* d_x, double* d_y) {
+    SE_CHECK(op && d_x && d_y, SE_ERR_ARG, "se_helm_apply_dev: NULL argument");
+    return op_apply_dev(op, d_x, d_y, 1.0, false);
+}
+
+static int stage_in(se_ctx* c, DevBuf& dst, const double* host, size_t count) {
+    int rc = dst.alloc(sizeof(double) * std::max<size_t>(count, 1));
+    if (rc) return rc;
+    if (count) SE_HIP(hipMemcpyAsync(dst.ptr, host, sizeof(double) * count, hipMemcpyHostToDevice, c->stream));
+    return SE_OK;
+}
+
+static int stage_out(se_ctx* c, double* host, const DevBuf& src, size_t count) {
+    if (count) SE_HIP(hipMemcpyAsync(host, src.ptr, sizeof(double) * count, hipMemcpyDeviceToHost, c->stream));
+    SE_HIP(hipStreamSynchronize(c->stream));
+    return SE_OK;
+}
+
+extern "C" int se_helm_apply(se_op* op, const double* x, double* y) {
+    SE_CHECK(op && x && y, SE_ERR_ARG, "se_helm_apply: NULL argument");
+    se_ctx* c = op->mesh->ctx;
+    SE_HIP(hipSetDevice(c->device));
+    int n = op->mesh->n;
+    int rc = stage_in(c, op->sx, x, n);
+    if (!rc) rc = op->sy.alloc(sizeof(double) * std::max(n, 1));
+    if (!rc) rc = op_apply_dev(op, op->sx.as<double>(), op->sy.as<double>(), 1.0, false);
+    if (!rc) rc = stage_out(c, y, op->sy, n);
+    return rc;
+}
+
+extern "C" int se_helm_inv_diag(se_op* op, double* dinv) {
+    SE_CHECK(op && dinv, SE_ERR_ARG, "se_helm_inv_diag: NULL argument");
+    SE_HIP(hipSetDevice(op->mesh->ctx->device));
+    SE_HIP(hipMemcpy(dinv, op->dinv.ptr, sizeof(double) * op->mesh->n, hipMemcpyDeviceToHost));
+    return SE_OK;
+}
+
+extern "C" int se_helm_set_dirichlet_dev(se_op* op, const double* d_sol_field) {
+    SE_CHECK(op && d_sol_field, SE_ERR_ARG, "se_helm_set_dirichlet_dev: NULL argument");
+    OpParams P = op_params(op, 1.0);
+    int rc = launch_tile<MODE_LIFT, false>(op->mesh->N, op->tile, P, d_sol_field, op->lift.as<double>(), nullptr,
+                                           nullptr, nullptr, op->mesh->ctx->stream);
+    if (rc) return rc;
+    SE_HIP(hipGetLastError());
+    return SE_OK;
+}
+
+extern "C" int se_helm_set_dirichlet(se_op* op, const double* sol_field) {
+    SE_CHECK(op && sol_field, SE_ERR_ARG, "se_helm_set_dirichlet: NULL argument");
+    se_ctx* c = op->mesh->ctx;
+    SE_HIP(hipSetDevice(c->device));
+    int rc = stage_in(c, op->sfield, sol_field, op->mesh->nV);
+    if (!rc) rc = se_helm_set_dirichlet_dev(op, op->sfield.as<double>());
+    if (!rc) SE_HIP(hipStreamSynchronize(c->stream));
+    return rc;
+}
+
+extern "C" int se_helm_lift(se_op* op, double* b) {
+    SE_CHECK(op && b, SE_ERR_ARG, "se_helm_lift: NULL argument");
+    SE_HIP(hipSetDevice(op->mesh->ctx->device));
+    SE_HIP(hipStreamSynchronize(op->mesh->ctx->stream));
+    SE_HIP(hipMemcpy(b, op->lift.ptr, sizeof(double) * op->mesh->n, hipMemcpyDeviceToHost));
+    return SE_OK;
+}
+
+extern "C" int se_helm_rhs_dev(se_op* op, const double* d_rhs_field, double* d_f) {
+    SE_CHECK(op && d_rhs_field && d_f, SE_ERR_ARG, "se_helm_rhs_dev: NULL argument");
+    OpParams P = op_params(op, 1.0);
+    int rc = launch_tile<MODE_RHS, false>(op->mesh->N, 2, P, d_rhs_field, d_f, op->lift.as<double>(), nullptr,
+                                          nullptr, op->mesh->ctx->stream);
+    if (rc) return rc;
+    SE_HIP(hipGetLastError());
+    return SE_OK;
+}
+
+extern "C" int se_helm_rhs(se_op* op, const double* rhs_field, double* f) {
+    SE_CHECK(op && rhs_field && f, SE_ERR_ARG, "se_helm_rhs: NULL argument");
+    se_ctx* c = op->mesh->ctx;
+    SE_HIP(hipSetDevice(c->device));
+    int rc = stage_in(c, op->sfield, rhs_field, op->mesh->nV);
+    if (!rc) rc = op->sy.alloc(sizeof(double) * std::max(op->mesh->n, 1));
+    if (!rc) rc = se_helm_rhs_dev(op, op->sfield.as<double>(), op->sy.as<double>());
+    if (!rc) rc = stage_out(c, f, op->sy, op->mesh->n);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 Jacobi-PCG
+// ---------------------------------------------------------------------------
+namespace se {
+
+// one CG iteration: q = A p (+ p.q), update, direction
+static int cg_iteration(se_op* op, double* hist) {
+    const se_mesh* m = op->mesh;
+    hipStream_t s = m->ctx->stream;
+    int n = m->n, vb = vec_blocks(n);
+    CgState* st = op->state.as<CgState>();
+    double* part = op->partials.as<double>();
+    int rc = op_apply_dev(op, op->p.as<double>(), op->q.as<double>(), (double)op->sign, true);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_cg_update, dim3(vb), dim3(kBlock), 0, s, n, op->p.as<double>(), op->q.as<double>(),
+                       op->dinv_s.as<double>(), op->x.as<double>(), op->r.as<double>(), hist, part, st);
+    hipLaunchKernelGGL(k_cg_dir, dim3(vb), dim3(kBlock), 0, s, n, op->dinv_s.as<double>(), op->r.as<double>(),
+                       op->p.as<double>(), st);
+    return SE_OK;
+}
+
+static int cg_batch(se_op* op, int k, double* hist) {
+    hipStream_t s = op->mesh->ctx->stream;
+    static const bool use_graph = getenv("SE_NO_GRAPH") == nullptr;
+    if (use_graph && hist == nullptr) {
+        if (!op->graph_exec || op->graph_iters != k) {
+            if (op->graph_exec) {
+                (void)hipGraphExecDestroy(op->graph_exec);
+                op->graph_exec = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            SE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int rc = SE_OK;
+            for (int i = 0; i < k && !rc; i++) rc = cg_iteration(op, nullptr);
+            hipError_t e = hipStreamEndCapture(s, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(SE_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            e = hipGraphInstantiate(&op->graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                op->graph_exec = nullptr;
+                return fail(SE_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+            }
+            op->graph_iters = k;
+        }
+        SE_HIP(hipGraphLaunch(op->graph_exec, s));
+        return SE_OK;
+    }
+    for (int i = 0; i < k; i++) {
+        int rc = cg_iteration(op, hist);
+        if (rc) return rc;
+    }
+    return SE_OK;
+}
+
+// Solve sign*A x = sign*f on device.  f, x: device vectors (n).
+int pcg_fp64(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, se_solve_info& info) {
+    se_mesh* m = op->mesh;
+    se_ctx* c = m->ctx;
+    hipStream_t s = c->stream;
+    int n = m->n, vb = vec_blocks(n);
+    SE_CHECK(op->definite, SE_ERR_UNSUPPORTED, "operator diagonal changes sign: not a definite system");
+    int rc = op_ensure_work(op);
+    if (rc) return rc;
+    if (!op->dinv_s.ptr) {
+        rc = op->dinv_s.alloc(sizeof(double) * std::max(n, 1));
+        if (rc) return rc;
+        // inverse diagonal of sign*A
+        hipLaunchKernelGGL(k_scale_copy, dim3(vb), dim3(kBlock), 0, s, n, (double)op->sign, op->dinv.as<double>(),
+                           op->dinv_s.as<double>());
+    }
+    CgState h{};
+    h.maxit = std::max(o.maxit, 0);
+    h.null_space = o.null_space ? 1 : 0;
+    h.n = n;
+    h.status = CG_RUNNING;
+    CgState* st = op->state.as<CgState>();
+    double* part = op->partials.as<double>();
+    DevBuf d_hist;
+    if (o.history) {
+        rc = d_hist.alloc(sizeof(double) * (size_t)(h.maxit + 1));
+        if (rc) return rc;
+    }
+    SE_HIP(hipEventRecord(c->ev0, s));
+    SE_HIP(hipMemcpyAsync(st, &h, sizeof(CgState), hipMemcpyHostToDevice, s));
+    if (h.null_space)
+        hipLaunchKernelGGL(k_sum, dim3(vb), dim3(kBlock), 0, s, n, d_f, (double)op->sign, part, st);
+    hipLaunchKernelGGL(k_cg_start, dim3(vb), dim3(kBlock), 0, s, n, (double)op->sign, o.rtol, d_f,
+                       op->dinv_s.as<double>(), op->b.as<double>(), op->x.as<double>(), op->r.as<double>(), part, st);
+    hipLaunchKernelGGL(k_cg_dir, dim3(vb), dim3(kBlock), 0, s, n, op->dinv_s.as<double>(), op->r.as<double>(),
+                       op->p.as<double>(), st);
+    SE_HIP(hipGetLastError());
+    if (!op->h_state) SE_HIP(hipHostMalloc(&op->h_state, sizeof(CgState), hipHostMallocDefault));
+    CgState* hs = (CgState*)op->h_state;
+    double* dh = o.history ? d_hist.as<double>() : nullptr;
+    int k = std::max(1, o.check_every);
+    int guard = 0;
+    for (;;) {
+        int left = h.maxit;  // batches overrun harmlessly: kernels return once stop is set
+        (void)left;
+        rc = cg_batch(op, k, dh);
+        if (rc) return rc;
+        SE_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        SE_HIP(hipStreamSynchronize(s));
+        if (!hs->stop) continue;
+        if (hs->status != CG_PENDING) break;
+        // recursive residual below target: confirm with the true residual
+        rc = op_apply_dev(op, op->x.as<double>(), op->q.as<double>(), (double)op->sign, false);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_true_resid, dim3(vb), dim3(kBlock), 0, s, n, op->b.as<double>(), op->q.as<double>(),
+                           op->r.as<double>(), part, st);
+        SE_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        SE_HIP(hipStreamSynchronize(s));
+        if (hs->tt <= hs->tol2) {
+            hs->status = 0;
+            break;
+        }
+        if (hs->iter >= h.maxit || ++guard > 1000) {
+            hs->status = 1;
+            break;
+        }
+        hipLaunchKernelGGL(k_cg_restart, dim3(vb), dim3(kBlock), 0, s, n, op->dinv_s.as<double>(),
+                           op->r.as<double>(), part, st);
+        hipLaunchKernelGGL(k_cg_dir, dim3(vb), dim3(kBlock), 0, s, n, op->dinv_s.as<double>(), op->r.as<double>(),
+                           op->p.as<double>(), st);
+    }
+    int status = hs->status;
+    int iters = hs->iter;
+    double bb = hs->bb, tt = hs->tt;
+    if (status != 0 && bb > 0.0) {
+        rc = op_apply_dev(op, op->x.as<double>(), op->q.as<double>(), (double)op->sign, false);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_true_resid, dim3(vb), dim3(kBlock), 0, s, n, op->b.as<double>(), op->q.as<double>(),
+                           op->r.as<double>(), part, st);
+        SE_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        SE_HIP(hipStreamSynchronize(s));
+        tt = hs->tt;
+    }
+    if (d_x != op->x.as<double>())
+        SE_HIP(hipMemcpyAsync(d_x, op->x.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    SE_HIP(hipEventRecord(c->ev1, s));
+    SE_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    if (o.history) {
+        o.history[0] = 1.0;
+        if (iters > 0)
+            SE_HIP(hipMemcpy(o.history + 1, d_hist.as<double>() + 1, sizeof(double) * iters, hipMemcpyDeviceToHost));
+    }
+    info.status = status == CG_RUNNING ? 1 : status;
+    info.iters = iters;
+    info.outer_iters = 1;
+    info.true_rel_residual = bb > 0.0 ? std::sqrt(std::max(tt, 0.0) / bb) : 0.0;
+    info.seconds = ms * 1e-3;
+    return SE_OK;
+}
+
+}  // namespace se
+
+extern "C" int se_helm_pcg_dev(se_op* op, const double* d_f, double* d_x, const se_solve_opts* opts,
+                               se_solve_info* info) {
+    SE_CHECK(op && d_f && d_x, SE_ERR_ARG, "se_helm_pcg_dev: NULL argument");
+    se_solve_opts o;
+    se_solve_opts_default(&o);
+    if (opts) o = *opts;
+    se_solve_info inf{};
+    SE_HIP(hipSetDevice(op->mesh->ctx->device));
+    int rc;
+    if (o.precision == SE_PREC_FP64) rc = pcg_fp64(op, d_f, d_x, o, inf);
+    else rc = pcg_mixed(op, d_f, d_x, o, inf);
+    if (info) *info = inf;
+    return rc;
+}
+
+extern "C" int se_helm_pcg(se_op* op, const double* f, double* x, const se_solve_opts* opts, se_solve_info* info) {
+    SE_CHECK(op && f && x, SE_ERR_ARG, "se_helm_pcg: NULL argument");
+    se_ctx* c = op->mesh->ctx;
+    SE_HIP(hipSetDevice(c->device));
+    int n = op->mesh->n;
+    int rc = stage_in(c, op->f, f, n);
+    if (rc) return rc;
+    rc = se_helm_pcg_dev(op, op->f.as<double>(), op->x.as<double>(), opts, info);
+    if (!rc) rc = stage_out(c, x, op->x, n);
+    return rc;
+}
+
+extern "C" int se_helm_solve_dev(se_op* op, const double* d_rhs_field, double* d_sol_field, const se_solve_opts* opts,
+                                 se_solve_info* info) {
+    SE_CHECK(op && d_rhs_field && d_sol_field, SE_ERR_ARG, "se_helm_solve_dev: NULL argument");
+    int rc = op_ensure_work(op);
+    if (!rc) rc = se_helm_rhs_dev(op, d_rhs_field, op->f.as<double>());
+    if (!rc) rc = se_helm_pcg_dev(op, op->f.as<double>(), op->x.as<double>(), opts, info);
+    if (!rc) rc = se_vec_to_field_dev(op->mesh, op->x.as<double>(), d_sol_field);
+    return rc;
+}
+
+extern "C" int se_helm_solve(se_op* op, const double* rhs_field, double* sol_field, const se_solve_opts* opts,
+                             se_solve_info* info) {
+    SE_CHECK(op && rhs_field && sol_field, SE_ERR_ARG, "se_helm_solve: NULL argument");
+    se_ctx* c = op->mesh->ctx;
+    SE_HIP(hipSetDevice(c->device));
+    size_t nV = op->mesh->nV;
+    int rc = stage_in(c, op->sfield, rhs_field, nV);
+    if (!rc) rc = stage_in(c, op->sfield2, sol_field, nV);
+    if (!rc) rc = se_helm_solve_dev(op, op->sfield.as<double>(), op->sfield2.as<double>(), opts, info);
+    if (!rc) rc = stage_out(c, sol_field, op->sfield2, nV);
+    return rc;
+}

@@ -1,0 +1,361 @@
+// helm_kernels.h -- device code of the matrix-free GLL Helmholtz operator.
+//
+// One workgroup owns a tile of TX x TY elements and produces the assembled
+// result on the tile's nodes (the half-open node block of its elements plus,
+// on a non-periodic domain edge, the closing line).  Each node value is the
+// sum of the contributions of every element containing the node (up to four,
+// including the periodic wrap), so the tile also evaluates the element column
+// to its left and the element row below it: no atomics, no second pass, and
+// the result is bitwise deterministic.
+//
+// Per element the operator is applied by sum factorisation with the exact
+// per-point 2x2 Jacobian of the reference (Mesh::ShapeFuncDerivs,
+// src/base/Mesh.cpp:217-250), rebuilt from small per-column / per-row tables:
+//   u_xi(q)  = sum_i D[qx][i] u(i,qy),   u_eta(q) = sum_j D[qy][j] u(qx,j)
+//   g = Jinv (u_xi, u_eta),  f = W_q Jinv^T g
+//   (K u)(r) = sum_qx D[qx][rx] f_xi(qx,ry) + sum_qy D[qy][ry] f_eta(rx,qy)
+// with W_q = w_qx w_qy c_lap (literal mode, detJ := 1 as in
+// Laplacian::AssembleElement, src/base/Operator.cpp:72) or det_q w_qx w_qy c_lap
+// (physical), plus the GLL-diagonal mass det_q w_qx w_qy c_mass.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace se {
+
+enum TileMode { MODE_APPLY = 0, MODE_LIFT = 1, MODE_RHS = 2, MODE_DIAG = 3 };
+
+struct OpParams {
+    int nEls0, nEls1, nv0, nv1, per0, per1;
+    int detj;              // 0 literal, 1 physical
+    double c_mass, c_lap;  // operator coefficients (reference sign)
+    double scale;          // output scale (+1, or -1 for the sign-normalised solve)
+    const int32_t* f2v;
+    const double* J00;  // [nEls0][N+1]
+    const double* J10;  // [nEls0][N+1 (qx)][N+1 (qy)]
+    const double* J11;  // [nEls1][N+1]
+    const double* J01;  // [nEls1][N+1 (qy)][N+1 (qx)]
+    const double* w;    // [N+1]
+    const double* D;    // [(N+1)^2]
+};
+
+// deterministic two-level reduction state: per-block partials, a counter, and
+// the scalars of the CG recurrence.  Lives in device memory.
+struct CgState {
+    double rho, pq, alpha, beta, rr, bb, tol2, mu, tt;
+    double nrm_f;   // ||f|| of the sign-normalised right-hand side
+    int iter, stop, status, maxit, first, null_space, n, pad;
+    unsigned counter;
+};
+
+enum CgStatus { CG_RUNNING = -1, CG_PENDING = -2 };
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block-reduce K values (blockDim 256), publish per-block partials and return
+// true in the last block to arrive, whose thread 0 then holds the totals in
+// tot[] (summed in fixed block order, independent of arrival order).
+template <int K>
+__device__ __forceinline__ bool block_reduce_last(const double (&v)[K], double* partials, unsigned* counter,
+                                                  double (&tot)[K]) {
+    __shared__ double s_red[K][4];
+    __shared__ int s_last;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double s = wave_sum(v[k]);
+        if (lane == 0) s_red[k][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            double s = ((s_red[k][0] + s_red[k][1]) + s_red[k][2]) + s_red[k][3];
+            __hip_atomic_store(&partials[(size_t)k * gridDim.x + blockIdx.x], s, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double s = 0.0;
+        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
+            s += __hip_atomic_load(&partials[(size_t)k * gridDim.x + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s = wave_sum(s);
+        if (lane == 0) s_red[k][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) tot[k] = ((s_red[k][0] + s_red[k][1]) + s_red[k][2]) + s_red[k][3];
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return true;
+}
+
+// Per-point inverse Jacobian and weights, in the reference's operation order
+// (Mesh.cpp:227-236, Operator.cpp:69-73, 165-167).
+struct PointGeom {
+    double i00, i01, i10, i11, det;
+};
+
+__device__ __forceinline__ PointGeom point_geom(double j00, double j01, double j10, double j11) {
+#pragma clang fp contract(off)
+    PointGeom g;
+    g.det = j00 * j11 - j01 * j10;
+    g.i00 = j11 / g.det;
+    g.i11 = j00 / g.det;
+    g.i01 = -j01 / g.det;
+    g.i10 = -j10 / g.det;
+    return g;
+}
+
+// LDS column / row slot of local node index i of tile element column c.  The
+// line shared by element columns 0 and 1 is stored twice, so that element 0
+// (the left neighbour, possibly wrapped across a periodic seam) keeps its own
+// view of that line; every other shared line is stored once.
+template <int N>
+__device__ __forceinline__ int slot(int c, int i) {
+    return c == 0 ? i : (N + 1) + (c - 1) * N + i;
+}
+
+template <int N, int TX, int TY, int MODE, bool DOT>
+__global__ __launch_bounds__(256) void k_helm_tile(OpParams P, const double* __restrict__ in,
+                                                    double* __restrict__ out, const double* __restrict__ aux,
+                                                    double* __restrict__ partials, CgState* __restrict__ st) {
+    constexpr int N1 = N + 1;
+    constexpr int NP = N1 * N1;
+    constexpr int BW = (TX + 1) * N + 2;
+    constexpr int BH = (TY + 1) * N + 2;
+    constexpr int NE = (TX + 1) * (TY + 1);
+
+    if (DOT && st->stop) return;
+
+    __shared__ double s_u[MODE == MODE_DIAG ? 1 : BH * BW];
+    __shared__ double s_fx[MODE == MODE_APPLY || MODE == MODE_LIFT ? NE * NP : 1];
+    __shared__ double s_fy[MODE == MODE_APPLY || MODE == MODE_LIFT ? NE * NP : 1];
+    __shared__ double s_D[NP];
+    __shared__ double s_w[N1];
+    __shared__ double s_J00[(TX + 1) * N1], s_J10[(TX + 1) * NP];
+    __shared__ double s_J11[(TY + 1) * N1], s_J01[(TY + 1) * NP];
+    __shared__ int s_exw[TX + 1], s_eyw[TY + 1];
+
+    const int tid = threadIdx.x;
+    const int tilesX = (P.nEls0 + TX - 1) / TX;
+    const int ex0 = (blockIdx.x % tilesX) * TX;
+    const int ey0 = (blockIdx.x / tilesX) * TY;
+
+    // wrapped element indices of the tile columns / rows (-1: none)
+    if (tid <= TX) {
+        int ex = ex0 - 1 + tid;
+        if (ex < 0) ex = P.per0 ? ex + P.nEls0 : -1;
+        if (ex >= P.nEls0) ex = -1;
+        s_exw[tid] = ex;
+    }
+    if (tid >= 64 && tid <= 64 + TY) {
+        int ey = ey0 - 1 + (tid - 64);
+        if (ey < 0) ey = P.per1 ? ey + P.nEls1 : -1;
+        if (ey >= P.nEls1) ey = -1;
+        s_eyw[tid - 64] = ey;
+    }
+    for (int i = tid; i < NP; i += 256) s_D[i] = P.D[i];
+    if (tid < N1) s_w[tid] = P.w[tid];
+    __syncthreads();
+    for (int i = tid; i < (TX + 1) * N1; i += 256) {
+        int c = i / N1, q = i % N1, e = s_exw[c];
+        s_J00[i] = e >= 0 ? P.J00[(size_t)e * N1 + q] : 1.0;
+    }
+    for (int i = tid; i < (TX + 1) * NP; i += 256) {
+        int c = i / NP, q = i % NP, e = s_exw[c];
+        s_J10[i] = e >= 0 ? P.J10[(size_t)e * NP + q] : 0.0;
+    }
+    for (int i = tid; i < (TY + 1) * N1; i += 256) {
+        int r = i / N1, q = i % N1, e = s_eyw[r];
+        s_J11[i] = e >= 0 ? P.J11[(size_t)e * N1 + q] : 1.0;
+    }
+    for (int i = tid; i < (TY + 1) * NP; i += 256) {
+        int r = i / NP, q = i % NP, e = s_eyw[r];
+        s_J01[i] = e >= 0 ? P.J01[(size_t)e * NP + q] : 0.0;
+    }
+
+    // ---- gather the tile's node block through the numbering map ----------
+    if (MODE != MODE_DIAG) {
+        for (int i = tid; i < BH * BW; i += 256) {
+            int a = i / BW, b = i % BW;
+            int c, ix, r, iy;
+            if (b <= N) { c = 0; ix = b; }
+            else { int bb = b - N1; c = 1 + bb / N; ix = bb % N; if (ix == 0 && c >= 2) { c -= 1; ix = N; } }
+            if (a <= N) { r = 0; iy = a; }
+            else { int aa = a - N1; r = 1 + aa / N; iy = aa % N; if (iy == 0 && r >= 2) { r -= 1; iy = N; } }
+            int ex = s_exw[c], ey = s_eyw[r];
+            double val = 0.0;
+            if (ex >= 0 && ey >= 0) {
+                int v = (ey * N + iy) * P.nv0 + ex * N + ix;
+                if (MODE == MODE_APPLY) {
+                    int u = P.f2v[v];
+                    val = u >= 0 ? in[u] : 0.0;
+                } else if (MODE == MODE_LIFT) {
+                    val = P.f2v[v] < 0 ? in[v] : 0.0;
+                } else {
+                    val = in[v];
+                }
+            }
+            s_u[i] = val;
+        }
+    }
+    __syncthreads();
+
+    // ---- per-point gradient, metric, flux --------------------------------
+    if (MODE == MODE_APPLY || MODE == MODE_LIFT) {
+        for (int i = tid; i < NE * NP; i += 256) {
+            int e = i / NP, q = i % NP;
+            int c = e % (TX + 1), r = e / (TX + 1);
+            int qx = q % N1, qy = q / N1;
+            double fx = 0.0, fy = 0.0;
+            if (s_exw[c] >= 0 && s_eyw[r] >= 0) {
+                const double* row = &s_u[slot<N>(r, qy) * BW];
+                double ux = 0.0, uy = 0.0;
+#pragma unroll
+                for (int k = 0; k < N1; k++) {
+                    ux += s_D[qx * N1 + k] * row[slot<N>(c, k)];
+                    uy += s_D[qy * N1 + k] * s_u[slot<N>(r, k) * BW + slot<N>(c, qx)];
+                }
+                PointGeom g = point_geom(s_J00[c * N1 + qx], s_J01[r * NP + qy * N1 + qx], s_J10[c * NP + qx * N1 + qy],
+                                         s_J11[r * N1 + qy]);
+                double W;
+                {
+#pragma clang fp contract(off)
+                    double wq = s_w[qx] * s_w[qy];
+                    W = (P.detj ? (g.det * wq) : (1.0 * wq)) * P.c_lap;
+                }
+                double gx = g.i00 * ux + g.i01 * uy;
+                double gy = g.i10 * ux + g.i11 * uy;
+                fx = W * (g.i00 * gx + g.i10 * gy);
+                fy = W * (g.i01 * gx + g.i11 * gy);
+            }
+            s_fx[i] = fx;
+            s_fy[i] = fy;
+        }
+        __syncthreads();
+    }
+
+    // ---- owned nodes: sum element contributions --------------------------
+    const int txo = min(TX, P.nEls0 - ex0), tyo = min(TY, P.nEls1 - ey0);
+    const int ownX = txo * N + ((!P.per0 && ex0 + txo == P.nEls0) ? 1 : 0);
+    const int ownY = tyo * N + ((!P.per1 && ey0 + tyo == P.nEls1) ? 1 : 0);
+    double dot = 0.0;
+    for (int i = tid; i < ownX * ownY; i += 256) {
+        int ox = i % ownX, oy = i / ownX;
+        int gI = ex0 * N + ox, gJ = ey0 * N + oy;
+        int u = P.f2v[gJ * P.nv0 + gI];
+        if (u < 0) continue;
+        // containing elements: (tile column, local index) pairs
+        int cs[2], xs[2], nc = 0, rs[2], ys[2], nr = 0;
+        if (ox % N == 0) { cs[nc] = ox / N; xs[nc] = N; nc++; }
+        if (ox / N < txo) { cs[nc] = 1 + ox / N; xs[nc] = ox % N; nc++; }
+        if (oy % N == 0) { rs[nr] = oy / N; ys[nr] = N; nr++; }
+        if (oy / N < tyo) { rs[nr] = 1 + oy / N; ys[nr] = oy % N; nr++; }
+        double sum = 0.0, own = 0.0;
+        for (int jr = 0; jr < nr; jr++) {
+            const int r = rs[jr], ry = ys[jr];
+            if (s_eyw[r] < 0) continue;
+            for (int jc = 0; jc < nc; jc++) {
+                const int c = cs[jc], rx = xs[jc];
+                if (s_exw[c] < 0) continue;
+                PointGeom g = point_geom(s_J00[c * N1 + rx], s_J01[r * NP + ry * N1 + rx], s_J10[c * NP + rx * N1 + ry],
+                                         s_J11[r * N1 + ry]);
+                double wq;
+                {
+#pragma clang fp contract(off)
+                    wq = s_w[rx] * s_w[ry];
+                }
+                if (MODE == MODE_APPLY || MODE == MODE_LIFT) {
+                    const int e = r * (TX + 1) + c;
+                    const double* fx = &s_fx[e * NP];
+                    const double* fy = &s_fy[e * NP];
+                    double k = 0.0;
+#pragma unroll
+                    for (int q = 0; q < N1; q++) {
+                        k += s_D[q * N1 + rx] * fx[ry * N1 + q];
+                        k += s_D[q * N1 + ry] * fy[q * N1 + rx];
+                    }
+                    double uv = s_u[slot<N>(r, ry) * BW + slot<N>(c, rx)];
+                    double mass;
+                    {
+#pragma clang fp contract(off)
+                        mass = (g.det * wq) * P.c_mass;
+                    }
+                    sum += k + mass * uv;
+                    own = uv;
+                } else if (MODE == MODE_RHS) {
+                    double fv = s_u[slot<N>(r, ry) * BW + slot<N>(c, rx)];
+                    double m;
+                    {
+#pragma clang fp contract(off)
+                        m = ((g.det * wq) * 1.0) * fv;
+                    }
+                    sum += m;
+                } else {  // MODE_DIAG: element diagonal K~[r][r] + M[r][r]
+                    double d = 0.0;
+                    // quadrature points on row ry: q = (qx, ry)
+                    for (int qx = 0; qx < N1; qx++) {
+                        PointGeom h = point_geom(s_J00[c * N1 + qx], s_J01[r * NP + ry * N1 + qx],
+                                                 s_J10[c * NP + qx * N1 + ry], s_J11[r * N1 + ry]);
+                        double a0 = s_D[qx * N1 + rx], a1 = (qx == rx) ? s_D[ry * N1 + ry] : 0.0;
+                        double g0 = a0 * h.i00 + a1 * h.i01, g1 = a0 * h.i10 + a1 * h.i11;
+                        double wqq = s_w[qx] * s_w[ry];
+                        double W = (P.detj ? h.det * wqq : wqq) * P.c_lap;
+                        d += W * (g0 * g0 + g1 * g1);
+                    }
+                    // points on column rx other than the node's own row
+                    for (int qy = 0; qy < N1; qy++) {
+                        if (qy == ry) continue;
+                        PointGeom h = point_geom(s_J00[c * N1 + rx], s_J01[r * NP + qy * N1 + rx],
+                                                 s_J10[c * NP + rx * N1 + qy], s_J11[r * N1 + qy]);
+                        double a1 = s_D[qy * N1 + ry];
+                        double g0 = a1 * h.i01, g1 = a1 * h.i11;
+                        double wqq = s_w[rx] * s_w[qy];
+                        double W = (P.detj ? h.det * wqq : wqq) * P.c_lap;
+                        d += W * (g0 * g0 + g1 * g1);
+                    }
+                    sum += d + (g.det * wq) * P.c_mass;
+                }
+            }
+        }
+        if (MODE == MODE_APPLY) {
+            double yv = P.scale * sum;
+            out[u] = yv;
+            if (DOT) dot += own * yv;
+        } else if (MODE == MODE_LIFT) {
+            out[u] = -sum;
+        } else if (MODE == MODE_RHS) {
+            out[u] = aux ? sum + 1.0 * aux[u] : sum;
+        } else {
+            out[u] = 1.0 / (P.scale * sum);
+        }
+    }
+
+    if (DOT) {
+        double v[1] = {dot}, tot[1];
+        if (block_reduce_last<1>(v, partials, &st->counter, tot) && tid == 0) {
+            double pq = tot[0];
+            st->pq = pq;
+            if (!(pq > 0.0)) {
+                st->stop = 1;
+                st->status = (pq == pq && pq - pq == 0.0) ? 2 : 3;
+            } else {
+                st->alpha = st->rho / pq;
+            }
+        }
+    }
+}
+
+}  // namespace se

@@ -1,0 +1,45 @@
+// se_op.h -- operator object shared by the FP64 and mixed-precision solvers.
+#pragma once
+#include "helm_kernels.h"
+#include "se_common.h"
+
+struct se_op {
+    se_mesh* mesh = nullptr;
+    double c_mass = 0.0, c_lap = 0.0;
+    int detj = 0;
+    int tile = 2;
+    int sign = 1;          // sign of the operator diagonal (solve runs on sign*A)
+    bool definite = true;
+    se::DevBuf dinv;       // inverse diagonal of A
+    se::DevBuf dinv_s;     // inverse diagonal of sign*A
+    se::DevBuf lift;       // Dirichlet lifting b
+    se::DevBuf state, partials;
+    se::DevBuf b, x, r, p, q, f;             // FP64 CG work vectors
+    se::DevBuf sx, sy, sfield, sfield2;      // host-staging buffers
+    // mixed-precision work (allocated on first use)
+    se::DevBuf lo_r, lo_p, lo_q, lo_z, lo_x, lo_dinv, lo_geo, d64;
+    void* h_state = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_iters = 0;
+    hipGraphExec_t graph_exec_lo = nullptr;
+    int graph_iters_lo = 0;
+
+    void release_all() {
+        se::DevBuf* bufs[] = {&dinv, &dinv_s, &lift, &state, &partials, &b, &x, &r, &p, &q, &f,
+                              &sx, &sy, &sfield, &sfield2, &lo_r, &lo_p, &lo_q, &lo_z, &lo_x, &lo_dinv,
+                              &lo_geo, &d64};
+        for (auto* b : bufs) b->release();
+        if (graph_exec_lo) (void)hipGraphExecDestroy(graph_exec_lo);
+        graph_exec_lo = nullptr;
+        if (h_state) (void)hipHostFree(h_state);
+        h_state = nullptr;
+    }
+};
+
+namespace se {
+OpParams op_params(const se_op* op, double scale);
+int op_apply_dev(se_op* op, const double* d_x, double* d_y, double scale, bool dot);
+int op_ensure_work(se_op* op);
+int pcg_fp64(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, se_solve_info& info);
+int pcg_mixed(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, se_solve_info& info);
+}  // namespace se

@@ -1,0 +1,522 @@
+// sl.hip -- semi-Lagrangian advection (departure-point search + GLL
+// interpolation), velocity from the stream function, CFL time step.
+//
+// One thread per field node.  The coordinate arithmetic that decides the
+// departure element (RK2 midpoint, wrap/clamp, GetEl's modf and its 1e-12
+// edge rule, the local-coordinate map) follows the reference's operation order
+// with FP contraction disabled, so departure elements agree with the
+// reference bit for bit except where a coordinate lies within rounding of an
+// element edge.  P_N(x_j) is tabulated on the host with the reference's
+// evaluation; P_N'(x) is evaluated on the device with the same monomial sum.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "se_common.h"
+
+using namespace se;
+
+namespace {
+
+struct SLGeom {
+    int N, nEls0, nEls1, nv0, nv1, per0, per1;
+    double min0, min1, max0, max1, dx0, dx1;
+    const double* xk;   // abscissae
+    const double* D;    // derivative matrix
+    const double* pnx;  // P_N(x_j)
+    const double* pn;   // P_N monomial coefficients
+    const double* X;    // field column coordinates
+    const double* Y;    // field row coordinates
+    const double* J00;
+    const double* J10;
+    const double* J11;
+    const double* J01;
+};
+
+#define SE_NUMAPPROX(var, val) ((var) >= (val)-1.0e-12 && (var) <= (val) + 1.0e-12)
+
+// Mesh::GetEl, src/base/Mesh.cpp:106-137
+__device__ __forceinline__ int get_el(const SLGeom& g, double cx, double cy) {
+#pragma clang fp contract(off)
+    int t0, t1;
+    double integer, frac;
+    if (SE_NUMAPPROX(cx - g.max0, 0.0)) {
+        t0 = g.nEls0 - 1;
+    } else {
+        double out = (cx - g.min0) / g.dx0;
+        frac = modf(out, &integer);
+        t0 = (int)integer;
+        if (t0 > 0 && SE_NUMAPPROX(frac, 0.0)) t0--;
+    }
+    if (SE_NUMAPPROX(cy - g.max1, 0.0)) {
+        t1 = g.nEls1 - 1;
+    } else {
+        double out = (cy - g.min1) / g.dx1;
+        frac = modf(out, &integer);
+        t1 = (int)integer;
+        if (t1 > 0 && SE_NUMAPPROX(frac, 0.0)) t1--;
+    }
+    // A coordinate outside the domain (displacement beyond one period, or a
+    // non-finite velocity) has no element in the reference either; clamp so
+    // the gather stays inside the field arrays.
+    t0 = t0 < 0 ? 0 : (t0 >= g.nEls0 ? g.nEls0 - 1 : t0);
+    t1 = t1 < 0 ? 0 : (t1 >= g.nEls1 ? g.nEls1 - 1 : t1);
+    return t1 * g.nEls0 + t0;
+}
+
+// Advector::PeriodicUpdate, src/base/Advector.cpp:122-134
+__device__ __forceinline__ void wrap(const SLGeom& g, double& cx, double& cy) {
+#pragma clang fp contract(off)
+    if (cx < g.min0) cx = g.per0 ? g.max0 - g.min0 + cx : g.min0;
+    if (cx > g.max0) cx = g.per0 ? g.min0 - g.max0 + cx : g.max0;
+    if (cy < g.min1) cy = g.per1 ? g.max1 - g.min1 + cy : g.min1;
+    if (cy > g.max1) cy = g.per1 ? g.min1 - g.max1 + cy : g.max1;
+}
+
+// Jacobi::EvalDeriv (Jacobi.cpp:58-66) of P_N
+__device__ __forceinline__ double pn_deriv(const SLGeom& g, double x) {
+#pragma clang fp contract(off)
+    double dp = 0.0;
+    for (int i = g.N; i >= 1; i -= 2) dp += g.pn[i] * i * pow(x, (double)(i - 1));
+    return dp;
+}
+
+// 1-D GLL cardinal functions at local coordinate x (Legendre::IsRoot and the
+// per-dimension factor of ShapeFuncsAtCoord, Legendre.cpp:145-169,291-305)
+template <int MAXN>
+__device__ __forceinline__ void cardinal(const SLGeom& g, double x, double (&L)[MAXN + 1]) {
+#pragma clang fp contract(off)
+    const int N = g.N;
+    int root = -1;
+    for (int k = 0; k <= N; k++)
+        if (fabs(x - g.xk[k]) < 1.0e-8) {
+            root = k;
+            break;
+        }
+    if (root >= 0) {
+        for (int j = 0; j <= N; j++) L[j] = (j == root) ? 1.0 : 0.0;
+        return;
+    }
+    const double dpx = pn_deriv(g, x);
+    for (int j = 0; j <= N; j++)
+        L[j] = -((1.0 - x * x) / (N * (N + 1) * g.pnx[j] * (x - g.xk[j]))) * dpx;
+}
+
+// Field::InterpGlobal (Field.cpp:57-76) through Mesh::GlobalToLocal
+// (Mesh.cpp:163-177); vals node-major with ND dofs
+template <int MAXN, int ND>
+__device__ __forceinline__ void interp_global(const SLGeom& g, const double* __restrict__ vals, int nd, double cx,
+                                              double cy, double (&out)[ND]) {
+#pragma clang fp contract(off)
+    const int N = g.N;
+    int el = get_el(g, cx, cy);
+    int ex = el % g.nEls0, ey = el / g.nEls0;
+    double lx = (cx - g.min0 - ex * g.dx0) * (2.0 / g.dx0) - 1.0;
+    double ly = (cy - g.min1 - ey * g.dx1) * (2.0 / g.dx1) - 1.0;
+    if (lx < -1.0) lx = -1;
+    if (lx > +1.0) lx = +1;
+    if (ly < -1.0) ly = -1;
+    if (ly > +1.0) ly = +1;
+    double Lx[MAXN + 1], Ly[MAXN + 1];
+    cardinal<MAXN>(g, lx, Lx);
+    cardinal<MAXN>(g, ly, Ly);
+    for (int d = 0; d < ND; d++) out[d] = 0.0;
+    const int n0 = N * ey * g.nv0 + N * ex;
+    for (int jj = 0; jj <= N; jj++)
+        for (int ii = 0; ii <= N; ii++) {
+            double Ni = Lx[ii] * Ly[jj];
+            const double* v = &vals[(size_t)(n0 + jj * g.nv0 + ii) * nd];
+            for (int d = 0; d < ND; d++)
+                if (d < nd) out[d] += Ni * v[d];
+        }
+}
+
+// Advector::InterpLagrange, Advector.cpp:248-275
+template <int ND>
+__device__ __forceinline__ void interp_lagrange(double x, const double* c, double (&vals)[4][ND], double (&psi)[ND]) {
+#pragma clang fp contract(off)
+    for (int d = 0; d < ND; d++) psi[d] = 0.0;
+    for (int node = 0; node < 4; node++) {
+        double factor = 1.0;
+        for (int o = 0; o < 4; o++)
+            if (o != node) factor *= (x - c[o]) / (c[node] - c[o]);
+        for (int d = 0; d < ND; d++) psi[d] += vals[node][d] * factor;
+    }
+}
+
+// Advector::Interpolate (bicubic on the 4x4 node stencil, used when N < 3),
+// Advector.cpp:278-337
+template <int ND>
+__device__ __forceinline__ void interp_bicubic(const SLGeom& g, const double* __restrict__ vals, int nd, double cx,
+                                               double cy, double (&out)[ND]) {
+#pragma clang fp contract(off)
+    const int N = g.N;
+    int el = get_el(g, cx, cy);
+    int ex = el % g.nEls0, ey = el / g.nEls0;
+    int node0 = N * ey * g.nv0 + N * ex;
+    int x0 = node0 % g.nv0;
+    int y0 = (node0 / g.nv0) % g.nv1;
+    double ddx = g.dx0 / N, ddy = g.dx1 / N;
+    if (cx <= g.X[x0 + 1]) x0--;
+    if (cy <= g.Y[ey * N + 3 / (N + 1)]) y0--;  // row of entry 3 of the element's node list
+    if (cx <= g.min0 + ddx) x0++;
+    else if (cx >= g.max0 - ddx) x0--;
+    if (cy <= g.min1 + ddy) y0++;
+    else if (cy >= g.max1 - ddy) y0--;
+    x0 = x0 < 0 ? 0 : (x0 > g.nv0 - 4 ? g.nv0 - 4 : x0);
+    y0 = y0 < 0 ? 0 : (y0 > g.nv1 - 4 ? g.nv1 - 4 : y0);
+    double px[4], py[4], pts[4][ND], rows[4][ND];
+    for (int i = 0; i < 4; i++) px[i] = g.X[x0 + i];
+    for (int j = 0; j < 4; j++) py[j] = g.Y[y0 + j];
+    for (int j = 0; j < 4; j++) {
+        for (int i = 0; i < 4; i++)
+            for (int d = 0; d < ND; d++) pts[i][d] = d < nd ? vals[(size_t)(x0 + i + (y0 + j) * g.nv0) * nd + d] : 0.0;
+        double r[ND];
+        interp_lagrange<ND>(cx, px, pts, r);
+        for (int d = 0; d < ND; d++) rows[j][d] = r[d];
+    }
+    interp_lagrange<ND>(cy, py, rows, out);
+}
+
+template <int MAXN, int ND>
+__device__ __forceinline__ void interp_any(const SLGeom& g, const double* __restrict__ vals, int nd, double cx,
+                                           double cy, double (&out)[ND]) {
+    if (g.N < 3) interp_bicubic<ND>(g, vals, nd, cx, cy, out);
+    else interp_global<MAXN, ND>(g, vals, nd, cx, cy, out);
+}
+
+// Advector::FirstOrder + IntegrateRK2 (Advector.cpp:136-148,181-201)
+template <int MAXN>
+__global__ __launch_bounds__(256) void k_sl_first(SLGeom g, const double* __restrict__ q, int nd,
+                                                   const double* __restrict__ vel, double dt, double* __restrict__ qsl,
+                                                   int32_t* __restrict__ dep) {
+#pragma clang fp contract(off)
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= g.nv0 * g.nv1) return;
+    const double ox = g.X[v % g.nv0], oy = g.Y[v / g.nv0];
+    // velocity at the origin: the origin is a GLL node, so InterpGlobal
+    // returns the nodal value exactly (IsRoot selects a Kronecker delta)
+    double vel0[2];
+    interp_global<MAXN, 2>(g, vel, 2, ox, oy, vel0);
+    double hx = ox - 0.5 * dt * vel0[0];
+    double hy = oy - 0.5 * dt * vel0[1];
+    wrap(g, hx, hy);
+    double vh[2];
+    interp_global<MAXN, 2>(g, vel, 2, hx, hy, vh);
+    double cx = ox - dt * vh[0];
+    double cy = oy - dt * vh[1];
+    wrap(g, cx, cy);
+    double val[3];
+    interp_any<MAXN, 3>(g, q, nd, cx, cy, val);
+    for (int d = 0; d < nd; d++) qsl[(size_t)v * nd + d] = val[d];
+    if (dep) dep[v] = get_el(g, cx, cy);
+}
+
+// Advector::ExtrapolateVelocities (Advector.cpp:339-349)
+__global__ void k_extrap(int n2, const double* __restrict__ vel, const double* __restrict__ velPrev,
+                         double* __restrict__ vp1, double* __restrict__ vph, double* __restrict__ vmh) {
+#pragma clang fp contract(off)
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n2) return;
+    double a = vel[i], b = velPrev[i];
+    vp1[i] = 2.0 * a - 1.0 * b;
+    vph[i] = 1.5 * a - 0.5 * b;
+    vmh[i] = 0.5 * a + 0.5 * b;
+}
+
+// Advector::SecondOrder (Advector.cpp:203-246)
+template <int MAXN>
+__global__ __launch_bounds__(256) void k_sl_second(SLGeom g, const double* __restrict__ q, const double* __restrict__ qp,
+                                                    int nd, const double* __restrict__ vel,
+                                                    const double* __restrict__ vp1, const double* __restrict__ vph,
+                                                    const double* __restrict__ vmh, double dt,
+                                                    double* __restrict__ qsl) {
+#pragma clang fp contract(off)
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= g.nv0 * g.nv1) return;
+    const double ox = g.X[v % g.nv0], oy = g.Y[v / g.nv0];
+    double w[2];
+    interp_global<MAXN, 2>(g, vp1, 2, ox, oy, w);
+    double hx = ox - 0.5 * dt * w[0], hy = oy - 0.5 * dt * w[1];
+    wrap(g, hx, hy);
+    interp_global<MAXN, 2>(g, vph, 2, hx, hy, w);
+    double x1 = ox - dt * w[0], y1 = oy - dt * w[1];
+    wrap(g, x1, y1);
+    interp_global<MAXN, 2>(g, vel, 2, x1, y1, w);
+    hx = x1 - 0.5 * dt * w[0];
+    hy = y1 - 0.5 * dt * w[1];
+    wrap(g, hx, hy);
+    interp_global<MAXN, 2>(g, vmh, 2, hx, hy, w);
+    double x2 = x1 - dt * w[0], y2 = y1 - dt * w[1];
+    wrap(g, x2, y2);
+    double p1[3], p2[3];
+    interp_any<MAXN, 3>(g, q, nd, x1, y1, p1);
+    interp_any<MAXN, 3>(g, qp, nd, x2, y2, p2);
+    for (int d = 0; d < nd; d++) qsl[(size_t)v * nd + d] = 2.0 * p1[d] - 0.5 * p2[d];
+}
+
+// Field::PeriodicUpdate (Field.cpp:232-252) applied to the advected field
+__global__ void k_periodic_copy(int nv0, int nv1, int per0, int per1, int nd, double* __restrict__ f) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv0 * nv1) return;
+    int I = v % nv0, J = v / nv0;
+    bool dupI = per0 && I == nv0 - 1, dupJ = per1 && J == nv1 - 1;
+    if (!dupI && !dupJ) return;
+    int src = (dupJ ? 0 : J) * nv0 + (dupI ? 0 : I);
+    for (int d = 0; d < nd; d++) f[(size_t)v * nd + d] = f[(size_t)src * nd + d];
+}
+
+// QuasiGeostrophicEqn::GenVel via Field::InterpDerivsGlobal and
+// Mesh::ShapeFuncDerivsAtCoord (Mesh.cpp:252-285).  At a vertex both local
+// coordinates are GLL roots, so the local derivative rows are columns of D
+// and the Jacobian is the tabulated one of that GLL point.
+template <int MAXN>
+__global__ __launch_bounds__(256) void k_genvel(SLGeom g, const double* __restrict__ psi, double* __restrict__ vel) {
+#pragma clang fp contract(off)
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= g.nv0 * g.nv1) return;
+    const int N = g.N, n1 = N + 1;
+    const double ox = g.X[v % g.nv0], oy = g.Y[v / g.nv0];
+    int el = get_el(g, ox, oy);
+    int ex = el % g.nEls0, ey = el / g.nEls0;
+    double lx = (ox - g.min0 - ex * g.dx0) * (2.0 / g.dx0) - 1.0;
+    double ly = (oy - g.min1 - ey * g.dx1) * (2.0 / g.dx1) - 1.0;
+    if (lx < -1.0) lx = -1;
+    if (lx > +1.0) lx = +1;
+    if (ly < -1.0) ly = -1;
+    if (ly > +1.0) ly = +1;
+    int xi = -1, yi = -1;
+    for (int k = 0; k <= N; k++)
+        if (xi < 0 && fabs(lx - g.xk[k]) < 1.0e-8) xi = k;
+    for (int k = 0; k <= N; k++)
+        if (yi < 0 && fabs(ly - g.xk[k]) < 1.0e-8) yi = k;
+    if (xi < 0 || yi < 0) {  // cannot happen at a vertex
+        vel[2 * (size_t)v] = vel[2 * (size_t)v + 1] = __builtin_nan("");
+        return;
+    }
+    double j00 = g.J00[ex * n1 + xi], j01 = g.J01[(ey * n1 + yi) * n1 + xi];
+    double j10 = g.J10[(ex * n1 + xi) * n1 + yi], j11 = g.J11[ey * n1 + yi];
+    double det = j00 * j11 - j01 * j10;
+    double i00 = j11 / det, i11 = j00 / det, i01 = -j01 / det, i10 = -j10 / det;
+    double g0 = 0.0, g1 = 0.0;
+    const int n0 = N * ey * g.nv0 + N * ex;
+    for (int jj = 0; jj <= N; jj++)
+        for (int ii = 0; ii <= N; ii++) {
+            double a0 = (jj == yi) ? g.D[xi * n1 + ii] : 0.0;  // GNx * Ny
+            double a1 = (ii == xi) ? g.D[yi * n1 + jj] : 0.0;  // Nx * GNy
+            if (a0 == 0.0 && a1 == 0.0) continue;
+            double t0 = 0.0;
+            t0 += a0 * i00;
+            t0 += a1 * i01;
+            double t1 = 0.0;
+            t1 += a0 * i10;
+            t1 += a1 * i11;
+            double pv = psi[n0 + jj * g.nv0 + ii];
+            g0 += t0 * pv;
+            g1 += t1 * pv;
+        }
+    vel[2 * (size_t)v] = -1.0 * g1;
+    vel[2 * (size_t)v + 1] = +1.0 * g0;
+}
+
+__global__ void k_maxvel2(int nV, const double* __restrict__ vel, unsigned long long* __restrict__ out) {
+    double m = 0.0;
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nV; v += gridDim.x * blockDim.x) {
+        double a = vel[2 * (size_t)v], b = vel[2 * (size_t)v + 1];
+        double s = a * a + b * b;
+        if (s > m) m = s;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        double o = __shfl_xor(m, off, 64);
+        m = o > m ? o : m;
+    }
+    // non-negative doubles order like their bit patterns
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+SLGeom sl_geom(const se_mesh* m) {
+    SLGeom g;
+    g.N = m->N;
+    g.nEls0 = m->nEls[0];
+    g.nEls1 = m->nEls[1];
+    g.nv0 = m->nVerts[0];
+    g.nv1 = m->nVerts[1];
+    g.per0 = m->periodic[0];
+    g.per1 = m->periodic[1];
+    g.min0 = m->min[0];
+    g.min1 = m->min[1];
+    g.max0 = m->max[0];
+    g.max1 = m->max[1];
+    g.dx0 = m->dx[0];
+    g.dx1 = m->dx[1];
+    const int n1 = m->N + 1;
+    const double* b = m->d_basis.as<double>();
+    g.xk = b;
+    g.D = b + 2 * n1;
+    g.pnx = b + 2 * n1 + n1 * n1;
+    g.pn = b + 3 * n1 + n1 * n1;
+    g.X = m->d_X.as<double>();
+    g.Y = m->d_Y.as<double>();
+    g.J00 = m->d_J00.as<double>();
+    g.J10 = m->d_J10.as<double>();
+    g.J11 = m->d_J11.as<double>();
+    g.J01 = m->d_J01.as<double>();
+    return g;
+}
+
+int blocks_for(int n) { return (n + 255) / 256; }
+
+struct SLScratch {
+    DevBuf a, b, c, d, e, f, g, h;
+    ~SLScratch() {
+        DevBuf* all[] = {&a, &b, &c, &d, &e, &f, &g, &h};
+        for (DevBuf* p : all) p->release();
+    }
+};
+
+}  // namespace
+
+extern "C" int se_sl_advect_dev(se_mesh* m, const double* d_q, int nd, const double* d_vel, double dt, double* d_qsl,
+                                int32_t* d_dep) {
+    SE_CHECK(m && d_q && d_vel && d_qsl, SE_ERR_ARG, "se_sl_advect_dev: NULL argument");
+    SE_CHECK(nd >= 1 && nd <= 3, SE_ERR_ARG, "se_sl_advect_dev: nDofs must be 1..3");
+    SE_CHECK(m->N <= 12, SE_ERR_UNSUPPORTED, "se_sl_advect_dev: order above 12");
+    SE_CHECK(m->N >= 3 || (m->nVerts[0] >= 4 && m->nVerts[1] >= 4), SE_ERR_UNSUPPORTED,
+             "bicubic advection (N < 3) needs at least 4 nodes per direction");
+    SLGeom g = sl_geom(m);
+    hipStream_t s = m->ctx->stream;
+    hipLaunchKernelGGL(k_sl_first<12>, dim3(blocks_for(m->nV)), dim3(256), 0, s, g, d_q, nd, d_vel, dt, d_qsl, d_dep);
+    if (m->periodic[0] || m->periodic[1])
+        hipLaunchKernelGGL(k_periodic_copy, dim3(blocks_for(m->nV)), dim3(256), 0, s, m->nVerts[0], m->nVerts[1],
+                           m->periodic[0], m->periodic[1], nd, d_qsl);
+    SE_HIP(hipGetLastError());
+    return SE_OK;
+}
+
+extern "C" int se_sl_advect2_dev(se_mesh* m, const double* d_q, const double* d_qp, int nd, const double* d_vel,
+                                 const double* d_velp, double dt, double* d_qsl) {
+    SE_CHECK(m && d_q && d_qp && d_vel && d_velp && d_qsl, SE_ERR_ARG, "se_sl_advect2_dev: NULL argument");
+    SE_CHECK(nd >= 1 && nd <= 3, SE_ERR_ARG, "se_sl_advect2_dev: nDofs must be 1..3");
+    SE_CHECK(m->N <= 12, SE_ERR_UNSUPPORTED, "se_sl_advect2_dev: order above 12");
+    SE_CHECK(m->N >= 3 || (m->nVerts[0] >= 4 && m->nVerts[1] >= 4), SE_ERR_UNSUPPORTED,
+             "bicubic advection (N < 3) needs at least 4 nodes per direction");
+    SLGeom g = sl_geom(m);
+    hipStream_t s = m->ctx->stream;
+    size_t n2 = 2 * (size_t)m->nV;
+    int rc = m->d_sl.alloc(3 * n2 * sizeof(double));
+    if (rc) return rc;
+    double* vp1 = m->d_sl.as<double>();
+    double* vph = vp1 + n2;
+    double* vmh = vph + n2;
+    hipLaunchKernelGGL(k_extrap, dim3(blocks_for((int)n2)), dim3(256), 0, s, (int)n2, d_vel, d_velp, vp1, vph, vmh);
+    hipLaunchKernelGGL(k_sl_second<12>, dim3(blocks_for(m->nV)), dim3(256), 0, s, g, d_q, d_qp, nd, d_vel, vp1, vph,
+                       vmh, dt, d_qsl);
+    if (m->periodic[0] || m->periodic[1])
+        hipLaunchKernelGGL(k_periodic_copy, dim3(blocks_for(m->nV)), dim3(256), 0, s, m->nVerts[0], m->nVerts[1],
+                           m->periodic[0], m->periodic[1], nd, d_qsl);
+    SE_HIP(hipGetLastError());
+    return SE_OK;
+}
+
+extern "C" int se_genvel_dev(se_mesh* m, const double* d_psi, double* d_vel) {
+    SE_CHECK(m && d_psi && d_vel, SE_ERR_ARG, "se_genvel_dev: NULL argument");
+    SLGeom g = sl_geom(m);
+    hipLaunchKernelGGL(k_genvel<12>, dim3(blocks_for(m->nV)), dim3(256), 0, m->ctx->stream, g, d_psi, d_vel);
+    SE_HIP(hipGetLastError());
+    return SE_OK;
+}
+
+extern "C" int se_calc_timestep_dev(se_mesh* m, const double* d_vel, double courant, double* dt) {
+    SE_CHECK(m && d_vel && dt, SE_ERR_ARG, "se_calc_timestep_dev: NULL argument");
+    int rc = m->d_red.alloc(sizeof(unsigned long long));
+    if (rc) return rc;
+    hipStream_t s = m->ctx->stream;
+    SE_HIP(hipMemsetAsync(m->d_red.ptr, 0, sizeof(unsigned long long), s));
+    int nb = std::min(blocks_for(m->nV), 1024);
+    hipLaunchKernelGGL(k_maxvel2, dim3(nb), dim3(256), 0, s, m->nV, d_vel, m->d_red.as<unsigned long long>());
+    unsigned long long bits = 0;
+    SE_HIP(hipMemcpyAsync(&bits, m->d_red.ptr, sizeof(bits), hipMemcpyDeviceToHost, s));
+    SE_HIP(hipStreamSynchronize(s));
+    double maxVelSq;
+    std::memcpy(&maxVelSq, &bits, sizeof(double));
+    // Utils.cpp:23-40
+    double minSep = (m->dx[0] < m->dx[1]) ? m->dx[0] : m->dx[1];
+    *dt = courant * minSep / std::sqrt(maxVelSq);
+    return SE_OK;
+}
+
+// ---- host-pointer wrappers ---------------------------------------------------
+namespace {
+int up(hipStream_t s, DevBuf& b, const void* h, size_t bytes) {
+    int rc = b.alloc(std::max<size_t>(bytes, 8));
+    if (rc) return rc;
+    SE_HIP(hipMemcpyAsync(b.ptr, h, bytes, hipMemcpyHostToDevice, s));
+    return SE_OK;
+}
+int down(hipStream_t s, void* h, const DevBuf& b, size_t bytes) {
+    SE_HIP(hipMemcpyAsync(h, b.ptr, bytes, hipMemcpyDeviceToHost, s));
+    SE_HIP(hipStreamSynchronize(s));
+    return SE_OK;
+}
+}  // namespace
+
+extern "C" int se_sl_advect(se_mesh* m, const double* q, int nd, const double* vel, double dt, double* qsl,
+                            int32_t* dep) {
+    SE_CHECK(m && q && vel && qsl, SE_ERR_ARG, "se_sl_advect: NULL argument");
+    SE_CHECK(nd >= 1 && nd <= 3, SE_ERR_ARG, "se_sl_advect: nDofs must be 1..3");
+    SE_HIP(hipSetDevice(m->ctx->device));
+    hipStream_t s = m->ctx->stream;
+    SLScratch b;
+    size_t nV = m->nV;
+    int rc = up(s, b.a, q, nV * nd * sizeof(double));
+    if (!rc) rc = up(s, b.b, vel, nV * 2 * sizeof(double));
+    if (!rc) rc = b.c.alloc(nV * nd * sizeof(double));
+    if (!rc) rc = b.d.alloc(nV * sizeof(int32_t));
+    if (!rc) rc = se_sl_advect_dev(m, b.a.as<double>(), nd, b.b.as<double>(), dt, b.c.as<double>(),
+                                   dep ? b.d.as<int32_t>() : nullptr);
+    if (!rc) rc = down(s, qsl, b.c, nV * nd * sizeof(double));
+    if (!rc && dep) rc = down(s, dep, b.d, nV * sizeof(int32_t));
+    return rc;
+}
+
+extern "C" int se_sl_advect2(se_mesh* m, const double* q, const double* qp, int nd, const double* vel,
+                             const double* velp, double dt, double* qsl) {
+    SE_CHECK(m && q && qp && vel && velp && qsl, SE_ERR_ARG, "se_sl_advect2: NULL argument");
+    SE_CHECK(nd >= 1 && nd <= 3, SE_ERR_ARG, "se_sl_advect2: nDofs must be 1..3");
+    SE_HIP(hipSetDevice(m->ctx->device));
+    hipStream_t s = m->ctx->stream;
+    SLScratch b;
+    size_t nV = m->nV;
+    int rc = up(s, b.a, q, nV * nd * sizeof(double));
+    if (!rc) rc = up(s, b.b, qp, nV * nd * sizeof(double));
+    if (!rc) rc = up(s, b.c, vel, nV * 2 * sizeof(double));
+    if (!rc) rc = up(s, b.d, velp, nV * 2 * sizeof(double));
+    if (!rc) rc = b.e.alloc(nV * nd * sizeof(double));
+    if (!rc) rc = se_sl_advect2_dev(m, b.a.as<double>(), b.b.as<double>(), nd, b.c.as<double>(), b.d.as<double>(), dt,
+                                    b.e.as<double>());
+    if (!rc) rc = down(s, qsl, b.e, nV * nd * sizeof(double));
+    return rc;
+}
+
+extern "C" int se_genvel(se_mesh* m, const double* psi, double* vel) {
+    SE_CHECK(m && psi && vel, SE_ERR_ARG, "se_genvel: NULL argument");
+    SE_HIP(hipSetDevice(m->ctx->device));
+    hipStream_t s = m->ctx->stream;
+    SLScratch b;
+    size_t nV = m->nV;
+    int rc = up(s, b.a, psi, nV * sizeof(double));
+    if (!rc) rc = b.b.alloc(nV * 2 * sizeof(double));
+    if (!rc) rc = se_genvel_dev(m, b.a.as<double>(), b.b.as<double>());
+    if (!rc) rc = down(s, vel, b.b, nV * 2 * sizeof(double));
+    return rc;
+}
+
+extern "C" int se_calc_timestep(se_mesh* m, const double* vel, double courant, double* dt) {
+    SE_CHECK(m && vel && dt, SE_ERR_ARG, "se_calc_timestep: NULL argument");
+    SE_HIP(hipSetDevice(m->ctx->device));
+    hipStream_t s = m->ctx->stream;
+    SLScratch b;
+    int rc = up(s, b.a, vel, (size_t)m->nV * 2 * sizeof(double));
+    if (!rc) rc = se_calc_timestep_dev(m, b.a.as<double>(), courant, dt);
+    return rc;
+}
