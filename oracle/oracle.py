@@ -67,6 +67,7 @@ def lib():
             "orc_helm_apply_ideal": (None, [vp, d, d, i, _D, _D]),
             "orc_advect_first": (None, [vp, _D, i, _D, d, _D, vp, vp]),
             "orc_advect_second": (None, [vp, _D, _D, i, _D, _D, d, _D]),
+            "orc_sl_points": (None, [vp, i, _D, _D, d, _D]),
             "orc_genvel": (None, [vp, _D, _D]),
             "orc_calc_timestep": (d, [vp, _D, d]),
             "orc_qg_step": (i, [vp, d, d, d, i, _D, _D, d, i, C.POINTER(i)]),
@@ -266,6 +267,27 @@ class OracleMesh(_MeshBase):
 
     def calc_timestep(self, vel, courant):
         return self.L.orc_calc_timestep(self.h, _d(vel), courant)
+
+    def sl_points(self, order, vel, dt, velprev=None):
+        K = 4 if order == 2 else 2
+        pts = np.zeros(self.nV * K * 2)
+        self.L.orc_sl_points(self.h, order, _d(vel), _d(velprev if velprev is not None else vel), dt, pts)
+        return pts.reshape(self.nV, K, 2)
+
+    def node_distance(self, pts):
+        """Smallest distance, in local element coordinates, from any of the given
+        points to a GLL line that is not a root hit (< 1e-8 counts as on the line,
+        where the reference evaluates a Kronecker delta exactly).  inf if none."""
+        a, _, _ = self.gll()
+        flat = pts.reshape(-1, 2)
+        d = np.full(len(flat), np.inf)
+        for k, p in enumerate(flat):
+            _, l = self.global_to_local(p)
+            for c in l:
+                dist = np.min(np.abs(c - a))
+                if dist >= 1e-8:
+                    d[k] = min(d[k], dist)
+        return d.reshape(pts.shape[:-1]).min(axis=-1)
 
     def update_field(self, x, field):
         f = _d(field).copy()

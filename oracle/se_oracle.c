@@ -1037,6 +1037,54 @@ void orc_advect_first(const orc_mesh* m, const double* q, int nDofs, const doubl
     if (m->periodic[0] || m->periodic[1]) orc_field_periodic_update(m, qSL, nDofs);
 }
 
+/* Interpolation points of the RK2 trajectories (Advector.cpp:136-148 for
+ * order 1: K = 2 points per node {x_half, x_dep}; Advector.cpp:203-246 for
+ * order 2: K = 4 points {x_half, x1, x_half2, x2}).  pts: nV x K x 2.  Used by
+ * the tests to locate departure points where the reference's cardinal-function
+ * formula is ill-conditioned (close to, but not within 1e-8 of, a GLL line). */
+void orc_sl_points(const orc_mesh* m, int order, const double* vel, const double* velPrev, double dt,
+                   double* pts) {
+    size_t nv2 = 2 * (size_t)m->nV, i;
+    int v, K = order == 2 ? 4 : 2;
+    double *vp1 = NULL, *vph = NULL, *vmh = NULL;
+    if (order == 2) {
+        vp1 = (double*)malloc(sizeof(double) * nv2);
+        vph = (double*)malloc(sizeof(double) * nv2);
+        vmh = (double*)malloc(sizeof(double) * nv2);
+        for (i = 0; i < nv2; i++) {
+            vp1[i] = 2.0 * vel[i] - 1.0 * velPrev[i];
+            vph[i] = 1.5 * vel[i] - 0.5 * velPrev[i];
+            vmh[i] = 0.5 * vel[i] + 0.5 * velPrev[i];
+        }
+    }
+    for (v = 0; v < m->nV; v++) {
+        const double* o = &m->verts[2 * (size_t)v];
+        double* P = &pts[(size_t)v * K * 2];
+        double xh[2], w[2], x1[2], x2[2];
+        orc_interp_global(m, order == 2 ? vp1 : vel, 2, o, w);
+        xh[0] = o[0] - 0.5 * dt * w[0];
+        xh[1] = o[1] - 0.5 * dt * w[1];
+        coord_wrap(m, xh);
+        orc_interp_global(m, order == 2 ? vph : vel, 2, xh, w);
+        x1[0] = o[0] - dt * w[0];
+        x1[1] = o[1] - dt * w[1];
+        coord_wrap(m, x1);
+        P[0] = xh[0]; P[1] = xh[1]; P[2] = x1[0]; P[3] = x1[1];
+        if (order == 2) {
+            orc_interp_global(m, vel, 2, x1, w);
+            xh[0] = x1[0] - 0.5 * dt * w[0];
+            xh[1] = x1[1] - 0.5 * dt * w[1];
+            coord_wrap(m, xh);
+            orc_interp_global(m, vmh, 2, xh, w);
+            x2[0] = x1[0] - dt * w[0];
+            x2[1] = x1[1] - dt * w[1];
+            coord_wrap(m, x2);
+            P[4] = xh[0]; P[5] = xh[1]; P[6] = x2[0]; P[7] = x2[1];
+        }
+    }
+    free(vp1); free(vph); free(vmh);
+}
+
 /* Advector::ExtrapolateVelocities + SecondOrder, Advector.cpp:203-246,339-349 */
 void orc_advect_second(const orc_mesh* m, const double* q, const double* qPrev, int nDofs,
                        const double* vel, const double* velPrev, double dt, double* qSL) {

@@ -87,6 +87,8 @@ void   orc_advect_first(const orc_mesh* m, const double* q, int nDofs, const dou
                         double dt, double* qSL, int* dep_el, double* dep_xy);
 void   orc_advect_second(const orc_mesh* m, const double* q, const double* qPrev, int nDofs,
                          const double* vel, const double* velPrev, double dt, double* qSL);
+void   orc_sl_points(const orc_mesh* m, int order, const double* vel, const double* velPrev, double dt,
+                     double* pts);
 void   orc_genvel(const orc_mesh* m, const double* psi, double* vel);
 double orc_calc_timestep(const orc_mesh* m, const double* vel, double courant);
 

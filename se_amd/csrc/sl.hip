@@ -73,11 +73,30 @@ __device__ __forceinline__ void wrap(const SLGeom& g, double& cx, double& cy) {
     if (cy > g.max1) cy = g.per1 ? g.min1 - g.max1 + cy : g.max1;
 }
 
+// x^k for a small non-negative integer k, correctly rounded: the running
+// product is carried as an unevaluated double-double (error-free products via
+// FMA) and rounded once.  glibc's pow, which the reference calls with integer
+// exponents, returns the correctly rounded power in all but ~0.1% of cases, so
+// this reproduces the reference's monomial terms bit for bit almost always.
+__device__ __forceinline__ double ipow_cr(double x, int k) {
+    if (k == 0) return 1.0;
+    double hi = x, lo = 0.0;
+    for (int i = 1; i < k; i++) {
+        double p = hi * x;
+        double e = fma(hi, x, -p);
+        double l = fma(lo, x, e);
+        double s = p + l;
+        lo = l - (s - p);
+        hi = s;
+    }
+    return hi + lo;
+}
+
 // Jacobi::EvalDeriv (Jacobi.cpp:58-66) of P_N
 __device__ __forceinline__ double pn_deriv(const SLGeom& g, double x) {
 #pragma clang fp contract(off)
     double dp = 0.0;
-    for (int i = g.N; i >= 1; i -= 2) dp += g.pn[i] * i * pow(x, (double)(i - 1));
+    for (int i = g.N; i >= 1; i -= 2) dp += g.pn[i] * i * ipow_cr(x, i - 1);
     return dp;
 }
 
