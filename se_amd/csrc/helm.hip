@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "apply_kernels.h"
 #include "helm_kernels.h"
 #include "se_common.h"
 #include "se_op.h"
@@ -13,74 +14,149 @@ using namespace se;
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;   // setup / tile kernels
+constexpr int kVB = 1024;     // vector kernels of the CG loop
 
 // ---------------------------------------------------------------------------
-// CG vector kernels (grid-stride; the last block of each reduction finalises
-// the recurrence scalars in fixed block order).
+// CG vector kernels.  Each kernel stores per-block partial sums with plain
+// stores; every block of the next kernel reduces them in the same fixed order
+// (reduce_partials), so the recurrence scalars are identical in all blocks and
+// deterministic, with no atomics or fences in the iteration.  Block 0 of the
+// consuming kernel also records the scalars in CgState for the host.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_sum(int n, const double* __restrict__ a, double scale,
-                                             double* __restrict__ partials, CgState* __restrict__ st) {
+__global__ __launch_bounds__(kVB) void k_sum(int n, const double* __restrict__ a, double scale,
+                                             double* __restrict__ part) {
     double s = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += scale * a[i];
-    double v[1] = {s}, tot[1];
-    if (block_reduce_last<1>(v, partials, &st->counter, tot) && threadIdx.x == 0) st->mu = tot[0] / n;
+    double v[1] = {s};
+    block_store_partials<1, kVB>(v, part);
 }
 
-// b = sgn*f (- mean), x = 0, r = b; first preconditioned residual
-__global__ __launch_bounds__(256) void k_cg_start(int n, double sgn, double rtol, const double* __restrict__ f,
+// b = sgn*f (- mean of sgn*f), x = 0, r = b; partials of (r.r, r.z, sum z, sum r)
+__global__ __launch_bounds__(kVB) void k_cg_start(int n, double sgn, const double* __restrict__ f,
                                                   const double* __restrict__ dinv, double* __restrict__ b,
                                                   double* __restrict__ x, double* __restrict__ r,
-                                                  double* __restrict__ partials, CgState* __restrict__ st) {
-    const double mu_b = st->null_space ? st->mu : 0.0;
-    double bb = 0.0, rz = 0.0, sz = 0.0, sr = 0.0;
+                                                  const double* __restrict__ part_s, int np_s,
+                                                  double* __restrict__ part_b, CgState* __restrict__ st) {
+    double mu_b = 0.0;
+    if (st->null_space) {
+        double t[1];
+        reduce_partials<1, kVB>(part_s, np_s, t);
+        mu_b = t[0] / n;
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->mu_b = mu_b;
+    }
+    double rr = 0.0, rz = 0.0, sz = 0.0, sr = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         double bi = sgn * f[i] - mu_b;
         b[i] = bi;
         r[i] = bi;
         x[i] = 0.0;
         double z = dinv[i] * bi;
-        bb += bi * bi;
+        rr += bi * bi;
         rz += bi * z;
         sz += z;
         sr += bi;
     }
-    double v[4] = {bb, rz, sz, sr}, t[4];
-    if (block_reduce_last<4>(v, partials, &st->counter, t) && threadIdx.x == 0) {
-        st->bb = t[0];
-        st->tol2 = rtol * rtol * t[0];
-        double mu = st->null_space ? t[2] / n : 0.0;
-        st->mu = mu;
-        st->rho = t[1] - mu * t[3];
-        st->iter = 0;
-        st->first = 1;
-        st->rr = t[0];
-        bool finite = (t[0] - t[0] == 0.0);
-        st->stop = (t[0] == 0.0 || !finite) ? 1 : 0;
-        st->status = t[0] == 0.0 ? 0 : (finite ? CG_RUNNING : 3);
-        st->tt = -1.0;
-    }
+    double v[4] = {rr, rz, sz, sr};
+    block_store_partials<4, kVB>(v, part_b);
 }
 
-// p = (D^-1 r - mu) + beta p   (first direction: p = D^-1 r - mu)
-__global__ __launch_bounds__(256) void k_cg_dir(int n, const double* __restrict__ dinv, const double* __restrict__ r,
-                                                double* __restrict__ p, CgState* __restrict__ st) {
-    if (st->stop) return;
-    const double beta = st->beta, mu = st->mu;
-    const bool first = st->first != 0;
+enum DirMode { DIR_FIRST = 0, DIR_NORMAL = 1, DIR_RESTART = 2 };
+
+// Reduce (r.r, r.z, sum z, sum r); decide convergence; p = (D^-1 r - mu) + beta p.
+__global__ __launch_bounds__(kVB) void k_cg_dir(int mode, int n, double rtol, const double* __restrict__ dinv,
+                                                const double* __restrict__ r, double* __restrict__ p,
+                                                const double* __restrict__ part_b, int np_b,
+                                                double* __restrict__ hist, CgState* __restrict__ st) {
+    if (mode == DIR_NORMAL && __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    double t[4];
+    reduce_partials<4, kVB>(part_b, np_b, t);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const double rr = t[0];
+    const bool finite = (rr - rr == 0.0) && (t[1] - t[1] == 0.0);
+    double mu = st->null_space ? t[2] / n : 0.0;
+    double rho_new = t[1] - mu * t[3];
+    double beta = 0.0;
+    if (mode == DIR_FIRST) {
+        if (lead) {
+            st->bb = rr;
+            st->tol2 = rtol * rtol * rr;
+            st->rr = rr;
+            st->rho = rho_new;
+            st->mu = mu;
+            st->iter = 0;
+            st->stop = (rr == 0.0 || !finite) ? 1 : 0;
+            st->status = rr == 0.0 ? 0 : (finite ? CG_RUNNING : 3);
+            if (hist) hist[0] = 1.0;
+        }
+        if (rr == 0.0 || !finite) return;
+    } else if (mode == DIR_NORMAL) {
+        const bool conv = finite && rr <= st->tol2;
+        if (lead) {
+            int it = st->iter + 1;
+            st->iter = it;
+            st->rr = rr;
+            if (hist) hist[it] = sqrt(rr / st->bb);
+            if (!finite) {
+                st->stop = 1;
+                st->status = 3;
+            } else if (conv) {
+                st->stop = 1;
+                st->status = CG_PENDING;
+            } else {
+                st->rho = rho_new;
+                st->mu = mu;
+                st->beta = rho_new / st->rho_prev;
+                if (it >= st->maxit) {
+                    st->stop = 1;
+                    st->status = 1;
+                }
+            }
+        }
+        if (!finite || conv) return;
+        beta = rho_new / st->rho_prev;
+    } else {  // DIR_RESTART: continue from the true residual, no convergence test
+        // rho_prev still holds the rho of the candidate iteration (its k_cg_dir
+        // stopped before updating rho)
+        beta = rho_new / st->rho_prev;
+        if (lead) {
+            st->rr = rr;
+            st->rho = rho_new;
+            st->mu = mu;
+            st->beta = beta;
+        }
+    }
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         double z = dinv[i] * r[i] - mu;
-        p[i] = first ? z : z + beta * p[i];
+        p[i] = mode == DIR_FIRST ? z : z + beta * p[i];
     }
 }
 
-// x += alpha p, r -= alpha q, then r.r, r.z (z = D^-1 r) and the next beta
-__global__ __launch_bounds__(256) void k_cg_update(int n, const double* __restrict__ p, const double* __restrict__ q,
+// alpha = rho / (p.Ap); x += alpha p, r -= alpha q; partials of (r.r, r.z, sum z, sum r)
+__global__ __launch_bounds__(kVB) void k_cg_update(int n, const double* __restrict__ p, const double* __restrict__ q,
                                                    const double* __restrict__ dinv, double* __restrict__ x,
-                                                   double* __restrict__ r, double* __restrict__ hist,
-                                                   double* __restrict__ partials, CgState* __restrict__ st) {
-    if (st->stop) return;
-    const double alpha = st->alpha;
+                                                   double* __restrict__ r, const double* __restrict__ part_a, int np_a,
+                                                   double* __restrict__ part_b, CgState* __restrict__ st) {
+    if (__hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    double t[1];
+    reduce_partials<1, kVB>(part_a, np_a, t);
+    const double pq = t[0];
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (!(pq > 0.0)) {
+        if (lead) {
+            st->pq = pq;
+            st->stop = 1;
+            st->status = (pq - pq == 0.0) ? 2 : 3;
+        }
+        return;
+    }
+    const double rho = st->rho;
+    const double alpha = rho / pq;
+    if (lead) {
+        st->pq = pq;
+        st->alpha = alpha;
+        st->rho_prev = rho;
+    }
     double rr = 0.0, rz = 0.0, sz = 0.0, sr = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         x[i] += alpha * p[i];
@@ -92,37 +168,13 @@ __global__ __launch_bounds__(256) void k_cg_update(int n, const double* __restri
         sz += z;
         sr += ri;
     }
-    double v[4] = {rr, rz, sz, sr}, t[4];
-    if (block_reduce_last<4>(v, partials, &st->counter, t) && threadIdx.x == 0) {
-        int it = st->iter + 1;
-        st->iter = it;
-        st->first = 0;
-        st->rr = t[0];
-        if (hist) hist[it] = sqrt(t[0] / st->bb);
-        if (!(t[0] - t[0] == 0.0)) {
-            st->stop = 1;
-            st->status = 3;
-        } else if (t[0] <= st->tol2) {
-            st->stop = 1;
-            st->status = CG_PENDING;
-        } else {
-            double mu = st->null_space ? t[2] / n : 0.0;
-            double rho_new = t[1] - mu * t[3];
-            st->mu = mu;
-            st->beta = rho_new / st->rho;
-            st->rho = rho_new;
-            if (it >= st->maxit) {
-                st->stop = 1;
-                st->status = 1;
-            }
-        }
-    }
+    double v[4] = {rr, rz, sz, sr};
+    block_store_partials<4, kVB>(v, part_b);
 }
 
-// t = b - q (q = A x); r = t; ||t - mean(t)||^2 into st->tt
-__global__ __launch_bounds__(256) void k_true_resid(int n, const double* __restrict__ b, const double* __restrict__ q,
-                                                    double* __restrict__ r, double* __restrict__ partials,
-                                                    CgState* __restrict__ st) {
+// t = b - q (q = A x); r = t; partials of (sum t, sum t^2)
+__global__ __launch_bounds__(kVB) void k_true_resid(int n, const double* __restrict__ b, const double* __restrict__ q,
+                                                    double* __restrict__ r, double* __restrict__ part_t) {
     double s = 0.0, s2 = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         double t = b[i] - q[i];
@@ -130,18 +182,26 @@ __global__ __launch_bounds__(256) void k_true_resid(int n, const double* __restr
         s += t;
         s2 += t * t;
     }
-    double v[2] = {s, s2}, t[2];
-    if (block_reduce_last<2>(v, partials, &st->counter, t) && threadIdx.x == 0) {
+    double v[2] = {s, s2};
+    block_store_partials<2, kVB>(v, part_t);
+}
+
+// single block: ||t - mean t||^2 and the mean, into the state
+__global__ __launch_bounds__(kVB) void k_true_fin(int n, const double* __restrict__ part_t, int np_t,
+                                                  CgState* __restrict__ st) {
+    double t[2];
+    reduce_partials<2, kVB>(part_t, np_t, t);
+    if (threadIdx.x == 0) {
         double mt = st->null_space ? t[0] / n : 0.0;
+        st->mu_t = mt;
         st->tt = st->null_space ? t[1] - t[0] * mt : t[1];
-        st->mu = mt;  // consumed by k_cg_restart
     }
 }
 
-// continue from the true residual: r -= mean (null space), new rho and beta
-__global__ __launch_bounds__(256) void k_cg_restart(int n, const double* __restrict__ dinv, double* __restrict__ r,
-                                                    double* __restrict__ partials, CgState* __restrict__ st) {
-    const double mt = st->null_space ? st->mu : 0.0;
+// continue from the true residual: r -= mean (null space); partials as k_cg_update
+__global__ __launch_bounds__(kVB) void k_cg_restart(int n, const double* __restrict__ dinv, double* __restrict__ r,
+                                                    double* __restrict__ part_b, const CgState* __restrict__ st) {
+    const double mt = st->null_space ? st->mu_t : 0.0;
     double rr = 0.0, rz = 0.0, sz = 0.0, sr = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         double ri = r[i] - mt;
@@ -152,17 +212,8 @@ __global__ __launch_bounds__(256) void k_cg_restart(int n, const double* __restr
         sz += z;
         sr += ri;
     }
-    double v[4] = {rr, rz, sz, sr}, t[4];
-    if (block_reduce_last<4>(v, partials, &st->counter, t) && threadIdx.x == 0) {
-        double mu = st->null_space ? t[2] / n : 0.0;
-        double rho_new = t[1] - mu * t[3];
-        st->rr = t[0];
-        st->mu = mu;
-        st->beta = rho_new / st->rho;
-        st->rho = rho_new;
-        st->stop = 0;
-        st->status = CG_RUNNING;
-    }
+    double v[4] = {rr, rz, sz, sr};
+    block_store_partials<4, kVB>(v, part_b);
 }
 
 __global__ void k_scale_copy(int n, double s, const double* __restrict__ a, double* __restrict__ b) {
@@ -187,6 +238,35 @@ void launch_tile_n(int tile, const OpParams& P, const double* in, double* out, c
     else launch_tile_t<N, 2, 2, MODE, DOT>(P, in, out, aux, partials, st, s);
 }
 
+template <int N, int TX, int TY, bool DOT>
+void launch_apply_t(const OpParams& P, const double* in, double* out, double* partials, const CgState* st,
+                    hipStream_t s) {
+    int tiles = ((P.nEls0 + TX - 1) / TX) * ((P.nEls1 + TY - 1) / TY);
+    hipLaunchKernelGGL((k_apply<N, TX, TY, DOT>), dim3(tiles), dim3(kBlock), 0, s, P, in, out, partials, st);
+}
+
+template <bool DOT>
+int launch_apply(int N, int tile, const OpParams& P, const double* in, double* out, double* partials,
+                 const CgState* st, hipStream_t s) {
+#define SE_APPLY_CASE(NN)                                                      \
+    case NN:                                                                   \
+        if (tile == 4) launch_apply_t<NN, 4, 4, DOT>(P, in, out, partials, st, s); \
+        else launch_apply_t<NN, 2, 2, DOT>(P, in, out, partials, st, s);          \
+        break;
+    switch (N) {
+        SE_APPLY_CASE(2)
+        SE_APPLY_CASE(3)
+        SE_APPLY_CASE(4)
+        SE_APPLY_CASE(5)
+        SE_APPLY_CASE(6)
+        SE_APPLY_CASE(7)
+        SE_APPLY_CASE(8)
+        default: return fail(SE_ERR_UNSUPPORTED, "operator kernels are built for orders 2..8");
+    }
+#undef SE_APPLY_CASE
+    return SE_OK;
+}
+
 template <int MODE, bool DOT>
 int launch_tile(int N, int tile, const OpParams& P, const double* in, double* out, const double* aux,
                 double* partials, CgState* st, hipStream_t s) {
@@ -204,8 +284,14 @@ int launch_tile(int N, int tile, const OpParams& P, const double* in, double* ou
 }
 
 int vec_blocks(int n) {
-    int b = (n + kBlock * 4 - 1) / (kBlock * 4);
-    return std::max(1, std::min(b, 2048));
+    int b = (n + kVB * 4 - 1) / (kVB * 4);
+    return std::max(1, std::min(b, 256));
+}
+
+int apply_tiles(const se_op* op) {
+    const se_mesh* m = op->mesh;
+    int t = op->tile;
+    return ((m->nEls[0] + t - 1) / t) * ((m->nEls[1] + t - 1) / t);
 }
 
 }  // namespace
@@ -247,13 +333,29 @@ int op_tile(const se_op* op) {
     return (t4 >= 2L * m->ctx->cu_count) ? 4 : 2;
 }
 
+// partial-sum areas in op->partials (doubles): A apply (one per tile),
+// B CG vectors (4 per block), S mean (1 per block), T true residual (2 per block)
+struct PartLayout {
+    size_t a, b, s, t, total;
+};
+
+static PartLayout part_layout(const se_mesh* m) {
+    PartLayout L;
+    size_t tiles = (size_t)((m->nEls[0] + 1) / 2) * ((m->nEls[1] + 1) / 2);
+    L.a = 0;
+    L.b = tiles;
+    L.s = L.b + 4 * 256;
+    L.t = L.s + 256;
+    L.total = L.t + 2 * 256;
+    return L;
+}
+
 int op_apply_dev(se_op* op, const double* d_x, double* d_y, double scale, bool dot) {
     OpParams P = op_params(op, scale);
     hipStream_t s = op->mesh->ctx->stream;
-    int rc = dot ? launch_tile<MODE_APPLY, true>(op->mesh->N, op->tile, P, d_x, d_y, nullptr,
-                                                 op->partials.as<double>(), op->state.as<CgState>(), s)
-                 : launch_tile<MODE_APPLY, false>(op->mesh->N, op->tile, P, d_x, d_y, nullptr,
-                                                  op->partials.as<double>(), op->state.as<CgState>(), s);
+    double* pa = op->partials.as<double>() + part_layout(op->mesh).a;
+    int rc = dot ? launch_apply<true>(op->mesh->N, op->tile, P, d_x, d_y, pa, op->state.as<CgState>(), s)
+                 : launch_apply<false>(op->mesh->N, op->tile, P, d_x, d_y, pa, op->state.as<CgState>(), s);
     if (rc) return rc;
     SE_HIP(hipGetLastError());
     return SE_OK;
@@ -262,8 +364,6 @@ int op_apply_dev(se_op* op, const double* d_x, double* d_y, double scale, bool d
 int op_ensure_work(se_op* op) {
     const se_mesh* m = op->mesh;
     size_t nb = sizeof(double) * (size_t)std::max(m->n, 1);
-    int tiles = ((m->nEls[0] + 1) / 2) * ((m->nEls[1] + 1) / 2);
-    size_t np = sizeof(double) * 4 * (size_t)std::max(tiles, 2048);
     int rc = SE_OK;
     if (!rc) rc = op->b.alloc(nb);
     if (!rc) rc = op->x.alloc(nb);
@@ -271,7 +371,7 @@ int op_ensure_work(se_op* op) {
     if (!rc) rc = op->p.alloc(nb);
     if (!rc) rc = op->q.alloc(nb);
     if (!rc) rc = op->f.alloc(nb);
-    if (!rc) rc = op->partials.alloc(np);
+    if (!rc) rc = op->partials.alloc(sizeof(double) * part_layout(m).total);
     return rc;
 }
 
@@ -439,19 +539,21 @@ extern "C" int se_helm_rhs(se_op* op, const double* rhs_field, double* f) {
 // ---------------------------------------------------------------------------
 namespace se {
 
-// one CG iteration: q = A p (+ p.q), update, direction
+// one CG iteration: q = A p (+ block partials of p.q), update, direction
 static int cg_iteration(se_op* op, double* hist) {
     const se_mesh* m = op->mesh;
     hipStream_t s = m->ctx->stream;
-    int n = m->n, vb = vec_blocks(n);
+    const int n = m->n, vb = vec_blocks(n);
+    const PartLayout L = part_layout(m);
     CgState* st = op->state.as<CgState>();
     double* part = op->partials.as<double>();
     int rc = op_apply_dev(op, op->p.as<double>(), op->q.as<double>(), (double)op->sign, true);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_cg_update, dim3(vb), dim3(kBlock), 0, s, n, op->p.as<double>(), op->q.as<double>(),
-                       op->dinv_s.as<double>(), op->x.as<double>(), op->r.as<double>(), hist, part, st);
-    hipLaunchKernelGGL(k_cg_dir, dim3(vb), dim3(kBlock), 0, s, n, op->dinv_s.as<double>(), op->r.as<double>(),
-                       op->p.as<double>(), st);
+    hipLaunchKernelGGL(k_cg_update, dim3(vb), dim3(kVB), 0, s, n, op->p.as<double>(), op->q.as<double>(),
+                       op->dinv_s.as<double>(), op->x.as<double>(), op->r.as<double>(), part + L.a, apply_tiles(op),
+                       part + L.b, st);
+    hipLaunchKernelGGL(k_cg_dir, dim3(vb), dim3(kVB), 0, s, (int)DIR_NORMAL, n, 0.0, op->dinv_s.as<double>(),
+                       op->r.as<double>(), op->p.as<double>(), part + L.b, vb, hist, st);
     return SE_OK;
 }
 
@@ -492,15 +594,33 @@ static int cg_batch(se_op* op, int k, double* hist) {
     return SE_OK;
 }
 
+// true residual of the current x: st->tt = ||b - sA x||^2 (mean removed with a null space)
+static int true_residual(se_op* op) {
+    const se_mesh* m = op->mesh;
+    hipStream_t s = m->ctx->stream;
+    const int n = m->n, vb = vec_blocks(n);
+    const PartLayout L = part_layout(m);
+    double* part = op->partials.as<double>();
+    int rc = op_apply_dev(op, op->x.as<double>(), op->q.as<double>(), (double)op->sign, false);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_true_resid, dim3(vb), dim3(kVB), 0, s, n, op->b.as<double>(), op->q.as<double>(),
+                       op->r.as<double>(), part + L.t);
+    hipLaunchKernelGGL(k_true_fin, dim3(1), dim3(kVB), 0, s, n, part + L.t, vb, op->state.as<CgState>());
+    SE_HIP(hipGetLastError());
+    return SE_OK;
+}
+
 // Solve sign*A x = sign*f on device.  f, x: device vectors (n).
 int pcg_fp64(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, se_solve_info& info) {
     se_mesh* m = op->mesh;
     se_ctx* c = m->ctx;
     hipStream_t s = c->stream;
-    int n = m->n, vb = vec_blocks(n);
+    const int n = m->n, vb = vec_blocks(n);
     SE_CHECK(op->definite, SE_ERR_UNSUPPORTED, "operator diagonal changes sign: not a definite system");
     int rc = op_ensure_work(op);
     if (rc) return rc;
+    const PartLayout L = part_layout(m);
+    double* part = op->partials.as<double>();
     if (!op->dinv_s.ptr) {
         rc = op->dinv_s.alloc(sizeof(double) * std::max(n, 1));
         if (rc) return rc;
@@ -514,40 +634,37 @@ int pcg_fp64(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, 
     h.n = n;
     h.status = CG_RUNNING;
     CgState* st = op->state.as<CgState>();
-    double* part = op->partials.as<double>();
     DevBuf d_hist;
     if (o.history) {
         rc = d_hist.alloc(sizeof(double) * (size_t)(h.maxit + 1));
         if (rc) return rc;
     }
+    double* dh = o.history ? d_hist.as<double>() : nullptr;
     SE_HIP(hipEventRecord(c->ev0, s));
     SE_HIP(hipMemcpyAsync(st, &h, sizeof(CgState), hipMemcpyHostToDevice, s));
     if (h.null_space)
-        hipLaunchKernelGGL(k_sum, dim3(vb), dim3(kBlock), 0, s, n, d_f, (double)op->sign, part, st);
-    hipLaunchKernelGGL(k_cg_start, dim3(vb), dim3(kBlock), 0, s, n, (double)op->sign, o.rtol, d_f,
-                       op->dinv_s.as<double>(), op->b.as<double>(), op->x.as<double>(), op->r.as<double>(), part, st);
-    hipLaunchKernelGGL(k_cg_dir, dim3(vb), dim3(kBlock), 0, s, n, op->dinv_s.as<double>(), op->r.as<double>(),
-                       op->p.as<double>(), st);
+        hipLaunchKernelGGL(k_sum, dim3(vb), dim3(kVB), 0, s, n, d_f, (double)op->sign, part + L.s);
+    hipLaunchKernelGGL(k_cg_start, dim3(vb), dim3(kVB), 0, s, n, (double)op->sign, d_f, op->dinv_s.as<double>(),
+                       op->b.as<double>(), op->x.as<double>(), op->r.as<double>(), part + L.s, vb, part + L.b, st);
+    hipLaunchKernelGGL(k_cg_dir, dim3(vb), dim3(kVB), 0, s, (int)DIR_FIRST, n, o.rtol, op->dinv_s.as<double>(),
+                       op->r.as<double>(), op->p.as<double>(), part + L.b, vb, dh, st);
     SE_HIP(hipGetLastError());
     if (!op->h_state) SE_HIP(hipHostMalloc(&op->h_state, sizeof(CgState), hipHostMallocDefault));
     CgState* hs = (CgState*)op->h_state;
-    double* dh = o.history ? d_hist.as<double>() : nullptr;
-    int k = std::max(1, o.check_every);
+    SE_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    SE_HIP(hipStreamSynchronize(s));
+    const int k = std::max(1, o.check_every);
     int guard = 0;
-    for (;;) {
-        int left = h.maxit;  // batches overrun harmlessly: kernels return once stop is set
-        (void)left;
+    while (!hs->stop) {
+        // batches overrun harmlessly: every kernel returns once stop is set
         rc = cg_batch(op, k, dh);
         if (rc) return rc;
         SE_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
         SE_HIP(hipStreamSynchronize(s));
-        if (!hs->stop) continue;
-        if (hs->status != CG_PENDING) break;
+        if (!hs->stop || hs->status != CG_PENDING) continue;
         // recursive residual below target: confirm with the true residual
-        rc = op_apply_dev(op, op->x.as<double>(), op->q.as<double>(), (double)op->sign, false);
+        rc = true_residual(op);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_true_resid, dim3(vb), dim3(kBlock), 0, s, n, op->b.as<double>(), op->q.as<double>(),
-                           op->r.as<double>(), part, st);
         SE_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
         SE_HIP(hipStreamSynchronize(s));
         if (hs->tt <= hs->tol2) {
@@ -558,19 +675,23 @@ int pcg_fp64(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, 
             hs->status = 1;
             break;
         }
-        hipLaunchKernelGGL(k_cg_restart, dim3(vb), dim3(kBlock), 0, s, n, op->dinv_s.as<double>(),
-                           op->r.as<double>(), part, st);
-        hipLaunchKernelGGL(k_cg_dir, dim3(vb), dim3(kBlock), 0, s, n, op->dinv_s.as<double>(), op->r.as<double>(),
-                           op->p.as<double>(), st);
+        // continue from the true residual (now in r)
+        int flags[2] = {0, CG_RUNNING};
+        SE_HIP(hipMemcpyAsync(&st->stop, flags, sizeof(flags), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_cg_restart, dim3(vb), dim3(kVB), 0, s, n, op->dinv_s.as<double>(), op->r.as<double>(),
+                           part + L.b, st);
+        hipLaunchKernelGGL(k_cg_dir, dim3(vb), dim3(kVB), 0, s, (int)DIR_RESTART, n, 0.0, op->dinv_s.as<double>(),
+                           op->r.as<double>(), op->p.as<double>(), part + L.b, vb, dh, st);
+        SE_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        SE_HIP(hipStreamSynchronize(s));
     }
     int status = hs->status;
-    int iters = hs->iter;
-    double bb = hs->bb, tt = hs->tt;
-    if (status != 0 && bb > 0.0) {
-        rc = op_apply_dev(op, op->x.as<double>(), op->q.as<double>(), (double)op->sign, false);
+    const int iters = hs->iter;
+    const double bb = hs->bb;
+    double tt = hs->tt;
+    if (status != 0 && bb > 0.0 && bb - bb == 0.0) {
+        rc = true_residual(op);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_true_resid, dim3(vb), dim3(kBlock), 0, s, n, op->b.as<double>(), op->q.as<double>(),
-                           op->r.as<double>(), part, st);
         SE_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
         SE_HIP(hipStreamSynchronize(s));
         tt = hs->tt;

@@ -1,5 +1,8 @@
 // helm_kernels.h -- device code of the matrix-free GLL Helmholtz operator.
 //
+// Setup-time modes (lifting, right-hand side, diagonal) and a reference form
+// of the apply; the hot apply is k_apply in apply_kernels.h.
+//
 // One workgroup owns a tile of TX x TY elements and produces the assembled
 // result on the tile's nodes (the half-open node block of its elements plus,
 // on a non-periodic domain edge, the closing line).  Each node value is the
@@ -39,13 +42,14 @@ struct OpParams {
     const double* D;    // [(N+1)^2]
 };
 
-// deterministic two-level reduction state: per-block partials, a counter, and
-// the scalars of the CG recurrence.  Lives in device memory.
+// Scalars of the CG recurrence, in device memory.  Written by block 0 of the
+// kernel that finalises them; every block of a consuming kernel recomputes the
+// same totals from the producer's per-block partials in a fixed order.
 struct CgState {
-    double rho, pq, alpha, beta, rr, bb, tol2, mu, tt;
-    double nrm_f;   // ||f|| of the sign-normalised right-hand side
-    int iter, stop, status, maxit, first, null_space, n, pad;
+    double rho, rho_prev, pq, alpha, beta, rr, bb, tol2, mu, mu_b, mu_t, tt;
+    int iter, stop, status, maxit, null_space, n;
     unsigned counter;
+    int pad;
 };
 
 enum CgStatus { CG_RUNNING = -1, CG_PENDING = -2 };
@@ -54,51 +58,6 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
-}
-
-// Block-reduce K values (blockDim 256), publish per-block partials and return
-// true in the last block to arrive, whose thread 0 then holds the totals in
-// tot[] (summed in fixed block order, independent of arrival order).
-template <int K>
-__device__ __forceinline__ bool block_reduce_last(const double (&v)[K], double* partials, unsigned* counter,
-                                                  double (&tot)[K]) {
-    __shared__ double s_red[K][4];
-    __shared__ int s_last;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        double s = wave_sum(v[k]);
-        if (lane == 0) s_red[k][wid] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            double s = ((s_red[k][0] + s_red[k][1]) + s_red[k][2]) + s_red[k][3];
-            __hip_atomic_store(&partials[(size_t)k * gridDim.x + blockIdx.x], s, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        }
-        unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!s_last) return false;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        double s = 0.0;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-            s += __hip_atomic_load(&partials[(size_t)k * gridDim.x + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s = wave_sum(s);
-        if (lane == 0) s_red[k][wid] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; k++) tot[k] = ((s_red[k][0] + s_red[k][1]) + s_red[k][2]) + s_red[k][3];
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return true;
 }
 
 // Per-point inverse Jacobian and weights, in the reference's operation order
@@ -136,8 +95,6 @@ __global__ __launch_bounds__(256) void k_helm_tile(OpParams P, const double* __r
     constexpr int BW = (TX + 1) * N + 2;
     constexpr int BH = (TY + 1) * N + 2;
     constexpr int NE = (TX + 1) * (TY + 1);
-
-    if (DOT && st->stop) return;
 
     __shared__ double s_u[MODE == MODE_DIAG ? 1 : BH * BW];
     __shared__ double s_fx[MODE == MODE_APPLY || MODE == MODE_LIFT ? NE * NP : 1];
@@ -251,7 +208,6 @@ __global__ __launch_bounds__(256) void k_helm_tile(OpParams P, const double* __r
     const int txo = min(TX, P.nEls0 - ex0), tyo = min(TY, P.nEls1 - ey0);
     const int ownX = txo * N + ((!P.per0 && ex0 + txo == P.nEls0) ? 1 : 0);
     const int ownY = tyo * N + ((!P.per1 && ey0 + tyo == P.nEls1) ? 1 : 0);
-    double dot = 0.0;
     for (int i = tid; i < ownX * ownY; i += 256) {
         int ox = i % ownX, oy = i / ownX;
         int gI = ex0 * N + ox, gJ = ey0 * N + oy;
@@ -331,9 +287,8 @@ __global__ __launch_bounds__(256) void k_helm_tile(OpParams P, const double* __r
             }
         }
         if (MODE == MODE_APPLY) {
-            double yv = P.scale * sum;
-            out[u] = yv;
-            if (DOT) dot += own * yv;
+            out[u] = P.scale * sum;
+            (void)own;
         } else if (MODE == MODE_LIFT) {
             out[u] = -sum;
         } else if (MODE == MODE_RHS) {
@@ -343,19 +298,6 @@ __global__ __launch_bounds__(256) void k_helm_tile(OpParams P, const double* __r
         }
     }
 
-    if (DOT) {
-        double v[1] = {dot}, tot[1];
-        if (block_reduce_last<1>(v, partials, &st->counter, tot) && tid == 0) {
-            double pq = tot[0];
-            st->pq = pq;
-            if (!(pq > 0.0)) {
-                st->stop = 1;
-                st->status = (pq == pq && pq - pq == 0.0) ? 2 : 3;
-            } else {
-                st->alpha = st->rho / pq;
-            }
-        }
-    }
 }
 
 }  // namespace se

@@ -79,6 +79,7 @@ __device__ __forceinline__ void wrap(const SLGeom& g, double& cx, double& cy) {
 // exponents, returns the correctly rounded power in all but ~0.1% of cases, so
 // this reproduces the reference's monomial terms bit for bit almost always.
 __device__ __forceinline__ double ipow_cr(double x, int k) {
+#pragma clang fp contract(off)
     if (k == 0) return 1.0;
     double hi = x, lo = 0.0;
     for (int i = 1; i < k; i++) {
