@@ -8,8 +8,9 @@
 //   2. forward contractions u_xi, u_eta by element lines: one thread holds a
 //      line of N+1 values in registers and produces N+1 outputs, D is read
 //      with wave-uniform (broadcast) LDS addresses;
-//   3. per quadrature point: exact reference Jacobian from the per-column /
-//      per-row tables, one division, symmetric 2x2 metric, fluxes in place;
+//   3. per quadrature point: symmetric 2x2 metric of the reference's exact
+//      per-point Jacobian, assembled from per-element-column and per-element-row
+//      factor tables (see op_metric_tables), fluxes in place;
 //   4. owned nodes: back-contractions of the up to four containing elements
 //      with the needed columns of D held in registers, plus the GLL-diagonal
 //      mass term; the block's partial of p.Ap is stored for the CG update.
@@ -73,9 +74,13 @@ __global__ __launch_bounds__(256) void k_apply(OpParams P, const double* __restr
     constexpr int NP = N1 * N1;
     constexpr int BW = ((TX + 1) * N + 2) | 1;  // odd row pitch (in doubles): conflict-free column access
     constexpr int BH = (TY + 1) * N + 2;
+    constexpr int BWU = (TX + 1) * N + 2;       // used columns
     constexpr int NE = (TX + 1) * (TY + 1);
     constexpr int NC = N * N;                   // (rx, ry) combinations of an owned element
     constexpr int NG = 256 / NC;                // thread groups in the owned-node phase
+    constexpr int NOWN = (TX * TY + NG - 1) / NG;
+    constexpr int CS = 4 * N1 + NP;             // metric table stride per element column / row
+    constexpr int NG1 = (BH * BWU + 255) / 256;
 
     if (DOT && __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
 
@@ -83,123 +88,143 @@ __global__ __launch_bounds__(256) void k_apply(OpParams P, const double* __restr
     __shared__ double s_a[NE * NP];  // u_xi, then f_xi     [e][qy][qx]
     __shared__ double s_b[NE * NP];  // u_eta, then f_eta   [e][qy][qx]
     __shared__ double s_D[NP];
-    __shared__ double s_w[N1];
-    __shared__ double s_J00[(TX + 1) * N1], s_J10[(TX + 1) * NP];
-    __shared__ double s_J11[(TY + 1) * N1], s_J01[(TY + 1) * NP];
+    __shared__ double s_col[(TX + 1) * CS], s_row[(TY + 1) * CS];
     __shared__ int s_exw[TX + 1], s_eyw[TY + 1];
 
     const int tid = threadIdx.x;
     const int tilesX = (P.nEls0 + TX - 1) / TX;
     const int ex0 = (blockIdx.x % tilesX) * TX;
     const int ey0 = (blockIdx.x / tilesX) * TY;
-
-    if (tid <= TX) {
-        int ex = ex0 - 1 + tid;
+    const int txo = min(TX, P.nEls0 - ex0), tyo = min(TY, P.nEls1 - ey0);
+    auto exw = [&](int c) {  // wrapped element index of tile column c (-1: none)
+        int ex = ex0 - 1 + c;
         if (ex < 0) ex = P.per0 ? ex + P.nEls0 : -1;
-        if (ex >= P.nEls0) ex = -1;
-        s_exw[tid] = ex;
-    }
-    if (tid >= 64 && tid <= 64 + TY) {
-        int ey = ey0 - 1 + (tid - 64);
+        return ex >= P.nEls0 ? -1 : ex;
+    };
+    auto eyw = [&](int r) {
+        int ey = ey0 - 1 + r;
         if (ey < 0) ey = P.per1 ? ey + P.nEls1 : -1;
-        if (ey >= P.nEls1) ey = -1;
-        s_eyw[tid - 64] = ey;
+        return ey >= P.nEls1 ? -1 : ey;
+    };
+    const int cmb = tid % NC, grp4 = tid / NC;
+    const int rx = cmb % N, ry = cmb / N;
+
+    // ---- 1. loads: map entries first, then everything independent of them,
+    // then the values they index; a single barrier ---------------------------
+    int um[NG1];
+#pragma unroll
+    for (int g = 0; g < NG1; g++) {
+        const int i = tid + g * 256;
+        um[g] = -1;
+        if (i < BH * BWU) {
+            const int a = i / BWU, b = i % BWU;
+            int c, ix, r, iy;
+            if (b <= N) { c = 0; ix = b; }
+            else { int bb = b - N1; c = 1 + bb / N; ix = bb % N; if (ix == 0 && c >= 2) { c -= 1; ix = N; } }
+            if (a <= N) { r = 0; iy = a; }
+            else { int aa = a - N1; r = 1 + aa / N; iy = aa % N; if (iy == 0 && r >= 2) { r -= 1; iy = N; } }
+            const int ex = exw(c), ey = eyw(r);
+            if (ex >= 0 && ey >= 0) um[g] = P.f2v[(ey * N + iy) * P.nv0 + ex * N + ix];
+        }
     }
+    int ou[NOWN];  // unknowns of this thread's owned nodes (phase 4)
+#pragma unroll
+    for (int j = 0; j < NOWN; j++) {
+        const int oe = grp4 + j * NG;
+        ou[j] = -1;
+        if (grp4 < NG && oe < txo * tyo) {
+            const int ox = (oe % txo) * N + rx, oy = (oe / txo) * N + ry;
+            ou[j] = P.f2v[(ey0 * N + oy) * P.nv0 + ex0 * N + ox];
+        }
+    }
+    if (tid <= TX) s_exw[tid] = exw(tid);
+    if (tid >= 64 && tid <= 64 + TY) s_eyw[tid - 64] = eyw(tid - 64);
     for (int i = tid; i < NP; i += 256) s_D[i] = P.D[i];
-    if (tid < N1) s_w[tid] = P.w[tid];
-    __syncthreads();
-    for (int i = tid; i < (TX + 1) * N1; i += 256) {
-        int c = i / N1, e = s_exw[c];
-        s_J00[i] = e >= 0 ? P.J00[(size_t)e * N1 + i % N1] : 1.0;
+    for (int i = tid; i < (TX + 1) * CS; i += 256) {
+        const int e = exw(i / CS);
+        s_col[i] = e >= 0 ? P.colTab[(size_t)e * CS + i % CS] : 0.0;
     }
-    for (int i = tid; i < (TX + 1) * NP; i += 256) {
-        int c = i / NP, e = s_exw[c];
-        s_J10[i] = e >= 0 ? P.J10[(size_t)e * NP + i % NP] : 0.0;
+    for (int i = tid; i < (TY + 1) * CS; i += 256) {
+        const int e = eyw(i / CS);
+        s_row[i] = e >= 0 ? P.rowTab[(size_t)e * CS + i % CS] : 0.0;
     }
-    for (int i = tid; i < (TY + 1) * N1; i += 256) {
-        int r = i / N1, e = s_eyw[r];
-        s_J11[i] = e >= 0 ? P.J11[(size_t)e * N1 + i % N1] : 1.0;
-    }
-    for (int i = tid; i < (TY + 1) * NP; i += 256) {
-        int r = i / NP, e = s_eyw[r];
-        s_J01[i] = e >= 0 ? P.J01[(size_t)e * NP + i % NP] : 0.0;
-    }
-
-    // ---- 1. gather ---------------------------------------------------------
-    constexpr int BWU = (TX + 1) * N + 2;  // used columns
-    for (int i = tid; i < BH * BWU; i += 256) {
-        int a = i / BWU, b = i % BWU;
-        int c, ix, r, iy;
-        if (b <= N) { c = 0; ix = b; }
-        else { int bb = b - N1; c = 1 + bb / N; ix = bb % N; if (ix == 0 && c >= 2) { c -= 1; ix = N; } }
-        if (a <= N) { r = 0; iy = a; }
-        else { int aa = a - N1; r = 1 + aa / N; iy = aa % N; if (iy == 0 && r >= 2) { r -= 1; iy = N; } }
-        int ex = s_exw[c], ey = s_eyw[r];
-        double val = 0.0;
-        if (ex >= 0 && ey >= 0) {
-            int u = P.f2v[(ey * N + iy) * P.nv0 + ex * N + ix];
-            val = u >= 0 ? in[u] : 0.0;
-        }
-        s_u[a * BW + b] = val;
+    double uv[NG1];
+#pragma unroll
+    for (int g = 0; g < NG1; g++) uv[g] = um[g] >= 0 ? in[um[g]] : 0.0;
+#pragma unroll
+    for (int g = 0; g < NG1; g++) {
+        const int i = tid + g * 256;
+        if (i < BH * BWU) s_u[(i / BWU) * BW + i % BWU] = uv[g];
     }
     __syncthreads();
 
-    // ---- 2. forward contractions by element lines -----------------------------
-    for (int l = tid; l < 2 * NE * N1; l += 256) {
-        const int e = l / (2 * N1), rem = l % (2 * N1), dir = rem / N1, idx = rem % N1;
-        const int c = e % (TX + 1), r = e / (TX + 1);
-        if (s_exw[c] < 0 || s_eyw[r] < 0) continue;
-        double u[N1];
-        if (dir == 0) {
-            const double* row = &s_u[slot<N>(r, idx) * BW];
+    // ---- 2. forward contractions by element lines (waves 0-1: x lines,
+    // waves 2-3: y lines; a thread keeps its line index across elements) -------
+    {
+        constexpr int LG = 128 / N1;  // line groups per direction
+        const int dir = tid >> 7, tt = tid & 127;
+        const int idx = tt % N1, lg = tt / N1;
+        if (lg < LG) {
+            for (int e = lg; e < NE; e += LG) {
+                const int c = e % (TX + 1), r = e / (TX + 1);
+                if (s_exw[c] < 0 || s_eyw[r] < 0) continue;
+                double u[N1];
+                if (dir == 0) {
+                    const double* row = &s_u[slot<N>(r, idx) * BW];
 #pragma unroll
-            for (int k = 0; k < N1; k++) u[k] = row[slot<N>(c, k)];
-        } else {
-            const int col = slot<N>(c, idx);
+                    for (int k = 0; k < N1; k++) u[k] = row[slot<N>(c, k)];
 #pragma unroll
-            for (int k = 0; k < N1; k++) u[k] = s_u[slot<N>(r, k) * BW + col];
-        }
+                    for (int q = 0; q < N1; q++) {
+                        double s = 0.0;
 #pragma unroll
-        for (int q = 0; q < N1; q++) {
-            double s = 0.0;
+                        for (int k = 0; k < N1; k++) s += s_D[q * N1 + k] * u[k];
+                        s_a[e * NP + idx * N1 + q] = s;  // u_xi(qx = q, qy = idx)
+                    }
+                } else {
+                    const int col = slot<N>(c, idx);
 #pragma unroll
-            for (int k = 0; k < N1; k++) s += s_D[q * N1 + k] * u[k];
-            if (dir == 0) s_a[e * NP + idx * N1 + q] = s;  // u_xi(qx = q, qy = idx)
-            else s_b[e * NP + q * N1 + idx] = s;           // u_eta(qx = idx, qy = q)
+                    for (int k = 0; k < N1; k++) u[k] = s_u[slot<N>(r, k) * BW + col];
+#pragma unroll
+                    for (int q = 0; q < N1; q++) {
+                        double s = 0.0;
+#pragma unroll
+                        for (int k = 0; k < N1; k++) s += s_D[q * N1 + k] * u[k];
+                        s_b[e * NP + q * N1 + idx] = s;  // u_eta(qx = idx, qy = q)
+                    }
+                }
+            }
         }
     }
     __syncthreads();
 
-    // ---- 3. metric and fluxes per quadrature point ----------------------------
-    for (int i = tid; i < NE * NP; i += 256) {
-        const int e = i / NP, q = i % NP;
-        const int c = e % (TX + 1), r = e / (TX + 1);
-        if (s_exw[c] < 0 || s_eyw[r] < 0) continue;
+    // ---- 3. metric and fluxes per quadrature point (fixed point per thread) -----
+    {
+        constexpr int QG = 256 / NP;
+        const int q = tid % NP, qg = tid / NP;
         const int qx = q % N1, qy = q / N1;
-        const double j00 = s_J00[c * N1 + qx], j01 = s_J01[r * NP + qy * N1 + qx];
-        const double j10 = s_J10[c * NP + qx * N1 + qy], j11 = s_J11[r * N1 + qy];
-        double det, W;
-        {
-#pragma clang fp contract(off)
-            det = j00 * j11 - j01 * j10;
-            const double wq = s_w[qx] * s_w[qy];
-            W = (P.detj ? det * wq : 1.0 * wq) * P.c_lap;
+        const double c = P.c_lap;
+        const double wq = P.w[qx] * P.w[qy];
+        if (qg < QG) {
+            for (int e = qg; e < NE; e += QG) {
+                const int ce = e % (TX + 1), re = e / (TX + 1);
+                if (s_exw[ce] < 0 || s_eyw[re] < 0) continue;
+                const double* cx = &s_col[ce * CS];
+                const double* ry_ = &s_row[re * CS];
+                const double g00 = c * cx[qx] * ry_[qy];
+                const double g11 = c * cx[N1 + qx] * ry_[N1 + qy];
+                const double g01 = -c * wq * (cx[2 * N1 + qx] * ry_[4 * N1 + qy * N1 + qx] +
+                                              cx[4 * N1 + qx * N1 + qy] * ry_[2 * N1 + qy]);
+                const int i = e * NP + q;
+                const double ux = s_a[i], uy = s_b[i];
+                s_a[i] = g00 * ux + g01 * uy;
+                s_b[i] = g01 * ux + g11 * uy;
+            }
         }
-        const double inv = 1.0 / det;
-        const double i00 = j11 * inv, i11 = j00 * inv, i01 = -j01 * inv, i10 = -j10 * inv;
-        const double g00 = W * (i00 * i00 + i10 * i10);
-        const double g01 = W * (i00 * i01 + i10 * i11);
-        const double g11 = W * (i01 * i01 + i11 * i11);
-        const double ux = s_a[i], uy = s_b[i];
-        s_a[i] = g00 * ux + g01 * uy;
-        s_b[i] = g01 * ux + g11 * uy;
     }
     __syncthreads();
 
     // ---- 4. owned nodes -------------------------------------------------------
-    const int txo = min(TX, P.nEls0 - ex0), tyo = min(TY, P.nEls1 - ey0);
     double dot = 0.0;
-
     // contribution of tile element (c, r) at its local node (lx, ly)
     auto contrib = [&](int c, int r, int lx, int ly, const double* Dx, const double* Dy) -> double {
         const int e = r * (TX + 1) + c;
@@ -210,19 +235,11 @@ __global__ __launch_bounds__(256) void k_apply(OpParams P, const double* __restr
         for (int q = 0; q < N1; q++) k += Dx[q] * fx[q];
 #pragma unroll
         for (int q = 0; q < N1; q++) k += Dy[q] * fy[q * N1];
-        double mass;
-        {
-#pragma clang fp contract(off)
-            const double det = s_J00[c * N1 + lx] * s_J11[r * N1 + ly] -
-                               s_J01[r * NP + ly * N1 + lx] * s_J10[c * NP + lx * N1 + ly];
-            mass = (det * (s_w[lx] * s_w[ly])) * P.c_mass;
-        }
+        const double mass = P.c_mass * s_col[c * CS + 3 * N1 + lx] * s_row[r * CS + 3 * N1 + ly];
         return k + mass * s_u[slot<N>(r, ly) * BW + slot<N>(c, lx)];
     };
 
-    if (tid < NG * NC) {
-        const int cmb = tid % NC, g = tid / NC;
-        const int rx = cmb % N, ry = cmb / N;
+    if (grp4 < NG) {
         double Dcx[N1], Dcy[N1], DcN[N1];
 #pragma unroll
         for (int q = 0; q < N1; q++) {
@@ -230,11 +247,13 @@ __global__ __launch_bounds__(256) void k_apply(OpParams P, const double* __restr
             Dcy[q] = s_D[q * N1 + ry];
             DcN[q] = s_D[q * N1 + N];
         }
-        for (int oe = g; oe < txo * tyo; oe += NG) {
-            const int ce = 1 + oe % txo, re = 1 + oe / txo;
-            const int ox = (ce - 1) * N + rx, oy = (re - 1) * N + ry;
-            const int u = P.f2v[(ey0 * N + oy) * P.nv0 + ex0 * N + ox];
+#pragma unroll
+        for (int j = 0; j < NOWN; j++) {
+            const int oe = grp4 + j * NG;
+            if (oe >= txo * tyo) break;
+            const int u = ou[j];
             if (u < 0) continue;
+            const int ce = 1 + oe % txo, re = 1 + oe / txo;
             double sum = 0.0;
             if (ry == 0 && s_eyw[re - 1] >= 0) {
                 if (rx == 0 && s_exw[ce - 1] >= 0) sum += contrib(ce - 1, re - 1, N, N, DcN, DcN);

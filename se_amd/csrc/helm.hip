@@ -323,7 +323,64 @@ OpParams op_params(const se_op* op, double scale) {
     const double* basis = m->d_basis.as<double>();
     P.w = basis + n1;
     P.D = basis + 2 * n1;
+    P.colTab = op->tabs.as<double>();
+    P.rowTab = op->tabs.as<double>() + (size_t)m->nEls[0] * (4 * n1 + n1 * n1);
     return P;
+}
+
+// Metric factor tables of the hot apply.  At quadrature point (qx,qy) of
+// element (ex,ey) the reference's Jacobian (Mesh.cpp:217-236) has diagonal
+// entries j00(ex,qx), j11(ey,qy) and round-off-sized off-diagonal entries
+// j01(ey,qy,qx), j10(ex,qx,qy) (their size is that of the cancellation in
+// sum_i D[q][i] * coordinate).  Keeping every term of the metric
+// W Jinv Jinv^T down to relative order (j01 j10)/(j00 j11), about 1e-26:
+//   g00 = c PX0(ex,qx) PY0(ey,qy),   g11 = c PX1(ex,qx) PY1(ey,qy),
+//   g01 = -c w_qx w_qy [PX2(ex,qx) QY(ey,qy,qx) + QX(ex,qx,qy) PY2(ey,qy)],
+//   mass = c_mass MX(ex,qx) MY(ey,qy)
+// literal  (W = w c):      PX0 = w/j00^2, PX1 = w, PX2 = 1/j00^2, QX = j10/j00,
+//                          PY0 = w, PY1 = w/j11^2, PY2 = 1/j11^2, QY = j01/j11
+// physical (W = det w c):  PX0 = w/j00, PX1 = w j00, PX2 = 1/j00, QX = j10,
+//                          PY0 = w j11, PY1 = w/j11, PY2 = 1/j11, QY = j01
+// and MX = w j00, MY = w j11.  Per column / row the layout is
+// [X0(N+1) X1(N+1) X2(N+1) M(N+1) Q((N+1)^2)].
+static int op_metric_tables(se_op* op) {
+    const se_mesh* m = op->mesh;
+    const int n1 = m->N + 1, CS = 4 * n1 + n1 * n1;
+    const std::vector<double>& w = m->gll.w;
+    std::vector<double> t((size_t)(m->nEls[0] + m->nEls[1]) * CS, 0.0);
+    const bool phys = op->detj == SE_DETJ_PHYSICAL;
+    for (int ex = 0; ex < m->nEls[0]; ex++) {
+        double* T = &t[(size_t)ex * CS];
+        for (int q = 0; q < n1; q++) {
+            const double j = m->J00[(size_t)ex * n1 + q];
+            T[q] = phys ? w[q] / j : w[q] / (j * j);
+            T[n1 + q] = phys ? w[q] * j : w[q];
+            T[2 * n1 + q] = phys ? 1.0 / j : 1.0 / (j * j);
+            T[3 * n1 + q] = w[q] * j;
+            for (int qy = 0; qy < n1; qy++) {
+                const double j10 = m->J10[((size_t)ex * n1 + q) * n1 + qy];
+                T[4 * n1 + q * n1 + qy] = phys ? j10 : j10 / j;
+            }
+        }
+    }
+    for (int ey = 0; ey < m->nEls[1]; ey++) {
+        double* T = &t[(size_t)(m->nEls[0] + ey) * CS];
+        for (int q = 0; q < n1; q++) {
+            const double j = m->J11[(size_t)ey * n1 + q];
+            T[q] = phys ? w[q] * j : w[q];
+            T[n1 + q] = phys ? w[q] / j : w[q] / (j * j);
+            T[2 * n1 + q] = phys ? 1.0 / j : 1.0 / (j * j);
+            T[3 * n1 + q] = w[q] * j;
+            for (int qx = 0; qx < n1; qx++) {
+                const double j01 = m->J01[((size_t)ey * n1 + q) * n1 + qx];
+                T[4 * n1 + q * n1 + qx] = phys ? j01 : j01 / j;
+            }
+        }
+    }
+    int rc = op->tabs.alloc(sizeof(double) * t.size());
+    if (rc) return rc;
+    SE_HIP(hipMemcpy(op->tabs.ptr, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
+    return SE_OK;
 }
 
 int op_tile(const se_op* op) {
@@ -405,6 +462,7 @@ extern "C" int se_helm_create(se_mesh* m, double c_mass, double c_lap, int detj_
     if (!rc) rc = op->lift.alloc(nb);
     if (!rc) rc = op->state.alloc(sizeof(CgState));
     if (!rc) rc = op_ensure_work(op);
+    if (!rc) rc = op_metric_tables(op);
     if (rc) {
         se_helm_destroy(op);
         return rc;

@@ -40,6 +40,10 @@ struct OpParams {
     const double* J01;  // [nEls1][N+1 (qy)][N+1 (qx)]
     const double* w;    // [N+1]
     const double* D;    // [(N+1)^2]
+    // metric factor tables of the hot apply, per element column / row
+    // (stride 4(N+1) + (N+1)^2, layout in op_metric_tables)
+    const double* colTab;
+    const double* rowTab;
 };
 
 // Scalars of the CG recurrence, in device memory.  Written by block 0 of the
