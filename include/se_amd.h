@@ -51,7 +51,10 @@ enum {
     SE_SOLVE_CONVERGED = 0, /* true ||b - A x||_2 <= rtol ||b||_2 */
     SE_SOLVE_MAXIT = 1,     /* iteration limit reached */
     SE_SOLVE_BREAKDOWN = 2, /* p.Ap <= 0: operator or preconditioner indefinite */
-    SE_SOLVE_NONFINITE = 3  /* NaN / Inf in the data */
+    SE_SOLVE_NONFINITE = 3, /* NaN / Inf in the data */
+    SE_SOLVE_STAGNATED = 4  /* mixed precision only: the inner precision cannot
+                               resolve this system (refinement stopped making
+                               progress); solve in FP64 or FP32_IR instead */
 };
 
 typedef struct se_ctx se_ctx;
@@ -152,6 +155,13 @@ typedef struct {
 } se_solve_info;
 
 void se_solve_opts_default(se_solve_opts* opts);
+/* Inner operator of the mixed-precision solves (no counterpart in the
+ * reference, which solves in FP64 only): y = sign*A~ x with A~ the separable
+ * (Kronecker-sum) form of A without its round-off-sized Jacobian cross terms,
+ * FP32 band tables; the input is rounded to FP32 (SE_PREC_FP32_IR) or BF16
+ * (SE_PREC_BF16_IR), arithmetic and output are FP32.  sign is +-1 so that
+ * sign*A has a positive diagonal.  Host arrays of n values. */
+int se_helm_apply_inner(se_op* op, int precision, const float* x, float* y, int* sign);
 /* Jacobi-PCG on A x = f, x0 = 0  [ref: KSPSolve with -ksp_type cg -pc_type
  *  jacobi, src/linsys/HelmholtzEqn.cpp:120-131] (sign-normalised: a
  *  negative-definite A is solved as -A x = -f) */

@@ -16,10 +16,11 @@ LIB_PATH = os.path.join(HERE, "lib", "libse_amd.so")
 
 DETJ_LITERAL, DETJ_PHYSICAL = 0, 1
 PREC_FP64, PREC_FP32_IR, PREC_BF16_IR = 0, 1, 2
-SOLVE_CONVERGED, SOLVE_MAXIT, SOLVE_BREAKDOWN, SOLVE_NONFINITE = 0, 1, 2, 3
+SOLVE_CONVERGED, SOLVE_MAXIT, SOLVE_BREAKDOWN, SOLVE_NONFINITE, SOLVE_STAGNATED = 0, 1, 2, 3, 4
 
 _D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _I = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_F = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _lib = None
 
 
@@ -76,6 +77,7 @@ def lib():
             "se_helm_destroy": (i, [vp]),
             "se_helm_apply": (i, [vp, _D, _D]),
             "se_helm_apply_dev": (i, [vp, vp, vp]),
+            "se_helm_apply_inner": (i, [vp, i, _F, _F, C.POINTER(i)]),
             "se_helm_inv_diag": (i, [vp, _D]),
             "se_helm_set_dirichlet": (i, [vp, _D]),
             "se_helm_set_dirichlet_dev": (i, [vp, vp]),
@@ -237,6 +239,14 @@ class Helm:
         y = np.zeros(self.mesh.n)
         check(lib().se_helm_apply(self.h, _d(x), y), "se_helm_apply")
         return y
+
+    def apply_inner(self, x, precision=PREC_FP32_IR):
+        """Separable low-precision inner operator: returns (sign*A~ x as float32, sign)."""
+        xf = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros(self.mesh.n, dtype=np.float32)
+        sg = C.c_int(0)
+        check(lib().se_helm_apply_inner(self.h, precision, xf, y, C.byref(sg)), "se_helm_apply_inner")
+        return y, sg.value
 
     def apply_dev(self, x_ptr, y_ptr):
         check(lib().se_helm_apply_dev(self.h, x_ptr, y_ptr), "se_helm_apply_dev")

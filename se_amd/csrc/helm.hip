@@ -380,6 +380,7 @@ static int op_metric_tables(se_op* op) {
     int rc = op->tabs.alloc(sizeof(double) * t.size());
     if (rc) return rc;
     SE_HIP(hipMemcpy(op->tabs.ptr, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
+    op->tabs_host = std::move(t);
     return SE_OK;
 }
 

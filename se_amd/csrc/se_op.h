@@ -14,6 +14,10 @@ struct se_op {
     se::DevBuf dinv_s;     // inverse diagonal of sign*A
     se::DevBuf lift;       // Dirichlet lifting b
     se::DevBuf tabs;       // metric factor tables: element columns, then element rows
+    std::vector<double> tabs_host;
+    // separable inner operator (FP32 band tables) and mixed-precision work
+    se::DevBuf sep, sep_state, sep_part;
+    int sep_ready = 0, sep_sign = 0;
     se::DevBuf state, partials;
     se::DevBuf b, x, r, p, q, f;             // FP64 CG work vectors
     se::DevBuf sx, sy, sfield, sfield2;      // host-staging buffers
@@ -26,7 +30,7 @@ struct se_op {
     int graph_iters_lo = 0;
 
     void release_all() {
-        se::DevBuf* bufs[] = {&dinv, &dinv_s, &lift, &tabs, &state, &partials, &b, &x, &r, &p, &q, &f,
+        se::DevBuf* bufs[] = {&sep, &sep_state, &sep_part, &dinv, &dinv_s, &lift, &tabs, &state, &partials, &b, &x, &r, &p, &q, &f,
                               &sx, &sy, &sfield, &sfield2, &lo_r, &lo_p, &lo_q, &lo_z, &lo_x, &lo_dinv,
                               &lo_geo, &d64};
         for (auto* b : bufs) b->release();
