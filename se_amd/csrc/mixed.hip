@@ -110,6 +110,51 @@ __global__ void k_to_f32(int n, const double* __restrict__ a, double s, float* _
 // recurrence r -= alpha A p stays in FP32 with p as stored, so r tracks the
 // inner system's residual; rounding p to bf16 only perturbs conjugacy.
 // ---------------------------------------------------------------------------
+// four consecutive elements per thread: one 16-byte (float) or 8-byte (bf16)
+// access per vector, scalar accesses on a partial last quad
+__device__ __forceinline__ void load4(const float* __restrict__ p, int i, int n, float (&v)[4]) {
+    if (i + 3 < n) {
+        const float4 t = *reinterpret_cast<const float4*>(p + i);
+        v[0] = t.x, v[1] = t.y, v[2] = t.z, v[3] = t.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = i + k < n ? p[i + k] : 0.0f;
+    }
+}
+__device__ __forceinline__ void load4(const __hip_bfloat16* __restrict__ p, int i, int n, float (&v)[4]) {
+    if (i + 3 < n) {
+        const uint2 t = *reinterpret_cast<const uint2*>(p + i);
+        v[0] = __uint_as_float(t.x << 16), v[1] = __uint_as_float(t.x & 0xffff0000u);
+        v[2] = __uint_as_float(t.y << 16), v[3] = __uint_as_float(t.y & 0xffff0000u);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = i + k < n ? __bfloat162float(p[i + k]) : 0.0f;
+    }
+}
+__device__ __forceinline__ void store4(float* __restrict__ p, int i, int n, const float (&v)[4]) {
+    if (i + 3 < n) {
+        *reinterpret_cast<float4*>(p + i) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (i + k < n) p[i + k] = v[k];
+    }
+}
+__device__ __forceinline__ unsigned bf16_bits(float x) {
+    return (unsigned)__bfloat16_as_ushort(__float2bfloat16(x));
+}
+__device__ __forceinline__ void store4(__hip_bfloat16* __restrict__ p, int i, int n, const float (&v)[4]) {
+    if (i + 3 < n) {
+        uint2 t;
+        t.x = bf16_bits(v[0]) | (bf16_bits(v[1]) << 16);
+        t.y = bf16_bits(v[2]) | (bf16_bits(v[3]) << 16);
+        *reinterpret_cast<uint2*>(p + i) = t;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (i + k < n) p[i + k] = __float2bfloat16(v[k]);
+    }
+}
 __global__ __launch_bounds__(kVB) void k_in_start(int n, const double* __restrict__ r64, const double* __restrict__ mu_scale,
                                                   const float* __restrict__ dinv, float* __restrict__ r, float* __restrict__ x,
                                                   double* __restrict__ part) {
@@ -135,6 +180,15 @@ template <typename T>
 __global__ __launch_bounds__(kVB) void k_in_dir(int mode, int n, double rtol, const float* __restrict__ dinv,
                                                 const float* __restrict__ r, T* __restrict__ p,
                                                 const double* __restrict__ part_b, int np_b, CgState* __restrict__ st) {
+    // the first quad's loads are issued before the stop flag and the partial
+    // sums are read, so their latencies overlap
+    const int i0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x), stride = 4 * gridDim.x * blockDim.x;
+    float rv[4], dv[4], pv[4];
+    if (i0 < n) {
+        load4(r, i0, n, rv);
+        load4(dinv, i0, n, dv);
+        if (mode != IN_FIRST) load4(p, i0, n, pv);
+    }
     if (mode == IN_NORMAL && __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     double t[4];
     reduce_partials<4, kVB>(part_b, np_b, t);
@@ -191,9 +245,18 @@ __global__ __launch_bounds__(kVB) void k_in_dir(int mode, int n, double rtol, co
         beta = rho_new / st->rho_prev;
     }
     const float fmu = (float)mu, fb = (float)beta;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const float z = dinv[i] * r[i] - fmu;
-        p[i] = from_f<T>(mode == IN_FIRST ? z : z + fb * to_f<T>(p[i]));
+    for (int i = i0; i < n; i += stride) {
+        if (i != i0) {
+            load4(r, i, n, rv);
+            load4(dinv, i, n, dv);
+            if (mode != IN_FIRST) load4(p, i, n, pv);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float z = dv[k] * rv[k] - fmu;
+            pv[k] = mode == IN_FIRST ? z : z + fb * pv[k];
+        }
+        store4(p, i, n, pv);
     }
 }
 
@@ -202,6 +265,15 @@ __global__ __launch_bounds__(kVB) void k_in_update(int n, const T* __restrict__ 
                                                    const float* __restrict__ dinv, float* __restrict__ x,
                                                    float* __restrict__ r, const double* __restrict__ part_a, int np_a,
                                                    double* __restrict__ part_b, CgState* __restrict__ st) {
+    const int i0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x), stride = 4 * gridDim.x * blockDim.x;
+    float pv[4], qv[4], dv[4], xv[4], rv[4];
+    if (i0 < n) {
+        load4(p, i0, n, pv);
+        load4(q, i0, n, qv);
+        load4(dinv, i0, n, dv);
+        load4(x, i0, n, xv);
+        load4(r, i0, n, rv);
+    }
     if (__hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     double t[1];
     reduce_partials<1, kVB>(part_a, np_a, t);
@@ -224,15 +296,26 @@ __global__ __launch_bounds__(kVB) void k_in_update(int n, const T* __restrict__ 
     }
     const float fa = (float)alpha;
     double rr = 0.0, rz = 0.0, sz = 0.0, sr = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        x[i] += fa * to_f<T>(p[i]);
-        const float rv = r[i] - fa * to_f<TQ>(q[i]);
-        r[i] = rv;
-        const float z = dinv[i] * rv;
-        rr += (double)rv * rv;
-        rz += (double)rv * z;
-        sz += z;
-        sr += rv;
+    for (int i = i0; i < n; i += stride) {
+        if (i != i0) {
+            load4(p, i, n, pv);
+            load4(q, i, n, qv);
+            load4(dinv, i, n, dv);
+            load4(x, i, n, xv);
+            load4(r, i, n, rv);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            xv[k] += fa * pv[k];
+            rv[k] -= fa * qv[k];
+            const float z = dv[k] * rv[k];
+            rr += (double)rv[k] * rv[k];
+            rz += (double)rv[k] * z;
+            sz += z;
+            sr += rv[k];
+        }
+        store4(x, i, n, xv);
+        store4(r, i, n, rv);
     }
     double v[4] = {rr, rz, sz, sr};
     block_store_partials<4, kVB>(v, part_b);
@@ -477,7 +560,7 @@ int ir_solve(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, 
     const float* dinv = op->sep.as<float>() + sep_layout(m).dinv;
     const double sgn = (double)op->sign;
     const int null = o.null_space ? 1 : 0;
-    const double inner_tol = o.inner_rtol > 0.0 ? o.inner_rtol : (sizeof(T) == 2 ? 1e-3 : 1e-5);
+    const double inner_tol = o.inner_rtol > 0.0 ? o.inner_rtol : (sizeof(T) == 2 ? 1e-3 : 1e-6);
 
     SE_HIP(hipEventRecord(c->ev0, s));
     if (null) hipLaunchKernelGGL(k_ir_sum, dim3(vb), dim3(kVB), 0, s, n, d_f, sgn, part + L.s);

@@ -12,7 +12,7 @@
 // with 2N+1 band coefficients per line (periodic lines wrap, band entries
 // beyond a non-periodic edge are zero).  One workgroup owns a 64 x 32 tile;
 // the x sweep keeps a column's band in registers, the y sweep a column window
-// in registers and the row's band in wave-uniform (scalar) loads.
+// in registers and reads the row's band from LDS (one address per wave).
 #pragma once
 #include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
@@ -38,57 +38,110 @@ struct SepParams {
     const float* my;  // [nU1]
 };
 
-constexpr int SEP_TX = 64, SEP_TY = 32;
+constexpr int SEP_TX = 64, SEP_TY = 32, SEP_NT = 256;
 
-template <int N, typename T, typename TQ, bool DOT>
-__global__ __launch_bounds__(256) void k_sep_apply(SepParams S, const T* __restrict__ in, TQ* __restrict__ out,
-                                                    double* __restrict__ partials, const CgState* __restrict__ st) {
+// Every global load (tile + halo, both band tables, the line scales) is issued
+// before the first barrier, so a workgroup waits for memory once.
+template <int N, typename T, typename TQ, bool DOT, int TY = SEP_TY, int NT = SEP_NT>
+__global__ __launch_bounds__(NT) void k_sep_apply(SepParams S, const T* __restrict__ in, TQ* __restrict__ out,
+                                                   double* __restrict__ partials, const CgState* __restrict__ st) {
     constexpr int B = 2 * N + 1;
-    constexpr int W = SEP_TX + 2 * N, H = SEP_TY + 2 * N;
+    constexpr int W = SEP_TX + 2 * N, H = TY + 2 * N;
     constexpr int WP = W + 1;
-    constexpr int RPT = SEP_TY / 4;  // rows per thread
-    if (DOT && __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    constexpr int RPT = TY / (NT / 64);  // rows per thread
+    constexpr int NG = (TY * B + NT - 1) / NT;
     __shared__ float s_u[H * WP];
-    __shared__ float s_x[SEP_TY * SEP_TX];
+    __shared__ float s_gy[TY * B];
+    __shared__ float s_sy[TY], s_my[TY];
 
     const int tid = threadIdx.x;
     const int tilesX = (S.nU0 + SEP_TX - 1) / SEP_TX;
     const int I0 = (blockIdx.x % tilesX) * SEP_TX;
-    const int J0 = (blockIdx.x / tilesX) * SEP_TY;
+    const int J0 = (blockIdx.x / tilesX) * TY;
 
-    // tile + halo; periodic lines wrap, other out-of-grid values are zero
-    for (int i = tid; i < H * W; i += 256) {
-        const int a = i / W, b = i % W;
-        int I = I0 - N + b, J = J0 - N + a;
-        bool ok = true;
-        if (I < 0 || I >= S.nU0) {
-            if (S.per0) I = (I % S.nU0 + S.nU0) % S.nU0;
-            else ok = false;
-        }
-        if (J < 0 || J >= S.nU1) {
-            if (S.per1) J = (J % S.nU1 + S.nU1) % S.nU1;
-            else ok = false;
-        }
-        s_u[a * WP + b] = ok ? to_f<T>(in[(size_t)J * S.nU0 + I]) : 0.0f;
+    // tile + halo; periodic lines wrap, other out-of-grid values are zero.
+    // Wave wv loads rows wv, wv + NW, ... (row index wave-uniform); lane l
+    // holds tile columns l and 64 + l (the latter for l < 2N).
+    constexpr int NW = NT / 64;
+    constexpr int RL = (H + NW - 1) / NW;  // rows loaded per wave
+    const int lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int Ic0 = I0 - N + lane, Ic1 = I0 - N + 64 + lane;
+    bool c0 = true, c1 = lane < 2 * N;
+    if (S.per0) {
+        Ic0 = Ic0 < 0 ? Ic0 + S.nU0 : (Ic0 >= S.nU0 ? Ic0 - S.nU0 : Ic0);
+        Ic1 = Ic1 >= S.nU0 ? Ic1 - S.nU0 : Ic1;
     }
-    const int j = tid & 63, g = tid >> 6;
+    // loads are unconditional from clamped (in-range) addresses, the value
+    // is then zeroed where the node is outside the grid: no divergent branches
+    c0 = c0 && Ic0 >= 0 && Ic0 < S.nU0;
+    c1 = c1 && Ic1 >= 0 && Ic1 < S.nU0;
+    Ic0 = c0 ? Ic0 : 0;
+    Ic1 = c1 ? Ic1 : 0;
+    float u0[RL], u1[RL];
+#pragma unroll
+    for (int k = 0; k < RL; k++) {
+        const int a = wv + NW * k;
+        int J = J0 - N + a;
+        if (S.per1) J = J < 0 ? J + S.nU1 : (J >= S.nU1 ? J - S.nU1 : J);
+        const bool rok = a < H && J >= 0 && J < S.nU1;
+        const T* row = in + (size_t)(rok ? J : 0) * S.nU0;
+        const float v0 = to_f<T>(row[Ic0]), v1 = to_f<T>(row[Ic1]);
+        u0[k] = (rok && c0) ? v0 : 0.0f;
+        u1[k] = (rok && c1) ? v1 : 0.0f;
+    }
+    float gyv[NG];
+#pragma unroll
+    for (int k = 0; k < NG; k++) {
+        const int i = tid + NT * k;
+        const bool ok = i < TY * B && J0 + i / B < S.nU1;
+        const float v = S.gy[ok ? (size_t)J0 * B + i : 0];
+        gyv[k] = ok ? v : 0.0f;
+    }
+    const bool lok = tid < TY && J0 + tid < S.nU1;
+    float syv = S.sy[lok ? J0 + tid : 0], myv = S.my[lok ? J0 + tid : 0];
+    syv = lok ? syv : 0.0f;
+    myv = lok ? myv : 0.0f;
+    const int j = lane, g = wv;
     const int I = I0 + j;
     const bool colok = I < S.nU0;
+    const int Ic = colok ? I : 0;
     float gxr[B];
 #pragma unroll
-    for (int s = 0; s < B; s++) gxr[s] = colok ? S.gx[(size_t)I * B + s] : 0.0f;
-    const float sxI = colok ? S.sx[I] : 0.0f, mxI = colok ? S.mx[I] : 0.0f;
+    for (int s = 0; s < B; s++) gxr[s] = S.gx[(size_t)Ic * B + s];
+    const float sxI = S.sx[Ic], mxI = S.mx[Ic];
+    // a stopped solve skips the work (checked after the loads are in flight;
+    // the flag is the same for every thread of the workgroup)
+    if (DOT && __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+
+#pragma unroll
+    for (int k = 0; k < RL; k++) {
+        const int a = wv + NW * k;
+        if (a < H) {
+            s_u[a * WP + lane] = u0[k];
+            if (lane < 2 * N) s_u[a * WP + 64 + lane] = u1[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NG; k++) {
+        const int i = tid + NT * k;
+        if (i < TY * B) s_gy[i] = gyv[k];
+    }
+    if (tid < TY) {
+        s_sy[tid] = syv;
+        s_my[tid] = myv;
+    }
     __syncthreads();
 
-    // x sweep: row r of the tile, band of column j
+    // x sweep of rows g*RPT .. g*RPT+RPT-1 with column j's band
+    float ax[RPT];
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
-        const int r = g * RPT + k;
-        const float* row = &s_u[(r + N) * WP + j];
+        const float* row = &s_u[(g * RPT + k + N) * WP + j];
         float acc = 0.0f;
 #pragma unroll
         for (int s = 0; s < B; s++) acc += gxr[s] * row[s];
-        s_x[r * SEP_TX + j] = acc;
+        ax[k] = acc;
     }
     // y sweep: column window of rows g*RPT - N .. g*RPT + RPT - 1 + N
     float win[RPT + 2 * N];
@@ -99,21 +152,20 @@ __global__ __launch_bounds__(256) void k_sep_apply(SepParams S, const T* __restr
     for (int k = 0; k < RPT; k++) {
         const int r = g * RPT + k;
         const int J = J0 + r;
-        if (J >= S.nU1) break;
-        const float* gy = &S.gy[(size_t)J * B];  // wave-uniform row
+        const float* gy = &s_gy[r * B];  // same address in every lane: broadcast
         float acc = 0.0f;
 #pragma unroll
         for (int s = 0; s < B; s++) acc += gy[s] * win[k + s];
         const float u = win[k + N];
-        const float y = S.sy[J] * s_x[r * SEP_TX + j] + sxI * acc + (mxI * S.my[J]) * u;
-        if (colok) {
+        const float y = s_sy[r] * ax[k] + sxI * acc + (mxI * s_my[r]) * u;
+        if (colok && J < S.nU1) {
             out[(size_t)J * S.nU0 + I] = from_f<TQ>(y);
             if (DOT) dot += (double)u * (double)y;
         }
     }
     if (DOT) {
         double v[1] = {dot};
-        block_store_partials<1, 256>(v, partials);
+        block_store_partials<1, NT>(v, partials);
     }
 }
 
