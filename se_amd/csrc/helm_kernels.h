@@ -54,10 +54,6 @@ struct CgState {
     int iter, stop, status, maxit, null_space, n;
     unsigned counter;
     int pad;
-    // mixed-precision inner solves: smallest residual so far, iterations
-    // since it, and the window after which the inner solve counts as stalled
-    double best;
-    int since_best, stall;
 };
 
 enum CgStatus { CG_RUNNING = -1, CG_PENDING = -2 };
