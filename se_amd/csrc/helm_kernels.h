@@ -58,10 +58,45 @@ struct CgState {
 
 enum CgStatus { CG_RUNNING = -1, CG_PENDING = -2 };
 
+// Sum over the 64 lanes of a wave, returned to every lane.  Inside each row of
+// 16 lanes a shift-and-add scan with DPP row shifts (lanes shifted in from
+// outside the row read 0) leaves the row total in its last lane; the four row
+// totals are then read as wave-uniform values and added in lane order.  The
+// summation order is fixed, so the result is deterministic.
+__device__ __forceinline__ double dpp_shr(double v, int sh) {
+    const long long b = __double_as_longlong(v);
+    int lo = (int)b, hi = (int)(b >> 32);
+    switch (sh) {
+        case 1:
+            lo = __builtin_amdgcn_update_dpp(0, lo, 0x111, 0xf, 0xf, true);
+            hi = __builtin_amdgcn_update_dpp(0, hi, 0x111, 0xf, 0xf, true);
+            break;
+        case 2:
+            lo = __builtin_amdgcn_update_dpp(0, lo, 0x112, 0xf, 0xf, true);
+            hi = __builtin_amdgcn_update_dpp(0, hi, 0x112, 0xf, 0xf, true);
+            break;
+        case 4:
+            lo = __builtin_amdgcn_update_dpp(0, lo, 0x114, 0xf, 0xf, true);
+            hi = __builtin_amdgcn_update_dpp(0, hi, 0x114, 0xf, 0xf, true);
+            break;
+        default:
+            lo = __builtin_amdgcn_update_dpp(0, lo, 0x118, 0xf, 0xf, true);
+            hi = __builtin_amdgcn_update_dpp(0, hi, 0x118, 0xf, 0xf, true);
+            break;
+    }
+    return __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double lane_value(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane), hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
+}
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_shr(v, 1);
+    v += dpp_shr(v, 2);
+    v += dpp_shr(v, 4);
+    v += dpp_shr(v, 8);
+    return ((lane_value(v, 15) + lane_value(v, 31)) + lane_value(v, 47)) + lane_value(v, 63);
 }
 
 // Per-point inverse Jacobian and weights, in the reference's operation order

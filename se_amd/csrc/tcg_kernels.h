@@ -28,8 +28,10 @@ constexpr int TCG_PRE = 4;  // preloaded partials per thread and value
 
 // Jacobi diagonal inverse and the new direction: shared by both kernels so
 // that the halo values one workgroup computes equal the owner's stored ones.
+// (v_rcp_f32: one instruction, within 1 ulp; the same instruction in every
+// kernel, so every kernel sees the same preconditioner)
 __device__ __forceinline__ float tcg_dinv(float gxN, float sxI, float mxI, float gyN, float syJ, float myJ) {
-    return 1.0f / __fmaf_rn(syJ, gxN, __fmaf_rn(sxI, gyN, mxI * myJ));
+    return __builtin_amdgcn_rcpf(__fmaf_rn(syJ, gxN, __fmaf_rn(sxI, gyN, mxI * myJ)));
 }
 template <typename T>
 __device__ __forceinline__ float tcg_pnew(float r, float dinv, float mu, float beta, float pold) {
@@ -128,7 +130,7 @@ __global__ __launch_bounds__(TCG_NT) void k_tcg_dir_apply(SepParams S, const flo
     c1 = c1 && Ic1 >= 0 && Ic1 < S.nU0;
     Ic0 = c0 ? Ic0 : 0;
     Ic1 = c1 ? Ic1 : 0;
-    float r0[RL], r1[RL], p0[RL], p1[RL], rg[RL], rs[RL], rm[RL];
+    float r0[RL], r1[RL], p0[RL], p1[RL];
     bool rok[RL];
 #pragma unroll
     for (int k = 0; k < RL; k++) {
@@ -144,9 +146,16 @@ __global__ __launch_bounds__(TCG_NT) void k_tcg_dir_apply(SepParams S, const flo
             p0[k] = to_f<T>(p[rowo + Ic0]);
             p1[k] = to_f<T>(p[rowo + Ic1]);
         }
-        rg[k] = S.gy[(size_t)Jc * B + N];
-        rs[k] = S.sy[Jc];
-        rm[k] = S.my[Jc];
+    }
+    // row tables of tile row `lane` (H <= 64 rows)
+    float rgl, rsl, rml;
+    {
+        int J = J0 - N + lane;
+        if (S.per1) J = J < 0 ? J + S.nU1 : (J >= S.nU1 ? J - S.nU1 : J);
+        const int Jc = (lane < H && J >= 0 && J < S.nU1) ? J : 0;
+        rgl = S.gy[(size_t)Jc * B + N];
+        rsl = S.sy[Jc];
+        rml = S.my[Jc];
     }
     const float cg0 = S.gx[(size_t)Ic0 * B + N], cs0 = S.sx[Ic0], cm0 = S.mx[Ic0];
     const float cg1 = S.gx[(size_t)Ic1 * B + N], cs1 = S.sx[Ic1], cm1 = S.mx[Ic1];
@@ -175,14 +184,10 @@ __global__ __launch_bounds__(TCG_NT) void k_tcg_dir_apply(SepParams S, const flo
     const double rho_prev = st->rho_prev, tol2_in = st->tol2;
 
     // ---- tables to LDS, totals, state ----
-#pragma unroll
-    for (int k = 0; k < RL; k++) {
-        const int a = wv + NW * k;
-        if (a < H && lane == 0) {
-            s_rg[a] = rg[k];
-            s_rs[a] = rs[k];
-            s_rm[a] = rm[k];
-        }
+    if (wv == 0 && lane < H) {
+        s_rg[lane] = rgl;
+        s_rs[lane] = rsl;
+        s_rm[lane] = rml;
     }
     s_cg[lane] = cg0, s_cs[lane] = cs0, s_cm[lane] = cm0;
     if (lane < 2 * N) s_cg[64 + lane] = cg1, s_cs[64 + lane] = cs1, s_cm[64 + lane] = cm1;
@@ -305,7 +310,7 @@ __global__ __launch_bounds__(TCG_NT) void k_tcg_update(SepParams S, T* __restric
     constexpr int B = 2 * N + 1, RPT = TCG_RPT;
     const TcgTile t(S);
     const int Ic = t.colok ? t.I : 0;
-    float pv[RPT], rv[RPT], qv[RPT], xv[RPT], rg[RPT], rs[RPT], rm[RPT];
+    float pv[RPT], rv[RPT], qv[RPT], xv[RPT];
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
         const int J = t.row(k);
@@ -315,9 +320,16 @@ __global__ __launch_bounds__(TCG_NT) void k_tcg_update(SepParams S, T* __restric
         rv[k] = r[o];
         qv[k] = q[o];
         xv[k] = x[o];
-        rg[k] = S.gy[(size_t)Jc * B + N];
-        rs[k] = S.sy[Jc];
-        rm[k] = S.my[Jc];
+    }
+    // row tables of the wave's row `lane` (lanes < RPT), read back per row
+    // with readlane
+    float rgl, rsl, rml;
+    {
+        const int J = t.row(t.lane < RPT ? t.lane : 0);
+        const int Jc = J < S.nU1 ? J : 0;
+        rgl = S.gy[(size_t)Jc * B + N];
+        rsl = S.sy[Jc];
+        rml = S.my[Jc];
     }
     const float cg = S.gx[(size_t)Ic * B + N], cs = S.sx[Ic], cm = S.mx[Ic];
     TcgPart<1> pa;
@@ -352,7 +364,9 @@ __global__ __launch_bounds__(TCG_NT) void k_tcg_update(SepParams S, T* __restric
         const int J = t.row(k);
         if (t.colok && J < S.nU1) {
             const size_t o = (size_t)J * S.nU0 + t.I;
-            const float dinv = tcg_dinv(cg, cs, cm, rg[k], rs[k], rm[k]);
+            const float dinv = tcg_dinv(cg, cs, cm, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rgl), k)),
+                                        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rsl), k)),
+                                        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rml), k)));
             const float pn = tcg_pnew<T>(rv[k], dinv, fmu, fb, pv[k]);
             const float xn = __fmaf_rn(fa, pn, xv[k]);
             const float rn = __fmaf_rn(-fa, qv[k], rv[k]);
