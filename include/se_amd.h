@@ -141,8 +141,9 @@ typedef struct {
     int precision;      /* SE_PREC_* (default FP64) */
     int null_space;     /* 1: constant null space (periodic Poisson) */
     int check_every;    /* iterations per host convergence check (default 16) */
-    double inner_rtol;  /* IR inner tolerance (0 = precision default) */
-    int max_outer;      /* IR outer step limit (default 50) */
+    double inner_rtol;  /* mixed solves: reliable update once the recursive residual
+                           fell by this factor since the last one (0 = 0.1) */
+    int max_outer;      /* limit on FP64 reliable updates of a mixed solve (default 1000) */
     double* history;    /* optional host array, maxit+1 recursive relative residuals (FP64 only) */
 } se_solve_opts;
 
@@ -174,6 +175,23 @@ int se_helm_solve(se_op* op, const double* rhs_field, double* sol_field, const s
                   se_solve_info* info);
 int se_helm_solve_dev(se_op* op, const double* d_rhs_field, double* d_sol_field, const se_solve_opts* opts,
                       se_solve_info* info);
+
+/* ---- batched solves ----------------------------------------------------------
+ * Many independent mixed-precision solves (SE_PREC_FP32_IR / SE_PREC_BF16_IR)
+ * at once.  No counterpart in the reference, which solves one HelmholtzEqn at
+ * a time (KSPSolve, src/linsys/HelmholtzEqn.cpp:120-131); each member's result
+ * is bitwise the one se_helm_pcg_dev gives for it alone with the same options.
+ * Members are operators created on the same context (any size, order 2..8 and
+ * boundary conditions); an operator may belong to one batch at a time. */
+typedef struct se_batch se_batch;
+int se_batch_create(se_ctx* ctx, int count, se_op* const* ops, se_batch** out);
+int se_batch_destroy(se_batch* batch);
+int se_batch_size(const se_batch* batch);
+/* d_f[i], d_x[i]: device vectors of member i; infos: count entries (nullable) */
+int se_batch_pcg_dev(se_batch* batch, const double* const* d_f, double* const* d_x, const se_solve_opts* opts,
+                     se_solve_info* infos);
+int se_batch_pcg(se_batch* batch, const double* const* f, double* const* x, const se_solve_opts* opts,
+                 se_solve_info* infos);
 
 /* ---- semi-Lagrangian advection ---------------------------------------------
  * [ref: Advector::Advect / FirstOrder src/base/Advector.cpp:108-120,181-201,

@@ -8,6 +8,7 @@ status.  There is no CPU fallback: without the built library every call fails.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -89,6 +90,11 @@ def lib():
             "se_helm_pcg_dev": (i, [vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
             "se_helm_solve": (i, [vp, _D, _D, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
             "se_helm_solve_dev": (i, [vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
+            "se_batch_create": (i, [vp, i, vp, pvp]),
+            "se_batch_destroy": (i, [vp]),
+            "se_batch_size": (i, [vp]),
+            "se_batch_pcg_dev": (i, [vp, vp, vp, C.POINTER(SolveOpts), vp]),
+            "se_batch_pcg": (i, [vp, vp, vp, C.POINTER(SolveOpts), vp]),
             "se_sl_advect": (i, [vp, _D, i, _D, d, _D, vp]),
             "se_sl_advect_dev": (i, [vp, vp, i, vp, d, vp, vp]),
             "se_sl_advect2": (i, [vp, _D, _D, i, _D, _D, d, _D]),
@@ -117,7 +123,7 @@ def _d(x):
 
 
 def solve_opts(rtol=1e-10, maxit=100000, precision=PREC_FP64, null_space=False, check_every=16,
-               inner_rtol=0.0, max_outer=50, history=None):
+               inner_rtol=0.0, max_outer=1000, history=None):
     o = SolveOpts()
     lib().se_solve_opts_default(C.byref(o))
     o.rtol, o.maxit, o.precision, o.null_space = rtol, maxit, precision, int(bool(null_space))
@@ -133,6 +139,10 @@ class Context:
         h = C.c_void_p()
         check(lib().se_ctx_create(device, C.byref(h)), "se_ctx_create")
         self.h = h
+        self._children = []  # weak references, in creation order
+
+    def _own(self, obj):
+        self._children.append(weakref.ref(obj))
 
     @property
     def stream(self):
@@ -142,7 +152,14 @@ class Context:
         check(lib().se_ctx_synchronize(self.h), "se_ctx_synchronize")
 
     def close(self):
+        """Destroy the context and, first, everything still alive that was
+        created from it (batches, operators, meshes: newest first)."""
         if self.h:
+            for ref in reversed(self._children):
+                obj = ref()
+                if obj is not None:
+                    obj.close()
+            self._children = []
             lib().se_ctx_destroy(self.h)
             self.h = None
 
@@ -158,6 +175,7 @@ class Mesh:
 
     def __init__(self, ctx, nEls, N, lo, hi, periodic, bc=(0, 0, 0, 0)):
         self.ctx = ctx
+        self.h = None
         h = C.c_void_p()
         check(lib().se_mesh_create(ctx.h, np.array(nEls, dtype=np.int32), N, _d(lo), _d(hi),
                                    np.array(periodic, dtype=np.int32), np.array(bc, dtype=np.int32),
@@ -168,6 +186,7 @@ class Mesh:
         self.N, self.nV, self.n, self.nElsTotal, self.size0 = info.N, info.nV, info.n, info.nElsTotal, info.size0
         self.nEls = (info.nEls[0], info.nEls[1])
         self.nVerts = (info.nVerts[0], info.nVerts[1])
+        ctx._own(self)
         self.nl = (self.N + 1) ** 2
         self.lo, self.hi, self.periodic, self.bc = tuple(lo), tuple(hi), tuple(periodic), tuple(bc)
 
@@ -231,9 +250,11 @@ class Helm:
 
     def __init__(self, mesh, c_mass, c_lap, detj_mode=DETJ_LITERAL):
         self.mesh = mesh
+        self.h = None
         h = C.c_void_p()
         check(lib().se_helm_create(mesh.h, c_mass, c_lap, detj_mode, C.byref(h)), "se_helm_create")
         self.h = h
+        mesh.ctx._own(self)
 
     def apply(self, x):
         y = np.zeros(self.mesh.n)
@@ -304,6 +325,52 @@ class Helm:
     def close(self):
         if self.h:
             lib().se_helm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Independent mixed-precision solves of several Helm operators at once."""
+
+    def __init__(self, ctx, helms):
+        self.helms = list(helms)
+        self.h = None
+        arr = (C.c_void_p * max(1, len(self.helms)))(*[h.h.value for h in self.helms])
+        h = C.c_void_p()
+        check(lib().se_batch_create(ctx.h, len(self.helms), arr, C.byref(h)), "se_batch_create")
+        self.h = h
+        ctx._own(self)
+
+    def _infos(self):
+        return (SolveInfo * max(1, len(self.helms)))()
+
+    def pcg(self, fs, precision=PREC_FP32_IR, **kw):
+        """fs: list of host vectors; returns (list of solutions, list of info dicts)."""
+        o = solve_opts(precision=precision, **kw)
+        fs = [_d(f) for f in fs]
+        xs = [np.zeros(h.mesh.n) for h in self.helms]
+        fp = (C.c_void_p * max(1, len(fs)))(*[f.ctypes.data for f in fs])
+        xp = (C.c_void_p * max(1, len(xs)))(*[x.ctypes.data for x in xs])
+        infos = self._infos()
+        check(lib().se_batch_pcg(self.h, fp, xp, C.byref(o), infos), "se_batch_pcg")
+        return xs, [infos[i].as_dict() for i in range(len(self.helms))]
+
+    def pcg_dev(self, f_ptrs, x_ptrs, precision=PREC_FP32_IR, **kw):
+        o = solve_opts(precision=precision, **kw)
+        fp = (C.c_void_p * max(1, len(f_ptrs)))(*f_ptrs)
+        xp = (C.c_void_p * max(1, len(x_ptrs)))(*x_ptrs)
+        infos = self._infos()
+        check(lib().se_batch_pcg_dev(self.h, fp, xp, C.byref(o), infos), "se_batch_pcg_dev")
+        return [infos[i].as_dict() for i in range(len(self.helms))]
+
+    def close(self):
+        if self.h:
+            lib().se_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -19,9 +19,10 @@
 
 namespace se {
 
-// block-sum K values (blockDim BS) and store them to partials[k*gridDim.x + b]
+// block-sum K values (blockDim BS) and store them to partials[k*np + idx]
 template <int K, int BS>
-__device__ __forceinline__ void block_store_partials(const double (&v)[K], double* __restrict__ partials) {
+__device__ __forceinline__ void block_store_partials_at(const double (&v)[K], double* __restrict__ partials, int idx,
+                                                        int np) {
     constexpr int NW = BS / 64;
     __shared__ double s_red[K][NW];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -35,8 +36,13 @@ __device__ __forceinline__ void block_store_partials(const double (&v)[K], doubl
         double s = 0.0;
 #pragma unroll
         for (int w = 0; w < NW; w++) s += s_red[threadIdx.x][w];
-        partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
+        partials[(size_t)threadIdx.x * np + idx] = s;
     }
+}
+// one partial set per workgroup of the grid: partials[k*gridDim.x + blockIdx.x]
+template <int K, int BS>
+__device__ __forceinline__ void block_store_partials(const double (&v)[K], double* __restrict__ partials) {
+    block_store_partials_at<K, BS>(v, partials, blockIdx.x, gridDim.x);
 }
 
 // Every block reduces the np partials of the previous kernel in the same fixed
@@ -67,9 +73,12 @@ __device__ __forceinline__ void reduce_partials(const double* __restrict__ parti
     for (int k = 0; k < K; k++) tot[k] = s_tot[k];
 }
 
+// One tile of TX x TY elements (tile index `tile`, row-major over the element
+// grid).  With DOT, the workgroup's partial of in.out goes to
+// partials[tile] (np = number of tiles).
 template <int N, int TX, int TY, bool DOT>
-__global__ __launch_bounds__(256) void k_apply(OpParams P, const double* __restrict__ in, double* __restrict__ out,
-                                                double* __restrict__ partials, const CgState* __restrict__ st) {
+__device__ __forceinline__ void apply_tile(const OpParams& P, int tile, const double* __restrict__ in,
+                                           double* __restrict__ out, double* __restrict__ partials, int np) {
     constexpr int N1 = N + 1;
     constexpr int NP = N1 * N1;
     constexpr int BW = ((TX + 1) * N + 2) | 1;  // odd row pitch (in doubles): conflict-free column access
@@ -82,8 +91,6 @@ __global__ __launch_bounds__(256) void k_apply(OpParams P, const double* __restr
     constexpr int CS = 4 * N1 + NP;             // metric table stride per element column / row
     constexpr int NG1 = (BH * BWU + 255) / 256;
 
-    if (DOT && __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-
     __shared__ double s_u[BH * BW];
     __shared__ double s_a[NE * NP];  // u_xi, then f_xi     [e][qy][qx]
     __shared__ double s_b[NE * NP];  // u_eta, then f_eta   [e][qy][qx]
@@ -93,8 +100,8 @@ __global__ __launch_bounds__(256) void k_apply(OpParams P, const double* __restr
 
     const int tid = threadIdx.x;
     const int tilesX = (P.nEls0 + TX - 1) / TX;
-    const int ex0 = (blockIdx.x % tilesX) * TX;
-    const int ey0 = (blockIdx.x / tilesX) * TY;
+    const int ex0 = (tile % tilesX) * TX;
+    const int ey0 = (tile / tilesX) * TY;
     const int txo = min(TX, P.nEls0 - ex0), tyo = min(TY, P.nEls1 - ey0);
     auto exw = [&](int c) {  // wrapped element index of tile column c (-1: none)
         int ex = ex0 - 1 + c;
@@ -305,7 +312,34 @@ __global__ __launch_bounds__(256) void k_apply(OpParams P, const double* __restr
 
     if (DOT) {
         double v[1] = {dot};
-        block_store_partials<1, 256>(v, partials);
+        block_store_partials_at<1, 256>(v, partials, tile, np);
+    }
+}
+
+template <int N, int TX, int TY, bool DOT>
+__global__ __launch_bounds__(256) void k_apply(OpParams P, const double* __restrict__ in, double* __restrict__ out,
+                                                double* __restrict__ partials, const CgState* __restrict__ st) {
+    if (DOT && __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    apply_tile<N, TX, TY, DOT>(P, blockIdx.x, in, out, partials, gridDim.x);
+}
+
+// Batched apply: work items (member, tile) over members of one order; the
+// grid loops over the items.  Each member brings its own operator, input and
+// output.
+struct ApplyMember {
+    OpParams P;
+    const double* in;
+    double* out;
+};
+
+template <int N, int TX, int TY>
+__global__ __launch_bounds__(256) void k_apply_batch(const ApplyMember* __restrict__ members,
+                                                      const int2* __restrict__ work, int nwork) {
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const int2 it = work[w];
+        const ApplyMember& a = members[it.x];
+        __syncthreads();  // the previous item's LDS reads are done
+        apply_tile<N, TX, TY, false>(a.P, it.y, a.in, a.out, nullptr, 0);
     }
 }
 

@@ -1,0 +1,504 @@
+// batch.hip -- many independent mixed-precision Helmholtz solves at once.
+//
+// Members (operators of any size, order and boundary conditions, created on
+// one context) are grouped by GLL order.  One CG iteration of a group is two
+// launches over the (member, tile) work items of all its members
+// (k_tcg_update_batch, k_tcg_dir_apply_batch); the k-iteration chains of the
+// groups are captured into one graph on forked streams, so the groups run
+// concurrently.  Each member keeps its own recurrence state, convergence test
+// and FP64 reliable updates: at every check the host reads all states, and
+// the members that are due get their update in batched launches (flush,
+// FP64 operator apply, residual, norm) over just their work items.  A
+// finished member stays stopped: its work items exit at the flag.
+//
+// Launch shape: the work-item kernels loop over their items, so the grid
+// size is free; an autotuner times the candidate grid sizes on the first
+// checks of the first solve (real iterations, nothing discarded) and keeps
+// the fastest for the batch.
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "se_common.h"
+#include "se_op.h"
+#include "tcg_host.h"
+#include "tcg_kernels.h"
+
+using namespace se;
+
+namespace {
+
+int vb_of(int n) { return std::max(1, std::min((n + 4095) / 4096, 256)); }
+
+__global__ __launch_bounds__(1024) void k_batch_sum(int n, const double* __restrict__ f, double sgn,
+                                                    double* __restrict__ part) {
+    double s = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += sgn * f[i];
+    double v[1] = {s};
+    block_store_partials<1, 1024>(v, part);
+}
+
+struct Group {
+    int N = 0;
+    std::vector<int> members;   // batch member indices
+    std::vector<int2> tiles;    // all (local member, tile) items
+    DevBuf d_members, d_tiles;  // TcgMember[], int2[]
+    DevBuf d_amembers;          // ApplyMember[] (FP64 operator of each member)
+    // lists of the members due for a reliable update (rebuilt per check);
+    // the host copies live here until the stream has consumed them
+    DevBuf d_ut, d_uf, d_ua, d_ul, d_rt;
+    std::vector<int2> h_ut, h_uf, h_ua, h_rt;
+    std::vector<int> h_ul;
+    std::vector<TcgMember> h_members;
+    std::vector<ApplyMember> h_amembers;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    int grid = 0;  // work-item kernel grid for the iteration chain
+};
+
+}  // namespace
+
+struct se_batch {
+    se_ctx* ctx = nullptr;
+    std::vector<se_op*> ops;
+    std::vector<Group> groups;
+    DevBuf d_states;
+    CgState* h_states = nullptr;
+    hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_key = -1;
+    int tuned = 0;          // grid factor chosen by the autotuner (0: not yet)
+    double tune_best = 0.0;
+
+    void release() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        for (auto& g : groups) {
+            if (g.stream) (void)hipStreamDestroy(g.stream);
+            if (g.done) (void)hipEventDestroy(g.done);
+            g.stream = nullptr;
+            g.done = nullptr;
+        }
+        if (fork) (void)hipEventDestroy(fork);
+        if (t0) (void)hipEventDestroy(t0);
+        if (t1) (void)hipEventDestroy(t1);
+        fork = t0 = t1 = nullptr;
+        if (h_states) (void)hipHostFree(h_states);
+        h_states = nullptr;
+    }
+};
+
+namespace {
+
+template <class V>
+int upload(DevBuf& d, const std::vector<V>& v, hipStream_t s) {
+    int rc = d.alloc(sizeof(V) * std::max<size_t>(v.size(), 1));
+    if (rc) return rc;
+    if (!v.empty()) SE_HIP(hipMemcpyAsync(d.ptr, v.data(), sizeof(V) * v.size(), hipMemcpyHostToDevice, s));
+    return SE_OK;
+}
+
+#define SE_BATCH_ORDERS(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8)
+
+// ---- launches of one group's work-item kernels ----
+template <typename T>
+int g_update(const Group& g, int nwork, const int2* work, int grid, hipStream_t s) {
+    if (nwork == 0) return SE_OK;
+    const TcgMember* Ms = g.d_members.as<TcgMember>();
+    switch (g.N) {
+#define SE_CASE(NN) \
+    case NN: hipLaunchKernelGGL((k_tcg_update_batch<NN, T>), dim3(grid), dim3(TCG_NT), 0, s, Ms, work, nwork); break;
+        SE_BATCH_ORDERS(SE_CASE)
+#undef SE_CASE
+        default: return fail(SE_ERR_UNSUPPORTED, "batched solve built for orders 2..8");
+    }
+    return SE_OK;
+}
+template <typename T, bool FIRST>
+int g_dir_apply(const Group& g, int nwork, const int2* work, int grid, hipStream_t s) {
+    if (nwork == 0) return SE_OK;
+    const TcgMember* Ms = g.d_members.as<TcgMember>();
+    switch (g.N) {
+#define SE_CASE(NN)                                                                                             \
+    case NN:                                                                                                    \
+        hipLaunchKernelGGL((k_tcg_dir_apply_batch<NN, T, FIRST>), dim3(grid), dim3(TCG_NT), 0, s, Ms, work, nwork); \
+        break;
+        SE_BATCH_ORDERS(SE_CASE)
+#undef SE_CASE
+        default: return fail(SE_ERR_UNSUPPORTED, "batched solve built for orders 2..8");
+    }
+    return SE_OK;
+}
+template <typename T, bool RESID>
+int g_start(const Group& g, int nwork, const int2* work, hipStream_t s) {
+    if (nwork == 0) return SE_OK;
+    const TcgMember* Ms = g.d_members.as<TcgMember>();
+    switch (g.N) {
+#define SE_CASE(NN) \
+    case NN: hipLaunchKernelGGL((k_tcg_start_batch<NN, T, RESID>), dim3(nwork), dim3(TCG_NT), 0, s, Ms, work, nwork); break;
+        SE_BATCH_ORDERS(SE_CASE)
+#undef SE_CASE
+        default: return fail(SE_ERR_UNSUPPORTED, "batched solve built for orders 2..8");
+    }
+    return SE_OK;
+}
+int g_apply64(const Group& g, int nwork, const int2* work, hipStream_t s) {
+    if (nwork == 0) return SE_OK;
+    const ApplyMember* As = g.d_amembers.as<ApplyMember>();
+    switch (g.N) {
+#define SE_CASE(NN) \
+    case NN: hipLaunchKernelGGL((k_apply_batch<NN, 2, 2>), dim3(nwork), dim3(256), 0, s, As, work, nwork); break;
+        SE_BATCH_ORDERS(SE_CASE)
+#undef SE_CASE
+        default: return fail(SE_ERR_UNSUPPORTED, "batched solve built for orders 2..8");
+    }
+    return SE_OK;
+}
+
+int apply_tiles22(const se_mesh* m) { return ((m->nEls[0] + 1) / 2) * ((m->nEls[1] + 1) / 2); }
+
+// one graph: the k-iteration chains of every group on their own streams
+template <typename T>
+int chains(se_batch* b, int k) {
+    hipStream_t s = b->ctx->stream;
+    const int key = k * 4 + (sizeof(T) == 2 ? 1 : 0) + 2 * (b->tuned > 0 ? 1 : 0);
+    if (!b->graph || b->graph_key != key) {
+        if (b->graph) (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+        hipGraph_t gr = nullptr;
+        SE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = SE_OK;
+        hipError_t e = hipEventRecord(b->fork, s);
+        for (auto& g : b->groups) {
+            if (e == hipSuccess) e = hipStreamWaitEvent(g.stream, b->fork, 0);
+            const int nw = (int)g.tiles.size();
+            for (int i = 0; i < k && !rc && e == hipSuccess; i++) {
+                rc = g_update<T>(g, nw, g.d_tiles.as<int2>(), g.grid, g.stream);
+                if (!rc) rc = g_dir_apply<T, false>(g, nw, g.d_tiles.as<int2>(), g.grid, g.stream);
+            }
+            if (e == hipSuccess) e = hipEventRecord(g.done, g.stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s, g.done, 0);
+        }
+        hipError_t e2 = hipStreamEndCapture(s, &gr);
+        if (rc || e != hipSuccess || e2 != hipSuccess) {
+            if (gr) (void)hipGraphDestroy(gr);
+            if (rc) return rc;
+            return fail(SE_ERR_HIP, std::string("batch graph capture: ") +
+                                        hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        e = hipGraphInstantiate(&b->graph, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (e != hipSuccess) {
+            b->graph = nullptr;
+            return fail(SE_ERR_HIP, std::string("batch graph instantiate: ") + hipGetErrorString(e));
+        }
+        b->graph_key = key;
+    }
+    SE_HIP(hipGraphLaunch(b->graph, s));
+    return SE_OK;
+}
+
+void set_grids(se_batch* b, int factor) {
+    for (auto& g : b->groups) {
+        const int nw = std::max(1, (int)g.tiles.size());
+        g.grid = factor <= 0 ? nw : std::min(nw, factor * b->ctx->cu_count);
+    }
+}
+
+template <typename T>
+int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const se_solve_opts& o,
+                se_solve_info* infos) {
+    se_ctx* c = b->ctx;
+    hipStream_t s = c->stream;
+    const int count = (int)b->ops.size();
+    const int null = o.null_space ? 1 : 0;
+    const int k = std::max(1, o.check_every);
+    std::vector<TcgControl> ctl(count);
+    for (auto& t : ctl) t.configure(o);
+
+    // descriptors, work items, states
+    for (auto& g : b->groups) {
+        std::vector<TcgMember>& Ms = g.h_members;
+        std::vector<ApplyMember>& As = g.h_amembers;
+        Ms.clear();
+        As.clear();
+        for (int i : g.members) {
+            se_op* op = b->ops[i];
+            TcgMember M = tcg_member(op, d_f[i], o.rtol, null, vb_of(op->mesh->n));
+            M.st = b->d_states.as<CgState>() + i;
+            Ms.push_back(M);
+            ApplyMember A;
+            A.P = op_params(op, (double)op->sign);
+            A.in = op->x.as<double>();
+            A.out = op->q.as<double>();
+            As.push_back(A);
+        }
+        int rc = upload(g.d_members, Ms, s);
+        if (!rc) rc = upload(g.d_amembers, As, s);
+        if (rc) return rc;
+    }
+    for (int i = 0; i < count; i++) {
+        CgState h{};
+        h.maxit = ctl[i].maxit;
+        h.null_space = null;
+        h.n = b->ops[i]->mesh->n;
+        h.status = CG_RUNNING;
+        b->h_states[i] = h;
+    }
+    auto push = [&]() -> int {
+        SE_HIP(hipMemcpyAsync(b->d_states.ptr, b->h_states, sizeof(CgState) * count, hipMemcpyHostToDevice, s));
+        return SE_OK;
+    };
+    auto pull = [&]() -> int {
+        SE_HIP(hipMemcpyAsync(b->h_states, b->d_states.ptr, sizeof(CgState) * count, hipMemcpyDeviceToHost, s));
+        SE_HIP(hipStreamSynchronize(s));
+        return SE_OK;
+    };
+
+    SE_HIP(hipEventRecord(c->ev0, s));
+    int rc = push();
+    if (rc) return rc;
+    if (null) {
+        for (int i = 0; i < count; i++) {
+            se_op* op = b->ops[i];
+            const int n = op->mesh->n;
+            hipLaunchKernelGGL(k_batch_sum, dim3(vb_of(n)), dim3(1024), 0, s, n, d_f[i], (double)op->sign,
+                               op->sep_part.as<double>() + 7 * (size_t)tcg_tiles(op->mesh));
+        }
+    }
+    for (auto& g : b->groups) {
+        const int nw = (int)g.tiles.size();
+        rc = g_start<T, false>(g, nw, g.d_tiles.as<int2>(), s);
+        if (!rc) rc = g_dir_apply<T, true>(g, nw, g.d_tiles.as<int2>(), nw, s);
+        if (rc) return rc;
+    }
+    SE_HIP(hipGetLastError());
+    rc = pull();
+    if (rc) return rc;
+    int running = 0;
+    for (int i = 0; i < count; i++) {
+        ctl[i].started(b->h_states[i]);
+        running += ctl[i].status == CG_RUNNING;
+    }
+
+    // autotuner candidates: grid = all items, or 8, 4, 2 workgroups per CU.
+    // Phase 2j captures candidate j and runs it once, phase 2j+1 times a
+    // replay; every launch is followed by the regular check, so the check
+    // cadence (and with it every result) is that of the single solve.
+    static const int cand[4] = {0, 8, 4, 2};
+    int phase = b->tuned ? -1 : 0;
+    set_grids(b, b->tuned > 0 ? b->tuned - 1 : 0);
+    while (running > 0) {
+        if (phase >= 0 && phase % 2 == 0) {
+            set_grids(b, cand[phase / 2]);
+            b->graph_key = -1;
+        }
+        if (phase >= 0 && phase % 2 == 1) SE_HIP(hipEventRecord(b->t0, s));
+        rc = chains<T>(b, k);
+        if (rc) return rc;
+        if (phase >= 0 && phase % 2 == 1) {
+            SE_HIP(hipEventRecord(b->t1, s));
+            SE_HIP(hipEventSynchronize(b->t1));
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, b->t0, b->t1);
+            if (phase == 1 || ms < b->tune_best) {
+                b->tune_best = ms;
+                b->tuned = cand[phase / 2] + 1;
+            }
+        }
+        if (phase >= 0 && ++phase == 8) {
+            phase = -1;
+            set_grids(b, b->tuned - 1);
+            b->graph_key = -1;
+        }
+        rc = pull();
+        if (rc) return rc;
+
+        // members due for a reliable update
+        std::vector<int> due;
+        for (int i = 0; i < count; i++)
+            if (ctl[i].status == CG_RUNNING && ctl[i].due(b->h_states[i], k)) {
+                ctl[i].before_update(b->h_states[i]);
+                due.push_back(i);
+            }
+        if (due.empty()) continue;
+        std::vector<char> is_due(count, 0);
+        for (int i : due) is_due[i] = 1;
+        for (auto& g : b->groups) {
+            std::vector<int2>&ut = g.h_ut, &uf = g.h_uf, &ua = g.h_ua;
+            std::vector<int>& ul = g.h_ul;
+            ut.clear(), uf.clear(), ua.clear(), ul.clear();
+            for (int j = 0; j < (int)g.members.size(); j++) {
+                const int i = g.members[j];
+                if (!is_due[i]) continue;
+                const se_mesh* m = b->ops[i]->mesh;
+                ul.push_back(j);
+                for (int t = 0; t < tcg_tiles(m); t++) ut.push_back(make_int2(j, t));
+                for (int t = 0; t < (m->n + TCG_FLUSH - 1) / TCG_FLUSH; t++) uf.push_back(make_int2(j, t));
+                for (int t = 0; t < apply_tiles22(m); t++) ua.push_back(make_int2(j, t));
+            }
+            if (ul.empty()) continue;
+            rc = upload(g.d_ut, ut, s);
+            if (!rc) rc = upload(g.d_uf, uf, s);
+            if (!rc) rc = upload(g.d_ua, ua, s);
+            if (!rc) rc = upload(g.d_ul, ul, s);
+            if (rc) return rc;
+            const TcgMember* Ms = g.d_members.as<TcgMember>();
+            hipLaunchKernelGGL(k_tcg_flush_batch, dim3((int)uf.size()), dim3(TCG_NT), 0, s, Ms, g.d_uf.as<int2>(),
+                               (int)uf.size());
+            rc = g_apply64(g, (int)ua.size(), g.d_ua.as<int2>(), s);
+            if (!rc) rc = g_start<T, true>(g, (int)ut.size(), g.d_ut.as<int2>(), s);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_tcg_true_batch, dim3((int)ul.size()), dim3(TCG_NT), 0, s, Ms, g.d_ul.as<int>(),
+                               (int)ul.size());
+        }
+        SE_HIP(hipGetLastError());
+        rc = pull();
+        if (rc) return rc;
+        std::vector<char> restart(count, 0);
+        for (int i : due) {
+            CgState hh = b->h_states[i];
+            if (ctl[i].after_update(hh) == CG_RUNNING) {
+                restart[i] = 1;
+            } else {
+                hh.stop = 1;  // finished: its work items stay idle
+                running--;
+            }
+            b->h_states[i] = hh;
+        }
+        rc = push();
+        if (rc) return rc;
+        for (auto& g : b->groups) {
+            std::vector<int2>& rt = g.h_rt;
+            rt.clear();
+            for (int j = 0; j < (int)g.members.size(); j++) {
+                const int i = g.members[j];
+                if (!restart[i]) continue;
+                for (int t = 0; t < tcg_tiles(b->ops[i]->mesh); t++) rt.push_back(make_int2(j, t));
+            }
+            if (rt.empty()) continue;
+            rc = upload(g.d_rt, rt, s);
+            if (!rc) rc = g_dir_apply<T, false>(g, (int)rt.size(), g.d_rt.as<int2>(), (int)rt.size(), s);
+            if (rc) return rc;
+        }
+    }
+    for (int i = 0; i < count; i++) {
+        se_op* op = b->ops[i];
+        if (d_x[i] != op->x.as<double>())
+            SE_HIP(hipMemcpyAsync(d_x[i], op->x.ptr, sizeof(double) * op->mesh->n, hipMemcpyDeviceToDevice, s));
+    }
+    SE_HIP(hipEventRecord(c->ev1, s));
+    SE_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    for (int i = 0; i < count; i++)
+        if (infos) ctl[i].fill(infos[i], b->h_states[i].iter, ms * 1e-3);
+    return SE_OK;
+}
+
+}  // namespace
+
+extern "C" int se_batch_create(se_ctx* ctx, int count, se_op* const* ops, se_batch** out) {
+    SE_CHECK(ctx && out && (count == 0 || ops), SE_ERR_ARG, "se_batch_create: NULL argument");
+    SE_CHECK(count >= 0, SE_ERR_ARG, "se_batch_create: negative count");
+    SE_HIP(hipSetDevice(ctx->device));
+    for (int i = 0; i < count; i++) {
+        SE_CHECK(ops[i] != nullptr, SE_ERR_ARG, "se_batch_create: NULL member");
+        SE_CHECK(ops[i]->mesh->ctx == ctx, SE_ERR_ARG, "se_batch_create: member created on another context");
+        SE_CHECK(ops[i]->definite, SE_ERR_UNSUPPORTED, "se_batch_create: member operator is not definite");
+        SE_CHECK(ops[i]->mesh->N >= 2 && ops[i]->mesh->N <= 8, SE_ERR_UNSUPPORTED,
+                 "se_batch_create: orders 2..8");
+        for (int j = 0; j < i; j++)
+            SE_CHECK(ops[j] != ops[i], SE_ERR_ARG, "se_batch_create: an operator appears twice");
+    }
+    se_batch* b = new se_batch;
+    b->ctx = ctx;
+    b->ops.assign(ops, ops + count);
+    int rc = SE_OK;
+    for (int i = 0; i < count && !rc; i++) rc = tcg_prepare(ops[i]);
+    if (!rc) rc = b->d_states.alloc(sizeof(CgState) * std::max(count, 1));
+    if (!rc && hipHostMalloc((void**)&b->h_states, sizeof(CgState) * std::max(count, 1), hipHostMallocDefault) !=
+                   hipSuccess)
+        rc = fail(SE_ERR_NOMEM, "se_batch_create: pinned state buffer");
+    if (!rc && (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreate(&b->t0) != hipSuccess || hipEventCreate(&b->t1) != hipSuccess))
+        rc = fail(SE_ERR_HIP, "se_batch_create: events");
+    for (int N = 2; N <= 8 && !rc; N++) {
+        Group g;
+        g.N = N;
+        for (int i = 0; i < count; i++)
+            if (ops[i]->mesh->N == N) g.members.push_back(i);
+        if (g.members.empty()) continue;
+        for (int j = 0; j < (int)g.members.size(); j++)
+            for (int t = 0; t < tcg_tiles(ops[g.members[j]]->mesh); t++) g.tiles.push_back(make_int2(j, t));
+        if (hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&g.done, hipEventDisableTiming) != hipSuccess)
+            rc = fail(SE_ERR_HIP, "se_batch_create: group stream");
+        if (!rc) rc = upload(g.d_tiles, g.tiles, ctx->stream);
+        b->groups.push_back(std::move(g));
+    }
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = fail(SE_ERR_HIP, std::string("se_batch_create: ") + hipGetErrorString(e));
+    }
+    if (rc) {
+        b->release();
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return SE_OK;
+}
+
+extern "C" int se_batch_destroy(se_batch* b) {
+    if (!b) return SE_OK;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    b->release();
+    delete b;
+    return SE_OK;
+}
+
+extern "C" int se_batch_size(const se_batch* b) { return b ? (int)b->ops.size() : 0; }
+
+extern "C" int se_batch_pcg_dev(se_batch* b, const double* const* d_f, double* const* d_x, const se_solve_opts* opts,
+                                se_solve_info* infos) {
+    SE_CHECK(b && (b->ops.empty() || (d_f && d_x)), SE_ERR_ARG, "se_batch_pcg_dev: NULL argument");
+    se_solve_opts o;
+    se_solve_opts_default(&o);
+    if (opts) o = *opts;
+    SE_CHECK(o.precision == SE_PREC_FP32_IR || o.precision == SE_PREC_BF16_IR, SE_ERR_UNSUPPORTED,
+             "se_batch_pcg_dev: the batched solver is mixed precision (SE_PREC_FP32_IR or SE_PREC_BF16_IR)");
+    for (size_t i = 0; i < b->ops.size(); i++)
+        SE_CHECK(d_f[i] && d_x[i], SE_ERR_ARG, "se_batch_pcg_dev: NULL member vector");
+    SE_HIP(hipSetDevice(b->ctx->device));
+    if (b->ops.empty()) return SE_OK;
+    if (o.precision == SE_PREC_FP32_IR) return batch_solve<float>(b, d_f, d_x, o, infos);
+    return batch_solve<__hip_bfloat16>(b, d_f, d_x, o, infos);
+}
+
+extern "C" int se_batch_pcg(se_batch* b, const double* const* f, double* const* x, const se_solve_opts* opts,
+                            se_solve_info* infos) {
+    SE_CHECK(b && (b->ops.empty() || (f && x)), SE_ERR_ARG, "se_batch_pcg: NULL argument");
+    const int count = (int)b->ops.size();
+    SE_HIP(hipSetDevice(b->ctx->device));
+    hipStream_t s = b->ctx->stream;
+    std::vector<const double*> df(count);
+    std::vector<double*> dx(count);
+    for (int i = 0; i < count; i++) {
+        se_op* op = b->ops[i];
+        SE_CHECK(f[i] && x[i], SE_ERR_ARG, "se_batch_pcg: NULL member vector");
+        int rc = op_ensure_work(op);
+        if (rc) return rc;
+        SE_HIP(hipMemcpyAsync(op->f.ptr, f[i], sizeof(double) * op->mesh->n, hipMemcpyHostToDevice, s));
+        df[i] = op->f.as<double>();
+        dx[i] = op->x.as<double>();
+    }
+    int rc = se_batch_pcg_dev(b, df.data(), dx.data(), opts, infos);
+    if (rc) return rc;
+    for (int i = 0; i < count; i++)
+        SE_HIP(hipMemcpyAsync(x[i], b->ops[i]->x.ptr, sizeof(double) * b->ops[i]->mesh->n, hipMemcpyDeviceToHost, s));
+    SE_HIP(hipStreamSynchronize(s));
+    return SE_OK;
+}

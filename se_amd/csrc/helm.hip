@@ -443,7 +443,7 @@ extern "C" void se_solve_opts_default(se_solve_opts* o) {
     o->null_space = 0;
     o->check_every = 16;
     o->inner_rtol = 0.0;
-    o->max_outer = 50;
+    o->max_outer = 1000;
     o->history = nullptr;
 }
 
@@ -699,8 +699,11 @@ int pcg_fp64(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, 
         if (rc) return rc;
     }
     double* dh = o.history ? d_hist.as<double>() : nullptr;
+    if (!op->h_state) SE_HIP(hipHostMalloc(&op->h_state, sizeof(CgState), hipHostMallocDefault));
+    CgState* hs = (CgState*)op->h_state;
     SE_HIP(hipEventRecord(c->ev0, s));
-    SE_HIP(hipMemcpyAsync(st, &h, sizeof(CgState), hipMemcpyHostToDevice, s));
+    *hs = h;  // uploads go through the pinned host copy
+    SE_HIP(hipMemcpyAsync(st, hs, sizeof(CgState), hipMemcpyHostToDevice, s));
     if (h.null_space)
         hipLaunchKernelGGL(k_sum, dim3(vb), dim3(kVB), 0, s, n, d_f, (double)op->sign, part + L.s);
     hipLaunchKernelGGL(k_cg_start, dim3(vb), dim3(kVB), 0, s, n, (double)op->sign, d_f, op->dinv_s.as<double>(),
@@ -708,8 +711,6 @@ int pcg_fp64(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, 
     hipLaunchKernelGGL(k_cg_dir, dim3(vb), dim3(kVB), 0, s, (int)DIR_FIRST, n, o.rtol, op->dinv_s.as<double>(),
                        op->r.as<double>(), op->p.as<double>(), part + L.b, vb, dh, st);
     SE_HIP(hipGetLastError());
-    if (!op->h_state) SE_HIP(hipHostMalloc(&op->h_state, sizeof(CgState), hipHostMallocDefault));
-    CgState* hs = (CgState*)op->h_state;
     SE_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
     SE_HIP(hipStreamSynchronize(s));
     const int k = std::max(1, o.check_every);
@@ -735,8 +736,9 @@ int pcg_fp64(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, 
             break;
         }
         // continue from the true residual (now in r)
-        int flags[2] = {0, CG_RUNNING};
-        SE_HIP(hipMemcpyAsync(&st->stop, flags, sizeof(flags), hipMemcpyHostToDevice, s));
+        hs->stop = 0;
+        hs->status = CG_RUNNING;
+        SE_HIP(hipMemcpyAsync(&st->stop, &hs->stop, 2 * sizeof(int), hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_cg_restart, dim3(vb), dim3(kVB), 0, s, n, op->dinv_s.as<double>(), op->r.as<double>(),
                            part + L.b, st);
         hipLaunchKernelGGL(k_cg_dir, dim3(vb), dim3(kVB), 0, s, (int)DIR_RESTART, n, 0.0, op->dinv_s.as<double>(),

@@ -17,6 +17,7 @@
 
 #include "se_common.h"
 #include "se_op.h"
+#include "tcg_host.h"
 #include "tcg_kernels.h"
 
 using namespace se;
@@ -185,52 +186,61 @@ int launch_sep(const se_op* op, const T* in, TQ* out, double* part, const CgStat
 // ---------------------------------------------------------------------------
 // tile CG
 // ---------------------------------------------------------------------------
-// partial-sum areas in op->sep_part (doubles): A p.q (tiles), B r stats
-// (4 x tiles), T FP64 residual stats (2 x tiles), S mean of f (VB)
-struct MixLayout {
-    size_t a, b, t, s, total;
-};
-MixLayout mix_layout(const se_mesh* m) {
-    const size_t tiles = (size_t)sep_tiles(m);
-    MixLayout L;
-    L.a = 0;
-    L.b = L.a + tiles;
-    L.t = L.b + 4 * tiles;
-    L.s = L.t + 2 * tiles;
-    L.total = L.s + 256;
-    return L;
+size_t part_total(const se_mesh* m) { return 7 * (size_t)sep_tiles(m) + 256; }
+
+}  // namespace
+
+namespace se {
+// tables and work buffers of the mixed-precision solve
+int tcg_prepare(se_op* op) {
+    const se_mesh* m = op->mesh;
+    const size_t nb = (size_t)std::max(m->n, 1);
+    int rc = op_ensure_work(op);
+    if (!rc) rc = sep_build(op);
+    if (!rc) rc = op->sep_part.alloc(sizeof(double) * part_total(m));
+    if (!rc) rc = op->sep_state.alloc(sizeof(CgState));
+    // p sized for 4-byte elements whatever its type, so switching precisions
+    // keeps the addresses a captured graph holds
+    if (!rc) rc = op->lo_r.alloc(sizeof(float) * nb);
+    if (!rc) rc = op->lo_p.alloc(sizeof(float) * nb);
+    if (!rc) rc = op->lo_q.alloc(sizeof(float) * nb);
+    if (!rc) rc = op->lo_x.alloc(sizeof(float) * nb);
+    return rc;
 }
 
-struct TcgArgs {
-    SepParams S;
-    int tiles;
-    double* part;
-    MixLayout L;
-    CgState* st;
-    hipStream_t s;
-};
-
-TcgArgs tcg_args(se_op* op) {
-    TcgArgs a;
-    a.S = sep_params(op);
-    a.tiles = sep_tiles(op->mesh);
-    a.part = op->sep_part.as<double>();
-    a.L = mix_layout(op->mesh);
-    a.st = op->sep_state.as<CgState>();
-    a.s = op->mesh->ctx->stream;
-    return a;
+TcgMember tcg_member(se_op* op, const double* d_f, double rtol, int null_space, int np_s) {
+    TcgMember M;
+    M.S = sep_params(op);
+    M.r = op->lo_r.as<float>();
+    M.q = op->lo_q.as<float>();
+    M.xlo = op->lo_x.as<float>();
+    M.p = op->lo_p.ptr;
+    M.b = op->b.as<double>();
+    M.x = op->x.as<double>();
+    M.q64 = op->q.as<double>();
+    M.f = d_f;
+    M.part = op->sep_part.as<double>();
+    M.st = op->sep_state.as<CgState>();
+    M.tiles = sep_tiles(op->mesh);
+    M.np_s = np_s;
+    M.null_space = null_space;
+    M.n = op->mesh->n;
+    M.sgn = (double)op->sign;
+    M.rtol = rtol;
+    return M;
 }
+
+int tcg_tiles(const se_mesh* m) { return sep_tiles(m); }
+}  // namespace se
+
+namespace {
 
 template <typename T, bool FIRST>
-int launch_dir_apply(se_op* op, double rtol) {
-    const TcgArgs a = tcg_args(op);
+int launch_dir_apply(se_op* op, const TcgMember& M) {
+    hipStream_t s = op->mesh->ctx->stream;
     switch (op->mesh->N) {
 #define SE_CASE(NN)                                                                                                   \
-    case NN:                                                                                                          \
-        hipLaunchKernelGGL((k_tcg_dir_apply<NN, T, FIRST>), dim3(a.tiles), dim3(TCG_NT), 0, a.s, a.S,                 \
-                           op->lo_r.as<float>(), op->lo_p.as<T>(), op->lo_q.as<float>(), a.part + a.L.b,              \
-                           a.part + a.L.t, a.tiles, a.part + a.L.a, a.st, rtol);                                      \
-        break;
+    case NN: hipLaunchKernelGGL((k_tcg_dir_apply<NN, T, FIRST>), dim3(M.tiles), dim3(TCG_NT), 0, s, M); break;
         SE_SEP_ORDERS(SE_CASE)
 #undef SE_CASE
         default: return fail(SE_ERR_UNSUPPORTED, "separable operator built for orders 2..8");
@@ -239,15 +249,11 @@ int launch_dir_apply(se_op* op, double rtol) {
 }
 
 template <typename T>
-int launch_update(se_op* op) {
-    const TcgArgs a = tcg_args(op);
+int launch_update(se_op* op, const TcgMember& M) {
+    hipStream_t s = op->mesh->ctx->stream;
     switch (op->mesh->N) {
-#define SE_CASE(NN)                                                                                                   \
-    case NN:                                                                                                          \
-        hipLaunchKernelGGL((k_tcg_update<NN, T>), dim3(a.tiles), dim3(TCG_NT), 0, a.s, a.S, op->lo_p.as<T>(),         \
-                           op->lo_r.as<float>(), op->lo_q.as<float>(), op->lo_x.as<float>(), a.part + a.L.a, a.tiles, \
-                           a.part + a.L.b, a.st);                                                                     \
-        break;
+#define SE_CASE(NN) \
+    case NN: hipLaunchKernelGGL((k_tcg_update<NN, T>), dim3(M.tiles), dim3(TCG_NT), 0, s, M); break;
         SE_SEP_ORDERS(SE_CASE)
 #undef SE_CASE
         default: return fail(SE_ERR_UNSUPPORTED, "separable operator built for orders 2..8");
@@ -257,17 +263,11 @@ int launch_update(se_op* op) {
 
 // RESID = false: start (b, x, x_lo, p, r from f); true: r = b - q64
 template <typename T, bool RESID>
-int launch_start(se_op* op, const double* d_f, int null_space, int np_s) {
-    const TcgArgs a = tcg_args(op);
-    const double sgn = (double)op->sign;
+int launch_start(se_op* op, const TcgMember& M) {
+    hipStream_t s = op->mesh->ctx->stream;
     switch (op->mesh->N) {
-#define SE_CASE(NN)                                                                                                   \
-    case NN:                                                                                                          \
-        hipLaunchKernelGGL((k_tcg_start<NN, T, RESID>), dim3(a.tiles), dim3(TCG_NT), 0, a.s, a.S, d_f, sgn,           \
-                           a.part + a.L.s, np_s, op->b.as<double>(), op->q.as<double>(), op->x.as<double>(),          \
-                           op->lo_x.as<float>(), op->lo_p.as<T>(), op->lo_r.as<float>(), a.part + a.L.b,              \
-                           a.part + a.L.t, null_space);                                                               \
-        break;
+#define SE_CASE(NN) \
+    case NN: hipLaunchKernelGGL((k_tcg_start<NN, T, RESID>), dim3(M.tiles), dim3(TCG_NT), 0, s, M); break;
         SE_SEP_ORDERS(SE_CASE)
 #undef SE_CASE
         default: return fail(SE_ERR_UNSUPPORTED, "separable operator built for orders 2..8");
@@ -276,15 +276,15 @@ int launch_start(se_op* op, const double* d_f, int null_space, int np_s) {
 }
 
 template <typename T>
-int tcg_iteration(se_op* op) {
-    int rc = launch_update<T>(op);
-    if (!rc) rc = launch_dir_apply<T, false>(op, 0.0);
+int tcg_iteration(se_op* op, const TcgMember& M) {
+    int rc = launch_update<T>(op, M);
+    if (!rc) rc = launch_dir_apply<T, false>(op, M);
     return rc;
 }
 
 // k iterations, captured once per (k, T) into a graph and replayed
 template <typename T>
-int tcg_batch(se_op* op, int k) {
+int tcg_batch(se_op* op, const TcgMember& M, int k) {
     hipStream_t s = op->mesh->ctx->stream;
     static const bool use_graph = getenv("SE_NO_GRAPH") == nullptr;
     const int key = k * 2 + (sizeof(T) == 2 ? 1 : 0);
@@ -297,7 +297,7 @@ int tcg_batch(se_op* op, int k) {
             hipGraph_t g = nullptr;
             SE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             int rc = SE_OK;
-            for (int i = 0; i < k && !rc; i++) rc = tcg_iteration<T>(op);
+            for (int i = 0; i < k && !rc; i++) rc = tcg_iteration<T>(op, M);
             hipError_t e = hipStreamEndCapture(s, &g);
             if (rc) {
                 if (g) (void)hipGraphDestroy(g);
@@ -316,7 +316,7 @@ int tcg_batch(se_op* op, int k) {
         return SE_OK;
     }
     for (int i = 0; i < k; i++) {
-        int rc = tcg_iteration<T>(op);
+        int rc = tcg_iteration<T>(op, M);
         if (rc) return rc;
     }
     return SE_OK;
@@ -324,16 +324,14 @@ int tcg_batch(se_op* op, int k) {
 
 // x += x_lo; r = b - A x (FP64); r_lo = r; st->tt = ||r||^2 (projected)
 template <typename T>
-int reliable_update(se_op* op) {
-    const se_mesh* m = op->mesh;
-    const TcgArgs a = tcg_args(op);
-    const int n = m->n;
-    hipLaunchKernelGGL(k_tcg_flush, dim3(std::max(1, std::min((n + 255) / 256, 4096))), dim3(256), 0, a.s, n,
-                       op->x.as<double>(), op->lo_x.as<float>());
+int reliable_update(se_op* op, const TcgMember& M) {
+    hipStream_t s = op->mesh->ctx->stream;
+    const int n = op->mesh->n;
+    hipLaunchKernelGGL(k_tcg_flush, dim3(std::max(1, std::min((n + 4095) / 4096, 2048))), dim3(TCG_NT), 0, s, M);
     int rc = op_apply_dev(op, op->x.as<double>(), op->q.as<double>(), (double)op->sign, false);
-    if (!rc) rc = launch_start<T, true>(op, nullptr, 0, 0);
+    if (!rc) rc = launch_start<T, true>(op, M);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_tcg_true, dim3(1), dim3(TCG_NT), 0, a.s, a.part + a.L.t, a.tiles, a.st);
+    hipLaunchKernelGGL(k_tcg_true, dim3(1), dim3(TCG_NT), 0, s, M);
     SE_HIP(hipGetLastError());
     return SE_OK;
 }
@@ -345,98 +343,49 @@ int tcg_solve(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o,
     hipStream_t s = c->stream;
     const int n = m->n, vb = vblocks(n);
     SE_CHECK(op->definite, SE_ERR_UNSUPPORTED, "operator diagonal changes sign: not a definite system");
-    int rc = op_ensure_work(op);
-    if (!rc) rc = sep_build(op);
-    const MixLayout L = mix_layout(m);
-    const size_t nb = (size_t)std::max(n, 1);
-    if (!rc) rc = op->sep_part.alloc(sizeof(double) * L.total);
-    if (!rc) rc = op->sep_state.alloc(sizeof(CgState));
-    // p sized for 4-byte elements whatever T is, so switching precisions keeps
-    // the addresses a captured graph holds
-    if (!rc) rc = op->lo_r.alloc(sizeof(float) * nb);
-    if (!rc) rc = op->lo_p.alloc(sizeof(float) * nb);
-    if (!rc) rc = op->lo_q.alloc(sizeof(float) * nb);
-    if (!rc) rc = op->lo_x.alloc(sizeof(float) * nb);
+    int rc = tcg_prepare(op);
     if (rc) return rc;
-    double* part = op->sep_part.as<double>();
-    CgState* st = op->sep_state.as<CgState>();
     const int null = o.null_space ? 1 : 0;
-    const double delta = o.inner_rtol > 0.0 ? std::min(o.inner_rtol, 0.5) : 0.1;
+    const TcgMember M = tcg_member(op, d_f, o.rtol, null, vb);
+    TcgControl ctl;
+    ctl.configure(o);
     const int k = std::max(1, o.check_every);
-    const int maxit = std::max(o.maxit, 0);
     if (!op->h_state) SE_HIP(hipHostMalloc(&op->h_state, sizeof(CgState), hipHostMallocDefault));
     CgState* hs = (CgState*)op->h_state;
+    CgState* st = M.st;
+    auto pull = [&]() -> int {
+        SE_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        SE_HIP(hipStreamSynchronize(s));
+        return SE_OK;
+    };
 
     SE_HIP(hipEventRecord(c->ev0, s));
-    CgState h0{};
-    h0.maxit = maxit;
-    h0.null_space = null;
-    h0.n = n;
-    h0.status = CG_RUNNING;
-    SE_HIP(hipMemcpyAsync(st, &h0, sizeof(CgState), hipMemcpyHostToDevice, s));
-    if (null) hipLaunchKernelGGL(k_ir_sum, dim3(vb), dim3(kVB), 0, s, n, d_f, (double)op->sign, part + L.s);
-    rc = launch_start<T, false>(op, d_f, null, vb);
-    if (!rc) rc = launch_dir_apply<T, true>(op, o.rtol);
+    // state uploads go through the pinned host copy
+    *hs = CgState{};
+    hs->maxit = ctl.maxit;
+    hs->null_space = null;
+    hs->n = n;
+    hs->status = CG_RUNNING;
+    SE_HIP(hipMemcpyAsync(st, hs, sizeof(CgState), hipMemcpyHostToDevice, s));
+    if (null) hipLaunchKernelGGL(k_ir_sum, dim3(vb), dim3(kVB), 0, s, n, d_f, (double)op->sign, M.part + 7 * (size_t)M.tiles);
+    rc = launch_start<T, false>(op, M);
+    if (!rc) rc = launch_dir_apply<T, true>(op, M);
+    if (!rc) rc = pull();
     if (rc) return rc;
     SE_HIP(hipGetLastError());
-    SE_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
-    SE_HIP(hipStreamSynchronize(s));
-    const double bb = hs->bb;
-    double tt = bb;
-    int status = hs->stop ? hs->status : CG_RUNNING;  // 0 (b = 0) or 3 (non-finite)
-    int updates = 0, since = 0, stall = 0;
-    double rr_ref = bb, best = bb;
-    while (status == CG_RUNNING) {
-        rc = tcg_batch<T>(op, k);
+    ctl.started(*hs);
+    while (ctl.status == CG_RUNNING) {
+        rc = tcg_batch<T>(op, M, k);
+        if (!rc) rc = pull();
         if (rc) return rc;
-        SE_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
-        SE_HIP(hipStreamSynchronize(s));
-        since += k;
-        const bool due = hs->stop || hs->rr <= delta * delta * rr_ref || since >= 256;
-        if (!due) continue;
-        const int inner_status = hs->stop ? hs->status : CG_RUNNING;
-        rc = reliable_update<T>(op);
+        if (!ctl.due(*hs, k)) continue;
+        ctl.before_update(*hs);
+        rc = reliable_update<T>(op, M);
+        if (!rc) rc = pull();
         if (rc) return rc;
-        SE_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
-        SE_HIP(hipStreamSynchronize(s));
-        updates++;
-        since = 0;
-        tt = hs->tt;
-        if (!(tt - tt == 0.0)) {
-            status = 3;
-            break;
-        }
-        if (tt <= hs->tol2) {
-            status = 0;
-            break;
-        }
-        if (inner_status == 2 || inner_status == 3) {
-            status = inner_status;
-            break;
-        }
-        if (hs->iter >= maxit || inner_status == 1 || updates >= o.max_outer) {
-            status = 1;
-            break;
-        }
-        // no progress of the true residual over several updates: the inner
-        // precision cannot resolve this system
-        if (tt < 0.5 * best) {
-            best = tt;
-            stall = 0;
-        } else if (++stall >= 4) {
-            status = 4;
-            break;
-        }
-        // the recursion claimed convergence but the true residual disagrees:
-        // ask the recursion for more from now on
-        const double tol2 = inner_status == CG_PENDING ? 0.25 * hs->tol2 : hs->tol2;
-        rr_ref = tt;
-        CgState hh = *hs;
-        hh.stop = 0;
-        hh.status = CG_RUNNING;
-        hh.tol2 = tol2;
-        SE_HIP(hipMemcpyAsync(st, &hh, sizeof(CgState), hipMemcpyHostToDevice, s));
-        rc = launch_dir_apply<T, false>(op, 0.0);
+        if (ctl.after_update(*hs) != CG_RUNNING) break;
+        SE_HIP(hipMemcpyAsync(st, hs, sizeof(CgState), hipMemcpyHostToDevice, s));
+        rc = launch_dir_apply<T, false>(op, M);
         if (rc) return rc;
     }
     if (d_x != op->x.as<double>())
@@ -445,11 +394,7 @@ int tcg_solve(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o,
     SE_HIP(hipEventSynchronize(c->ev1));
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
-    info.status = status;
-    info.iters = hs->iter;
-    info.outer_iters = updates;
-    info.true_rel_residual = bb > 0.0 ? std::sqrt(std::max(tt, 0.0) / bb) : 0.0;
-    info.seconds = ms * 1e-3;
+    ctl.fill(info, hs->iter, ms * 1e-3);
     return SE_OK;
 }
 

@@ -48,3 +48,9 @@ int op_ensure_work(se_op* op);
 int pcg_fp64(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, se_solve_info& info);
 int pcg_mixed(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, se_solve_info& info);
 }  // namespace se
+
+// mixed-precision solve setup (mixed.hip), used by the batched solver
+namespace se {
+int tcg_prepare(se_op* op);
+int tcg_tiles(const se_mesh* m);
+}  // namespace se

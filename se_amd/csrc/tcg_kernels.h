@@ -21,6 +21,8 @@
 #pragma once
 #include "sep_kernels.h"
 
+struct se_op;
+
 namespace se {
 
 constexpr int TCG_NT = 256, TCG_TY = 32, TCG_RPT = TCG_TY / (TCG_NT / 64);
@@ -84,10 +86,10 @@ struct TcgPart {
 struct TcgTile {
     int I0, J0, I, lane, wv;
     bool colok;
-    __device__ __forceinline__ TcgTile(const SepParams& S) {
+    __device__ __forceinline__ TcgTile(const SepParams& S, int tile) {
         const int tilesX = (S.nU0 + SEP_TX - 1) / SEP_TX;
-        I0 = (blockIdx.x % tilesX) * SEP_TX;
-        J0 = (blockIdx.x / tilesX) * TCG_TY;
+        I0 = (tile % tilesX) * SEP_TX;
+        J0 = (tile / tilesX) * TCG_TY;
         lane = threadIdx.x & 63;
         wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         I = I0 + lane;
@@ -96,16 +98,40 @@ struct TcgTile {
     __device__ __forceinline__ int row(int k) const { return J0 + wv * TCG_RPT + k; }
 };
 
+// One solve: the separable tables, the vectors and the partial-sum areas
+// (part: A p.q [tiles] | B r stats [4 tiles] | T FP64 residual stats
+// [2 tiles] | S mean of f [256]).
+struct TcgMember {
+    SepParams S;
+    float *r, *q, *xlo;
+    void* p;  // T
+    double *b, *x, *q64;
+    const double* f;
+    double* part;
+    CgState* st;
+    int tiles, np_s, null_space, n;
+    double sgn, rtol;
+    __device__ __forceinline__ double* part_a() const { return part; }
+    __device__ __forceinline__ double* part_b() const { return part + tiles; }
+    __device__ __forceinline__ double* part_t() const { return part + 5 * (size_t)tiles; }
+    __device__ __forceinline__ double* part_s() const { return part + 7 * (size_t)tiles; }
+};
+
 // ---------------------------------------------------------------------------
 // k_tcg_dir_apply
 // ---------------------------------------------------------------------------
 template <int N, typename T, bool FIRST>
-__global__ __launch_bounds__(TCG_NT) void k_tcg_dir_apply(SepParams S, const float* __restrict__ r,
-                                                          const T* __restrict__ p, float* __restrict__ q,
-                                                          const double* __restrict__ part_b,
-                                                          const double* __restrict__ part_t, int np,
-                                                          double* __restrict__ part_a, CgState* __restrict__ st,
-                                                          double rtol) {
+__device__ __forceinline__ void tcg_dir_apply_tile(const TcgMember& M, int tile) {
+    const SepParams& S = M.S;
+    const float* __restrict__ r = M.r;
+    const T* __restrict__ p = (const T*)M.p;
+    float* __restrict__ q = M.q;
+    const int np = M.tiles;
+    const double* __restrict__ part_b = M.part_b();
+    const double* __restrict__ part_t = M.part_t();
+    double* __restrict__ part_a = M.part_a();
+    CgState* __restrict__ st = M.st;
+    const double rtol = M.rtol;
     constexpr int B = 2 * N + 1;
     constexpr int TY = TCG_TY, NT = TCG_NT, NW = NT / 64, RPT = TCG_RPT;
     constexpr int W = SEP_TX + 2 * N, H = TY + 2 * N, WP = W + 1;
@@ -116,7 +142,7 @@ __global__ __launch_bounds__(TCG_NT) void k_tcg_dir_apply(SepParams S, const flo
     __shared__ float s_cg[W], s_cs[W], s_cm[W];  // per tile column: gx centre, sx, mx
     __shared__ float s_rg[H], s_rs[H], s_rm[H];  // per tile row: gy centre, sy, my
 
-    const TcgTile t(S);
+    const TcgTile t(S, tile);
     const int lane = t.lane, wv = t.wv, I0 = t.I0, J0 = t.J0;
 
     // ---- every global load first ----
@@ -203,7 +229,7 @@ __global__ __launch_bounds__(TCG_NT) void k_tcg_dir_apply(SepParams S, const flo
     const bool finite = (rr - rr == 0.0) && (tb[1] - tb[1] == 0.0);
     const double mu = null_space ? tb[2] / st->n : 0.0;
     const double rho_new = tb[1] - mu * tb[3];
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const bool lead = tile == 0 && threadIdx.x == 0;  // the member's first tile writes its state
     double beta = 0.0;
     if (FIRST) {
         double tt[2];
@@ -296,19 +322,25 @@ __global__ __launch_bounds__(TCG_NT) void k_tcg_dir_apply(SepParams S, const flo
         }
     }
     double v[1] = {dot};
-    block_store_partials<1, NT>(v, part_a);
+    block_store_partials_at<1, NT>(v, part_a, tile, np);
 }
 
 // ---------------------------------------------------------------------------
 // k_tcg_update
 // ---------------------------------------------------------------------------
 template <int N, typename T>
-__global__ __launch_bounds__(TCG_NT) void k_tcg_update(SepParams S, T* __restrict__ p, float* __restrict__ r,
-                                                       const float* __restrict__ q, float* __restrict__ x,
-                                                       const double* __restrict__ part_a, int np,
-                                                       double* __restrict__ part_b, CgState* __restrict__ st) {
+__device__ __forceinline__ void tcg_update_tile(const TcgMember& M, int tile) {
+    const SepParams& S = M.S;
+    T* __restrict__ p = (T*)M.p;
+    float* __restrict__ r = M.r;
+    const float* __restrict__ q = M.q;
+    float* __restrict__ x = M.xlo;
+    const int np = M.tiles;
+    const double* __restrict__ part_a = M.part_a();
+    double* __restrict__ part_b = M.part_b();
+    CgState* __restrict__ st = M.st;
     constexpr int B = 2 * N + 1, RPT = TCG_RPT;
-    const TcgTile t(S);
+    const TcgTile t(S, tile);
     const int Ic = t.colok ? t.I : 0;
     float pv[RPT], rv[RPT], qv[RPT], xv[RPT];
 #pragma unroll
@@ -341,7 +373,7 @@ __global__ __launch_bounds__(TCG_NT) void k_tcg_update(SepParams S, T* __restric
     pa.reduce(part_a, np, ta);
     if (s_stop) return;
     const double pq = ta[0];
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const bool lead = tile == 0 && threadIdx.x == 0;  // the member's first tile writes its state
     if (!(pq > 0.0)) {
         if (lead) {
             st->pq = pq;
@@ -381,7 +413,7 @@ __global__ __launch_bounds__(TCG_NT) void k_tcg_update(SepParams S, T* __restric
         }
     }
     double v[4] = {srr, srz, ssz, ssr};
-    block_store_partials<4, TCG_NT>(v, part_b);
+    block_store_partials_at<4, TCG_NT>(v, part_b, tile, np);
 }
 
 // ---------------------------------------------------------------------------
@@ -391,15 +423,23 @@ __global__ __launch_bounds__(TCG_NT) void k_tcg_update(SepParams S, T* __restric
 // and of b in FP64 (part_t: sum b^2, sum b).  RESID: r = b - q64 instead, with
 // x, x_lo, p left alone.
 template <int N, typename T, bool RESID>
-__global__ __launch_bounds__(TCG_NT) void k_tcg_start(SepParams S, const double* __restrict__ f, double sgn,
-                                                      const double* __restrict__ part_s, int np_s,
-                                                      double* __restrict__ b, const double* __restrict__ q64,
-                                                      double* __restrict__ x, float* __restrict__ xlo,
-                                                      T* __restrict__ p, float* __restrict__ r,
-                                                      double* __restrict__ part_b, double* __restrict__ part_t,
-                                                      int null_space) {
+__device__ __forceinline__ void tcg_start_tile(const TcgMember& M, int tile) {
+    const SepParams& S = M.S;
+    const double* __restrict__ f = M.f;
+    const double sgn = M.sgn;
+    const double* __restrict__ part_s = M.part_s();
+    const int np_s = M.np_s;
+    double* __restrict__ b = M.b;
+    const double* __restrict__ q64 = M.q64;
+    double* __restrict__ x = M.x;
+    float* __restrict__ xlo = M.xlo;
+    T* __restrict__ p = (T*)M.p;
+    float* __restrict__ r = M.r;
+    double* __restrict__ part_b = M.part_b();
+    double* __restrict__ part_t = M.part_t();
+    const int null_space = M.null_space;
     constexpr int B = 2 * N + 1, RPT = TCG_RPT;
-    const TcgTile t(S);
+    const TcgTile t(S, tile);
     const int Ic = t.colok ? t.I : 0;
     const size_t n = (size_t)S.nU0 * S.nU1;
     double mu0 = 0.0;
@@ -438,25 +478,99 @@ __global__ __launch_bounds__(TCG_NT) void k_tcg_start(SepParams S, const double*
         }
     }
     double v[4] = {srr, srz, ssz, ssr};
-    block_store_partials<4, TCG_NT>(v, part_b);
+    block_store_partials_at<4, TCG_NT>(v, part_b, tile, M.tiles);
     double w[2] = {sbb, sb};
-    block_store_partials<2, TCG_NT>(w, part_t);
+    block_store_partials_at<2, TCG_NT>(w, part_t, tile, M.tiles);
 }
 
-// x += x_lo (FP64); x_lo = 0
-__global__ void k_tcg_flush(int n, double* __restrict__ x, float* __restrict__ xlo) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        x[i] += (double)xlo[i];
-        xlo[i] = 0.0f;
+// x += x_lo (FP64); x_lo = 0, on elements [i0, i1) of a member
+__device__ __forceinline__ void tcg_flush_range(const TcgMember& M, int i0, int i1, int stride) {
+    for (int i = i0 + threadIdx.x; i < i1; i += stride) {
+        M.x[i] += (double)M.xlo[i];
+        M.xlo[i] = 0.0f;
     }
 }
 
-// single workgroup: st->tt = ||r - mean||^2 (or ||r||^2) from part_t
-__global__ __launch_bounds__(TCG_NT) void k_tcg_true(const double* __restrict__ part_t, int np,
-                                                     CgState* __restrict__ st) {
+// st->tt = ||r - mean||^2 (or ||r||^2) from part_t (one workgroup)
+__device__ __forceinline__ void tcg_true_member(const TcgMember& M) {
     double t[2];
-    reduce_partials<2, TCG_NT>(part_t, np, t);
-    if (threadIdx.x == 0) st->tt = st->null_space ? t[0] - t[1] * (t[1] / st->n) : t[0];
+    reduce_partials<2, TCG_NT>(M.part_t(), M.tiles, t);
+    if (threadIdx.x == 0) M.st->tt = M.st->null_space ? t[0] - t[1] * (t[1] / M.st->n) : t[0];
 }
+
+// ---------------------------------------------------------------------------
+// kernels: one member (descriptor as kernel argument, one tile per workgroup)
+// and batches (descriptor array, (member, tile) work items, the grid loops
+// over the items)
+// ---------------------------------------------------------------------------
+template <int N, typename T, bool FIRST>
+__global__ __launch_bounds__(TCG_NT) void k_tcg_dir_apply(TcgMember M) {
+    tcg_dir_apply_tile<N, T, FIRST>(M, blockIdx.x);
+}
+template <int N, typename T>
+__global__ __launch_bounds__(TCG_NT) void k_tcg_update(TcgMember M) {
+    tcg_update_tile<N, T>(M, blockIdx.x);
+}
+template <int N, typename T, bool RESID>
+__global__ __launch_bounds__(TCG_NT) void k_tcg_start(TcgMember M) {
+    tcg_start_tile<N, T, RESID>(M, blockIdx.x);
+}
+static __global__ __launch_bounds__(TCG_NT) void k_tcg_flush(TcgMember M) {
+    const int n = M.n;
+    const int per = (n + gridDim.x - 1) / gridDim.x;
+    const int i0 = blockIdx.x * per;
+    tcg_flush_range(M, i0, min(n, i0 + per), TCG_NT);
+}
+static __global__ __launch_bounds__(TCG_NT) void k_tcg_true(TcgMember M) { tcg_true_member(M); }
+
+template <int N, typename T, bool FIRST>
+__global__ __launch_bounds__(TCG_NT) void k_tcg_dir_apply_batch(const TcgMember* __restrict__ Ms,
+                                                                const int2* __restrict__ work, int nwork) {
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const int2 it = work[w];
+        __syncthreads();
+        tcg_dir_apply_tile<N, T, FIRST>(Ms[it.x], it.y);
+    }
+}
+template <int N, typename T>
+__global__ __launch_bounds__(TCG_NT) void k_tcg_update_batch(const TcgMember* __restrict__ Ms,
+                                                             const int2* __restrict__ work, int nwork) {
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const int2 it = work[w];
+        __syncthreads();
+        tcg_update_tile<N, T>(Ms[it.x], it.y);
+    }
+}
+template <int N, typename T, bool RESID>
+__global__ __launch_bounds__(TCG_NT) void k_tcg_start_batch(const TcgMember* __restrict__ Ms,
+                                                            const int2* __restrict__ work, int nwork) {
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const int2 it = work[w];
+        __syncthreads();
+        tcg_start_tile<N, T, RESID>(Ms[it.x], it.y);
+    }
+}
+// work items (member, chunk of TCG_FLUSH elements)
+constexpr int TCG_FLUSH = 4096;
+static __global__ __launch_bounds__(TCG_NT) void k_tcg_flush_batch(const TcgMember* __restrict__ Ms,
+                                                            const int2* __restrict__ work, int nwork) {
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const int2 it = work[w];
+        const TcgMember& M = Ms[it.x];
+        const int i0 = it.y * TCG_FLUSH;
+        tcg_flush_range(M, i0, min(M.n, i0 + TCG_FLUSH), TCG_NT);
+    }
+}
+// work items: member indices
+static __global__ __launch_bounds__(TCG_NT) void k_tcg_true_batch(const TcgMember* __restrict__ Ms,
+                                                           const int* __restrict__ list, int count) {
+    for (int w = blockIdx.x; w < count; w += gridDim.x) {
+        __syncthreads();
+        tcg_true_member(Ms[list[w]]);
+    }
+}
+
+// descriptor of an operator's mixed-precision solve (host, mixed.hip)
+TcgMember tcg_member(se_op* op, const double* d_f, double rtol, int null_space, int np_s);
 
 }  // namespace se
