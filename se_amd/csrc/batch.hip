@@ -115,14 +115,14 @@ int g_update(const Group& g, int nwork, const int2* work, int grid, hipStream_t 
     }
     return SE_OK;
 }
-template <typename T, bool FIRST>
+template <typename T, int MODE>
 int g_dir_apply(const Group& g, int nwork, const int2* work, int grid, hipStream_t s) {
     if (nwork == 0) return SE_OK;
     const TcgMember* Ms = g.d_members.as<TcgMember>();
     switch (g.N) {
 #define SE_CASE(NN)                                                                                             \
     case NN:                                                                                                    \
-        hipLaunchKernelGGL((k_tcg_dir_apply_batch<NN, T, FIRST>), dim3(grid), dim3(TCG_NT), 0, s, Ms, work, nwork); \
+        hipLaunchKernelGGL((k_tcg_dir_apply_batch<NN, T, MODE>), dim3(grid), dim3(TCG_NT), 0, s, Ms, work, nwork); \
         break;
         SE_BATCH_ORDERS(SE_CASE)
 #undef SE_CASE
@@ -175,7 +175,7 @@ int chains(se_batch* b, int k) {
             const int nw = (int)g.tiles.size();
             for (int i = 0; i < k && !rc && e == hipSuccess; i++) {
                 rc = g_update<T>(g, nw, g.d_tiles.as<int2>(), g.grid, g.stream);
-                if (!rc) rc = g_dir_apply<T, false>(g, nw, g.d_tiles.as<int2>(), g.grid, g.stream);
+                if (!rc) rc = g_dir_apply<T, TCG_NORMAL>(g, nw, g.d_tiles.as<int2>(), g.grid, g.stream);
             }
             if (e == hipSuccess) e = hipEventRecord(g.done, g.stream);
             if (e == hipSuccess) e = hipStreamWaitEvent(s, g.done, 0);
@@ -270,7 +270,7 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
     for (auto& g : b->groups) {
         const int nw = (int)g.tiles.size();
         rc = g_start<T, false>(g, nw, g.d_tiles.as<int2>(), s);
-        if (!rc) rc = g_dir_apply<T, true>(g, nw, g.d_tiles.as<int2>(), nw, s);
+        if (!rc) rc = g_dir_apply<T, TCG_FIRST>(g, nw, g.d_tiles.as<int2>(), nw, s);
         if (rc) return rc;
     }
     SE_HIP(hipGetLastError());
@@ -356,11 +356,12 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
         SE_HIP(hipGetLastError());
         rc = pull();
         if (rc) return rc;
+        // 1: continue the direction, 2: restart it
         std::vector<char> restart(count, 0);
         for (int i : due) {
             CgState hh = b->h_states[i];
             if (ctl[i].after_update(hh) == CG_RUNNING) {
-                restart[i] = 1;
+                restart[i] = ctl[i].reset_dir ? 2 : 1;
             } else {
                 hh.stop = 1;  // finished: its work items stay idle
                 running--;
@@ -370,16 +371,24 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
         rc = push();
         if (rc) return rc;
         for (auto& g : b->groups) {
+            // items of continuing members first, then of restarting ones
             std::vector<int2>& rt = g.h_rt;
             rt.clear();
-            for (int j = 0; j < (int)g.members.size(); j++) {
-                const int i = g.members[j];
-                if (!restart[i]) continue;
-                for (int t = 0; t < tcg_tiles(b->ops[i]->mesh); t++) rt.push_back(make_int2(j, t));
+            int ncont = 0;
+            for (int pass = 1; pass <= 2; pass++) {
+                for (int j = 0; j < (int)g.members.size(); j++) {
+                    const int i = g.members[j];
+                    if (restart[i] != pass) continue;
+                    for (int t = 0; t < tcg_tiles(b->ops[i]->mesh); t++) rt.push_back(make_int2(j, t));
+                }
+                if (pass == 1) ncont = (int)rt.size();
             }
             if (rt.empty()) continue;
             rc = upload(g.d_rt, rt, s);
-            if (!rc) rc = g_dir_apply<T, false>(g, (int)rt.size(), g.d_rt.as<int2>(), (int)rt.size(), s);
+            const int2* w = g.d_rt.as<int2>();
+            const int nres = (int)rt.size() - ncont;
+            if (!rc) rc = g_dir_apply<T, TCG_NORMAL>(g, ncont, w, ncont, s);
+            if (!rc) rc = g_dir_apply<T, TCG_RESET>(g, nres, w + ncont, nres, s);
             if (rc) return rc;
         }
     }

@@ -235,12 +235,12 @@ int tcg_tiles(const se_mesh* m) { return sep_tiles(m); }
 
 namespace {
 
-template <typename T, bool FIRST>
+template <typename T, int MODE>
 int launch_dir_apply(se_op* op, const TcgMember& M) {
     hipStream_t s = op->mesh->ctx->stream;
     switch (op->mesh->N) {
 #define SE_CASE(NN)                                                                                                   \
-    case NN: hipLaunchKernelGGL((k_tcg_dir_apply<NN, T, FIRST>), dim3(M.tiles), dim3(TCG_NT), 0, s, M); break;
+    case NN: hipLaunchKernelGGL((k_tcg_dir_apply<NN, T, MODE>), dim3(M.tiles), dim3(TCG_NT), 0, s, M); break;
         SE_SEP_ORDERS(SE_CASE)
 #undef SE_CASE
         default: return fail(SE_ERR_UNSUPPORTED, "separable operator built for orders 2..8");
@@ -278,7 +278,7 @@ int launch_start(se_op* op, const TcgMember& M) {
 template <typename T>
 int tcg_iteration(se_op* op, const TcgMember& M) {
     int rc = launch_update<T>(op, M);
-    if (!rc) rc = launch_dir_apply<T, false>(op, M);
+    if (!rc) rc = launch_dir_apply<T, TCG_NORMAL>(op, M);
     return rc;
 }
 
@@ -369,7 +369,7 @@ int tcg_solve(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o,
     SE_HIP(hipMemcpyAsync(st, hs, sizeof(CgState), hipMemcpyHostToDevice, s));
     if (null) hipLaunchKernelGGL(k_ir_sum, dim3(vb), dim3(kVB), 0, s, n, d_f, (double)op->sign, M.part + 7 * (size_t)M.tiles);
     rc = launch_start<T, false>(op, M);
-    if (!rc) rc = launch_dir_apply<T, true>(op, M);
+    if (!rc) rc = launch_dir_apply<T, TCG_FIRST>(op, M);
     if (!rc) rc = pull();
     if (rc) return rc;
     SE_HIP(hipGetLastError());
@@ -385,7 +385,7 @@ int tcg_solve(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o,
         if (rc) return rc;
         if (ctl.after_update(*hs) != CG_RUNNING) break;
         SE_HIP(hipMemcpyAsync(st, hs, sizeof(CgState), hipMemcpyHostToDevice, s));
-        rc = launch_dir_apply<T, false>(op, M);
+        rc = ctl.reset_dir ? launch_dir_apply<T, TCG_RESET>(op, M) : launch_dir_apply<T, TCG_NORMAL>(op, M);
         if (rc) return rc;
     }
     if (d_x != op->x.as<double>())

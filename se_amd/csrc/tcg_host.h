@@ -16,7 +16,8 @@ struct TcgControl {
     int maxit = 100000, max_outer = 50, window = 256;
     // progress
     double bb = 0.0, tt = 0.0, rr_ref = 0.0, best = 0.0, rr_rec = 0.0;
-    int status = CG_RUNNING, updates = 0, since = 0, flat = 0, mismatch = 0, inner_status = CG_RUNNING;
+    int status = CG_RUNNING, updates = 0, since = 0, flat = 0, inner_status = CG_RUNNING;
+    bool reset_dir = false;  // restart the direction after this update
 
     void configure(const se_solve_opts& o) {
         const double delta = o.inner_rtol > 0.0 ? std::min(o.inner_rtol, 0.5) : 0.1;
@@ -28,7 +29,7 @@ struct TcgControl {
     void started(const CgState& h) {
         bb = tt = rr_ref = best = h.bb;
         status = h.stop ? h.status : CG_RUNNING;  // 0 (b = 0) or 3 (non-finite)
-        updates = since = flat = mismatch = 0;
+        updates = since = flat = 0;
     }
     // after k more iterations: is a reliable update due?
     bool due(const CgState& h, int k) {
@@ -51,20 +52,17 @@ struct TcgControl {
         if (inner_status == 2 || inner_status == 3) return status = inner_status;
         if (h.iter >= maxit || inner_status == 1 || updates >= max_outer) return status = 1;
         // The inner precision cannot resolve this system when the true
-        // residual repeatedly disagrees with the recursion (by more than 10x
-        // in norm) or stops improving altogether.  A slowly converging
-        // solve fails neither test: its recursion and true residual agree.
+        // residual stops improving altogether over several updates.
         if (tt < best) {
             best = tt;
             flat = 0;
         } else if (++flat >= 6) {
             return status = 4;
         }
-        if (tt > 100.0 * rr_rec) {
-            if (++mismatch >= 3) return status = 4;
-        } else {
-            mismatch = 0;
-        }
+        // A residual that moved far from the recursion's (more than 2x in
+        // norm) is no longer orthogonal to the old direction: restart the
+        // direction from it.  Otherwise the recursion continues unchanged.
+        reset_dir = tt > 4.0 * rr_rec;
         // the recursion claimed convergence but the true residual disagrees:
         // ask the recursion for more from now on
         if (inner_status == CG_PENDING) h.tol2 *= 0.25;

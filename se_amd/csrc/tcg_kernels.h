@@ -28,6 +28,12 @@ namespace se {
 constexpr int TCG_NT = 256, TCG_TY = 32, TCG_RPT = TCG_TY / (TCG_NT / 64);
 constexpr int TCG_PRE = 4;  // preloaded partials per thread and value
 
+// direction kernel modes: FIRST starts a solve (p = z, sets the reference
+// norms), NORMAL continues (p = z + beta p), RESET continues the solve with
+// p = z (after a reliable update that moved the residual far from the
+// recursion's, the old direction is no longer conjugate to anything useful)
+enum { TCG_FIRST = 0, TCG_NORMAL = 1, TCG_RESET = 2 };
+
 // Jacobi diagonal inverse and the new direction: shared by both kernels so
 // that the halo values one workgroup computes equal the owner's stored ones.
 // (v_rcp_f32: one instruction, within 1 ulp; the same instruction in every
@@ -120,8 +126,9 @@ struct TcgMember {
 // ---------------------------------------------------------------------------
 // k_tcg_dir_apply
 // ---------------------------------------------------------------------------
-template <int N, typename T, bool FIRST>
+template <int N, typename T, int MODE>
 __device__ __forceinline__ void tcg_dir_apply_tile(const TcgMember& M, int tile) {
+    constexpr bool FIRST = MODE == TCG_FIRST, NOP = MODE != TCG_NORMAL;  // NOP: the old p is not used
     const SepParams& S = M.S;
     const float* __restrict__ r = M.r;
     const T* __restrict__ p = (const T*)M.p;
@@ -168,7 +175,7 @@ __device__ __forceinline__ void tcg_dir_apply_tile(const TcgMember& M, int tile)
         const size_t rowo = (size_t)Jc * S.nU0;
         r0[k] = r[rowo + Ic0];
         r1[k] = r[rowo + Ic1];
-        if (!FIRST) {
+        if (!NOP) {
             p0[k] = to_f<T>(p[rowo + Ic0]);
             p1[k] = to_f<T>(p[rowo + Ic1]);
         }
@@ -264,12 +271,12 @@ __device__ __forceinline__ void tcg_dir_apply_tile(const TcgMember& M, int tile)
                 st->status = 1;
             } else {
                 st->rho = rho_new;
-                st->beta = rho_new / rho_prev;
+                st->beta = NOP ? 0.0 : rho_new / rho_prev;
                 st->mu = mu;
             }
         }
         if (!finite || conv || over) return;
-        beta = rho_new / rho_prev;
+        beta = NOP ? 0.0 : rho_new / rho_prev;
     }
     const float fmu = (float)mu, fb = (float)beta;
 
@@ -279,12 +286,12 @@ __device__ __forceinline__ void tcg_dir_apply_tile(const TcgMember& M, int tile)
         const int a = wv + NW * k;
         if (a < H) {
             const float d0 = tcg_dinv(s_cg[lane], s_cs[lane], s_cm[lane], s_rg[a], s_rs[a], s_rm[a]);
-            const float v0 = tcg_pnew<T>(r0[k], d0, fmu, fb, FIRST ? 0.0f : p0[k]);
+            const float v0 = tcg_pnew<T>(r0[k], d0, fmu, fb, NOP ? 0.0f : p0[k]);
             s_u[a * WP + lane] = (rok[k] && c0) ? v0 : 0.0f;
             if (lane < 2 * N) {
                 const float d1 =
                     tcg_dinv(s_cg[64 + lane], s_cs[64 + lane], s_cm[64 + lane], s_rg[a], s_rs[a], s_rm[a]);
-                const float v1 = tcg_pnew<T>(r1[k], d1, fmu, fb, FIRST ? 0.0f : p1[k]);
+                const float v1 = tcg_pnew<T>(r1[k], d1, fmu, fb, NOP ? 0.0f : p1[k]);
                 s_u[a * WP + 64 + lane] = (rok[k] && c1) ? v1 : 0.0f;
             }
         }
@@ -503,9 +510,9 @@ __device__ __forceinline__ void tcg_true_member(const TcgMember& M) {
 // and batches (descriptor array, (member, tile) work items, the grid loops
 // over the items)
 // ---------------------------------------------------------------------------
-template <int N, typename T, bool FIRST>
+template <int N, typename T, int MODE>
 __global__ __launch_bounds__(TCG_NT) void k_tcg_dir_apply(TcgMember M) {
-    tcg_dir_apply_tile<N, T, FIRST>(M, blockIdx.x);
+    tcg_dir_apply_tile<N, T, MODE>(M, blockIdx.x);
 }
 template <int N, typename T>
 __global__ __launch_bounds__(TCG_NT) void k_tcg_update(TcgMember M) {
@@ -523,13 +530,13 @@ static __global__ __launch_bounds__(TCG_NT) void k_tcg_flush(TcgMember M) {
 }
 static __global__ __launch_bounds__(TCG_NT) void k_tcg_true(TcgMember M) { tcg_true_member(M); }
 
-template <int N, typename T, bool FIRST>
+template <int N, typename T, int MODE>
 __global__ __launch_bounds__(TCG_NT) void k_tcg_dir_apply_batch(const TcgMember* __restrict__ Ms,
                                                                 const int2* __restrict__ work, int nwork) {
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
         const int2 it = work[w];
         __syncthreads();
-        tcg_dir_apply_tile<N, T, FIRST>(Ms[it.x], it.y);
+        tcg_dir_apply_tile<N, T, MODE>(Ms[it.x], it.y);
     }
 }
 template <int N, typename T>
