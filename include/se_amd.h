@@ -163,6 +163,13 @@ void se_solve_opts_default(se_solve_opts* opts);
  * (SE_PREC_BF16_IR), arithmetic and output are FP32.  sign is +-1 so that
  * sign*A has a positive diagonal.  Host arrays of n values. */
 int se_helm_apply_inner(se_op* op, int precision, const float* x, float* y, int* sign);
+/* Measurement hook (no counterpart in the reference): after a mixed-precision
+ * solve of op, times `reps` back-to-back launches of the inner iteration's
+ * direction+operator kernel and update kernel with HIP events on the context
+ * stream; outputs microseconds and algorithmic bytes per launch.  The
+ * operator's mixed solver state is left as the probe's. */
+int se_helm_probe(se_op* op, int precision, int reps, double* us_dir, double* us_upd, double* bytes_dir,
+                  double* bytes_upd);
 /* Jacobi-PCG on A x = f, x0 = 0  [ref: KSPSolve with -ksp_type cg -pc_type
  *  jacobi, src/linsys/HelmholtzEqn.cpp:120-131] (sign-normalised: a
  *  negative-definite A is solved as -A x = -f) */
@@ -192,6 +199,15 @@ int se_batch_pcg_dev(se_batch* batch, const double* const* d_f, double* const* d
                      se_solve_info* infos);
 int se_batch_pcg(se_batch* batch, const double* const* f, double* const* x, const se_solve_opts* opts,
                  se_solve_info* infos);
+/* Measurement hook (no counterpart in the reference): after a solve of this
+ * batch, times `reps` back-to-back launches of each group's direction+operator
+ * kernel and update kernel over all members, with HIP events on the context
+ * stream.  Per order group (at most max_groups): the order, microseconds per
+ * launch and algorithmic bytes per launch of each kernel.  Returns the number
+ * of groups (>= 0) or a negative SE_ERR_* code; the members' solver states are
+ * left as the probe's (the next solve starts afresh). */
+int se_batch_probe(se_batch* batch, int precision, int reps, int max_groups, int* orders, double* us_dir,
+                   double* us_upd, double* bytes_dir, double* bytes_upd);
 
 /* ---- semi-Lagrangian advection ---------------------------------------------
  * [ref: Advector::Advect / FirstOrder src/base/Advector.cpp:108-120,181-201,

@@ -95,6 +95,8 @@ def lib():
             "se_batch_size": (i, [vp]),
             "se_batch_pcg_dev": (i, [vp, vp, vp, C.POINTER(SolveOpts), vp]),
             "se_batch_pcg": (i, [vp, vp, vp, C.POINTER(SolveOpts), vp]),
+            "se_batch_probe": (i, [vp, i, i, i, _I, _D, _D, _D, _D]),
+            "se_helm_probe": (i, [vp, i, i, C.POINTER(d), C.POINTER(d), C.POINTER(d), C.POINTER(d)]),
             "se_sl_advect": (i, [vp, _D, i, _D, d, _D, vp]),
             "se_sl_advect_dev": (i, [vp, vp, i, vp, d, vp, vp]),
             "se_sl_advect2": (i, [vp, _D, _D, i, _D, _D, d, _D]),
@@ -269,6 +271,12 @@ class Helm:
         check(lib().se_helm_apply_inner(self.h, precision, xf, y, C.byref(sg)), "se_helm_apply_inner")
         return y, sg.value
 
+    def probe(self, precision=PREC_FP32_IR, reps=50):
+        """After a mixed solve: dict(us_dir, us_upd, bytes_dir, bytes_upd) of the inner kernels."""
+        v = [C.c_double() for _ in range(4)]
+        check(lib().se_helm_probe(self.h, precision, reps, *[C.byref(x) for x in v]), "se_helm_probe")
+        return dict(us_dir=v[0].value, us_upd=v[1].value, bytes_dir=v[2].value, bytes_upd=v[3].value)
+
     def apply_dev(self, x_ptr, y_ptr):
         check(lib().se_helm_apply_dev(self.h, x_ptr, y_ptr), "se_helm_apply_dev")
 
@@ -367,6 +375,14 @@ class Batch:
         infos = self._infos()
         check(lib().se_batch_pcg_dev(self.h, fp, xp, C.byref(o), infos), "se_batch_pcg_dev")
         return [infos[i].as_dict() for i in range(len(self.helms))]
+
+    def probe(self, precision=PREC_FP32_IR, reps=50, max_groups=8):
+        """Per order group: dict(order, us_dir, us_upd, bytes_dir, bytes_upd) (after a solve)."""
+        o = np.zeros(max_groups, dtype=np.int32)
+        a, b_, c, e = (np.zeros(max_groups) for _ in range(4))
+        ng = lib().se_batch_probe(self.h, precision, reps, max_groups, o, a, b_, c, e)
+        check(min(ng, 0), "se_batch_probe")
+        return [dict(order=int(o[g]), us_dir=a[g], us_upd=b_[g], bytes_dir=c[g], bytes_upd=e[g]) for g in range(ng)]
 
     def close(self):
         if self.h:

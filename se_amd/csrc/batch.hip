@@ -471,6 +471,74 @@ extern "C" int se_batch_destroy(se_batch* b) {
 
 extern "C" int se_batch_size(const se_batch* b) { return b ? (int)b->ops.size() : 0; }
 
+namespace {
+// time `reps` back-to-back launches of each work-item kernel of every group,
+// continuing the CG recurrence of the last solve with every member running
+template <typename T>
+int batch_probe(se_batch* b, int reps, int max_groups, int* orders, double* us_dir, double* us_upd,
+                double* bytes_dir, double* bytes_upd) {
+    hipStream_t s = b->ctx->stream;
+    const int count = (int)b->ops.size();
+    for (auto& g : b->groups)
+        SE_CHECK(!g.h_members.empty(), SE_ERR_STATE, "se_batch_probe: run a solve of this batch first");
+    for (int i = 0; i < count; i++) {
+        CgState& h = b->h_states[i];
+        h.stop = 0;
+        h.status = CG_RUNNING;
+        h.tol2 = 0.0;
+        h.iter = 0;
+        h.maxit = 1 << 30;
+        h.rho = h.rho_prev = 1.0;
+        h.beta = 0.0;
+    }
+    SE_HIP(hipMemcpyAsync(b->d_states.ptr, b->h_states, sizeof(CgState) * count, hipMemcpyHostToDevice, s));
+    int ng = 0;
+    for (auto& g : b->groups) {
+        if (ng >= max_groups) break;
+        const int nw = (int)g.tiles.size();
+        double n = 0.0;
+        for (int i : g.members) n += b->ops[i]->mesh->n;
+        float ms_d = 0.f, ms_u = 0.f;
+        int rc = g_dir_apply<T, TCG_NORMAL>(g, nw, g.d_tiles.as<int2>(), g.grid > 0 ? g.grid : nw, s);
+        if (!rc) rc = g_update<T>(g, nw, g.d_tiles.as<int2>(), g.grid > 0 ? g.grid : nw, s);
+        if (rc) return rc;
+        SE_HIP(hipEventRecord(b->t0, s));
+        for (int r = 0; r < reps && !rc; r++)
+            rc = g_dir_apply<T, TCG_NORMAL>(g, nw, g.d_tiles.as<int2>(), g.grid > 0 ? g.grid : nw, s);
+        SE_HIP(hipEventRecord(b->t1, s));
+        SE_HIP(hipEventSynchronize(b->t1));
+        (void)hipEventElapsedTime(&ms_d, b->t0, b->t1);
+        SE_HIP(hipEventRecord(b->t0, s));
+        for (int r = 0; r < reps && !rc; r++)
+            rc = g_update<T>(g, nw, g.d_tiles.as<int2>(), g.grid > 0 ? g.grid : nw, s);
+        SE_HIP(hipEventRecord(b->t1, s));
+        SE_HIP(hipEventSynchronize(b->t1));
+        (void)hipEventElapsedTime(&ms_u, b->t0, b->t1);
+        if (rc) return rc;
+        const double sT = (double)sizeof(T);
+        orders[ng] = g.N;
+        us_dir[ng] = 1e3 * ms_d / reps;
+        us_upd[ng] = 1e3 * ms_u / reps;
+        bytes_dir[ng] = n * (8.0 + sT);         // r, p read; q written
+        bytes_upd[ng] = n * (20.0 + 2.0 * sT);  // p, r, q, x read; p, x, r written
+        ng++;
+    }
+    // the members' states are now those of the probe: the next solve starts afresh
+    return ng;
+}
+}  // namespace
+
+extern "C" int se_batch_probe(se_batch* b, int precision, int reps, int max_groups, int* orders, double* us_dir,
+                              double* us_upd, double* bytes_dir, double* bytes_upd) {
+    SE_CHECK(b && orders && us_dir && us_upd && bytes_dir && bytes_upd, SE_ERR_ARG, "se_batch_probe: NULL argument");
+    SE_CHECK(reps > 0 && max_groups > 0, SE_ERR_ARG, "se_batch_probe: reps and max_groups must be positive");
+    SE_CHECK(precision == SE_PREC_FP32_IR || precision == SE_PREC_BF16_IR, SE_ERR_ARG,
+             "se_batch_probe: precision must be SE_PREC_FP32_IR or SE_PREC_BF16_IR");
+    SE_HIP(hipSetDevice(b->ctx->device));
+    if (precision == SE_PREC_FP32_IR) return batch_probe<float>(b, reps, max_groups, orders, us_dir, us_upd, bytes_dir, bytes_upd);
+    return batch_probe<__hip_bfloat16>(b, reps, max_groups, orders, us_dir, us_upd, bytes_dir, bytes_upd);
+}
+
 extern "C" int se_batch_pcg_dev(se_batch* b, const double* const* d_f, double* const* d_x, const se_solve_opts* opts,
                                 se_solve_info* infos) {
     SE_CHECK(b && (b->ops.empty() || (d_f && d_x)), SE_ERR_ARG, "se_batch_pcg_dev: NULL argument");

@@ -431,6 +431,64 @@ extern "C" int se_helm_apply_inner(se_op* op, int precision, const float* x, flo
     return SE_OK;
 }
 
+namespace {
+// time `reps` back-to-back launches of the direction+operator kernel and of
+// the update kernel, continuing the recurrence of the last solve
+template <typename T>
+int helm_probe(se_op* op, int reps, double* out) {
+    hipStream_t s = op->mesh->ctx->stream;
+    se_ctx* c = op->mesh->ctx;
+    const TcgMember M = tcg_member(op, nullptr, 0.0, 0, 1);
+    CgState* hs = (CgState*)op->h_state;
+    SE_HIP(hipMemcpyAsync(hs, M.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    SE_HIP(hipStreamSynchronize(s));
+    hs->stop = 0;
+    hs->status = CG_RUNNING;
+    hs->tol2 = 0.0;
+    hs->iter = 0;
+    hs->maxit = 1 << 30;
+    hs->rho = hs->rho_prev = 1.0;
+    hs->beta = 0.0;
+    SE_HIP(hipMemcpyAsync(M.st, hs, sizeof(CgState), hipMemcpyHostToDevice, s));
+    int rc = launch_dir_apply<T, TCG_NORMAL>(op, M);
+    if (!rc) rc = launch_update<T>(op, M);
+    float ms_d = 0.f, ms_u = 0.f;
+    SE_HIP(hipEventRecord(c->ev0, s));
+    for (int r = 0; r < reps && !rc; r++) rc = launch_dir_apply<T, TCG_NORMAL>(op, M);
+    SE_HIP(hipEventRecord(c->ev1, s));
+    SE_HIP(hipEventSynchronize(c->ev1));
+    (void)hipEventElapsedTime(&ms_d, c->ev0, c->ev1);
+    SE_HIP(hipEventRecord(c->ev0, s));
+    for (int r = 0; r < reps && !rc; r++) rc = launch_update<T>(op, M);
+    SE_HIP(hipEventRecord(c->ev1, s));
+    SE_HIP(hipEventSynchronize(c->ev1));
+    (void)hipEventElapsedTime(&ms_u, c->ev0, c->ev1);
+    if (rc) return rc;
+    const double n = op->mesh->n, sT = (double)sizeof(T);
+    out[0] = 1e3 * ms_d / reps;
+    out[1] = 1e3 * ms_u / reps;
+    out[2] = n * (8.0 + sT);
+    out[3] = n * (20.0 + 2.0 * sT);
+    return SE_OK;
+}
+}  // namespace
+
+extern "C" int se_helm_probe(se_op* op, int precision, int reps, double* us_dir, double* us_upd, double* bytes_dir,
+                             double* bytes_upd) {
+    SE_CHECK(op && us_dir && us_upd && bytes_dir && bytes_upd, SE_ERR_ARG, "se_helm_probe: NULL argument");
+    SE_CHECK(reps > 0, SE_ERR_ARG, "se_helm_probe: reps must be positive");
+    SE_CHECK(precision == SE_PREC_FP32_IR || precision == SE_PREC_BF16_IR, SE_ERR_ARG,
+             "se_helm_probe: precision must be SE_PREC_FP32_IR or SE_PREC_BF16_IR");
+    SE_CHECK(op->sep_ready && op->h_state && op->sep_state.ptr, SE_ERR_STATE,
+             "se_helm_probe: run a mixed-precision solve of this operator first");
+    SE_HIP(hipSetDevice(op->mesh->ctx->device));
+    double out[4];
+    int rc = precision == SE_PREC_FP32_IR ? helm_probe<float>(op, reps, out) : helm_probe<__hip_bfloat16>(op, reps, out);
+    if (rc) return rc;
+    *us_dir = out[0], *us_upd = out[1], *bytes_dir = out[2], *bytes_upd = out[3];
+    return SE_OK;
+}
+
 namespace se {
 int pcg_mixed(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, se_solve_info& info) {
     if (o.precision == SE_PREC_FP32_IR) return tcg_solve<float>(op, d_f, d_x, o, info);
