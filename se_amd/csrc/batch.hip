@@ -44,6 +44,11 @@ struct Group {
     std::vector<int> members;   // batch member indices
     std::vector<int2> tiles;    // all (local member, tile) items
     DevBuf d_members, d_tiles;  // TcgMember[], int2[]
+    // the iteration chain's work list: the items of the members still
+    // running, compacted by the host between replays (count on the device)
+    DevBuf d_active, d_nactive;
+    std::vector<int2> h_active;
+    int* h_nactive = nullptr;  // pinned
     DevBuf d_amembers;          // ApplyMember[] (FP64 operator of each member)
     // lists of the members due for a reliable update (rebuilt per check);
     // the host copies live here until the stream has consumed them
@@ -75,6 +80,8 @@ struct se_batch {
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
         for (auto& g : groups) {
+            if (g.h_nactive) (void)hipHostFree(g.h_nactive);
+            g.h_nactive = nullptr;
             if (g.stream) (void)hipStreamDestroy(g.stream);
             if (g.done) (void)hipEventDestroy(g.done);
             g.stream = nullptr;
@@ -103,12 +110,14 @@ int upload(DevBuf& d, const std::vector<V>& v, hipStream_t s) {
 
 // ---- launches of one group's work-item kernels ----
 template <typename T>
-int g_update(const Group& g, int nwork, const int2* work, int grid, hipStream_t s) {
+int g_update(const Group& g, int nwork, const int2* work, int grid, hipStream_t s, const int* ndev = nullptr) {
     if (nwork == 0) return SE_OK;
     const TcgMember* Ms = g.d_members.as<TcgMember>();
     switch (g.N) {
-#define SE_CASE(NN) \
-    case NN: hipLaunchKernelGGL((k_tcg_update_batch<NN, T>), dim3(grid), dim3(TCG_NT), 0, s, Ms, work, nwork); break;
+#define SE_CASE(NN)                                                                                              \
+    case NN:                                                                                                     \
+        hipLaunchKernelGGL((k_tcg_update_batch<NN, T>), dim3(grid), dim3(TCG_NT), 0, s, Ms, work, nwork, ndev); \
+        break;
         SE_BATCH_ORDERS(SE_CASE)
 #undef SE_CASE
         default: return fail(SE_ERR_UNSUPPORTED, "batched solve built for orders 2..8");
@@ -116,13 +125,14 @@ int g_update(const Group& g, int nwork, const int2* work, int grid, hipStream_t 
     return SE_OK;
 }
 template <typename T, int MODE>
-int g_dir_apply(const Group& g, int nwork, const int2* work, int grid, hipStream_t s) {
+int g_dir_apply(const Group& g, int nwork, const int2* work, int grid, hipStream_t s, const int* ndev = nullptr) {
     if (nwork == 0) return SE_OK;
     const TcgMember* Ms = g.d_members.as<TcgMember>();
     switch (g.N) {
-#define SE_CASE(NN)                                                                                             \
-    case NN:                                                                                                    \
-        hipLaunchKernelGGL((k_tcg_dir_apply_batch<NN, T, MODE>), dim3(grid), dim3(TCG_NT), 0, s, Ms, work, nwork); \
+#define SE_CASE(NN)                                                                                                   \
+    case NN:                                                                                                          \
+        hipLaunchKernelGGL((k_tcg_dir_apply_batch<NN, T, MODE>), dim3(grid), dim3(TCG_NT), 0, s, Ms, work, nwork,     \
+                           ndev);                                                                                     \
         break;
         SE_BATCH_ORDERS(SE_CASE)
 #undef SE_CASE
@@ -172,10 +182,13 @@ int chains(se_batch* b, int k) {
         hipError_t e = hipEventRecord(b->fork, s);
         for (auto& g : b->groups) {
             if (e == hipSuccess) e = hipStreamWaitEvent(g.stream, b->fork, 0);
+            // full-size grid argument; the kernels read the active count
             const int nw = (int)g.tiles.size();
+            const int2* w = g.d_active.as<int2>();
+            const int* nd = g.d_nactive.as<int>();
             for (int i = 0; i < k && !rc && e == hipSuccess; i++) {
-                rc = g_update<T>(g, nw, g.d_tiles.as<int2>(), g.grid, g.stream);
-                if (!rc) rc = g_dir_apply<T, TCG_NORMAL>(g, nw, g.d_tiles.as<int2>(), g.grid, g.stream);
+                rc = g_update<T>(g, nw, w, g.grid, g.stream, nd);
+                if (!rc) rc = g_dir_apply<T, TCG_NORMAL>(g, nw, w, g.grid, g.stream, nd);
             }
             if (e == hipSuccess) e = hipEventRecord(g.done, g.stream);
             if (e == hipSuccess) e = hipStreamWaitEvent(s, g.done, 0);
@@ -196,6 +209,21 @@ int chains(se_batch* b, int k) {
         b->graph_key = key;
     }
     SE_HIP(hipGraphLaunch(b->graph, s));
+    return SE_OK;
+}
+
+// each group's iteration work list := the items of its members still running
+int compact(se_batch* b, const std::vector<TcgControl>& ctl, hipStream_t s) {
+    for (auto& g : b->groups) {
+        g.h_active.clear();
+        for (const int2& it : g.tiles)
+            if (ctl[g.members[it.x]].status == CG_RUNNING) g.h_active.push_back(it);
+        *g.h_nactive = (int)g.h_active.size();
+        if (!g.h_active.empty())
+            SE_HIP(hipMemcpyAsync(g.d_active.ptr, g.h_active.data(), sizeof(int2) * g.h_active.size(),
+                                  hipMemcpyHostToDevice, s));
+        SE_HIP(hipMemcpyAsync(g.d_nactive.ptr, g.h_nactive, sizeof(int), hipMemcpyHostToDevice, s));
+    }
     return SE_OK;
 }
 
@@ -281,6 +309,8 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
         ctl[i].started(b->h_states[i]);
         running += ctl[i].status == CG_RUNNING;
     }
+    rc = compact(b, ctl, s);
+    if (rc) return rc;
 
     // autotuner candidates: grid = all items, or 8, 4, 2 workgroups per CU.
     // Phase 2j captures candidate j and runs it once, phase 2j+1 times a
@@ -358,17 +388,20 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
         if (rc) return rc;
         // 1: continue the direction, 2: restart it
         std::vector<char> restart(count, 0);
+        bool finished = false;
         for (int i : due) {
             CgState hh = b->h_states[i];
             if (ctl[i].after_update(hh) == CG_RUNNING) {
                 restart[i] = ctl[i].reset_dir ? 2 : 1;
             } else {
-                hh.stop = 1;  // finished: its work items stay idle
+                hh.stop = 1;  // finished: its work items leave the iteration list
                 running--;
+                finished = true;
             }
             b->h_states[i] = hh;
         }
         rc = push();
+        if (!rc && finished) rc = compact(b, ctl, s);
         if (rc) return rc;
         for (auto& g : b->groups) {
             // items of continuing members first, then of restarting ones
@@ -445,6 +478,10 @@ extern "C" int se_batch_create(se_ctx* ctx, int count, se_op* const* ops, se_bat
             hipEventCreateWithFlags(&g.done, hipEventDisableTiming) != hipSuccess)
             rc = fail(SE_ERR_HIP, "se_batch_create: group stream");
         if (!rc) rc = upload(g.d_tiles, g.tiles, ctx->stream);
+        if (!rc) rc = g.d_active.alloc(sizeof(int2) * std::max<size_t>(g.tiles.size(), 1));
+        if (!rc) rc = g.d_nactive.alloc(sizeof(int));
+        if (!rc && hipHostMalloc((void**)&g.h_nactive, sizeof(int), hipHostMallocDefault) != hipSuccess)
+            rc = fail(SE_ERR_NOMEM, "se_batch_create: pinned count");
         b->groups.push_back(std::move(g));
     }
     if (!rc) {

@@ -26,7 +26,7 @@ struct se_op;
 namespace se {
 
 constexpr int TCG_NT = 256, TCG_TY = 32, TCG_RPT = TCG_TY / (TCG_NT / 64);
-constexpr int TCG_PRE = 4;  // preloaded partials per thread and value
+constexpr int TCG_PRE = 2;  // preloaded partials per thread and value
 
 // direction kernel modes: FIRST starts a solve (p = z, sets the reference
 // norms), NORMAL continues (p = z + beta p), RESET continues the solve with
@@ -123,47 +123,73 @@ struct TcgMember {
     __device__ __forceinline__ double* part_s() const { return part + 7 * (size_t)tiles; }
 };
 
+// Address spaces: the solver vectors are global memory; the band tables are
+// read-only, so they are read through the constant space, where a
+// wave-uniform address (a row's coefficients) becomes a scalar load and the
+// value an SGPR operand, and a per-lane address a vector load.
+template <typename T> using gp = __attribute__((address_space(1))) T*;
+template <typename T> using cgp = const __attribute__((address_space(1))) T*;
+template <typename T> using kp = const __attribute__((address_space(4))) T*;
+
+// element loads/stores of the direction vector through those address spaces
+template <typename T> __device__ __forceinline__ float ldf(cgp<T> p, size_t i);
+template <> __device__ __forceinline__ float ldf<float>(cgp<float> p, size_t i) { return p[i]; }
+template <> __device__ __forceinline__ float ldf<__hip_bfloat16>(cgp<__hip_bfloat16> p, size_t i) {
+    return __uint_as_float((unsigned)((cgp<unsigned short>)p)[i] << 16);
+}
+template <typename T> __device__ __forceinline__ void stf(gp<T> p, size_t i, float v);
+template <> __device__ __forceinline__ void stf<float>(gp<float> p, size_t i, float v) { p[i] = v; }
+template <> __device__ __forceinline__ void stf<__hip_bfloat16>(gp<__hip_bfloat16> p, size_t i, float v) {
+    ((gp<unsigned short>)p)[i] = __bfloat16_as_ushort(__float2bfloat16(v));
+}
+
+struct TcgTab {
+    kp<float> gx, gy, sx, sy, mx, my;
+    __device__ __forceinline__ TcgTab(const SepParams& S)
+        : gx((kp<float>)S.gx), gy((kp<float>)S.gy), sx((kp<float>)S.sx), sy((kp<float>)S.sy),
+          mx((kp<float>)S.mx), my((kp<float>)S.my) {}
+};
+
 // ---------------------------------------------------------------------------
 // k_tcg_dir_apply
 // ---------------------------------------------------------------------------
 template <int N, typename T, int MODE>
 __device__ __forceinline__ void tcg_dir_apply_tile(const TcgMember& M, int tile) {
     constexpr bool FIRST = MODE == TCG_FIRST, NOP = MODE != TCG_NORMAL;  // NOP: the old p is not used
-    const SepParams& S = M.S;
-    const float* __restrict__ r = M.r;
-    const T* __restrict__ p = (const T*)M.p;
-    float* __restrict__ q = M.q;
-    const int np = M.tiles;
-    const double* __restrict__ part_b = M.part_b();
-    const double* __restrict__ part_t = M.part_t();
-    double* __restrict__ part_a = M.part_a();
-    CgState* __restrict__ st = M.st;
-    const double rtol = M.rtol;
     constexpr int B = 2 * N + 1;
     constexpr int TY = TCG_TY, NT = TCG_NT, NW = NT / 64, RPT = TCG_RPT;
     constexpr int W = SEP_TX + 2 * N, H = TY + 2 * N, WP = W + 1;
-    constexpr int RL = (H + NW - 1) / NW;
+    constexpr int RL = (H + NW - 1) / NW;           // main-column rows per wave
+    constexpr int NX = (H * 2 * N + NT - 1) / NT;   // right-halo cells per thread
+    const SepParams& S = M.S;
+    const TcgTab tb(S);
+    cgp<float> r = (cgp<float>)M.r;
+    cgp<T> p = (cgp<T>)M.p;
+    gp<float> q = (gp<float>)M.q;
+    const int np = M.tiles;
+    CgState* __restrict__ st = M.st;
     constexpr int NG = (TY * B + NT - 1) / NT;
     __shared__ float s_u[H * WP];
-    __shared__ float s_gy[TY * B];
-    __shared__ float s_cg[W], s_cs[W], s_cm[W];  // per tile column: gx centre, sx, mx
-    __shared__ float s_rg[H], s_rs[H], s_rm[H];  // per tile row: gy centre, sy, my
+    __shared__ float s_gy[TY * B];  // row bands of the owned rows
 
     const TcgTile t(S, tile);
     const int lane = t.lane, wv = t.wv, I0 = t.I0, J0 = t.J0;
 
     // ---- every global load first ----
-    int Ic0 = I0 - N + lane, Ic1 = I0 - N + 64 + lane;
-    bool c0 = true, c1 = lane < 2 * N;
-    if (S.per0) {
-        Ic0 = Ic0 < 0 ? Ic0 + S.nU0 : (Ic0 >= S.nU0 ? Ic0 - S.nU0 : Ic0);
-        Ic1 = Ic1 >= S.nU0 ? Ic1 - S.nU0 : Ic1;
+    float gyv[NG];
+#pragma unroll
+    for (int k = 0; k < NG; k++) {
+        const int i = threadIdx.x + NT * k;
+        const bool ok = i < TY * B && J0 + i / B < S.nU1;
+        const float v = tb.gy[ok ? (size_t)J0 * B + i : 0];
+        gyv[k] = ok ? v : 0.0f;
     }
-    c0 = c0 && Ic0 >= 0 && Ic0 < S.nU0;
-    c1 = c1 && Ic1 >= 0 && Ic1 < S.nU0;
+    // main columns: lane l holds tile column l of rows wv + NW*k
+    int Ic0 = I0 - N + lane;
+    if (S.per0) Ic0 = Ic0 < 0 ? Ic0 + S.nU0 : (Ic0 >= S.nU0 ? Ic0 - S.nU0 : Ic0);
+    const bool c0 = Ic0 >= 0 && Ic0 < S.nU0;
     Ic0 = c0 ? Ic0 : 0;
-    Ic1 = c1 ? Ic1 : 0;
-    float r0[RL], r1[RL], p0[RL], p1[RL];
+    float r0[RL], p0[RL], rg[RL], rs[RL], rm[RL];
     bool rok[RL];
 #pragma unroll
     for (int k = 0; k < RL; k++) {
@@ -174,41 +200,39 @@ __device__ __forceinline__ void tcg_dir_apply_tile(const TcgMember& M, int tile)
         const int Jc = rok[k] ? J : 0;
         const size_t rowo = (size_t)Jc * S.nU0;
         r0[k] = r[rowo + Ic0];
-        r1[k] = r[rowo + Ic1];
-        if (!NOP) {
-            p0[k] = to_f<T>(p[rowo + Ic0]);
-            p1[k] = to_f<T>(p[rowo + Ic1]);
-        }
+        if (!NOP) p0[k] = ldf<T>(p, rowo + Ic0);
+        rg[k] = tb.gy[(size_t)Jc * B + N];  // wave-uniform: scalar loads
+        rs[k] = tb.sy[Jc];
+        rm[k] = tb.my[Jc];
     }
-    // row tables of tile row `lane` (H <= 64 rows)
-    float rgl, rsl, rml;
-    {
-        int J = J0 - N + lane;
-        if (S.per1) J = J < 0 ? J + S.nU1 : (J >= S.nU1 ? J - S.nU1 : J);
-        const int Jc = (lane < H && J >= 0 && J < S.nU1) ? J : 0;
-        rgl = S.gy[(size_t)Jc * B + N];
-        rsl = S.sy[Jc];
-        rml = S.my[Jc];
-    }
-    const float cg0 = S.gx[(size_t)Ic0 * B + N], cs0 = S.sx[Ic0], cm0 = S.mx[Ic0];
-    const float cg1 = S.gx[(size_t)Ic1 * B + N], cs1 = S.sx[Ic1], cm1 = S.mx[Ic1];
-    float gyv[NG];
+    const float cg0 = tb.gx[(size_t)Ic0 * B + N], cs0 = tb.sx[Ic0], cm0 = tb.mx[Ic0];
+    // right-halo columns 64 .. 64+2N-1, spread over all threads
+    float rx[NX], px[NX], dx[NX];
+    bool okx[NX];
 #pragma unroll
-    for (int k = 0; k < NG; k++) {
+    for (int k = 0; k < NX; k++) {
         const int i = threadIdx.x + NT * k;
-        const bool ok = i < TY * B && J0 + i / B < S.nU1;
-        const float v = S.gy[ok ? (size_t)J0 * B + i : 0];
-        gyv[k] = ok ? v : 0.0f;
+        const int a = i / (2 * N), cc = 64 + i % (2 * N);
+        int J = J0 - N + a, I = I0 - N + cc;
+        if (S.per1) J = J < 0 ? J + S.nU1 : (J >= S.nU1 ? J - S.nU1 : J);
+        if (S.per0) I = I >= S.nU0 ? I - S.nU0 : I;
+        okx[k] = i < H * 2 * N && J >= 0 && J < S.nU1 && I >= 0 && I < S.nU0;
+        const int Jc = okx[k] ? J : 0, Icc = okx[k] ? I : 0;
+        const size_t o = (size_t)Jc * S.nU0 + Icc;
+        rx[k] = r[o];
+        px[k] = NOP ? 0.0f : ldf<T>(p, o);
+        dx[k] = tcg_dinv(tb.gx[(size_t)Icc * B + N], tb.sx[Icc], tb.mx[Icc], tb.gy[(size_t)Jc * B + N], tb.sy[Jc],
+                         tb.my[Jc]);
     }
     const int Ic = t.colok ? t.I : 0;
     float gxr[B];
 #pragma unroll
-    for (int s = 0; s < B; s++) gxr[s] = S.gx[(size_t)Ic * B + s];
-    const float sxI = S.sx[Ic], mxI = S.mx[Ic];
+    for (int s = 0; s < B; s++) gxr[s] = tb.gx[(size_t)Ic * B + s];
+    const float sxI = tb.sx[Ic], mxI = tb.mx[Ic];
     TcgPart<4> pb;
-    pb.load(part_b, np);
+    pb.load(M.part_b(), np);
     TcgPart<2> pt;
-    if (FIRST) pt.load(part_t, np);
+    if (FIRST) pt.load(M.part_t(), np);
     // the stop flag, read by one thread and shared through LDS, so that the
     // whole workgroup takes the same branch even if the flag changes meanwhile
     __shared__ int s_stop;
@@ -216,36 +240,24 @@ __device__ __forceinline__ void tcg_dir_apply_tile(const TcgMember& M, int tile)
     const int null_space = st->null_space, iter = st->iter, maxit = st->maxit;
     const double rho_prev = st->rho_prev, tol2_in = st->tol2;
 
-    // ---- tables to LDS, totals, state ----
-    if (wv == 0 && lane < H) {
-        s_rg[lane] = rgl;
-        s_rs[lane] = rsl;
-        s_rm[lane] = rml;
-    }
-    s_cg[lane] = cg0, s_cs[lane] = cs0, s_cm[lane] = cm0;
-    if (lane < 2 * N) s_cg[64 + lane] = cg1, s_cs[64 + lane] = cs1, s_cm[64 + lane] = cm1;
-#pragma unroll
-    for (int k = 0; k < NG; k++) {
-        const int i = threadIdx.x + NT * k;
-        if (i < TY * B) s_gy[i] = gyv[k];
-    }
-    double tb[4];
-    pb.reduce(part_b, np, tb);  // (its barriers publish the tables and s_stop)
+    // ---- totals, state ----
+    double tbv[4];
+    pb.reduce(M.part_b(), np, tbv);  // (its barriers publish s_stop)
     if (s_stop) return;
-    const double rr = tb[0];
-    const bool finite = (rr - rr == 0.0) && (tb[1] - tb[1] == 0.0);
-    const double mu = null_space ? tb[2] / st->n : 0.0;
-    const double rho_new = tb[1] - mu * tb[3];
+    const double rr = tbv[0];
+    const bool finite = (rr - rr == 0.0) && (tbv[1] - tbv[1] == 0.0);
+    const double mu = null_space ? tbv[2] / st->n : 0.0;
+    const double rho_new = tbv[1] - mu * tbv[3];
     const bool lead = tile == 0 && threadIdx.x == 0;  // the member's first tile writes its state
     double beta = 0.0;
     if (FIRST) {
         double tt[2];
-        pt.reduce(part_t, np, tt);
+        pt.reduce(M.part_t(), np, tt);
         const double bb = null_space ? tt[0] - tt[1] * (tt[1] / st->n) : tt[0];
         const bool fin = finite && (bb - bb == 0.0);
         if (lead) {
             st->bb = bb;
-            st->tol2 = rtol * rtol * bb;
+            st->tol2 = M.rtol * M.rtol * bb;
             st->rr = rr;
             st->rho = rho_new;
             st->beta = 0.0;
@@ -282,18 +294,25 @@ __device__ __forceinline__ void tcg_dir_apply_tile(const TcgMember& M, int tile)
 
     // ---- new direction on the tile and its halo ----
 #pragma unroll
+    for (int k = 0; k < NG; k++) {
+        const int i = threadIdx.x + NT * k;
+        if (i < TY * B) s_gy[i] = gyv[k];
+    }
+#pragma unroll
     for (int k = 0; k < RL; k++) {
         const int a = wv + NW * k;
         if (a < H) {
-            const float d0 = tcg_dinv(s_cg[lane], s_cs[lane], s_cm[lane], s_rg[a], s_rs[a], s_rm[a]);
-            const float v0 = tcg_pnew<T>(r0[k], d0, fmu, fb, NOP ? 0.0f : p0[k]);
-            s_u[a * WP + lane] = (rok[k] && c0) ? v0 : 0.0f;
-            if (lane < 2 * N) {
-                const float d1 =
-                    tcg_dinv(s_cg[64 + lane], s_cs[64 + lane], s_cm[64 + lane], s_rg[a], s_rs[a], s_rm[a]);
-                const float v1 = tcg_pnew<T>(r1[k], d1, fmu, fb, NOP ? 0.0f : p1[k]);
-                s_u[a * WP + 64 + lane] = (rok[k] && c1) ? v1 : 0.0f;
-            }
+            const float d = tcg_dinv(cg0, cs0, cm0, rg[k], rs[k], rm[k]);
+            const float v = tcg_pnew<T>(r0[k], d, fmu, fb, NOP ? 0.0f : p0[k]);
+            s_u[a * WP + lane] = (rok[k] && c0) ? v : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NX; k++) {
+        const int i = threadIdx.x + NT * k;
+        if (i < H * 2 * N) {
+            const float v = tcg_pnew<T>(rx[k], dx[k], fmu, fb, px[k]);
+            s_u[(i / (2 * N)) * WP + 64 + i % (2 * N)] = okx[k] ? v : 0.0f;
         }
     }
     __syncthreads();
@@ -315,21 +334,20 @@ __device__ __forceinline__ void tcg_dir_apply_tile(const TcgMember& M, int tile)
     double dot = 0.0;
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
-        const int rr_ = g * RPT + k;
-        const int J = J0 + rr_;
-        const float* gy = &s_gy[rr_ * B];
+        const int J = J0 + g * RPT + k;
+        const int Jc = J < S.nU1 ? J : 0;
         float acc = 0.0f;
 #pragma unroll
-        for (int s = 0; s < B; s++) acc += gy[s] * win[k + s];
+        for (int s = 0; s < B; s++) acc += s_gy[(g * RPT + k) * B + s] * win[k + s];  // one address per wave
         const float u = win[k + N];
-        const float y = s_rs[rr_ + N] * ax[k] + sxI * acc + (mxI * s_rm[rr_ + N]) * u;
+        const float y = tb.sy[Jc] * ax[k] + sxI * acc + (mxI * tb.my[Jc]) * u;
         if (t.colok && J < S.nU1) {
             q[(size_t)J * S.nU0 + t.I] = y;
             dot += (double)u * (double)y;
         }
     }
     double v[1] = {dot};
-    block_store_partials_at<1, NT>(v, part_a, tile, np);
+    block_store_partials_at<1, NT>(v, M.part_a(), tile, np);
 }
 
 // ---------------------------------------------------------------------------
@@ -337,47 +355,39 @@ __device__ __forceinline__ void tcg_dir_apply_tile(const TcgMember& M, int tile)
 // ---------------------------------------------------------------------------
 template <int N, typename T>
 __device__ __forceinline__ void tcg_update_tile(const TcgMember& M, int tile) {
-    const SepParams& S = M.S;
-    T* __restrict__ p = (T*)M.p;
-    float* __restrict__ r = M.r;
-    const float* __restrict__ q = M.q;
-    float* __restrict__ x = M.xlo;
-    const int np = M.tiles;
-    const double* __restrict__ part_a = M.part_a();
-    double* __restrict__ part_b = M.part_b();
-    CgState* __restrict__ st = M.st;
     constexpr int B = 2 * N + 1, RPT = TCG_RPT;
+    const SepParams& S = M.S;
+    const TcgTab tb(S);
+    gp<T> p = (gp<T>)M.p;
+    gp<float> r = (gp<float>)M.r;
+    cgp<float> q = (cgp<float>)M.q;
+    gp<float> x = (gp<float>)M.xlo;
+    const int np = M.tiles;
+    CgState* __restrict__ st = M.st;
     const TcgTile t(S, tile);
     const int Ic = t.colok ? t.I : 0;
-    float pv[RPT], rv[RPT], qv[RPT], xv[RPT];
+    float pv[RPT], rv[RPT], qv[RPT], xv[RPT], rg[RPT], rs[RPT], rm[RPT];
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
         const int J = t.row(k);
         const int Jc = J < S.nU1 ? J : 0;
         const size_t o = (size_t)Jc * S.nU0 + Ic;
-        pv[k] = to_f<T>(p[o]);
+        pv[k] = ldf<T>((cgp<T>)p, o);
         rv[k] = r[o];
         qv[k] = q[o];
         xv[k] = x[o];
+        rg[k] = tb.gy[(size_t)Jc * B + N];  // wave-uniform: scalar loads
+        rs[k] = tb.sy[Jc];
+        rm[k] = tb.my[Jc];
     }
-    // row tables of the wave's row `lane` (lanes < RPT), read back per row
-    // with readlane
-    float rgl, rsl, rml;
-    {
-        const int J = t.row(t.lane < RPT ? t.lane : 0);
-        const int Jc = J < S.nU1 ? J : 0;
-        rgl = S.gy[(size_t)Jc * B + N];
-        rsl = S.sy[Jc];
-        rml = S.my[Jc];
-    }
-    const float cg = S.gx[(size_t)Ic * B + N], cs = S.sx[Ic], cm = S.mx[Ic];
+    const float cg = tb.gx[(size_t)Ic * B + N], cs = tb.sx[Ic], cm = tb.mx[Ic];
     TcgPart<1> pa;
-    pa.load(part_a, np);
+    pa.load(M.part_a(), np);
     __shared__ int s_stop;
     if (threadIdx.x == 0) s_stop = __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const double rho = st->rho, beta = st->beta, mu = st->mu;
     double ta[1];
-    pa.reduce(part_a, np, ta);
+    pa.reduce(M.part_a(), np, ta);
     if (s_stop) return;
     const double pq = ta[0];
     const bool lead = tile == 0 && threadIdx.x == 0;  // the member's first tile writes its state
@@ -403,14 +413,12 @@ __device__ __forceinline__ void tcg_update_tile(const TcgMember& M, int tile) {
         const int J = t.row(k);
         if (t.colok && J < S.nU1) {
             const size_t o = (size_t)J * S.nU0 + t.I;
-            const float dinv = tcg_dinv(cg, cs, cm, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rgl), k)),
-                                        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rsl), k)),
-                                        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rml), k)));
+            const float dinv = tcg_dinv(cg, cs, cm, rg[k], rs[k], rm[k]);
             const float pn = tcg_pnew<T>(rv[k], dinv, fmu, fb, pv[k]);
             const float xn = __fmaf_rn(fa, pn, xv[k]);
             const float rn = __fmaf_rn(-fa, qv[k], rv[k]);
             const float z = dinv * rn;
-            p[o] = from_f<T>(pn);
+            stf<T>(p, o, pn);
             x[o] = xn;
             r[o] = rn;
             srr += (double)rn * rn;
@@ -420,7 +428,7 @@ __device__ __forceinline__ void tcg_update_tile(const TcgMember& M, int tile) {
         }
     }
     double v[4] = {srr, srz, ssz, ssr};
-    block_store_partials_at<4, TCG_NT>(v, part_b, tile, np);
+    block_store_partials_at<4, TCG_NT>(v, M.part_b(), tile, np);
 }
 
 // ---------------------------------------------------------------------------
@@ -530,9 +538,17 @@ static __global__ __launch_bounds__(TCG_NT) void k_tcg_flush(TcgMember M) {
 }
 static __global__ __launch_bounds__(TCG_NT) void k_tcg_true(TcgMember M) { tcg_true_member(M); }
 
+// Batched kernels: the item count is `nwork`, or *nwork_dev when that is
+// given (a work list the host compacts between graph replays).
+__device__ __forceinline__ int work_count(int nwork, const int* __restrict__ nwork_dev) {
+    return nwork_dev ? __builtin_amdgcn_readfirstlane(*nwork_dev) : nwork;
+}
+
 template <int N, typename T, int MODE>
 __global__ __launch_bounds__(TCG_NT) void k_tcg_dir_apply_batch(const TcgMember* __restrict__ Ms,
-                                                                const int2* __restrict__ work, int nwork) {
+                                                                const int2* __restrict__ work, int nwork,
+                                                                const int* __restrict__ nwork_dev) {
+    nwork = work_count(nwork, nwork_dev);
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
         const int2 it = work[w];
         __syncthreads();
@@ -541,7 +557,9 @@ __global__ __launch_bounds__(TCG_NT) void k_tcg_dir_apply_batch(const TcgMember*
 }
 template <int N, typename T>
 __global__ __launch_bounds__(TCG_NT) void k_tcg_update_batch(const TcgMember* __restrict__ Ms,
-                                                             const int2* __restrict__ work, int nwork) {
+                                                             const int2* __restrict__ work, int nwork,
+                                                             const int* __restrict__ nwork_dev) {
+    nwork = work_count(nwork, nwork_dev);
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
         const int2 it = work[w];
         __syncthreads();
