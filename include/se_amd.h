@@ -233,6 +233,76 @@ int se_genvel_dev(se_mesh* mesh, const double* d_psi, double* d_vel);
 int se_calc_timestep(se_mesh* mesh, const double* vel, double courant, double* dt);
 int se_calc_timestep_dev(se_mesh* mesh, const double* d_vel, double courant, double* dt);
 
+/* ---- two-layer quasi-geostrophic step ---------------------------------------
+ * One first-order step of TwoLayerQGEqn [ref: TwoLayerQGEqn::FirstOrder
+ * src/qg/TwoLayerQGEqn.cpp:174-243, GenVel :335-352]: GenVel and GenPV of both
+ * layers, semi-Lagrangian advection of both PVs, baroclinic Helmholtz
+ * (lap - (F1+F2)) tau = a1^ - a2^, barotropic Poisson
+ * lap chi = a1^ + a2^ + (F1-F2) tau - 2 beta y, then psi/omega of both layers.
+ * The inversions run with the precision of `opts` (FP64 or mixed). */
+typedef struct se_qg2 se_qg2;
+int se_qg2_create(se_mesh* mesh, double F1, double F2, double beta, int detj_mode, se_qg2** out);
+int se_qg2_destroy(se_qg2* q);
+/* host fields, nV each */
+int se_qg2_set_state(se_qg2* q, const double* psi1, const double* omega1, const double* psi2, const double* omega2);
+int se_qg2_get_state(se_qg2* q, double* psi1, double* omega1, double* psi2, double* omega2);
+/* device addresses of the state fields (nullable outputs) */
+int se_qg2_state_dev(se_qg2* q, double** psi1, double** omega1, double** psi2, double** omega2);
+/* CFL time step of both layers' velocities [ref: CalcTimeStep src/base/Utils.cpp:23-40] */
+int se_qg2_timestep(se_qg2* q, double courant, double* dt);
+/* infos: 2 entries (baroclinic, barotropic), nullable */
+int se_qg2_step(se_qg2* q, double dt, const se_solve_opts* opts, se_solve_info* infos);
+
+/* ---- request server -----------------------------------------------------------
+ * No counterpart in the reference (a model driver steps one TwoLayerQGEqn at a
+ * time).  A server owns a context on `device` and one host thread; clients
+ * submit two-layer QG step requests from any thread.  Requests are admitted in
+ * arrival order while their device memory fits `memory_budget`; every tick
+ * advances each admitted request by one se_qg2_step-equivalent step, the
+ * inversions of all admitted requests running as batched solves (continuous
+ * batching: requests join and leave between ticks).  A request's result is
+ * bitwise what it gets when served alone. */
+typedef struct se_server se_server;
+typedef struct {
+    double memory_budget; /* bytes of device memory for admitted requests (default 200e9) */
+    int max_active;       /* limit on admitted requests, 0 = none */
+    int precision;        /* SE_PREC_* of the inversions (default SE_PREC_FP32_IR) */
+    double rtol;          /* default 1e-10 */
+    int detj_mode;        /* SE_DETJ_* of the operators (default literal, as the reference) */
+} se_server_opts;
+typedef struct {
+    int nEls[2];
+    int N;
+    double min[2], max[2];
+    int periodic[2];
+    int bc[4]; /* bottom, top, left, right Dirichlet */
+    double F1, F2, beta;
+    int steps;  /* time steps to run */
+    double dt;  /* > 0: this step; otherwise the CFL-0.5 step of the initial state */
+    const double *psi1, *omega1, *psi2, *omega2; /* initial fields (nV each), copied at submission */
+} se_qg2_request;
+typedef struct {
+    int64_t id;
+    int status;                     /* SE_OK or an SE_ERR_* code */
+    int steps_done;
+    double t_submit, t_admit, t_done; /* seconds on the server's clock */
+    int64_t inner_iters;            /* CG iterations over all its inversions */
+    double unknowns;                /* sum of the unknowns of its inversions */
+} se_request_result;
+typedef struct {
+    int64_t submitted, admitted, completed, steps, ticks;
+    double device_bytes; /* admitted + pooled model memory (estimate) */
+} se_server_stats;
+void se_server_opts_default(se_server_opts* opts);
+int se_server_create(int device, const se_server_opts* opts, se_server** out);
+int se_server_submit(se_server* server, const se_qg2_request* req, int64_t* id);
+/* blocks until request `id` is done; final fields into the (nullable) arrays */
+int se_server_wait(se_server* server, int64_t id, se_request_result* res, double* psi1, double* omega1,
+                   double* psi2, double* omega2);
+int se_server_get_stats(se_server* server, se_server_stats* stats);
+/* serves what was submitted, then stops */
+int se_server_destroy(se_server* server);
+
 #ifdef __cplusplus
 }
 #endif

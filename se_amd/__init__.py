@@ -43,6 +43,32 @@ class SolveInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("memory_budget", C.c_double), ("max_active", C.c_int), ("precision", C.c_int),
+                ("rtol", C.c_double), ("detj_mode", C.c_int)]
+
+
+class QG2Request(C.Structure):
+    _fields_ = [("nEls", C.c_int * 2), ("N", C.c_int), ("min", C.c_double * 2), ("max", C.c_double * 2),
+                ("periodic", C.c_int * 2), ("bc", C.c_int * 4), ("F1", C.c_double), ("F2", C.c_double),
+                ("beta", C.c_double), ("steps", C.c_int), ("dt", C.c_double), ("psi1", C.c_void_p),
+                ("omega1", C.c_void_p), ("psi2", C.c_void_p), ("omega2", C.c_void_p)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int), ("steps_done", C.c_int), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double), ("inner_iters", C.c_int64),
+                ("unknowns", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("admitted", C.c_int64), ("completed", C.c_int64),
+                ("steps", C.c_int64), ("ticks", C.c_int64), ("device_bytes", C.c_double)]
+
+
 class MeshInfo(C.Structure):
     _fields_ = [("N", C.c_int), ("nEls", C.c_int * 2), ("nVerts", C.c_int * 2), ("nV", C.c_int),
                 ("nElsTotal", C.c_int), ("n", C.c_int), ("size0", C.c_int)]
@@ -97,6 +123,19 @@ def lib():
             "se_batch_pcg": (i, [vp, vp, vp, C.POINTER(SolveOpts), vp]),
             "se_batch_probe": (i, [vp, i, i, i, _I, _D, _D, _D, _D]),
             "se_helm_probe": (i, [vp, i, i, C.POINTER(d), C.POINTER(d), C.POINTER(d), C.POINTER(d)]),
+            "se_qg2_create": (i, [vp, d, d, d, i, pvp]),
+            "se_qg2_destroy": (i, [vp]),
+            "se_qg2_set_state": (i, [vp, _D, _D, _D, _D]),
+            "se_qg2_get_state": (i, [vp, _D, _D, _D, _D]),
+            "se_qg2_state_dev": (i, [vp, pvp, pvp, pvp, pvp]),
+            "se_qg2_timestep": (i, [vp, d, C.POINTER(d)]),
+            "se_qg2_step": (i, [vp, d, C.POINTER(SolveOpts), vp]),
+            "se_server_opts_default": (None, [C.POINTER(ServerOpts)]),
+            "se_server_create": (i, [i, C.POINTER(ServerOpts), pvp]),
+            "se_server_submit": (i, [vp, C.POINTER(QG2Request), C.POINTER(C.c_int64)]),
+            "se_server_wait": (i, [vp, C.c_int64, C.POINTER(RequestResult), vp, vp, vp, vp]),
+            "se_server_get_stats": (i, [vp, C.POINTER(ServerStats)]),
+            "se_server_destroy": (i, [vp]),
             "se_sl_advect": (i, [vp, _D, i, _D, d, _D, vp]),
             "se_sl_advect_dev": (i, [vp, vp, i, vp, d, vp, vp]),
             "se_sl_advect2": (i, [vp, _D, _D, i, _D, _D, d, _D]),
@@ -387,6 +426,104 @@ class Batch:
     def close(self):
         if self.h:
             lib().se_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class QG2:
+    """Two-layer QG model on a mesh (TwoLayerQGEqn first-order steps on the device)."""
+
+    def __init__(self, mesh, F1, F2, beta, detj_mode=DETJ_LITERAL):
+        self.mesh = mesh
+        self.h = None
+        h = C.c_void_p()
+        check(lib().se_qg2_create(mesh.h, F1, F2, beta, detj_mode, C.byref(h)), "se_qg2_create")
+        self.h = h
+        mesh.ctx._own(self)
+
+    def set_state(self, psi1, om1, psi2, om2):
+        check(lib().se_qg2_set_state(self.h, _d(psi1), _d(om1), _d(psi2), _d(om2)), "se_qg2_set_state")
+
+    def state(self):
+        out = [np.zeros(self.mesh.nV) for _ in range(4)]
+        check(lib().se_qg2_get_state(self.h, *out), "se_qg2_get_state")
+        return out
+
+    def timestep(self, courant=0.5):
+        dt = C.c_double()
+        check(lib().se_qg2_timestep(self.h, courant, C.byref(dt)), "se_qg2_timestep")
+        return dt.value
+
+    def step(self, dt, **kw):
+        o = solve_opts(**kw)
+        infos = (SolveInfo * 2)()
+        check(lib().se_qg2_step(self.h, dt, C.byref(o), infos), "se_qg2_step")
+        return [infos[0].as_dict(), infos[1].as_dict()]
+
+    def close(self):
+        if self.h:
+            lib().se_qg2_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request server for two-layer QG step requests (own context and thread)."""
+
+    def __init__(self, device=0, memory_budget=200e9, max_active=0, precision=PREC_FP32_IR, rtol=1e-10,
+                 detj_mode=DETJ_LITERAL):
+        o = ServerOpts()
+        lib().se_server_opts_default(C.byref(o))
+        o.memory_budget, o.max_active, o.precision, o.rtol, o.detj_mode = memory_budget, max_active, precision, \
+            rtol, detj_mode
+        h = C.c_void_p()
+        check(lib().se_server_create(device, C.byref(o), C.byref(h)), "se_server_create")
+        self.h = h
+        self._keep = {}
+
+    def submit(self, req, psi1, om1, psi2, om2, dt=0.0):
+        """req: dict with nEls, N, lo, hi, periodic, bc, F1, F2, beta, steps."""
+        r = QG2Request()
+        r.nEls[:] = list(req["nEls"])
+        r.N = req["N"]
+        r.min[:] = list(req["lo"])
+        r.max[:] = list(req["hi"])
+        r.periodic[:] = list(req["periodic"])
+        r.bc[:] = list(req["bc"])
+        r.F1, r.F2, r.beta, r.steps, r.dt = req["F1"], req["F2"], req["beta"], req["steps"], dt
+        arrs = [_d(a) for a in (psi1, om1, psi2, om2)]
+        r.psi1, r.omega1, r.psi2, r.omega2 = [a.ctypes.data for a in arrs]
+        i = C.c_int64()
+        check(lib().se_server_submit(self.h, C.byref(r), C.byref(i)), "se_server_submit")
+        self._keep[i.value] = len(arrs[0])
+        return i.value
+
+    def wait(self, rid, fields=True):
+        nV = self._keep.pop(rid)
+        res = RequestResult()
+        out = [np.zeros(nV) for _ in range(4)] if fields else [None] * 4
+        ptrs = [a.ctypes.data if a is not None else None for a in out]
+        check(lib().se_server_wait(self.h, rid, C.byref(res), *ptrs), "se_server_wait")
+        return res.as_dict(), (out if fields else None)
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().se_server_get_stats(self.h, C.byref(st)), "se_server_get_stats")
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def close(self):
+        if self.h:
+            lib().se_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -312,6 +312,17 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
     rc = compact(b, ctl, s);
     if (rc) return rc;
 
+    // shape key of this batch: its largest group
+    const Group* big = &b->groups[0];
+    for (auto& g : b->groups)
+        if (g.tiles.size() > big->tiles.size()) big = &g;
+    int lg = 0;
+    while ((size_t(1) << lg) < big->tiles.size()) lg++;
+    const std::pair<int, int> shape(big->N, lg);
+    if (!b->tuned) {
+        auto it = c->tuned_grid.find(shape);
+        if (it != c->tuned_grid.end()) b->tuned = it->second;
+    }
     // autotuner candidates: grid = all items, or 8, 4, 2 workgroups per CU.
     // Phase 2j captures candidate j and runs it once, phase 2j+1 times a
     // replay; every launch is followed by the regular check, so the check
@@ -341,6 +352,7 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
             phase = -1;
             set_grids(b, b->tuned - 1);
             b->graph_key = -1;
+            c->tuned_grid[shape] = b->tuned;
         }
         rc = pull();
         if (rc) return rc;

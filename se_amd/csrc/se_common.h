@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -57,6 +58,9 @@ struct se_ctx {
     size_t h_stage_bytes = 0;
     se::DevBuf d_stage;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // launch shapes chosen by the batched solver's autotuner, per
+    // (order, log2 of the work items): later batches of that shape reuse them
+    std::map<std::pair<int, int>, int> tuned_grid;
 };
 
 struct se_mesh {

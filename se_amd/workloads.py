@@ -183,3 +183,17 @@ def qg2_stream(n_requests=200, rate=2000.0, seed=5, N=6):
                          seed=seed * 100003 + i, F1=200.0, F2=200.0, beta=1.0, lo=(0.0, 0.0), hi=(1.0, 1.0),
                          periodic=(1, 0), bc=(1, 1, 0, 0)))
     return reqs
+
+
+def qg2_initial(req, X, Y):
+    """Initial two-layer state of a stream request on the vertex lines X, Y:
+    psi_1, psi_2 seeded Gaussian vortices times sin(pi y / L_y) (zero on the
+    walls), omega_1 = omega_2 = 0."""
+    rng = Rng(req["seed"])
+    lo, hi = req["lo"], req["hi"]
+    wall = np.sin(np.pi * (np.asarray(Y) - lo[1]) / (hi[1] - lo[1]))
+    win = np.repeat(wall, len(X))
+    p1 = gaussian_vortices(X, Y, 10, rng, lo, hi) * win
+    p2 = 0.5 * gaussian_vortices(X, Y, 10, rng, lo, hi) * win
+    z = np.zeros_like(p1)
+    return p1, z, p2, z.copy()
