@@ -12,6 +12,11 @@ Workloads (--workload):
       the same members once and reported beside it.
   helmholtz_128 -- one solve on 128x128 elements at p = 8 (1 048 576
       unknowns), lambda = 100, same kind of right-hand side.
+  qg2_stream -- served two-layer QG step requests (Poisson arrivals at
+      2000/s, m^2 elements with m log-uniform in [16, 96] at p = 6, 1..50 steps
+      each) on the request server; value = unknowns of all inversions / wall
+      time, with p50/p99 latency and steps/s in the config, beside the same
+      stream served with plain FP64 inversions.
 
 value = unknowns solved / second (GDOF/s to solution), with every input
 resident in HBM before the timed region.  The JSON line carries the roofline
@@ -49,6 +54,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
     p.add_argument("--probe-reps", type=int, default=50)
+    p.add_argument("--requests", type=int, default=32, help="qg2_stream: number of requests")
+    p.add_argument("--budget", type=float, default=200e9, help="qg2_stream: server memory budget (bytes)")
     return p.parse_args()
 
 
@@ -273,6 +280,71 @@ def run_single(a, prec_name):
     return base_line(a, value, ms, DTYPE[prec_name], config, roof, cpu)
 
 
+def run_stream(a, prec_name):
+    """Served two-layer QG step requests arriving as a Poisson process."""
+    import numpy as np
+
+    import se_amd
+    from se_amd import workloads as W
+    prec = {"fp64": se_amd.PREC_FP64, "fp32_ir": se_amd.PREC_FP32_IR, "bf16_ir": se_amd.PREC_BF16_IR}[prec_name]
+    reqs = W.qg2_stream(n_requests=a.requests)
+    ctx = se_amd.Context(0)
+    abc = {}
+    for r in reqs:
+        if r["N"] not in abc:
+            m = se_amd.Mesh(ctx, (2, 2), r["N"], (0, 0), (1, 1), (0, 0), (0, 0, 0, 0))
+            abc[r["N"]] = m.gll()[0]
+            m.close()
+        X, Y = W.mesh_coords(r["nEls"], r["N"], r["lo"], r["hi"], abc[r["N"]])
+        r["init"] = W.qg2_initial(r, X, Y)
+    ctx.close()
+
+    def serve(p):
+        sv = se_amd.Server(memory_budget=a.budget, precision=p, rtol=1e-10)
+        t0 = time.perf_counter()
+        ids = []
+        for r in reqs:  # submit at the arrival times
+            wait = r["t_arrive"] - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            ids.append(sv.submit(r, *r["init"]))
+        res = []
+        for k, i in enumerate(ids):
+            res.append(sv.wait(i, fields=False)[0])
+            if (k + 1) % 8 == 0 or k + 1 == len(ids):
+                print("[qg2_stream %s] %d/%d requests done, %.1f s" % (
+                    {se_amd.PREC_FP64: "fp64", se_amd.PREC_FP32_IR: "fp32", se_amd.PREC_BF16_IR: "bf16"}[p],
+                    k + 1, len(ids), time.perf_counter() - t0), file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t0
+        st = sv.stats()
+        sv.close()
+        lat = np.array([x["t_done"] - x["t_submit"] for x in res])
+        ok = all(x["status"] == 0 for x in res)
+        return dict(wall=wall, lat=lat, ok=ok, steps=sum(x["steps_done"] for x in res),
+                    unknowns=sum(x["unknowns"] for x in res), ticks=st["ticks"],
+                    iters=sum(x["inner_iters"] for x in res))
+
+    for _ in range(a.warmup):
+        serve(prec)
+    runs = [serve(prec) for _ in range(a.steps)]
+    r = min(runs, key=lambda x: x["wall"])
+    value = r["unknowns"] / r["wall"] / 1e9
+    config = {"workload": "qg2_stream_%d" % a.requests, "requests": a.requests, "rate_per_s": 2000.0,
+              "mesh": "m^2 elements, m log-uniform in [16, 96], p = 6, x-periodic channel", "precision": prec_name,
+              "rtol": 1e-10, "memory_budget_bytes": a.budget, "steps_served": r["steps"], "ticks": r["ticks"],
+              "steps_per_s": r["steps"] / r["wall"], "latency_p50_s": float(np.percentile(r["lat"], 50)),
+              "latency_p99_s": float(np.percentile(r["lat"], 99)), "wall_s": r["wall"], "all_converged": r["ok"],
+              "inner_iterations": r["iters"]}
+    if not a.no_fp64 and prec != se_amd.PREC_FP64:
+        r64 = serve(se_amd.PREC_FP64)
+        config.update({"fp64_gpu_value": r64["unknowns"] / r64["wall"] / 1e9,
+                       "fp64_steps_per_s": r64["steps"] / r64["wall"],
+                       "fp64_latency_p50_s": float(np.percentile(r64["lat"], 50)),
+                       "fp64_latency_p99_s": float(np.percentile(r64["lat"], 99)),
+                       "speedup_vs_fp64_gpu": r64["wall"] / r["wall"]})
+    return base_line(a, value, r["wall"] * 1e3, DTYPE[prec_name], config, None, None)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -284,6 +356,8 @@ def main():
         if prec == "fp64":
             raise SystemExit("ensemble_512 runs batched mixed precision; its FP64 comparison is part of the line")
         out = run_ensemble(a, prec)
+    elif a.workload.startswith("qg2_stream"):
+        out = run_stream(a, a.precision or "fp32_ir")
     elif a.workload.startswith("helmholtz"):
         out = run_single(a, a.precision or "fp64")
     else:
