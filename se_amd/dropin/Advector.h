@@ -1,0 +1,25 @@
+// Drop-in replacement for the reference's src/base/Advector.h: first-order
+// (two-argument constructor) and second-order (four-argument constructor)
+// semi-Lagrangian advection of `field` by `velocity` over dt into `fieldSL`
+// (src/base/Advector.cpp:26-349), run by libse_amd on the GPU.  Include after
+// the reference's Field.h.
+#ifndef SE_DROPIN_ADVECTOR_H
+#define SE_DROPIN_ADVECTOR_H
+
+class Field;
+
+class Advector {
+   public:
+    Advector(Field* _field, Field* _velocity);
+    Advector(Field* _field, Field* _velocity, Field* _fieldMinusOne, Field* _velMinusOne);
+    ~Advector();
+    void Advect(double dt);
+    Field* field;
+    Field* velocity;
+    Field* fieldMinusOne;
+    Field* velMinusOne;
+    Field* fieldSL;
+    bool secondOrder;
+};
+
+#endif

@@ -1,0 +1,214 @@
+// se_dropin.cpp -- HelmholtzEqn, PoissonEqn and Advector with the reference's
+// interface, on libse_amd.  Compiled inside the reference tree (its
+// src/base headers on the include path) in place of src/linsys/HelmholtzEqn.cpp,
+// src/linsys/PoissonEqn.cpp and src/base/Advector.cpp; links libse_amd.so.
+//
+// The reference's Mesh / BCs / Field stay the callers' objects.  Each distinct
+// (Mesh, Dirichlet sides) pair gets one device mesh (numbering maps and
+// geometry built on the GPU, kept for the process), and the field values are
+// gathered from / scattered to Field::vals around each call.  Errors print to
+// cerr and exit(1), as the reference does on misuse.
+#include <cmath>
+#include <cstdlib>
+#include <iostream>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "QuadPoint.h"
+#include "LinAlg.h"
+#include "Jacobi.h"
+#include "Element.h"
+#include "Mesh.h"
+#include "BCs.h"
+#include "Field.h"
+
+#include "Advector.h"
+#include "HelmholtzEqn.h"
+#include "PoissonEqn.h"
+#include "se_amd.h"
+
+namespace {
+
+std::mutex g_mu;
+se_ctx* g_ctx = nullptr;
+
+struct MeshEntry {
+    const Mesh* mesh;
+    int bc[4];
+    se_mesh* m;
+};
+std::vector<MeshEntry> g_meshes;
+
+void die(const std::string& what, int rc) {
+    std::cerr << "libse_amd: " << what << " failed (" << rc << "): " << se_last_error() << std::endl;
+    exit(1);
+}
+
+se_ctx* ctx() {
+    if (!g_ctx) {
+        const char* d = getenv("SE_DEVICE");
+        int rc = se_ctx_create(d ? atoi(d) : 0, &g_ctx);
+        if (rc) die("se_ctx_create", rc);
+    }
+    return g_ctx;
+}
+
+// the device mesh of a field: geometry of its Mesh, Dirichlet sides of its BCs (dof 0)
+se_mesh* device_mesh(const Field* f) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    const Mesh* M = f->mesh;
+    int bc[4] = {0, 0, 0, 0};
+    if (f->bcs) {
+        // BCs marks the duplicate line of a periodic direction as a boundary
+        // (BCs.cpp:38-44); those are not Dirichlet sides
+        bc[0] = M->periodic[1] ? 0 : f->bcs->bcBottom[0];
+        bc[1] = M->periodic[1] ? 0 : f->bcs->bcTop[0];
+        bc[2] = M->periodic[0] ? 0 : f->bcs->bcLeft[0];
+        bc[3] = M->periodic[0] ? 0 : f->bcs->bcRight[0];
+    }
+    for (const MeshEntry& e : g_meshes)
+        if (e.mesh == M && e.bc[0] == bc[0] && e.bc[1] == bc[1] && e.bc[2] == bc[2] && e.bc[3] == bc[3]) return e.m;
+    int nEls[2] = {M->nEls[0], M->nEls[1]};
+    int per[2] = {M->periodic[0] ? 1 : 0, M->periodic[1] ? 1 : 0};
+    double lo[2] = {M->min[0], M->min[1]}, hi[2] = {M->max[0], M->max[1]};
+    se_mesh* m = nullptr;
+    int rc = se_mesh_create(ctx(), nEls, M->N, lo, hi, per, bc, &m);
+    if (rc) die("se_mesh_create", rc);
+    g_meshes.push_back(MeshEntry{M, {bc[0], bc[1], bc[2], bc[3]}, m});
+    return m;
+}
+
+void gather(const Field* f, std::vector<double>& out) {
+    const int nV = f->mesh->nVertsTotal, nd = f->nDofs;
+    out.resize((size_t)nV * nd);
+    for (int v = 0; v < nV; v++)
+        for (int d = 0; d < nd; d++) out[(size_t)v * nd + d] = f->vals[v][d];
+}
+
+void scatter(const std::vector<double>& in, Field* f) {
+    const int nV = f->mesh->nVertsTotal, nd = f->nDofs;
+    for (int v = 0; v < nV; v++)
+        for (int d = 0; d < nd; d++) f->vals[v][d] = in[(size_t)v * nd + d];
+}
+
+se_solve_opts solve_opts(bool null_space) {
+    se_solve_opts o;
+    se_solve_opts_default(&o);
+    if (const char* r = getenv("SE_RTOL")) o.rtol = atof(r);
+    if (const char* p = getenv("SE_PRECISION")) {
+        const std::string s(p);
+        if (s == "fp32") o.precision = SE_PREC_FP32_IR;
+        if (s == "bf16") o.precision = SE_PREC_BF16_IR;
+    }
+    o.null_space = null_space ? 1 : 0;
+    return o;
+}
+
+void check_scalar(const Field* f, const char* who) {
+    if (f->nDofs != 1) {
+        std::cerr << who << ": only single-dof fields are solved by libse_amd" << std::endl;
+        exit(1);
+    }
+}
+
+// Field-level solve shared by both equations: f = M rhs + b, solve, UpdateField
+void field_solve(se_op* op, Field* sol, Field* rhs, bool null_space, se_solve_info* info) {
+    std::vector<double> r, s;
+    gather(rhs, r);
+    gather(sol, s);
+    se_solve_opts o = solve_opts(null_space);
+    int rc = se_helm_solve(op, r.data(), s.data(), &o, info);
+    if (rc) die("se_helm_solve", rc);
+    if (info->status != SE_SOLVE_CONVERGED)
+        std::cerr << "libse_amd: solve ended with status " << info->status << " after " << info->iters
+                  << " iterations, relative residual " << info->true_rel_residual << std::endl;
+    scatter(s, sol);
+}
+
+}  // namespace
+
+// ---- HelmholtzEqn (src/linsys/HelmholtzEqn.cpp:21-136) ----------------------
+HelmholtzEqn::HelmholtzEqn(Field* _solField, Field* _rhsField, double _c, double _nu, double _dt, int _svvCutoff)
+    : solField(_solField), rhsField(_rhsField), c(_c), nu(_nu), dt(_dt), svvCutoff(_svvCutoff), info(),
+      assembled(false), op(nullptr) {
+    check_scalar(solField, "HelmholtzEqn");
+    if (svvCutoff) {
+        std::cerr << "HelmholtzEqn: the spectral vanishing viscosity operator is not provided by libse_amd"
+                  << std::endl;
+        exit(1);
+    }
+    int rc = se_helm_create(device_mesh(solField), c, nu * dt, SE_DETJ_LITERAL, &op);
+    if (rc) die("se_helm_create", rc);
+}
+
+HelmholtzEqn::~HelmholtzEqn() { se_helm_destroy(op); }
+
+void HelmholtzEqn::Assemble() {
+    std::vector<double> s;
+    gather(solField, s);
+    int rc = se_helm_set_dirichlet(op, s.data());
+    if (rc) die("se_helm_set_dirichlet", rc);
+    assembled = true;
+}
+
+void HelmholtzEqn::Solve(std::string name, bool removeNullSpace) {
+    (void)name;  // PETSc options prefix in the reference; options come from SE_* here
+    if (!assembled) Assemble();
+    field_solve(op, solField, rhsField, removeNullSpace, &info);
+}
+
+// ---- PoissonEqn (src/linsys/PoissonEqn.cpp:30-134) ----------------------------
+PoissonEqn::PoissonEqn(Field* _solField, Field* _rhsField, double _c)
+    : solField(_solField), rhsField(_rhsField), c(_c), info(), assembled(false), op(nullptr) {
+    check_scalar(solField, "PoissonEqn");
+    int rc = se_helm_create(device_mesh(solField), 0.0, c, SE_DETJ_LITERAL, &op);
+    if (rc) die("se_helm_create", rc);
+}
+
+PoissonEqn::~PoissonEqn() { se_helm_destroy(op); }
+
+void PoissonEqn::Assemble() {
+    std::vector<double> s;
+    gather(solField, s);
+    int rc = se_helm_set_dirichlet(op, s.data());
+    if (rc) die("se_helm_set_dirichlet", rc);
+    assembled = true;
+}
+
+void PoissonEqn::Solve(bool removeNullSpace) {
+    if (!assembled) Assemble();
+    field_solve(op, solField, rhsField, removeNullSpace, &info);
+}
+
+// ---- Advector (src/base/Advector.cpp:26-349) ----------------------------------
+Advector::Advector(Field* _field, Field* _velocity)
+    : field(_field), velocity(_velocity), fieldMinusOne(nullptr), velMinusOne(nullptr), secondOrder(false) {
+    fieldSL = new Field(field->name + "SL", field->mesh, field->nDofs, field->bcs);
+}
+
+Advector::Advector(Field* _field, Field* _velocity, Field* _fieldMinusOne, Field* _velMinusOne)
+    : field(_field), velocity(_velocity), fieldMinusOne(_fieldMinusOne), velMinusOne(_velMinusOne),
+      secondOrder(true) {
+    fieldSL = new Field(field->name + "SL", field->mesh, field->nDofs, field->bcs);
+}
+
+Advector::~Advector() { delete fieldSL; }
+
+void Advector::Advect(double dt) {
+    se_mesh* m = device_mesh(field);
+    std::vector<double> q, v, out((size_t)field->mesh->nVertsTotal * field->nDofs);
+    gather(field, q);
+    gather(velocity, v);
+    int rc;
+    if (secondOrder) {
+        std::vector<double> qp, vp;
+        gather(fieldMinusOne, qp);
+        gather(velMinusOne, vp);
+        rc = se_sl_advect2(m, q.data(), qp.data(), field->nDofs, v.data(), vp.data(), dt, out.data());
+    } else {
+        rc = se_sl_advect(m, q.data(), field->nDofs, v.data(), dt, out.data(), nullptr);
+    }
+    if (rc) die("se_sl_advect", rc);
+    scatter(out, fieldSL);
+}

@@ -1,0 +1,130 @@
+// Drop-in check: the reference's own Mesh / BCs / Field objects (compiled from
+// the reference tree) as the caller, the drop-in HelmholtzEqn / PoissonEqn /
+// Advector (se_amd/dropin) as the solver, the C oracle as the checker.  One
+// QuasiGeostrophicEqn-style sequence on an x-periodic channel
+// (src/qg/QuasiGeostrophicEqn.cpp:48-52,117-147 pattern): Helmholtz inversion
+// with Dirichlet walls, first- and second-order advection, a Poisson solve.
+// Prints "dropin ok" and exits 0 when every comparison passes.
+#include <cmath>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "QuadPoint.h"
+#include "LinAlg.h"
+#include "Jacobi.h"
+#include "Element.h"
+#include "Mesh.h"
+#include "BCs.h"
+#include "Field.h"
+
+#include "Advector.h"
+#include "HelmholtzEqn.h"
+#include "PoissonEqn.h"
+#include "se_oracle.h"
+
+static int failures = 0;
+
+static void compare(const char* what, const std::vector<double>& a, const std::vector<double>& b, double tol) {
+    double d = 0.0, nb = 0.0;
+    for (size_t i = 0; i < a.size(); i++) {
+        d += (a[i] - b[i]) * (a[i] - b[i]);
+        nb += b[i] * b[i];
+    }
+    const double rel = std::sqrt(d / (nb > 0 ? nb : 1.0));
+    const bool ok = rel <= tol;
+    std::printf("%-28s rel diff %.3e (bar %.0e) %s\n", what, rel, tol, ok ? "ok" : "FAIL");
+    if (!ok) failures++;
+}
+
+static std::vector<double> values(const Field* f) {
+    std::vector<double> v((size_t)f->mesh->nVertsTotal * f->nDofs);
+    for (int i = 0; i < f->mesh->nVertsTotal; i++)
+        for (int d = 0; d < f->nDofs; d++) v[(size_t)i * f->nDofs + d] = f->vals[i][d];
+    return v;
+}
+
+int main() {
+    int nEls[2] = {8, 6};
+    double lo[2] = {0.0, 0.0}, hi[2] = {1.0, 1.0};
+    bool per[2] = {true, false};
+    const int N = 6;
+    Mesh* mesh = new Mesh("mesh", nEls, "legendre", N, lo, hi, per);
+    BCs* bcs = new BCs(true, true, false, false, mesh, 0);
+    Field* psi = new Field("psi", mesh, 1, bcs);
+    Field* rhs = new Field("rhs", mesh, 1, NULL);
+    Field* q = new Field("q", mesh, 1, NULL);
+    Field* qPrev = new Field("qPrev", mesh, 1, NULL);
+    Field* vel = new Field("vel", mesh, 2, NULL);
+    Field* velPrev = new Field("velPrev", mesh, 2, NULL);
+    const int nV = mesh->nVertsTotal;
+    for (int i = 0; i < nV; i++) {
+        const double x = mesh->verts[i][0], y = mesh->verts[i][1];
+        psi->vals[i][0] = 0.0;  // walls: psi = 0
+        rhs->vals[i][0] = std::cos(2.0 * M_PI * x) * std::sin(M_PI * y) + 0.3 * std::sin(4.0 * M_PI * x);
+        q->vals[i][0] = std::exp(-40.0 * ((x - 0.4) * (x - 0.4) + (y - 0.5) * (y - 0.5)));
+        qPrev->vals[i][0] = 0.9 * q->vals[i][0];
+        vel->vals[i][0] = 0.3 * std::sin(M_PI * y);
+        vel->vals[i][1] = 0.1 * std::sin(2.0 * M_PI * x) * std::sin(M_PI * y);
+        velPrev->vals[i][0] = 0.95 * vel->vals[i][0];
+        velPrev->vals[i][1] = 0.95 * vel->vals[i][1];
+    }
+    const int nE[2] = {nEls[0], nEls[1]}, pi[2] = {1, 0}, bc[4] = {1, 1, 0, 0};
+    int st = 0;
+    orc_mesh* om = orc_mesh_create(nE, N, lo, hi, pi, bc, &st);
+    std::vector<double> verts(2 * (size_t)nV), r = values(rhs), s0 = values(psi);
+
+    // Helmholtz inversion (QG: c = -F, nu = -1, dt = 1)
+    HelmholtzEqn* he = new HelmholtzEqn(psi, rhs, -100.0, -1.0, 1.0, 0);
+    he->Assemble();
+    he->Solve("helm_");
+    orc_helm* oh = orc_helm_create(om, -100.0, -1.0, 0);
+    orc_helm_assemble(oh, s0.data());
+    int it = 0;
+    double rr = 0.0;
+    std::vector<double> s = s0;
+    orc_helm_solve(oh, r.data(), s.data(), 1e-10, 100000, 0, &it, &rr);
+    std::printf("HelmholtzEqn: status %d, %d iterations (oracle %d)\n", he->info.status, he->info.iters, it);
+    if (he->info.status != 0) failures++;
+    compare("HelmholtzEqn::Solve", values(psi), s, 1e-8);
+
+    // first-order advection
+    Advector* a1 = new Advector(q, vel);
+    a1->Advect(0.01);
+    std::vector<double> qs(nV), qv = values(q), vv = values(vel);
+    orc_advect_first(om, qv.data(), 1, vv.data(), 0.01, qs.data(), NULL, NULL);
+    compare("Advector::Advect (1st)", values(a1->fieldSL), qs, 1e-12);
+
+    // second-order advection
+    Advector* a2 = new Advector(q, vel, qPrev, velPrev);
+    a2->Advect(0.01);
+    std::vector<double> qp = values(qPrev), vp = values(velPrev);
+    orc_advect_second(om, qv.data(), qp.data(), 1, vv.data(), vp.data(), 0.01, qs.data());
+    compare("Advector::Advect (2nd)", values(a2->fieldSL), qs, 1e-12);
+
+    // Poisson (barotropic: c = -1, Dirichlet walls)
+    Field* chi = new Field("chi", mesh, 1, bcs);
+    PoissonEqn* pe = new PoissonEqn(chi, rhs, -1.0);
+    pe->Solve(false);
+    orc_helm* op = orc_helm_create(om, 0.0, -1.0, 0);
+    std::vector<double> c0(nV, 0.0);
+    orc_helm_assemble(op, c0.data());
+    orc_helm_solve(op, r.data(), c0.data(), 1e-10, 100000, 0, &it, &rr);
+    std::printf("PoissonEqn: status %d, %d iterations (oracle %d)\n", pe->info.status, pe->info.iters, it);
+    if (pe->info.status != 0) failures++;
+    compare("PoissonEqn::Solve", values(chi), c0, 1e-8);
+
+    delete he;
+    delete pe;
+    delete a1;
+    delete a2;
+    orc_helm_destroy(oh);
+    orc_helm_destroy(op);
+    orc_mesh_destroy(om);
+    if (failures) {
+        std::printf("dropin FAILED (%d)\n", failures);
+        return 1;
+    }
+    std::printf("dropin ok\n");
+    return 0;
+}
