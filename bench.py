@@ -112,6 +112,35 @@ def base_line(a, value, ms, dtype, config, roofline, cpu):
             "roofline": roofline, "cpu_baseline": cpu}
 
 
+COPY_BYTES = 1 << 30  # yardstick copy: 1 GiB each way, far beyond the 256 MB last-level cache
+
+
+def copy_yardstick(ctx, reps=5):
+    """A plain device copy of known size, timed like the roofline kernel: the
+    attainable streaming bandwidth, and the reference for the byte counters."""
+    us = ctx.copy_probe(COPY_BYTES, reps)
+    return {"kernel": "k_copy_probe", "bytes_read": COPY_BYTES, "bytes_written": COPY_BYTES,
+            "us_per_launch": us, "achieved": 2.0 * COPY_BYTES / (us * 1e-6) / 1e9}
+
+
+def pmc_traffic(roof, profile):
+    """HBM bytes per launch of the roofline kernel from its committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by
+    what the counters read for the copy yardstick); None when the summary is
+    absent or was taken on another kernel."""
+    path = os.path.join(ROOT, "profiles", profile)
+    if not os.path.exists(path):
+        return None
+    fields = {}
+    for line in open(path):
+        if ":" in line and not line.startswith(("|", "#", " ")):
+            k, v = line.split(":", 1)
+            fields[k.strip("- `")] = v.strip().strip("`")
+    if fields.get("kernel") != roof["kernel"] or "traffic_bytes_per_launch" not in fields:
+        return None
+    return float(fields["traffic_bytes_per_launch"])
+
+
 def mixed_roofline(kernel, pr):
     """pr: us_dir, us_upd, bytes_dir, bytes_upd of the inner iteration kernels."""
     ach = pr["bytes_dir"] / (pr["us_dir"] * 1e-6) / 1e9
@@ -167,6 +196,8 @@ def run_ensemble(a, prec_name):
     T = "float" if prec == se_amd.PREC_FP32_IR else "bf16"
     roof = mixed_roofline("k_tcg_dir_apply_batch<%d, %s, NORMAL>" % (gmax["order"], T), gmax)
     roof["groups"] = groups
+    roof["traffic"] = pmc_traffic(roof, "k_tcg_dir_apply_batch__ensemble_512_%s_pmc.md" % prec_name.split("_")[0])
+    roof["copy_yardstick"] = copy_yardstick(ctx)
     config = {"workload": "ensemble_512" if a.members == 512 else "ensemble_%d" % a.members,
               "members": len(hs), "unknowns": ntot, "precision": prec_name, "detj": a.detj, "rtol": 1e-10,
               "iters_min_median_max": [int(its.min()), int(np.median(its)), int(its.max())],
@@ -262,13 +293,17 @@ def run_single(a, prec_name):
         pr = h.probe(prec, reps=a.probe_reps)
         T = "float" if prec == se_amd.PREC_FP32_IR else "bf16"
         roof = mixed_roofline("k_tcg_dir_apply<%d, %s, NORMAL>" % (m.N, T), pr)
-        if not a.no_fp64:
-            d_x64 = torch.empty_like(d_x)
-            ms64, _, r64 = timed(ctx, stream, 1, lambda: solve(se_amd.PREC_FP64, d_x64))
-            config.update({"fp64_gpu_value": m.n / (ms64 * 1e-3) / 1e9, "fp64_gpu_ms": ms64,
-                           "speedup_vs_fp64_gpu": ms64 / ms,
-                           "max_rel_diff_vs_fp64_solution":
-                               float(torch.linalg.norm(d_x - d_x64) / torch.linalg.norm(d_x64))})
+    if cfg["workload"] == "helmholtz_128":
+        roof["traffic"] = pmc_traffic(roof, "%s__helmholtz_128_%s_pmc.md" % (
+            roof["kernel"].split("<")[0], prec_name.split("_")[0]))
+    roof["copy_yardstick"] = copy_yardstick(ctx)
+    if prec != se_amd.PREC_FP64 and not a.no_fp64:
+        d_x64 = torch.empty_like(d_x)
+        ms64, _, r64 = timed(ctx, stream, 1, lambda: solve(se_amd.PREC_FP64, d_x64))
+        config.update({"fp64_gpu_value": m.n / (ms64 * 1e-3) / 1e9, "fp64_gpu_ms": ms64,
+                       "speedup_vs_fp64_gpu": ms64 / ms,
+                       "max_rel_diff_vs_fp64_solution":
+                           float(torch.linalg.norm(d_x - d_x64) / torch.linalg.norm(d_x64))})
     cpu = None
     if not a.no_cpu_baseline:
         s = cpu_sample(cfg, d_f.cpu().numpy(), info["iters"], a.cpu_iters)

@@ -71,6 +71,11 @@ int se_ctx_create(int device, se_ctx** out);
 int se_ctx_destroy(se_ctx* ctx);
 int se_ctx_synchronize(se_ctx* ctx);
 void* se_ctx_stream(se_ctx* ctx); /* the context's hipStream_t */
+/* Average time (us) of one plain device copy of `bytes` (src and dst of that
+ * size, allocated and freed by the call), over `reps` launches.  No
+ * counterpart in the reference: the yardstick for the HBM byte counters and
+ * the attainable copy bandwidth in the measurements. */
+int se_ctx_copy_probe(se_ctx* ctx, size_t bytes, int reps, double* us);
 
 /* ---- mesh and global numbering --------------------------------------------
  * [ref: Mesh::Mesh src/base/Mesh.cpp:17-84 with elType "legendre";

@@ -90,6 +90,7 @@ def lib():
             "se_ctx_destroy": (i, [vp]),
             "se_ctx_synchronize": (i, [vp]),
             "se_ctx_stream": (vp, [vp]),
+            "se_ctx_copy_probe": (i, [vp, C.c_size_t, i, C.POINTER(d)]),
             "se_mesh_create": (i, [vp, _I, i, _D, _D, _I, _I, pvp]),
             "se_mesh_destroy": (i, [vp]),
             "se_mesh_get_info": (i, [vp, C.POINTER(MeshInfo)]),
@@ -191,6 +192,13 @@ class Context:
 
     def synchronize(self):
         check(lib().se_ctx_synchronize(self.h), "se_ctx_synchronize")
+
+    def copy_probe(self, nbytes, reps=10):
+        """Average microseconds of one plain device copy of nbytes (the
+        yardstick for the HBM byte counters)."""
+        us = C.c_double()
+        check(lib().se_ctx_copy_probe(self.h, int(nbytes), int(reps), C.byref(us)), "se_ctx_copy_probe")
+        return us.value
 
     def close(self):
         """Destroy the context and, first, everything still alive that was

@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <mutex>
 
 #include "se_common.h"
@@ -263,6 +264,45 @@ extern "C" int se_ctx_synchronize(se_ctx* c) {
 }
 
 extern "C" void* se_ctx_stream(se_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+// Plain streaming copy of known size: the yardstick for the HBM byte counters
+// (FETCH_SIZE / WRITE_SIZE) and the attainable copy bandwidth.
+static __global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ in, float4* __restrict__ out,
+                                                          size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) out[i] = in[i];
+}
+
+extern "C" int se_ctx_copy_probe(se_ctx* c, size_t bytes, int reps, double* us) {
+    SE_CHECK(c && us, SE_ERR_ARG, "se_ctx_copy_probe: NULL argument");
+    SE_CHECK(reps > 0 && bytes >= 16 && bytes % 16 == 0, SE_ERR_ARG,
+             "se_ctx_copy_probe: reps must be positive and bytes a positive multiple of 16");
+    SE_HIP(hipSetDevice(c->device));
+    se::DevBuf a, b;
+    if (int rc = a.alloc(bytes)) return rc;
+    if (int rc = b.alloc(bytes)) {
+        a.release();
+        return rc;
+    }
+    const size_t n = bytes / 16;
+    const int grid = (int)std::min<size_t>((n + 255) / 256, (size_t)c->cu_count * 32);
+    hipError_t e = hipMemsetAsync(a.ptr, 0, bytes, c->stream);
+    hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, c->stream, a.as<const float4>(), b.as<float4>(), n);
+    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+    for (int i = 0; i < reps && e == hipSuccess; i++) {
+        hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, c->stream, a.as<const float4>(), b.as<float4>(), n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    (void)hipStreamSynchronize(c->stream);
+    a.release();
+    b.release();
+    if (e != hipSuccess) return se::fail(SE_ERR_HIP, std::string("se_ctx_copy_probe: ") + hipGetErrorString(e));
+    *us = 1e3 * (double)ms / reps;
+    return SE_OK;
+}
 
 // ---------------------------------------------------------------------------
 // C ABI: mesh

@@ -154,281 +154,368 @@ struct TcgTab {
 // k_tcg_dir_apply
 // ---------------------------------------------------------------------------
 template <int N, typename T, int MODE>
-__device__ __forceinline__ void tcg_dir_apply_tile(const TcgMember& M, int tile) {
-    constexpr bool FIRST = MODE == TCG_FIRST, NOP = MODE != TCG_NORMAL;  // NOP: the old p is not used
-    constexpr int B = 2 * N + 1;
-    constexpr int TY = TCG_TY, NT = TCG_NT, NW = NT / 64, RPT = TCG_RPT;
-    constexpr int W = SEP_TX + 2 * N, H = TY + 2 * N, WP = W + 1;
-    constexpr int RL = (H + NW - 1) / NW;           // main-column rows per wave
-    constexpr int NX = (H * 2 * N + NT - 1) / NT;   // right-halo cells per thread
-    const SepParams& S = M.S;
-    const TcgTab tb(S);
-    cgp<float> r = (cgp<float>)M.r;
-    cgp<T> p = (cgp<T>)M.p;
-    gp<float> q = (gp<float>)M.q;
-    const int np = M.tiles;
-    CgState* __restrict__ st = M.st;
-    constexpr int NG = (TY * B + NT - 1) / NT;
-    __shared__ float s_u[H * WP];
-    __shared__ float s_gy[TY * B];  // row bands of the owned rows
+struct TcgDir {
+    static constexpr bool FIRST = MODE == TCG_FIRST, NOP = MODE != TCG_NORMAL;  // NOP: the old p is not used
+    static constexpr int B = 2 * N + 1;
+    static constexpr int TY = TCG_TY, NT = TCG_NT, NW = NT / 64, RPT = TCG_RPT;
+    static constexpr int W = SEP_TX + 2 * N, H = TY + 2 * N, WP = W + 1;
+    static constexpr int RL = (H + NW - 1) / NW;          // main-column rows per wave
+    static constexpr int NX = (H * 2 * N + NT - 1) / NT;  // right-halo cells per thread
+    static constexpr int NG = (TY * B + NT - 1) / NT;
+    static_assert(H + W <= NT, "one thread per halo row and column");
 
-    const TcgTile t(S, tile);
-    const int lane = t.lane, wv = t.wv, I0 = t.I0, J0 = t.J0;
-
-    // ---- every global load first ----
-    float gyv[NG];
-#pragma unroll
-    for (int k = 0; k < NG; k++) {
-        const int i = threadIdx.x + NT * k;
-        const bool ok = i < TY * B && J0 + i / B < S.nU1;
-        const float v = tb.gy[ok ? (size_t)J0 * B + i : 0];
-        gyv[k] = ok ? v : 0.0f;
-    }
-    // main columns: lane l holds tile column l of rows wv + NW*k
-    int Ic0 = I0 - N + lane;
-    if (S.per0) Ic0 = Ic0 < 0 ? Ic0 + S.nU0 : (Ic0 >= S.nU0 ? Ic0 - S.nU0 : Ic0);
-    const bool c0 = Ic0 >= 0 && Ic0 < S.nU0;
-    Ic0 = c0 ? Ic0 : 0;
-    float r0[RL], p0[RL], rg[RL], rs[RL], rm[RL];
-    bool rok[RL];
-#pragma unroll
-    for (int k = 0; k < RL; k++) {
-        const int a = wv + NW * k;
-        int J = J0 - N + a;
-        if (S.per1) J = J < 0 ? J + S.nU1 : (J >= S.nU1 ? J - S.nU1 : J);
-        rok[k] = a < H && J >= 0 && J < S.nU1;
-        const int Jc = rok[k] ? J : 0;
-        const size_t rowo = (size_t)Jc * S.nU0;
-        r0[k] = r[rowo + Ic0];
-        if (!NOP) p0[k] = ldf<T>(p, rowo + Ic0);
-        rg[k] = tb.gy[(size_t)Jc * B + N];  // wave-uniform: scalar loads
-        rs[k] = tb.sy[Jc];
-        rm[k] = tb.my[Jc];
-    }
-    const float cg0 = tb.gx[(size_t)Ic0 * B + N], cs0 = tb.sx[Ic0], cm0 = tb.mx[Ic0];
-    // right-halo columns 64 .. 64+2N-1, spread over all threads
-    float rx[NX], px[NX], dx[NX];
-    bool okx[NX];
-#pragma unroll
-    for (int k = 0; k < NX; k++) {
-        const int i = threadIdx.x + NT * k;
-        const int a = i / (2 * N), cc = 64 + i % (2 * N);
-        int J = J0 - N + a, I = I0 - N + cc;
-        if (S.per1) J = J < 0 ? J + S.nU1 : (J >= S.nU1 ? J - S.nU1 : J);
-        if (S.per0) I = I >= S.nU0 ? I - S.nU0 : I;
-        okx[k] = i < H * 2 * N && J >= 0 && J < S.nU1 && I >= 0 && I < S.nU0;
-        const int Jc = okx[k] ? J : 0, Icc = okx[k] ? I : 0;
-        const size_t o = (size_t)Jc * S.nU0 + Icc;
-        rx[k] = r[o];
-        px[k] = NOP ? 0.0f : ldf<T>(p, o);
-        dx[k] = tcg_dinv(tb.gx[(size_t)Icc * B + N], tb.sx[Icc], tb.mx[Icc], tb.gy[(size_t)Jc * B + N], tb.sy[Jc],
-                         tb.my[Jc]);
-    }
-    const int Ic = t.colok ? t.I : 0;
-    float gxr[B];
-#pragma unroll
-    for (int s = 0; s < B; s++) gxr[s] = tb.gx[(size_t)Ic * B + s];
-    const float sxI = tb.sx[Ic], mxI = tb.mx[Ic];
-    TcgPart<4> pb;
-    pb.load(M.part_b(), np);
-    TcgPart<2> pt;
-    if (FIRST) pt.load(M.part_t(), np);
-    // the stop flag, read by one thread and shared through LDS, so that the
-    // whole workgroup takes the same branch even if the flag changes meanwhile
-    __shared__ int s_stop;
-    if (threadIdx.x == 0) s_stop = FIRST ? 0 : __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int null_space = st->null_space, iter = st->iter, maxit = st->maxit;
-    const double rho_prev = st->rho_prev, tol2_in = st->tol2;
-
-    // ---- totals, state ----
-    double tbv[4];
-    pb.reduce(M.part_b(), np, tbv);  // (its barriers publish s_stop)
-    if (s_stop) return;
-    const double rr = tbv[0];
-    const bool finite = (rr - rr == 0.0) && (tbv[1] - tbv[1] == 0.0);
-    const double mu = null_space ? tbv[2] / st->n : 0.0;
-    const double rho_new = tbv[1] - mu * tbv[3];
-    const bool lead = tile == 0 && threadIdx.x == 0;  // the member's first tile writes its state
-    double beta = 0.0;
-    if (FIRST) {
-        double tt[2];
-        pt.reduce(M.part_t(), np, tt);
-        const double bb = null_space ? tt[0] - tt[1] * (tt[1] / st->n) : tt[0];
-        const bool fin = finite && (bb - bb == 0.0);
-        if (lead) {
-            st->bb = bb;
-            st->tol2 = M.rtol * M.rtol * bb;
-            st->rr = rr;
-            st->rho = rho_new;
-            st->beta = 0.0;
-            st->mu = mu;
-            st->iter = 0;
-            st->stop = (bb == 0.0 || !fin) ? 1 : 0;
-            st->status = bb == 0.0 ? 0 : (fin ? CG_RUNNING : 3);
+    // ---- the member's totals and state (once per member and workgroup run) --
+    struct Setup {
+        TcgPart<4> pb;
+        TcgPart<2> pt;
+        __device__ __forceinline__ void load(const TcgMember& M) {
+            pb.load(M.part_b(), M.tiles);
+            if (FIRST) pt.load(M.part_t(), M.tiles);
         }
-        if (bb == 0.0 || !fin) return;
-    } else {
-        const bool conv = finite && rr <= tol2_in;
-        const bool over = iter >= maxit;
-        if (lead) {
-            st->rr = rr;
-            if (!finite) {
-                st->stop = 1;
-                st->status = 3;
-            } else if (conv) {
-                st->stop = 1;
-                st->status = CG_PENDING;
-            } else if (over) {
-                st->stop = 1;
-                st->status = 1;
+        // totals of the update kernel's partials -> beta, mu; convergence and
+        // limits; the member's first tile writes the state.  False: no work.
+        __device__ __forceinline__ bool run(const TcgMember& M, int tile, float& fb, float& fmu) {
+            CgState* __restrict__ st = M.st;
+            const int np = M.tiles;
+            // the stop flag, read by one thread and shared through LDS, so that the
+            // whole workgroup takes the same branch even if the flag changes meanwhile
+            __shared__ int s_stop;
+            if (threadIdx.x == 0)
+                s_stop = FIRST ? 0 : __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int null_space = st->null_space, iter = st->iter, maxit = st->maxit;
+            const double rho_prev = st->rho_prev, tol2_in = st->tol2;
+            double tbv[4];
+            pb.reduce(M.part_b(), np, tbv);  // (its barriers publish s_stop)
+            if (s_stop) return false;
+            const double rr = tbv[0];
+            const bool finite = (rr - rr == 0.0) && (tbv[1] - tbv[1] == 0.0);
+            const double mu = null_space ? tbv[2] / st->n : 0.0;
+            const double rho_new = tbv[1] - mu * tbv[3];
+            const bool lead = tile == 0 && threadIdx.x == 0;  // the member's first tile writes its state
+            double beta = 0.0;
+            if (FIRST) {
+                double tt[2];
+                pt.reduce(M.part_t(), np, tt);
+                const double bb = null_space ? tt[0] - tt[1] * (tt[1] / st->n) : tt[0];
+                const bool fin = finite && (bb - bb == 0.0);
+                if (lead) {
+                    st->bb = bb;
+                    st->tol2 = M.rtol * M.rtol * bb;
+                    st->rr = rr;
+                    st->rho = rho_new;
+                    st->beta = 0.0;
+                    st->mu = mu;
+                    st->iter = 0;
+                    st->stop = (bb == 0.0 || !fin) ? 1 : 0;
+                    st->status = bb == 0.0 ? 0 : (fin ? CG_RUNNING : 3);
+                }
+                if (bb == 0.0 || !fin) return false;
             } else {
-                st->rho = rho_new;
-                st->beta = NOP ? 0.0 : rho_new / rho_prev;
-                st->mu = mu;
+                const bool conv = finite && rr <= tol2_in;
+                const bool over = iter >= maxit;
+                if (lead) {
+                    st->rr = rr;
+                    if (!finite) {
+                        st->stop = 1;
+                        st->status = 3;
+                    } else if (conv) {
+                        st->stop = 1;
+                        st->status = CG_PENDING;
+                    } else if (over) {
+                        st->stop = 1;
+                        st->status = 1;
+                    } else {
+                        st->rho = rho_new;
+                        st->beta = NOP ? 0.0 : rho_new / rho_prev;
+                        st->mu = mu;
+                    }
+                }
+                if (!finite || conv || over) return false;
+                beta = NOP ? 0.0 : rho_new / rho_prev;
+            }
+            fmu = (float)mu;
+            fb = (float)beta;
+            return true;
+        }
+    };
+
+    // ---- one tile: its loads (issued first), then the work ------------------
+    float gyv[NG];
+    float4 meta;
+    float r0[RL], p0[RL], rx[NX], px[NX];
+    float gxr[B];
+    float sxI, mxI;
+
+    __device__ __forceinline__ void load(const TcgMember& M, int tile) {
+        const SepParams& S = M.S;
+        const TcgTab tb(S);
+        cgp<float> r = (cgp<float>)M.r;
+        cgp<T> p = (cgp<T>)M.p;
+        const TcgTile t(S, tile);
+        const int lane = t.lane, wv = t.wv, I0 = t.I0, J0 = t.J0;
+#pragma unroll
+        for (int k = 0; k < NG; k++) {
+            const int i = threadIdx.x + NT * k;
+            const bool ok = i < TY * B && J0 + i / B < S.nU1;
+            const float v = tb.gy[ok ? (size_t)J0 * B + i : 0];
+            gyv[k] = ok ? v : 0.0f;
+        }
+        // line factors: thread a < H takes halo row a, thread H + c column c
+        // (per-lane vector loads, all in flight together)
+        meta = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        {
+            const bool isrow = threadIdx.x < H;
+            const int c = threadIdx.x - H;
+            int L = isrow ? J0 - N + (int)threadIdx.x : I0 - N + c;
+            const int nL = isrow ? S.nU1 : S.nU0;
+            if (isrow ? S.per1 : S.per0) L = L < 0 ? L + nL : (L >= nL ? L - nL : L);
+            const bool ok = (isrow || c < W) && L >= 0 && L < nL;
+            const int Lc = ok ? L : 0;
+            if (isrow) {
+                meta = make_float4(tb.gy[(size_t)Lc * B + N], tb.sy[Lc], tb.my[Lc], ok ? 1.0f : 0.0f);
+            } else if (c < W) {
+                meta = make_float4(tb.gx[(size_t)Lc * B + N], tb.sx[Lc], tb.mx[Lc], ok ? 1.0f : 0.0f);
             }
         }
-        if (!finite || conv || over) return;
-        beta = NOP ? 0.0 : rho_new / rho_prev;
+        // main columns: lane l holds tile column l of rows wv + NW*k
+        int Ic0 = I0 - N + lane;
+        if (S.per0) Ic0 = Ic0 < 0 ? Ic0 + S.nU0 : (Ic0 >= S.nU0 ? Ic0 - S.nU0 : Ic0);
+        Ic0 = (Ic0 >= 0 && Ic0 < S.nU0) ? Ic0 : 0;
+#pragma unroll
+        for (int k = 0; k < RL; k++) {
+            const int a = wv + NW * k;
+            int J = J0 - N + a;
+            if (S.per1) J = J < 0 ? J + S.nU1 : (J >= S.nU1 ? J - S.nU1 : J);
+            const bool rok = a < H && J >= 0 && J < S.nU1;
+            const size_t rowo = (size_t)(rok ? J : 0) * S.nU0;
+            r0[k] = r[rowo + Ic0];
+            if (!NOP) p0[k] = ldf<T>(p, rowo + Ic0);
+        }
+        // right-halo columns 64 .. 64+2N-1, spread over all threads
+#pragma unroll
+        for (int k = 0; k < NX; k++) {
+            const int i = threadIdx.x + NT * k;
+            const int a = i / (2 * N), cc = 64 + i % (2 * N);
+            int J = J0 - N + a, I = I0 - N + cc;
+            if (S.per1) J = J < 0 ? J + S.nU1 : (J >= S.nU1 ? J - S.nU1 : J);
+            if (S.per0) I = I >= S.nU0 ? I - S.nU0 : I;
+            const bool ok = i < H * 2 * N && J >= 0 && J < S.nU1 && I >= 0 && I < S.nU0;
+            const size_t o = ok ? (size_t)J * S.nU0 + I : 0;
+            rx[k] = r[o];
+            px[k] = NOP ? 0.0f : ldf<T>(p, o);
+        }
+        const int Ic = t.colok ? t.I : 0;
+#pragma unroll
+        for (int s = 0; s < B; s++) gxr[s] = tb.gx[(size_t)Ic * B + s];
+        sxI = tb.sx[Ic];
+        mxI = tb.mx[Ic];
     }
-    const float fmu = (float)mu, fb = (float)beta;
 
-    // ---- new direction on the tile and its halo ----
+    // p' = z + beta p on the tile and its halo, q = A~ p' on the owned
+    // cells, partial p'.q.  Starts and ends with every thread past its last
+    // LDS access of the previous step (the caller's barrier).
+    __device__ __forceinline__ void compute(const TcgMember& M, int tile, float fb, float fmu) {
+        __shared__ float s_u[H * WP];
+        __shared__ float s_gy[TY * B];  // row bands of the owned rows
+        // diagonal factors of the tile's rows and columns (halo included):
+        // {band centre, line scale, mass factor, 1 inside the domain else 0}
+        __shared__ float4 s_row[H];
+        __shared__ float4 s_col[W];
+        const SepParams& S = M.S;
+        gp<float> q = (gp<float>)M.q;
+        const TcgTile t(S, tile);
+        const int lane = t.lane, wv = t.wv, J0 = t.J0;
+        if (threadIdx.x < H) s_row[threadIdx.x] = meta;
+        else if (threadIdx.x < H + W) s_col[threadIdx.x - H] = meta;
 #pragma unroll
-    for (int k = 0; k < NG; k++) {
-        const int i = threadIdx.x + NT * k;
-        if (i < TY * B) s_gy[i] = gyv[k];
-    }
-#pragma unroll
-    for (int k = 0; k < RL; k++) {
-        const int a = wv + NW * k;
-        if (a < H) {
-            const float d = tcg_dinv(cg0, cs0, cm0, rg[k], rs[k], rm[k]);
-            const float v = tcg_pnew<T>(r0[k], d, fmu, fb, NOP ? 0.0f : p0[k]);
-            s_u[a * WP + lane] = (rok[k] && c0) ? v : 0.0f;
+        for (int k = 0; k < NG; k++) {
+            const int i = threadIdx.x + NT * k;
+            if (i < TY * B) s_gy[i] = gyv[k];
         }
-    }
-#pragma unroll
-    for (int k = 0; k < NX; k++) {
-        const int i = threadIdx.x + NT * k;
-        if (i < H * 2 * N) {
-            const float v = tcg_pnew<T>(rx[k], dx[k], fmu, fb, px[k]);
-            s_u[(i / (2 * N)) * WP + 64 + i % (2 * N)] = okx[k] ? v : 0.0f;
-        }
-    }
-    __syncthreads();
+        __syncthreads();
 
-    // ---- q = A~ p' on the owned cells ----
-    const int j = lane, g = wv;
-    float ax[RPT];
+        // ---- new direction on the tile and its halo ----
+        const float4 cl0 = s_col[lane];
 #pragma unroll
-    for (int k = 0; k < RPT; k++) {
-        const float* rowp = &s_u[(g * RPT + k + N) * WP + j];
-        float acc = 0.0f;
-#pragma unroll
-        for (int s = 0; s < B; s++) acc += gxr[s] * rowp[s];
-        ax[k] = acc;
-    }
-    float win[RPT + 2 * N];
-#pragma unroll
-    for (int tt = 0; tt < RPT + 2 * N; tt++) win[tt] = s_u[(g * RPT + tt) * WP + j + N];
-    double dot = 0.0;
-#pragma unroll
-    for (int k = 0; k < RPT; k++) {
-        const int J = J0 + g * RPT + k;
-        const int Jc = J < S.nU1 ? J : 0;
-        float acc = 0.0f;
-#pragma unroll
-        for (int s = 0; s < B; s++) acc += s_gy[(g * RPT + k) * B + s] * win[k + s];  // one address per wave
-        const float u = win[k + N];
-        const float y = tb.sy[Jc] * ax[k] + sxI * acc + (mxI * tb.my[Jc]) * u;
-        if (t.colok && J < S.nU1) {
-            q[(size_t)J * S.nU0 + t.I] = y;
-            dot += (double)u * (double)y;
+        for (int k = 0; k < RL; k++) {
+            const int a = wv + NW * k;
+            if (H % NW == 0 || a < H) {  // (H % NW == 0: every a < H)
+                const float4 rw = s_row[a];
+                const float d = tcg_dinv(cl0.x, cl0.y, cl0.z, rw.x, rw.y, rw.z);
+                const float v = tcg_pnew<T>(r0[k], d, fmu, fb, NOP ? 0.0f : p0[k]);
+                s_u[a * WP + lane] = (rw.w != 0.0f && cl0.w != 0.0f) ? v : 0.0f;
+            }
         }
+#pragma unroll
+        for (int k = 0; k < NX; k++) {
+            const int i = threadIdx.x + NT * k;
+            if (i < H * 2 * N) {
+                const int a = i / (2 * N), cc = 64 + i % (2 * N);
+                const float4 rw = s_row[a], cl = s_col[cc];
+                const float d = tcg_dinv(cl.x, cl.y, cl.z, rw.x, rw.y, rw.z);
+                const float v = tcg_pnew<T>(rx[k], d, fmu, fb, px[k]);
+                s_u[a * WP + cc] = (rw.w != 0.0f && cl.w != 0.0f) ? v : 0.0f;
+            }
+        }
+        __syncthreads();
+
+        // ---- q = A~ p' on the owned cells ----
+        const int j = lane, g = wv;
+        float ax[RPT];
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            const float* rowp = &s_u[(g * RPT + k + N) * WP + j];
+            float acc = 0.0f;
+#pragma unroll
+            for (int s = 0; s < B; s++) acc += gxr[s] * rowp[s];
+            ax[k] = acc;
+        }
+        float win[RPT + 2 * N];
+#pragma unroll
+        for (int tt = 0; tt < RPT + 2 * N; tt++) win[tt] = s_u[(g * RPT + tt) * WP + j + N];
+        double dot = 0.0;
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            const int J = J0 + g * RPT + k;
+            float acc = 0.0f;
+#pragma unroll
+            for (int s = 0; s < B; s++) acc += s_gy[(g * RPT + k) * B + s] * win[k + s];  // one address per wave
+            const float u = win[k + N];
+            const float4 rw = s_row[g * RPT + k + N];
+            const float y = rw.y * ax[k] + sxI * acc + (mxI * rw.z) * u;
+            if (t.colok && J < S.nU1) {
+                q[(size_t)J * S.nU0 + t.I] = y;
+                dot += (double)u * (double)y;
+            }
+        }
+        double v[1] = {dot};
+        block_store_partials_at<1, TCG_NT>(v, M.part_a(), tile, M.tiles);
     }
-    double v[1] = {dot};
-    block_store_partials_at<1, NT>(v, M.part_a(), tile, np);
+};
+
+// one tile per workgroup (single-member kernel)
+template <int N, typename T, int MODE>
+__device__ __forceinline__ void tcg_dir_apply_tile(const TcgMember& M, int tile) {
+    using D = TcgDir<N, T, MODE>;
+    D d;
+    d.load(M, tile);
+    typename D::Setup su;
+    su.load(M);
+    float fb = 0.0f, fmu = 0.0f;
+    if (!su.run(M, tile, fb, fmu)) return;
+    d.compute(M, tile, fb, fmu);
 }
 
 // ---------------------------------------------------------------------------
 // k_tcg_update
 // ---------------------------------------------------------------------------
 template <int N, typename T>
-__device__ __forceinline__ void tcg_update_tile(const TcgMember& M, int tile) {
-    constexpr int B = 2 * N + 1, RPT = TCG_RPT;
-    const SepParams& S = M.S;
-    const TcgTab tb(S);
-    gp<T> p = (gp<T>)M.p;
-    gp<float> r = (gp<float>)M.r;
-    cgp<float> q = (cgp<float>)M.q;
-    gp<float> x = (gp<float>)M.xlo;
-    const int np = M.tiles;
-    CgState* __restrict__ st = M.st;
-    const TcgTile t(S, tile);
-    const int Ic = t.colok ? t.I : 0;
+struct TcgUpd {
+    static constexpr int B = 2 * N + 1, RPT = TCG_RPT;
+    struct Setup {
+        TcgPart<1> pa;
+        __device__ __forceinline__ void load(const TcgMember& M) { pa.load(M.part_a(), M.tiles); }
+        // total of p'.q -> alpha; the member's first tile writes the state
+        __device__ __forceinline__ bool run(const TcgMember& M, int tile, float& fa, float& fb, float& fmu) {
+            CgState* __restrict__ st = M.st;
+            __shared__ int s_stop;
+            if (threadIdx.x == 0) s_stop = __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const double rho = st->rho, beta = st->beta, mu = st->mu;
+            double ta[1];
+            pa.reduce(M.part_a(), M.tiles, ta);
+            if (s_stop) return false;
+            const double pq = ta[0];
+            const bool lead = tile == 0 && threadIdx.x == 0;  // the member's first tile writes its state
+            if (!(pq > 0.0)) {
+                if (lead) {
+                    st->pq = pq;
+                    st->stop = 1;
+                    st->status = (pq - pq == 0.0) ? 2 : 3;
+                }
+                return false;
+            }
+            const double alpha = rho / pq;
+            if (lead) {
+                st->pq = pq;
+                st->alpha = alpha;
+                st->rho_prev = rho;
+                st->iter = st->iter + 1;
+            }
+            fa = (float)alpha;
+            fb = (float)beta;
+            fmu = (float)mu;
+            return true;
+        }
+    };
+
     float pv[RPT], rv[RPT], qv[RPT], xv[RPT], rg[RPT], rs[RPT], rm[RPT];
+    float cg, cs, cm;
+
+    __device__ __forceinline__ void load(const TcgMember& M, int tile) {
+        const SepParams& S = M.S;
+        const TcgTab tb(S);
+        const TcgTile t(S, tile);
+        const int Ic = t.colok ? t.I : 0;
 #pragma unroll
-    for (int k = 0; k < RPT; k++) {
-        const int J = t.row(k);
-        const int Jc = J < S.nU1 ? J : 0;
-        const size_t o = (size_t)Jc * S.nU0 + Ic;
-        pv[k] = ldf<T>((cgp<T>)p, o);
-        rv[k] = r[o];
-        qv[k] = q[o];
-        xv[k] = x[o];
-        rg[k] = tb.gy[(size_t)Jc * B + N];  // wave-uniform: scalar loads
-        rs[k] = tb.sy[Jc];
-        rm[k] = tb.my[Jc];
-    }
-    const float cg = tb.gx[(size_t)Ic * B + N], cs = tb.sx[Ic], cm = tb.mx[Ic];
-    TcgPart<1> pa;
-    pa.load(M.part_a(), np);
-    __shared__ int s_stop;
-    if (threadIdx.x == 0) s_stop = __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const double rho = st->rho, beta = st->beta, mu = st->mu;
-    double ta[1];
-    pa.reduce(M.part_a(), np, ta);
-    if (s_stop) return;
-    const double pq = ta[0];
-    const bool lead = tile == 0 && threadIdx.x == 0;  // the member's first tile writes its state
-    if (!(pq > 0.0)) {
-        if (lead) {
-            st->pq = pq;
-            st->stop = 1;
-            st->status = (pq - pq == 0.0) ? 2 : 3;
+        for (int k = 0; k < RPT; k++) {
+            const int J = t.row(k);
+            const int Jc = J < S.nU1 ? J : 0;
+            const size_t o = (size_t)Jc * S.nU0 + Ic;
+            pv[k] = ldf<T>((cgp<T>)M.p, o);
+            rv[k] = ((cgp<float>)M.r)[o];
+            qv[k] = ((cgp<float>)M.q)[o];
+            xv[k] = ((cgp<float>)M.xlo)[o];
+            rg[k] = tb.gy[(size_t)Jc * B + N];  // wave-uniform: scalar loads
+            rs[k] = tb.sy[Jc];
+            rm[k] = tb.my[Jc];
         }
-        return;
+        cg = tb.gx[(size_t)Ic * B + N];
+        cs = tb.sx[Ic];
+        cm = tb.mx[Ic];
     }
-    const double alpha = rho / pq;
-    if (lead) {
-        st->pq = pq;
-        st->alpha = alpha;
-        st->rho_prev = rho;
-        st->iter = st->iter + 1;
-    }
-    const float fa = (float)alpha, fb = (float)beta, fmu = (float)mu;
-    double srr = 0.0, srz = 0.0, ssz = 0.0, ssr = 0.0;
+
+    // p = p', x_lo += alpha p, r -= alpha q; partials of r.r, r.z, sum z, sum r
+    __device__ __forceinline__ void compute(const TcgMember& M, int tile, float fa, float fb, float fmu) {
+        const SepParams& S = M.S;
+        gp<T> p = (gp<T>)M.p;
+        gp<float> r = (gp<float>)M.r;
+        gp<float> x = (gp<float>)M.xlo;
+        const TcgTile t(S, tile);
+        double srr = 0.0, srz = 0.0, ssz = 0.0, ssr = 0.0;
 #pragma unroll
-    for (int k = 0; k < RPT; k++) {
-        const int J = t.row(k);
-        if (t.colok && J < S.nU1) {
-            const size_t o = (size_t)J * S.nU0 + t.I;
-            const float dinv = tcg_dinv(cg, cs, cm, rg[k], rs[k], rm[k]);
-            const float pn = tcg_pnew<T>(rv[k], dinv, fmu, fb, pv[k]);
-            const float xn = __fmaf_rn(fa, pn, xv[k]);
-            const float rn = __fmaf_rn(-fa, qv[k], rv[k]);
-            const float z = dinv * rn;
-            stf<T>(p, o, pn);
-            x[o] = xn;
-            r[o] = rn;
-            srr += (double)rn * rn;
-            srz += (double)rn * z;
-            ssz += z;
-            ssr += rn;
+        for (int k = 0; k < RPT; k++) {
+            const int J = t.row(k);
+            if (t.colok && J < S.nU1) {
+                const size_t o = (size_t)J * S.nU0 + t.I;
+                const float dinv = tcg_dinv(cg, cs, cm, rg[k], rs[k], rm[k]);
+                const float pn = tcg_pnew<T>(rv[k], dinv, fmu, fb, pv[k]);
+                const float xn = __fmaf_rn(fa, pn, xv[k]);
+                const float rn = __fmaf_rn(-fa, qv[k], rv[k]);
+                const float z = dinv * rn;
+                stf<T>(p, o, pn);
+                x[o] = xn;
+                r[o] = rn;
+                srr += (double)rn * rn;
+                srz += (double)rn * z;
+                ssz += z;
+                ssr += rn;
+            }
         }
+        double v[4] = {srr, srz, ssz, ssr};
+        block_store_partials_at<4, TCG_NT>(v, M.part_b(), tile, M.tiles);
     }
-    double v[4] = {srr, srz, ssz, ssr};
-    block_store_partials_at<4, TCG_NT>(v, M.part_b(), tile, np);
+};
+
+template <int N, typename T>
+__device__ __forceinline__ void tcg_update_tile(const TcgMember& M, int tile) {
+    using U = TcgUpd<N, T>;
+    U u;
+    u.load(M, tile);
+    typename U::Setup su;
+    su.load(M);
+    float fa = 0.0f, fb = 0.0f, fmu = 0.0f;
+    if (!su.run(M, tile, fa, fb, fmu)) return;
+    u.compute(M, tile, fa, fb, fmu);
 }
 
 // ---------------------------------------------------------------------------
@@ -544,26 +631,71 @@ __device__ __forceinline__ int work_count(int nwork, const int* __restrict__ nwo
     return nwork_dev ? __builtin_amdgcn_readfirstlane(*nwork_dev) : nwork;
 }
 
+// Each workgroup takes one contiguous run of the work list.  The list is
+// ordered by member, then tile, so a run is a strip of neighbouring tiles of
+// one or two members: their halo rows meet in one L2, and the member's totals
+// are reduced once per run instead of once per tile.  Workgroups are
+// dispatched round-robin over the 8 XCDs; the workgroups of one XCD take
+// consecutive runs, so that neighbours one tile row apart share that XCD's L2
+// too.
+__device__ __forceinline__ void tcg_run(int nwork, int& w0, int& w1) {
+    const int G = gridDim.x;
+    const int r = (G & 7) ? (int)blockIdx.x : (int)(blockIdx.x & 7) * (G >> 3) + (int)(blockIdx.x >> 3);
+    const int chunk = (nwork + G - 1) / G;
+    w0 = min(nwork, r * chunk);
+    w1 = min(nwork, w0 + chunk);
+}
+
 template <int N, typename T, int MODE>
 __global__ __launch_bounds__(TCG_NT) void k_tcg_dir_apply_batch(const TcgMember* __restrict__ Ms,
                                                                 const int2* __restrict__ work, int nwork,
                                                                 const int* __restrict__ nwork_dev) {
+    using D = TcgDir<N, T, MODE>;
     nwork = work_count(nwork, nwork_dev);
-    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    int w0, w1;
+    tcg_run(nwork, w0, w1);
+    int cur = -1;
+    bool go = false;
+    float fb = 0.0f, fmu = 0.0f;
+    for (int w = w0; w < w1; w++) {
         const int2 it = work[w];
-        __syncthreads();
-        tcg_dir_apply_tile<N, T, MODE>(Ms[it.x], it.y);
+        const TcgMember& M = Ms[it.x];
+        __syncthreads();  // the previous tile's LDS reads are done
+        D d;
+        d.load(M, it.y);
+        if (it.x != cur) {  // a new member: its totals (loads in flight with the tile's)
+            typename D::Setup su;
+            su.load(M);
+            go = su.run(M, it.y, fb, fmu);
+            cur = it.x;
+        }
+        if (go) d.compute(M, it.y, fb, fmu);
     }
 }
 template <int N, typename T>
 __global__ __launch_bounds__(TCG_NT) void k_tcg_update_batch(const TcgMember* __restrict__ Ms,
                                                              const int2* __restrict__ work, int nwork,
                                                              const int* __restrict__ nwork_dev) {
+    using U = TcgUpd<N, T>;
     nwork = work_count(nwork, nwork_dev);
-    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    int w0, w1;
+    tcg_run(nwork, w0, w1);
+    int cur = -1;
+    bool go = false;
+    float fa = 0.0f, fb = 0.0f, fmu = 0.0f;
+    for (int w = w0; w < w1; w++) {
         const int2 it = work[w];
+        const TcgMember& M = Ms[it.x];
         __syncthreads();
-        tcg_update_tile<N, T>(Ms[it.x], it.y);
+        U u;
+        u.load(M, it.y);
+        if (it.x != cur) {
+            typename U::Setup su;
+            su.load(M);
+            go = su.run(M, it.y, fa, fb, fmu);
+            cur = it.x;
+        }
+        if (go) u.compute(M, it.y, fa, fb, fmu);
     }
 }
 template <int N, typename T, bool RESID>
