@@ -323,11 +323,13 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
         auto it = c->tuned_grid.find(shape);
         if (it != c->tuned_grid.end()) b->tuned = it->second;
     }
-    // autotuner candidates: grid = all items, or 8, 4, 2 workgroups per CU.
+    // autotuner candidates: grid = all items (one item per workgroup), or
+    // 16, 8, 6, 4, 3, 2 workgroups per CU (each a contiguous run of items).
     // Phase 2j captures candidate j and runs it once, phase 2j+1 times a
     // replay; every launch is followed by the regular check, so the check
     // cadence (and with it every result) is that of the single solve.
-    static const int cand[4] = {0, 8, 4, 2};
+    static const int cand[] = {0, 16, 8, 6, 4, 3, 2};
+    constexpr int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
     int phase = b->tuned ? -1 : 0;
     set_grids(b, b->tuned > 0 ? b->tuned - 1 : 0);
     while (running > 0) {
@@ -348,7 +350,7 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
                 b->tuned = cand[phase / 2] + 1;
             }
         }
-        if (phase >= 0 && ++phase == 8) {
+        if (phase >= 0 && ++phase == 2 * ncand) {
             phase = -1;
             set_grids(b, b->tuned - 1);
             b->graph_key = -1;
@@ -519,6 +521,12 @@ extern "C" int se_batch_destroy(se_batch* b) {
 }
 
 extern "C" int se_batch_size(const se_batch* b) { return b ? (int)b->ops.size() : 0; }
+
+extern "C" int se_batch_launch_shape(const se_batch* b, int* workgroups_per_cu) {
+    SE_CHECK(b && workgroups_per_cu, SE_ERR_ARG, "se_batch_launch_shape: NULL argument");
+    *workgroups_per_cu = b->tuned - 1;
+    return SE_OK;
+}
 
 namespace {
 // time `reps` back-to-back launches of each work-item kernel of every group,

@@ -211,7 +211,7 @@ int tcg_prepare(se_op* op) {
 TcgMember tcg_member(se_op* op, const double* d_f, double rtol, int null_space, int np_s) {
     TcgMember M;
     M.S = sep_params(op);
-    M.r = op->lo_r.as<float>();
+    M.z = op->lo_r.as<float>();
     M.q = op->lo_q.as<float>();
     M.xlo = op->lo_x.as<float>();
     M.p = op->lo_p.ptr;

@@ -3,16 +3,18 @@
 // iteration:
 //
 //   k_tcg_dir_apply : totals of the previous update -> beta, convergence;
-//                     p' = z + beta p on the tile and its halo (z = D~^-1 r,
-//                     D~ the diagonal of the separable operator, computed from
-//                     the line tables), q = A~ p', partial p'.q
+//                     p' = z + beta p on the tile and its halo, q = A~ p',
+//                     partial p'.q
 //   k_tcg_update    : totals of p'.q -> alpha; p = p' (recomputed with the same
-//                     instructions), x_lo += alpha p, r -= alpha q, partials of
-//                     r.r, r.z, sum z, sum r
+//                     instructions), x_lo += alpha p, z -= alpha D~^-1 q,
+//                     partials of r.r, r.z, sum z, sum r with r = D~ z
 //
-// The direction p is read with its halo by k_tcg_dir_apply while only
+// The recursion stores the preconditioned residual z = D~^-1 r (D~ the
+// diagonal of the separable operator, rebuilt from the line tables where it
+// is needed), so the halo work of k_tcg_dir_apply is one FMA per cell.  The
+// direction p is read with its halo by k_tcg_dir_apply while only
 // k_tcg_update (which has no halo) writes it, so no workgroup can see a
-// neighbour's new values.  Vectors r, q, x_lo are FP32, p is FP32 or BF16;
+// neighbour's new values.  Vectors z, q, x_lo are FP32, p is FP32 or BF16;
 // every reduction is FP64 and deterministic (consumer-side fixed-order sums).
 // FP64 enters through reliable updates driven by the host: x += x_lo in FP64,
 // r = b - A x with the exact FP64 operator (k_tcg_flush, op apply,
@@ -34,16 +36,19 @@ constexpr int TCG_PRE = 2;  // preloaded partials per thread and value
 // recursion's, the old direction is no longer conjugate to anything useful)
 enum { TCG_FIRST = 0, TCG_NORMAL = 1, TCG_RESET = 2 };
 
-// Jacobi diagonal inverse and the new direction: shared by both kernels so
-// that the halo values one workgroup computes equal the owner's stored ones.
-// (v_rcp_f32: one instruction, within 1 ulp; the same instruction in every
-// kernel, so every kernel sees the same preconditioner)
-__device__ __forceinline__ float tcg_dinv(float gxN, float sxI, float mxI, float gyN, float syJ, float myJ) {
-    return __builtin_amdgcn_rcpf(__fmaf_rn(syJ, gxN, __fmaf_rn(sxI, gyN, mxI * myJ)));
+// Jacobi diagonal of the separable operator from its line factors, and its
+// inverse (v_rcp_f32: one instruction, within 1 ulp; the same instructions in
+// every kernel, so every kernel sees the same preconditioner).
+__device__ __forceinline__ float tcg_diag(float gxN, float sxI, float mxI, float gyN, float syJ, float myJ) {
+    return __fmaf_rn(syJ, gxN, __fmaf_rn(sxI, gyN, mxI * myJ));
 }
+__device__ __forceinline__ float tcg_dinv(float diag) { return __builtin_amdgcn_rcpf(diag); }
+// The new direction p' = (z - mu) + beta p from the stored preconditioned
+// residual z = D~^-1 r: shared by both kernels so that the halo values one
+// workgroup computes equal the owner's stored ones.
 template <typename T>
-__device__ __forceinline__ float tcg_pnew(float r, float dinv, float mu, float beta, float pold) {
-    return to_f<T>(from_f<T>(__fmaf_rn(beta, pold, __fmaf_rn(r, dinv, -mu))));
+__device__ __forceinline__ float tcg_pnew(float z, float mu, float beta, float pold) {
+    return to_f<T>(from_f<T>(__fmaf_rn(beta, pold, z - mu)));
 }
 
 // Partial sums of the producer kernel, preloaded into registers together with
@@ -109,8 +114,8 @@ struct TcgTile {
 // [2 tiles] | S mean of f [256]).
 struct TcgMember {
     SepParams S;
-    float *r, *q, *xlo;
-    void* p;  // T
+    float *z, *q, *xlo;  // z = D~^-1 r, the preconditioned residual of the recursion
+    void* p;             // T
     double *b, *x, *q64;
     const double* f;
     double* part;
@@ -162,7 +167,7 @@ struct TcgDir {
     static constexpr int RL = (H + NW - 1) / NW;          // main-column rows per wave
     static constexpr int NX = (H * 2 * N + NT - 1) / NT;  // right-halo cells per thread
     static constexpr int NG = (TY * B + NT - 1) / NT;
-    static_assert(H + W <= NT, "one thread per halo row and column");
+    static_assert(2 * TY <= NT, "one thread per owned row and line factor");
 
     // ---- the member's totals and state (once per member and workgroup run) --
     struct Setup {
@@ -240,16 +245,19 @@ struct TcgDir {
     };
 
     // ---- one tile: its loads (issued first), then the work ------------------
+    // Halo cells outside the domain (non-periodic sides) load a clamped, valid
+    // cell: the band tables hold zero weights for neighbours outside the
+    // domain, so those finite values never reach q.
     float gyv[NG];
-    float4 meta;
-    float r0[RL], p0[RL], rx[NX], px[NX];
+    float sm;  // thread t < TY: sy of owned row t; TY <= t < 2 TY: my of row t - TY
+    float z0[RL], p0[RL], zx[NX], px[NX];
     float gxr[B];
     float sxI, mxI;
 
     __device__ __forceinline__ void load(const TcgMember& M, int tile) {
         const SepParams& S = M.S;
         const TcgTab tb(S);
-        cgp<float> r = (cgp<float>)M.r;
+        cgp<float> z = (cgp<float>)M.z;
         cgp<T> p = (cgp<T>)M.p;
         const TcgTile t(S, tile);
         const int lane = t.lane, wv = t.wv, I0 = t.I0, J0 = t.J0;
@@ -260,48 +268,37 @@ struct TcgDir {
             const float v = tb.gy[ok ? (size_t)J0 * B + i : 0];
             gyv[k] = ok ? v : 0.0f;
         }
-        // line factors: thread a < H takes halo row a, thread H + c column c
-        // (per-lane vector loads, all in flight together)
-        meta = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         {
-            const bool isrow = threadIdx.x < H;
-            const int c = threadIdx.x - H;
-            int L = isrow ? J0 - N + (int)threadIdx.x : I0 - N + c;
-            const int nL = isrow ? S.nU1 : S.nU0;
-            if (isrow ? S.per1 : S.per0) L = L < 0 ? L + nL : (L >= nL ? L - nL : L);
-            const bool ok = (isrow || c < W) && L >= 0 && L < nL;
-            const int Lc = ok ? L : 0;
-            if (isrow) {
-                meta = make_float4(tb.gy[(size_t)Lc * B + N], tb.sy[Lc], tb.my[Lc], ok ? 1.0f : 0.0f);
-            } else if (c < W) {
-                meta = make_float4(tb.gx[(size_t)Lc * B + N], tb.sx[Lc], tb.mx[Lc], ok ? 1.0f : 0.0f);
-            }
+            const int i = threadIdx.x & (TY - 1);
+            const int J = min(J0 + i, S.nU1 - 1);
+            sm = threadIdx.x < TY ? tb.sy[J] : tb.my[J];
         }
         // main columns: lane l holds tile column l of rows wv + NW*k
         int Ic0 = I0 - N + lane;
         if (S.per0) Ic0 = Ic0 < 0 ? Ic0 + S.nU0 : (Ic0 >= S.nU0 ? Ic0 - S.nU0 : Ic0);
-        Ic0 = (Ic0 >= 0 && Ic0 < S.nU0) ? Ic0 : 0;
+        Ic0 = min(max(Ic0, 0), S.nU0 - 1);
 #pragma unroll
         for (int k = 0; k < RL; k++) {
             const int a = wv + NW * k;
             int J = J0 - N + a;
             if (S.per1) J = J < 0 ? J + S.nU1 : (J >= S.nU1 ? J - S.nU1 : J);
-            const bool rok = a < H && J >= 0 && J < S.nU1;
-            const size_t rowo = (size_t)(rok ? J : 0) * S.nU0;
-            r0[k] = r[rowo + Ic0];
+            J = min(max(J, 0), S.nU1 - 1);
+            const size_t rowo = (size_t)J * S.nU0;
+            z0[k] = z[rowo + Ic0];
             if (!NOP) p0[k] = ldf<T>(p, rowo + Ic0);
         }
         // right-halo columns 64 .. 64+2N-1, spread over all threads
 #pragma unroll
         for (int k = 0; k < NX; k++) {
-            const int i = threadIdx.x + NT * k;
+            const int i = min((int)threadIdx.x + NT * k, H * 2 * N - 1);
             const int a = i / (2 * N), cc = 64 + i % (2 * N);
             int J = J0 - N + a, I = I0 - N + cc;
             if (S.per1) J = J < 0 ? J + S.nU1 : (J >= S.nU1 ? J - S.nU1 : J);
             if (S.per0) I = I >= S.nU0 ? I - S.nU0 : I;
-            const bool ok = i < H * 2 * N && J >= 0 && J < S.nU1 && I >= 0 && I < S.nU0;
-            const size_t o = ok ? (size_t)J * S.nU0 + I : 0;
-            rx[k] = r[o];
+            J = min(max(J, 0), S.nU1 - 1);
+            I = min(I, S.nU0 - 1);
+            const size_t o = (size_t)J * S.nU0 + I;
+            zx[k] = z[o];
             px[k] = NOP ? 0.0f : ldf<T>(p, o);
         }
         const int Ic = t.colok ? t.I : 0;
@@ -312,77 +309,58 @@ struct TcgDir {
     }
 
     // p' = z + beta p on the tile and its halo, q = A~ p' on the owned
-    // cells, partial p'.q.  Starts and ends with every thread past its last
-    // LDS access of the previous step (the caller's barrier).
+    // cells, partial p'.q.  Starts with every thread past its last LDS access
+    // of the previous tile (the caller's barrier).
     __device__ __forceinline__ void compute(const TcgMember& M, int tile, float fb, float fmu) {
         __shared__ float s_u[H * WP];
         __shared__ float s_gy[TY * B];  // row bands of the owned rows
-        // diagonal factors of the tile's rows and columns (halo included):
-        // {band centre, line scale, mass factor, 1 inside the domain else 0}
-        __shared__ float4 s_row[H];
-        __shared__ float4 s_col[W];
+        __shared__ float s_sm[2 * TY];  // sy, my of the owned rows
         const SepParams& S = M.S;
         gp<float> q = (gp<float>)M.q;
         const TcgTile t(S, tile);
         const int lane = t.lane, wv = t.wv, J0 = t.J0;
-        if (threadIdx.x < H) s_row[threadIdx.x] = meta;
-        else if (threadIdx.x < H + W) s_col[threadIdx.x - H] = meta;
 #pragma unroll
         for (int k = 0; k < NG; k++) {
             const int i = threadIdx.x + NT * k;
             if (i < TY * B) s_gy[i] = gyv[k];
         }
-        __syncthreads();
+        if (threadIdx.x < 2 * TY) s_sm[threadIdx.x] = sm;
 
         // ---- new direction on the tile and its halo ----
-        const float4 cl0 = s_col[lane];
 #pragma unroll
         for (int k = 0; k < RL; k++) {
             const int a = wv + NW * k;
-            if (H % NW == 0 || a < H) {  // (H % NW == 0: every a < H)
-                const float4 rw = s_row[a];
-                const float d = tcg_dinv(cl0.x, cl0.y, cl0.z, rw.x, rw.y, rw.z);
-                const float v = tcg_pnew<T>(r0[k], d, fmu, fb, NOP ? 0.0f : p0[k]);
-                s_u[a * WP + lane] = (rw.w != 0.0f && cl0.w != 0.0f) ? v : 0.0f;
-            }
+            if (H % NW == 0 || a < H)  // (H % NW == 0: every a < H)
+                s_u[a * WP + lane] = tcg_pnew<T>(z0[k], fmu, fb, NOP ? 0.0f : p0[k]);
         }
 #pragma unroll
         for (int k = 0; k < NX; k++) {
             const int i = threadIdx.x + NT * k;
-            if (i < H * 2 * N) {
-                const int a = i / (2 * N), cc = 64 + i % (2 * N);
-                const float4 rw = s_row[a], cl = s_col[cc];
-                const float d = tcg_dinv(cl.x, cl.y, cl.z, rw.x, rw.y, rw.z);
-                const float v = tcg_pnew<T>(rx[k], d, fmu, fb, px[k]);
-                s_u[a * WP + cc] = (rw.w != 0.0f && cl.w != 0.0f) ? v : 0.0f;
-            }
+            if (i < H * 2 * N) s_u[(i / (2 * N)) * WP + 64 + i % (2 * N)] = tcg_pnew<T>(zx[k], fmu, fb, px[k]);
         }
         __syncthreads();
 
         // ---- q = A~ p' on the owned cells ----
+        // One row at a time (not unrolled): each row reads its 2N+1 x- and
+        // y-neighbours from LDS, so only one row's operands are live and the
+        // kernel keeps several waves per SIMD to hide memory latency.
         const int j = lane, g = wv;
-        float ax[RPT];
-#pragma unroll
-        for (int k = 0; k < RPT; k++) {
-            const float* rowp = &s_u[(g * RPT + k + N) * WP + j];
-            float acc = 0.0f;
-#pragma unroll
-            for (int s = 0; s < B; s++) acc += gxr[s] * rowp[s];
-            ax[k] = acc;
-        }
-        float win[RPT + 2 * N];
-#pragma unroll
-        for (int tt = 0; tt < RPT + 2 * N; tt++) win[tt] = s_u[(g * RPT + tt) * WP + j + N];
         double dot = 0.0;
-#pragma unroll
+#pragma unroll 1
         for (int k = 0; k < RPT; k++) {
-            const int J = J0 + g * RPT + k;
-            float acc = 0.0f;
+            const int row = g * RPT + k;
+            const float* xr = &s_u[(row + N) * WP + j];  // row row+N, columns j .. j+2N
+            const float* yc = &s_u[row * WP + j + N];    // column j+N, rows row .. row+2N
+            const float* gyr = &s_gy[row * B];           // one address per wave
+            float ax = 0.0f, ay = 0.0f;
 #pragma unroll
-            for (int s = 0; s < B; s++) acc += s_gy[(g * RPT + k) * B + s] * win[k + s];  // one address per wave
-            const float u = win[k + N];
-            const float4 rw = s_row[g * RPT + k + N];
-            const float y = rw.y * ax[k] + sxI * acc + (mxI * rw.z) * u;
+            for (int s = 0; s < B; s++) {
+                ax += gxr[s] * xr[s];
+                ay += gyr[s] * yc[s * WP];
+            }
+            const float u = yc[N * WP];
+            const float y = s_sm[row] * ax + sxI * ay + (mxI * s_sm[TY + row]) * u;
+            const int J = J0 + row;
             if (t.colok && J < S.nU1) {
                 q[(size_t)J * S.nU0 + t.I] = y;
                 dot += (double)u * (double)y;
@@ -448,7 +426,7 @@ struct TcgUpd {
         }
     };
 
-    float pv[RPT], rv[RPT], qv[RPT], xv[RPT], rg[RPT], rs[RPT], rm[RPT];
+    float pv[RPT], zv[RPT], qv[RPT], xv[RPT], rg[RPT], rs[RPT], rm[RPT];
     float cg, cs, cm;
 
     __device__ __forceinline__ void load(const TcgMember& M, int tile) {
@@ -462,7 +440,7 @@ struct TcgUpd {
             const int Jc = J < S.nU1 ? J : 0;
             const size_t o = (size_t)Jc * S.nU0 + Ic;
             pv[k] = ldf<T>((cgp<T>)M.p, o);
-            rv[k] = ((cgp<float>)M.r)[o];
+            zv[k] = ((cgp<float>)M.z)[o];
             qv[k] = ((cgp<float>)M.q)[o];
             xv[k] = ((cgp<float>)M.xlo)[o];
             rg[k] = tb.gy[(size_t)Jc * B + N];  // wave-uniform: scalar loads
@@ -474,11 +452,12 @@ struct TcgUpd {
         cm = tb.mx[Ic];
     }
 
-    // p = p', x_lo += alpha p, r -= alpha q; partials of r.r, r.z, sum z, sum r
+    // p = p', x_lo += alpha p, z -= alpha D~^-1 q (r = D~ z); partials of r.r,
+    // r.z, sum z, sum r
     __device__ __forceinline__ void compute(const TcgMember& M, int tile, float fa, float fb, float fmu) {
         const SepParams& S = M.S;
         gp<T> p = (gp<T>)M.p;
-        gp<float> r = (gp<float>)M.r;
+        gp<float> z = (gp<float>)M.z;
         gp<float> x = (gp<float>)M.xlo;
         const TcgTile t(S, tile);
         double srr = 0.0, srz = 0.0, ssz = 0.0, ssr = 0.0;
@@ -487,17 +466,18 @@ struct TcgUpd {
             const int J = t.row(k);
             if (t.colok && J < S.nU1) {
                 const size_t o = (size_t)J * S.nU0 + t.I;
-                const float dinv = tcg_dinv(cg, cs, cm, rg[k], rs[k], rm[k]);
-                const float pn = tcg_pnew<T>(rv[k], dinv, fmu, fb, pv[k]);
+                const float diag = tcg_diag(cg, cs, cm, rg[k], rs[k], rm[k]);
+                const float dinv = tcg_dinv(diag);
+                const float pn = tcg_pnew<T>(zv[k], fmu, fb, pv[k]);
                 const float xn = __fmaf_rn(fa, pn, xv[k]);
-                const float rn = __fmaf_rn(-fa, qv[k], rv[k]);
-                const float z = dinv * rn;
+                const float zn = __fmaf_rn(-fa, dinv * qv[k], zv[k]);
+                const float rn = diag * zn;
                 stf<T>(p, o, pn);
                 x[o] = xn;
-                r[o] = rn;
+                z[o] = zn;
                 srr += (double)rn * rn;
-                srz += (double)rn * z;
-                ssz += z;
+                srz += (double)rn * zn;
+                ssz += zn;
                 ssr += rn;
             }
         }
@@ -521,9 +501,9 @@ __device__ __forceinline__ void tcg_update_tile(const TcgMember& M, int tile) {
 // ---------------------------------------------------------------------------
 // start and reliable update
 // ---------------------------------------------------------------------------
-// b = sgn f - mu0; x = 0; x_lo = 0; p = 0; r_lo = b; partials of r_lo (part_b)
-// and of b in FP64 (part_t: sum b^2, sum b).  RESID: r = b - q64 instead, with
-// x, x_lo, p left alone.
+// b = sgn f - mu0; x = 0; x_lo = 0; p = 0; z = D~^-1 r_lo with r_lo = b in
+// FP32; partials of r_lo, z (part_b) and of b in FP64 (part_t: sum b^2,
+// sum b).  RESID: r_lo = b - q64 instead, with x, x_lo, p left alone.
 template <int N, typename T, bool RESID>
 __device__ __forceinline__ void tcg_start_tile(const TcgMember& M, int tile) {
     const SepParams& S = M.S;
@@ -536,7 +516,7 @@ __device__ __forceinline__ void tcg_start_tile(const TcgMember& M, int tile) {
     double* __restrict__ x = M.x;
     float* __restrict__ xlo = M.xlo;
     T* __restrict__ p = (T*)M.p;
-    float* __restrict__ r = M.r;
+    float* __restrict__ zs = M.z;
     double* __restrict__ part_b = M.part_b();
     double* __restrict__ part_t = M.part_t();
     const int null_space = M.null_space;
@@ -568,9 +548,8 @@ __device__ __forceinline__ void tcg_start_tile(const TcgMember& M, int tile) {
                 p[o] = from_f<T>(0.0f);
             }
             const float rn = (float)bi;
-            r[o] = rn;
-            const float dinv = tcg_dinv(cg, cs, cm, S.gy[(size_t)J * B + N], S.sy[J], S.my[J]);
-            const float z = dinv * rn;
+            const float z = tcg_dinv(tcg_diag(cg, cs, cm, S.gy[(size_t)J * B + N], S.sy[J], S.my[J])) * rn;
+            zs[o] = z;
             srr += (double)rn * rn;
             srz += (double)rn * z;
             ssz += z;
