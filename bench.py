@@ -202,7 +202,7 @@ def run_ensemble(a, prec_name):
               "members": len(hs), "unknowns": ntot, "precision": prec_name, "detj": a.detj, "rtol": 1e-10,
               "iters_min_median_max": [int(its.min()), int(np.median(its)), int(its.max())],
               "reliable_updates_max": max(i["outer_iters"] for i in infos), "all_converged": ok,
-              "tuned_workgroups_per_cu": b.launch_shape(),
+              "tuned_workgroups_per_cu": dict(zip(("direction", "update"), b.launch_shape())),
               "max_true_rel_residual": max(i["true_rel_residual"] for i in infos), "wall_s_timed": t_wall}
     if not a.no_fp64:
         # the plain FP64 GPU path on the same members, one after another

@@ -199,10 +199,11 @@ typedef struct se_batch se_batch;
 int se_batch_create(se_ctx* ctx, int count, se_op* const* ops, se_batch** out);
 int se_batch_destroy(se_batch* batch);
 int se_batch_size(const se_batch* batch);
-/* The work-item kernels' launch shape chosen by the autotuner on the first
- * solve: workgroups per CU, 0 for one workgroup per (member, tile) item, -1
- * before the first solve.  No counterpart in the reference. */
-int se_batch_launch_shape(const se_batch* batch, int* workgroups_per_cu);
+/* The launch shapes the autotuner chose on the first solve for the
+ * direction and the update kernel: workgroups per CU, 0 for one workgroup
+ * per (member, tile) item, -1 before the first solve.  No counterpart in the
+ * reference. */
+int se_batch_launch_shape(const se_batch* batch, int* dir_per_cu, int* update_per_cu);
 /* d_f[i], d_x[i]: device vectors of member i; infos: count entries (nullable) */
 int se_batch_pcg_dev(se_batch* batch, const double* const* d_f, double* const* d_x, const se_solve_opts* opts,
                      se_solve_info* infos);

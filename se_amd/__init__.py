@@ -120,7 +120,7 @@ def lib():
             "se_batch_create": (i, [vp, i, vp, pvp]),
             "se_batch_destroy": (i, [vp]),
             "se_batch_size": (i, [vp]),
-            "se_batch_launch_shape": (i, [vp, C.POINTER(i)]),
+            "se_batch_launch_shape": (i, [vp, C.POINTER(i), C.POINTER(i)]),
             "se_batch_pcg_dev": (i, [vp, vp, vp, C.POINTER(SolveOpts), vp]),
             "se_batch_pcg": (i, [vp, vp, vp, C.POINTER(SolveOpts), vp]),
             "se_batch_probe": (i, [vp, i, i, i, _I, _D, _D, _D, _D]),
@@ -433,10 +433,11 @@ class Batch:
         return [dict(order=int(o[g]), us_dir=a[g], us_upd=b_[g], bytes_dir=c[g], bytes_upd=e[g]) for g in range(ng)]
 
     def launch_shape(self):
-        """Workgroups per CU chosen by the autotuner (0: one per item; -1: not yet tuned)."""
-        v = C.c_int()
-        check(lib().se_batch_launch_shape(self.h, C.byref(v)), "se_batch_launch_shape")
-        return v.value
+        """(direction, update) workgroups per CU chosen by the autotuner (0: one
+        per item; -1: not yet tuned)."""
+        d, u = C.c_int(), C.c_int()
+        check(lib().se_batch_launch_shape(self.h, C.byref(d), C.byref(u)), "se_batch_launch_shape")
+        return d.value, u.value
 
     def close(self):
         if self.h:

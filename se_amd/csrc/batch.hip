@@ -59,7 +59,7 @@ struct Group {
     std::vector<ApplyMember> h_amembers;
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    int grid = 0;  // work-item kernel grid for the iteration chain
+    int grid_dir = 0, grid_upd = 0;  // work-item kernel grids of the iteration chain
 };
 
 }  // namespace
@@ -73,7 +73,9 @@ struct se_batch {
     hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_key = -1;
-    int tuned = 0;          // grid factor chosen by the autotuner (0: not yet)
+    // grid factors (workgroups per CU, 0: one per item) chosen by the
+    // autotuner for the direction and the update kernel, plus one (0: not yet)
+    int tuned_dir = 0, tuned_upd = 0;
     double tune_best = 0.0;
 
     void release() {
@@ -172,7 +174,7 @@ int apply_tiles22(const se_mesh* m) { return ((m->nEls[0] + 1) / 2) * ((m->nEls[
 template <typename T>
 int chains(se_batch* b, int k) {
     hipStream_t s = b->ctx->stream;
-    const int key = k * 4 + (sizeof(T) == 2 ? 1 : 0) + 2 * (b->tuned > 0 ? 1 : 0);
+    const int key = k * 4 + (sizeof(T) == 2 ? 1 : 0) + 2 * (b->tuned_dir > 0 ? 1 : 0);
     if (!b->graph || b->graph_key != key) {
         if (b->graph) (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
@@ -187,8 +189,8 @@ int chains(se_batch* b, int k) {
             const int2* w = g.d_active.as<int2>();
             const int* nd = g.d_nactive.as<int>();
             for (int i = 0; i < k && !rc && e == hipSuccess; i++) {
-                rc = g_update<T>(g, nw, w, g.grid, g.stream, nd);
-                if (!rc) rc = g_dir_apply<T, TCG_NORMAL>(g, nw, w, g.grid, g.stream, nd);
+                rc = g_update<T>(g, nw, w, g.grid_upd, g.stream, nd);
+                if (!rc) rc = g_dir_apply<T, TCG_NORMAL>(g, nw, w, g.grid_dir, g.stream, nd);
             }
             if (e == hipSuccess) e = hipEventRecord(g.done, g.stream);
             if (e == hipSuccess) e = hipStreamWaitEvent(s, g.done, 0);
@@ -227,10 +229,11 @@ int compact(se_batch* b, const std::vector<TcgControl>& ctl, hipStream_t s) {
     return SE_OK;
 }
 
-void set_grids(se_batch* b, int factor) {
+void set_grids(se_batch* b, int fdir, int fupd) {
     for (auto& g : b->groups) {
         const int nw = std::max(1, (int)g.tiles.size());
-        g.grid = factor <= 0 ? nw : std::min(nw, factor * b->ctx->cu_count);
+        g.grid_dir = fdir <= 0 ? nw : std::min(nw, fdir * b->ctx->cu_count);
+        g.grid_upd = fupd <= 0 ? nw : std::min(nw, fupd * b->ctx->cu_count);
     }
 }
 
@@ -319,22 +322,29 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
     int lg = 0;
     while ((size_t(1) << lg) < big->tiles.size()) lg++;
     const std::pair<int, int> shape(big->N, lg);
-    if (!b->tuned) {
+    if (!b->tuned_dir) {
         auto it = c->tuned_grid.find(shape);
-        if (it != c->tuned_grid.end()) b->tuned = it->second;
+        if (it != c->tuned_grid.end()) {
+            b->tuned_dir = it->second / 64;
+            b->tuned_upd = it->second % 64;
+        }
     }
     // autotuner candidates: grid = all items (one item per workgroup), or
-    // 16, 8, 6, 4, 3, 2 workgroups per CU (each a contiguous run of items).
-    // Phase 2j captures candidate j and runs it once, phase 2j+1 times a
-    // replay; every launch is followed by the regular check, so the check
-    // cadence (and with it every result) is that of the single solve.
+    // 16, 8, 6, 4, 3, 2 workgroups per CU (each a contiguous run of items),
+    // one factor for both kernels: the order groups run concurrently in the
+    // graph, so the whole chain is timed (a kernel's best grid alone is not
+    // its best beside the other groups).  Phase 2j captures candidate j and
+    // runs it once, phase 2j+1 times a replay; every launch is followed by
+    // the regular check, so the check cadence (and with it every result) is
+    // that of the single solve.
     static const int cand[] = {0, 16, 8, 6, 4, 3, 2};
     constexpr int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
-    int phase = b->tuned ? -1 : 0;
-    set_grids(b, b->tuned > 0 ? b->tuned - 1 : 0);
+    int phase = b->tuned_dir ? -1 : 0;
+    if (phase < 0) set_grids(b, b->tuned_dir - 1, b->tuned_upd - 1);
     while (running > 0) {
+        const int ci = phase / 2;
         if (phase >= 0 && phase % 2 == 0) {
-            set_grids(b, cand[phase / 2]);
+            set_grids(b, cand[ci], cand[ci]);
             b->graph_key = -1;
         }
         if (phase >= 0 && phase % 2 == 1) SE_HIP(hipEventRecord(b->t0, s));
@@ -345,16 +355,16 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
             SE_HIP(hipEventSynchronize(b->t1));
             float ms = 0.f;
             (void)hipEventElapsedTime(&ms, b->t0, b->t1);
-            if (phase == 1 || ms < b->tune_best) {
+            if (ci == 0 || ms < b->tune_best) {
                 b->tune_best = ms;
-                b->tuned = cand[phase / 2] + 1;
+                b->tuned_dir = b->tuned_upd = cand[ci] + 1;
             }
         }
         if (phase >= 0 && ++phase == 2 * ncand) {
             phase = -1;
-            set_grids(b, b->tuned - 1);
+            set_grids(b, b->tuned_dir - 1, b->tuned_upd - 1);
             b->graph_key = -1;
-            c->tuned_grid[shape] = b->tuned;
+            c->tuned_grid[shape] = b->tuned_dir * 64 + b->tuned_upd;
         }
         rc = pull();
         if (rc) return rc;
@@ -522,9 +532,10 @@ extern "C" int se_batch_destroy(se_batch* b) {
 
 extern "C" int se_batch_size(const se_batch* b) { return b ? (int)b->ops.size() : 0; }
 
-extern "C" int se_batch_launch_shape(const se_batch* b, int* workgroups_per_cu) {
-    SE_CHECK(b && workgroups_per_cu, SE_ERR_ARG, "se_batch_launch_shape: NULL argument");
-    *workgroups_per_cu = b->tuned - 1;
+extern "C" int se_batch_launch_shape(const se_batch* b, int* dir_per_cu, int* update_per_cu) {
+    SE_CHECK(b && dir_per_cu && update_per_cu, SE_ERR_ARG, "se_batch_launch_shape: NULL argument");
+    *dir_per_cu = b->tuned_dir - 1;
+    *update_per_cu = b->tuned_upd - 1;
     return SE_OK;
 }
 
@@ -556,18 +567,18 @@ int batch_probe(se_batch* b, int reps, int max_groups, int* orders, double* us_d
         double n = 0.0;
         for (int i : g.members) n += b->ops[i]->mesh->n;
         float ms_d = 0.f, ms_u = 0.f;
-        int rc = g_dir_apply<T, TCG_NORMAL>(g, nw, g.d_tiles.as<int2>(), g.grid > 0 ? g.grid : nw, s);
-        if (!rc) rc = g_update<T>(g, nw, g.d_tiles.as<int2>(), g.grid > 0 ? g.grid : nw, s);
+        int rc = g_dir_apply<T, TCG_NORMAL>(g, nw, g.d_tiles.as<int2>(), g.grid_dir > 0 ? g.grid_dir : nw, s);
+        if (!rc) rc = g_update<T>(g, nw, g.d_tiles.as<int2>(), g.grid_upd > 0 ? g.grid_upd : nw, s);
         if (rc) return rc;
         SE_HIP(hipEventRecord(b->t0, s));
         for (int r = 0; r < reps && !rc; r++)
-            rc = g_dir_apply<T, TCG_NORMAL>(g, nw, g.d_tiles.as<int2>(), g.grid > 0 ? g.grid : nw, s);
+            rc = g_dir_apply<T, TCG_NORMAL>(g, nw, g.d_tiles.as<int2>(), g.grid_dir > 0 ? g.grid_dir : nw, s);
         SE_HIP(hipEventRecord(b->t1, s));
         SE_HIP(hipEventSynchronize(b->t1));
         (void)hipEventElapsedTime(&ms_d, b->t0, b->t1);
         SE_HIP(hipEventRecord(b->t0, s));
         for (int r = 0; r < reps && !rc; r++)
-            rc = g_update<T>(g, nw, g.d_tiles.as<int2>(), g.grid > 0 ? g.grid : nw, s);
+            rc = g_update<T>(g, nw, g.d_tiles.as<int2>(), g.grid_upd > 0 ? g.grid_upd : nw, s);
         SE_HIP(hipEventRecord(b->t1, s));
         SE_HIP(hipEventSynchronize(b->t1));
         (void)hipEventElapsedTime(&ms_u, b->t0, b->t1);

@@ -94,13 +94,15 @@ struct TcgPart {
 };
 
 // Owned cells of a tile: lane j owns column I0 + j, wave g rows J0 + g*RPT + k.
+// Tiles are numbered column-major (consecutive tiles are vertical
+// neighbours), so that a run of consecutive tiles is a vertical strip.
 struct TcgTile {
     int I0, J0, I, lane, wv;
     bool colok;
     __device__ __forceinline__ TcgTile(const SepParams& S, int tile) {
-        const int tilesX = (S.nU0 + SEP_TX - 1) / SEP_TX;
-        I0 = (tile % tilesX) * SEP_TX;
-        J0 = (tile / tilesX) * TCG_TY;
+        const int tilesY = (S.nU1 + TCG_TY - 1) / TCG_TY;
+        I0 = (tile / tilesY) * SEP_TX;
+        J0 = (tile % tilesY) * TCG_TY;
         lane = threadIdx.x & 63;
         wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         I = I0 + lane;
@@ -166,8 +168,6 @@ struct TcgDir {
     static constexpr int W = SEP_TX + 2 * N, H = TY + 2 * N, WP = W + 1;
     static constexpr int RL = (H + NW - 1) / NW;          // main-column rows per wave
     static constexpr int NX = (H * 2 * N + NT - 1) / NT;  // right-halo cells per thread
-    static constexpr int NG = (TY * B + NT - 1) / NT;
-    static_assert(2 * TY <= NT, "one thread per owned row and line factor");
 
     // ---- the member's totals and state (once per member and workgroup run) --
     struct Setup {
@@ -244,130 +244,156 @@ struct TcgDir {
         }
     };
 
-    // ---- one tile: its loads (issued first), then the work ------------------
-    // Halo cells outside the domain (non-periodic sides) load a clamped, valid
-    // cell: the band tables hold zero weights for neighbours outside the
-    // domain, so those finite values never reach q.
-    float gyv[NG];
-    float sm;  // thread t < TY: sy of owned row t; TY <= t < 2 TY: my of row t - TY
-    float z0[RL], p0[RL], zx[NX], px[NX];
-    float gxr[B];
-    float sxI, mxI;
+    // ---- one tile ---------------------------------------------------------
+    // Tiles are numbered column-major within a member, so a workgroup's run
+    // of consecutive items is a vertical strip.  The window of a tile is its
+    // 32 owned rows with N halo rows above and below (H rows of 64 + 2N
+    // columns); the next tile of the same strip shares 2N of those rows, so
+    // it copies them from the previous window and loads and builds only its
+    // 32 new rows.  Halo cells outside the domain (non-periodic sides) load a
+    // clamped, valid cell: the band tables hold zero weights for neighbours
+    // outside the domain, so those finite values never reach q.
+    static constexpr int SU = H * WP;  // floats of one window in LDS
 
-    __device__ __forceinline__ void load(const TcgMember& M, int tile) {
+    // inputs of the window rows a >= a0 (a0 = 0: every row; a0 = 2N: the
+    // rows a strip's next tile adds), held in registers until built
+    float zr[RL], pr[RL], zh[NX], ph[NX];
+    int a0;
+
+    __device__ __forceinline__ void load(const TcgMember& M, int tile, bool cont) {
         const SepParams& S = M.S;
         const TcgTab tb(S);
         cgp<float> z = (cgp<float>)M.z;
         cgp<T> p = (cgp<T>)M.p;
         const TcgTile t(S, tile);
         const int lane = t.lane, wv = t.wv, I0 = t.I0, J0 = t.J0;
-#pragma unroll
-        for (int k = 0; k < NG; k++) {
-            const int i = threadIdx.x + NT * k;
-            const bool ok = i < TY * B && J0 + i / B < S.nU1;
-            const float v = tb.gy[ok ? (size_t)J0 * B + i : 0];
-            gyv[k] = ok ? v : 0.0f;
-        }
-        {
-            const int i = threadIdx.x & (TY - 1);
-            const int J = min(J0 + i, S.nU1 - 1);
-            sm = threadIdx.x < TY ? tb.sy[J] : tb.my[J];
-        }
-        // main columns: lane l holds tile column l of rows wv + NW*k
+        a0 = cont ? 2 * N : 0;
+        // main columns: lane l holds window column l of rows a0 + wv + NW*k
         int Ic0 = I0 - N + lane;
         if (S.per0) Ic0 = Ic0 < 0 ? Ic0 + S.nU0 : (Ic0 >= S.nU0 ? Ic0 - S.nU0 : Ic0);
         Ic0 = min(max(Ic0, 0), S.nU0 - 1);
 #pragma unroll
         for (int k = 0; k < RL; k++) {
-            const int a = wv + NW * k;
-            int J = J0 - N + a;
-            if (S.per1) J = J < 0 ? J + S.nU1 : (J >= S.nU1 ? J - S.nU1 : J);
-            J = min(max(J, 0), S.nU1 - 1);
-            const size_t rowo = (size_t)J * S.nU0;
-            z0[k] = z[rowo + Ic0];
-            if (!NOP) p0[k] = ldf<T>(p, rowo + Ic0);
+            const int a = a0 + wv + NW * k;
+            if (a < H) {
+                int J = J0 - N + a;
+                if (S.per1) J = J < 0 ? J + S.nU1 : (J >= S.nU1 ? J - S.nU1 : J);
+                J = min(max(J, 0), S.nU1 - 1);
+                const size_t rowo = (size_t)J * S.nU0;
+                zr[k] = z[rowo + Ic0];
+                if (!NOP) pr[k] = ldf<T>(p, rowo + Ic0);
+            }
         }
-        // right-halo columns 64 .. 64+2N-1, spread over all threads
+        // right-halo columns 64 .. 64+2N-1 of those rows, spread over all threads
+        const int nh = (H - a0) * 2 * N;
 #pragma unroll
         for (int k = 0; k < NX; k++) {
-            const int i = min((int)threadIdx.x + NT * k, H * 2 * N - 1);
-            const int a = i / (2 * N), cc = 64 + i % (2 * N);
-            int J = J0 - N + a, I = I0 - N + cc;
-            if (S.per1) J = J < 0 ? J + S.nU1 : (J >= S.nU1 ? J - S.nU1 : J);
-            if (S.per0) I = I >= S.nU0 ? I - S.nU0 : I;
-            J = min(max(J, 0), S.nU1 - 1);
-            I = min(I, S.nU0 - 1);
-            const size_t o = (size_t)J * S.nU0 + I;
-            zx[k] = z[o];
-            px[k] = NOP ? 0.0f : ldf<T>(p, o);
+            const int i = threadIdx.x + NT * k;
+            if (i < nh) {
+                const int a = a0 + i / (2 * N), cc = 64 + i % (2 * N);
+                int J = J0 - N + a, I = I0 - N + cc;
+                if (S.per1) J = J < 0 ? J + S.nU1 : (J >= S.nU1 ? J - S.nU1 : J);
+                if (S.per0) I = I >= S.nU0 ? I - S.nU0 : I;
+                J = min(max(J, 0), S.nU1 - 1);
+                I = min(I, S.nU0 - 1);
+                const size_t o = (size_t)J * S.nU0 + I;
+                zh[k] = z[o];
+                ph[k] = NOP ? 0.0f : ldf<T>(p, o);
+            }
         }
-        const int Ic = t.colok ? t.I : 0;
-#pragma unroll
-        for (int s = 0; s < B; s++) gxr[s] = tb.gx[(size_t)Ic * B + s];
-        sxI = tb.sx[Ic];
-        mxI = tb.mx[Ic];
     }
 
-    // p' = z + beta p on the tile and its halo, q = A~ p' on the owned
-    // cells, partial p'.q.  Starts with every thread past its last LDS access
-    // of the previous tile (the caller's barrier).
-    __device__ __forceinline__ void compute(const TcgMember& M, int tile, float fb, float fmu) {
-        __shared__ float s_u[H * WP];
-        __shared__ float s_gy[TY * B];  // row bands of the owned rows
-        __shared__ float s_sm[2 * TY];  // sy, my of the owned rows
-        const SepParams& S = M.S;
-        gp<float> q = (gp<float>)M.q;
-        const TcgTile t(S, tile);
-        const int lane = t.lane, wv = t.wv, J0 = t.J0;
-#pragma unroll
-        for (int k = 0; k < NG; k++) {
-            const int i = threadIdx.x + NT * k;
-            if (i < TY * B) s_gy[i] = gyv[k];
+    // the window in su: rows a < a0 from the previous window prev (its rows
+    // TY .. TY+2N-1), rows a >= a0 as p' = z + beta p of the loaded inputs
+    __device__ __forceinline__ void build(float* su, const float* prev, float fb, float fmu) const {
+        const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        if (a0 > 0) {
+            for (int i = threadIdx.x; i < 2 * N * W; i += NT) {
+                const int a = i / W, c = i - a * W;
+                su[a * WP + c] = prev[(a + TY) * WP + c];
+            }
         }
-        if (threadIdx.x < 2 * TY) s_sm[threadIdx.x] = sm;
-
-        // ---- new direction on the tile and its halo ----
 #pragma unroll
         for (int k = 0; k < RL; k++) {
-            const int a = wv + NW * k;
-            if (H % NW == 0 || a < H)  // (H % NW == 0: every a < H)
-                s_u[a * WP + lane] = tcg_pnew<T>(z0[k], fmu, fb, NOP ? 0.0f : p0[k]);
+            const int a = a0 + wv + NW * k;
+            if (a < H) su[a * WP + lane] = tcg_pnew<T>(zr[k], fmu, fb, NOP ? 0.0f : pr[k]);
         }
+        const int nh = (H - a0) * 2 * N;
 #pragma unroll
         for (int k = 0; k < NX; k++) {
             const int i = threadIdx.x + NT * k;
-            if (i < H * 2 * N) s_u[(i / (2 * N)) * WP + 64 + i % (2 * N)] = tcg_pnew<T>(zx[k], fmu, fb, px[k]);
+            if (i < nh) su[(a0 + i / (2 * N)) * WP + 64 + i % (2 * N)] = tcg_pnew<T>(zh[k], fmu, fb, ph[k]);
         }
-        __syncthreads();
+    }
 
-        // ---- q = A~ p' on the owned cells ----
-        // One row at a time (not unrolled): each row reads its 2N+1 x- and
-        // y-neighbours from LDS, so only one row's operands are live and the
-        // kernel keeps several waves per SIMD to hide memory latency.
-        const int j = lane, g = wv;
+    // the owned column's band row and line factors (the same for every tile
+    // of a strip)
+    struct Col {
+        float gxr[B];
+        float sxI, mxI;
+        __device__ __forceinline__ void load(const TcgMember& M, int tile) {
+            const TcgTab tb(M.S);
+            const TcgTile t(M.S, tile);
+            const int Ic = t.colok ? t.I : 0;
+#pragma unroll
+            for (int s = 0; s < B; s++) gxr[s] = tb.gx[(size_t)Ic * B + s];
+            sxI = tb.sx[Ic];
+            mxI = tb.mx[Ic];
+        }
+    };
+
+    // q = A~ p' on the owned cells of the built window su, partial p'.q.
+    // Two rows per step (not unrolled further): they share one y-window of
+    // 2N+2 values, and the rows' band coefficients and line factors are
+    // wave-uniform scalar loads (SGPR operands), so LDS carries only the
+    // window and the x-neighbours and only a step's operands are live.
+    static __device__ __forceinline__ void stencil(const TcgMember& M, int tile, const float* su, const Col& cl) {
+        static_assert(RPT % 2 == 0, "rows are taken in pairs");
+        const SepParams& S = M.S;
+        const TcgTab tb(S);
+        gp<float> q = (gp<float>)M.q;
+        const TcgTile t(S, tile);
+        const int j = t.lane, g = t.wv, J0 = t.J0;
         double dot = 0.0;
 #pragma unroll 1
-        for (int k = 0; k < RPT; k++) {
+        for (int k = 0; k < RPT; k += 2) {
             const int row = g * RPT + k;
-            const float* xr = &s_u[(row + N) * WP + j];  // row row+N, columns j .. j+2N
-            const float* yc = &s_u[row * WP + j + N];    // column j+N, rows row .. row+2N
-            const float* gyr = &s_gy[row * B];           // one address per wave
-            float ax = 0.0f, ay = 0.0f;
+            const int Ja = min(J0 + row, S.nU1 - 1), Jb = min(J0 + row + 1, S.nU1 - 1);
+            const kp<float> ga = tb.gy + (size_t)Ja * B, gb = tb.gy + (size_t)Jb * B;
+            const float* yc = &su[row * WP + j + N];    // column j+N, rows row .. row+2N+1
+            const float* xa = &su[(row + N) * WP + j];  // row row+N, columns j .. j+2N
+            const float* xb = xa + WP;
+            float win[2 * N + 2];
+#pragma unroll
+            for (int s = 0; s < 2 * N + 2; s++) win[s] = yc[s * WP];
+            float axa = 0.0f, axb = 0.0f, aya = 0.0f, ayb = 0.0f;
 #pragma unroll
             for (int s = 0; s < B; s++) {
-                ax += gxr[s] * xr[s];
-                ay += gyr[s] * yc[s * WP];
+                axa += cl.gxr[s] * xa[s];
+                axb += cl.gxr[s] * xb[s];
+                aya += ga[s] * win[s];
+                ayb += gb[s] * win[s + 1];
             }
-            const float u = yc[N * WP];
-            const float y = s_sm[row] * ax + sxI * ay + (mxI * s_sm[TY + row]) * u;
-            const int J = J0 + row;
-            if (t.colok && J < S.nU1) {
-                q[(size_t)J * S.nU0 + t.I] = y;
-                dot += (double)u * (double)y;
+            const float ua = win[N], ub = win[N + 1];
+            const float ya = tb.sy[Ja] * axa + cl.sxI * aya + (cl.mxI * tb.my[Ja]) * ua;
+            const float yb = tb.sy[Jb] * axb + cl.sxI * ayb + (cl.mxI * tb.my[Jb]) * ub;
+            if (t.colok && J0 + row < S.nU1) {
+                q[(size_t)(J0 + row) * S.nU0 + t.I] = ya;
+                dot += (double)ua * (double)ya;
+            }
+            if (t.colok && J0 + row + 1 < S.nU1) {
+                q[(size_t)(J0 + row + 1) * S.nU0 + t.I] = yb;
+                dot += (double)ub * (double)yb;
             }
         }
         double v[1] = {dot};
         block_store_partials_at<1, TCG_NT>(v, M.part_a(), tile, M.tiles);
+    }
+
+    // the next item continues this tile's strip
+    static __device__ __forceinline__ bool next_in_strip(const SepParams& S, int tile, int next) {
+        const int tilesY = (S.nU1 + TY - 1) / TY;
+        return next == tile + 1 && next % tilesY != 0;
     }
 };
 
@@ -375,13 +401,18 @@ struct TcgDir {
 template <int N, typename T, int MODE>
 __device__ __forceinline__ void tcg_dir_apply_tile(const TcgMember& M, int tile) {
     using D = TcgDir<N, T, MODE>;
+    __shared__ float s_u[D::SU];
     D d;
-    d.load(M, tile);
+    d.load(M, tile, false);
+    typename D::Col cl;
+    cl.load(M, tile);
     typename D::Setup su;
     su.load(M);
     float fb = 0.0f, fmu = 0.0f;
     if (!su.run(M, tile, fb, fmu)) return;
-    d.compute(M, tile, fb, fmu);
+    d.build(s_u, s_u, fb, fmu);
+    __syncthreads();
+    D::stencil(M, tile, s_u, cl);
 }
 
 // ---------------------------------------------------------------------------
@@ -630,25 +661,40 @@ __global__ __launch_bounds__(TCG_NT) void k_tcg_dir_apply_batch(const TcgMember*
                                                                 const int2* __restrict__ work, int nwork,
                                                                 const int* __restrict__ nwork_dev) {
     using D = TcgDir<N, T, MODE>;
+    __shared__ float s_u[2][D::SU];  // windows of consecutive tiles (ping-pong)
     nwork = work_count(nwork, nwork_dev);
     int w0, w1;
     tcg_run(nwork, w0, w1);
+    if (w0 >= w1) return;
     int cur = -1;
     bool go = false;
     float fb = 0.0f, fmu = 0.0f;
+    D d;
+    typename D::Col cl;
+    int2 it = work[w0];
+    d.load(Ms[it.x], it.y, false);
     for (int w = w0; w < w1; w++) {
-        const int2 it = work[w];
         const TcgMember& M = Ms[it.x];
-        __syncthreads();  // the previous tile's LDS reads are done
-        D d;
-        d.load(M, it.y);
-        if (it.x != cur) {  // a new member: its totals (loads in flight with the tile's)
+        const bool cont = d.a0 > 0;  // this tile continues the previous tile's strip
+        if (!cont) cl.load(M, it.y);
+        if (it.x != cur) {  // a new member: its totals
             typename D::Setup su;
             su.load(M);
             go = su.run(M, it.y, fb, fmu);
             cur = it.x;
         }
-        if (go) d.compute(M, it.y, fb, fmu);
+        float* sb = s_u[w & 1];
+        if (go) d.build(sb, s_u[(w & 1) ^ 1], fb, fmu);
+        __syncthreads();  // the window is built
+        // the next item's inputs are loaded while this tile computes
+        int2 itn = it;
+        if (w + 1 < w1) {
+            itn = work[w + 1];
+            const bool contn = go && itn.x == it.x && D::next_in_strip(M.S, it.y, itn.y);
+            d.load(Ms[itn.x], itn.y, contn);
+        }
+        if (go) D::stencil(M, it.y, sb, cl);  // (ends with a barrier)
+        it = itn;
     }
 }
 template <int N, typename T>

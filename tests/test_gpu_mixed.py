@@ -71,7 +71,9 @@ def test_refined_solve_meets_fp64_target(case, prec):
     x, info = h.pcg(f, rtol=1e-10, null_space=null, precision=prec)
     if prec == se_amd.PREC_BF16_IR and case["label"] in BF16_UNRESOLVABLE:
         assert info["status"] == se_amd.SOLVE_STAGNATED, info
-        assert info["iters"] <= 4000
+        # declared within 32 reliable-update windows of 256 iterations, far
+        # below the 1000-update limit it would otherwise run to
+        assert info["iters"] <= 32 * 256, info
         return
     assert info["status"] == se_amd.SOLVE_CONVERGED, info
     assert info["true_rel_residual"] <= 1e-10
