@@ -221,7 +221,7 @@ TcgMember tcg_member(se_op* op, const double* d_f, double rtol, int null_space, 
     M.f = d_f;
     M.part = op->sep_part.as<double>();
     M.st = op->sep_state.as<CgState>();
-    M.tiles = sep_tiles(op->mesh);
+    M.tiles = tcg_tiles(op->mesh);
     M.np_s = np_s;
     M.null_space = null_space;
     M.n = op->mesh->n;
@@ -230,7 +230,9 @@ TcgMember tcg_member(se_op* op, const double* d_f, double rtol, int null_space, 
     return M;
 }
 
-int tcg_tiles(const se_mesh* m) { return sep_tiles(m); }
+int tcg_tiles(const se_mesh* m) {
+    return ((m->nU[0] + TCG_TX - 1) / TCG_TX) * ((m->nU[1] + TCG_TY - 1) / TCG_TY);
+}
 }  // namespace se
 
 namespace {

@@ -27,7 +27,11 @@ struct se_op;
 
 namespace se {
 
-constexpr int TCG_NT = 256, TCG_TY = 32, TCG_RPT = TCG_TY / (TCG_NT / 64);
+// Tiles of TCG_TX x TCG_TY unknowns, one workgroup of eight waves each: wave w
+// takes the 64 columns of half (w & 1) and the TCG_RPT rows of group (w >> 1).
+constexpr int TCG_TX = 128, TCG_TY = 32, TCG_NT = 512;
+constexpr int TCG_RG = TCG_NT / 64 / (TCG_TX / 64);  // row groups
+constexpr int TCG_RPT = TCG_TY / TCG_RG;
 constexpr int TCG_PRE = 2;  // preloaded partials per thread and value
 
 // direction kernel modes: FIRST starts a solve (p = z, sets the reference
@@ -93,22 +97,25 @@ struct TcgPart {
     }
 };
 
-// Owned cells of a tile: lane j owns column I0 + j, wave g rows J0 + g*RPT + k.
-// Tiles are numbered column-major (consecutive tiles are vertical
-// neighbours), so that a run of consecutive tiles is a vertical strip.
+// Owned cells of a tile: lane j of wave w owns column I0 + 64 (w & 1) + j and
+// rows J0 + (w >> 1) RPT + k.  Tiles are numbered column-major (consecutive
+// tiles are vertical neighbours), so that a run of consecutive tiles is a
+// vertical strip.
 struct TcgTile {
-    int I0, J0, I, lane, wv;
+    int I0, J0, I, c, lane, wv, g;  // c: column within the tile, g: row group
     bool colok;
     __device__ __forceinline__ TcgTile(const SepParams& S, int tile) {
         const int tilesY = (S.nU1 + TCG_TY - 1) / TCG_TY;
-        I0 = (tile / tilesY) * SEP_TX;
+        I0 = (tile / tilesY) * TCG_TX;
         J0 = (tile % tilesY) * TCG_TY;
         lane = threadIdx.x & 63;
         wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-        I = I0 + lane;
+        g = wv >> 1;
+        c = 64 * (wv & 1) + lane;
+        I = I0 + c;
         colok = I < S.nU0;
     }
-    __device__ __forceinline__ int row(int k) const { return J0 + wv * TCG_RPT + k; }
+    __device__ __forceinline__ int row(int k) const { return J0 + g * TCG_RPT + k; }
 };
 
 // One solve: the separable tables, the vectors and the partial-sum areas
@@ -164,9 +171,9 @@ template <int N, typename T, int MODE>
 struct TcgDir {
     static constexpr bool FIRST = MODE == TCG_FIRST, NOP = MODE != TCG_NORMAL;  // NOP: the old p is not used
     static constexpr int B = 2 * N + 1;
-    static constexpr int TY = TCG_TY, NT = TCG_NT, NW = NT / 64, RPT = TCG_RPT;
-    static constexpr int W = SEP_TX + 2 * N, H = TY + 2 * N, WP = W + 1;
-    static constexpr int RL = (H + NW - 1) / NW;          // main-column rows per wave
+    static constexpr int TY = TCG_TY, TX = TCG_TX, NT = TCG_NT, RG = TCG_RG, RPT = TCG_RPT;
+    static constexpr int W = TX + 2 * N, H = TY + 2 * N, WP = W + 1;
+    static constexpr int RL = (H + RG - 1) / RG;          // main-column window rows per thread
     static constexpr int NX = (H * 2 * N + NT - 1) / NT;  // right-halo cells per thread
 
     // ---- the member's totals and state (once per member and workgroup run) --
@@ -266,15 +273,16 @@ struct TcgDir {
         cgp<float> z = (cgp<float>)M.z;
         cgp<T> p = (cgp<T>)M.p;
         const TcgTile t(S, tile);
-        const int lane = t.lane, wv = t.wv, I0 = t.I0, J0 = t.J0;
+        const int c = t.c, g = t.g, I0 = t.I0, J0 = t.J0;
         a0 = cont ? 2 * N : 0;
-        // main columns: lane l holds window column l of rows a0 + wv + NW*k
-        int Ic0 = I0 - N + lane;
+        // main columns: the thread of tile column c holds window column c of
+        // rows a0 + g + RG*k
+        int Ic0 = I0 - N + c;
         if (S.per0) Ic0 = Ic0 < 0 ? Ic0 + S.nU0 : (Ic0 >= S.nU0 ? Ic0 - S.nU0 : Ic0);
         Ic0 = min(max(Ic0, 0), S.nU0 - 1);
 #pragma unroll
         for (int k = 0; k < RL; k++) {
-            const int a = a0 + wv + NW * k;
+            const int a = a0 + g + RG * k;
             if (a < H) {
                 int J = J0 - N + a;
                 if (S.per1) J = J < 0 ? J + S.nU1 : (J >= S.nU1 ? J - S.nU1 : J);
@@ -284,13 +292,13 @@ struct TcgDir {
                 if (!NOP) pr[k] = ldf<T>(p, rowo + Ic0);
             }
         }
-        // right-halo columns 64 .. 64+2N-1 of those rows, spread over all threads
+        // right-halo columns TX .. TX+2N-1 of those rows, spread over all threads
         const int nh = (H - a0) * 2 * N;
 #pragma unroll
         for (int k = 0; k < NX; k++) {
             const int i = threadIdx.x + NT * k;
             if (i < nh) {
-                const int a = a0 + i / (2 * N), cc = 64 + i % (2 * N);
+                const int a = a0 + i / (2 * N), cc = TX + i % (2 * N);
                 int J = J0 - N + a, I = I0 - N + cc;
                 if (S.per1) J = J < 0 ? J + S.nU1 : (J >= S.nU1 ? J - S.nU1 : J);
                 if (S.per0) I = I >= S.nU0 ? I - S.nU0 : I;
@@ -306,7 +314,8 @@ struct TcgDir {
     // the window in su: rows a < a0 from the previous window prev (its rows
     // TY .. TY+2N-1), rows a >= a0 as p' = z + beta p of the loaded inputs
     __device__ __forceinline__ void build(float* su, const float* prev, float fb, float fmu) const {
-        const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        const int c = 64 * (wv & 1) + (threadIdx.x & 63), g = wv >> 1;
         if (a0 > 0) {
             for (int i = threadIdx.x; i < 2 * N * W; i += NT) {
                 const int a = i / W, c = i - a * W;
@@ -315,14 +324,14 @@ struct TcgDir {
         }
 #pragma unroll
         for (int k = 0; k < RL; k++) {
-            const int a = a0 + wv + NW * k;
-            if (a < H) su[a * WP + lane] = tcg_pnew<T>(zr[k], fmu, fb, NOP ? 0.0f : pr[k]);
+            const int a = a0 + g + RG * k;
+            if (a < H) su[a * WP + c] = tcg_pnew<T>(zr[k], fmu, fb, NOP ? 0.0f : pr[k]);
         }
         const int nh = (H - a0) * 2 * N;
 #pragma unroll
         for (int k = 0; k < NX; k++) {
             const int i = threadIdx.x + NT * k;
-            if (i < nh) su[(a0 + i / (2 * N)) * WP + 64 + i % (2 * N)] = tcg_pnew<T>(zh[k], fmu, fb, ph[k]);
+            if (i < nh) su[(a0 + i / (2 * N)) * WP + TX + i % (2 * N)] = tcg_pnew<T>(zh[k], fmu, fb, ph[k]);
         }
     }
 
@@ -353,7 +362,7 @@ struct TcgDir {
         const TcgTab tb(S);
         gp<float> q = (gp<float>)M.q;
         const TcgTile t(S, tile);
-        const int j = t.lane, g = t.wv, J0 = t.J0;
+        const int j = t.c, g = t.g, J0 = t.J0;
         double dot = 0.0;
 #pragma unroll 1
         for (int k = 0; k < RPT; k += 2) {
