@@ -243,17 +243,26 @@ int se_genvel_dev(se_mesh* mesh, const double* d_psi, double* d_vel);
 int se_calc_timestep(se_mesh* mesh, const double* vel, double courant, double* dt);
 int se_calc_timestep_dev(se_mesh* mesh, const double* d_vel, double courant, double* dt);
 
-/* ---- two-layer quasi-geostrophic step ---------------------------------------
- * One first-order step of TwoLayerQGEqn [ref: TwoLayerQGEqn::FirstOrder
- * src/qg/TwoLayerQGEqn.cpp:174-243, GenVel :335-352]: GenVel and GenPV of both
+/* ---- quasi-geostrophic steps ---------------------------------------------------
+ * Time steps of the QG models with the dissipation off and no topography.
+ * The first step after set_state is the reference's first-order step, every
+ * later one its second-order step from the state before the previous step
+ * (as the reference's drivers call Solve).  SolveQ's PV update under the
+ * diagonal GLL mass leaves Dirichlet (wall) nodes at their GenPV value, so
+ * the wall vorticity is kept.  The inversions run with the precision of
+ * `opts` (FP64 or mixed). */
+
+/* Two layers [ref: TwoLayerQGEqn::FirstOrder / SecondOrder
+ * src/qg/TwoLayerQGEqn.cpp:174-333, GenVel :335-352]: GenVel and GenPV of both
  * layers, semi-Lagrangian advection of both PVs, baroclinic Helmholtz
- * (lap - (F1+F2)) tau = a1^ - a2^, barotropic Poisson
- * lap chi = a1^ + a2^ + (F1-F2) tau - 2 beta y, then psi/omega of both layers.
- * The inversions run with the precision of `opts` (FP64 or mixed). */
+ * (lap - (F1+F2)) tau = a1 - a2, barotropic Poisson
+ * lap chi = a1 + a2 + (F1-F2) tau - 2 beta y, then psi/omega of both layers.
+ * As in the reference, the first-order step updates layer 1's PV only
+ * (SolveQ for layer 1, TwoLayerQGEqn.cpp:214). */
 typedef struct se_qg2 se_qg2;
 int se_qg2_create(se_mesh* mesh, double F1, double F2, double beta, int detj_mode, se_qg2** out);
 int se_qg2_destroy(se_qg2* q);
-/* host fields, nV each */
+/* host fields, nV each; the next step is first order */
 int se_qg2_set_state(se_qg2* q, const double* psi1, const double* omega1, const double* psi2, const double* omega2);
 int se_qg2_get_state(se_qg2* q, double* psi1, double* omega1, double* psi2, double* omega2);
 /* device addresses of the state fields (nullable outputs) */
@@ -262,6 +271,18 @@ int se_qg2_state_dev(se_qg2* q, double** psi1, double** omega1, double** psi2, d
 int se_qg2_timestep(se_qg2* q, double courant, double* dt);
 /* infos: 2 entries (baroclinic, barotropic), nullable */
 int se_qg2_step(se_qg2* q, double dt, const se_solve_opts* opts, se_solve_info* infos);
+
+/* One layer, the channel model [ref: QuasiGeostrophicEqn::Solve / FirstOrder /
+ * SecondOrder src/qg/QuasiGeostrophicEqn.cpp:83-205, GenVel :207-223, GenPV
+ * :225-232]: alpha = omega - F psi + beta y advected, (lap - F) psi =
+ * alpha - beta y, omega = alpha + F psi - beta y. */
+typedef struct se_qg se_qg;
+int se_qg_create(se_mesh* mesh, double F, double beta, int detj_mode, se_qg** out);
+int se_qg_destroy(se_qg* q);
+int se_qg_set_state(se_qg* q, const double* psi, const double* omega);
+int se_qg_get_state(se_qg* q, double* psi, double* omega);
+int se_qg_timestep(se_qg* q, double courant, double* dt);
+int se_qg_step(se_qg* q, double dt, const se_solve_opts* opts, se_solve_info* info);
 
 /* ---- request server -----------------------------------------------------------
  * No counterpart in the reference (a model driver steps one TwoLayerQGEqn at a

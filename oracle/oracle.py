@@ -70,8 +70,8 @@ def lib():
             "orc_sl_points": (None, [vp, i, _D, _D, d, _D]),
             "orc_genvel": (None, [vp, _D, _D]),
             "orc_calc_timestep": (d, [vp, _D, d]),
-            "orc_qg_step": (i, [vp, d, d, d, i, _D, _D, d, i, C.POINTER(i)]),
-            "orc_qg2_step": (i, [vp, d, d, d, d, i, _D, _D, _D, _D, d, i, C.POINTER(i)]),
+            "orc_qg_step": (i, [vp, d, d, d, i, _D, _D, vp, vp, d, i, C.POINTER(i)]),
+            "orc_qg2_step": (i, [vp, d, d, d, d, i, _D, _D, _D, _D, vp, vp, vp, vp, d, i, C.POINTER(i)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -304,16 +304,25 @@ class OracleMesh(_MeshBase):
         self.L.orc_helm_apply_ideal(self.h, c_mass, c_lap, detj_mode, _d(x), y)
         return y
 
-    def qg_step(self, F, beta, dt, psi, omega, detj_mode=1, rtol=1e-10, maxit=100000):
+    def qg_step(self, F, beta, dt, psi, omega, detj_mode=1, rtol=1e-10, maxit=100000, prev=None):
+        """One QG channel step; prev = (psi, omega) before the previous step
+        selects the second-order step."""
         psi, omega = _d(psi).copy(), _d(omega).copy()
+        pp = [_d(v) for v in prev] if prev is not None else [None, None]
         it = C.c_int()
-        st = self.L.orc_qg_step(self.h, F, beta, dt, detj_mode, psi, omega, rtol, maxit, C.byref(it))
+        st = self.L.orc_qg_step(self.h, F, beta, dt, detj_mode, psi, omega,
+                                *[None if v is None else v.ctypes.data for v in pp], rtol, maxit, C.byref(it))
         return st, it.value, psi, omega
 
-    def qg2_step(self, F1, F2, beta, dt, psi1, omega1, psi2, omega2, detj_mode=1, rtol=1e-10, maxit=100000):
+    def qg2_step(self, F1, F2, beta, dt, psi1, omega1, psi2, omega2, detj_mode=1, rtol=1e-10, maxit=100000,
+                 prev=None):
+        """One two-layer step; prev = (psi1, omega1, psi2, omega2) before the
+        previous step selects the second-order step."""
         a = [_d(v).copy() for v in (psi1, omega1, psi2, omega2)]
+        pp = [_d(v) for v in prev] if prev is not None else [None] * 4
         it = C.c_int()
-        st = self.L.orc_qg2_step(self.h, F1, F2, beta, dt, detj_mode, a[0], a[1], a[2], a[3], rtol, maxit, C.byref(it))
+        st = self.L.orc_qg2_step(self.h, F1, F2, beta, dt, detj_mode, a[0], a[1], a[2], a[3],
+                                 *[None if v is None else v.ctypes.data for v in pp], rtol, maxit, C.byref(it))
         return st, it.value, a[0], a[1], a[2], a[3]
 
 

@@ -1210,19 +1210,34 @@ double orc_calc_timestep(const orc_mesh* m, const double* vel, double courant) {
 /* callers: QG steps                                                         */
 /* ------------------------------------------------------------------------- */
 
-/* QuasiGeostrophicEqn::FirstOrder, QuasiGeostrophicEqn.cpp:117-135, with
- * nu = r = 0 and no topography: GenVel, GenPV, SL advection of the PV, the
- * Helmholtz inversion he1 and the omega-hat update (:131-136) applied as the
- * new state.  With the dissipation off, SolveQ (:138) makes alpha = q_SL on
- * every unknown, so the reference's second inversion he2 (:141-147) repeats
- * he1 on the same unknown-row RHS and is not repeated here.  Dirichlet values
- * of psi are the wall data (lifting).  psi / omega are nV fields updated in
- * place.  Returns the solve status. */
-int orc_qg_step(const orc_mesh* m, double F, double beta, double dt, int detj_mode,
-                double* psi, double* omega, double rtol, int maxit, int* iters) {
+/* The QG steps with the dissipation off (nu = r = 0) and no topography.
+ *
+ * QuasiGeostrophicEqn::FirstOrder (QuasiGeostrophicEqn.cpp:117-157): GenVel,
+ * GenPV (:225-232), SL advection of the PV; SolveQ (:277-301) solves the
+ * mass-matrix system for alpha, which under the diagonal GLL mass is alpha =
+ * alpha^ on the unknowns, and Vector::UpdateField (Vector.cpp:40-61) writes
+ * only the unknowns (and the periodic copies), so at Dirichlet nodes alpha
+ * keeps its GenPV value; he2 inverts (lap - F) psi = alpha - beta y with the
+ * wall data of psi, and omega = alpha + F psi - beta y (:141-154).  The first
+ * inversion he1 (:126-136) only feeds the dissipation terms and is not
+ * repeated here.
+ * SecondOrder (:159-205, Solve :83-115): the same with the PV of the previous
+ * state and second-order SL advection (2 alpha(x1) - alpha_prev(x2) / 2, with
+ * extrapolated velocities, Advector.cpp:203-246), and SolveQ on a 1.5 x mass
+ * matrix (AssembleQMat(1.5)), so alpha = alpha^ / 1.5 on the unknowns.
+ * psi / omega are nV fields updated in place; psi_prev / omega_prev (NULL
+ * for the first-order step) are the state before the previous step.
+ * Returns the solve status. */
+static int qg_dirichlet(const orc_mesh* m, int v) { return m->f2v[v] < 0; }
+
+int orc_qg_step(const orc_mesh* m, double F, double beta, double dt, int detj_mode, double* psi, double* omega,
+                const double* psi_prev, const double* omega_prev, double rtol, int maxit, int* iters) {
     size_t nV = (size_t)m->nV, v;
+    const int second = psi_prev && omega_prev;
     double* vel = (double*)malloc(sizeof(double) * 2 * nV);
+    double* velp = (double*)malloc(sizeof(double) * 2 * nV);
     double* alpha = (double*)malloc(sizeof(double) * nV);
+    double* alphap = (double*)malloc(sizeof(double) * nV);
     double* qsl = (double*)malloc(sizeof(double) * nV);
     double* rhs = (double*)malloc(sizeof(double) * nV);
     orc_helm* h = orc_helm_create(m, -F, -1.0 * 1.0, detj_mode);
@@ -1230,30 +1245,53 @@ int orc_qg_step(const orc_mesh* m, double F, double beta, double dt, int detj_mo
     int st;
     orc_genvel(m, psi, vel);
     for (v = 0; v < nV; v++) alpha[v] = omega[v] - F * psi[v] + beta * m->verts[2 * v + 1];
-    orc_advect_first(m, alpha, 1, vel, dt, qsl, NULL, NULL);
-    for (v = 0; v < nV; v++) rhs[v] = qsl[v] - beta * m->verts[2 * v + 1];
+    if (second) {
+        orc_genvel(m, psi_prev, velp);
+        for (v = 0; v < nV; v++) alphap[v] = omega_prev[v] - F * psi_prev[v] + beta * m->verts[2 * v + 1];
+        orc_advect_second(m, alpha, alphap, 1, vel, velp, dt, qsl);
+    } else {
+        orc_advect_first(m, alpha, 1, vel, dt, qsl, NULL, NULL);
+    }
+    /* SolveQ + UpdateField: the unknowns take alpha^ / a, Dirichlet nodes keep alpha */
+    for (v = 0; v < nV; v++)
+        if (!qg_dirichlet(m, (int)v)) alpha[v] = second ? qsl[v] / 1.5 : qsl[v];
+    for (v = 0; v < nV; v++) rhs[v] = alpha[v] - beta * m->verts[2 * v + 1];
     orc_helm_assemble(h, psi);
     st = orc_helm_solve(h, rhs, psi, rtol, maxit, 0, iters, &rr);
-    for (v = 0; v < nV; v++) omega[v] = qsl[v] + F * psi[v] - beta * m->verts[2 * v + 1];
+    for (v = 0; v < nV; v++) omega[v] = alpha[v] + F * psi[v] - beta * m->verts[2 * v + 1];
     orc_helm_destroy(h);
-    free(vel); free(alpha); free(qsl); free(rhs);
+    free(vel); free(velp); free(alpha); free(alphap); free(qsl); free(rhs);
     return st;
 }
 
-/* TwoLayerQGEqn::FirstOrder, TwoLayerQGEqn.cpp:174-243 steps 1-4 (advect both
- * layers' PV, baroclinic Helmholtz for tau = psi1 - psi2 with lambda = F1+F2,
- * barotropic Poisson for chi = psi1 + psi2), dissipation off; psi/omega are
- * then updated from (tau, chi) as in steps 4/8 of the reference.  The second
- * solve pair of the reference (which repeats the first after the layer-1
- * dissipation solve) is not repeated. */
+/* TwoLayerQGEqn::FirstOrder (TwoLayerQGEqn.cpp:174-243) and SecondOrder
+ * (:245-333), dissipation off.  Both advect the two layers' PV (GenPV
+ * :326-333); the result is set by the second inversion pair (steps 6-8), which
+ * uses alpha_i after SolveQ (:396-416).  As in the one-layer step SolveQ
+ * gives alpha_i = alpha_i^ (FirstOrder) or alpha_i^ / 1.5 (SecondOrder) on
+ * the unknowns and leaves the Dirichlet nodes at their GenPV values.
+ * FirstOrder calls SolveQ for layer 1 only (:214), so layer 2's alpha keeps
+ * its GenPV value on every node: the reference's first step inverts the
+ * advected layer-1 PV against the unadvected layer-2 PV.  SecondOrder solves
+ * both layers (:291-292).  Then tau = psi1 - psi2 from
+ * (lap - (F1+F2)) tau = alpha1 - alpha2, chi = psi1 + psi2 from
+ * lap chi = alpha1 + alpha2 + (F1-F2) tau - 2 beta y, and
+ * omega_i = alpha_i - F_i (psi_j - psi_i) - beta y.  The first inversion pair
+ * (steps 2-4) only feeds the dissipation terms and is not repeated. */
 int orc_qg2_step(const orc_mesh* m, double F1, double F2, double beta, double dt, int detj_mode,
                  double* psi1, double* omega1, double* psi2, double* omega2,
+                 const double* psi1p, const double* omega1p, const double* psi2p, const double* omega2p,
                  double rtol, int maxit, int* iters) {
     size_t nV = (size_t)m->nV, v;
+    const int second = psi1p && omega1p && psi2p && omega2p;
     double* vel1 = (double*)malloc(sizeof(double) * 2 * nV);
     double* vel2 = (double*)malloc(sizeof(double) * 2 * nV);
+    double* velp1 = (double*)malloc(sizeof(double) * 2 * nV);
+    double* velp2 = (double*)malloc(sizeof(double) * 2 * nV);
     double* a1 = (double*)malloc(sizeof(double) * nV);
     double* a2 = (double*)malloc(sizeof(double) * nV);
+    double* ap1 = (double*)malloc(sizeof(double) * nV);
+    double* ap2 = (double*)malloc(sizeof(double) * nV);
     double* h1 = (double*)malloc(sizeof(double) * nV);
     double* h2 = (double*)malloc(sizeof(double) * nV);
     double* rhs = (double*)malloc(sizeof(double) * nV);
@@ -1272,14 +1310,31 @@ int orc_qg2_step(const orc_mesh* m, double F1, double F2, double beta, double dt
         tau[v] = psi1[v] - psi2[v];
         chi[v] = psi1[v] + psi2[v];
     }
-    orc_advect_first(m, a1, 1, vel1, dt, h1, NULL, NULL);
-    orc_advect_first(m, a2, 1, vel2, dt, h2, NULL, NULL);
-    for (v = 0; v < nV; v++) rhs[v] = h1[v] - h2[v];
+    if (second) {
+        orc_genvel(m, psi1p, velp1);
+        orc_genvel(m, psi2p, velp2);
+        for (v = 0; v < nV; v++) {
+            double y = m->verts[2 * v + 1];
+            ap1[v] = omega1p[v] + F1 * (psi2p[v] - psi1p[v]) + beta * y;
+            ap2[v] = omega2p[v] + F2 * (psi1p[v] - psi2p[v]) + beta * y;
+        }
+        orc_advect_second(m, a1, ap1, 1, vel1, velp1, dt, h1);
+        orc_advect_second(m, a2, ap2, 1, vel2, velp2, dt, h2);
+    } else {
+        orc_advect_first(m, a1, 1, vel1, dt, h1, NULL, NULL);
+        orc_advect_first(m, a2, 1, vel2, dt, h2, NULL, NULL);
+    }
+    for (v = 0; v < nV; v++) {
+        if (qg_dirichlet(m, (int)v)) continue;
+        a1[v] = second ? h1[v] / 1.5 : h1[v];
+        if (second) a2[v] = h2[v] / 1.5;
+    }
+    for (v = 0; v < nV; v++) rhs[v] = a1[v] - a2[v];
     orc_helm_assemble(hh, tau);
     st = orc_helm_solve(hh, rhs, tau, rtol, maxit, 0, &it1, &rr);
     for (v = 0; v < nV; v++) {
         double y = m->verts[2 * v + 1];
-        rhs[v] = h1[v] + h2[v] + (F1 - F2) * tau[v] - 2.0 * beta * y;
+        rhs[v] = a1[v] + a2[v] + (F1 - F2) * tau[v] - 2.0 * beta * y;
     }
     orc_helm_assemble(hp, chi);
     st2 = orc_helm_solve(hp, rhs, chi, rtol, maxit, 0, &it2, &rr);
@@ -1287,12 +1342,13 @@ int orc_qg2_step(const orc_mesh* m, double F1, double F2, double beta, double dt
         double y = m->verts[2 * v + 1];
         psi1[v] = 0.5 * (tau[v] + chi[v]);
         psi2[v] = 0.5 * (chi[v] - tau[v]);
-        omega1[v] = h1[v] - F1 * (psi2[v] - psi1[v]) - beta * y;
-        omega2[v] = h2[v] - F2 * (psi1[v] - psi2[v]) - beta * y;
+        omega1[v] = a1[v] - F1 * (psi2[v] - psi1[v]) - beta * y;
+        omega2[v] = a2[v] - F2 * (psi1[v] - psi2[v]) - beta * y;
     }
     if (iters) *iters = it1 + it2;
     orc_helm_destroy(hh);
     orc_helm_destroy(hp);
-    free(vel1); free(vel2); free(a1); free(a2); free(h1); free(h2); free(rhs); free(tau); free(chi);
+    free(vel1); free(vel2); free(velp1); free(velp2); free(a1); free(a2); free(ap1); free(ap2);
+    free(h1); free(h2); free(rhs); free(tau); free(chi);
     return st != 0 ? st : st2;
 }

@@ -132,6 +132,12 @@ def lib():
             "se_qg2_state_dev": (i, [vp, pvp, pvp, pvp, pvp]),
             "se_qg2_timestep": (i, [vp, d, C.POINTER(d)]),
             "se_qg2_step": (i, [vp, d, C.POINTER(SolveOpts), vp]),
+            "se_qg_create": (i, [vp, d, d, i, pvp]),
+            "se_qg_destroy": (i, [vp]),
+            "se_qg_set_state": (i, [vp, _D, _D]),
+            "se_qg_get_state": (i, [vp, _D, _D]),
+            "se_qg_timestep": (i, [vp, d, C.POINTER(d)]),
+            "se_qg_step": (i, [vp, d, C.POINTER(SolveOpts), vp]),
             "se_server_opts_default": (None, [C.POINTER(ServerOpts)]),
             "se_server_create": (i, [i, C.POINTER(ServerOpts), pvp]),
             "se_server_submit": (i, [vp, C.POINTER(QG2Request), C.POINTER(C.c_int64)]),
@@ -452,7 +458,8 @@ class Batch:
 
 
 class QG2:
-    """Two-layer QG model on a mesh (TwoLayerQGEqn first-order steps on the device)."""
+    """Two-layer QG model on a mesh (TwoLayerQGEqn steps on the device: first
+    order after set_state, second order after that)."""
 
     def __init__(self, mesh, F1, F2, beta, detj_mode=DETJ_LITERAL):
         self.mesh = mesh
@@ -484,6 +491,49 @@ class QG2:
     def close(self):
         if self.h:
             lib().se_qg2_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class QG:
+    """One-layer QG channel model on a mesh (QuasiGeostrophicEqn steps on the
+    device: first order after set_state, second order after that)."""
+
+    def __init__(self, mesh, F, beta, detj_mode=DETJ_LITERAL):
+        self.mesh = mesh
+        self.h = None
+        h = C.c_void_p()
+        check(lib().se_qg_create(mesh.h, F, beta, detj_mode, C.byref(h)), "se_qg_create")
+        self.h = h
+        mesh.ctx._own(self)
+
+    def set_state(self, psi, omega):
+        check(lib().se_qg_set_state(self.h, _d(psi), _d(omega)), "se_qg_set_state")
+
+    def state(self):
+        out = [np.zeros(self.mesh.nV) for _ in range(2)]
+        check(lib().se_qg_get_state(self.h, *out), "se_qg_get_state")
+        return out
+
+    def timestep(self, courant=0.5):
+        dt = C.c_double()
+        check(lib().se_qg_timestep(self.h, courant, C.byref(dt)), "se_qg_timestep")
+        return dt.value
+
+    def step(self, dt, **kw):
+        o = solve_opts(**kw)
+        info = SolveInfo()
+        check(lib().se_qg_step(self.h, dt, C.byref(o), C.byref(info)), "se_qg_step")
+        return info.as_dict()
+
+    def close(self):
+        if self.h:
+            lib().se_qg_destroy(self.h)
             self.h = None
 
     def __del__(self):

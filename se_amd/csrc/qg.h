@@ -9,7 +9,18 @@ struct se_qg2 {
     se_op* hh = nullptr;  // baroclinic Helmholtz (tau)
     se_op* hp = nullptr;  // barotropic Poisson (chi)
     double F1 = 0.0, F2 = 0.0, beta = 0.0;
+    int nstep = 0;  // steps since the state was set (0: the next step is first order)
     se::DevBuf psi1, om1, psi2, om2, vel1, vel2, a1, a2, h1, h2, rhs, tau, chi;
+    se::DevBuf pp1, po1, pp2, po2, ap1, ap2, vp1, vp2;  // the state before the last step, its PV, velocities
+};
+
+// one-layer QG channel model
+struct se_qg {
+    se_mesh* mesh = nullptr;
+    se_op* h = nullptr;  // (lap - F) psi
+    double F = 0.0, beta = 0.0;
+    int nstep = 0;
+    se::DevBuf psi, om, vel, a, hsl, rhs, psin, pp, po, ap, vp;
 };
 
 namespace se {

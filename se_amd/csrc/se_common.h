@@ -27,9 +27,29 @@ int fail(int code, const std::string& msg);
     } while (0)
 
 // ---- device buffer (owned by a context) ------------------------------------
+// Device allocation owned by its holder: freed by release() or on
+// destruction (the owner makes its device current first); move-only.
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+        o.ptr = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            ptr = o.ptr;
+            bytes = o.bytes;
+            o.ptr = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
     int alloc(size_t b);
     void release();
     template <class T> T* as() const { return static_cast<T*>(ptr); }

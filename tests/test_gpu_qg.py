@@ -1,7 +1,10 @@
-"""Two-layer QG step on the device against the oracle's restatement of
-TwoLayerQGEqn::FirstOrder (src/qg/TwoLayerQGEqn.cpp:174-243).
+"""QG steps on the device against the oracle's restatements: the two-layer
+TwoLayerQGEqn::FirstOrder / SecondOrder (src/qg/TwoLayerQGEqn.cpp:174-333) and
+the one-layer QuasiGeostrophicEqn::FirstOrder / SecondOrder
+(src/qg/QuasiGeostrophicEqn.cpp:83-205).  A model's first step after its
+state is set is first order, the later ones second order.
 
-The step composes pieces pinned elsewhere (GenVel bit-exact, SL advection
+The steps compose pieces pinned elsewhere (GenVel bit-exact, SL advection
 within its conditioning bound, inversions to 1e-10), so the fields agree to
 the solver tolerance amplified by the inversion's conditioning."""
 import numpy as np
@@ -34,11 +37,13 @@ def test_qg2_steps_match_oracle(gpu_ctx, detj):
     q = se_amd.QG2(m, req["F1"], req["F2"], req["beta"], detj)
     q.set_state(*st)
     dt = q.timestep(0.5)
-    ref = list(st)
+    ref, prev = list(st), None
     for k in range(3):
         infos = q.step(dt, rtol=1e-10)
         assert all(i["status"] == 0 for i in infos)
-        s, it, *ref = om.qg2_step(req["F1"], req["F2"], req["beta"], dt, *ref, detj_mode=detj, rtol=1e-10)
+        s, it, *nxt = om.qg2_step(req["F1"], req["F2"], req["beta"], dt, *ref, detj_mode=detj, rtol=1e-10,
+                                  prev=prev)
+        prev, ref = ref, nxt
         assert s == 0
         got = q.state()
         errs = [rel(a, b) for a, b in zip(got, ref)]
@@ -60,3 +65,34 @@ def test_qg2_mixed_precision_step(gpu_ctx):
         assert all(i["status"] == 0 and i["true_rel_residual"] <= 1e-10 for i in infos)
     errs = [rel(a, b) for a, b in zip(q32.state(), q64.state())]
     assert max(errs) <= 1e-8, errs
+
+
+@pytest.mark.parametrize("detj", [0, 1], ids=["literal", "physical"])
+def test_qg_channel_steps_match_oracle(gpu_ctx, detj):
+    """The one-layer channel (32x32-like configuration at test size): beta = 1,
+    F = 100, psi = 0 on the walls, Gaussian vortices; wall vorticity kept."""
+    cfg = W.qg_channel(nEls=10, N=6)
+    m = se_amd.Mesh(gpu_ctx, cfg["nEls"], cfg["N"], cfg["lo"], cfg["hi"], cfg["periodic"], cfg["bc"])
+    om = O.OracleMesh(cfg["nEls"], cfg["N"], cfg["lo"], cfg["hi"], cfg["periodic"], cfg["bc"])
+    X, Y = W.mesh_coords(cfg["nEls"], cfg["N"], cfg["lo"], cfg["hi"], m.gll()[0])
+    rng = W.Rng(cfg["seed"])
+    wall = np.repeat(np.sin(np.pi * np.asarray(Y)), len(X))
+    psi = 0.01 * W.gaussian_vortices(X, Y, cfg["vortices"], rng, cfg["lo"], cfg["hi"]) * wall
+    omega = W.gaussian_vortices(X, Y, cfg["vortices"], rng, cfg["lo"], cfg["hi"])
+    q = se_amd.QG(m, cfg["F"], cfg["beta"], detj)
+    q.set_state(psi, omega)
+    dt = q.timestep(cfg["courant"])
+    ref, prev = [psi, omega], None
+    for k in range(3):
+        info = q.step(dt, rtol=1e-10)
+        assert info["status"] == 0
+        s, it, *nxt = om.qg_step(cfg["F"], cfg["beta"], dt, *ref, detj_mode=detj, rtol=1e-10, prev=prev)
+        prev, ref = ref, nxt
+        assert s == 0
+        got = q.state()
+        errs = [rel(a, b) for a, b in zip(got, ref)]
+        print("step", k, "rel errors psi omega", ["%.2e" % e for e in errs])
+        assert max(errs) <= 1e-9, errs
+    on_wall = (np.abs(om.verts()[:, 1]) < 1e-12) | (np.abs(om.verts()[:, 1] - 1.0) < 1e-12)
+    assert np.max(np.abs(q.state()[1][on_wall] - omega[on_wall])) <= 1e-12 * np.max(np.abs(omega))
+    q.close()
