@@ -67,11 +67,13 @@ def test_qg2_mixed_precision_step(gpu_ctx):
     assert max(errs) <= 1e-8, errs
 
 
-@pytest.mark.parametrize("detj", [0, 1], ids=["literal", "physical"])
-def test_qg_channel_steps_match_oracle(gpu_ctx, detj):
-    """The one-layer channel (32x32-like configuration at test size): beta = 1,
-    F = 100, psi = 0 on the walls, Gaussian vortices; wall vorticity kept."""
-    cfg = W.qg_channel(nEls=10, N=6)
+@pytest.mark.parametrize("detj,nEls,nsteps", [(0, 10, 3), (1, 10, 3), (0, 32, 2)],
+                         ids=["literal", "physical", "literal-32x32"])
+def test_qg_channel_steps_match_oracle(gpu_ctx, detj, nEls, nsteps):
+    """The one-layer channel of the 32 x 32, p = 6 configuration (and a 10 x 10
+    mesh for both Jacobian modes): beta = 1, F = 100 (Ld = 0.1 L), psi = 0 on
+    the walls, Gaussian vortices, CFL 0.5; the wall vorticity is kept."""
+    cfg = W.qg_channel(nEls=nEls, N=6)
     m = se_amd.Mesh(gpu_ctx, cfg["nEls"], cfg["N"], cfg["lo"], cfg["hi"], cfg["periodic"], cfg["bc"])
     om = O.OracleMesh(cfg["nEls"], cfg["N"], cfg["lo"], cfg["hi"], cfg["periodic"], cfg["bc"])
     X, Y = W.mesh_coords(cfg["nEls"], cfg["N"], cfg["lo"], cfg["hi"], m.gll()[0])
@@ -83,7 +85,7 @@ def test_qg_channel_steps_match_oracle(gpu_ctx, detj):
     q.set_state(psi, omega)
     dt = q.timestep(cfg["courant"])
     ref, prev = [psi, omega], None
-    for k in range(3):
+    for k in range(nsteps):
         info = q.step(dt, rtol=1e-10)
         assert info["status"] == 0
         s, it, *nxt = om.qg_step(cfg["F"], cfg["beta"], dt, *ref, detj_mode=detj, rtol=1e-10, prev=prev)
