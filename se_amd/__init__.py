@@ -181,6 +181,58 @@ def solve_opts(rtol=1e-10, maxit=100000, precision=PREC_FP64, null_space=False, 
     return o
 
 
+IO_LIB_PATH = os.path.join(os.path.dirname(LIB_PATH), "libse_amd_io.so")
+_io = None
+
+
+def io_lib():
+    """Load libse_amd_io.so (field files in the reference's HDF5 layout)."""
+    global _io
+    if _io is None:
+        if not os.path.exists(IO_LIB_PATH):
+            raise SEError("se_amd: %s is not built (needs HDF5); run `python -m se_amd.build`" % IO_LIB_PATH)
+        L = C.CDLL(IO_LIB_PATH)
+        i, pd, pi = C.c_int, _D, C.POINTER(C.c_int)
+        sig = {"se_io_last_error": (C.c_char_p, []),
+               "se_field_save": (i, [C.c_char_p, C.c_uint, pd, i, i, pi]),
+               "se_field_read": (i, [C.c_char_p, C.c_uint, pd, i, i]),
+               "se_field_info": (i, [C.c_char_p, C.c_uint, pi, pi, pi, pi])}
+        for n, (r, a) in sig.items():
+            f = getattr(L, n)
+            f.restype, f.argtypes = r, a
+        _io = L
+    return _io
+
+
+def _io_check(rc, what):
+    if rc:
+        raise SEError("%s: %s" % (what, io_lib().se_io_last_error().decode()))
+
+
+def field_save(name, step, vals, nVerts):
+    """Field::Save: vals (nV,) or (nV, nDofs) node-major -> <name>.<step:05u>.h5."""
+    v = np.ascontiguousarray(vals, dtype=np.float64)
+    nV = v.shape[0]
+    nd = 1 if v.ndim == 1 else v.shape[1]
+    nv = (C.c_int * 2)(*nVerts)
+    _io_check(io_lib().se_field_save(name.encode(), step, v.ravel(), nV, nd, nv), "se_field_save")
+
+
+def field_read(name, step, nV, nDofs=1):
+    """Field::Read: the (nV, nDofs) values of <name>.<step:05u>.h5."""
+    out = np.zeros(nV * nDofs)
+    _io_check(io_lib().se_field_read(name.encode(), step, out, nV, nDofs), "se_field_read")
+    return out.reshape(nV, nDofs)
+
+
+def field_info(name, step):
+    """(nVertsTotal, nDofs, dimensions, mesh resolution) of a field file."""
+    a, b, c = C.c_int(), C.c_int(), C.c_int()
+    res = (C.c_int * 2)()
+    _io_check(io_lib().se_field_info(name.encode(), step, C.byref(a), C.byref(b), C.byref(c), res), "se_field_info")
+    return a.value, b.value, c.value, (res[0], res[1])
+
+
 class Context:
     """One HIP stream + the device memory of everything created from it."""
 

@@ -59,6 +59,29 @@ def build(verbose=True):
     os.replace(tmp, out)
     if verbose:
         print("built", out)
+    build_io(verbose)
+    return out
+
+
+def build_io(verbose=True):
+    """libse_amd_io.so: field files in the reference's HDF5 layout (host code,
+    g++ against HDF5 1.10 at $SE_HDF5_PREFIX, default /opt/conda)."""
+    prefix = os.environ.get("SE_HDF5_PREFIX", "/opt/conda")
+    inc, lib = os.path.join(prefix, "include"), os.path.join(prefix, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")) or not os.path.exists(os.path.join(lib, "libhdf5.so")):
+        if verbose:
+            print("field I/O library skipped: no HDF5 under", prefix)
+        return None
+    out = os.path.join(LIBDIR, "libse_amd_io.so")
+    tmp = out + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I" + os.path.join(ROOT, "include"), "-I" + inc,
+           os.path.join(CSRC, "field_io.cpp"), "-o", tmp, os.path.join(lib, "libhdf5.so"), "-Wl,-rpath," + lib]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("field I/O build failed: %s\n%s" % (" ".join(cmd), r.stderr))
+    os.replace(tmp, out)
+    if verbose:
+        print("built", out)
     return out
 
 

@@ -48,6 +48,7 @@ struct Request {
     se_qg2* q = nullptr;
     double dt = 0.0;
     size_t bytes = 0;
+    std::string error;  // why the request failed (status != SE_OK)
 };
 
 using MeshKey = std::tuple<int, int, int, double, double, double, double, int, int, int, int, int, int>;
@@ -192,6 +193,7 @@ int se_server::admit(std::vector<std::shared_ptr<Request>>& active) {
 void se_server::retire(const std::shared_ptr<Request>& r, int status) {
     // server thread, with mu held
     r->res.status = status;
+    if (status != SE_OK && r->error.empty()) r->error = se_last_error();
     r->res.t_done = now();
     if (r->q) {
         se_mesh* m = r->q->mesh;
@@ -241,6 +243,8 @@ int se_server::tick(std::vector<std::shared_ptr<Request>>& active) {
             r.res.unknowns += ops[j]->mesh->n;
             if (inf[j].status != SE_SOLVE_CONVERGED) {
                 failed[idx[j]] = SE_ERR_STATE;
+                r.error = std::string(which ? "barotropic" : "baroclinic") + " inversion ended with status " +
+                          std::to_string(inf[j].status) + " after " + std::to_string(inf[j].iters) + " iterations";
                 continue;
             }
             int rc2 = qg2_phase_update(r.q, which);
@@ -325,7 +329,7 @@ extern "C" int se_server_submit(se_server* sv, const se_qg2_request* req, int64_
     SE_CHECK(req->steps >= 1, SE_ERR_ARG, "se_server_submit: steps must be >= 1");
     SE_CHECK(req->N >= 2 && req->N <= 8 && req->nEls[0] > 0 && req->nEls[1] > 0, SE_ERR_ARG,
              "se_server_submit: bad mesh");
-    const size_t nV = (size_t)(req->nEls[0] * req->N + 1) * (req->nEls[1] * req->N + 1);
+    const size_t nV = ((size_t)req->nEls[0] * req->N + 1) * ((size_t)req->nEls[1] * req->N + 1);
     auto r = std::make_shared<Request>();
     r->spec = *req;
     r->init.resize(4 * nV);
@@ -361,12 +365,13 @@ extern "C" int se_server_wait(se_server* sv, int64_t id, se_request_result* res,
         sv->cv_done.wait(lk, [&] { return r->done; });
         sv->all.erase(id);
     }
+    // r is done: the server thread no longer touches it
     if (res) *res = r->res;
     const size_t nV = r->out.size() / 4;
     double* dst[4] = {psi1, omega1, psi2, omega2};
     for (int k = 0; k < 4; k++)
         if (dst[k] && nV) std::memcpy(dst[k], r->out.data() + k * nV, sizeof(double) * nV);
-    if (r->res.status != SE_OK) return fail(r->res.status, "se_server_wait: request failed: " + sv->error);
+    if (r->res.status != SE_OK) return fail(r->res.status, "se_server_wait: request failed: " + r->error);
     return SE_OK;
 }
 
