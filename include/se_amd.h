@@ -244,13 +244,17 @@ int se_calc_timestep(se_mesh* mesh, const double* vel, double courant, double* d
 int se_calc_timestep_dev(se_mesh* mesh, const double* d_vel, double courant, double* dt);
 
 /* ---- quasi-geostrophic steps ---------------------------------------------------
- * Time steps of the QG models with the dissipation off and no topography.
- * The first step after set_state is the reference's first-order step, every
- * later one its second-order step from the state before the previous step
- * (as the reference's drivers call Solve).  SolveQ's PV update under the
- * diagonal GLL mass leaves Dirichlet (wall) nodes at their GenPV value, so
- * the wall vorticity is kept.  The inversions run with the precision of
- * `opts` (FP64 or mixed). */
+ * Time steps of the QG models.  The first step after set_state is the
+ * reference's first-order step, every later one its second-order step from the
+ * state before the previous step (as the reference's drivers call Solve).
+ * SolveQ's PV update runs on the diagonal GLL mass matrix: each unknown is its
+ * assembled right-hand side over its mass (the two copies of a periodic node
+ * are averaged), Dirichlet (wall) nodes keep their GenPV value, so the wall
+ * vorticity is kept.  With dissipation set, the step first inverts the
+ * advected PV (the reference's first inversion, or pair) for the omega^ and
+ * tau^ its dissipation terms use; without, that inversion only feeds terms
+ * multiplied by zero and is skipped.  The inversions run with the precision
+ * of `opts` (FP64 or mixed). */
 
 /* Two layers [ref: TwoLayerQGEqn::FirstOrder / SecondOrder
  * src/qg/TwoLayerQGEqn.cpp:174-333, GenVel :335-352]: GenVel and GenPV of both
@@ -269,19 +273,36 @@ int se_qg2_get_state(se_qg2* q, double* psi1, double* omega1, double* psi2, doub
 int se_qg2_state_dev(se_qg2* q, double** psi1, double** omega1, double** psi2, double** omega2);
 /* CFL time step of both layers' velocities [ref: CalcTimeStep src/base/Utils.cpp:23-40] */
 int se_qg2_timestep(se_qg2* q, double courant, double* dt);
-/* infos: 2 entries (baroclinic, barotropic), nullable */
+/* The dissipation of TwoLayerQGEqn(psi1, omega1, psi2, omega2, F1, F2, beta, E2,
+ * r, nu, tau_e, dt) [ref: TwoLayerQGEqn.cpp:17-87, DissipationRHS :419-462]:
+ * layer 1 relaxes tau = psi1 - psi2 toward tau_e at rate r F1, layer 2 at
+ * -r F2 with Ekman drag E2 on omega2, both layers take viscosity nu.  tau_e
+ * is a host field (nV) or NULL (zero); all zero (the default) is the
+ * undissipated step. */
+int se_qg2_set_dissipation(se_qg2* q, double E2, double r, double nu, const double* tau_e);
+/* infos: 2 entries (baroclinic, barotropic), nullable; with dissipation each
+ * describes that operator's second solve, with the iterations of both */
 int se_qg2_step(se_qg2* q, double dt, const se_solve_opts* opts, se_solve_info* infos);
 
 /* One layer, the channel model [ref: QuasiGeostrophicEqn::Solve / FirstOrder /
  * SecondOrder src/qg/QuasiGeostrophicEqn.cpp:83-205, GenVel :207-223, GenPV
- * :225-232]: alpha = omega - F psi + beta y advected, (lap - F) psi =
- * alpha - beta y, omega = alpha + F psi - beta y. */
+ * :225-232]: alpha = omega - F psi + beta y + topo advected, (lap - F) psi =
+ * alpha - beta y - topo, omega = alpha + F psi - beta y - topo. */
 typedef struct se_qg se_qg;
 int se_qg_create(se_mesh* mesh, double F, double beta, int detj_mode, se_qg** out);
 int se_qg_destroy(se_qg* q);
+/* viscosity nu and linear drag r on omega [ref: the nu, r arguments of
+ * QuasiGeostrophicEqn(phi, omega, F, beta, nu, r, dt, topo)
+ * QuasiGeostrophicEqn.cpp:28-64, DisRHS :303-337]; zero (the default) is the
+ * undissipated step */
+int se_qg_set_dissipation(se_qg* q, double nu, double r);
+/* topography [ref: the topo argument, QuasiGeostrophicEqn.cpp:28-64]: a host
+ * field (nV), or NULL for none */
+int se_qg_set_topography(se_qg* q, const double* topo);
 int se_qg_set_state(se_qg* q, const double* psi, const double* omega);
 int se_qg_get_state(se_qg* q, double* psi, double* omega);
 int se_qg_timestep(se_qg* q, double courant, double* dt);
+/* info (nullable): the last inversion, with the iterations of both when dissipative */
 int se_qg_step(se_qg* q, double dt, const se_solve_opts* opts, se_solve_info* info);
 
 /* ---- request server -----------------------------------------------------------
@@ -311,6 +332,8 @@ typedef struct {
     int steps;  /* time steps to run */
     double dt;  /* > 0: this step; otherwise the CFL-0.5 step of the initial state */
     const double *psi1, *omega1, *psi2, *omega2; /* initial fields (nV each), copied at submission */
+    double E2, r, nu;    /* dissipation as se_qg2_set_dissipation (zero: none) */
+    const double* tau_e; /* relaxation target (nV) or NULL, copied at submission */
 } se_qg2_request;
 typedef struct {
     int64_t id;

@@ -70,8 +70,10 @@ def lib():
             "orc_sl_points": (None, [vp, i, _D, _D, d, _D]),
             "orc_genvel": (None, [vp, _D, _D]),
             "orc_calc_timestep": (d, [vp, _D, d]),
-            "orc_qg_step": (i, [vp, d, d, d, i, _D, _D, vp, vp, d, i, C.POINTER(i)]),
-            "orc_qg2_step": (i, [vp, d, d, d, d, i, _D, _D, _D, _D, vp, vp, vp, vp, d, i, C.POINTER(i)]),
+            "orc_qg_solveq": (None, [vp, d, d, _D, d, d, d, vp, vp, vp, _D]),
+            "orc_qg_step": (i, [vp, d, d, d, d, vp, d, i, _D, _D, vp, vp, d, i, C.POINTER(i)]),
+            "orc_qg2_step": (i, [vp, d, d, d, d, d, d, vp, d, i, _D, _D, _D, _D, vp, vp, vp, vp, d, i,
+                                 C.POINTER(i)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -304,24 +306,40 @@ class OracleMesh(_MeshBase):
         self.L.orc_helm_apply_ideal(self.h, c_mass, c_lap, detj_mode, _d(x), y)
         return y
 
-    def qg_step(self, F, beta, dt, psi, omega, detj_mode=1, rtol=1e-10, maxit=100000, prev=None):
+    def qg_solveq(self, a, dt, qhat, alpha, rF=0.0, Ei=0.0, nu=0.0, tau=None, omega_hat=None, tau_e=None):
+        """SolveQ with the dissipation right-hand side; returns alpha with its
+        unknowns replaced (Dirichlet nodes keep the given alpha)."""
+        out = _d(alpha).copy()
+        keep = [None if v is None else _d(v) for v in (tau, omega_hat, tau_e)]
+        self.L.orc_qg_solveq(self.h, a, dt, _d(qhat), rF, Ei, nu,
+                             *[None if v is None else v.ctypes.data for v in keep], out)
+        return out
+
+    def qg_step(self, F, beta, dt, psi, omega, detj_mode=1, rtol=1e-10, maxit=100000, prev=None, nu=0.0, r=0.0,
+                topo=None):
         """One QG channel step; prev = (psi, omega) before the previous step
-        selects the second-order step."""
+        selects the second-order step; nu, r, topo: viscosity, linear drag,
+        topography."""
         psi, omega = _d(psi).copy(), _d(omega).copy()
         pp = [_d(v) for v in prev] if prev is not None else [None, None]
+        tp = None if topo is None else _d(topo)
         it = C.c_int()
-        st = self.L.orc_qg_step(self.h, F, beta, dt, detj_mode, psi, omega,
-                                *[None if v is None else v.ctypes.data for v in pp], rtol, maxit, C.byref(it))
+        st = self.L.orc_qg_step(self.h, F, beta, nu, r, None if tp is None else tp.ctypes.data, dt, detj_mode,
+                                psi, omega, *[None if v is None else v.ctypes.data for v in pp], rtol, maxit,
+                                C.byref(it))
         return st, it.value, psi, omega
 
     def qg2_step(self, F1, F2, beta, dt, psi1, omega1, psi2, omega2, detj_mode=1, rtol=1e-10, maxit=100000,
-                 prev=None):
+                 prev=None, E2=0.0, r=0.0, nu=0.0, tau_e=None):
         """One two-layer step; prev = (psi1, omega1, psi2, omega2) before the
-        previous step selects the second-order step."""
+        previous step selects the second-order step; E2, r, nu, tau_e: Ekman
+        drag of layer 2, thermal relaxation toward tau_e, viscosity."""
         a = [_d(v).copy() for v in (psi1, omega1, psi2, omega2)]
         pp = [_d(v) for v in prev] if prev is not None else [None] * 4
+        te = None if tau_e is None else _d(tau_e)
         it = C.c_int()
-        st = self.L.orc_qg2_step(self.h, F1, F2, beta, dt, detj_mode, a[0], a[1], a[2], a[3],
+        st = self.L.orc_qg2_step(self.h, F1, F2, beta, E2, r, nu, None if te is None else te.ctypes.data, dt,
+                                 detj_mode, a[0], a[1], a[2], a[3],
                                  *[None if v is None else v.ctypes.data for v in pp], rtol, maxit, C.byref(it))
         return st, it.value, a[0], a[1], a[2], a[3]
 

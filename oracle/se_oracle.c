@@ -1171,23 +1171,33 @@ static void shape_func_derivs_at_coord(const orc_mesh* m, const double* coord, d
 /* QuasiGeostrophicEqn::GenVel, QuasiGeostrophicEqn.cpp:207-223, through
  * Field::InterpDerivsGlobal (Field.cpp:96-118) and
  * Mesh::ShapeFuncDerivsAtCoord (Mesh.cpp:252-285).  vel is nV x 2. */
+/* Field::InterpDerivsGlobal of a scalar field at global coordinate g:
+ * d = (d/dx, d/dy) in the element Mesh::GlobalToLocal picks.  GNix, gGNix
+ * (2 nl) and nodes (nl) are work arrays. */
+static void interp_derivs_global(const orc_mesh* m, const double* vals, const double* g, double* d,
+                                 double* GNix, double* gGNix, int* nodes) {
+    int el, node, nl = m->nl;
+    double l[2], detJac;
+    orc_global_to_local(m, g, &el, l);
+    shape_func_derivs_at_coord(m, l, GNix);
+    global_derivs(m, el, GNix, gGNix, &detJac, nodes);
+    d[0] = d[1] = 0.0;
+    for (node = 0; node < nl; node++) {
+        d[0] += gGNix[node] * vals[nodes[node]];
+        d[1] += gGNix[nl + node] * vals[nodes[node]];
+    }
+}
+
 void orc_genvel(const orc_mesh* m, const double* psi, double* vel) {
-    int nl = m->nl, v, node;
+    int nl = m->nl, v;
     double* GNix = (double*)malloc(sizeof(double) * 2 * (size_t)nl);
     double* gGNix = (double*)malloc(sizeof(double) * 2 * (size_t)nl);
     int* nodes = (int*)malloc(sizeof(int) * (size_t)nl);
     for (v = 0; v < m->nV; v++) {
-        int el;
-        double l[2], detJac, g0 = 0.0, g1 = 0.0;
-        orc_global_to_local(m, &m->verts[2 * (size_t)v], &el, l);
-        shape_func_derivs_at_coord(m, l, GNix);
-        global_derivs(m, el, GNix, gGNix, &detJac, nodes);
-        for (node = 0; node < nl; node++) {
-            g0 += gGNix[node] * psi[nodes[node]];
-            g1 += gGNix[nl + node] * psi[nodes[node]];
-        }
-        vel[2 * (size_t)v + 0] = -1.0 * g1;
-        vel[2 * (size_t)v + 1] = +1.0 * g0;
+        double d[2];
+        interp_derivs_global(m, psi, &m->verts[2 * (size_t)v], d, GNix, gGNix, nodes);
+        vel[2 * (size_t)v + 0] = -1.0 * d[1];
+        vel[2 * (size_t)v + 1] = +1.0 * d[0];
     }
     free(GNix);
     free(gGNix);
@@ -1205,85 +1215,201 @@ double orc_calc_timestep(const orc_mesh* m, const double* vel, double courant) {
     }
     return courant * minSep / sqrt(maxVelSq);
 }
-
 /* ------------------------------------------------------------------------- */
 /* callers: QG steps                                                         */
 /* ------------------------------------------------------------------------- */
 
-/* The QG steps with the dissipation off (nu = r = 0) and no topography.
+/* Mesh::LocalToGlobal, Mesh.cpp:178-181 */
+static void local_to_global(const orc_mesh* m, const double* l, int el, double* g) {
+    g[0] = m->min[0] + (el % m->nEls[0]) * m->dx[0] + 0.5 * (l[0] + 1.0) * m->dx[0];
+    g[1] = m->min[1] + (el / m->nEls[0]) * m->dx[1] + 0.5 * (l[1] + 1.0) * m->dx[1];
+}
+
+/* SolveQ with its dissipation right-hand side.
  *
- * QuasiGeostrophicEqn::FirstOrder (QuasiGeostrophicEqn.cpp:117-157): GenVel,
- * GenPV (:225-232), SL advection of the PV; SolveQ (:277-301) solves the
- * mass-matrix system for alpha, which under the diagonal GLL mass is alpha =
- * alpha^ on the unknowns, and Vector::UpdateField (Vector.cpp:40-61) writes
- * only the unknowns (and the periodic copies), so at Dirichlet nodes alpha
- * keeps its GenPV value; he2 inverts (lap - F) psi = alpha - beta y with the
- * wall data of psi, and omega = alpha + F psi - beta y (:141-154).  The first
- * inversion he1 (:126-136) only feeds the dissipation terms and is not
- * repeated here.
- * SecondOrder (:159-205, Solve :83-115): the same with the PV of the previous
- * state and second-order SL advection (2 alpha(x1) - alpha_prev(x2) / 2, with
- * extrapolated velocities, Advector.cpp:203-246), and SolveQ on a 1.5 x mass
- * matrix (AssembleQMat(1.5)), so alpha = alpha^ / 1.5 on the unknowns.
- * psi / omega are nV fields updated in place; psi_prev / omega_prev (NULL
- * for the first-order step) are the state before the previous step.
- * Returns the solve status. */
+ * TwoLayerQGEqn::SolveQ (TwoLayerQGEqn.cpp:396-416) over
+ * DissipationRHS::AssembleElement (:437-462), and QuasiGeostrophicEqn::SolveQ
+ * (QuasiGeostrophicEqn.cpp:277-301) over DisRHS::AssembleElement (:315-337),
+ * which is the same form with rF = 0 and Ei = r.  The matrix is AssembleQMat's
+ * a x mass matrix (MassMatrix, Operator.cpp:157-177), diagonal under GLL
+ * collocation, so its Dirichlet lifting is zero and KSPSolve is the division
+ * by its diagonal (to the solver's tolerance).  The right-hand side is
+ * assembled element by element through fieldToVecMap as Vector::Assemble
+ * (Vector.cpp:89-120) does: Dirichlet rows are dropped and the two copies of a
+ * periodic node add into one unknown.  At each quadrature point of each
+ * element, with c = detJ w dt,
+ *   rhs_i += detJ w qhat N_i + c rF tau N_i - c Ei omegaHat N_i - c rF tau_e N_i
+ *            - c nu (dN_i/dx dW/dx + dN_i/dy dW/dy),
+ * where the gradient of W = omegaHat comes from Field::InterpDerivsGlobal at
+ * the point's global coordinate (Mesh::LocalToGlobal), i.e. in the element
+ * Mesh::GlobalToLocal picks there, not necessarily the element being
+ * assembled.  Vector::UpdateField then writes the unknowns and the periodic
+ * copies into alpha; alpha's Dirichlet nodes keep what they hold.  tau,
+ * omegaHat and tau_e may be NULL (read as zero). */
+void orc_qg_solveq(const orc_mesh* m, double a, double dt, const double* qhat, double rF, double Ei, double nu,
+                   const double* tau, const double* omegaHat, const double* tau_e, double* alpha) {
+    int nl = m->nl, N = m->N, n = m->n, el, pt, node, u;
+    double* f = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    double* diag = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    double* x = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    double* rhs = (double*)malloc(sizeof(double) * (size_t)nl);
+    double* Ni = (double*)malloc(sizeof(double) * (size_t)nl);
+    double* gGNix = (double*)malloc(sizeof(double) * 2 * (size_t)nl);
+    double* dGNix = (double*)malloc(sizeof(double) * 2 * (size_t)nl);
+    double* dgGNix = (double*)malloc(sizeof(double) * 2 * (size_t)nl);
+    int* nodes = (int*)malloc(sizeof(int) * (size_t)nl);
+    int* dnodes = (int*)malloc(sizeof(int) * (size_t)nl);
+    const int grad = omegaHat && nu != 0.0;
+    for (el = 0; el < m->nElsTotal; el++) {
+        for (node = 0; node < nl; node++) rhs[node] = 0.0;
+        for (pt = 0; pt < nl; pt++) {
+            int px = pt % (N + 1), py = pt / (N + 1);
+            double coord[2] = {m->abcissa[px], m->abcissa[py]};
+            double weight = m->weights[px] * m->weights[py];
+            double detJac, c, q, t = 0.0, w = 0.0, te = 0.0, g[2], gW[2] = {0.0, 0.0};
+            orc_shape_func_derivs(m, el, pt, gGNix, &detJac);
+            c = detJac * weight * dt;
+            local_to_global(m, coord, el, g);
+            interp_local(m, qhat, 1, el, coord, &q, Ni, nodes);
+            if (tau) interp_local(m, tau, 1, el, coord, &t, Ni, nodes);
+            if (omegaHat) interp_local(m, omegaHat, 1, el, coord, &w, Ni, nodes);
+            if (tau_e) interp_local(m, tau_e, 1, el, coord, &te, Ni, nodes);
+            if (grad) interp_derivs_global(m, omegaHat, g, gW, dGNix, dgGNix, dnodes);
+            for (node = 0; node < nl; node++) {
+                /* the shape functions at a GLL quadrature point are Kronecker deltas */
+                double Nn = (node == pt) ? 1.0 : 0.0;
+                rhs[node] += detJac * weight * q * Nn;
+                rhs[node] += c * rF * t * Nn;
+                rhs[node] -= c * Ei * w * Nn;
+                rhs[node] -= c * rF * te * Nn;
+                rhs[node] -= c * nu * (gGNix[node] * gW[0] + gGNix[nl + node] * gW[1]);
+            }
+        }
+        orc_el_nodes(m, el, nodes);
+        for (node = 0; node < nl; node++) {
+            u = m->f2v[nodes[node]];
+            if (u >= 0) f[u] += rhs[node];
+        }
+        /* the diagonal of AssembleQMat's matrix */
+        for (pt = 0; pt < nl; pt++) {
+            int px = pt % (N + 1), py = pt / (N + 1);
+            u = m->f2v[nodes[pt]];
+            if (u >= 0) diag[u] += orc_det_jac(m, el, pt) * (m->weights[px] * m->weights[py]) * a;
+        }
+    }
+    for (u = 0; u < n; u++) x[u] = f[u] / diag[u];
+    orc_update_field(m, x, alpha);
+    free(f); free(diag); free(x); free(rhs); free(Ni); free(gGNix); free(dGNix); free(dgGNix);
+    free(nodes); free(dnodes);
+}
+
+/* The one-layer QG channel step, QuasiGeostrophicEqn::FirstOrder
+ * (QuasiGeostrophicEqn.cpp:117-157) and SecondOrder (:159-205), with the
+ * dissipation of DisRHS (linear drag r, viscosity nu) and topography.
+ *   1. GenVel (:207-223); alpha = omega - F psi + beta y + topo (GenPV
+ *      :225-232); SL advection of alpha: alpha^ (first order), or
+ *      2 alpha(x1) - alpha_prev(x2) / 2 from the previous state's PV with
+ *      extrapolated velocities (second order, Advector.cpp:203-246).
+ *   2. he1: (lap - F) psi^ = alpha^ - a beta y - a topo, and
+ *      omega^ = alpha^ + F psi^ - a beta y - a topo (a = 1 first order, 1.5
+ *      second order).  he1 was assembled by the constructor (:50-53) while
+ *      psi^ was zero, so its Dirichlet lifting is zero; psi^'s wall values are
+ *      zero until the second-order path sets psi^ = 1.5 psi (:86-93).
+ *   3. SolveQ (orc_qg_solveq) with rF = 0, Ei = r on AssembleQMat(a).
+ *   4. he2: (lap - F) psi = alpha - beta y - topo with psi's wall data,
+ *      omega = alpha + F psi - beta y - topo.
+ * Step 2 only feeds the r and nu terms: with both zero it is skipped (its
+ * values are multiplied by zero).  psi / omega are updated in place;
+ * psi_prev / omega_prev (NULL for the first-order step) are the state before
+ * the previous step; topo may be NULL.  Returns the first failing solve's
+ * status. */
 static int qg_dirichlet(const orc_mesh* m, int v) { return m->f2v[v] < 0; }
 
-int orc_qg_step(const orc_mesh* m, double F, double beta, double dt, int detj_mode, double* psi, double* omega,
-                const double* psi_prev, const double* omega_prev, double rtol, int maxit, int* iters) {
+int orc_qg_step(const orc_mesh* m, double F, double beta, double nu, double r, const double* topo, double dt,
+                int detj_mode, double* psi, double* omega, const double* psi_prev, const double* omega_prev,
+                double rtol, int maxit, int* iters) {
     size_t nV = (size_t)m->nV, v;
     const int second = psi_prev && omega_prev;
+    const int diss = nu != 0.0 || r != 0.0;
+    const double a = second ? 1.5 : 1.0;
     double* vel = (double*)malloc(sizeof(double) * 2 * nV);
     double* velp = (double*)malloc(sizeof(double) * 2 * nV);
     double* alpha = (double*)malloc(sizeof(double) * nV);
     double* alphap = (double*)malloc(sizeof(double) * nV);
     double* qsl = (double*)malloc(sizeof(double) * nV);
     double* rhs = (double*)malloc(sizeof(double) * nV);
+    double* phat = (double*)calloc(nV, sizeof(double));
+    double* omh = (double*)calloc(nV, sizeof(double));
     orc_helm* h = orc_helm_create(m, -F, -1.0 * 1.0, detj_mode);
     double rr;
-    int st;
+    int st = 0, st1 = 0, it = 0, it1 = 0;
     orc_genvel(m, psi, vel);
-    for (v = 0; v < nV; v++) alpha[v] = omega[v] - F * psi[v] + beta * m->verts[2 * v + 1];
+    for (v = 0; v < nV; v++)
+        alpha[v] = omega[v] - F * psi[v] + beta * m->verts[2 * v + 1] + (topo ? topo[v] : 0.0);
     if (second) {
         orc_genvel(m, psi_prev, velp);
-        for (v = 0; v < nV; v++) alphap[v] = omega_prev[v] - F * psi_prev[v] + beta * m->verts[2 * v + 1];
+        for (v = 0; v < nV; v++)
+            alphap[v] = omega_prev[v] - F * psi_prev[v] + beta * m->verts[2 * v + 1] + (topo ? topo[v] : 0.0);
         orc_advect_second(m, alpha, alphap, 1, vel, velp, dt, qsl);
     } else {
         orc_advect_first(m, alpha, 1, vel, dt, qsl, NULL, NULL);
     }
-    /* SolveQ + UpdateField: the unknowns take alpha^ / a, Dirichlet nodes keep alpha */
-    for (v = 0; v < nV; v++)
-        if (!qg_dirichlet(m, (int)v)) alpha[v] = second ? qsl[v] / 1.5 : qsl[v];
-    for (v = 0; v < nV; v++) rhs[v] = alpha[v] - beta * m->verts[2 * v + 1];
+    if (diss) {
+        orc_helm* h1 = orc_helm_create(m, -F, -1.0 * 1.0, detj_mode);
+        orc_helm_assemble(h1, phat); /* zero wall data */
+        for (v = 0; v < nV; v++) {
+            double y = m->verts[2 * v + 1];
+            rhs[v] = qsl[v] - a * beta * y - (topo ? a * topo[v] : 0.0);
+            if (second && qg_dirichlet(m, (int)v)) phat[v] = 1.5 * psi[v];
+        }
+        st1 = orc_helm_solve(h1, rhs, phat, rtol, maxit, 0, &it1, &rr);
+        for (v = 0; v < nV; v++)
+            omh[v] = qsl[v] + F * phat[v] - a * beta * m->verts[2 * v + 1] - (topo ? a * topo[v] : 0.0);
+        orc_helm_destroy(h1);
+    }
+    orc_qg_solveq(m, a, dt, qsl, 0.0, r, nu, NULL, diss ? omh : NULL, NULL, alpha);
+    for (v = 0; v < nV; v++) rhs[v] = alpha[v] - beta * m->verts[2 * v + 1] - (topo ? topo[v] : 0.0);
     orc_helm_assemble(h, psi);
-    st = orc_helm_solve(h, rhs, psi, rtol, maxit, 0, iters, &rr);
-    for (v = 0; v < nV; v++) omega[v] = alpha[v] + F * psi[v] - beta * m->verts[2 * v + 1];
+    st = orc_helm_solve(h, rhs, psi, rtol, maxit, 0, &it, &rr);
+    for (v = 0; v < nV; v++)
+        omega[v] = alpha[v] + F * psi[v] - beta * m->verts[2 * v + 1] - (topo ? topo[v] : 0.0);
+    if (iters) *iters = it1 + it;
     orc_helm_destroy(h);
-    free(vel); free(velp); free(alpha); free(alphap); free(qsl); free(rhs);
-    return st;
+    free(vel); free(velp); free(alpha); free(alphap); free(qsl); free(rhs); free(phat); free(omh);
+    return st1 != 0 ? st1 : st;
 }
 
-/* TwoLayerQGEqn::FirstOrder (TwoLayerQGEqn.cpp:174-243) and SecondOrder
- * (:245-333), dissipation off.  Both advect the two layers' PV (GenPV
- * :326-333); the result is set by the second inversion pair (steps 6-8), which
- * uses alpha_i after SolveQ (:396-416).  As in the one-layer step SolveQ
- * gives alpha_i = alpha_i^ (FirstOrder) or alpha_i^ / 1.5 (SecondOrder) on
- * the unknowns and leaves the Dirichlet nodes at their GenPV values.
- * FirstOrder calls SolveQ for layer 1 only (:214), so layer 2's alpha keeps
- * its GenPV value on every node: the reference's first step inverts the
- * advected layer-1 PV against the unadvected layer-2 PV.  SecondOrder solves
- * both layers (:291-292).  Then tau = psi1 - psi2 from
- * (lap - (F1+F2)) tau = alpha1 - alpha2, chi = psi1 + psi2 from
- * lap chi = alpha1 + alpha2 + (F1-F2) tau - 2 beta y, and
- * omega_i = alpha_i - F_i (psi_j - psi_i) - beta y.  The first inversion pair
- * (steps 2-4) only feeds the dissipation terms and is not repeated. */
-int orc_qg2_step(const orc_mesh* m, double F1, double F2, double beta, double dt, int detj_mode,
-                 double* psi1, double* omega1, double* psi2, double* omega2,
-                 const double* psi1p, const double* omega1p, const double* psi2p, const double* omega2p,
-                 double rtol, int maxit, int* iters) {
+/* The two-layer step, TwoLayerQGEqn::FirstOrder (TwoLayerQGEqn.cpp:174-243)
+ * and SecondOrder (:245-324), with the dissipation of DissipationRHS
+ * (thermal relaxation r toward tau_e, Ekman drag E2 on layer 2, viscosity nu).
+ *   1. GenVel (:335-352) and GenPV (:326-333) of both layers,
+ *      alpha_i = omega_i + F_i (psi_j - psi_i) + beta y, SL advection to
+ *      alpha_i^ (second order as in the one-layer step).
+ *   2-4. (lap - (F1+F2)) tau^ = alpha1^ - alpha2^,
+ *      lap chi^ = alpha1^ + alpha2^ + (F1-F2) tau^ - 2 a beta y, with the wall
+ *      data of tau = psi1 - psi2 and chi = psi1 + psi2 (the constructor's,
+ *      :46-53, unscaled in the second-order step);
+ *      psi1^ = (tau^ + chi^)/2, psi2^ = (chi^ - tau^)/2,
+ *      omega_i^ = alpha_i^ - F_i (psi_j^ - psi_i^) - a beta y.
+ *   5. SolveQ (orc_qg_solveq): layer 1 with rF = r F1, Ei = 0 (:78-81);
+ *      layer 2 with rF = -r F2, Ei = E2 (:83-86), both with tau^ and tau_e.
+ *      FirstOrder calls SolveQ for layer 1 only (:214), so there layer 2's
+ *      alpha keeps its GenPV value on every node; SecondOrder solves both
+ *      (:291-292; its SolveQ of layer 2 passes layer 1's lifting b1, which is
+ *      zero like b2 under the diagonal mass).
+ *   6-8. (lap - (F1+F2)) tau = alpha1 - alpha2,
+ *      lap chi = alpha1 + alpha2 + (F1-F2) tau - 2 beta y, psi_i from tau and
+ *      chi, omega_i = alpha_i - F_i (psi_j - psi_i) - beta y.
+ * Steps 2-4 only feed the dissipation terms: with r = E2 = nu = 0 they are
+ * skipped (their values are multiplied by zero).  tau_e may be NULL. */
+int orc_qg2_step(const orc_mesh* m, double F1, double F2, double beta, double E2, double r, double nu,
+                 const double* tau_e, double dt, int detj_mode, double* psi1, double* omega1, double* psi2,
+                 double* omega2, const double* psi1p, const double* omega1p, const double* psi2p,
+                 const double* omega2p, double rtol, int maxit, int* iters) {
     size_t nV = (size_t)m->nV, v;
     const int second = psi1p && omega1p && psi2p && omega2p;
+    const int diss = r != 0.0 || E2 != 0.0 || nu != 0.0;
+    const double a = second ? 1.5 : 1.0;
     double* vel1 = (double*)malloc(sizeof(double) * 2 * nV);
     double* vel2 = (double*)malloc(sizeof(double) * 2 * nV);
     double* velp1 = (double*)malloc(sizeof(double) * 2 * nV);
@@ -1297,10 +1423,14 @@ int orc_qg2_step(const orc_mesh* m, double F1, double F2, double beta, double dt
     double* rhs = (double*)malloc(sizeof(double) * nV);
     double* tau = (double*)malloc(sizeof(double) * nV);
     double* chi = (double*)malloc(sizeof(double) * nV);
+    double* th = (double*)malloc(sizeof(double) * nV);
+    double* ch = (double*)malloc(sizeof(double) * nV);
+    double* om1h = (double*)calloc(nV, sizeof(double));
+    double* om2h = (double*)calloc(nV, sizeof(double));
     orc_helm* hh = orc_helm_create(m, -(F1 + F2), -1.0 * 1.0, detj_mode);
     orc_helm* hp = orc_helm_create(m, 0.0, -1.0, detj_mode);
     double rr;
-    int st, st2, it1 = 0, it2 = 0;
+    int st = 0, st2 = 0, sth = 0, sth2 = 0, it1 = 0, it2 = 0, ith1 = 0, ith2 = 0;
     orc_genvel(m, psi1, vel1);
     orc_genvel(m, psi2, vel2);
     for (v = 0; v < nV; v++) {
@@ -1324,19 +1454,31 @@ int orc_qg2_step(const orc_mesh* m, double F1, double F2, double beta, double dt
         orc_advect_first(m, a1, 1, vel1, dt, h1, NULL, NULL);
         orc_advect_first(m, a2, 1, vel2, dt, h2, NULL, NULL);
     }
-    for (v = 0; v < nV; v++) {
-        if (qg_dirichlet(m, (int)v)) continue;
-        a1[v] = second ? h1[v] / 1.5 : h1[v];
-        if (second) a2[v] = h2[v] / 1.5;
-    }
-    for (v = 0; v < nV; v++) rhs[v] = a1[v] - a2[v];
     orc_helm_assemble(hh, tau);
+    orc_helm_assemble(hp, chi);
+    if (diss) {
+        memcpy(th, tau, sizeof(double) * nV);
+        memcpy(ch, chi, sizeof(double) * nV);
+        for (v = 0; v < nV; v++) rhs[v] = h1[v] - h2[v];
+        sth = orc_helm_solve(hh, rhs, th, rtol, maxit, 0, &ith1, &rr);
+        for (v = 0; v < nV; v++) rhs[v] = h1[v] + h2[v] + (F1 - F2) * th[v] - a * 2.0 * beta * m->verts[2 * v + 1];
+        sth2 = orc_helm_solve(hp, rhs, ch, rtol, maxit, 0, &ith2, &rr);
+        for (v = 0; v < nV; v++) {
+            double y = m->verts[2 * v + 1];
+            double p1 = 0.5 * (th[v] + ch[v]), p2 = 0.5 * (ch[v] - th[v]);
+            om1h[v] = h1[v] - F1 * (p2 - p1) - a * beta * y;
+            om2h[v] = h2[v] - F2 * (p1 - p2) - a * beta * y;
+        }
+    }
+    orc_qg_solveq(m, a, dt, h1, r * F1, 0.0, nu, diss ? th : NULL, diss ? om1h : NULL, diss ? tau_e : NULL, a1);
+    if (second)
+        orc_qg_solveq(m, a, dt, h2, -r * F2, E2, nu, diss ? th : NULL, diss ? om2h : NULL, diss ? tau_e : NULL, a2);
+    for (v = 0; v < nV; v++) rhs[v] = a1[v] - a2[v];
     st = orc_helm_solve(hh, rhs, tau, rtol, maxit, 0, &it1, &rr);
     for (v = 0; v < nV; v++) {
         double y = m->verts[2 * v + 1];
         rhs[v] = a1[v] + a2[v] + (F1 - F2) * tau[v] - 2.0 * beta * y;
     }
-    orc_helm_assemble(hp, chi);
     st2 = orc_helm_solve(hp, rhs, chi, rtol, maxit, 0, &it2, &rr);
     for (v = 0; v < nV; v++) {
         double y = m->verts[2 * v + 1];
@@ -1345,10 +1487,12 @@ int orc_qg2_step(const orc_mesh* m, double F1, double F2, double beta, double dt
         omega1[v] = a1[v] - F1 * (psi2[v] - psi1[v]) - beta * y;
         omega2[v] = a2[v] - F2 * (psi1[v] - psi2[v]) - beta * y;
     }
-    if (iters) *iters = it1 + it2;
+    if (iters) *iters = ith1 + ith2 + it1 + it2;
     orc_helm_destroy(hh);
     orc_helm_destroy(hp);
     free(vel1); free(vel2); free(velp1); free(velp2); free(a1); free(a2); free(ap1); free(ap2);
-    free(h1); free(h2); free(rhs); free(tau); free(chi);
+    free(h1); free(h2); free(rhs); free(tau); free(chi); free(th); free(ch); free(om1h); free(om2h);
+    if (sth) return sth;
+    if (sth2) return sth2;
     return st != 0 ? st : st2;
 }

@@ -92,16 +92,20 @@ void   orc_sl_points(const orc_mesh* m, int order, const double* vel, const doub
 void   orc_genvel(const orc_mesh* m, const double* psi, double* vel);
 double orc_calc_timestep(const orc_mesh* m, const double* vel, double courant);
 
-/* ---- callers: QG steps (QuasiGeostrophicEqn.cpp:83-232;
- *      TwoLayerQGEqn.cpp:174-352).  NULL previous state: the first-order
+/* ---- callers: QG steps (QuasiGeostrophicEqn.cpp:83-337;
+ *      TwoLayerQGEqn.cpp:174-462).  NULL previous state: the first-order
  *      step; otherwise the second-order step from (prev, current). ---- */
-int orc_qg_step(const orc_mesh* m, double F, double beta, double dt, int detj_mode,
-                double* psi, double* omega, const double* psi_prev, const double* omega_prev,
+/* SolveQ + DisRHS / DissipationRHS: alpha's unknowns from the a x mass system
+ * (Dirichlet nodes keep alpha's values); tau, omegaHat, tau_e nullable */
+void orc_qg_solveq(const orc_mesh* m, double a, double dt, const double* qhat, double rF, double Ei, double nu,
+                   const double* tau, const double* omegaHat, const double* tau_e, double* alpha);
+int orc_qg_step(const orc_mesh* m, double F, double beta, double nu, double r, const double* topo, double dt,
+                int detj_mode, double* psi, double* omega, const double* psi_prev, const double* omega_prev,
                 double rtol, int maxit, int* iters);
-int orc_qg2_step(const orc_mesh* m, double F1, double F2, double beta, double dt, int detj_mode,
-                 double* psi1, double* omega1, double* psi2, double* omega2,
-                 const double* psi1p, const double* omega1p, const double* psi2p, const double* omega2p,
-                 double rtol, int maxit, int* iters);
+int orc_qg2_step(const orc_mesh* m, double F1, double F2, double beta, double E2, double r, double nu,
+                 const double* tau_e, double dt, int detj_mode, double* psi1, double* omega1, double* psi2,
+                 double* omega2, const double* psi1p, const double* omega1p, const double* psi2p,
+                 const double* omega2p, double rtol, int maxit, int* iters);
 
 #ifdef __cplusplus
 }

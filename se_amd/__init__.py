@@ -52,7 +52,8 @@ class QG2Request(C.Structure):
     _fields_ = [("nEls", C.c_int * 2), ("N", C.c_int), ("min", C.c_double * 2), ("max", C.c_double * 2),
                 ("periodic", C.c_int * 2), ("bc", C.c_int * 4), ("F1", C.c_double), ("F2", C.c_double),
                 ("beta", C.c_double), ("steps", C.c_int), ("dt", C.c_double), ("psi1", C.c_void_p),
-                ("omega1", C.c_void_p), ("psi2", C.c_void_p), ("omega2", C.c_void_p)]
+                ("omega1", C.c_void_p), ("psi2", C.c_void_p), ("omega2", C.c_void_p), ("E2", C.c_double),
+                ("r", C.c_double), ("nu", C.c_double), ("tau_e", C.c_void_p)]
 
 
 class RequestResult(C.Structure):
@@ -132,12 +133,15 @@ def lib():
             "se_qg2_state_dev": (i, [vp, pvp, pvp, pvp, pvp]),
             "se_qg2_timestep": (i, [vp, d, C.POINTER(d)]),
             "se_qg2_step": (i, [vp, d, C.POINTER(SolveOpts), vp]),
+            "se_qg2_set_dissipation": (i, [vp, d, d, d, vp]),
             "se_qg_create": (i, [vp, d, d, i, pvp]),
             "se_qg_destroy": (i, [vp]),
             "se_qg_set_state": (i, [vp, _D, _D]),
             "se_qg_get_state": (i, [vp, _D, _D]),
             "se_qg_timestep": (i, [vp, d, C.POINTER(d)]),
             "se_qg_step": (i, [vp, d, C.POINTER(SolveOpts), vp]),
+            "se_qg_set_dissipation": (i, [vp, d, d]),
+            "se_qg_set_topography": (i, [vp, vp]),
             "se_server_opts_default": (None, [C.POINTER(ServerOpts)]),
             "se_server_create": (i, [i, C.POINTER(ServerOpts), pvp]),
             "se_server_submit": (i, [vp, C.POINTER(QG2Request), C.POINTER(C.c_int64)]),
@@ -524,6 +528,12 @@ class QG2:
     def set_state(self, psi1, om1, psi2, om2):
         check(lib().se_qg2_set_state(self.h, _d(psi1), _d(om1), _d(psi2), _d(om2)), "se_qg2_set_state")
 
+    def set_dissipation(self, E2=0.0, r=0.0, nu=0.0, tau_e=None):
+        """Ekman drag of layer 2, thermal relaxation toward tau_e (None: zero), viscosity."""
+        te = None if tau_e is None else _d(tau_e)
+        check(lib().se_qg2_set_dissipation(self.h, E2, r, nu, None if te is None else te.ctypes.data),
+              "se_qg2_set_dissipation")
+
     def state(self):
         out = [np.zeros(self.mesh.nV) for _ in range(4)]
         check(lib().se_qg2_get_state(self.h, *out), "se_qg2_get_state")
@@ -566,6 +576,15 @@ class QG:
 
     def set_state(self, psi, omega):
         check(lib().se_qg_set_state(self.h, _d(psi), _d(omega)), "se_qg_set_state")
+
+    def set_dissipation(self, nu=0.0, r=0.0):
+        """Viscosity and linear drag on the vorticity."""
+        check(lib().se_qg_set_dissipation(self.h, nu, r), "se_qg_set_dissipation")
+
+    def set_topography(self, topo):
+        """Topography field (nV), or None for none."""
+        tp = None if topo is None else _d(topo)
+        check(lib().se_qg_set_topography(self.h, None if tp is None else tp.ctypes.data), "se_qg_set_topography")
 
     def state(self):
         out = [np.zeros(self.mesh.nV) for _ in range(2)]
@@ -610,7 +629,8 @@ class Server:
         self._keep = {}
 
     def submit(self, req, psi1, om1, psi2, om2, dt=0.0):
-        """req: dict with nEls, N, lo, hi, periodic, bc, F1, F2, beta, steps."""
+        """req: dict with nEls, N, lo, hi, periodic, bc, F1, F2, beta, steps and
+        optionally E2, r, nu, tau_e (dissipation, as QG2.set_dissipation)."""
         r = QG2Request()
         r.nEls[:] = list(req["nEls"])
         r.N = req["N"]
@@ -621,6 +641,9 @@ class Server:
         r.F1, r.F2, r.beta, r.steps, r.dt = req["F1"], req["F2"], req["beta"], req["steps"], dt
         arrs = [_d(a) for a in (psi1, om1, psi2, om2)]
         r.psi1, r.omega1, r.psi2, r.omega2 = [a.ctypes.data for a in arrs]
+        r.E2, r.r, r.nu = req.get("E2", 0.0), req.get("r", 0.0), req.get("nu", 0.0)
+        te = None if req.get("tau_e") is None else _d(req["tau_e"])
+        r.tau_e = None if te is None else te.ctypes.data
         i = C.c_int64()
         check(lib().se_server_submit(self.h, C.byref(r), C.byref(i)), "se_server_submit")
         self._keep[i.value] = len(arrs[0])

@@ -42,6 +42,7 @@ struct Request {
     int64_t id = 0;
     se_qg2_request spec{};
     std::vector<double> init;  // psi1, om1, psi2, om2 (nV each)
+    std::vector<double> tau_e; // relaxation target (nV) or empty
     std::vector<double> out;   // final fields
     se_request_result res{};
     bool done = false;
@@ -176,6 +177,8 @@ int se_server::admit(std::vector<std::shared_ptr<Request>>& active) {
         r->bytes = need;
         int rc = se_qg2_set_state(q, r->init.data(), r->init.data() + m->nV, r->init.data() + 2 * (size_t)m->nV,
                                   r->init.data() + 3 * (size_t)m->nV);
+        // a pooled instance carries the previous request's dissipation: set it every time
+        if (!rc) rc = se_qg2_set_dissipation(q, s.E2, s.r, s.nu, r->tau_e.empty() ? nullptr : r->tau_e.data());
         double dt = s.dt;
         if (!rc && !(dt > 0.0)) rc = se_qg2_timestep(q, 0.5, &dt);
         if (rc) {
@@ -221,14 +224,17 @@ int se_server::tick(std::vector<std::shared_ptr<Request>>& active) {
         if (!rc) rc = qg2_phase_rhs(active[i]->q, 0);
         if (rc) failed[i] = rc;
     }
-    for (int which = 0; which < 2; which++) {
+    // inversion k of every request that has one: 2 per step, 4 for the
+    // dissipative ones (their first pair runs beside the others' only pair)
+    for (int k = 0; k < 4; k++) {
         std::vector<se_op*> ops;
         std::vector<size_t> idx;
         for (size_t i = 0; i < active.size(); i++)
-            if (!failed[i]) {
-                ops.push_back(which ? active[i]->q->hp : active[i]->q->hh);
+            if (!failed[i] && k < qg2_passes(active[i]->q)) {
+                ops.push_back(qg2_pass_op(active[i]->q, k));
                 idx.push_back(i);
             }
+        if (ops.empty()) break;
         std::vector<se_solve_info> inf;
         int rc = opts.precision == SE_PREC_FP64 ? SE_OK : batch_solve(ctx, ops, o, inf);
         if (opts.precision == SE_PREC_FP64) {  // the plain path: one solve after another
@@ -239,17 +245,18 @@ int se_server::tick(std::vector<std::shared_ptr<Request>>& active) {
         if (rc) return rc;
         for (size_t j = 0; j < idx.size(); j++) {
             Request& r = *active[idx[j]];
+            const int passes = qg2_passes(r.q);
             r.res.inner_iters += inf[j].iters;
             r.res.unknowns += ops[j]->mesh->n;
             if (inf[j].status != SE_SOLVE_CONVERGED) {
                 failed[idx[j]] = SE_ERR_STATE;
-                r.error = std::string(which ? "barotropic" : "baroclinic") + " inversion ended with status " +
-                          std::to_string(inf[j].status) + " after " + std::to_string(inf[j].iters) + " iterations";
+                r.error = std::string(passes == 4 && k < 2 ? "first " : "") + ((k & 1) ? "barotropic" : "baroclinic") +
+                          " inversion ended with status " + std::to_string(inf[j].status) + " after " +
+                          std::to_string(inf[j].iters) + " iterations";
                 continue;
             }
-            int rc2 = qg2_phase_update(r.q, which);
-            if (!rc2 && which == 0) rc2 = qg2_phase_rhs(r.q, 1);
-            if (!rc2 && which == 1) rc2 = qg2_phase_post(r.q);
+            int rc2 = qg2_phase_update(r.q, k);
+            if (!rc2) rc2 = k + 1 < passes ? qg2_phase_rhs(r.q, k + 1) : qg2_phase_post(r.q);
             if (rc2) failed[idx[j]] = rc2;
         }
     }
@@ -337,7 +344,8 @@ extern "C" int se_server_submit(se_server* sv, const se_qg2_request* req, int64_
     std::memcpy(r->init.data() + nV, req->omega1, sizeof(double) * nV);
     std::memcpy(r->init.data() + 2 * nV, req->psi2, sizeof(double) * nV);
     std::memcpy(r->init.data() + 3 * nV, req->omega2, sizeof(double) * nV);
-    r->spec.psi1 = r->spec.omega1 = r->spec.psi2 = r->spec.omega2 = nullptr;
+    if (req->tau_e) r->tau_e.assign(req->tau_e, req->tau_e + nV);
+    r->spec.psi1 = r->spec.omega1 = r->spec.psi2 = r->spec.omega2 = r->spec.tau_e = nullptr;
     {
         std::lock_guard<std::mutex> lk(sv->mu);
         SE_CHECK(!sv->stopping, SE_ERR_STATE, "se_server_submit: server is stopping");

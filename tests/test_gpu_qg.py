@@ -98,3 +98,76 @@ def test_qg_channel_steps_match_oracle(gpu_ctx, detj, nEls, nsteps):
     on_wall = (np.abs(om.verts()[:, 1]) < 1e-12) | (np.abs(om.verts()[:, 1] - 1.0) < 1e-12)
     assert np.max(np.abs(q.state()[1][on_wall] - omega[on_wall])) <= 1e-12 * np.max(np.abs(omega))
     q.close()
+
+
+def smooth(xy, kx, ly, amp):
+    return amp * np.sin(ly * np.pi * xy[:, 1]) * np.cos(kx * 2.0 * np.pi * xy[:, 0])
+
+
+@pytest.mark.parametrize("detj", [0, 1], ids=["literal", "physical"])
+def test_qg2_dissipative_steps_match_oracle(gpu_ctx, detj):
+    """TwoLayerQGEqn with thermal relaxation toward tau_e, Ekman drag and
+    viscosity (DissipationRHS, TwoLayerQGEqn.cpp:419-462): the first inversion
+    pair, the dissipative SolveQ of both layers and the final pair."""
+    req, m, om, st = setup(gpu_ctx)
+    xy = om.verts()
+    diss = dict(E2=0.2, r=0.05, nu=2e-4, tau_e=smooth(xy, 1, 1, 0.02))
+    q = se_amd.QG2(m, req["F1"], req["F2"], req["beta"], detj)
+    q.set_dissipation(**diss)
+    q.set_state(*st)
+    dt = q.timestep(0.5)
+    ref, prev = list(st), None
+    for k in range(3):
+        infos = q.step(dt, rtol=1e-10)
+        assert all(i["status"] == 0 for i in infos)
+        s, it, *nxt = om.qg2_step(req["F1"], req["F2"], req["beta"], dt, *ref, detj_mode=detj, rtol=1e-10,
+                                  prev=prev, **diss)
+        assert s == 0
+        _, _, *plain = om.qg2_step(req["F1"], req["F2"], req["beta"], dt, *ref, detj_mode=detj, rtol=1e-10,
+                                   prev=prev)
+        assert max(rel(a, b) for a, b in zip(nxt, plain)) > 1e-6  # the dissipation moves the state
+        prev, ref = ref, nxt
+        got = q.state()
+        errs = [rel(a, b) for a, b in zip(got, ref)]
+        print("step", k, "rel errors psi1 om1 psi2 om2", ["%.2e" % e for e in errs])
+        assert max(errs) <= 1e-9, errs
+    q.close()
+
+
+@pytest.mark.parametrize("detj", [0, 1], ids=["literal", "physical"])
+def test_qg_dissipative_steps_match_oracle(gpu_ctx, detj):
+    """QuasiGeostrophicEqn with viscosity, linear drag and topography (DisRHS,
+    QuasiGeostrophicEqn.cpp:303-337): he1 with zero lifting, the dissipative
+    SolveQ, he2."""
+    cfg = W.qg_channel(nEls=10, N=6)
+    m = se_amd.Mesh(gpu_ctx, cfg["nEls"], cfg["N"], cfg["lo"], cfg["hi"], cfg["periodic"], cfg["bc"])
+    om = O.OracleMesh(cfg["nEls"], cfg["N"], cfg["lo"], cfg["hi"], cfg["periodic"], cfg["bc"])
+    X, Y = W.mesh_coords(cfg["nEls"], cfg["N"], cfg["lo"], cfg["hi"], m.gll()[0])
+    rng = W.Rng(cfg["seed"])
+    wall = np.repeat(np.sin(np.pi * np.asarray(Y)), len(X))
+    psi = 0.01 * W.gaussian_vortices(X, Y, cfg["vortices"], rng, cfg["lo"], cfg["hi"]) * wall
+    omega = W.gaussian_vortices(X, Y, cfg["vortices"], rng, cfg["lo"], cfg["hi"])
+    xy = om.verts()
+    topo = smooth(xy, 2, 2, 5.0)
+    diss = dict(nu=1e-4, r=0.3)
+    q = se_amd.QG(m, cfg["F"], cfg["beta"], detj)
+    q.set_dissipation(**diss)
+    q.set_topography(topo)
+    q.set_state(psi, omega)
+    dt = q.timestep(cfg["courant"])
+    ref, prev = [psi, omega], None
+    for k in range(3):
+        info = q.step(dt, rtol=1e-10)
+        assert info["status"] == 0
+        s, it, *nxt = om.qg_step(cfg["F"], cfg["beta"], dt, *ref, detj_mode=detj, rtol=1e-10, prev=prev,
+                                 topo=topo, **diss)
+        assert s == 0
+        _, _, *plain = om.qg_step(cfg["F"], cfg["beta"], dt, *ref, detj_mode=detj, rtol=1e-10, prev=prev,
+                                  topo=topo)
+        assert max(rel(a, b) for a, b in zip(nxt, plain)) > 1e-6
+        prev, ref = ref, nxt
+        got = q.state()
+        errs = [rel(a, b) for a, b in zip(got, ref)]
+        print("step", k, "rel errors psi omega", ["%.2e" % e for e in errs])
+        assert max(errs) <= 1e-9, errs
+    q.close()

@@ -75,13 +75,50 @@ def test_memory_budget_serialises_admission(gpu_ctx):
     # one 9x9 request's estimate fits, two do not
     nV = (9 * 6 + 1) ** 2
     n = 9 * 6 * (9 * 6 - 1)
-    one = 8 * 26 * nV + n * (2 * (8 * 10 + 4 * 4) + 8 * 4)
+    one = 8 * 29 * nV + n * (2 * (8 * 10 + 4 * 4) + 8 * 5)
     out, st = serve(reqs, memory_budget=1.5 * one)
     t = sorted((out[j][0]["t_admit"], out[j][0]["t_done"]) for j in range(3))
     for (a0, d0), (a1, d1) in zip(t, t[1:]):
         assert a1 >= d0  # the next request is admitted only after the previous one left
     for j in range(3):
         for a, b in zip(alone[j][1], out[j][1]):
+            assert np.array_equal(a, b)
+
+
+def step_directly(r, diss):
+    q = se_amd.QG2(r["mesh"], r["F1"], r["F2"], r["beta"], se_amd.DETJ_LITERAL)
+    if diss:
+        q.set_dissipation(**diss)
+    q.set_state(*r["init"])
+    dt = q.timestep(0.5)
+    for _ in range(r["steps"]):
+        q.step(dt, rtol=1e-10, precision=se_amd.PREC_FP32_IR)
+    out = q.state()
+    q.close()
+    return out
+
+
+@pytest.mark.parametrize("max_active", [0, 1], ids=["mixed", "one-at-a-time"])
+def test_dissipative_requests_in_a_mix(gpu_ctx, max_active):
+    """Requests with dissipation (four inversions per step, the first pair
+    batched beside the other requests' only pair) and without, on the same
+    mesh shape: each result is bitwise the direct se_qg2_step one.  One at a
+    time, the plain request reuses the pooled instance the dissipative one
+    left, which must not keep its dissipation."""
+    reqs = make_requests(gpu_ctx, [8, 8, 10], [3, 2, 3])
+    xy = np.stack(np.meshgrid(*W.mesh_coords(reqs[0]["nEls"], reqs[0]["N"], reqs[0]["lo"], reqs[0]["hi"],
+                                              reqs[0]["mesh"].gll()[0]), indexing="xy"), -1).reshape(-1, 2)
+    tau_e = 0.02 * np.sin(np.pi * xy[:, 1]) * np.cos(2 * np.pi * xy[:, 0])
+    diss = [dict(E2=0.2, r=0.05, nu=2e-4, tau_e=tau_e), None, dict(E2=0.1, r=0.0, nu=1e-4)]
+    for r, d in zip(reqs, diss):
+        if d:
+            r.update(d)
+    out, st = serve(reqs, max_active=max_active)
+    assert st["completed"] == 3
+    for j, r in enumerate(reqs):
+        res, f = out[j]
+        assert res["status"] == 0 and res["steps_done"] == r["steps"]
+        for a, b in zip(step_directly(r, diss[j]), f):
             assert np.array_equal(a, b)
 
 
