@@ -200,7 +200,13 @@ def io_lib():
         sig = {"se_io_last_error": (C.c_char_p, []),
                "se_field_save": (i, [C.c_char_p, C.c_uint, pd, i, i, pi]),
                "se_field_read": (i, [C.c_char_p, C.c_uint, pd, i, i]),
-               "se_field_info": (i, [C.c_char_p, C.c_uint, pi, pi, pi, pi])}
+               "se_field_info": (i, [C.c_char_p, C.c_uint, pi, pi, pi, pi]),
+               "se_mesh_save": (i, [C.c_char_p, pi, pd, pd, pd]),
+               "se_mesh_read": (i, [C.c_char_p, pi, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+               "se_xdmf_header": (i, [C.c_uint]),
+               "se_xdmf_footer": (i, [C.c_uint]),
+               "se_xdmf_write": (i, [C.c_char_p, pi, C.c_void_p, i, C.c_uint, C.c_double, C.c_double]),
+               "se_xdmf_temporal": (i, [i, i])}
         for n, (r, a) in sig.items():
             f = getattr(L, n)
             f.restype, f.argtypes = r, a
@@ -235,6 +241,54 @@ def field_info(name, step):
     res = (C.c_int * 2)()
     _io_check(io_lib().se_field_info(name.encode(), step, C.byref(a), C.byref(b), C.byref(c), res), "se_field_info")
     return a.value, b.value, c.value, (res[0], res[1])
+
+
+def mesh_save(name, nVerts, lo, hi, verts):
+    """Mesh::Save: <name>.h5 with the vertices (nVertsTotal, 2) and the linear-quad connectivity."""
+    v = np.ascontiguousarray(verts, dtype=np.float64).reshape(-1, 2)
+    nv = (C.c_int * 2)(*nVerts)
+    _io_check(io_lib().se_mesh_save(name.encode(), nv, _d(lo), _d(hi), v.ravel()), "se_mesh_save")
+
+
+def mesh_read(name):
+    """(nVerts, min, max, vertices (nV, 2), connectivity (nQuads, 4)) of a mesh file."""
+    nv = (C.c_int * 2)()
+    _io_check(io_lib().se_mesh_read(name.encode(), nv, None, None, None, None), "se_mesh_read")
+    nV, nq = nv[0] * nv[1], (nv[0] - 1) * (nv[1] - 1)
+    lo, hi, verts = np.zeros(2), np.zeros(2), np.zeros((nV, 2))
+    conn = np.zeros((nq, 4), dtype=np.int32)
+    _io_check(io_lib().se_mesh_read(name.encode(), nv, lo.ctypes.data, hi.ctypes.data, verts.ctypes.data,
+                                    conn.ctypes.data), "se_mesh_read")
+    return (nv[0], nv[1]), lo, hi, verts, conn
+
+
+class XdmfField(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("nDofs", C.c_int), ("vals", C.c_void_p)]
+
+
+def xdmf_write(mesh_name, nVerts, fields, step, time, dt):
+    """WriteXDMF: fields = [(name, vals (nV,) or (nV, nDofs))]; the field files, time.<step>.ts and
+    the step's grid in XDMF.<step>.xmf (working directory)."""
+    arrs = [np.ascontiguousarray(v, dtype=np.float64) for _, v in fields]
+    names = [n.encode() for n, _ in fields]
+    F = (XdmfField * max(1, len(fields)))()
+    for k, (nm, a) in enumerate(zip(names, arrs)):
+        F[k].name, F[k].nDofs, F[k].vals = nm, 1 if a.ndim == 1 else a.shape[1], a.ctypes.data
+    nv = (C.c_int * 2)(*nVerts)
+    _io_check(io_lib().se_xdmf_write(mesh_name.encode(), nv, C.cast(F, C.c_void_p), len(fields), step, time, dt),
+              "se_xdmf_write")
+
+
+def xdmf_header(step):
+    _io_check(io_lib().se_xdmf_header(step), "se_xdmf_header")
+
+
+def xdmf_footer(step):
+    _io_check(io_lib().se_xdmf_footer(step), "se_xdmf_footer")
+
+
+def xdmf_temporal(n_steps, dump_every):
+    _io_check(io_lib().se_xdmf_temporal(n_steps, dump_every), "se_xdmf_temporal")
 
 
 class Context:

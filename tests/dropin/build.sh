@@ -33,7 +33,8 @@ echo "dropin build: wrote $OUT/dropin_check"
 # field-file interop with the reference's Field::Save / Read (needs libse_amd_io.so)
 if [ -f "$ROOT/se_amd/lib/libse_amd_io.so" ]; then
     g++ -O2 -std=c++11 -w -I"$REF/src/base" -I"$ROOT/include" -I"$HDF5/include" "$HERE/io_check.cpp" \
-        -o "$OUT/io_check" -L"$OUT" -lsebase -L"$ROOT/se_amd/lib" -lse_amd_io \
+        -o "$OUT/io_check" -L"$OUT" -lsebase -L"$ROOT/se_amd/lib" -lse_amd_io "$HDF5/lib/libhdf5.so" \
+        -Wl,-rpath,"$HDF5/lib" \
         -Wl,-rpath,'$ORIGIN' -Wl,-rpath,'$ORIGIN/../../se_amd/lib' -Wl,-rpath-link,"$HDF5/lib"
     echo "dropin build: wrote $OUT/io_check"
 fi
