@@ -123,6 +123,14 @@ int se_vec_to_field_dev(se_mesh* mesh, const double* d_vec, double* d_field);
  *       MassMatrix::AssembleElement Operator.cpp:157-177] */
 int se_helm_create(se_mesh* mesh, double c_mass, double c_lap, int detj_mode, se_op** out);
 int se_helm_destroy(se_op* op);
+/* adds the spectral vanishing viscosity term of HelmholtzEqn(..., dt,
+ * svvCutoff): A += dt (Lxx + Lyy) per element, the modal filter of the modes
+ * above `cutoff` (1 .. N-1) in both directions, Lyy built from the x terms as
+ * the reference builds it [ref: HelmholtzEqn.cpp:71-76, SVV::AssembleElement
+ * src/base/SVV.cpp:120-176].  Before set_dirichlet; once per operator.  Such
+ * operators solve in FP64 (the mixed-precision and batched solvers refuse
+ * them). */
+int se_helm_set_svv(se_op* op, double dt, int cutoff);
 /* y = A x   [ref: MatMult on the matrix built by Matrix::Assemble
  *            src/base/Matrix.cpp:47-133] */
 int se_helm_apply(se_op* op, const double* x, double* y);

@@ -71,6 +71,8 @@ def lib():
             "orc_genvel": (None, [vp, _D, _D]),
             "orc_calc_timestep": (d, [vp, _D, d]),
             "orc_qg_solveq": (None, [vp, d, d, _D, d, d, d, vp, vp, vp, _D]),
+            "orc_elem_svv": (None, [vp, i, d, i, _D]),
+            "orc_helm_set_svv": (None, [vp, d, i]),
             "orc_qg_step": (i, [vp, d, d, d, d, vp, d, i, _D, _D, vp, vp, d, i, C.POINTER(i)]),
             "orc_qg2_step": (i, [vp, d, d, d, d, d, d, vp, d, i, _D, _D, _D, _D, vp, vp, vp, vp, d, i,
                                  C.POINTER(i)]),
@@ -306,6 +308,11 @@ class OracleMesh(_MeshBase):
         self.L.orc_helm_apply_ideal(self.h, c_mass, c_lap, detj_mode, _d(x), y)
         return y
 
+    def elem_svv(self, el, c, mN):
+        A = np.zeros(self.nl * self.nl)
+        self.L.orc_elem_svv(self.h, el, c, mN, A)
+        return A.reshape(self.nl, self.nl)
+
     def qg_solveq(self, a, dt, qhat, alpha, rF=0.0, Ei=0.0, nu=0.0, tau=None, omega_hat=None, tau_e=None):
         """SolveQ with the dissipation right-hand side; returns alpha with its
         unknowns replaced (Dirichlet nodes keep the given alpha)."""
@@ -366,10 +373,13 @@ class RefMesh(_MeshBase):
 class Helm:
     """orc_helm: PETSc-pattern assembled c_mass*M + c_lap*K~ with lifting and PCG."""
 
-    def __init__(self, mesh, c_mass, c_lap, detj_mode=0, sol_field=None):
+    def __init__(self, mesh, c_mass, c_lap, detj_mode=0, sol_field=None, svv=None):
+        """svv = (dt, cutoff): HelmholtzEqn's SVV operator."""
         self.m = mesh
         self.L = mesh.L
         self.h = self.L.orc_helm_create(mesh.h, c_mass, c_lap, detj_mode)
+        if svv is not None:
+            self.L.orc_helm_set_svv(self.h, float(svv[0]), int(svv[1]))
         if sol_field is None:
             sol_field = np.zeros(mesh.nV)
         self.L.orc_helm_assemble(self.h, _d(sol_field))

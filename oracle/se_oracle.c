@@ -579,6 +579,167 @@ void orc_elem_field_rhs(const orc_mesh* m, int el, double c, const double* field
     free(nodes);
 }
 
+/* Legendre::ModalBasis (Legendre.cpp:307-338): monomial coefficients of the
+ * hierarchical modes, phi[k*(N+1) + i] the coefficient of x^i in phi_k:
+ * phi_0 = (1 + x)/2, phi_1 = (1 - x)/2, phi_k = (1 - x^2) P_{k-2} / 4 with
+ * P_j the Jacobi(j, 1) polynomials of Jacobi::Gen (Jacobi.cpp:21-36):
+ * P_0 = 1, P_1 = 2x. */
+static void modal_basis(int N, double* phi) {
+    int n1 = N + 1, k, i;
+    double* P = (double*)calloc((size_t)n1 * n1, sizeof(double)); /* P[j*n1 + i] */
+    memset(phi, 0, sizeof(double) * (size_t)n1 * n1);
+    P[0] = 1.0;
+    if (N >= 1) P[n1 + 1] = 2.0;
+    for (k = 2; k <= N; k++) {
+        const double a = 1.0;
+        double c1 = 2 * k * (2 * a + k) * (2 * a + 2 * k - 2);
+        double c2 = (2 * a + 2 * k - 1) * (2 * a + 2 * k) * (2 * a + 2 * k - 2);
+        double c3 = 2 * (a + k - 1) * (a + k - 1) * (2 * a + 2 * k);
+        c2 /= c1;
+        c3 /= c1;
+        for (i = 0; i <= k - 1; i++) P[k * n1 + i + 1] += c2 * P[(k - 1) * n1 + i];
+        for (i = 0; i <= k - 2; i++) P[k * n1 + i] -= c3 * P[(k - 2) * n1 + i];
+    }
+    phi[0] = 0.5;
+    phi[1] = 0.5;
+    if (N >= 1) {
+        phi[n1 + 0] = 0.5;
+        phi[n1 + 1] = -0.5;
+    }
+    for (k = 2; k <= N; k++)
+        for (i = 0; i < k - 1; i++) {
+            phi[k * n1 + i] += 0.25 * P[(k - 2) * n1 + i];
+            phi[k * n1 + i + 2] -= 0.25 * P[(k - 2) * n1 + i];
+        }
+    free(P);
+}
+
+/* Jacobi::EvalTotal (Jacobi.cpp:48-56): highest power first */
+static double eval_total(const double* c, int n, double x) {
+    double p = 0.0;
+    int i;
+    for (i = n; i >= 0; i--) p += c[i] * pow(x, i);
+    return p;
+}
+
+/* dense products of n x n row-major matrices */
+static void mat_mul(const double* A, const double* B, double* C, int n) {
+    int i, j, k;
+    for (i = 0; i < n; i++)
+        for (j = 0; j < n; j++) {
+            double t = 0.0;
+            for (k = 0; k < n; k++) t += A[i * n + k] * B[k * n + j];
+            C[i * n + j] = t;
+        }
+}
+static void mat_tran(const double* A, double* At, int n) {
+    int i, j;
+    for (i = 0; i < n; i++)
+        for (j = 0; j < n; j++) At[j * n + i] = A[i * n + j];
+}
+/* inverse by Gauss-Jordan elimination with full pivoting (the method of
+ * Inv, LinAlg.cpp:52-110) */
+static void mat_inv(const double* A, double* R, int n) {
+    int* piv = (int*)calloc((size_t)n, sizeof(int));
+    int* rowof = (int*)malloc(sizeof(int) * (size_t)n);
+    int* colof = (int*)malloc(sizeof(int) * (size_t)n);
+    int i, j, k, r = 0, c = 0;
+    memcpy(R, A, sizeof(double) * (size_t)n * n);
+    for (i = 0; i < n; i++) {
+        double big = 0.0, inv;
+        for (j = 0; j < n; j++)
+            if (piv[j] != 1)
+                for (k = 0; k < n; k++)
+                    if (piv[k] == 0 && fabs(R[j * n + k]) >= big) {
+                        big = fabs(R[j * n + k]);
+                        r = j;
+                        c = k;
+                    }
+        piv[c]++;
+        if (r != c)
+            for (k = 0; k < n; k++) {
+                double t = R[r * n + k];
+                R[r * n + k] = R[c * n + k];
+                R[c * n + k] = t;
+            }
+        rowof[i] = r;
+        colof[i] = c;
+        inv = 1.0 / R[c * n + c];
+        R[c * n + c] = 1.0;
+        for (k = 0; k < n; k++) R[c * n + k] *= inv;
+        for (j = 0; j < n; j++)
+            if (j != c) {
+                double d = R[j * n + c];
+                R[j * n + c] = 0.0;
+                for (k = 0; k < n; k++) R[j * n + k] -= R[c * n + k] * d;
+            }
+    }
+    for (i = n - 1; i >= 0; i--)
+        if (rowof[i] != colof[i])
+            for (k = 0; k < n; k++) {
+                double t = R[k * n + rowof[i]];
+                R[k * n + rowof[i]] = R[k * n + colof[i]];
+                R[k * n + colof[i]] = t;
+            }
+    free(piv); free(rowof); free(colof);
+}
+
+/* SVV::AssembleElement, SVV.cpp:120-176 (nDofs = 1).  Adds into A (nl x nl)
+ * c (Lxx + Lyy) with, for the element's quadrature points i and nodes j,
+ * (ix, iy) = (i / (N+1), i % (N+1)) as SVV's GetRowCol numbers them:
+ *   B_ij = phi_jx(x_ix) phi_jy(x_iy)       nodal values of the modes
+ *   W = diag(DetJac w_i), Q = diag(sqrt(exp(-(N-jx)^2/(mN-jx)^2
+ *       - (N-jy)^2/(mN-jy)^2))) on the modes with jx, jy > mN (else 0)
+ *   Dx_ij = dN_j/dxi at point i (the reference element's derivative)
+ *   M = (B^T W B)^-1 B^T W,  F = M^-1 Q M,  Lxx = (F Dx)^T W (F Dx)
+ * and Lyy built from the same x terms as Lxx (SVV.cpp:165). */
+void orc_elem_svv(const orc_mesh* m, int el, double c, int mN, double* A) {
+    const int N = m->N, n1 = N + 1, nl = m->nl;
+    const size_t sz = sizeof(double) * (size_t)nl * nl;
+    double *phi = (double*)malloc(sizeof(double) * (size_t)n1 * n1), *GNix = (double*)malloc(sizeof(double) * 2 * nl);
+    double *B = (double*)malloc(sz), *W = (double*)calloc((size_t)nl * nl, sizeof(double));
+    double *Q = (double*)calloc((size_t)nl * nl, sizeof(double)), *Dx = (double*)malloc(sz);
+    double *Bt = (double*)malloc(sz), *BtW = (double*)malloc(sz), *BtWB = (double*)malloc(sz);
+    double *BtWBinv = (double*)malloc(sz), *M = (double*)malloc(sz), *Minv = (double*)malloc(sz);
+    double *T1 = (double*)malloc(sz), *F = (double*)malloc(sz), *FD = (double*)malloc(sz);
+    double *FDt = (double*)malloc(sz), *FDtW = (double*)malloc(sz), *L = (double*)malloc(sz);
+    int i, j;
+    modal_basis(N, phi);
+    for (i = 0; i < nl; i++) {
+        int ix = i / n1, iy = i % n1;
+        double weight = m->weights[i % n1] * m->weights[i / n1];
+        ref_shape_func_derivs(m, i, GNix);
+        for (j = 0; j < nl; j++) {
+            int jx = j / n1, jy = j % n1;
+            B[i * nl + j] = eval_total(&phi[jx * n1], N, m->abcissa[ix]) * eval_total(&phi[jy * n1], N, m->abcissa[iy]);
+            if (i == j) {
+                W[i * nl + j] = orc_det_jac(m, el, i) * weight;
+                if (jx > mN && jy > mN) {
+                    double qx = -1.0 * ((double)(N - jx) * (N - jx)) / ((mN - jx) * (mN - jx));
+                    double qy = -1.0 * ((double)(N - jy) * (N - jy)) / ((mN - jy) * (mN - jy));
+                    Q[i * nl + j] = sqrt(exp(qx + qy));
+                }
+            }
+            Dx[i * nl + j] = GNix[j];
+        }
+    }
+    mat_tran(B, Bt, nl);
+    mat_mul(Bt, W, BtW, nl);
+    mat_mul(BtW, B, BtWB, nl);
+    mat_inv(BtWB, BtWBinv, nl);
+    mat_mul(BtWBinv, BtW, M, nl);
+    mat_inv(M, Minv, nl);
+    mat_mul(Minv, Q, T1, nl);
+    mat_mul(T1, M, F, nl);
+    mat_mul(F, Dx, FD, nl);
+    mat_tran(FD, FDt, nl);
+    mat_mul(FDt, W, FDtW, nl);
+    mat_mul(FDtW, FD, L, nl); /* Lxx, and Lyy the same */
+    for (i = 0; i < nl * nl; i++) A[i] += c * (L[i] + L[i]);
+    free(phi); free(GNix); free(B); free(W); free(Q); free(Dx); free(Bt); free(BtW); free(BtWB);
+    free(BtWBinv); free(M); free(Minv); free(T1); free(F); free(FD); free(FDt); free(FDtW); free(L);
+}
+
 /* ------------------------------------------------------------------------- */
 /* a8-a10: assembled operator (PETSc AIJ pattern of dense element blocks)    */
 /* ------------------------------------------------------------------------- */
@@ -594,7 +755,16 @@ struct orc_helm {
     double* b;        /* Dirichlet lifting, HelmholtzEqn::Assemble */
     double* dinv;     /* Matrix::InverseDiagonal */
     int assembled;
+    double svv_c;     /* SVV operator constant (dt) and cutoff; svv_mN = 0: none */
+    int svv_mN;
 };
+
+/* HelmholtzEqn's third operator, SVV(solField, solField, dt, svvCutoff)
+ * (HelmholtzEqn.cpp:71-76); call before orc_helm_assemble */
+void orc_helm_set_svv(orc_helm* h, double dt, int mN) {
+    h->svv_c = dt;
+    h->svv_mN = mN;
+}
 
 orc_helm* orc_helm_create(const orc_mesh* m, double c_mass, double c_lap, int detj_mode) {
     orc_helm* h = (orc_helm*)calloc(1, sizeof(orc_helm));
@@ -616,12 +786,13 @@ static int cmp_int(const void* a, const void* b) {
     return (x > y) - (x < y);
 }
 
-/* element block: ops applied in the reference order {mass, laplacian}
+/* element block: ops applied in the reference order {mass, laplacian, svv}
  * (HelmholtzEqn.cpp:69-81); PoissonEqn has the Laplacian only. */
 static void elem_block(const orc_helm* h, int el, double* Ae) {
     memset(Ae, 0, sizeof(double) * (size_t)h->m->nl * (size_t)h->m->nl);
     if (h->c_mass != 0.0) orc_elem_mass(h->m, el, h->c_mass, Ae);
     orc_elem_laplacian(h->m, el, h->c_lap, h->detj_mode, Ae);
+    if (h->svv_mN > 0) orc_elem_svv(h->m, el, h->svv_c, h->svv_mN, Ae);
 }
 
 /* Matrix::Assemble, Matrix.cpp:47-133 + HelmholtzEqn::Assemble :91-103.

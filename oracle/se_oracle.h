@@ -54,6 +54,8 @@ void   orc_shape_func_derivs(const orc_mesh* m, int el, int pt, double* gGNix, d
 void orc_elem_laplacian(const orc_mesh* m, int el, double c, int detj_mode, double* K);
 void orc_elem_mass(const orc_mesh* m, int el, double c, double* M);
 void orc_elem_field_rhs(const orc_mesh* m, int el, double c, const double* field, double* rhs);
+/* SVV::AssembleElement (SVV.cpp:120-176): adds c (Lxx + Lyy) into A (nl x nl) */
+void orc_elem_svv(const orc_mesh* m, int el, double c, int mN, double* A);
 
 /* ---- a8-a10: assembled operator + PCG (Matrix.cpp:47-184, Vector.cpp:40-121,
  *      HelmholtzEqn.cpp:21-136, PoissonEqn.cpp:30-134) ---- */
@@ -61,6 +63,8 @@ typedef struct orc_helm orc_helm;
 /* A = c_mass*M + c_lap*K~ ; c_mass == 0 reproduces PoissonEqn (Laplacian only). */
 orc_helm* orc_helm_create(const orc_mesh* m, double c_mass, double c_lap, int detj_mode);
 void orc_helm_destroy(orc_helm* h);
+/* the SVV operator of HelmholtzEqn(..., dt, svvCutoff); before assembly */
+void orc_helm_set_svv(orc_helm* h, double dt, int mN);
 void orc_helm_assemble(orc_helm* h, const double* sol_field);
 long orc_helm_nnz(const orc_helm* h);
 void orc_helm_csr(const orc_helm* h, long* rowptr, int* cols, double* vals);

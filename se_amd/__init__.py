@@ -126,6 +126,7 @@ def lib():
             "se_batch_pcg": (i, [vp, vp, vp, C.POINTER(SolveOpts), vp]),
             "se_batch_probe": (i, [vp, i, i, i, _I, _D, _D, _D, _D]),
             "se_helm_probe": (i, [vp, i, i, C.POINTER(d), C.POINTER(d), C.POINTER(d), C.POINTER(d)]),
+            "se_helm_set_svv": (i, [vp, d, i]),
             "se_qg2_create": (i, [vp, d, d, d, i, pvp]),
             "se_qg2_destroy": (i, [vp]),
             "se_qg2_set_state": (i, [vp, _D, _D, _D, _D]),
@@ -421,6 +422,10 @@ class Helm:
         check(lib().se_helm_create(mesh.h, c_mass, c_lap, detj_mode, C.byref(h)), "se_helm_create")
         self.h = h
         mesh.ctx._own(self)
+
+    def set_svv(self, dt, cutoff):
+        """HelmholtzEqn's spectral vanishing viscosity term (FP64 solves only)."""
+        check(lib().se_helm_set_svv(self.h, dt, cutoff), "se_helm_set_svv")
 
     def apply(self, x):
         y = np.zeros(self.mesh.n)

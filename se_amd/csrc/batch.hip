@@ -473,6 +473,7 @@ extern "C" int se_batch_create(se_ctx* ctx, int count, se_op* const* ops, se_bat
         SE_CHECK(ops[i] != nullptr, SE_ERR_ARG, "se_batch_create: NULL member");
         SE_CHECK(ops[i]->mesh->ctx == ctx, SE_ERR_ARG, "se_batch_create: member created on another context");
         SE_CHECK(ops[i]->definite, SE_ERR_UNSUPPORTED, "se_batch_create: member operator is not definite");
+        SE_CHECK(ops[i]->svv_mN == 0, SE_ERR_UNSUPPORTED, "se_batch_create: members with SVV solve in FP64 alone");
         SE_CHECK(ops[i]->mesh->N >= 2 && ops[i]->mesh->N <= 8, SE_ERR_UNSUPPORTED,
                  "se_batch_create: orders 2..8");
         for (int j = 0; j < i; j++)

@@ -412,6 +412,16 @@ int op_apply_dev(se_op* op, const double* d_x, double* d_y, double scale, bool d
     OpParams P = op_params(op, scale);
     hipStream_t s = op->mesh->ctx->stream;
     double* pa = op->partials.as<double>() + part_layout(op->mesh).a;
+    if (op->svv_mN > 0) {
+        // mass + Laplacian, the SVV term, then the partials of x.y the CG
+        // update reads (as many as the apply kernel would write)
+        int rc = launch_apply<false>(op->mesh->N, op->tile, P, d_x, d_y, pa, op->state.as<CgState>(), s);
+        if (rc) return rc;
+        SE_HIP(hipGetLastError());
+        rc = svv_apply(op, d_x, d_y, scale);
+        if (!rc && dot) rc = svv_dot(op, d_x, d_y, pa, apply_tiles(op));
+        return rc;
+    }
     int rc = dot ? launch_apply<true>(op->mesh->N, op->tile, P, d_x, d_y, pa, op->state.as<CgState>(), s)
                  : launch_apply<false>(op->mesh->N, op->tile, P, d_x, d_y, pa, op->state.as<CgState>(), s);
     if (rc) return rc;
@@ -551,6 +561,7 @@ extern "C" int se_helm_set_dirichlet_dev(se_op* op, const double* d_sol_field) {
                                            nullptr, nullptr, op->mesh->ctx->stream);
     if (rc) return rc;
     SE_HIP(hipGetLastError());
+    if (op->svv_mN > 0) return svv_lift(op, d_sol_field, op->lift.as<double>());
     return SE_OK;
 }
 

@@ -193,6 +193,7 @@ size_t part_total(const se_mesh* m) { return 7 * (size_t)sep_tiles(m) + 256; }
 namespace se {
 // tables and work buffers of the mixed-precision solve
 int tcg_prepare(se_op* op) {
+    SE_CHECK(op->svv_mN == 0, SE_ERR_UNSUPPORTED, "mixed-precision solve: operators with SVV solve in FP64");
     const se_mesh* m = op->mesh;
     const size_t nb = (size_t)std::max(m->n, 1);
     int rc = op_ensure_work(op);
@@ -493,6 +494,9 @@ extern "C" int se_helm_probe(se_op* op, int precision, int reps, double* us_dir,
 
 namespace se {
 int pcg_mixed(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, se_solve_info& info) {
+    // the inner operator is the separable Sy Gx + Gy Sx + My Mx; the SVV term
+    // (band x band) is not of that form
+    SE_CHECK(op->svv_mN == 0, SE_ERR_UNSUPPORTED, "mixed-precision solve: operators with SVV solve in FP64");
     if (o.precision == SE_PREC_FP32_IR) return tcg_solve<float>(op, d_f, d_x, o, info);
     if (o.precision == SE_PREC_BF16_IR) return tcg_solve<__hip_bfloat16>(op, d_f, d_x, o, info);
     return fail(SE_ERR_ARG, "unknown precision");

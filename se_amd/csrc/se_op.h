@@ -28,11 +28,16 @@ struct se_op {
     int graph_iters = 0;
     hipGraphExec_t graph_exec_lo = nullptr;
     int graph_iters_lo = 0;
+    // spectral vanishing viscosity term (svv.hip): dt, cutoff (0: none), the
+    // 1-D factors Mf | Kf
+    double svv_dt = 0.0;
+    int svv_mN = 0;
+    se::DevBuf svv_mk;
 
     void release_all() {
         se::DevBuf* bufs[] = {&sep, &sep_state, &sep_part, &dinv, &dinv_s, &lift, &tabs, &state, &partials, &b, &x, &r, &p, &q, &f,
                               &sx, &sy, &sfield, &sfield2, &lo_r, &lo_p, &lo_q, &lo_z, &lo_x, &lo_dinv,
-                              &lo_geo, &d64};
+                              &lo_geo, &d64, &svv_mk};
         for (auto* b : bufs) b->release();
         if (graph_exec_lo) (void)hipGraphExecDestroy(graph_exec_lo);
         graph_exec_lo = nullptr;
@@ -47,6 +52,10 @@ int op_apply_dev(se_op* op, const double* d_x, double* d_y, double scale, bool d
 int op_ensure_work(se_op* op);
 int pcg_fp64(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, se_solve_info& info);
 int pcg_mixed(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, se_solve_info& info);
+// SVV term (svv.hip): y += scale S x; np partials of x.y; lift -= S x_D
+int svv_apply(se_op* op, const double* d_x, double* d_y, double scale);
+int svv_dot(se_op* op, const double* d_x, const double* d_y, double* partials, int np);
+int svv_lift(se_op* op, const double* d_field, double* d_lift);
 }  // namespace se
 
 // mixed-precision solve setup (mixed.hip), used by the batched solver

@@ -1,12 +1,14 @@
 // Drop-in replacement for the reference's src/linsys/HelmholtzEqn.h: the same
 // public interface (constructor, solField/rhsField/c/nu/dt/svvCutoff,
 // Assemble, Solve) without the PETSc members, solved by libse_amd on the GPU.
-// A = c M + nu dt K~ with the reference's element operators
-// (src/linsys/HelmholtzEqn.cpp:21-82); Solve assembles f = M rhs + b, solves
-// A x = f by Jacobi-PCG to a true relative residual of 1e-10 (SE_RTOL) and
-// writes x back into solField (HelmholtzEqn.cpp:105-136).  SE_PRECISION=fp32
-// or bf16 selects the mixed-precision solver.  Include after the reference's
-// Field.h, like the original.
+// A = c M + nu dt K~ (+ dt SVV with svvCutoff > 0) with the reference's element
+// operators (src/linsys/HelmholtzEqn.cpp:21-82); Solve assembles f = M rhs + b,
+// solves A x = f by Jacobi-PCG to a true relative residual of 1e-10 (SE_RTOL)
+// and writes x back into solField (HelmholtzEqn.cpp:105-136).  Fields of 1 or
+// 2 dofs (a 2-dof field solves one operator per dof, with that dof's Dirichlet
+// sides).  SE_PRECISION=fp32 or bf16 selects the mixed-precision solver
+// (operators without SVV).  Include after the reference's Field.h, like the
+// original.
 #ifndef SE_DROPIN_HELMHOLTZ_EQN_H
 #define SE_DROPIN_HELMHOLTZ_EQN_H
 #include <string>
@@ -31,7 +33,8 @@ class HelmholtzEqn {
 
    private:
     bool assembled;
-    se_op* op;
+    se_op* op;   // dof 0
+    se_op* op2;  // dof 1 of a 2-dof field
 };
 
 #endif

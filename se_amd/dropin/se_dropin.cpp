@@ -8,6 +8,7 @@
 // geometry built on the GPU, kept for the process), and the field values are
 // gathered from / scattered to Field::vals around each call.  Errors print to
 // cerr and exit(1), as the reference does on misuse.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <iostream>
@@ -54,18 +55,19 @@ se_ctx* ctx() {
     return g_ctx;
 }
 
-// the device mesh of a field: geometry of its Mesh, Dirichlet sides of its BCs (dof 0)
-se_mesh* device_mesh(const Field* f) {
+// the device mesh of a field's dof: geometry of its Mesh, Dirichlet sides of
+// its BCs for that dof (a 2-dof BCs holds one set per dof, BCs.cpp:176-322)
+se_mesh* device_mesh(const Field* f, int dof = 0) {
     std::lock_guard<std::mutex> lk(g_mu);
     const Mesh* M = f->mesh;
     int bc[4] = {0, 0, 0, 0};
     if (f->bcs) {
         // BCs marks the duplicate line of a periodic direction as a boundary
         // (BCs.cpp:38-44); those are not Dirichlet sides
-        bc[0] = M->periodic[1] ? 0 : f->bcs->bcBottom[0];
-        bc[1] = M->periodic[1] ? 0 : f->bcs->bcTop[0];
-        bc[2] = M->periodic[0] ? 0 : f->bcs->bcLeft[0];
-        bc[3] = M->periodic[0] ? 0 : f->bcs->bcRight[0];
+        bc[0] = M->periodic[1] ? 0 : f->bcs->bcBottom[dof];
+        bc[1] = M->periodic[1] ? 0 : f->bcs->bcTop[dof];
+        bc[2] = M->periodic[0] ? 0 : f->bcs->bcLeft[dof];
+        bc[3] = M->periodic[0] ? 0 : f->bcs->bcRight[dof];
     }
     for (const MeshEntry& e : g_meshes)
         if (e.mesh == M && e.bc[0] == bc[0] && e.bc[1] == bc[1] && e.bc[2] == bc[2] && e.bc[3] == bc[3]) return e.m;
@@ -112,6 +114,13 @@ void check_scalar(const Field* f, const char* who) {
     }
 }
 
+// dof d of a node-major field array
+std::vector<double> column(const std::vector<double>& a, int nd, int d) {
+    std::vector<double> c(a.size() / nd);
+    for (size_t v = 0; v < c.size(); v++) c[v] = a[v * nd + d];
+    return c;
+}
+
 // Field-level solve shared by both equations: f = M rhs + b, solve, UpdateField
 void field_solve(se_op* op, Field* sol, Field* rhs, bool null_space, se_solve_info* info) {
     std::vector<double> r, s;
@@ -129,33 +138,81 @@ void field_solve(se_op* op, Field* sol, Field* rhs, bool null_space, se_solve_in
 }  // namespace
 
 // ---- HelmholtzEqn (src/linsys/HelmholtzEqn.cpp:21-136) ----------------------
+// A 2-dof field (a velocity) is the reference's block-diagonal system of one
+// operator per dof, each with that dof's Dirichlet sides (the element
+// operators couple equal dofs only, Operator.cpp:67-80,161-175, SVV.cpp:166-175;
+// BCs numbers dof 0's unknowns first): one device operator per dof.
 HelmholtzEqn::HelmholtzEqn(Field* _solField, Field* _rhsField, double _c, double _nu, double _dt, int _svvCutoff)
     : solField(_solField), rhsField(_rhsField), c(_c), nu(_nu), dt(_dt), svvCutoff(_svvCutoff), info(),
-      assembled(false), op(nullptr) {
-    check_scalar(solField, "HelmholtzEqn");
-    if (svvCutoff) {
-        std::cerr << "HelmholtzEqn: the spectral vanishing viscosity operator is not provided by libse_amd"
-                  << std::endl;
+      assembled(false), op(nullptr), op2(nullptr) {
+    const int nd = solField->nDofs;
+    if (nd != 1 && nd != 2) {
+        std::cerr << "HelmholtzEqn: libse_amd solves fields of 1 or 2 dofs" << std::endl;
         exit(1);
     }
-    int rc = se_helm_create(device_mesh(solField), c, nu * dt, SE_DETJ_LITERAL, &op);
-    if (rc) die("se_helm_create", rc);
+    if (rhsField && rhsField->nDofs != nd) {
+        std::cerr << "HelmholtzEqn: the rhs field must have the solution's dofs" << std::endl;
+        exit(1);
+    }
+    for (int d = 0; d < nd; d++) {
+        se_op** o = d ? &op2 : &op;
+        int rc = se_helm_create(device_mesh(solField, d), c, nu * dt, SE_DETJ_LITERAL, o);
+        if (rc) die("se_helm_create", rc);
+        if (svvCutoff) {
+            rc = se_helm_set_svv(*o, dt, svvCutoff);
+            if (rc) die("se_helm_set_svv", rc);
+        }
+    }
 }
 
-HelmholtzEqn::~HelmholtzEqn() { se_helm_destroy(op); }
+HelmholtzEqn::~HelmholtzEqn() {
+    se_helm_destroy(op);
+    se_helm_destroy(op2);
+}
 
 void HelmholtzEqn::Assemble() {
     std::vector<double> s;
     gather(solField, s);
-    int rc = se_helm_set_dirichlet(op, s.data());
-    if (rc) die("se_helm_set_dirichlet", rc);
+    const int nd = solField->nDofs;
+    for (int d = 0; d < nd; d++) {
+        std::vector<double> sd = column(s, nd, d);
+        int rc = se_helm_set_dirichlet(d ? op2 : op, sd.data());
+        if (rc) die("se_helm_set_dirichlet", rc);
+    }
     assembled = true;
 }
 
 void HelmholtzEqn::Solve(std::string name, bool removeNullSpace) {
     (void)name;  // PETSc options prefix in the reference; options come from SE_* here
     if (!assembled) Assemble();
-    field_solve(op, solField, rhsField, removeNullSpace, &info);
+    const int nd = solField->nDofs;
+    if (nd == 1) {
+        field_solve(op, solField, rhsField, removeNullSpace, &info);
+        return;
+    }
+    // both dofs gathered before either is written (ShallowWaterEqnSC solves with rhs == sol)
+    std::vector<double> r, s;
+    gather(rhsField, r);
+    gather(solField, s);
+    se_solve_opts o = solve_opts(removeNullSpace);
+    info = se_solve_info();
+    for (int d = 0; d < nd; d++) {
+        std::vector<double> rd = column(r, nd, d), sd = column(s, nd, d);
+        se_solve_info one;
+        int rc = se_helm_solve(d ? op2 : op, rd.data(), sd.data(), &o, &one);
+        if (rc) die("se_helm_solve", rc);
+        if (one.status != SE_SOLVE_CONVERGED)
+            std::cerr << "libse_amd: solve of dof " << d << " ended with status " << one.status << " after "
+                      << one.iters << " iterations" << std::endl;
+        for (size_t v = 0; v < sd.size(); v++) s[v * nd + d] = sd[v];
+        // the pair's outcome: the worse status, all iterations, the larger residual and both times
+        if (one.status > info.status) info.status = one.status;
+        info.iters += one.iters;
+        info.outer_iters += one.outer_iters;
+        info.true_rel_residual = std::max(info.true_rel_residual, one.true_rel_residual);
+        info.seconds += one.seconds;
+    }
+    scatter(s, solField);
 }
 
 // ---- PoissonEqn (src/linsys/PoissonEqn.cpp:30-134) ----------------------------

@@ -3,8 +3,11 @@
 // Advector (se_amd/dropin) as the solver, the C oracle as the checker.  One
 // QuasiGeostrophicEqn-style sequence on an x-periodic channel
 // (src/qg/QuasiGeostrophicEqn.cpp:48-52,117-147 pattern): Helmholtz inversion
-// with Dirichlet walls, first- and second-order advection, a Poisson solve.
-// Prints "dropin ok" and exits 0 when every comparison passes.
+// with Dirichlet walls, first- and second-order advection, a Poisson solve;
+// then a NavierStokesEqn-style velocity solve (src/ns/NavierStokesEqn.cpp:90
+// pattern): HelmholtzEqn on a 2-dof field with per-dof walls (the reference's
+// 2-dof BCs) and the SVV operator.  Prints "dropin ok" and exits 0 when every
+// comparison passes.
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -113,6 +116,51 @@ int main() {
     std::printf("PoissonEqn: status %d, %d iterations (oracle %d)\n", pe->info.status, pe->info.iters, it);
     if (pe->info.status != 0) failures++;
     compare("PoissonEqn::Solve", values(chi), c0, 1e-8);
+
+    // velocity Helmholtz with SVV: u walls bottom/top, v walls on all sides,
+    // non-zero wall data, rhs a 2-dof field
+    {
+        int nE2[2] = {5, 4};
+        double lo2[2] = {0.0, 0.0}, hi2[2] = {2.0, 1.0};
+        bool per2[2] = {false, false};
+        Mesh* mesh2 = new Mesh("box", nE2, "legendre", N, lo2, hi2, per2);
+        bool bb[2] = {true, true}, bt[2] = {true, true}, bl[2] = {false, true}, br[2] = {false, true};
+        BCs* bcs2 = new BCs(bb, bt, bl, br, mesh2, 0);
+        Field* u = new Field("u", mesh2, 2, bcs2);
+        Field* f2 = new Field("f", mesh2, 2, NULL);
+        const int nV2 = mesh2->nVertsTotal;
+        for (int i = 0; i < nV2; i++) {
+            const double x = mesh2->verts[i][0], y = mesh2->verts[i][1];
+            u->vals[i][0] = y * (1.0 - y) + 0.5 * y;  // walls: 0 at the bottom, 0.5 at the top
+            u->vals[i][1] = 0.1 * x * y;
+            f2->vals[i][0] = std::sin(3.0 * x) * std::cos(2.0 * y) + 1.0;
+            f2->vals[i][1] = std::exp(-((x - 1.0) * (x - 1.0) + (y - 0.5) * (y - 0.5)) * 8.0);
+        }
+        const double dt2 = 0.01, nu2 = 1e-3;
+        const int cutoff = 3;
+        std::vector<double> u0 = values(u), fv = values(f2);
+        HelmholtzEqn* hv = new HelmholtzEqn(u, f2, 1.0, nu2, dt2, cutoff);
+        hv->Solve("vel_");
+        std::printf("HelmholtzEqn (2 dofs, SVV): status %d, %d iterations\n", hv->info.status, hv->info.iters);
+        if (hv->info.status != 0) failures++;
+        std::vector<double> got = values(u);
+        const int sides[2][4] = {{1, 1, 0, 0}, {1, 1, 1, 1}};
+        const int np2[2] = {0, 0};
+        for (int d = 0; d < 2; d++) {
+            orc_mesh* om2 = orc_mesh_create(nE2, N, lo2, hi2, np2, sides[d], &st);
+            orc_helm* ov = orc_helm_create(om2, 1.0, nu2 * dt2, 0);
+            orc_helm_set_svv(ov, dt2, cutoff);
+            std::vector<double> sd(nV2), rd(nV2), gd(nV2);
+            for (int i = 0; i < nV2; i++) sd[i] = u0[2 * (size_t)i + d], rd[i] = fv[2 * (size_t)i + d],
+                                        gd[i] = got[2 * (size_t)i + d];
+            orc_helm_assemble(ov, sd.data());
+            orc_helm_solve(ov, rd.data(), sd.data(), 1e-12, 100000, 0, &it, &rr);
+            compare(d ? "HelmholtzEqn SVV, dof 1" : "HelmholtzEqn SVV, dof 0", gd, sd, 1e-8);
+            orc_helm_destroy(ov);
+            orc_mesh_destroy(om2);
+        }
+        delete hv;
+    }
 
     delete he;
     delete pe;
