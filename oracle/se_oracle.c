@@ -17,9 +17,20 @@
  *                                algorithm is the build's definition (Jacobi-
  *                                PCG, x0 = 0, true residual stop); pinned by
  *                                direct dense solves and manufactured solutions.
- *   QG steps                  -- compositions of the above; the diagonal mass
- *                                solve of SolveQ is taken as exact division
- *                                (parity unpinned for that one ulp).
+ *   SVV element operator      -- the literal 2-D construction, pinned on its
+ *                                separable closed form (1-D modal filter from
+ *                                scipy's Jacobi polynomials), its null space
+ *                                and definiteness (tests/test_oracle_svv.py).
+ *   SolveQ + DisRHS /         -- pinned on closed forms: pointwise terms with
+ *   DissipationRHS               the mass cancelled, periodic copies averaged,
+ *                                the viscous term equal to the assembled
+ *                                stiffness for continuous gradients; the
+ *                                diagonal mass solve is taken as exact
+ *                                division (parity unpinned for that one ulp).
+ *   QG steps                  -- compositions of the above, pinned on exact
+ *                                solutions: Rossby waves (advection, beta, F)
+ *                                and damped steady modes (drag, relaxation,
+ *                                Ekman drag, viscosity, topography).
  */
 #include <math.h>
 #include <stdlib.h>
