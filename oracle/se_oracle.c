@@ -710,7 +710,8 @@ void orc_elem_svv(const orc_mesh* m, int el, double c, int mN, double* A) {
     double *phi = (double*)malloc(sizeof(double) * (size_t)n1 * n1), *GNix = (double*)malloc(sizeof(double) * 2 * nl);
     double *B = (double*)malloc(sz), *W = (double*)calloc((size_t)nl * nl, sizeof(double));
     double *Q = (double*)calloc((size_t)nl * nl, sizeof(double)), *Dx = (double*)malloc(sz);
-    double *Bt = (double*)malloc(sz), *BtW = (double*)malloc(sz), *BtWB = (double*)malloc(sz);
+    double *Bt = (double*)calloc((size_t)nl * nl, sizeof(double)), *BtW = (double*)malloc(sz);
+    double *BtWB = (double*)malloc(sz);
     double *BtWBinv = (double*)malloc(sz), *M = (double*)malloc(sz), *Minv = (double*)malloc(sz);
     double *T1 = (double*)malloc(sz), *F = (double*)malloc(sz), *FD = (double*)malloc(sz);
     double *FDt = (double*)malloc(sz), *FDtW = (double*)malloc(sz), *L = (double*)malloc(sz);
