@@ -54,6 +54,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
     p.add_argument("--probe-reps", type=int, default=50)
+    p.add_argument("--check-every", type=int, default=32,
+                   help="mixed-precision solves: CG iterations between host checks (one graph replay)")
     p.add_argument("--requests", type=int, default=32, help="qg2_stream: number of requests")
     p.add_argument("--budget", type=float, default=200e9, help="qg2_stream: server memory budget (bytes)")
     return p.parse_args()
@@ -180,7 +182,7 @@ def run_ensemble(a, prec_name):
     stream = torch.cuda.ExternalStream(ctx.stream)
 
     def step():
-        return b.pcg_dev(fp, xp, precision=prec, rtol=1e-10)
+        return b.pcg_dev(fp, xp, precision=prec, rtol=1e-10, check_every=a.check_every)
 
     for _ in range(a.warmup):
         step()
@@ -203,6 +205,7 @@ def run_ensemble(a, prec_name):
               "iters_min_median_max": [int(its.min()), int(np.median(its)), int(its.max())],
               "reliable_updates_max": max(i["outer_iters"] for i in infos), "all_converged": ok,
               "tuned_workgroups_per_cu": dict(zip(("direction", "update"), b.launch_shape())),
+              "check_every": a.check_every,
               "max_true_rel_residual": max(i["true_rel_residual"] for i in infos), "wall_s_timed": t_wall}
     if not a.no_fp64:
         # the plain FP64 GPU path on the same members, one after another
@@ -263,7 +266,7 @@ def run_single(a, prec_name):
     stream = torch.cuda.ExternalStream(ctx.stream)
 
     def solve(p=prec, x=d_x):
-        return h.pcg_dev(d_f.data_ptr(), x.data_ptr(), rtol=1e-10, precision=p, check_every=16)
+        return h.pcg_dev(d_f.data_ptr(), x.data_ptr(), rtol=1e-10, precision=p, check_every=a.check_every)
 
     for _ in range(a.warmup):
         solve()

@@ -153,7 +153,7 @@ typedef struct {
     int maxit;          /* iteration limit, inner iterations counted (default 100000) */
     int precision;      /* SE_PREC_* (default FP64) */
     int null_space;     /* 1: constant null space (periodic Poisson) */
-    int check_every;    /* iterations per host convergence check (default 16) */
+    int check_every;    /* iterations per host convergence check, one graph replay (default 32) */
     double inner_rtol;  /* mixed solves: reliable update once the recursive residual
                            fell by this factor since the last one (0 = 0.1) */
     int max_outer;      /* limit on FP64 reliable updates of a mixed solve (default 1000) */

@@ -176,7 +176,7 @@ def _d(x):
     return np.ascontiguousarray(x, dtype=np.float64)
 
 
-def solve_opts(rtol=1e-10, maxit=100000, precision=PREC_FP64, null_space=False, check_every=16,
+def solve_opts(rtol=1e-10, maxit=100000, precision=PREC_FP64, null_space=False, check_every=32,
                inner_rtol=0.0, max_outer=1000, history=None):
     o = SolveOpts()
     lib().se_solve_opts_default(C.byref(o))

@@ -451,7 +451,7 @@ extern "C" void se_solve_opts_default(se_solve_opts* o) {
     o->maxit = 100000;
     o->precision = SE_PREC_FP64;
     o->null_space = 0;
-    o->check_every = 16;
+    o->check_every = 32;
     o->inner_rtol = 0.0;
     o->max_outer = 1000;
     o->history = nullptr;
