@@ -341,14 +341,16 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
     constexpr int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
     int phase = b->tuned_dir ? -1 : 0;
     if (phase < 0) set_grids(b, b->tuned_dir - 1, b->tuned_upd - 1);
-    while (running > 0) {
+    for (int round = 0; running > 0; round++) {
         const int ci = phase / 2;
         if (phase >= 0 && phase % 2 == 0) {
             set_grids(b, cand[ci], cand[ci]);
             b->graph_key = -1;
         }
         if (phase >= 0 && phase % 2 == 1) SE_HIP(hipEventRecord(b->t0, s));
-        rc = chains<T>(b, k);
+        // tuning ends before the replay count grows (14 phases < 16 rounds)
+        const int reps = replays_for_round(round);
+        for (int r = 0; r < reps && !rc; r++) rc = chains<T>(b, k);
         if (rc) return rc;
         if (phase >= 0 && phase % 2 == 1) {
             SE_HIP(hipEventRecord(b->t1, s));
@@ -372,7 +374,7 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
         // members due for a reliable update
         std::vector<int> due;
         for (int i = 0; i < count; i++)
-            if (ctl[i].status == CG_RUNNING && ctl[i].due(b->h_states[i], k)) {
+            if (ctl[i].status == CG_RUNNING && ctl[i].due(b->h_states[i], k * reps)) {
                 ctl[i].before_update(b->h_states[i]);
                 due.push_back(i);
             }

@@ -377,11 +377,12 @@ int tcg_solve(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o,
     if (rc) return rc;
     SE_HIP(hipGetLastError());
     ctl.started(*hs);
-    while (ctl.status == CG_RUNNING) {
-        rc = tcg_batch<T>(op, M, k);
+    for (int round = 0; ctl.status == CG_RUNNING; round++) {
+        const int reps = replays_for_round(round);
+        for (int r = 0; r < reps && !rc; r++) rc = tcg_batch<T>(op, M, k);
         if (!rc) rc = pull();
         if (rc) return rc;
-        if (!ctl.due(*hs, k)) continue;
+        if (!ctl.due(*hs, k * reps)) continue;
         ctl.before_update(*hs);
         rc = reliable_update<T>(op, M);
         if (!rc) rc = pull();

@@ -10,6 +10,15 @@
 
 namespace se {
 
+// Graph replays per host check in round `round` of a solve (both the single
+// solve and the batched solver count rounds from the solve's start, so their
+// cadences, and with them their results, agree): one for the first 16 rounds
+// (the batched solver's launch-shape tuning times single replays there), then
+// two, then four -- a solve that needs many iterations is in its tail, where
+// few members remain and a host check costs more than the iterations it
+// would save.
+inline int replays_for_round(int round) { return round < 16 ? 1 : (round < 32 ? 2 : 4); }
+
 struct TcgControl {
     // settings
     double delta2 = 0.01;  // reliable update once rr <= delta^2 * rr_ref
