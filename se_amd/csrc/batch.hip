@@ -17,6 +17,7 @@
 // the fastest for the batch.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -77,10 +78,18 @@ struct se_batch {
     // autotuner for the direction and the update kernel, plus one (0: not yet)
     int tuned_dir = 0, tuned_upd = 0;
     double tune_best = 0.0;
+    // pinned staging of the host -> device list uploads: filled between two
+    // synchronisations of the stream, so no byte is reused while a copy of
+    // it can still be pending (pageable copies would stage and block)
+    char* h_arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
 
     void release() {
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
+        if (h_arena) (void)hipHostFree(h_arena);
+        h_arena = nullptr;
+        arena_cap = arena_used = 0;
         for (auto& g : groups) {
             if (g.h_nactive) (void)hipHostFree(g.h_nactive);
             g.h_nactive = nullptr;
@@ -100,11 +109,29 @@ struct se_batch {
 
 namespace {
 
+// the device copy of a host list, staged through the batch's pinned arena
 template <class V>
-int upload(DevBuf& d, const std::vector<V>& v, hipStream_t s) {
+int upload(se_batch* b, DevBuf& d, const std::vector<V>& v, hipStream_t s) {
     int rc = d.alloc(sizeof(V) * std::max<size_t>(v.size(), 1));
-    if (rc) return rc;
-    if (!v.empty()) SE_HIP(hipMemcpyAsync(d.ptr, v.data(), sizeof(V) * v.size(), hipMemcpyHostToDevice, s));
+    if (rc || v.empty()) return rc;
+    const size_t nb = sizeof(V) * v.size();
+    if (b->arena_used + nb > b->arena_cap) {
+        // the staged copies complete before the arena is reused or replaced
+        SE_HIP(hipStreamSynchronize(s));
+        b->arena_used = 0;
+        if (nb > b->arena_cap) {
+            if (b->h_arena) (void)hipHostFree(b->h_arena);
+            b->h_arena = nullptr;
+            b->arena_cap = 0;
+            const size_t cap = std::max<size_t>(nb, (size_t)4 << 20);
+            SE_HIP(hipHostMalloc((void**)&b->h_arena, cap, hipHostMallocDefault));
+            b->arena_cap = cap;
+        }
+    }
+    char* src = b->h_arena + b->arena_used;
+    std::memcpy(src, v.data(), nb);
+    b->arena_used += (nb + 255) & ~(size_t)255;
+    SE_HIP(hipMemcpyAsync(d.ptr, src, nb, hipMemcpyHostToDevice, s));
     return SE_OK;
 }
 
@@ -221,9 +248,8 @@ int compact(se_batch* b, const std::vector<TcgControl>& ctl, hipStream_t s) {
         for (const int2& it : g.tiles)
             if (ctl[g.members[it.x]].status == CG_RUNNING) g.h_active.push_back(it);
         *g.h_nactive = (int)g.h_active.size();
-        if (!g.h_active.empty())
-            SE_HIP(hipMemcpyAsync(g.d_active.ptr, g.h_active.data(), sizeof(int2) * g.h_active.size(),
-                                  hipMemcpyHostToDevice, s));
+        int rc = upload(b, g.d_active, g.h_active, s);
+        if (rc) return rc;
         SE_HIP(hipMemcpyAsync(g.d_nactive.ptr, g.h_nactive, sizeof(int), hipMemcpyHostToDevice, s));
     }
     return SE_OK;
@@ -247,6 +273,7 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
     const int k = std::max(1, o.check_every);
     std::vector<TcgControl> ctl(count);
     for (auto& t : ctl) t.configure(o);
+    b->arena_used = 0;  // the previous solve ended with a synchronisation
 
     // descriptors, work items, states
     for (auto& g : b->groups) {
@@ -265,8 +292,8 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
             A.out = op->q.as<double>();
             As.push_back(A);
         }
-        int rc = upload(g.d_members, Ms, s);
-        if (!rc) rc = upload(g.d_amembers, As, s);
+        int rc = upload(b, g.d_members, Ms, s);
+        if (!rc) rc = upload(b, g.d_amembers, As, s);
         if (rc) return rc;
     }
     for (int i = 0; i < count; i++) {
@@ -284,6 +311,7 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
     auto pull = [&]() -> int {
         SE_HIP(hipMemcpyAsync(b->h_states, b->d_states.ptr, sizeof(CgState) * count, hipMemcpyDeviceToHost, s));
         SE_HIP(hipStreamSynchronize(s));
+        b->arena_used = 0;  // every staged upload has completed
         return SE_OK;
     };
 
@@ -395,10 +423,10 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
                 for (int t = 0; t < apply_tiles22(m); t++) ua.push_back(make_int2(j, t));
             }
             if (ul.empty()) continue;
-            rc = upload(g.d_ut, ut, s);
-            if (!rc) rc = upload(g.d_uf, uf, s);
-            if (!rc) rc = upload(g.d_ua, ua, s);
-            if (!rc) rc = upload(g.d_ul, ul, s);
+            rc = upload(b, g.d_ut, ut, s);
+            if (!rc) rc = upload(b, g.d_uf, uf, s);
+            if (!rc) rc = upload(b, g.d_ua, ua, s);
+            if (!rc) rc = upload(b, g.d_ul, ul, s);
             if (rc) return rc;
             const TcgMember* Ms = g.d_members.as<TcgMember>();
             hipLaunchKernelGGL(k_tcg_flush_batch, dim3((int)uf.size()), dim3(TCG_NT), 0, s, Ms, g.d_uf.as<int2>(),
@@ -443,7 +471,7 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
                 if (pass == 1) ncont = (int)rt.size();
             }
             if (rt.empty()) continue;
-            rc = upload(g.d_rt, rt, s);
+            rc = upload(b, g.d_rt, rt, s);
             const int2* w = g.d_rt.as<int2>();
             const int nres = (int)rt.size() - ncont;
             if (!rc) rc = g_dir_apply<T, TCG_NORMAL>(g, ncont, w, ncont, s);
@@ -504,7 +532,7 @@ extern "C" int se_batch_create(se_ctx* ctx, int count, se_op* const* ops, se_bat
         if (hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&g.done, hipEventDisableTiming) != hipSuccess)
             rc = fail(SE_ERR_HIP, "se_batch_create: group stream");
-        if (!rc) rc = upload(g.d_tiles, g.tiles, ctx->stream);
+        if (!rc) rc = upload(b, g.d_tiles, g.tiles, ctx->stream);
         if (!rc) rc = g.d_active.alloc(sizeof(int2) * std::max<size_t>(g.tiles.size(), 1));
         if (!rc) rc = g.d_nactive.alloc(sizeof(int));
         if (!rc && hipHostMalloc((void**)&g.h_nactive, sizeof(int), hipHostMallocDefault) != hipSuccess)
