@@ -347,10 +347,10 @@ int qg2_solveq(se_qg2* q) {
 namespace se {
 
 size_t qg2_bytes(int n, int nV) {
-    // device fields (29 nV doubles: state, previous state, PVs, velocities,
-    // omega^, tau_e, work), SolveQ's unknowns and the two operators' work (an
-    // estimate for admission)
-    return (size_t)nV * 8 * 29 + (size_t)n * (2 * (8 * 10 + 4 * 4) + 8 * 5);
+    // device fields (28 nV doubles: state, previous state, PVs, work, omega^,
+    // tau_e -- 20 fields -- and four velocity pairs), SolveQ's unknowns and
+    // the two operators' work (an estimate for admission)
+    return (size_t)nV * 8 * 28 + (size_t)n * (2 * (8 * 10 + 4 * 4) + 8 * 5);
 }
 
 int qg2_passes(const se_qg2* q) { return qg2_dissipative(q) ? 4 : 2; }

@@ -75,7 +75,7 @@ def test_memory_budget_serialises_admission(gpu_ctx):
     # one 9x9 request's estimate fits, two do not
     nV = (9 * 6 + 1) ** 2
     n = 9 * 6 * (9 * 6 - 1)
-    one = 8 * 29 * nV + n * (2 * (8 * 10 + 4 * 4) + 8 * 5)
+    one = 8 * 28 * nV + n * (2 * (8 * 10 + 4 * 4) + 8 * 5)
     out, st = serve(reqs, memory_budget=1.5 * one)
     t = sorted((out[j][0]["t_admit"], out[j][0]["t_done"]) for j in range(3))
     for (a0, d0), (a1, d1) in zip(t, t[1:]):
