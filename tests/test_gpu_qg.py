@@ -134,12 +134,13 @@ def test_qg2_dissipative_steps_match_oracle(gpu_ctx, detj):
     q.close()
 
 
-@pytest.mark.parametrize("detj", [0, 1], ids=["literal", "physical"])
-def test_qg_dissipative_steps_match_oracle(gpu_ctx, detj):
+@pytest.mark.parametrize("detj,nEls,nsteps", [(0, 10, 3), (1, 10, 3), (0, 32, 2)],
+                         ids=["literal", "physical", "literal-32x32"])
+def test_qg_dissipative_steps_match_oracle(gpu_ctx, detj, nEls, nsteps):
     """QuasiGeostrophicEqn with viscosity, linear drag and topography (DisRHS,
     QuasiGeostrophicEqn.cpp:303-337): he1 with zero lifting, the dissipative
-    SolveQ, he2."""
-    cfg = W.qg_channel(nEls=10, N=6)
+    SolveQ, he2; also at the channel's full 32 x 32, p = 6 size."""
+    cfg = W.qg_channel(nEls=nEls, N=6)
     m = se_amd.Mesh(gpu_ctx, cfg["nEls"], cfg["N"], cfg["lo"], cfg["hi"], cfg["periodic"], cfg["bc"])
     om = O.OracleMesh(cfg["nEls"], cfg["N"], cfg["lo"], cfg["hi"], cfg["periodic"], cfg["bc"])
     X, Y = W.mesh_coords(cfg["nEls"], cfg["N"], cfg["lo"], cfg["hi"], m.gll()[0])
@@ -156,7 +157,7 @@ def test_qg_dissipative_steps_match_oracle(gpu_ctx, detj):
     q.set_state(psi, omega)
     dt = q.timestep(cfg["courant"])
     ref, prev = [psi, omega], None
-    for k in range(3):
+    for k in range(nsteps):
         info = q.step(dt, rtol=1e-10)
         assert info["status"] == 0
         s, it, *nxt = om.qg_step(cfg["F"], cfg["beta"], dt, *ref, detj_mode=detj, rtol=1e-10, prev=prev,
