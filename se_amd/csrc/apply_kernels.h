@@ -319,7 +319,7 @@ __device__ __forceinline__ void apply_tile(const OpParams& P, int tile, const do
 template <int N, int TX, int TY, bool DOT>
 __global__ __launch_bounds__(256) void k_apply(OpParams P, const double* __restrict__ in, double* __restrict__ out,
                                                 double* __restrict__ partials, const CgState* __restrict__ st) {
-    if (DOT && __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    if (DOT && cg_stopped(st)) return;
     apply_tile<N, TX, TY, DOT>(P, blockIdx.x, in, out, partials, gridDim.x);
 }
 

@@ -68,7 +68,7 @@ __global__ __launch_bounds__(kVB) void k_cg_dir(int mode, int n, double rtol, co
                                                 const double* __restrict__ r, double* __restrict__ p,
                                                 const double* __restrict__ part_b, int np_b,
                                                 double* __restrict__ hist, CgState* __restrict__ st) {
-    if (mode == DIR_NORMAL && __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    if (mode == DIR_NORMAL && cg_stopped(st)) return;
     double t[4];
     reduce_partials<4, kVB>(part_b, np_b, t);
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(kVB) void k_cg_update(int n, const double* __restri
                                                    const double* __restrict__ dinv, double* __restrict__ x,
                                                    double* __restrict__ r, const double* __restrict__ part_a, int np_a,
                                                    double* __restrict__ part_b, CgState* __restrict__ st) {
-    if (__hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    if (cg_stopped(st)) return;
     double t[1];
     reduce_partials<1, kVB>(part_a, np_a, t);
     const double pq = t[0];

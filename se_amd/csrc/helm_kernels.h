@@ -58,6 +58,18 @@ struct CgState {
 
 enum CgStatus { CG_RUNNING = -1, CG_PENDING = -2 };
 
+// The stop flag as a kernel sees it at its start: the value earlier kernels
+// (or a stream-ordered host copy) left, which the kernel boundary makes
+// visible to an ordinary load.  A write by the lead workgroup of the same
+// kernel may or may not be seen, and either is fine: every workgroup takes
+// the same decision from the same totals.  A plain read on purpose: read
+// before the kernel stores anything, it compiles to one scalar load per wave.
+// Any vector load of the one word (atomic at agent, workgroup or wavefront
+// scope alike) has every wave of hundreds of resident workgroups ask for
+// the same line, and those requests queue: about 12 us per launch of 1024
+// workgroups, a third of a p = 8 apply.
+__device__ __forceinline__ int cg_stopped(const CgState* st) { return st->stop; }
+
 // Sum over the 64 lanes of a wave, returned to every lane.  Inside each row of
 // 16 lanes a shift-and-add scan with DPP row shifts (lanes shifted in from
 // outside the row read 0) leaves the row total in its last lane; the four row

@@ -112,7 +112,7 @@ __global__ __launch_bounds__(NT) void k_sep_apply(SepParams S, const T* __restri
     const float sxI = S.sx[Ic], mxI = S.mx[Ic];
     // a stopped solve skips the work (checked after the loads are in flight;
     // the flag is the same for every thread of the workgroup)
-    if (DOT && __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    if (DOT && cg_stopped(st)) return;
 
 #pragma unroll
     for (int k = 0; k < RL; k++) {

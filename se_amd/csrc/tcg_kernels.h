@@ -193,7 +193,7 @@ struct TcgDir {
             // whole workgroup takes the same branch even if the flag changes meanwhile
             __shared__ int s_stop;
             if (threadIdx.x == 0)
-                s_stop = FIRST ? 0 : __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_stop = FIRST ? 0 : cg_stopped(st);
             const int null_space = st->null_space, iter = st->iter, maxit = st->maxit;
             const double rho_prev = st->rho_prev, tol2_in = st->tol2;
             double tbv[4];
@@ -437,7 +437,7 @@ struct TcgUpd {
         __device__ __forceinline__ bool run(const TcgMember& M, int tile, float& fa, float& fb, float& fmu) {
             CgState* __restrict__ st = M.st;
             __shared__ int s_stop;
-            if (threadIdx.x == 0) s_stop = __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (threadIdx.x == 0) s_stop = cg_stopped(st);
             const double rho = st->rho, beta = st->beta, mu = st->mu;
             double ta[1];
             pa.reduce(M.part_a(), M.tiles, ta);
