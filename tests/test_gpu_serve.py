@@ -43,10 +43,16 @@ def serve(reqs, order=None, gap=0.0, **kw):
     return out, st
 
 
-def test_served_result_same_alone_and_in_mix(gpu_ctx):
+PRECISIONS = [se_amd.PREC_FP32_IR, se_amd.PREC_BF16_IR, se_amd.PREC_FP64]
+
+
+@pytest.mark.parametrize("prec", PRECISIONS, ids=["fp32", "bf16", "fp64"])
+def test_served_result_same_alone_and_in_mix(gpu_ctx, prec):
+    """FP32 / BF16 inversions run batched, FP64 ones one after another (the
+    plain path); in each case the served result is the directly stepped one."""
     reqs = make_requests(gpu_ctx, [8, 10, 12, 9, 11], [3, 6, 2, 4, 5])
-    alone, _ = serve(reqs, order=[1])
-    mix, st = serve(reqs, order=[0, 2, 1, 3, 4], gap=0.002)
+    alone, _ = serve(reqs, order=[1], precision=prec)
+    mix, st = serve(reqs, order=[0, 2, 1, 3, 4], gap=0.002, precision=prec)
     assert st["completed"] == 5 and st["steps"] == sum(r["steps"] for r in reqs)
     for j in range(5):
         res, f = mix[j]
@@ -63,7 +69,7 @@ def test_served_result_same_alone_and_in_mix(gpu_ctx):
     q.set_state(*r["init"])
     dt = q.timestep(0.5)
     for _ in range(r["steps"]):
-        q.step(dt, rtol=1e-10, precision=se_amd.PREC_FP32_IR)
+        q.step(dt, rtol=1e-10, precision=prec)
     for a, b in zip(q.state(), fm):
         assert np.array_equal(a, b)
     q.close()
