@@ -153,7 +153,10 @@ typedef struct {
     int maxit;          /* iteration limit, inner iterations counted (default 100000) */
     int precision;      /* SE_PREC_* (default FP64) */
     int null_space;     /* 1: constant null space (periodic Poisson) */
-    int check_every;    /* iterations per host convergence check, one graph replay (default 32) */
+    int check_every;    /* iterations per graph replay (default 32).  FP64: a host convergence
+                           check after every replay.  Mixed precision (single and batched):
+                           after every replay for the first 16 checks, then after every 2nd,
+                           from the 32nd check on after every 4th */
     double inner_rtol;  /* mixed solves: reliable update once the recursive residual
                            fell by this factor since the last one (0 = 0.1) */
     int max_outer;      /* limit on FP64 reliable updates of a mixed solve (default 1000) */
