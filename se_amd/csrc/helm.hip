@@ -126,7 +126,26 @@ __global__ __launch_bounds__(kVB) void k_cg_dir(int mode, int n, double rtol, co
             st->beta = beta;
         }
     }
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    // unknowns in pairs, one 16-byte access per vector and pair (the operator's
+    // own buffers, 256-byte aligned); the odd last unknown goes to thread 0
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    const double2* __restrict__ d2 = reinterpret_cast<const double2*>(dinv);
+    const double2* __restrict__ r2 = reinterpret_cast<const double2*>(r);
+    double2* __restrict__ p2 = reinterpret_cast<double2*>(p);
+    for (int i = tid; i < (n >> 1); i += stride) {
+        const double2 dv = d2[i], rv = r2[i];
+        double2 pv;
+        pv.x = dv.x * rv.x - mu;
+        pv.y = dv.y * rv.y - mu;
+        if (mode != DIR_FIRST) {
+            const double2 po = p2[i];
+            pv.x += beta * po.x;
+            pv.y += beta * po.y;
+        }
+        p2[i] = pv;
+    }
+    if ((n & 1) && tid == 0) {
+        const int i = n - 1;
         double z = dinv[i] * r[i] - mu;
         p[i] = mode == DIR_FIRST ? z : z + beta * p[i];
     }
@@ -158,15 +177,39 @@ __global__ __launch_bounds__(kVB) void k_cg_update(int n, const double* __restri
         st->rho_prev = rho;
     }
     double rr = 0.0, rz = 0.0, sz = 0.0, sr = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        x[i] += alpha * p[i];
-        double ri = r[i] - alpha * q[i];
-        r[i] = ri;
-        double z = dinv[i] * ri;
+    auto acc = [&](double ri, double di) {
+        const double z = di * ri;
         rr += ri * ri;
         rz += ri * z;
         sz += z;
         sr += ri;
+    };
+    // unknowns in pairs, one 16-byte access per vector and pair (the operator's
+    // own buffers, 256-byte aligned); the odd last unknown goes to thread 0
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    const double2* __restrict__ p2 = reinterpret_cast<const double2*>(p);
+    const double2* __restrict__ q2 = reinterpret_cast<const double2*>(q);
+    const double2* __restrict__ d2 = reinterpret_cast<const double2*>(dinv);
+    double2* __restrict__ x2 = reinterpret_cast<double2*>(x);
+    double2* __restrict__ r2 = reinterpret_cast<double2*>(r);
+    for (int i = tid; i < (n >> 1); i += stride) {
+        const double2 pv = p2[i], qv = q2[i], dv = d2[i];
+        double2 xv = x2[i], rv = r2[i];
+        xv.x += alpha * pv.x;
+        xv.y += alpha * pv.y;
+        rv.x -= alpha * qv.x;
+        rv.y -= alpha * qv.y;
+        x2[i] = xv;
+        r2[i] = rv;
+        acc(rv.x, dv.x);
+        acc(rv.y, dv.y);
+    }
+    if ((n & 1) && tid == 0) {
+        const int i = n - 1;
+        x[i] += alpha * p[i];
+        const double ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        acc(ri, dinv[i]);
     }
     double v[4] = {rr, rz, sz, sr};
     block_store_partials<4, kVB>(v, part_b);
