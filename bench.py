@@ -387,7 +387,7 @@ def run_stream(a, prec_name):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+        print("bench.py measures one GPU only: se_amd has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
     if a.workload.startswith("ensemble"):
