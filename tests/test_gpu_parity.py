@@ -79,6 +79,9 @@ OP_CASES = [
     ("xper_top_6x5_p3", (6, 5), 3, (0, -2), (3, 2), (1, 0), (0, 1, 0, 0), 0.5, 1.0, 0),
     ("neumann_5x4_p2", (5, 4), 2, (0, 0), (1, 1), (0, 0), (0, 0, 0, 0), 4.0, 1.0, 1),
     ("odd_13x11_p4", (13, 11), 4, (0, 0), (1.3, 1.1), (1, 1), (0, 0, 0, 0), 1.0, 1.0, 0),
+    # an odd unknown count (165 x 149 = 24585) over several workgroups of the CG's vector
+    # kernels, which take unknowns in pairs and the last one alone
+    ("odd_n_41x37_p4", (41, 37), 4, (0, 0), (2.0, 1.8), (0, 0), (0, 0, 0, 0), 3.0, 1.0, 1),
 ]
 
 
