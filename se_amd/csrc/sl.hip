@@ -4,14 +4,16 @@
 // One thread per field node.  The coordinate arithmetic that decides the
 // departure element (RK2 midpoint, wrap/clamp, GetEl's modf and its 1e-12
 // edge rule, the local-coordinate map) follows the reference's operation order
-// with FP contraction disabled, so departure elements agree with the
-// reference bit for bit except where a coordinate lies within rounding of an
-// element edge.  P_N(x_j) is tabulated on the host with the reference's
-// evaluation; P_N'(x) is evaluated on the device with the same monomial sum.
+// with FP contraction disabled.  P_N(x_j) is tabulated on the host with the
+// reference's evaluation; P_N'(x) is evaluated on the device with the same
+// monomial sum, its powers bit-identical to the C library's pow
+// (glibc_pow.h), so interpolated values and departure elements agree with
+// the reference's bit for bit.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 
+#include "glibc_pow.h"
 #include "se_common.h"
 
 using namespace se;
@@ -73,31 +75,11 @@ __device__ __forceinline__ void wrap(const SLGeom& g, double& cx, double& cy) {
     if (cy > g.max1) cy = g.per1 ? g.min1 - g.max1 + cy : g.max1;
 }
 
-// x^k for a small non-negative integer k, correctly rounded: the running
-// product is carried as an unevaluated double-double (error-free products via
-// FMA) and rounded once.  glibc's pow, which the reference calls with integer
-// exponents, returns the correctly rounded power in all but ~0.1% of cases, so
-// this reproduces the reference's monomial terms bit for bit almost always.
-__device__ __forceinline__ double ipow_cr(double x, int k) {
-#pragma clang fp contract(off)
-    if (k == 0) return 1.0;
-    double hi = x, lo = 0.0;
-    for (int i = 1; i < k; i++) {
-        double p = hi * x;
-        double e = fma(hi, x, -p);
-        double l = fma(lo, x, e);
-        double s = p + l;
-        lo = l - (s - p);
-        hi = s;
-    }
-    return hi + lo;
-}
-
 // Jacobi::EvalDeriv (Jacobi.cpp:58-66) of P_N
 __device__ __forceinline__ double pn_deriv(const SLGeom& g, double x) {
 #pragma clang fp contract(off)
     double dp = 0.0;
-    for (int i = g.N; i >= 1; i -= 2) dp += g.pn[i] * i * ipow_cr(x, i - 1);
+    for (int i = g.N; i >= 1; i -= 2) dp += g.pn[i] * i * se_pow::pow_int(x, i - 1);
     return dp;
 }
 

@@ -1,8 +1,8 @@
 """Parity of the HIP path (through the C ABI) with the reference's golden vectors
 and the CPU oracle.
 
-Bars: bit-exact for numbering maps, vertex coordinates, GLL tables, GenVel and
-departure elements (off near-ties); <= 1e-12 relative for the FP64 operator
+Bars: bit-exact for numbering maps, vertex coordinates, GLL tables, GenVel,
+departure elements (every node) and semi-Lagrangian values; <= 1e-12 relative for the FP64 operator
 applies, lifting, RHS, diagonal and the first CG iterates; solutions of two
 independent CG runs agree to the accuracy the 1e-10 residual allows."""
 import math
@@ -22,17 +22,6 @@ CASES = mesh_cases()
 
 def rel(a, b):
     return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
-
-
-def near_tie(m, xy):
-    """Departure coordinates within rounding of an element edge, where GetEl's
-    1e-12 rule makes the element index depend on the last bits."""
-    out = np.zeros(len(xy), dtype=bool)
-    for d in range(2):
-        dx = (m.hi[d] - m.lo[d]) / m.nEls[d]
-        t = (xy[:, d] - m.lo[d]) / dx
-        out |= np.abs(t - np.rint(t)) < 1e-10
-    return out
 
 
 # ---- a1-a3: basis, coordinates, numbering --------------------------------------------
@@ -233,22 +222,22 @@ def test_solve_status_codes(gpu_ctx):
 
 
 # ---- a11-a16: semi-Lagrangian step ---------------------------------------------------------
-def sl_tolerance(om, pts, scale):
-    """Per-node bar for interpolated values: 1e-12 of the field scale, plus the
-    forward-error bound of the reference's cardinal-function formula
-    (Legendre.cpp:158-166), which evaluates P_N'(x) as a monomial sum and so
-    loses about eps*|c|/|P_N''| / delta when a coordinate lies a distance delta
-    (>= 1e-8, else IsRoot returns the delta function exactly) from a GLL line."""
-    delta = om.node_distance(pts)
-    return 1e-12 * scale + 1e-14 * scale / delta
-
-
-def report_sl(tag, got, ref, tol):
+def report_sl(tag, got, ref):
+    """Interpolated values: norm-wise relative difference <= 1e-12 and no node
+    off by more than 1e-12 of the field's maximum.  The device evaluates the
+    reference's formulas in the reference's operation order, with powers
+    bit-identical to the C library's pow, so the values are expected to be
+    bit-exact; the fraction that is is printed and must be 1."""
+    got, ref = np.asarray(got), np.asarray(ref)
     diff = np.abs(got - ref)
-    exact = np.mean(diff == 0.0)
-    print("%s: bit-exact %.4f, max diff %.2e, max diff/tol %.3f, nodes over 1e-12: %d"
-          % (tag, exact, diff.max(), np.max(diff / tol), int(np.sum(diff > 1e-12 * np.max(np.abs(ref))))))
-    assert np.all(diff <= tol)
+    scale = np.max(np.abs(ref))
+    exact = np.mean(got == ref)
+    over = int(np.sum(diff > 1e-12 * scale))
+    print("%s: bit-exact %.6f, max diff %.2e (%.2e of max), norm-wise rel %.2e, nodes over 1e-12 of max: %d"
+          % (tag, exact, diff.max(), diff.max() / scale, rel(got, ref), over))
+    assert rel(got, ref) <= 1e-12
+    assert over == 0
+    assert exact == 1.0
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -256,14 +245,11 @@ def test_advect_first_against_reference(gpu_ctx, name):
     d = load_mesh_case(name)
     m = se_amd.Mesh(gpu_ctx, *mesh_args(d))
     q, dep = m.advect(d["q"], d["vel"], float(d["dt"]))
-    tie = near_tie(m, d["adv1_xy"])
-    assert np.array_equal(dep[~tie], d["adv1_dep"][~tie])
-    om = O.OracleMesh(*mesh_args(d))
-    pts = om.sl_points(1, d["vel"], float(d["dt"]))
-    report_sl(name, q, d["adv1_q"], sl_tolerance(om, pts, np.max(np.abs(d["q"]))))
+    # departure elements of every node, ties at element edges and walls included
+    assert np.array_equal(dep, d["adv1_dep"]), np.flatnonzero(dep != d["adv1_dep"])[:20]
+    report_sl(name, q, d["adv1_q"])
     v2 = m.advect(d["vel"], d["vel"], float(d["dt"]), nDofs=2, want_dep=False).reshape(-1, 2)
-    tolv = sl_tolerance(om, pts, np.max(np.abs(d["vel"])))
-    report_sl(name + " vel", v2, d["adv1_vel"].reshape(-1, 2), tolv[:, None])
+    report_sl(name + " vel", v2, d["adv1_vel"].reshape(-1, 2))
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -271,10 +257,7 @@ def test_advect_second_against_reference(gpu_ctx, name):
     d = load_mesh_case(name)
     m = se_amd.Mesh(gpu_ctx, *mesh_args(d))
     q = m.advect2(d["q"], d["qprev"], d["vel"], d["velprev"], float(d["dt"]))
-    om = O.OracleMesh(*mesh_args(d))
-    pts = om.sl_points(2, d["vel"], float(d["dt"]), d["velprev"])
-    scale = 2.5 * max(np.max(np.abs(d["q"])), np.max(np.abs(d["qprev"])))
-    report_sl(name, q, d["adv2_q"], sl_tolerance(om, pts, scale))
+    report_sl(name, q, d["adv2_q"])
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -300,7 +283,5 @@ def test_qg_channel_advection_vs_oracle(gpu_ctx):
     assert dt == om.calc_timestep(vel.ravel(), cfg["courant"])
     qg, dep = m.advect(q, vel.ravel(), dt)
     qo, depo, xy = om.advect_first(q, 1, vel.ravel(), dt)
-    tie = near_tie(m, xy)
-    assert np.array_equal(dep[~tie], depo[~tie])
-    pts = om.sl_points(1, vel.ravel(), dt)
-    report_sl("qg_channel", qg, qo, sl_tolerance(om, pts, np.max(np.abs(q))))
+    assert np.array_equal(dep, depo), np.flatnonzero(dep != depo)[:20]
+    report_sl("qg_channel", qg, qo)
