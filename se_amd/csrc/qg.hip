@@ -483,6 +483,8 @@ extern "C" int se_qg2_set_dissipation(se_qg2* q, double E2, double r, double nu,
     SE_CHECK(std::isfinite(E2) && std::isfinite(r) && std::isfinite(nu), SE_ERR_ARG,
              "se_qg2_set_dissipation: non-finite coefficient");
     SE_HIP(hipSetDevice(q->mesh->ctx->device));
+    // the context's stream is non-blocking: a step still queued on it reads tau_e
+    SE_HIP(hipStreamSynchronize(q->mesh->ctx->stream));
     if (tau_e) SE_HIP(hipMemcpy(q->tau_e.ptr, tau_e, sizeof(double) * q->mesh->nV, hipMemcpyHostToDevice));
     q->E2 = E2;
     q->r = r;
@@ -504,6 +506,8 @@ extern "C" int se_qg2_set_state(se_qg2* q, const double* psi1, const double* om1
                                 const double* om2) {
     SE_CHECK(q && psi1 && om1 && psi2 && om2, SE_ERR_ARG, "se_qg2_set_state: NULL argument");
     SE_HIP(hipSetDevice(q->mesh->ctx->device));
+    // a step may still be writing the state on the (non-blocking) context stream
+    SE_HIP(hipStreamSynchronize(q->mesh->ctx->stream));
     const size_t b = sizeof(double) * q->mesh->nV;
     SE_HIP(hipMemcpy(q->psi1.ptr, psi1, b, hipMemcpyHostToDevice));
     SE_HIP(hipMemcpy(q->om1.ptr, om1, b, hipMemcpyHostToDevice));
@@ -612,6 +616,7 @@ extern "C" int se_qg_set_dissipation(se_qg* q, double nu, double r) {
 extern "C" int se_qg_set_topography(se_qg* q, const double* topo) {
     SE_CHECK(q, SE_ERR_ARG, "se_qg_set_topography: NULL argument");
     SE_HIP(hipSetDevice(q->mesh->ctx->device));
+    SE_HIP(hipStreamSynchronize(q->mesh->ctx->stream));  // a queued step reads the topography
     if (topo) SE_HIP(hipMemcpy(q->topo.ptr, topo, sizeof(double) * q->mesh->nV, hipMemcpyHostToDevice));
     q->has_topo = topo != nullptr;
     return SE_OK;
@@ -620,6 +625,8 @@ extern "C" int se_qg_set_topography(se_qg* q, const double* topo) {
 extern "C" int se_qg_set_state(se_qg* q, const double* psi, const double* omega) {
     SE_CHECK(q && psi && omega, SE_ERR_ARG, "se_qg_set_state: NULL argument");
     SE_HIP(hipSetDevice(q->mesh->ctx->device));
+    // a step may still be writing the state on the (non-blocking) context stream
+    SE_HIP(hipStreamSynchronize(q->mesh->ctx->stream));
     const size_t b = sizeof(double) * q->mesh->nV;
     SE_HIP(hipMemcpy(q->psi.ptr, psi, b, hipMemcpyHostToDevice));
     SE_HIP(hipMemcpy(q->om.ptr, omega, b, hipMemcpyHostToDevice));

@@ -172,3 +172,41 @@ def test_qg_dissipative_steps_match_oracle(gpu_ctx, detj, nEls, nsteps):
         print("step", k, "rel errors psi omega", ["%.2e" % e for e in errs])
         assert max(errs) <= 1e-9, errs
     q.close()
+
+
+def test_set_state_right_after_step_is_ordered(gpu_ctx):
+    """set_state (and set_dissipation / set_topography) called right after a
+    step replaces the state that step leaves behind, even though the step's
+    last kernels may still be queued: the result equals a fresh model's."""
+    req, m, _, st = setup(gpu_ctx, m_els=12, seed=11)
+    a = se_amd.QG2(m, req["F1"], req["F2"], req["beta"], 0)
+    a.set_state(*st)
+    dt = a.timestep(0.5)
+    a.step(dt, rtol=1e-10)
+    a.set_state(*st)
+    a.step(dt, rtol=1e-10)
+    b = se_amd.QG2(m, req["F1"], req["F2"], req["beta"], 0)
+    b.set_state(*st)
+    b.step(dt, rtol=1e-10)
+    assert all(np.array_equal(x, y) for x, y in zip(a.state(), b.state()))
+    a.close()
+    b.close()
+
+    cfg = W.qg_channel(nEls=10, N=6)
+    mc = se_amd.Mesh(gpu_ctx, cfg["nEls"], cfg["N"], cfg["lo"], cfg["hi"], cfg["periodic"], cfg["bc"])
+    X, Y = W.mesh_coords(cfg["nEls"], cfg["N"], cfg["lo"], cfg["hi"], mc.gll()[0])
+    q0 = W.gaussian_vortices(X, Y, cfg["vortices"], W.Rng(cfg["seed"]), cfg["lo"], cfg["hi"])
+    c = se_amd.QG(mc, cfg["F"], cfg["beta"], 0)
+    c.set_state(np.zeros(mc.nV), q0)
+    dtc = 1e-3
+    c.step(dtc, rtol=1e-10)
+    c.set_topography(np.zeros(mc.nV))
+    c.set_state(np.zeros(mc.nV), q0)
+    c.step(dtc, rtol=1e-10)
+    d = se_amd.QG(mc, cfg["F"], cfg["beta"], 0)
+    d.set_topography(np.zeros(mc.nV))
+    d.set_state(np.zeros(mc.nV), q0)
+    d.step(dtc, rtol=1e-10)
+    assert all(np.array_equal(x, y) for x, y in zip(c.state(), d.state()))
+    c.close()
+    d.close()
