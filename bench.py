@@ -36,6 +36,8 @@ sys.path.insert(0, ROOT)
 METRIC = "Helmholtz CG GDOF/s at 1e-10 rel. residual vs FP64; p50/p99 latency under load"
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector
+# MI355X dense matrix-core peaks by operand type (TFLOP/s)
+MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32_ir": 157.3, "bf16_ir": 2500.0}
 DTYPE = {"fp64": "f64", "fp32_ir": "f32+f64", "bf16_ir": "bf16+f64"}
 
 
@@ -47,6 +49,8 @@ def parse():
     p.add_argument("--workload", default="ensemble_512")
     p.add_argument("--precision", default=None, choices=["fp64", "fp32_ir", "bf16_ir"],
                    help="default: fp32_ir for ensemble_512, fp64 for helmholtz_*")
+    p.add_argument("--pc", default="default", choices=["default", "jacobi", "fdm"],
+                   help="preconditioner: default = FDM for fp32_ir / bf16_ir, Jacobi for fp64")
     p.add_argument("--members", type=int, default=512, help="ensemble size")
     p.add_argument("--detj", default="physical", choices=["physical", "literal"])
     p.add_argument("--cpu-iters", type=int, default=12, help="CG iterations per CPU baseline sample")
@@ -154,13 +158,37 @@ def mixed_roofline(kernel, pr):
                               "achieved": upd, "frac": upd / HBM_PEAK_GBS}}
 
 
+def pc_code(a, prec_name):
+    import se_amd
+    if a.pc == "jacobi":
+        return se_amd.PC_JACOBI
+    if a.pc == "fdm" or prec_name != "fp64":
+        return se_amd.PC_FDM
+    return se_amd.PC_JACOBI
+
+
+def fdm_roofline(pr, prec_name, kernel):
+    """pr: per-pass us / flops of the FDM transforms and us / bytes of the FP64 apply."""
+    us, fl = sum(pr["us_pass"]), sum(pr["flops_pass"])
+    ach = fl / (us * 1e-6) / 1e12
+    peak = MFMA_PEAK_TFLOPS[prec_name]
+    apply_gbs = pr["bytes_apply"] / (pr["us_apply"] * 1e-6) / 1e9
+    return {"kernel": kernel, "bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+            "frac": ach / peak, "traffic": None, "us_per_launch": us / 4.0, "flops_per_launch": fl / 4.0,
+            "passes": [{"us": u, "flops": f, "tflops": f / (u * 1e-6) / 1e12} for u, f in zip(pr["us_pass"],
+                                                                                             pr["flops_pass"])],
+            "fp64_apply": {"us_per_launch": pr["us_apply"], "bytes_per_launch": pr["bytes_apply"],
+                           "achieved_gbs": apply_gbs, "frac_hbm": apply_gbs / HBM_PEAK_GBS}}
+
+
 def run_ensemble(a, prec_name):
     import numpy as np
     import torch
 
     import se_amd
     from se_amd import workloads as W
-    prec = {"fp32_ir": se_amd.PREC_FP32_IR, "bf16_ir": se_amd.PREC_BF16_IR}[prec_name]
+    prec = {"fp64": se_amd.PREC_FP64, "fp32_ir": se_amd.PREC_FP32_IR, "bf16_ir": se_amd.PREC_BF16_IR}[prec_name]
+    pc = pc_code(a, prec_name)
     cfgs = W.ensemble_512(count=a.members, detj_mode=1 if a.detj == "physical" else 0)
     ctx = se_amd.Context(0)
     dev = torch.device("cuda:0")
@@ -181,8 +209,8 @@ def run_ensemble(a, prec_name):
     fp, xp = [f.data_ptr() for f in fs], [x.data_ptr() for x in xs]
     stream = torch.cuda.ExternalStream(ctx.stream)
 
-    def step():
-        return b.pcg_dev(fp, xp, precision=prec, rtol=1e-10, check_every=a.check_every)
+    def step(p=prec, c=pc, x=xp):
+        return b.pcg_dev(fp, x, precision=p, rtol=1e-10, check_every=a.check_every, preconditioner=c)
 
     for _ in range(a.warmup):
         step()
@@ -191,22 +219,29 @@ def run_ensemble(a, prec_name):
     ok = all(i["status"] == 0 and i["true_rel_residual"] <= 1e-10 for r in res for i in r)
     its = np.array([i["iters"] for i in infos])
     value = ntot / (ms * 1e-3) / 1e9
-    # dominant kernel: the direction + separable operator kernel of the
-    # largest order group (the probe continues the last solve's recurrences)
-    groups = b.probe(prec, reps=a.probe_reps)
-    gmax = max(groups, key=lambda g: g["bytes_dir"])
-    T = "float" if prec == se_amd.PREC_FP32_IR else "bf16"
-    roof = mixed_roofline("k_tcg_dir_apply_batch<%d, %s, NORMAL>" % (gmax["order"], T), gmax)
-    roof["groups"] = groups
-    roof["traffic"] = pmc_traffic(roof, "k_tcg_dir_apply_batch__ensemble_512_%s_pmc.md" % prec_name.split("_")[0])
+    T = {se_amd.PREC_FP64: "double", se_amd.PREC_FP32_IR: "float", se_amd.PREC_BF16_IR: "bf16"}[prec]
+    if pc == se_amd.PC_FDM:
+        # dominant kernel: the FDM transforms on the matrix cores (four passes
+        # over all members, timed back to back after the last solve)
+        roof = fdm_roofline(b.fdm_probe(reps=a.probe_reps), prec_name, "k_fdm_gemm<%s>" % T)
+    else:
+        # dominant kernel: the direction + separable operator kernel of the
+        # largest order group (the probe continues the last solve's recurrences)
+        groups = b.probe(prec, reps=a.probe_reps)
+        gmax = max(groups, key=lambda g: g["bytes_dir"])
+        roof = mixed_roofline("k_tcg_dir_apply_batch<%d, %s, NORMAL>" % (gmax["order"], T), gmax)
+        roof["groups"] = groups
+        roof["traffic"] = pmc_traffic(roof, "k_tcg_dir_apply_batch__ensemble_512_%s_pmc.md" % prec_name.split("_")[0])
     roof["copy_yardstick"] = copy_yardstick(ctx)
     config = {"workload": "ensemble_512" if a.members == 512 else "ensemble_%d" % a.members,
-              "members": len(hs), "unknowns": ntot, "precision": prec_name, "detj": a.detj, "rtol": 1e-10,
+              "members": len(hs), "unknowns": ntot, "precision": prec_name,
+              "preconditioner": "fdm" if pc == se_amd.PC_FDM else "jacobi", "detj": a.detj, "rtol": 1e-10,
               "iters_min_median_max": [int(its.min()), int(np.median(its)), int(its.max())],
               "reliable_updates_max": max(i["outer_iters"] for i in infos), "all_converged": ok,
-              "tuned_workgroups_per_cu": dict(zip(("direction", "update"), b.launch_shape())),
-              "check_every": a.check_every,
               "max_true_rel_residual": max(i["true_rel_residual"] for i in infos), "wall_s_timed": t_wall}
+    if pc != se_amd.PC_FDM:
+        config.update({"tuned_workgroups_per_cu": dict(zip(("direction", "update"), b.launch_shape())),
+                       "check_every": a.check_every})
     if not a.no_fp64:
         # the plain FP64 GPU path on the same members, one after another
         xs64 = [torch.zeros_like(x) for x in xs]
@@ -219,6 +254,24 @@ def run_ensemble(a, prec_name):
         config.update({"fp64_gpu_value": v64, "fp64_gpu_ms": ms64, "speedup_vs_fp64_gpu": value / v64,
                        "fp64_all_converged": all(i["status"] == 0 for i in res64[-1]),
                        "max_rel_diff_vs_fp64_solution": diff})
+        # the other legs of the same batch, to separate the gains: batching
+        # (FP64 FDM batched), precision (this line's FDM at another
+        # precision) and the preconditioner (the inner Jacobi-PCG mixed solve)
+        legs = {}
+        for lab, p2, c2 in (("fp64_fdm_batched", se_amd.PREC_FP64, se_amd.PC_FDM),
+                            ("fp32_fdm_batched", se_amd.PREC_FP32_IR, se_amd.PC_FDM),
+                            ("bf16_fdm_batched", se_amd.PREC_BF16_IR, se_amd.PC_FDM),
+                            ("fp32_jacobi_batched", se_amd.PREC_FP32_IR, se_amd.PC_JACOBI)):
+            if p2 == prec and c2 == pc:
+                continue
+            xs2 = [torch.zeros_like(x) for x in xs]
+            xp2 = [x.data_ptr() for x in xs2]
+            step(p2, c2, xp2)  # warm (setup of that precision)
+            ms2, _, r2 = timed(ctx, stream, 1, lambda: step(p2, c2, xp2))
+            legs[lab] = {"ms": ms2, "value": ntot / (ms2 * 1e-3) / 1e9,
+                         "all_converged": all(i["status"] == 0 and i["true_rel_residual"] <= 1e-10 for i in r2[-1]),
+                         "iters_max": max(i["iters"] for i in r2[-1])}
+        config["other_legs"] = legs
     cpu = None
     if not a.no_cpu_baseline:
         cpu = ensemble_cpu(a, cfgs, hs, fs, infos)
@@ -251,6 +304,7 @@ def run_single(a, prec_name):
     N = int(parts[2][1:]) if len(parts) > 2 else 8
     cfg = W.helmholtz_128(nEls=ne, N=N, detj_mode=1 if a.detj == "physical" else 0)
     prec = {"fp64": se_amd.PREC_FP64, "fp32_ir": se_amd.PREC_FP32_IR, "bf16_ir": se_amd.PREC_BF16_IR}[prec_name]
+    pc = pc_code(a, prec_name)
     ctx = se_amd.Context(0)
     m = se_amd.Mesh(ctx, cfg["nEls"], cfg["N"], cfg["lo"], cfg["hi"], cfg["periodic"], cfg["bc"])
     h = se_amd.Helm(m, cfg["c_mass"], cfg["c_lap"], cfg["detj_mode"])
@@ -265,8 +319,9 @@ def run_single(a, prec_name):
     ctx.synchronize()
     stream = torch.cuda.ExternalStream(ctx.stream)
 
-    def solve(p=prec, x=d_x):
-        return h.pcg_dev(d_f.data_ptr(), x.data_ptr(), rtol=1e-10, precision=p, check_every=a.check_every)
+    def solve(p=prec, x=d_x, c=pc):
+        return h.pcg_dev(d_f.data_ptr(), x.data_ptr(), rtol=1e-10, precision=p, check_every=a.check_every,
+                         preconditioner=c)
 
     for _ in range(a.warmup):
         solve()
@@ -275,10 +330,14 @@ def run_single(a, prec_name):
     ok = all(i["status"] == 0 and i["true_rel_residual"] <= 1e-10 for i in res)
     value = m.n / (ms * 1e-3) / 1e9
     config = {"workload": cfg["workload"], "nEls": list(cfg["nEls"]), "N": cfg["N"], "lambda": cfg["c_mass"],
-              "n": m.n, "precision": prec_name, "detj": a.detj, "rtol": 1e-10, "iters": info["iters"],
+              "n": m.n, "precision": prec_name, "preconditioner": "fdm" if pc == se_amd.PC_FDM else "jacobi",
+              "detj": a.detj, "rtol": 1e-10, "iters": info["iters"],
               "outer_iters": info["outer_iters"], "true_rel_residual": info["true_rel_residual"],
               "all_converged": ok, "n_iter_per_s": m.n * info["iters"] / (ms * 1e-3), "wall_s_timed": t_wall}
-    if prec == se_amd.PREC_FP64:
+    T = {se_amd.PREC_FP64: "double", se_amd.PREC_FP32_IR: "float", se_amd.PREC_BF16_IR: "bf16"}[prec]
+    if pc == se_amd.PC_FDM:
+        roof = fdm_roofline(h.fdm_probe(reps=a.probe_reps), prec_name, "k_fdm_gemm<%s>" % T)
+    elif prec == se_amd.PREC_FP64:
         # dominant kernel: the FP64 operator apply, timed on its stream
         for _ in range(5):
             h.apply_dev(d_x.data_ptr(), d_y.data_ptr())
@@ -295,19 +354,32 @@ def run_single(a, prec_name):
                 "fp64_tflops": fl / t_apply / 1e12, "fp64_frac": fl / t_apply / 1e12 / FP64_PEAK_TFLOPS}
     else:
         pr = h.probe(prec, reps=a.probe_reps)
-        T = "float" if prec == se_amd.PREC_FP32_IR else "bf16"
         roof = mixed_roofline("k_tcg_dir_apply<%d, %s, NORMAL>" % (m.N, T), pr)
-    if cfg["workload"] == "helmholtz_128":
+    if cfg["workload"] == "helmholtz_128" and pc != se_amd.PC_FDM:
         roof["traffic"] = pmc_traffic(roof, "%s__helmholtz_128_%s_pmc.md" % (
             roof["kernel"].split("<")[0], prec_name.split("_")[0]))
     roof["copy_yardstick"] = copy_yardstick(ctx)
-    if prec != se_amd.PREC_FP64 and not a.no_fp64:
+    if not (prec == se_amd.PREC_FP64 and pc == se_amd.PC_JACOBI) and not a.no_fp64:
+        # the plain FP64 GPU path: the reference's algorithm (Jacobi-PCG) in FP64
         d_x64 = torch.empty_like(d_x)
-        ms64, _, r64 = timed(ctx, stream, 1, lambda: solve(se_amd.PREC_FP64, d_x64))
+        ms64, _, r64 = timed(ctx, stream, 1, lambda: solve(se_amd.PREC_FP64, d_x64, se_amd.PC_JACOBI))
         config.update({"fp64_gpu_value": m.n / (ms64 * 1e-3) / 1e9, "fp64_gpu_ms": ms64,
                        "speedup_vs_fp64_gpu": ms64 / ms,
                        "max_rel_diff_vs_fp64_solution":
                            float(torch.linalg.norm(d_x - d_x64) / torch.linalg.norm(d_x64))})
+        legs = {}
+        for lab, p2, c2 in (("fp64_fdm", se_amd.PREC_FP64, se_amd.PC_FDM),
+                            ("fp32_fdm", se_amd.PREC_FP32_IR, se_amd.PC_FDM),
+                            ("bf16_fdm", se_amd.PREC_BF16_IR, se_amd.PC_FDM),
+                            ("fp32_jacobi", se_amd.PREC_FP32_IR, se_amd.PC_JACOBI)):
+            if p2 == prec and c2 == pc:
+                continue
+            d_x2 = torch.empty_like(d_x)
+            solve(p2, d_x2, c2)
+            ms2, _, r2 = timed(ctx, stream, 1, lambda: solve(p2, d_x2, c2))
+            legs[lab] = {"ms": ms2, "value": m.n / (ms2 * 1e-3) / 1e9, "iters": r2[-1]["iters"],
+                         "converged": r2[-1]["status"] == 0 and r2[-1]["true_rel_residual"] <= 1e-10}
+        config["other_legs"] = legs
     cpu = None
     if not a.no_cpu_baseline:
         s = cpu_sample(cfg, d_f.cpu().numpy(), info["iters"], a.cpu_iters)
@@ -392,8 +464,9 @@ def main():
         sys.exit(2)
     if a.workload.startswith("ensemble"):
         prec = a.precision or "fp32_ir"
-        if prec == "fp64":
-            raise SystemExit("ensemble_512 runs batched mixed precision; its FP64 comparison is part of the line")
+        if prec == "fp64" and a.pc != "fdm":
+            raise SystemExit("batched FP64 solves take the FDM preconditioner (--pc fdm); the plain FP64 "
+                             "Jacobi path is timed member by member inside every ensemble line")
         out = run_ensemble(a, prec)
     elif a.workload.startswith("qg2_stream"):
         out = run_stream(a, a.precision or "fp32_ir")

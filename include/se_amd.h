@@ -41,10 +41,22 @@ enum {
  * src/base/Operator.cpp:72]; PHYSICAL keeps the true detJ. */
 enum { SE_DETJ_LITERAL = 0, SE_DETJ_PHYSICAL = 1 };
 
-/* Solve precision.  FP64: plain FP64 Jacobi-PCG.  FP32_IR / BF16_IR: inner
- * Jacobi-PCG in FP32 (resp. BF16 storage, FP32 arithmetic on MFMA) inside an
- * FP64 iterative-refinement loop whose residual, norms and update are FP64. */
+/* Solve precision.  FP64: everything in FP64.  FP32_IR / BF16_IR: the
+ * low-precision work (the inner Jacobi-PCG iteration, or the
+ * fast-diagonalisation preconditioner's transforms) in FP32, resp. with BF16
+ * operands (Jacobi: BF16 search direction, FP32 VALU arithmetic; FDM: BF16
+ * inputs to the matrix cores with FP32 accumulation), inside an FP64 loop
+ * whose residual, norms and update are FP64. */
 enum { SE_PREC_FP64 = 0, SE_PREC_FP32_IR = 1, SE_PREC_BF16_IR = 2 };
+
+/* Preconditioner.  JACOBI: the reference's (PETSc -pc_type jacobi; FP64) or
+ * the inner Jacobi-PCG with FP64 reliable updates (FP32_IR / BF16_IR).
+ * FDM: fast diagonalisation of the operator's separable form (dense 1-D
+ * eigenvector transforms on the matrix cores, in the solve's precision)
+ * inside FP64 CG -- a near-exact inverse on the reference's uniform meshes,
+ * so the CG needs a handful of iterations.  DEFAULT: JACOBI for FP64, FDM for
+ * FP32_IR / BF16_IR. */
+enum { SE_PC_DEFAULT = 0, SE_PC_JACOBI = 1, SE_PC_FDM = 2 };
 
 /* Solve status */
 enum {
@@ -161,6 +173,7 @@ typedef struct {
                            fell by this factor since the last one (0 = 0.1) */
     int max_outer;      /* limit on FP64 reliable updates of a mixed solve (default 1000) */
     double* history;    /* optional host array, maxit+1 recursive relative residuals (FP64 only) */
+    int preconditioner; /* SE_PC_* (default SE_PC_DEFAULT) */
 } se_solve_opts;
 
 typedef struct {
@@ -179,6 +192,21 @@ void se_solve_opts_default(se_solve_opts* opts);
  * (SE_PREC_BF16_IR), arithmetic and output are FP32.  sign is +-1 so that
  * sign*A has a positive diagonal.  Host arrays of n values. */
 int se_helm_apply_inner(se_op* op, int precision, const float* x, float* y, int* sign);
+/* Fast-diagonalisation preconditioner (no counterpart in the reference,
+ * whose PETSc solve takes -pc_type jacobi): z ~= (sign*A)^-1 r through the
+ * eigenvectors of the operator's separable form, the four transforms on the
+ * matrix cores in `precision` (SE_PREC_FP64 / FP32_IR / BF16_IR); the null
+ * mode of a singular operator is left out of z.  Host arrays of n values.
+ * Uses the operator's solver work vectors. */
+int se_helm_apply_fdm(se_op* op, int precision, const double* r, double* z);
+/* Measurement hook (no counterpart in the reference): after an FDM solve of
+ * op, times `reps` back-to-back launches of each of the four transform
+ * passes (the first with its residual pack) and of the FP64 operator apply,
+ * with HIP events on the context stream: us_pass[4], flops_pass[4] (2 M N K
+ * per member, padded sizes), us_apply, bytes_apply (16 B per unknown, 4 B
+ * per field node). */
+int se_helm_fdm_probe(se_op* op, int reps, double* us_pass, double* flops_pass, double* us_apply,
+                      double* bytes_apply);
 /* Measurement hook (no counterpart in the reference): after a mixed-precision
  * solve of op, times `reps` back-to-back launches of the inner iteration's
  * direction+operator kernel and update kernel with HIP events on the context
@@ -200,10 +228,12 @@ int se_helm_solve_dev(se_op* op, const double* d_rhs_field, double* d_sol_field,
                       se_solve_info* info);
 
 /* ---- batched solves ----------------------------------------------------------
- * Many independent mixed-precision solves (SE_PREC_FP32_IR / SE_PREC_BF16_IR)
- * at once.  No counterpart in the reference, which solves one HelmholtzEqn at
- * a time (KSPSolve, src/linsys/HelmholtzEqn.cpp:120-131); each member's result
- * is bitwise the one se_helm_pcg_dev gives for it alone with the same options.
+ * Many independent solves at once: mixed precision (SE_PREC_FP32_IR /
+ * SE_PREC_BF16_IR) with either preconditioner, or SE_PREC_FP64 with the FDM
+ * preconditioner.  No counterpart in the reference, which solves one
+ * HelmholtzEqn at a time (KSPSolve, src/linsys/HelmholtzEqn.cpp:120-131); each
+ * member's result is bitwise the one se_helm_pcg_dev gives for it alone with
+ * the same options.
  * Members are operators created on the same context (any size, order 2..8 and
  * boundary conditions); an operator may belong to one batch at a time. */
 typedef struct se_batch se_batch;
@@ -229,6 +259,10 @@ int se_batch_pcg(se_batch* batch, const double* const* f, double* const* x, cons
  * left as the probe's (the next solve starts afresh). */
 int se_batch_probe(se_batch* batch, int precision, int reps, int max_groups, int* orders, double* us_dir,
                    double* us_upd, double* bytes_dir, double* bytes_upd);
+/* The FDM counterpart of se_batch_probe, over all members at once (as
+ * se_helm_fdm_probe); needs an FDM solve of the batch first. */
+int se_batch_fdm_probe(se_batch* batch, int reps, double* us_pass, double* flops_pass, double* us_apply,
+                       double* bytes_apply);
 
 /* ---- semi-Lagrangian advection ---------------------------------------------
  * [ref: Advector::Advect / FirstOrder src/base/Advector.cpp:108-120,181-201,

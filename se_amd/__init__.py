@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libse_amd.so")
 
 DETJ_LITERAL, DETJ_PHYSICAL = 0, 1
 PREC_FP64, PREC_FP32_IR, PREC_BF16_IR = 0, 1, 2
+PC_DEFAULT, PC_JACOBI, PC_FDM = 0, 1, 2
 SOLVE_CONVERGED, SOLVE_MAXIT, SOLVE_BREAKDOWN, SOLVE_NONFINITE, SOLVE_STAGNATED = 0, 1, 2, 3, 4
 
 _D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -32,7 +33,7 @@ class SEError(RuntimeError):
 class SolveOpts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("maxit", C.c_int), ("precision", C.c_int), ("null_space", C.c_int),
                 ("check_every", C.c_int), ("inner_rtol", C.c_double), ("max_outer", C.c_int),
-                ("history", C.c_void_p)]
+                ("history", C.c_void_p), ("preconditioner", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -107,6 +108,9 @@ def lib():
             "se_helm_apply": (i, [vp, _D, _D]),
             "se_helm_apply_dev": (i, [vp, vp, vp]),
             "se_helm_apply_inner": (i, [vp, i, _F, _F, C.POINTER(i)]),
+            "se_helm_apply_fdm": (i, [vp, i, _D, _D]),
+            "se_helm_fdm_probe": (i, [vp, i, _D, _D, C.POINTER(d), C.POINTER(d)]),
+            "se_batch_fdm_probe": (i, [vp, i, _D, _D, C.POINTER(d), C.POINTER(d)]),
             "se_helm_inv_diag": (i, [vp, _D]),
             "se_helm_set_dirichlet": (i, [vp, _D]),
             "se_helm_set_dirichlet_dev": (i, [vp, vp]),
@@ -177,13 +181,21 @@ def _d(x):
 
 
 def solve_opts(rtol=1e-10, maxit=100000, precision=PREC_FP64, null_space=False, check_every=32,
-               inner_rtol=0.0, max_outer=1000, history=None):
+               inner_rtol=0.0, max_outer=1000, history=None, preconditioner=PC_DEFAULT):
     o = SolveOpts()
     lib().se_solve_opts_default(C.byref(o))
     o.rtol, o.maxit, o.precision, o.null_space = rtol, maxit, precision, int(bool(null_space))
+    o.preconditioner = preconditioner
     o.check_every, o.inner_rtol, o.max_outer = check_every, inner_rtol, max_outer
     o.history = history.ctypes.data if history is not None else None
     return o
+
+
+def _fdm_probe(fn, h, reps):
+    us, fl = np.zeros(4), np.zeros(4)
+    ua, ba = C.c_double(), C.c_double()
+    check(fn(h, reps, us, fl, C.byref(ua), C.byref(ba)), "fdm probe")
+    return dict(us_pass=us.tolist(), flops_pass=fl.tolist(), us_apply=ua.value, bytes_apply=ba.value)
 
 
 IO_LIB_PATH = os.path.join(os.path.dirname(LIB_PATH), "libse_amd_io.so")
@@ -440,6 +452,17 @@ class Helm:
         check(lib().se_helm_apply_inner(self.h, precision, xf, y, C.byref(sg)), "se_helm_apply_inner")
         return y, sg.value
 
+    def apply_fdm(self, r, precision=PREC_FP64):
+        """Fast-diagonalisation preconditioner: z ~= (sign*A)^-1 r (separable
+        form; the null mode of a singular operator is left out)."""
+        z = np.zeros(self.mesh.n)
+        check(lib().se_helm_apply_fdm(self.h, precision, _d(r), z), "se_helm_apply_fdm")
+        return z
+
+    def fdm_probe(self, reps=20):
+        """After an FDM solve: per-pass us and flops of the transforms, us and bytes of the FP64 apply."""
+        return _fdm_probe(lib().se_helm_fdm_probe, self.h, reps)
+
     def probe(self, precision=PREC_FP32_IR, reps=50):
         """After a mixed solve: dict(us_dir, us_upd, bytes_dir, bytes_upd) of the inner kernels."""
         v = [C.c_double() for _ in range(4)]
@@ -544,6 +567,10 @@ class Batch:
         infos = self._infos()
         check(lib().se_batch_pcg_dev(self.h, fp, xp, C.byref(o), infos), "se_batch_pcg_dev")
         return [infos[i].as_dict() for i in range(len(self.helms))]
+
+    def fdm_probe(self, reps=20):
+        """After an FDM solve of the batch: as Helm.fdm_probe over all members."""
+        return _fdm_probe(lib().se_batch_fdm_probe, self.h, reps)
 
     def probe(self, precision=PREC_FP32_IR, reps=50, max_groups=8):
         """Per order group: dict(order, us_dir, us_upd, bytes_dir, bytes_upd) (after a solve)."""

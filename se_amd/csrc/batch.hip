@@ -78,6 +78,8 @@ struct se_batch {
     // autotuner for the direction and the update kernel, plus one (0: not yet)
     int tuned_dir = 0, tuned_upd = 0;
     double tune_best = 0.0;
+    // FDM-preconditioned solves of the members (fdm.hip), built on first use
+    FdmPlan* fdm = nullptr;
     // pinned staging of the host -> device list uploads: filled between two
     // synchronisations of the stream, so no byte is reused while a copy of
     // it can still be pending (pageable copies would stage and block)
@@ -85,6 +87,8 @@ struct se_batch {
     size_t arena_cap = 0, arena_used = 0;
 
     void release() {
+        if (fdm) fdm_plan_destroy(fdm);
+        fdm = nullptr;
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
         if (h_arena) (void)hipHostFree(h_arena);
@@ -644,12 +648,24 @@ extern "C" int se_batch_pcg_dev(se_batch* b, const double* const* d_f, double* c
     se_solve_opts o;
     se_solve_opts_default(&o);
     if (opts) o = *opts;
-    SE_CHECK(o.precision == SE_PREC_FP32_IR || o.precision == SE_PREC_BF16_IR, SE_ERR_UNSUPPORTED,
-             "se_batch_pcg_dev: the batched solver is mixed precision (SE_PREC_FP32_IR or SE_PREC_BF16_IR)");
+    SE_CHECK(o.preconditioner >= SE_PC_DEFAULT && o.preconditioner <= SE_PC_FDM, SE_ERR_ARG,
+             "se_batch_pcg_dev: bad preconditioner");
+    const bool fdm = solve_pc(o) == SE_PC_FDM;
+    SE_CHECK(o.precision == SE_PREC_FP32_IR || o.precision == SE_PREC_BF16_IR || (fdm && o.precision == SE_PREC_FP64),
+             SE_ERR_UNSUPPORTED,
+             "se_batch_pcg_dev: batched Jacobi solves are mixed precision (SE_PREC_FP32_IR or SE_PREC_BF16_IR); "
+             "FP64 batches take the FDM preconditioner");
     for (size_t i = 0; i < b->ops.size(); i++)
         SE_CHECK(d_f[i] && d_x[i], SE_ERR_ARG, "se_batch_pcg_dev: NULL member vector");
     SE_HIP(hipSetDevice(b->ctx->device));
     if (b->ops.empty()) return SE_OK;
+    if (fdm) {
+        if (!b->fdm) {
+            int rc = fdm_plan_create(b->ctx, (int)b->ops.size(), b->ops.data(), &b->fdm);
+            if (rc) return rc;
+        }
+        return fdm_plan_solve(b->fdm, d_f, d_x, o, infos);
+    }
     if (o.precision == SE_PREC_FP32_IR) return batch_solve<float>(b, d_f, d_x, o, infos);
     return batch_solve<__hip_bfloat16>(b, d_f, d_x, o, infos);
 }
@@ -677,4 +693,13 @@ extern "C" int se_batch_pcg(se_batch* b, const double* const* f, double* const* 
         SE_HIP(hipMemcpyAsync(x[i], b->ops[i]->x.ptr, sizeof(double) * b->ops[i]->mesh->n, hipMemcpyDeviceToHost, s));
     SE_HIP(hipStreamSynchronize(s));
     return SE_OK;
+}
+
+extern "C" int se_batch_fdm_probe(se_batch* b, int reps, double* us_pass, double* flops_pass, double* us_apply,
+                                  double* bytes_apply) {
+    SE_CHECK(b && us_pass && flops_pass && us_apply && bytes_apply && reps > 0, SE_ERR_ARG,
+             "se_batch_fdm_probe: bad argument");
+    SE_CHECK(b->fdm != nullptr, SE_ERR_STATE, "se_batch_fdm_probe: run an FDM solve of this batch first");
+    SE_HIP(hipSetDevice(b->ctx->device));
+    return fdm_plan_probe(b->fdm, reps, us_pass, flops_pass, us_apply, bytes_apply);
 }

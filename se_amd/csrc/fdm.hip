@@ -1,0 +1,1377 @@
+// fdm.hip -- fast-diagonalisation (FDM) preconditioner on the matrix cores
+// and the FDM-preconditioned conjugate-gradient solves.
+//
+// The operator in its separable form (mixed.hip, assemble_line) has the
+// near-exact inverse (Vy (x) Vx) D^-1 (Vy (x) Vx)^T (fdm_host.h).  On the
+// unknown grid R (rows J, columns I) applying it is four dense transforms:
+//   pass 1  T1t = (R Vx)^T        x-transform, stored transposed
+//   pass 2  T2  = (T1t Vy) ./ D   y-transform, scaled by 1/D
+//   pass 3  T3t = (T2 Vy^T)^T     y back-transform, stored transposed
+//   pass 4  Z   =  T3t Vx^T       x back-transform
+// each an "NT" GEMM C = A Bt^T whose operands both have the summed index
+// contiguous, on 64 x 64 output tiles (four waves of 32 x 32) with MFMA:
+// v_mfma_f64_16x16x4f64 (FP64), v_mfma_f32_32x32x2f32 (FP32) or
+// v_mfma_f32_32x32x16_bf16 (BF16 operands, FP32 accumulation).  Matrices are
+// padded to multiples of 64 with zeros; pass 1 reads the FP64 residual with
+// its true extent and pass 4 writes the FP64 result with it, together with
+// per-tile partial sums of r.z, sum z and sum r.
+//
+// The solver around it is FP64 CG (x0 = 0, true-residual stop, constant null
+// space by Euclidean-mean projection, as the Jacobi-PCG of helm.hip) whose
+// preconditioner is the FDM in the solve's precision.  It runs members of a
+// batch in lockstep, one launch per phase over (member, item) work lists; a
+// single solve is a batch of one, so batched results are the single ones bit
+// for bit.  The exact FP64 operator apply is k_apply_batch (2 x 2 element
+// tiles); every reduction is per item with consumer-side fixed-order sums.
+#include <hip/hip_bf16.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "apply_kernels.h"
+#include "fdm_host.h"
+#include "se_common.h"
+#include "se_op.h"
+
+namespace se {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+enum { FDM_OUT_T = 0, FDM_OUT_SCALE = 1, FDM_OUT_FINAL = 2 };
+enum { FDM_F64 = 0, FDM_F32 = 1, FDM_BF16 = 2 };  // element types of the transforms
+
+// One GEMM of one member: C = A Bt^T.  A: M x K (lda), Bt: N x K (ldb), both
+// row-major, element type T.  Pass 1's A is the residual packed to T and
+// padded by k_fdm_pack from r (true extent arows x acols, leading dimension
+// acols).
+struct GemmTask {
+    const void* A;
+    const void* Bt;
+    void* C;
+    const double* r;                       // FINAL: residual (for the partials)
+    double* part;                          // FINAL: [3][tiles] partials r.z, sum z, sum r
+    const double *lx, *ly, *ex, *ey;       // SCALE: D = lx[row] + ly[col] + ex[row] ey[col]
+    double dtol;                           // SCALE: D below this is a null mode (1/D := 0)
+    int M, N, K, lda, ldb, ldc;
+    int arows, acols;
+    int crows, ccols;                      // valid extent (SCALE: modes; FINAL: the unknown grid)
+    int mode, tilesN;
+};
+
+// K chunk per LDS stage (KC) and padded LDS row length (KP, a multiple of 16
+// bytes).  Every padded dimension is a multiple of 64, so KC must divide 64.
+template <typename T> struct GemmCfg;
+template <> struct GemmCfg<double> {
+    static constexpr int KC = 32, KP = 34;
+    using S = double;  // staging type of the transposed store
+};
+template <> struct GemmCfg<float> {
+    static constexpr int KC = 64, KP = 68;
+    using S = float;
+};
+template <> struct GemmCfg<__bf16> {
+    static constexpr int KC = 64, KP = 72;
+    using S = float;
+};
+static_assert(64 % GemmCfg<double>::KC == 0 && 64 % GemmCfg<float>::KC == 0 && 64 % GemmCfg<__bf16>::KC == 0,
+              "a K chunk must divide the padding granule");
+template <typename T>
+constexpr int gemm_lds_bytes() {
+    return (int)std::max(2 * 64 * GemmCfg<T>::KP * sizeof(T), 64 * 65 * sizeof(typename GemmCfg<T>::S));
+}
+
+template <typename T> struct Acc;
+template <> struct Acc<float> {
+    f32x16 v;
+};
+template <> struct Acc<__bf16> {
+    f32x16 v;
+};
+template <> struct Acc<double> {
+    f64x4 c[2][2];
+};
+
+template <typename T> __device__ __forceinline__ T to_t(double v);
+template <> __device__ __forceinline__ double to_t<double>(double v) { return v; }
+template <> __device__ __forceinline__ float to_t<float>(double v) { return (float)v; }
+template <> __device__ __forceinline__ __bf16 to_t<__bf16>(double v) { return (__bf16)(float)v; }
+template <typename T> __device__ __forceinline__ double from_t(T v) { return (double)v; }
+template <> __device__ __forceinline__ double from_t<__bf16>(__bf16 v) { return (double)(float)v; }
+
+// one K chunk of the wave's 32 x 32 block.  Lanes of half (l >> 5) (FP64:
+// quarter (l >> 4)) supply a contiguous run of the chunk's K values, so each
+// lane reads its operands from LDS with wide loads; both operands use the
+// same K assignment, so the sum covers the chunk once.
+template <typename T>
+__device__ __forceinline__ void mma_chunk(const T* As, const T* Bs, int wr, int wc, int l, Acc<T>& acc);
+
+template <>
+__device__ __forceinline__ void mma_chunk<float>(const float* As, const float* Bs, int wr, int wc, int l,
+                                                 Acc<float>& acc) {
+    constexpr int KC = GemmCfg<float>::KC, KP = GemmCfg<float>::KP;
+    const float* a = As + (wr * 32 + (l & 31)) * KP + (l >> 5) * (KC / 2);
+    const float* b = Bs + (wc * 32 + (l & 31)) * KP + (l >> 5) * (KC / 2);
+#pragma unroll
+    for (int m = 0; m < KC / 2; m++) acc.v = __builtin_amdgcn_mfma_f32_32x32x2f32(a[m], b[m], acc.v, 0, 0, 0);
+}
+
+template <>
+__device__ __forceinline__ void mma_chunk<__bf16>(const __bf16* As, const __bf16* Bs, int wr, int wc, int l,
+                                                  Acc<__bf16>& acc) {
+    constexpr int KC = GemmCfg<__bf16>::KC, KP = GemmCfg<__bf16>::KP;
+    const bf16x8* a = (const bf16x8*)(As + (wr * 32 + (l & 31)) * KP + (l >> 5) * (KC / 2));
+    const bf16x8* b = (const bf16x8*)(Bs + (wc * 32 + (l & 31)) * KP + (l >> 5) * (KC / 2));
+#pragma unroll
+    for (int m = 0; m < KC / 16; m++) acc.v = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[m], b[m], acc.v, 0, 0, 0);
+}
+
+template <>
+__device__ __forceinline__ void mma_chunk<double>(const double* As, const double* Bs, int wr, int wc, int l,
+                                                  Acc<double>& acc) {
+    constexpr int KC = GemmCfg<double>::KC, KP = GemmCfg<double>::KP;
+    const double* a0 = As + (wr * 32 + (l & 15)) * KP + (l >> 4) * (KC / 4);
+    const double* a1 = a0 + 16 * KP;
+    const double* b0 = Bs + (wc * 32 + (l & 15)) * KP + (l >> 4) * (KC / 4);
+    const double* b1 = b0 + 16 * KP;
+#pragma unroll
+    for (int m = 0; m < KC / 4; m++) {
+        acc.c[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[m], b0[m], acc.c[0][0], 0, 0, 0);
+        acc.c[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[m], b1[m], acc.c[0][1], 0, 0, 0);
+        acc.c[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[m], b0[m], acc.c[1][0], 0, 0, 0);
+        acc.c[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[m], b1[m], acc.c[1][1], 0, 0, 0);
+    }
+}
+
+// every accumulator element with its (row, column) in the 64 x 64 tile:
+// 32 x 32 blocks hold rows (v & 3) + 4 (l >> 5) + 8 (v >> 2), column l & 31;
+// 16 x 16 FP64 blocks rows (l >> 4) + 4 v, column l & 15 (both measured with
+// a one-wave product of random operands)
+template <typename T, typename F>
+__device__ __forceinline__ void for_each_acc(const Acc<T>& acc, int wr, int wc, int l, F&& f) {
+    if constexpr (std::is_same<T, double>::value) {
+#pragma unroll
+        for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+            for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    f(wr * 32 + 16 * bi + (l >> 4) + 4 * v, wc * 32 + 16 * bj + (l & 15), acc.c[bi][bj][v]);
+    } else {
+#pragma unroll
+        for (int v = 0; v < 16; v++)
+            f(wr * 32 + (v & 3) + 4 * (l >> 5) + 8 * (v >> 2), wc * 32 + (l & 31), (double)acc.v[v]);
+    }
+}
+
+// One operand chunk (64 rows x KC) in registers: 16-byte vectors, VPR per
+// row segment, NV per thread.  Loaded one chunk ahead of the MFMA work that
+// uses the LDS copy of the previous one.
+template <typename T>
+struct OperandRegs {
+    static constexpr int KC = GemmCfg<T>::KC, KP = GemmCfg<T>::KP;
+    static constexpr int VEC = 16 / (int)sizeof(T), VPR = KC / VEC, NV = 64 * VPR / 256;
+    static_assert(KC % VEC == 0 && (64 * VPR) % 256 == 0 && 256 % VPR == 0, "chunk / thread shape");
+    uint4 v[NV];
+    __device__ __forceinline__ void load(const T* __restrict__ src, int ld, int r0, int k0) {
+        const int tid = threadIdx.x;
+#pragma unroll
+        for (int s = 0; s < NV; s++) {
+            const int row = tid / VPR + (256 / VPR) * s, cv = tid % VPR;
+            v[s] = *(const uint4*)(src + (size_t)(r0 + row) * ld + k0 + cv * VEC);
+        }
+    }
+    __device__ __forceinline__ void store(T* S) const {
+        const int tid = threadIdx.x;
+#pragma unroll
+        for (int s = 0; s < NV; s++) {
+            const int row = tid / VPR + (256 / VPR) * s, cv = tid % VPR;
+            *(uint4*)(S + row * KP + cv * VEC) = v[s];
+        }
+    }
+};
+template <typename T>
+__device__ __forceinline__ void gemm_tile(const GemmTask& t, int tile, char* lds) {
+    constexpr int KC = GemmCfg<T>::KC, KP = GemmCfg<T>::KP;
+    using S = typename GemmCfg<T>::S;
+    T* As = (T*)lds;
+    T* Bs = As + 64 * KP;
+    const int tid = threadIdx.x, l = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wr = wv >> 1, wc = wv & 1;
+    const int tm = tile / t.tilesN, tn = tile - tm * t.tilesN;
+    const int m0 = tm * 64, n0 = tn * 64;
+    Acc<T> acc;
+    if constexpr (std::is_same<T, double>::value) {
+        for (int bi = 0; bi < 2; bi++)
+            for (int bj = 0; bj < 2; bj++) acc.c[bi][bj] = f64x4{0.0, 0.0, 0.0, 0.0};
+    } else {
+        for (int v = 0; v < 16; v++) acc.v[v] = 0.0f;
+    }
+    const T* __restrict__ A = (const T*)t.A;
+    const T* __restrict__ Bt = (const T*)t.Bt;
+    OperandRegs<T> ra, rb;
+    ra.load(A, t.lda, m0, 0);
+    rb.load(Bt, t.ldb, n0, 0);
+    for (int k0 = 0; k0 < t.K; k0 += KC) {
+        __syncthreads();  // the previous chunk (or item) is done with the tiles
+        ra.store(As);
+        rb.store(Bs);
+        __syncthreads();
+        if (k0 + KC < t.K) {  // next chunk's loads in flight during this chunk's MFMAs
+            ra.load(A, t.lda, m0, k0 + KC);
+            rb.load(Bt, t.ldb, n0, k0 + KC);
+        }
+        mma_chunk<T>(As, Bs, wr, wc, l, acc);
+    }
+    if (t.mode == FDM_OUT_T) {
+        S* Cs = (S*)lds;
+        __syncthreads();  // done with the operand tiles
+        for_each_acc<T>(acc, wr, wc, l, [&](int row, int col, double v) { Cs[col * 65 + row] = (S)v; });
+        __syncthreads();
+        T* __restrict__ C = (T*)t.C;
+        for (int e = tid; e < 64 * 64; e += 256) {
+            const int cr = e >> 6, cc = e & 63;
+            C[(size_t)(n0 + cr) * t.ldc + m0 + cc] = to_t<T>((double)Cs[cr * 65 + cc]);
+        }
+    } else if (t.mode == FDM_OUT_SCALE) {
+        T* __restrict__ C = (T*)t.C;
+        for_each_acc<T>(acc, wr, wc, l, [&](int row, int col, double v) {
+            const int gr = m0 + row, gc = n0 + col;
+            double s = 0.0;
+            if (gr < t.crows && gc < t.ccols) {
+                const double D = t.lx[gr] + t.ly[gc] + t.ex[gr] * t.ey[gc];
+                s = D > t.dtol ? 1.0 / D : 0.0;
+            }
+            C[(size_t)gr * t.ldc + gc] = to_t<T>(v * s);
+        });
+    } else {  // FDM_OUT_FINAL: FP64 z on the unknown grid, partials of r.z, sum z, sum r
+        double* __restrict__ Z = (double*)t.C;
+        const double* __restrict__ R = t.r;
+        double rz = 0.0, sz = 0.0, sr = 0.0;
+        for_each_acc<T>(acc, wr, wc, l, [&](int row, int col, double v) {
+            const int gr = m0 + row, gc = n0 + col;
+            if (gr < t.crows && gc < t.ccols) {
+                const size_t i = (size_t)gr * t.ldc + gc;
+                Z[i] = v;
+                const double ri = R[i];
+                rz += ri * v;
+                sz += v;
+                sr += ri;
+            }
+        });
+        const double pv[3] = {rz, sz, sr};
+        block_store_partials_at<3, 256>(pv, t.part, tile, (t.M / 64) * t.tilesN);
+    }
+}
+
+// pass `pass` of the members' transforms; work items (member, tile), or
+// tiles of member 0 when work is null
+template <typename T>
+__global__ __launch_bounds__(256) void k_fdm_gemm(const GemmTask* __restrict__ tasks, int pass,
+                                                  const int2* __restrict__ work, int nwork) {
+    __shared__ __attribute__((aligned(16))) char lds[gemm_lds_bytes<T>()];
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const int2 it = work ? work[w] : make_int2(0, w);
+        const GemmTask t = tasks[it.x * 4 + pass];
+        gemm_tile<T>(t, it.y, lds);
+    }
+}
+
+// pass 1's A operand: the FP64 residual r (nU1 x nU0) rounded to T into the
+// padded nP1 x nP0 array, zeros outside; one 64 x 64 block per item
+template <typename T>
+__global__ __launch_bounds__(256) void k_fdm_pack(const GemmTask* __restrict__ tasks, const int2* __restrict__ work,
+                                                  int nwork) {
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const int2 it = work ? work[w] : make_int2(0, w);
+        const GemmTask& t = tasks[it.x * 4];
+        const int tilesN = t.K / 64;  // pass 1: K = nP0 columns
+        const int tm = it.y / tilesN, tn = it.y - tm * tilesN;
+        T* __restrict__ P = (T*)t.A;
+        const double* __restrict__ r = t.r;
+        for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+            const int gr = tm * 64 + (e >> 6), gc = tn * 64 + (e & 63);
+            const double v = (gr < t.arows && gc < t.acols) ? r[(size_t)gr * t.acols + gc] : 0.0;
+            P[(size_t)gr * t.lda + gc] = to_t<T>(v);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CG vector kernels over (member, chunk) items
+// ---------------------------------------------------------------------------
+constexpr int FB_NT = 256;
+constexpr int FB_CH = 4096;  // unknowns per chunk
+
+struct FbMember {
+    const double* f;
+    double *b, *x, *r, *p, *q, *z;
+    double* pv;  // [nch]      r.r after start / update / restart
+    double* pz;  // [3][ntz]   FDM partials r.z, sum z, sum r
+    double* pa;  // [nch]      p.q
+    double* pt;  // [2][nch]   sum t, sum t^2 of the true residual
+    double* ps;  // [nch]      sum of sgn f
+    CgState* st;
+    int n, nch, ntz, null_space;
+    double sgn, rtol;
+};
+
+__device__ __forceinline__ void chunk_range(const FbMember& M, int ch, int& i0, int& i1) {
+    i0 = ch * FB_CH;
+    i1 = min(M.n, i0 + FB_CH);
+}
+
+__global__ __launch_bounds__(FB_NT) void k_fb_sum(const FbMember* __restrict__ Ms, const int2* __restrict__ work,
+                                                  int nwork) {
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        __syncthreads();  // the previous item's reduction is done with its LDS
+        const int2 it = work[w];
+        const FbMember& M = Ms[it.x];
+        int i0, i1;
+        chunk_range(M, it.y, i0, i1);
+        double s = 0.0;
+        for (int i = i0 + threadIdx.x; i < i1; i += FB_NT) s += M.sgn * M.f[i];
+        const double v[1] = {s};
+        block_store_partials_at<1, FB_NT>(v, M.ps, it.y, M.nch);
+    }
+}
+
+// b = sgn f - mean (null space), x = 0, r = b; partials of r.r
+__global__ __launch_bounds__(FB_NT) void k_fb_start(const FbMember* __restrict__ Ms, const int2* __restrict__ work,
+                                                    int nwork) {
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const int2 it = work[w];
+        const FbMember& M = Ms[it.x];
+        __syncthreads();
+        double mu = 0.0;
+        if (M.null_space) {
+            double t[1];
+            reduce_partials<1, FB_NT>(M.ps, M.nch, t);
+            mu = t[0] / M.n;
+        }
+        int i0, i1;
+        chunk_range(M, it.y, i0, i1);
+        double rr = 0.0;
+        for (int i = i0 + threadIdx.x; i < i1; i += FB_NT) {
+            const double bi = M.sgn * M.f[i] - mu;
+            M.b[i] = bi;
+            M.r[i] = bi;
+            M.x[i] = 0.0;
+            rr += bi * bi;
+        }
+        const double v[1] = {rr};
+        block_store_partials_at<1, FB_NT>(v, M.pv, it.y, M.nch);
+    }
+}
+
+enum { FB_FIRST = 0, FB_NORMAL = 1, FB_RESTART = 2 };
+
+// totals of r.r and of the FDM partials -> convergence, beta;
+// p = (z - mean z) + beta p.  The member's chunk 0 writes its state.
+__global__ __launch_bounds__(FB_NT) void k_fb_dir(const FbMember* __restrict__ Ms, const int2* __restrict__ work,
+                                                  int nwork, int mode) {
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const int2 it = work[w];
+        const FbMember& M = Ms[it.x];
+        CgState* __restrict__ st = M.st;
+        __syncthreads();
+        __shared__ int s_stop;
+        if (threadIdx.x == 0) s_stop = mode == FB_NORMAL ? cg_stopped(st) : 0;
+        double tr[1], tz[3];
+        reduce_partials<1, FB_NT>(M.pv, M.nch, tr);  // (its barriers publish s_stop)
+        if (s_stop) continue;
+        reduce_partials<3, FB_NT>(M.pz, M.ntz, tz);
+        const double rr = tr[0];
+        const bool finite = (rr - rr == 0.0) && (tz[0] - tz[0] == 0.0);
+        const double mu = M.null_space ? tz[1] / M.n : 0.0;
+        const double rho = tz[0] - mu * tz[2];
+        const bool lead = it.y == 0 && threadIdx.x == 0;
+        double beta = 0.0;
+        bool go = true;
+        if (mode == FB_FIRST) {
+            if (lead) {
+                st->bb = rr;
+                st->tol2 = M.rtol * M.rtol * rr;
+                st->rr = rr;
+                st->rho = rho;
+                st->mu = mu;
+                st->iter = 0;
+                st->stop = (rr == 0.0 || !finite) ? 1 : 0;
+                st->status = rr == 0.0 ? 0 : (finite ? CG_RUNNING : 3);
+            }
+            go = rr != 0.0 && finite;
+        } else if (mode == FB_NORMAL) {
+            const bool conv = finite && rr <= st->tol2;
+            if (lead) {
+                const int itn = st->iter + 1;
+                const bool over = itn >= st->maxit;
+                st->iter = itn;
+                st->rr = rr;
+                if (!finite) {
+                    st->stop = 1;
+                    st->status = 3;
+                } else if (conv) {
+                    st->stop = 1;
+                    st->status = CG_PENDING;
+                } else if (over) {
+                    st->stop = 1;
+                    st->status = 1;
+                } else {
+                    st->rho = rho;
+                    st->mu = mu;
+                    st->beta = rho / st->rho_prev;
+                }
+            }
+            // (the iteration limit is the lead's decision alone: other workgroups
+            // may already see its new iteration count)
+            go = finite && !conv;
+            beta = rho / st->rho_prev;
+        } else {  // restart from the true residual: p = z - mean z
+            if (lead) {
+                st->rr = rr;
+                st->rho = rho;
+                st->mu = mu;
+                st->beta = 0.0;
+            }
+        }
+        if (!go) continue;
+        int i0, i1;
+        chunk_range(M, it.y, i0, i1);
+        for (int i = i0 + threadIdx.x; i < i1; i += FB_NT) {
+            const double zi = M.z[i] - mu;
+            M.p[i] = mode == FB_NORMAL ? zi + beta * M.p[i] : zi;
+        }
+    }
+}
+
+// partials of p.q
+__global__ __launch_bounds__(FB_NT) void k_fb_dot(const FbMember* __restrict__ Ms, const int2* __restrict__ work,
+                                                  int nwork) {
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        __syncthreads();  // the previous item's reduction is done with its LDS
+        const int2 it = work[w];
+        const FbMember& M = Ms[it.x];
+        int i0, i1;
+        chunk_range(M, it.y, i0, i1);
+        double s = 0.0;
+        for (int i = i0 + threadIdx.x; i < i1; i += FB_NT) s += M.p[i] * M.q[i];
+        const double v[1] = {s};
+        block_store_partials_at<1, FB_NT>(v, M.pa, it.y, M.nch);
+    }
+}
+
+// alpha = rho / p.q; x += alpha p, r -= alpha q; partials of r.r
+__global__ __launch_bounds__(FB_NT) void k_fb_update(const FbMember* __restrict__ Ms, const int2* __restrict__ work,
+                                                     int nwork) {
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const int2 it = work[w];
+        const FbMember& M = Ms[it.x];
+        CgState* __restrict__ st = M.st;
+        __syncthreads();
+        __shared__ int s_stop;
+        if (threadIdx.x == 0) s_stop = cg_stopped(st);
+        double t[1];
+        reduce_partials<1, FB_NT>(M.pa, M.nch, t);
+        if (s_stop) continue;
+        const double pq = t[0];
+        const bool lead = it.y == 0 && threadIdx.x == 0;
+        if (!(pq > 0.0)) {
+            if (lead) {
+                st->pq = pq;
+                st->stop = 1;
+                st->status = (pq - pq == 0.0) ? 2 : 3;
+            }
+            continue;
+        }
+        const double rho = st->rho;
+        const double alpha = rho / pq;
+        if (lead) {
+            st->pq = pq;
+            st->alpha = alpha;
+            st->rho_prev = rho;
+        }
+        int i0, i1;
+        chunk_range(M, it.y, i0, i1);
+        double rr = 0.0;
+        for (int i = i0 + threadIdx.x; i < i1; i += FB_NT) {
+            M.x[i] += alpha * M.p[i];
+            const double ri = M.r[i] - alpha * M.q[i];
+            M.r[i] = ri;
+            rr += ri * ri;
+        }
+        const double v[1] = {rr};
+        block_store_partials_at<1, FB_NT>(v, M.pv, it.y, M.nch);
+    }
+}
+
+// true residual t = b - q (q = A x): r = t; partials of sum t, sum t^2
+__global__ __launch_bounds__(FB_NT) void k_fb_true(const FbMember* __restrict__ Ms, const int2* __restrict__ work,
+                                                   int nwork) {
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        __syncthreads();  // the previous item's reduction is done with its LDS
+        const int2 it = work[w];
+        const FbMember& M = Ms[it.x];
+        int i0, i1;
+        chunk_range(M, it.y, i0, i1);
+        double s = 0.0, s2 = 0.0;
+        for (int i = i0 + threadIdx.x; i < i1; i += FB_NT) {
+            const double ti = M.b[i] - M.q[i];
+            M.r[i] = ti;
+            s += ti;
+            s2 += ti * ti;
+        }
+        const double v[2] = {s, s2};
+        block_store_partials_at<2, FB_NT>(v, M.pt, it.y, M.nch);
+    }
+}
+
+// per member (one workgroup each): ||t - mean t||^2 into the state
+__global__ __launch_bounds__(FB_NT) void k_fb_true_fin(const FbMember* __restrict__ Ms, const int* __restrict__ list,
+                                                       int count) {
+    for (int w = blockIdx.x; w < count; w += gridDim.x) {
+        const FbMember& M = Ms[list[w]];
+        __syncthreads();
+        double t[2];
+        reduce_partials<2, FB_NT>(M.pt, M.nch, t);
+        if (threadIdx.x == 0) {
+            const double mt = M.null_space ? t[0] / M.n : 0.0;
+            M.st->mu_t = mt;
+            M.st->tt = M.null_space ? t[1] - t[0] * mt : t[1];
+        }
+    }
+}
+
+// continue from the true residual: r -= mean (null space); partials of r.r
+__global__ __launch_bounds__(FB_NT) void k_fb_restart(const FbMember* __restrict__ Ms, const int2* __restrict__ work,
+                                                      int nwork) {
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        __syncthreads();  // the previous item's reduction is done with its LDS
+        const int2 it = work[w];
+        const FbMember& M = Ms[it.x];
+        const double mt = M.null_space ? M.st->mu_t : 0.0;
+        int i0, i1;
+        chunk_range(M, it.y, i0, i1);
+        double rr = 0.0;
+        for (int i = i0 + threadIdx.x; i < i1; i += FB_NT) {
+            const double ri = M.r[i] - mt;
+            M.r[i] = ri;
+            rr += ri * ri;
+        }
+        const double v[1] = {rr};
+        block_store_partials_at<1, FB_NT>(v, M.pv, it.y, M.nch);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host: per-operator FDM data
+// ---------------------------------------------------------------------------
+int pad64(int n) { return std::max(64, (n + 63) / 64 * 64); }
+
+struct FdmPlan;
+void fdm_plan_destroy(FdmPlan* pl);
+
+struct FdmOp {
+    FdmPlan* self = nullptr;  // the one-member plan of this operator's own solves
+    int nU0 = 0, nU1 = 0, nP0 = 0, nP1 = 0;
+    double dtol = 0.0;
+    DevBuf lx, ly, ex, ey;   // FP64 eigenvalues and mass factors, padded
+    DevBuf z, pv, pz, pa, pt, ps;
+    DevBuf st;               // CgState
+    struct Prec {
+        DevBuf vx, vxT, vy, vyT, t1, t2;
+        bool ready = false;
+    } prec[3];
+    std::shared_ptr<const se_fdm::Fdm1D> ex1, ey1;
+    std::vector<double> hx_vx, hx_vy;  // FP64 padded Vx, Vy (host), for the other precisions' uploads
+};
+
+void fdm_release(se_op* op) {
+    if (op->fdm) {
+        if (op->fdm->self) fdm_plan_destroy(op->fdm->self);
+        delete op->fdm;
+    }
+    op->fdm = nullptr;
+}
+
+namespace {
+
+// eigen-decompositions of (G, S) shared by operators of the same line
+// (keyed by the exact bytes of G and S)
+std::mutex g_eig_mu;
+std::map<std::string, std::shared_ptr<se_fdm::Fdm1D>> g_eig_cache;
+
+std::shared_ptr<const se_fdm::Fdm1D> eig_1d(int nU, int N, int per, const std::vector<double>& g,
+                                         const std::vector<double>& s) {
+    std::string key((const char*)g.data(), g.size() * sizeof(double));
+    key.append((const char*)s.data(), s.size() * sizeof(double));
+    key.append((const char*)&N, sizeof(int));
+    key.append((const char*)&per, sizeof(int));
+    {
+        std::lock_guard<std::mutex> lk(g_eig_mu);
+        auto it = g_eig_cache.find(key);
+        if (it != g_eig_cache.end()) return it->second;
+    }
+    auto e = std::make_shared<se_fdm::Fdm1D>();
+    std::vector<double> zero(nU, 0.0);
+    if (!se_fdm::fdm_1d(nU, N, per, g, s, zero, *e)) return nullptr;
+    std::lock_guard<std::mutex> lk(g_eig_mu);
+    g_eig_cache[key] = e;
+    return e;
+}
+
+uint16_t bf16_bits(double v) {
+    float f = (float)v;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);  // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+
+// device copy of a padded FP64 matrix (rows x cols) in element type `prec`,
+// transposed when `trans`
+int upload_matrix(DevBuf& d, const std::vector<double>& h, int rows, int cols, bool trans, int prec) {
+    const size_t cnt = (size_t)rows * cols;
+    std::vector<double> t(cnt);
+    for (int i = 0; i < rows; i++)
+        for (int j = 0; j < cols; j++) t[trans ? (size_t)j * rows + i : (size_t)i * cols + j] = h[(size_t)i * cols + j];
+    if (prec == FDM_F64) {
+        int rc = d.alloc(sizeof(double) * cnt);
+        if (rc) return rc;
+        SE_HIP(hipMemcpy(d.ptr, t.data(), sizeof(double) * cnt, hipMemcpyHostToDevice));
+    } else if (prec == FDM_F32) {
+        std::vector<float> f(cnt);
+        for (size_t i = 0; i < cnt; i++) f[i] = (float)t[i];
+        int rc = d.alloc(sizeof(float) * cnt);
+        if (rc) return rc;
+        SE_HIP(hipMemcpy(d.ptr, f.data(), sizeof(float) * cnt, hipMemcpyHostToDevice));
+    } else {
+        std::vector<uint16_t> f(cnt);
+        for (size_t i = 0; i < cnt; i++) f[i] = bf16_bits(t[i]);
+        int rc = d.alloc(sizeof(uint16_t) * cnt);
+        if (rc) return rc;
+        SE_HIP(hipMemcpy(d.ptr, f.data(), sizeof(uint16_t) * cnt, hipMemcpyHostToDevice));
+    }
+    return SE_OK;
+}
+
+size_t elem_size(int prec) { return prec == FDM_F64 ? 8 : (prec == FDM_F32 ? 4 : 2); }
+
+}  // namespace
+
+// FDM tables of op (FP64 part: eigenvectors, eigenvalues, mass factors and
+// work vectors) and the transforms in precision `prec`
+int fdm_prepare(se_op* op, int prec) {
+    SE_CHECK(op->svv_mN == 0, SE_ERR_UNSUPPORTED, "FDM preconditioner: operators with SVV solve with Jacobi");
+    SE_CHECK(op->definite, SE_ERR_UNSUPPORTED, "FDM preconditioner: the operator is not definite");
+    SE_CHECK(op->c_lap != 0.0, SE_ERR_UNSUPPORTED,
+             "FDM preconditioner: a mass-only operator is diagonal (use Jacobi, which is exact for it)");
+    const se_mesh* m = op->mesh;
+    int rc = op_ensure_work(op);
+    if (rc) return rc;
+    if (!op->fdm) {
+        std::unique_ptr<FdmOp> F(new FdmOp);
+        const int N = m->N;
+        std::vector<double> gx, X0, X1, MX, gy, Y0, Y1, MY;
+        assemble_line(m, op, 0, gx, X0, X1, MX);
+        assemble_line(m, op, 1, gy, Y0, Y1, MY);
+        const double sgn = (double)op->sign;
+        // sign*A = Sy (x) Gx + Gy (x) Sx + My (x) Mx  (the separable tables of mixed.hip)
+        std::vector<double> sx(m->nU[0]), mx(m->nU[0]), sy(m->nU[1]), my(m->nU[1]);
+        for (int I = 0; I < m->nU[0]; I++) {
+            sx[I] = sgn * op->c_lap * X1[I];
+            mx[I] = sgn * op->c_mass * MX[I];
+        }
+        for (int J = 0; J < m->nU[1]; J++) {
+            sy[J] = sgn * op->c_lap * Y0[J];
+            my[J] = MY[J];
+        }
+        F->ex1 = eig_1d(m->nU[0], N, m->periodic[0], gx, sx);
+        F->ey1 = eig_1d(m->nU[1], N, m->periodic[1], gy, sy);
+        SE_CHECK(F->ex1 && F->ey1, SE_ERR_STATE, "FDM preconditioner: 1-D eigenproblem failed");
+        F->nU0 = m->nU[0];
+        F->nU1 = m->nU[1];
+        F->nP0 = pad64(F->nU0);
+        F->nP1 = pad64(F->nU1);
+        const se_fdm::Fdm1D &Ex = *F->ex1, &Ey = *F->ey1;
+        std::vector<double> lx(F->nP0, 0.0), ex(F->nP0, 0.0), ly(F->nP1, 0.0), ey(F->nP1, 0.0);
+        double lmax = 0.0, emx = 0.0, emy = 0.0;
+        for (int k = 0; k < F->nU0; k++) {
+            lx[k] = Ex.lambda[k];
+            double e = 0.0;
+            for (int i = 0; i < F->nU0; i++) e += Ex.V[(size_t)i * F->nU0 + k] * Ex.V[(size_t)i * F->nU0 + k] * mx[i];
+            ex[k] = e;
+            lmax = std::max(lmax, std::fabs(lx[k]));
+            emx = std::max(emx, std::fabs(e));
+        }
+        double lmy = 0.0;
+        for (int k = 0; k < F->nU1; k++) {
+            ly[k] = Ey.lambda[k];
+            double e = 0.0;
+            for (int j = 0; j < F->nU1; j++) e += Ey.V[(size_t)j * F->nU1 + k] * Ey.V[(size_t)j * F->nU1 + k] * my[j];
+            ey[k] = e;
+            lmy = std::max(lmy, std::fabs(ly[k]));
+            emy = std::max(emy, std::fabs(e));
+        }
+        // a mode whose D is this small is the constant null mode of a
+        // singular (periodic / Neumann Poisson) operator
+        F->dtol = 1e-11 * (lmax + lmy + emx * emy);
+        F->hx_vx.assign((size_t)F->nP0 * F->nP0, 0.0);
+        F->hx_vy.assign((size_t)F->nP1 * F->nP1, 0.0);
+        for (int i = 0; i < F->nU0; i++)
+            for (int k = 0; k < F->nU0; k++) F->hx_vx[(size_t)i * F->nP0 + k] = Ex.V[(size_t)i * F->nU0 + k];
+        for (int j = 0; j < F->nU1; j++)
+            for (int k = 0; k < F->nU1; k++) F->hx_vy[(size_t)j * F->nP1 + k] = Ey.V[(size_t)j * F->nU1 + k];
+        rc = F->lx.alloc(sizeof(double) * F->nP0);
+        if (!rc) rc = F->ex.alloc(sizeof(double) * F->nP0);
+        if (!rc) rc = F->ly.alloc(sizeof(double) * F->nP1);
+        if (!rc) rc = F->ey.alloc(sizeof(double) * F->nP1);
+        if (rc) return rc;
+        SE_HIP(hipMemcpy(F->lx.ptr, lx.data(), sizeof(double) * F->nP0, hipMemcpyHostToDevice));
+        SE_HIP(hipMemcpy(F->ex.ptr, ex.data(), sizeof(double) * F->nP0, hipMemcpyHostToDevice));
+        SE_HIP(hipMemcpy(F->ly.ptr, ly.data(), sizeof(double) * F->nP1, hipMemcpyHostToDevice));
+        SE_HIP(hipMemcpy(F->ey.ptr, ey.data(), sizeof(double) * F->nP1, hipMemcpyHostToDevice));
+        const int n = std::max(m->n, 1);
+        const int nch = (n + FB_CH - 1) / FB_CH;
+        const int ntz = (F->nP1 / 64) * (F->nP0 / 64);
+        rc = F->z.alloc(sizeof(double) * n);
+        if (!rc) rc = F->pv.alloc(sizeof(double) * nch);
+        if (!rc) rc = F->pa.alloc(sizeof(double) * nch);
+        if (!rc) rc = F->ps.alloc(sizeof(double) * nch);
+        if (!rc) rc = F->pt.alloc(sizeof(double) * 2 * nch);
+        if (!rc) rc = F->pz.alloc(sizeof(double) * 3 * ntz);
+        if (!rc) rc = F->st.alloc(sizeof(CgState));
+        if (rc) return rc;
+        op->fdm = F.release();
+    }
+    FdmOp* F = op->fdm;
+    FdmOp::Prec& P = F->prec[prec];
+    if (!P.ready) {
+        rc = upload_matrix(P.vx, F->hx_vx, F->nP0, F->nP0, false, prec);
+        if (!rc) rc = upload_matrix(P.vxT, F->hx_vx, F->nP0, F->nP0, true, prec);
+        if (!rc) rc = upload_matrix(P.vy, F->hx_vy, F->nP1, F->nP1, false, prec);
+        if (!rc) rc = upload_matrix(P.vyT, F->hx_vy, F->nP1, F->nP1, true, prec);
+        if (!rc) rc = P.t1.alloc(elem_size(prec) * (size_t)F->nP0 * F->nP1);
+        if (!rc) rc = P.t2.alloc(elem_size(prec) * (size_t)F->nP0 * F->nP1);
+        if (rc) return rc;
+        P.ready = true;
+    }
+    return SE_OK;
+}
+
+// the four passes of op's transforms, z = FDM(r), in precision prec
+void fdm_tasks(se_op* op, int prec, const double* r, double* z, double* part, GemmTask (&t)[4]) {
+    FdmOp* F = op->fdm;
+    FdmOp::Prec& P = F->prec[prec];
+    const int nP0 = F->nP0, nP1 = F->nP1;
+    for (int k = 0; k < 4; k++) {
+        t[k] = GemmTask{};
+        t[k].dtol = F->dtol;
+    }
+    // pass 1: T1t[k'][J] = sum_I R[J][I] Vx[I][k']   (A = R packed into t2, Bt = Vx^T,
+    // store transposed)
+    t[0].A = P.t2.ptr;
+    t[0].r = r;
+    t[0].arows = F->nU1;
+    t[0].acols = F->nU0;
+    t[0].lda = nP0;
+    t[0].Bt = P.vxT.ptr;
+    t[0].ldb = nP0;
+    t[0].C = P.t1.ptr;
+    t[0].ldc = nP1;
+    t[0].M = nP1;
+    t[0].N = nP0;
+    t[0].K = nP0;
+    t[0].mode = FDM_OUT_T;
+    // pass 2: T2[k'][j'] = sum_J T1t[k'][J] Vy[J][j'] / D(k', j')   (Bt = Vy^T)
+    t[1].A = P.t1.ptr;
+    t[1].lda = nP1;
+    t[1].Bt = P.vyT.ptr;
+    t[1].ldb = nP1;
+    t[1].C = P.t2.ptr;
+    t[1].ldc = nP1;
+    t[1].M = nP0;
+    t[1].N = nP1;
+    t[1].K = nP1;
+    t[1].mode = FDM_OUT_SCALE;
+    t[1].lx = F->lx.as<double>();
+    t[1].ex = F->ex.as<double>();
+    t[1].ly = F->ly.as<double>();
+    t[1].ey = F->ey.as<double>();
+    t[1].crows = F->nU0;
+    t[1].ccols = F->nU1;
+    // pass 3: T3t[J][k'] = sum_j' T2[k'][j'] Vy[J][j']   (Bt = Vy, store transposed)
+    t[2].A = P.t2.ptr;
+    t[2].lda = nP1;
+    t[2].Bt = P.vy.ptr;
+    t[2].ldb = nP1;
+    t[2].C = P.t1.ptr;
+    t[2].ldc = nP0;
+    t[2].M = nP0;
+    t[2].N = nP1;
+    t[2].K = nP1;
+    t[2].mode = FDM_OUT_T;
+    // pass 4: Z[J][I] = sum_k' T3t[J][k'] Vx[I][k']   (Bt = Vx), FP64 on the unknown grid
+    t[3].A = P.t1.ptr;
+    t[3].lda = nP0;
+    t[3].Bt = P.vx.ptr;
+    t[3].ldb = nP0;
+    t[3].C = z;
+    t[3].ldc = F->nU0;
+    t[3].r = r;
+    t[3].part = part;
+    t[3].M = nP1;
+    t[3].N = nP0;
+    t[3].K = nP0;
+    t[3].mode = FDM_OUT_FINAL;
+    t[3].crows = F->nU1;
+    t[3].ccols = F->nU0;
+    for (int k = 0; k < 4; k++) t[k].tilesN = t[k].N / 64;
+}
+
+// the kernels' shape assumptions, checked on the host before any launch
+bool fdm_tasks_valid(const GemmTask (&t)[4], int prec) {
+    const int kc = prec == FDM_F64 ? GemmCfg<double>::KC : (prec == FDM_F32 ? GemmCfg<float>::KC : GemmCfg<__bf16>::KC);
+    for (int k = 0; k < 4; k++) {
+        if (t[k].M <= 0 || t[k].N <= 0 || t[k].K <= 0) return false;
+        if (t[k].M % 64 || t[k].N % 64 || t[k].K % 64 || t[k].K % kc) return false;
+        if (t[k].lda < t[k].K || t[k].ldb < t[k].K || t[k].lda % 64 || t[k].ldb % 64) return false;
+    }
+    return t[0].arows <= t[0].M && t[0].acols <= t[0].K && t[3].crows <= t[3].M && t[3].ccols <= t[3].N;
+}
+
+int fdm_tiles(const se_op* op) { return (op->fdm->nP0 / 64) * (op->fdm->nP1 / 64); }
+
+int launch_gemm(int prec, const GemmTask* tasks, int pass, const int2* work, int nwork, hipStream_t s) {
+    if (nwork <= 0) return SE_OK;
+    const dim3 grid(nwork), block(256);
+    if (pass == 0) {  // the residual, packed for pass 1
+        if (prec == FDM_F64) hipLaunchKernelGGL(k_fdm_pack<double>, grid, block, 0, s, tasks, work, nwork);
+        else if (prec == FDM_F32) hipLaunchKernelGGL(k_fdm_pack<float>, grid, block, 0, s, tasks, work, nwork);
+        else hipLaunchKernelGGL(k_fdm_pack<__bf16>, grid, block, 0, s, tasks, work, nwork);
+    }
+    if (prec == FDM_F64) hipLaunchKernelGGL(k_fdm_gemm<double>, grid, block, 0, s, tasks, pass, work, nwork);
+    else if (prec == FDM_F32) hipLaunchKernelGGL(k_fdm_gemm<float>, grid, block, 0, s, tasks, pass, work, nwork);
+    else hipLaunchKernelGGL(k_fdm_gemm<__bf16>, grid, block, 0, s, tasks, pass, work, nwork);
+    return SE_OK;
+}
+
+int fdm_prec_of(int precision) {
+    return precision == SE_PREC_FP64 ? FDM_F64 : (precision == SE_PREC_FP32_IR ? FDM_F32 : FDM_BF16);
+}
+
+// ---------------------------------------------------------------------------
+// host: the lockstep FDM-preconditioned CG over a set of operators
+// ---------------------------------------------------------------------------
+struct FdmPlan {
+    se_ctx* ctx = nullptr;
+    std::vector<se_op*> ops;
+    int prec = -1;
+    DevBuf d_members, d_tasks;  // FbMember[count], GemmTask[4 count]
+    std::vector<FbMember> h_members;
+    // FP64 apply descriptors per order (p -> q and x -> q)
+    struct Order {
+        int N;
+        std::vector<int> members;
+        DevBuf d_ap, d_ax;  // ApplyMember[]
+        DevBuf d_work;      // (local member, tile) items of the active members
+        std::vector<int2> h_work;
+    };
+    std::vector<Order> orders;
+    DevBuf d_vwork, d_gwork, d_list;  // active (member, chunk), (member, tile) items; member list
+    std::vector<int2> h_vwork, h_gwork;
+    std::vector<int> h_list;
+    CgState* h_states = nullptr;  // pinned
+    DevBuf d_states;
+    std::vector<char> lists_on, apply_on;  // member flags the device lists were built for
+
+    ~FdmPlan() {
+        if (h_states) (void)hipHostFree(h_states);
+    }
+};
+
+namespace {
+
+int launch_apply_batch(int N, const ApplyMember* A, const int2* work, int nw, hipStream_t s) {
+    switch (N) {
+#define SE_CASE(NN) \
+    case NN: hipLaunchKernelGGL((k_apply_batch<NN, 2, 2>), dim3(nw), dim3(256), 0, s, A, work, nw); break;
+        SE_CASE(2) SE_CASE(3) SE_CASE(4) SE_CASE(5) SE_CASE(6) SE_CASE(7) SE_CASE(8)
+#undef SE_CASE
+        default: return fail(SE_ERR_UNSUPPORTED, "FDM solve built for orders 2..8");
+    }
+    return SE_OK;
+}
+
+// q = sign A p (or sign A x) of the members flagged in `on`.  The host lists
+// are rebuilt only after the stream has been synchronised (every caller's
+// loop pulls the states in between), so no pending copy reads a changed list.
+int plan_apply(FdmPlan* pl, bool from_x, const std::vector<char>& on, hipStream_t s) {
+    const bool same = on == pl->apply_on;
+    if (!same) pl->apply_on = on;
+    for (auto& o : pl->orders) {
+        if (same) {
+            if (o.h_work.empty()) continue;
+            const ApplyMember* A = from_x ? o.d_ax.as<ApplyMember>() : o.d_ap.as<ApplyMember>();
+            const int nw = (int)o.h_work.size();
+            int rc = launch_apply_batch(o.N, A, o.d_work.as<int2>(), nw, s);
+            if (rc) return rc;
+            continue;
+        }
+        o.h_work.clear();
+        for (int j = 0; j < (int)o.members.size(); j++) {
+            const int i = o.members[j];
+            if (!on[i]) continue;
+            const se_mesh* m = pl->ops[i]->mesh;
+            const int t22 = ((m->nEls[0] + 1) / 2) * ((m->nEls[1] + 1) / 2);
+            for (int t = 0; t < t22; t++) o.h_work.push_back(make_int2(j, t));
+        }
+        if (o.h_work.empty()) continue;
+        int rc = o.d_work.alloc(sizeof(int2) * o.h_work.size());
+        if (rc) return rc;
+        SE_HIP(hipMemcpyAsync(o.d_work.ptr, o.h_work.data(), sizeof(int2) * o.h_work.size(), hipMemcpyHostToDevice, s));
+        const ApplyMember* A = from_x ? o.d_ax.as<ApplyMember>() : o.d_ap.as<ApplyMember>();
+        rc = launch_apply_batch(o.N, A, o.d_work.as<int2>(), (int)o.h_work.size(), s);
+        if (rc) return rc;
+    }
+    return SE_OK;
+}
+
+// work lists of the members flagged in `on`
+int plan_lists(FdmPlan* pl, const std::vector<char>& on, hipStream_t s) {
+    if (on == pl->lists_on) return SE_OK;  // the device lists are current
+    pl->lists_on = on;
+    pl->h_vwork.clear();
+    pl->h_gwork.clear();
+    pl->h_list.clear();
+    for (int i = 0; i < (int)pl->ops.size(); i++) {
+        if (!on[i]) continue;
+        pl->h_list.push_back(i);
+        for (int c = 0; c < pl->h_members[i].nch; c++) pl->h_vwork.push_back(make_int2(i, c));
+        for (int t = 0; t < pl->h_members[i].ntz; t++) pl->h_gwork.push_back(make_int2(i, t));
+    }
+    int rc = pl->d_vwork.alloc(sizeof(int2) * std::max<size_t>(pl->h_vwork.size(), 1));
+    if (!rc) rc = pl->d_gwork.alloc(sizeof(int2) * std::max<size_t>(pl->h_gwork.size(), 1));
+    if (!rc) rc = pl->d_list.alloc(sizeof(int) * std::max<size_t>(pl->h_list.size(), 1));
+    if (rc) return rc;
+    if (!pl->h_vwork.empty())
+        SE_HIP(hipMemcpyAsync(pl->d_vwork.ptr, pl->h_vwork.data(), sizeof(int2) * pl->h_vwork.size(),
+                              hipMemcpyHostToDevice, s));
+    if (!pl->h_gwork.empty())
+        SE_HIP(hipMemcpyAsync(pl->d_gwork.ptr, pl->h_gwork.data(), sizeof(int2) * pl->h_gwork.size(),
+                              hipMemcpyHostToDevice, s));
+    if (!pl->h_list.empty())
+        SE_HIP(hipMemcpyAsync(pl->d_list.ptr, pl->h_list.data(), sizeof(int) * pl->h_list.size(),
+                              hipMemcpyHostToDevice, s));
+    return SE_OK;
+}
+
+int grid_of(size_t items, const se_ctx* c) {
+    return (int)std::max<size_t>(1, std::min<size_t>(items, (size_t)16 * c->cu_count));
+}
+
+// z = FDM(r) of the listed members: four passes
+int plan_fdm(FdmPlan* pl, hipStream_t s) {
+    const int nw = (int)pl->h_gwork.size();
+    for (int pass = 0; pass < 4; pass++) {
+        int rc = launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, pl->d_gwork.as<int2>(), nw, s);
+        if (rc) return rc;
+    }
+    return SE_OK;
+}
+
+}  // namespace
+
+int fdm_plan_create(se_ctx* c, int count, se_op* const* ops, FdmPlan** out) {
+    std::unique_ptr<FdmPlan> pl(new FdmPlan);
+    pl->ctx = c;
+    pl->ops.assign(ops, ops + count);
+    for (int N = 2; N <= 8; N++) {
+        FdmPlan::Order o;
+        o.N = N;
+        for (int i = 0; i < count; i++)
+            if (ops[i]->mesh->N == N) o.members.push_back(i);
+        if (!o.members.empty()) pl->orders.push_back(std::move(o));
+    }
+    SE_HIP(hipHostMalloc((void**)&pl->h_states, sizeof(CgState) * std::max(count, 1), hipHostMallocDefault));
+    int rc = pl->d_members.alloc(sizeof(FbMember) * std::max(count, 1));
+    if (!rc) rc = pl->d_tasks.alloc(sizeof(GemmTask) * 4 * std::max(count, 1));
+    if (rc) return rc;
+    *out = pl.release();
+    return SE_OK;
+}
+
+void fdm_plan_destroy(FdmPlan* pl) { delete pl; }
+
+int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, const se_solve_opts& o,
+                   se_solve_info* infos) {
+    se_ctx* c = pl->ctx;
+    hipStream_t s = c->stream;
+    const int count = (int)pl->ops.size();
+    const int prec = fdm_prec_of(o.precision);
+    const int null = o.null_space ? 1 : 0;
+    for (int i = 0; i < count; i++) {
+        int rc = fdm_prepare(pl->ops[i], prec);
+        if (rc) return rc;
+    }
+    // descriptors (f changes per call; the rest is fixed per plan and precision)
+    pl->h_members.assign(count, FbMember{});
+    std::vector<GemmTask> tasks(4 * (size_t)count);
+    for (int i = 0; i < count; i++) {
+        se_op* op = pl->ops[i];
+        FdmOp* F = op->fdm;
+        FbMember& M = pl->h_members[i];
+        M.f = d_f[i];
+        M.b = op->b.as<double>();
+        M.x = op->x.as<double>();
+        M.r = op->r.as<double>();
+        M.p = op->p.as<double>();
+        M.q = op->q.as<double>();
+        M.z = F->z.as<double>();
+        M.pv = F->pv.as<double>();
+        M.pz = F->pz.as<double>();
+        M.pa = F->pa.as<double>();
+        M.pt = F->pt.as<double>();
+        M.ps = F->ps.as<double>();
+        M.n = op->mesh->n;
+        M.nch = (std::max(M.n, 1) + FB_CH - 1) / FB_CH;
+        M.ntz = fdm_tiles(op);
+        M.null_space = null;
+        M.sgn = (double)op->sign;
+        M.rtol = o.rtol;
+        GemmTask t4[4];
+        fdm_tasks(op, prec, M.r, M.z, M.pz, t4);
+        SE_CHECK(fdm_tasks_valid(t4, prec), SE_ERR_STATE, "FDM: transform shapes break the kernels' assumptions");
+        for (int k = 0; k < 4; k++) tasks[4 * (size_t)i + k] = t4[k];
+    }
+    int rc = pl->d_states.alloc(sizeof(CgState) * std::max(count, 1));
+    if (rc) return rc;
+    for (int i = 0; i < count; i++) pl->h_members[i].st = pl->d_states.as<CgState>() + i;
+    if (count > 0) {
+        SE_HIP(hipMemcpyAsync(pl->d_members.ptr, pl->h_members.data(), sizeof(FbMember) * count, hipMemcpyHostToDevice,
+                              s));
+        SE_HIP(hipMemcpyAsync(pl->d_tasks.ptr, tasks.data(), sizeof(GemmTask) * 4 * count, hipMemcpyHostToDevice, s));
+    }
+    pl->prec = prec;
+    // FP64 apply descriptors
+    for (auto& od : pl->orders) {
+        std::vector<ApplyMember> ap, ax;
+        for (int i : od.members) {
+            se_op* op = pl->ops[i];
+            ApplyMember A;
+            A.P = op_params(op, (double)op->sign);
+            A.in = op->p.as<double>();
+            A.out = op->q.as<double>();
+            ap.push_back(A);
+            A.in = op->x.as<double>();
+            ax.push_back(A);
+        }
+        rc = od.d_ap.alloc(sizeof(ApplyMember) * ap.size());
+        if (!rc) rc = od.d_ax.alloc(sizeof(ApplyMember) * ax.size());
+        if (rc) return rc;
+        SE_HIP(hipMemcpyAsync(od.d_ap.ptr, ap.data(), sizeof(ApplyMember) * ap.size(), hipMemcpyHostToDevice, s));
+        SE_HIP(hipMemcpyAsync(od.d_ax.ptr, ax.data(), sizeof(ApplyMember) * ax.size(), hipMemcpyHostToDevice, s));
+        SE_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
+    }
+    for (int i = 0; i < count; i++) {
+        CgState h{};
+        h.maxit = std::max(o.maxit, 0);
+        h.null_space = null;
+        h.n = pl->ops[i]->mesh->n;
+        h.status = CG_RUNNING;
+        pl->h_states[i] = h;
+    }
+    const FbMember* Ms = pl->d_members.as<FbMember>();
+    auto pull = [&]() -> int {
+        SE_HIP(hipMemcpyAsync(pl->h_states, pl->d_states.ptr, sizeof(CgState) * count, hipMemcpyDeviceToHost, s));
+        SE_HIP(hipStreamSynchronize(s));
+        return SE_OK;
+    };
+    auto vlaunch = [&](void (*k)(const FbMember*, const int2*, int)) {
+        const int nw = (int)pl->h_vwork.size();
+        if (nw > 0) hipLaunchKernelGGL(k, dim3(grid_of(nw, c)), dim3(FB_NT), 0, s, Ms, pl->d_vwork.as<int2>(), nw);
+    };
+    auto dlaunch = [&](int mode) {
+        const int nw = (int)pl->h_vwork.size();
+        if (nw > 0)
+            hipLaunchKernelGGL(k_fb_dir, dim3(grid_of(nw, c)), dim3(FB_NT), 0, s, Ms, pl->d_vwork.as<int2>(), nw, mode);
+    };
+
+    SE_HIP(hipEventRecord(c->ev0, s));
+    SE_HIP(hipMemcpyAsync(pl->d_states.ptr, pl->h_states, sizeof(CgState) * count, hipMemcpyHostToDevice, s));
+    pl->apply_on.clear();  // the apply descriptors were re-uploaded: rebuild the lists too
+    std::vector<char> on(count, 1);
+    rc = plan_lists(pl, on, s);
+    if (rc) return rc;
+    if (null) vlaunch(k_fb_sum);
+    vlaunch(k_fb_start);
+    rc = plan_fdm(pl, s);
+    if (rc) return rc;
+    dlaunch(FB_FIRST);
+    SE_HIP(hipGetLastError());
+    rc = pull();
+    if (rc) return rc;
+    std::vector<int> status(count, CG_RUNNING);
+    int running = 0;
+    for (int i = 0; i < count; i++) {
+        if (pl->h_states[i].stop) {
+            status[i] = pl->h_states[i].status;
+            on[i] = 0;
+        } else {
+            running++;
+        }
+    }
+    // progress of each member's recursive residual: a member whose r.r has
+    // not fallen 4x within FB_STALL iterations cannot be resolved by the
+    // preconditioner's precision (a near-singular system with BF16
+    // transforms) and stops with SE_SOLVE_STAGNATED
+    constexpr int FB_STALL = 40;
+    std::vector<double> best(count, 0.0);
+    std::vector<int> best_it(count, 0);
+    for (int i = 0; i < count; i++) best[i] = pl->h_states[i].rr;
+    int guard = 0;
+    while (running > 0) {
+        rc = plan_lists(pl, on, s);
+        if (rc) return rc;
+        // q = A p, alpha, x and r, z = FDM(r), beta and p
+        rc = plan_apply(pl, false, on, s);
+        if (rc) return rc;
+        vlaunch(k_fb_dot);
+        vlaunch(k_fb_update);
+        rc = plan_fdm(pl, s);
+        if (rc) return rc;
+        dlaunch(FB_NORMAL);
+        SE_HIP(hipGetLastError());
+        rc = pull();
+        if (rc) return rc;
+        // members whose recursive residual reached the target: confirm with the true residual
+        std::vector<char> chk(count, 0);
+        bool any = false;
+        for (int i = 0; i < count; i++) {
+            if (!on[i]) continue;
+            const CgState& h = pl->h_states[i];
+            if (!h.stop) {
+                if (h.rr < 0.25 * best[i]) {
+                    best[i] = h.rr;
+                    best_it[i] = h.iter;
+                } else if (h.iter - best_it[i] >= FB_STALL) {
+                    status[i] = SE_SOLVE_STAGNATED;
+                    on[i] = 0;
+                    running--;
+                }
+                continue;
+            }
+            if (h.status == CG_PENDING) {
+                chk[i] = 1;
+                any = true;
+            } else {
+                status[i] = h.status;
+                on[i] = 0;
+                running--;
+            }
+        }
+        if (!any) continue;
+        rc = plan_lists(pl, chk, s);
+        if (rc) return rc;
+        rc = plan_apply(pl, true, chk, s);
+        if (rc) return rc;
+        vlaunch(k_fb_true);
+        hipLaunchKernelGGL(k_fb_true_fin, dim3((int)pl->h_list.size()), dim3(FB_NT), 0, s, Ms, pl->d_list.as<int>(),
+                           (int)pl->h_list.size());
+        SE_HIP(hipGetLastError());
+        rc = pull();
+        if (rc) return rc;
+        std::vector<char> again(count, 0);
+        bool restart = false;
+        for (int i = 0; i < count; i++) {
+            if (!chk[i]) continue;
+            CgState& h = pl->h_states[i];
+            if (h.tt <= h.tol2) {
+                status[i] = 0;
+                on[i] = 0;
+                running--;
+            } else if (h.iter >= h.maxit || ++guard > 100 * count + 1000) {
+                status[i] = 1;
+                on[i] = 0;
+                running--;
+            } else {
+                again[i] = 1;
+                restart = true;
+                h.stop = 0;
+                h.status = CG_RUNNING;
+            }
+        }
+        if (!restart) continue;
+        // continue from the true residual (now in r) with a fresh direction
+        SE_HIP(hipMemcpyAsync(pl->d_states.ptr, pl->h_states, sizeof(CgState) * count, hipMemcpyHostToDevice, s));
+        rc = plan_lists(pl, again, s);
+        if (rc) return rc;
+        vlaunch(k_fb_restart);
+        rc = plan_fdm(pl, s);
+        if (rc) return rc;
+        dlaunch(FB_RESTART);
+        SE_HIP(hipGetLastError());
+        SE_HIP(hipStreamSynchronize(s));
+    }
+    // final true residuals of the members that did not converge
+    std::vector<char> fin(count, 0);
+    bool anyfin = false;
+    for (int i = 0; i < count; i++)
+        if (status[i] != 0 && pl->h_states[i].bb > 0.0 && pl->h_states[i].bb - pl->h_states[i].bb == 0.0) {
+            fin[i] = 1;
+            anyfin = true;
+        }
+    if (anyfin) {
+        rc = plan_lists(pl, fin, s);
+        if (!rc) rc = plan_apply(pl, true, fin, s);
+        if (rc) return rc;
+        vlaunch(k_fb_true);
+        hipLaunchKernelGGL(k_fb_true_fin, dim3((int)pl->h_list.size()), dim3(FB_NT), 0, s, Ms, pl->d_list.as<int>(),
+                           (int)pl->h_list.size());
+        rc = pull();
+        if (rc) return rc;
+    }
+    for (int i = 0; i < count; i++) {
+        se_op* op = pl->ops[i];
+        if (d_x[i] != op->x.as<double>())
+            SE_HIP(hipMemcpyAsync(d_x[i], op->x.ptr, sizeof(double) * op->mesh->n, hipMemcpyDeviceToDevice, s));
+    }
+    SE_HIP(hipEventRecord(c->ev1, s));
+    SE_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    for (int i = 0; i < count; i++) {
+        if (!infos) break;
+        const CgState& h = pl->h_states[i];
+        se_solve_info& inf = infos[i];
+        inf.status = status[i] == CG_RUNNING ? 1 : status[i];
+        inf.iters = h.iter;
+        inf.outer_iters = 1;
+        const double tt = (status[i] == 0 || fin[i]) ? h.tt : h.rr;
+        inf.true_rel_residual = h.bb > 0.0 ? std::sqrt(std::max(tt, 0.0) / h.bb) : 0.0;
+        inf.seconds = ms * 1e-3;
+    }
+    return SE_OK;
+}
+
+// Measurement: after a solve of the plan, time `reps` back-to-back launches
+// of each transform pass over all members (pass 1 with its residual pack)
+// and of the FP64 operator apply, with HIP events on the context stream.
+// Per pass: microseconds per launch and flops per launch (2 M N K summed over
+// the members' padded sizes); the apply: microseconds and algorithmic bytes
+// (16 B per unknown for x and y, 4 B per field node for the numbering map).
+int fdm_plan_probe(FdmPlan* pl, int reps, double* us_pass, double* flops_pass, double* us_apply, double* bytes_apply) {
+    SE_CHECK(pl->prec >= 0, SE_ERR_STATE, "FDM probe: run an FDM solve first");
+    se_ctx* c = pl->ctx;
+    hipStream_t s = c->stream;
+    const int count = (int)pl->ops.size();
+    std::vector<char> all(count, 1);
+    int rc = plan_lists(pl, all, s);
+    if (rc) return rc;
+    const int nw = (int)pl->h_gwork.size();
+    for (int pass = 0; pass < 4; pass++) {
+        double fl = 0.0;
+        for (se_op* op : pl->ops) {
+            const double nP0 = op->fdm->nP0, nP1 = op->fdm->nP1;
+            fl += 2.0 * nP0 * nP1 * ((pass == 0 || pass == 3) ? nP0 : nP1);
+        }
+        flops_pass[pass] = fl;
+        rc = launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, pl->d_gwork.as<int2>(), nw, s);
+        if (rc) return rc;
+        SE_HIP(hipEventRecord(c->ev0, s));
+        for (int r = 0; r < reps && !rc; r++)
+            rc = launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, pl->d_gwork.as<int2>(), nw, s);
+        SE_HIP(hipEventRecord(c->ev1, s));
+        SE_HIP(hipEventSynchronize(c->ev1));
+        if (rc) return rc;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        us_pass[pass] = 1e3 * ms / reps;
+    }
+    double by = 0.0;
+    for (se_op* op : pl->ops) by += 16.0 * op->mesh->n + 4.0 * op->mesh->nV;
+    *bytes_apply = by;
+    rc = plan_apply(pl, false, all, s);
+    if (rc) return rc;
+    SE_HIP(hipEventRecord(c->ev0, s));
+    for (int r = 0; r < reps && !rc; r++) rc = plan_apply(pl, false, all, s);
+    SE_HIP(hipEventRecord(c->ev1, s));
+    SE_HIP(hipEventSynchronize(c->ev1));
+    if (rc) return rc;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    *us_apply = 1e3 * ms / reps;
+    SE_HIP(hipGetLastError());
+    return SE_OK;
+}
+
+// solve of one operator: a plan of one member, kept with the operator
+int pcg_fdm(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, se_solve_info& info) {
+    SE_CHECK(op->definite, SE_ERR_UNSUPPORTED, "operator diagonal changes sign: not a definite system");
+    int rc = fdm_prepare(op, fdm_prec_of(o.precision));
+    if (rc) return rc;
+    if (!op->fdm->self) {
+        rc = fdm_plan_create(op->mesh->ctx, 1, &op, &op->fdm->self);
+        if (rc) return rc;
+    }
+    return fdm_plan_solve(op->fdm->self, &d_f, &d_x, o, &info);
+}
+
+// resolved preconditioner of a solve
+int solve_pc(const se_solve_opts& o) {
+    if (o.preconditioner == SE_PC_JACOBI || o.preconditioner == SE_PC_FDM) return o.preconditioner;
+    return o.precision == SE_PREC_FP64 ? SE_PC_JACOBI : SE_PC_FDM;
+}
+
+}  // namespace se
+
+// z = FDM(r) of sign*A (host arrays of n values), in `precision`
+extern "C" int se_helm_apply_fdm(se_op* op, int precision, const double* r, double* z) {
+    using namespace se;
+    SE_CHECK(op && r && z, SE_ERR_ARG, "se_helm_apply_fdm: NULL argument");
+    SE_CHECK(precision == SE_PREC_FP64 || precision == SE_PREC_FP32_IR || precision == SE_PREC_BF16_IR, SE_ERR_ARG,
+             "se_helm_apply_fdm: bad precision");
+    SE_HIP(hipSetDevice(op->mesh->ctx->device));
+    const int prec = fdm_prec_of(precision);
+    int rc = fdm_prepare(op, prec);
+    if (rc) return rc;
+    const int n = op->mesh->n;
+    hipStream_t s = op->mesh->ctx->stream;
+    SE_HIP(hipStreamSynchronize(s));
+    if (n > 0) SE_HIP(hipMemcpy(op->r.ptr, r, sizeof(double) * n, hipMemcpyHostToDevice));
+    GemmTask t[4];
+    fdm_tasks(op, prec, op->r.as<double>(), op->fdm->z.as<double>(), op->fdm->pz.as<double>(), t);
+    SE_CHECK(fdm_tasks_valid(t, prec), SE_ERR_STATE, "se_helm_apply_fdm: transform shapes break the kernels' assumptions");
+    DevBuf d;
+    rc = d.alloc(sizeof(t));
+    if (rc) return rc;
+    SE_HIP(hipMemcpy(d.ptr, t, sizeof(t), hipMemcpyHostToDevice));
+    for (int pass = 0; pass < 4; pass++) {
+        rc = launch_gemm(prec, d.as<GemmTask>(), pass, nullptr, fdm_tiles(op), s);
+        if (rc) return rc;
+    }
+    SE_HIP(hipGetLastError());
+    SE_HIP(hipStreamSynchronize(s));
+    if (n > 0) SE_HIP(hipMemcpy(z, op->fdm->z.ptr, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return SE_OK;
+}
+
+// measurement hook of a single operator's FDM solve (after one)
+extern "C" int se_helm_fdm_probe(se_op* op, int reps, double* us_pass, double* flops_pass, double* us_apply,
+                                 double* bytes_apply) {
+    using namespace se;
+    SE_CHECK(op && us_pass && flops_pass && us_apply && bytes_apply && reps > 0, SE_ERR_ARG,
+             "se_helm_fdm_probe: bad argument");
+    SE_CHECK(op->fdm && op->fdm->self, SE_ERR_STATE, "se_helm_fdm_probe: run an FDM solve of this operator first");
+    SE_HIP(hipSetDevice(op->mesh->ctx->device));
+    return fdm_plan_probe(op->fdm->self, reps, us_pass, flops_pass, us_apply, bytes_apply);
+}

@@ -498,6 +498,7 @@ extern "C" void se_solve_opts_default(se_solve_opts* o) {
     o->inner_rtol = 0.0;
     o->max_outer = 1000;
     o->history = nullptr;
+    o->preconditioner = SE_PC_DEFAULT;
 }
 
 extern "C" int se_helm_create(se_mesh* m, double c_mass, double c_lap, int detj_mode, se_op** out) {
@@ -841,7 +842,12 @@ extern "C" int se_helm_pcg_dev(se_op* op, const double* d_f, double* d_x, const 
     se_solve_info inf{};
     SE_HIP(hipSetDevice(op->mesh->ctx->device));
     int rc;
-    if (o.precision == SE_PREC_FP64) rc = pcg_fp64(op, d_f, d_x, o, inf);
+    SE_CHECK(o.precision == SE_PREC_FP64 || o.precision == SE_PREC_FP32_IR || o.precision == SE_PREC_BF16_IR,
+             SE_ERR_ARG, "se_helm_pcg_dev: bad precision");
+    SE_CHECK(o.preconditioner >= SE_PC_DEFAULT && o.preconditioner <= SE_PC_FDM, SE_ERR_ARG,
+             "se_helm_pcg_dev: bad preconditioner");
+    if (solve_pc(o) == SE_PC_FDM) rc = pcg_fdm(op, d_f, d_x, o, inf);
+    else if (o.precision == SE_PREC_FP64) rc = pcg_fp64(op, d_f, d_x, o, inf);
     else rc = pcg_mixed(op, d_f, d_x, o, inf);
     if (info) *info = inf;
     return rc;

@@ -22,14 +22,7 @@
 
 using namespace se;
 
-namespace {
-
-constexpr int kVB = 1024;
-
-int vblocks(int n) {
-    int b = (n + kVB * 4 - 1) / (kVB * 4);
-    return std::max(1, std::min(b, 256));
-}
+namespace se {
 
 // unknown index of a field line, -1 on a Dirichlet line (BCs.cpp:91-172)
 int line_unknown(int I, int nv, int per, int lo, int hi) {
@@ -37,38 +30,6 @@ int line_unknown(int I, int nv, int per, int lo, int hi) {
     if (lo && I == 0) return -1;
     if (hi && I == nv - 1) return -1;
     return I - lo;
-}
-
-__global__ __launch_bounds__(kVB) void k_ir_sum(int n, const double* __restrict__ f, double sgn,
-                                                double* __restrict__ part) {
-    double s = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += sgn * f[i];
-    double v[1] = {s};
-    block_store_partials<1, kVB>(v, part);
-}
-
-__global__ void k_f32_to_bf16(int n, const float* __restrict__ a, __hip_bfloat16* __restrict__ b) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = __float2bfloat16(a[i]);
-}
-
-// ---------------------------------------------------------------------------
-// separable band tables (host assembly, FP32 upload)
-// ---------------------------------------------------------------------------
-struct SepLayout {
-    size_t gx, gy, sx, sy, mx, my, total;  // float offsets
-};
-
-SepLayout sep_layout(const se_mesh* m) {
-    const int B = 2 * m->N + 1;
-    SepLayout L;
-    L.gx = 0;
-    L.gy = L.gx + (size_t)m->nU[0] * B;
-    L.sx = L.gy + (size_t)m->nU[1] * B;
-    L.sy = L.sx + m->nU[0];
-    L.mx = L.sy + m->nU[1];
-    L.my = L.mx + m->nU[0];
-    L.total = L.my + m->nU[1];
-    return L;
 }
 
 // Per direction: assembled band rows g (from the element operators
@@ -112,6 +73,49 @@ void assemble_line(const se_mesh* m, const se_op* op, int axis, std::vector<doub
             }
         }
     }
+}
+
+}  // namespace se
+
+namespace {
+
+constexpr int kVB = 1024;
+
+int vblocks(int n) {
+    int b = (n + kVB * 4 - 1) / (kVB * 4);
+    return std::max(1, std::min(b, 256));
+}
+
+__global__ __launch_bounds__(kVB) void k_ir_sum(int n, const double* __restrict__ f, double sgn,
+                                                double* __restrict__ part) {
+    double s = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += sgn * f[i];
+    double v[1] = {s};
+    block_store_partials<1, kVB>(v, part);
+}
+
+__global__ void k_f32_to_bf16(int n, const float* __restrict__ a, __hip_bfloat16* __restrict__ b) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = __float2bfloat16(a[i]);
+}
+
+// ---------------------------------------------------------------------------
+// separable band tables (host assembly, FP32 upload)
+// ---------------------------------------------------------------------------
+struct SepLayout {
+    size_t gx, gy, sx, sy, mx, my, total;  // float offsets
+};
+
+SepLayout sep_layout(const se_mesh* m) {
+    const int B = 2 * m->N + 1;
+    SepLayout L;
+    L.gx = 0;
+    L.gy = L.gx + (size_t)m->nU[0] * B;
+    L.sx = L.gy + (size_t)m->nU[1] * B;
+    L.sy = L.sx + m->nU[0];
+    L.mx = L.sy + m->nU[1];
+    L.my = L.mx + m->nU[0];
+    L.total = L.my + m->nU[1];
+    return L;
 }
 
 int sep_build(se_op* op) {

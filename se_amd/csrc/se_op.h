@@ -3,6 +3,12 @@
 #include "helm_kernels.h"
 #include "se_common.h"
 
+struct se_op;
+namespace se {
+struct FdmOp;                   // fast-diagonalisation data of an operator (fdm.hip)
+void fdm_release(se_op* op);
+}  // namespace se
+
 struct se_op {
     se_mesh* mesh = nullptr;
     double c_mass = 0.0, c_lap = 0.0;
@@ -33,8 +39,11 @@ struct se_op {
     double svv_dt = 0.0;
     int svv_mN = 0;
     se::DevBuf svv_mk;
+    // fast-diagonalisation preconditioner (fdm.hip), built on first use
+    se::FdmOp* fdm = nullptr;
 
     void release_all() {
+        se::fdm_release(this);
         se::DevBuf* bufs[] = {&sep, &sep_state, &sep_part, &dinv, &dinv_s, &lift, &tabs, &state, &partials, &b, &x, &r, &p, &q, &f,
                               &sx, &sy, &sfield, &sfield2, &lo_r, &lo_p, &lo_q, &lo_z, &lo_x, &lo_dinv,
                               &lo_geo, &d64, &svv_mk};
@@ -52,6 +61,16 @@ int op_apply_dev(se_op* op, const double* d_x, double* d_y, double scale, bool d
 int op_ensure_work(se_op* op);
 int pcg_fp64(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, se_solve_info& info);
 int pcg_mixed(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, se_solve_info& info);
+// FDM-preconditioned CG (fdm.hip); solve_pc resolves SE_PC_DEFAULT
+int pcg_fdm(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, se_solve_info& info);
+int solve_pc(const se_solve_opts& o);
+struct FdmPlan;
+int fdm_plan_create(se_ctx* c, int count, se_op* const* ops, FdmPlan** out);
+void fdm_plan_destroy(FdmPlan* pl);
+int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, const se_solve_opts& o,
+                   se_solve_info* infos);
+int fdm_plan_probe(FdmPlan* pl, int reps, double* us_pass, double* flops_pass, double* us_apply,
+                   double* bytes_apply);
 // SVV term (svv.hip): y += scale S x; np partials of x.y; lift -= S x_D
 int svv_apply(se_op* op, const double* d_x, double* d_y, double scale);
 int svv_dot(se_op* op, const double* d_x, const double* d_y, double* partials, int np);
@@ -62,4 +81,9 @@ int svv_lift(se_op* op, const double* d_field, double* d_lift);
 namespace se {
 int tcg_prepare(se_op* op);
 int tcg_tiles(const se_mesh* m);
+// separable form of the operator along one axis (FP64, before any sign or
+// coefficient): band rows g [nU][2N+1], line sums s0, s1 of the metric
+// factors and the mass factor mm [nU]
+void assemble_line(const se_mesh* m, const se_op* op, int axis, std::vector<double>& g, std::vector<double>& s0,
+                   std::vector<double>& s1, std::vector<double>& mm);
 }  // namespace se

@@ -39,9 +39,9 @@ def test_batch_equals_single_solves(gpu_ctx, prec):
         mem = [x for x in mem if x["lab"] != "odd_13x11_p4"]
     rng = np.random.default_rng(41)
     fs = [rng.standard_normal(x["m"].n) for x in mem]
-    single = [x["h"].pcg(f, rtol=1e-10, precision=prec) for x, f in zip(mem, fs)]
+    single = [x["h"].pcg(f, rtol=1e-10, precision=prec, preconditioner=se_amd.PC_JACOBI) for x, f in zip(mem, fs)]
     b = se_amd.Batch(gpu_ctx, [x["h"] for x in mem])
-    xs, infos = b.pcg(fs, precision=prec, rtol=1e-10)
+    xs, infos = b.pcg(fs, precision=prec, rtol=1e-10, preconditioner=se_amd.PC_JACOBI)
     for x, f, (x1, i1), xb, ib in zip(mem, fs, single, xs, infos):
         assert ib["status"] == se_amd.SOLVE_CONVERGED, (x["lab"], ib)
         assert ib["iters"] == i1["iters"] and ib["outer_iters"] == i1["outer_iters"], (x["lab"], ib, i1)
@@ -49,7 +49,7 @@ def test_batch_equals_single_solves(gpu_ctx, prec):
         r = f - x["oh"].matvec(xb)
         assert np.linalg.norm(r) <= 1.5e-10 * np.linalg.norm(f), x["lab"]
     # a second solve with the same batch (tuned launch shape, cached graph)
-    xs2, infos2 = b.pcg(fs, precision=prec, rtol=1e-10)
+    xs2, infos2 = b.pcg(fs, precision=prec, rtol=1e-10, preconditioner=se_amd.PC_JACOBI)
     for xb, xb2 in zip(xs, xs2):
         assert np.array_equal(xb, xb2)
     b.close()
@@ -60,9 +60,9 @@ def test_batch_null_space_members(gpu_ctx):
     rng = np.random.default_rng(43)
     fs = [rng.standard_normal(x["m"].n) for x in mem]
     b = se_amd.Batch(gpu_ctx, [x["h"] for x in mem])
-    xs, infos = b.pcg(fs, rtol=1e-10, null_space=True)
+    xs, infos = b.pcg(fs, rtol=1e-10, null_space=True, preconditioner=se_amd.PC_JACOBI)
     for x, f, xb, ib in zip(mem, fs, xs, infos):
-        x1, i1 = x["h"].pcg(f, rtol=1e-10, null_space=True, precision=se_amd.PREC_FP32_IR)
+        x1, i1 = x["h"].pcg(f, rtol=1e-10, null_space=True, precision=se_amd.PREC_FP32_IR, preconditioner=se_amd.PC_JACOBI)
         assert ib["status"] == 0 and np.array_equal(xb, x1)
         fp = f - f.mean()
         r = fp - x["oh"].matvec(xb)
@@ -77,11 +77,11 @@ def test_batch_same_alone_and_in_any_mix(gpu_ctx):
     rng = np.random.default_rng(47)
     fs = [rng.standard_normal(x["m"].n) for x in mem]
     b1 = se_amd.Batch(gpu_ctx, [x["h"] for x in mem])
-    xs1, _ = b1.pcg(fs, rtol=1e-10)
+    xs1, _ = b1.pcg(fs, rtol=1e-10, preconditioner=se_amd.PC_JACOBI)
     b1.close()
     order = [4, 1, 5, 0]
     b2 = se_amd.Batch(gpu_ctx, [mem[j]["h"] for j in order])
-    xs2, _ = b2.pcg([fs[j] for j in order], rtol=1e-10)
+    xs2, _ = b2.pcg([fs[j] for j in order], rtol=1e-10, preconditioner=se_amd.PC_JACOBI)
     b2.close()
     for k, j in enumerate(order):
         assert np.array_equal(xs2[k], xs1[j])
@@ -99,13 +99,13 @@ def test_batch_ensemble_members(gpu_ctx):
         hs.append(se_amd.Helm(m, c["c_mass"], c["c_lap"], c["detj_mode"]))
         fs.append(np.random.default_rng(c["seed"]).standard_normal(m.n))
     b = se_amd.Batch(gpu_ctx, hs)
-    xs, infos = b.pcg(fs, rtol=1e-10)
+    xs, infos = b.pcg(fs, rtol=1e-10, preconditioner=se_amd.PC_JACOBI)
     for h, f, xb, ib in zip(hs, fs, xs, infos):
         assert ib["status"] == 0
         r = f - h.apply(xb)
         assert np.linalg.norm(r) <= 1.5e-10 * np.linalg.norm(f)
     for h, f, xb, ib in list(zip(hs, fs, xs, infos))[:6]:
-        x1, i1 = h.pcg(f, rtol=1e-10, precision=se_amd.PREC_FP32_IR)
+        x1, i1 = h.pcg(f, rtol=1e-10, precision=se_amd.PREC_FP32_IR, preconditioner=se_amd.PC_JACOBI)
         assert np.array_equal(xb, x1) and i1["iters"] == ib["iters"]
     b.close()
 
@@ -117,5 +117,5 @@ def test_batch_argument_errors(gpu_ctx):
         se_amd.Batch(gpu_ctx, [h, h])
     b = se_amd.Batch(gpu_ctx, [h])
     with pytest.raises(se_amd.SEError):
-        b.pcg([np.ones(m.n)], precision=se_amd.PREC_FP64)
+        b.pcg([np.ones(m.n)], precision=se_amd.PREC_FP64, preconditioner=se_amd.PC_JACOBI)
     b.close()

@@ -1,0 +1,128 @@
+"""Fast-diagonalisation (FDM) preconditioner on the matrix cores and the
+FDM-preconditioned CG.
+
+The reference solves with PETSc's Jacobi-PCG in FP64 (KSPSolve,
+src/linsys/HelmholtzEqn.cpp:120-131); the FDM path is a different
+preconditioner for the same system, so its bars are the system's: the FDM is
+checked as an inverse of the oracle's assembled operator (a restatement of
+Matrix::Assemble, src/base/Matrix.cpp:47-133) -- to what the separable form
+and the transform precision allow -- and every FDM-preconditioned solve must
+meet the 1e-10 true-residual target against that operator.  The MFMA lane
+layouts of the three GEMM variants are pinned by the FP64 case: a wrong
+layout does not give an inverse at all.
+"""
+import numpy as np
+import pytest
+
+import se_amd
+from oracle import oracle as O
+
+from test_gpu_parity import OP_CASES, rel
+
+pytestmark = pytest.mark.gpu
+
+PRECS = [(se_amd.PREC_FP64, "fp64"), (se_amd.PREC_FP32_IR, "fp32"), (se_amd.PREC_BF16_IR, "bf16")]
+# ||sign*A z - r|| / ||r|| of one FDM application: the separable form drops
+# round-off-sized Jacobian cross terms (FP64), then the transforms' rounding
+# (FP32: 2^-24 per operand and sum; BF16 operands: 2^-9)
+# and the residual amplifies the last transform's rounding of the high modes
+# (BF16: up to ~0.3 measured)
+INV_TOL = {se_amd.PREC_FP64: 1e-9, se_amd.PREC_FP32_IR: 1e-4, se_amd.PREC_BF16_IR: 0.5}
+# FDM-preconditioned CG iterations to 1e-10
+ITER_MAX = {se_amd.PREC_FP64: 3, se_amd.PREC_FP32_IR: 6, se_amd.PREC_BF16_IR: 20}
+# Near-singular system: periodic literal-mode Laplacian with c_mass = 1 (the
+# constant mode's eigenvalue is ~1e-5 of the spectrum).  Its huge constant
+# component swamps the other modes in low precision: the FP32 transforms
+# leave ~2e-2 of the residual (the CG still converges), BF16 ones cannot
+# resolve the system and the solve must say so (SOLVE_STAGNATED).
+NEAR_SINGULAR = "odd_13x11_p4"
+
+
+@pytest.fixture(scope="module", params=OP_CASES, ids=[c[0] for c in OP_CASES])
+def case(request, gpu_ctx):
+    lab, nEls, N, lo, hi, per, bc, cm, cl, detj = request.param
+    om = O.OracleMesh(nEls, N, lo, hi, per, bc)
+    oh = O.Helm(om, cm, cl, detj)
+    m = se_amd.Mesh(gpu_ctx, nEls, N, lo, hi, per, bc)
+    h = se_amd.Helm(m, cm, cl, detj)
+    null = cm == 0.0 and per == (1, 1)
+    sgn = 1.0 if oh.inv_diag()[0] > 0 else -1.0
+    yield dict(om=om, oh=oh, m=m, h=h, null=null, sgn=sgn, label=lab)
+    h.close()
+    m.close()
+
+
+@pytest.mark.parametrize("prec", [p for p, _ in PRECS], ids=[n for _, n in PRECS])
+def test_fdm_inverts_the_assembled_operator(case, prec):
+    oh, h, n = case["oh"], case["h"], case["m"].n
+    r = np.random.default_rng(5).standard_normal(n)
+    if case["null"]:
+        r -= r.mean()
+    z = h.apply_fdm(r, prec)
+    res = rel(case["sgn"] * oh.matvec(z), r)
+    print("%s prec %d: ||sA z - r|| / ||r|| = %.2e" % (case["label"], prec, res))
+    tol = INV_TOL[prec]
+    if case["label"] == NEAR_SINGULAR and prec != se_amd.PREC_FP64:
+        tol = 5e-2 if prec == se_amd.PREC_FP32_IR else 1.0
+    assert res <= tol, res
+
+
+@pytest.mark.parametrize("prec", [p for p, _ in PRECS], ids=[n for _, n in PRECS])
+def test_fdm_pcg_meets_fp64_target(case, prec):
+    oh, h, n, null = case["oh"], case["h"], case["m"].n, case["null"]
+    f = np.random.default_rng(19).standard_normal(n)
+    x, info = h.pcg(f, rtol=1e-10, null_space=null, precision=prec, preconditioner=se_amd.PC_FDM)
+    print("%s prec %d: %d iterations, true rel residual %.2e" % (case["label"], prec, info["iters"],
+                                                                info["true_rel_residual"]))
+    if case["label"] == NEAR_SINGULAR and prec == se_amd.PREC_BF16_IR:
+        assert info["status"] == se_amd.SOLVE_STAGNATED, info
+        assert info["iters"] <= 200, info  # declared after a few stalled windows, not at maxit
+        return
+    assert info["status"] == se_amd.SOLVE_CONVERGED, info
+    assert info["true_rel_residual"] <= 1e-10
+    assert info["iters"] <= ITER_MAX[prec], info
+    fb = f - f.mean() if null else f
+    xr = x - x.mean() if null else x
+    assert np.linalg.norm(fb - oh.matvec(xr)) <= 1.5e-10 * np.linalg.norm(fb)
+    # the same system solved by the reference's algorithm (FP64 Jacobi-PCG)
+    x64, _ = h.pcg(f, rtol=1e-12, null_space=null, preconditioner=se_amd.PC_JACOBI)
+    if null:
+        x64 = x64 - x64.mean()
+    assert rel(xr, x64) <= 1e-4  # two 1e-10 solutions of systems with condition numbers up to ~1e6
+
+
+def test_fdm_batch_equals_single_solves(gpu_ctx):
+    """Members of different orders, sizes and boundary conditions: each batched
+    result is bitwise its single solve, in all three precisions."""
+    cases = [c for c in OP_CASES if not (c[7] == 0.0 and c[5] == (1, 1)) and c[0] != NEAR_SINGULAR]
+    mem = []
+    for lab, nEls, N, lo, hi, per, bc, cm, cl, detj in cases:
+        m = se_amd.Mesh(gpu_ctx, nEls, N, lo, hi, per, bc)
+        mem.append((lab, m, se_amd.Helm(m, cm, cl, detj)))
+    rng = np.random.default_rng(23)
+    fs = [rng.standard_normal(m.n) for _, m, _ in mem]
+    b = se_amd.Batch(gpu_ctx, [h for _, _, h in mem])
+    for prec, _ in PRECS:
+        single = [h.pcg(f, rtol=1e-10, precision=prec, preconditioner=se_amd.PC_FDM) for (_, _, h), f in zip(mem, fs)]
+        xs, infos = b.pcg(fs, precision=prec, rtol=1e-10, preconditioner=se_amd.PC_FDM)
+        for (lab, _, _), (x1, i1), xb, ib in zip(mem, single, xs, infos):
+            assert ib["status"] == se_amd.SOLVE_CONVERGED, (lab, ib)
+            assert ib["iters"] == i1["iters"], (lab, ib, i1)
+            assert np.array_equal(xb, x1), lab
+    b.close()
+    for _, m, h in mem:
+        h.close()
+        m.close()
+
+
+def test_fdm_refuses_what_it_cannot_invert(gpu_ctx):
+    m = se_amd.Mesh(gpu_ctx, (4, 4), 4, (0, 0), (1, 1), (0, 0), (1, 1, 1, 1))
+    mass = se_amd.Helm(m, 1.0, 0.0, 1)  # diagonal operator: Jacobi is its exact inverse
+    with pytest.raises(se_amd.SEError):
+        mass.apply_fdm(np.ones(m.n))
+    x, info = mass.pcg(np.ones(m.n), rtol=1e-10, preconditioner=se_amd.PC_JACOBI)
+    assert info["status"] == se_amd.SOLVE_CONVERGED
+    with pytest.raises(se_amd.SEError):
+        mass.pcg(np.ones(m.n), rtol=1e-10, preconditioner=7)
+    mass.close()
+    m.close()

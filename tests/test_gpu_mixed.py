@@ -68,7 +68,7 @@ BF16_UNRESOLVABLE = {"odd_13x11_p4"}
 def test_refined_solve_meets_fp64_target(case, prec):
     oh, h, n, null = case["oh"], case["h"], case["m"].n, case["null"]
     f = np.random.default_rng(17).standard_normal(n)
-    x, info = h.pcg(f, rtol=1e-10, null_space=null, precision=prec)
+    x, info = h.pcg(f, rtol=1e-10, null_space=null, precision=prec, preconditioner=se_amd.PC_JACOBI)
     if prec == se_amd.PREC_BF16_IR and case["label"] in BF16_UNRESOLVABLE:
         assert info["status"] == se_amd.SOLVE_STAGNATED, info
         # declared within 32 reliable-update windows of 256 iterations, far
@@ -83,7 +83,7 @@ def test_refined_solve_meets_fp64_target(case, prec):
     if null:
         r = r - r.mean()
     assert np.linalg.norm(r) <= 1.5e-10 * np.linalg.norm(fp)
-    x64, i64 = h.pcg(f, rtol=1e-10, null_space=null)
+    x64, i64 = h.pcg(f, rtol=1e-10, null_space=null, preconditioner=se_amd.PC_JACOBI)
     if null:
         x, x64 = x - x.mean(), x64 - x64.mean()
     if case["N"] <= 6:
@@ -93,8 +93,8 @@ def test_refined_solve_meets_fp64_target(case, prec):
 def test_refined_solve_deterministic(case):
     h, n, null = case["h"], case["m"].n, case["null"]
     f = np.random.default_rng(3).standard_normal(n)
-    x1, i1 = h.pcg(f, rtol=1e-10, null_space=null, precision=se_amd.PREC_FP32_IR)
-    x2, i2 = h.pcg(f, rtol=1e-10, null_space=null, precision=se_amd.PREC_FP32_IR)
+    x1, i1 = h.pcg(f, rtol=1e-10, null_space=null, precision=se_amd.PREC_FP32_IR, preconditioner=se_amd.PC_JACOBI)
+    x2, i2 = h.pcg(f, rtol=1e-10, null_space=null, precision=se_amd.PREC_FP32_IR, preconditioner=se_amd.PC_JACOBI)
     assert np.array_equal(x1, x2) and i1["iters"] == i2["iters"]
 
 
@@ -102,14 +102,14 @@ def test_refined_status_codes(gpu_ctx):
     m = se_amd.Mesh(gpu_ctx, (4, 4), 4, (0, 0), (1, 1), (0, 0), (1, 1, 1, 1))
     h = se_amd.Helm(m, 1.0, 1.0, 1)
     for prec in (se_amd.PREC_FP32_IR, se_amd.PREC_BF16_IR):
-        _, info = h.pcg(np.zeros(m.n), precision=prec)
+        _, info = h.pcg(np.zeros(m.n), precision=prec, preconditioner=se_amd.PC_JACOBI)
         assert info["status"] == se_amd.SOLVE_CONVERGED and info["iters"] == 0
         f = np.ones(m.n)
         f[2] = np.inf
-        _, info = h.pcg(f, precision=prec)
+        _, info = h.pcg(f, precision=prec, preconditioner=se_amd.PC_JACOBI)
         assert info["status"] == se_amd.SOLVE_NONFINITE
         # an outer-step limit that cannot reach the target reports maxit
-        _, info = h.pcg(np.ones(m.n), rtol=1e-14, precision=prec, max_outer=1, inner_rtol=1e-2)
+        _, info = h.pcg(np.ones(m.n), rtol=1e-14, precision=prec, max_outer=1, inner_rtol=1e-2, preconditioner=se_amd.PC_JACOBI)
         assert info["status"] == se_amd.SOLVE_MAXIT and info["outer_iters"] == 1
 
 
@@ -121,10 +121,10 @@ def test_refined_full_size(gpu_ctx):
     m = se_amd.Mesh(gpu_ctx, cfg["nEls"], cfg["N"], cfg["lo"], cfg["hi"], cfg["periodic"], cfg["bc"])
     h = se_amd.Helm(m, cfg["c_mass"], cfg["c_lap"], cfg["detj_mode"])
     f = np.random.default_rng(29).standard_normal(m.n)
-    x64, i64 = h.pcg(f, rtol=1e-10)
+    x64, i64 = h.pcg(f, rtol=1e-10, preconditioner=se_amd.PC_JACOBI)
     assert i64["status"] == 0
     for prec in (se_amd.PREC_FP32_IR, se_amd.PREC_BF16_IR):
-        x, info = h.pcg(f, rtol=1e-10, precision=prec)
+        x, info = h.pcg(f, rtol=1e-10, precision=prec, preconditioner=se_amd.PC_JACOBI)
         print("prec", prec, info, "fp64 iters", i64["iters"])
         assert info["status"] == 0 and info["true_rel_residual"] <= 1e-10
         r = f - h.apply(x)
