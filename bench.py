@@ -61,6 +61,8 @@ def parse():
     p.add_argument("--check-every", type=int, default=32,
                    help="mixed-precision solves: CG iterations between host checks (one graph replay)")
     p.add_argument("--requests", type=int, default=32, help="qg2_stream: number of requests")
+    p.add_argument("--stream-requests", type=int, default=16,
+                   help="ensemble_512: requests of the served-stream leg that gives the latency figures (0: none)")
     p.add_argument("--budget", type=float, default=200e9, help="qg2_stream: server memory budget (bytes)")
     return p.parse_args()
 
@@ -78,10 +80,27 @@ def apply_flops(m):
     return m.nElsTotal * (8.0 * n1 ** 3 + 32.0 * n1 * n1)
 
 
-def cpu_sample(cfg, f, iters_gpu, sample_iters):
-    """Oracle CSR Jacobi-PCG (reference algorithm), single thread: seconds per
-    iteration on this system, scaled to the GPU's iteration count."""
+def cpu_model():
+    """The host CPU's model string (as lscpu reports it) and its logical CPU count."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_sample(cfg, f, iters_gpu, sample_iters, threads=1):
+    """Oracle CSR Jacobi-PCG (reference algorithm) on `threads` host threads
+    (row-parallel SpMV and vector updates, fixed-block dot products): seconds
+    per iteration on this system, scaled to the reference algorithm's
+    iteration count iters_gpu."""
     from oracle import oracle as O
+    O.set_threads(threads)
     om = O.OracleMesh(cfg["nEls"], cfg["N"], cfg["lo"], cfg["hi"], cfg["periodic"], cfg["bc"])
     t0 = time.perf_counter()
     oh = O.Helm(om, cfg["c_mass"], cfg["c_lap"], cfg["detj_mode"])
@@ -92,6 +111,14 @@ def cpu_sample(cfg, f, iters_gpu, sample_iters):
     t_it = (time.perf_counter() - t0) / max(info["iters"], 1)
     return dict(t_it=t_it, t_solve=t_it * iters_gpu, t_asm=t_asm, iters=info["iters"],
                 nnz=oh.L.orc_helm_nnz(oh.h), n=om.n)
+
+
+def host_threads():
+    """Threads for the all-core CPU baseline: the CPUs this process may run on."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
 
 
 def timed(ctx, stream, steps, fn):
@@ -272,26 +299,46 @@ def run_ensemble(a, prec_name):
                          "all_converged": all(i["status"] == 0 and i["true_rel_residual"] <= 1e-10 for i in r2[-1]),
                          "iters_max": max(i["iters"] for i in r2[-1])}
         config["other_legs"] = legs
+    if a.stream_requests > 0:
+        # the metric's latency half: the served two-layer QG stream (its first
+        # requests, Poisson arrivals at 2000/s, 16^2..96^2 elements at p = 6),
+        # served with this line's precision and preconditioner
+        sr = serve_stream(stream_requests(a.stream_requests), prec, a.budget, pc, prec_name)
+        ss = stream_summary(sr)
+        config.update({"latency_p50_s": ss["latency_p50_s"], "latency_p99_s": ss["latency_p99_s"],
+                       "steps_per_s": ss["steps_per_s"]})
+        ss.update({"requests": a.stream_requests, "rate_per_s": 2000.0})
+        config["stream"] = ss
     cpu = None
     if not a.no_cpu_baseline:
-        cpu = ensemble_cpu(a, cfgs, hs, fs, infos)
+        iters_ref = [i["iters"] for i in res64[-1]] if not a.no_fp64 else [i["iters"] for i in infos]
+        cpu = ensemble_cpu(a, cfgs, hs, fs, iters_ref)
     return base_line(a, value, ms, DTYPE[prec_name], config, roof, cpu)
 
 
-def ensemble_cpu(a, cfgs, hs, fs, infos):
-    """Every (512 / cpu_members)-th member: single-thread oracle CSR PCG time per
-    iteration x the GPU's iteration count for that member, scaled to all members."""
+def ensemble_cpu(a, cfgs, hs, fs, iters_ref):
+    """Every (512 / cpu_members)-th member: oracle CSR Jacobi-PCG time per
+    iteration x the reference algorithm's iteration count for that member
+    (the plain FP64 GPU path's, the same algorithm), scaled to all members;
+    on all host threads, and on one (the serial reference)."""
     step = max(1, len(cfgs) // a.cpu_members)
-    t_all, n_all, samples = 0.0, 0, 0
-    for i in range(0, len(cfgs), step):
-        s = cpu_sample(cfgs[i], fs[i].cpu().numpy(), infos[i]["iters"], a.cpu_iters)
-        t_all += s["t_solve"]
-        n_all += s["n"]
-        samples += 1
-    return {"value": n_all / t_all / 1e9, "unit": "GDOF/s", "cores": 1, "kind": "port",
+    nth = host_threads()
+    out = {}
+    for th in (nth, 1):
+        t_all, n_all, samples = 0.0, 0, 0
+        for i in range(0, len(cfgs), step):
+            s = cpu_sample(cfgs[i], fs[i].cpu().numpy(), iters_ref[i], a.cpu_iters, threads=th)
+            t_all += s["t_solve"]
+            n_all += s["n"]
+            samples += 1
+        out[th] = n_all / t_all / 1e9
+    model, ncpu = cpu_model()
+    return {"value": out[nth], "unit": "GDOF/s", "cores": nth, "kind": "port", "cpu_model": model,
+            "logical_cpus": ncpu, "single_thread_value": out[1],
             "sample": "members 0, %d, 2x%d, ... (%d of %d): %d Jacobi-PCG iterations each of the assembled "
-                      "PETSc-pattern CSR, single thread, scaled to each member's GPU iteration count; CSR "
-                      "assembly not counted" % (step, step, samples, len(cfgs), a.cpu_iters)}
+                      "PETSc-pattern CSR (the reference's algorithm) on %d threads (and on 1), scaled to each "
+                      "member's iteration count with that algorithm; CSR assembly not counted"
+                      % (step, step, samples, len(cfgs), a.cpu_iters, nth)}
 
 
 def run_single(a, prec_name):
@@ -382,23 +429,29 @@ def run_single(a, prec_name):
         config["other_legs"] = legs
     cpu = None
     if not a.no_cpu_baseline:
-        s = cpu_sample(cfg, d_f.cpu().numpy(), info["iters"], a.cpu_iters)
-        cpu = {"value": m.n / s["t_solve"] / 1e9, "unit": "GDOF/s", "cores": 1, "kind": "port",
+        # the reference's algorithm: Jacobi-PCG, as many iterations as the plain FP64 GPU path needs
+        it_ref = info["iters"] if (prec == se_amd.PREC_FP64 and pc == se_amd.PC_JACOBI) else (
+            r64[-1]["iters"] if "fp64_gpu_ms" in config else info["iters"])
+        nth = host_threads()
+        s = cpu_sample(cfg, d_f.cpu().numpy(), it_ref, a.cpu_iters, threads=nth)
+        s1 = cpu_sample(cfg, d_f.cpu().numpy(), it_ref, a.cpu_iters, threads=1)
+        model, ncpu = cpu_model()
+        cpu = {"value": m.n / s["t_solve"] / 1e9, "unit": "GDOF/s", "cores": nth, "kind": "port",
+               "cpu_model": model, "logical_cpus": ncpu, "single_thread_value": m.n / s1["t_solve"] / 1e9,
                "sample": "%d Jacobi-PCG iterations of the assembled PETSc-pattern CSR (%d nnz) on the same "
-                         "system, single thread, scaled to the GPU's %d-iteration solve; CSR assembly %.1f s "
-                         "not counted" % (s["iters"], s["nnz"], info["iters"], s["t_asm"]),
-               "seconds_per_iteration": s["t_it"]}
+                         "system, %d threads (and 1), scaled to the reference algorithm's %d-iteration solve; "
+                         "CSR assembly %.1f s not counted" % (s["iters"], s["nnz"], nth, it_ref, s["t_asm"]),
+               "seconds_per_iteration": s["t_it"], "seconds_per_iteration_1_thread": s1["t_it"]}
     return base_line(a, value, ms, DTYPE[prec_name], config, roof, cpu)
 
 
-def run_stream(a, prec_name):
-    """Served two-layer QG step requests arriving as a Poisson process."""
+def stream_requests(n):
+    """The served stream's first n requests with their seeded initial fields."""
     import numpy as np
 
     import se_amd
     from se_amd import workloads as W
-    prec = {"fp64": se_amd.PREC_FP64, "fp32_ir": se_amd.PREC_FP32_IR, "bf16_ir": se_amd.PREC_BF16_IR}[prec_name]
-    reqs = W.qg2_stream(n_requests=a.requests)
+    reqs = W.qg2_stream(n_requests=n)
     ctx = se_amd.Context(0)
     abc = {}
     for r in reqs:
@@ -409,49 +462,69 @@ def run_stream(a, prec_name):
         X, Y = W.mesh_coords(r["nEls"], r["N"], r["lo"], r["hi"], abc[r["N"]])
         r["init"] = W.qg2_initial(r, X, Y)
     ctx.close()
+    return reqs
 
-    def serve(p):
-        sv = se_amd.Server(memory_budget=a.budget, precision=p, rtol=1e-10)
-        t0 = time.perf_counter()
-        ids = []
-        for r in reqs:  # submit at the arrival times
-            wait = r["t_arrive"] - (time.perf_counter() - t0)
-            if wait > 0:
-                time.sleep(wait)
-            ids.append(sv.submit(r, *r["init"]))
-        res = []
-        for k, i in enumerate(ids):
-            res.append(sv.wait(i, fields=False)[0])
-            if (k + 1) % 8 == 0 or k + 1 == len(ids):
-                print("[qg2_stream %s] %d/%d requests done, %.1f s" % (
-                    {se_amd.PREC_FP64: "fp64", se_amd.PREC_FP32_IR: "fp32", se_amd.PREC_BF16_IR: "bf16"}[p],
-                    k + 1, len(ids), time.perf_counter() - t0), file=sys.stderr, flush=True)
-        wall = time.perf_counter() - t0
-        st = sv.stats()
-        sv.close()
-        lat = np.array([x["t_done"] - x["t_submit"] for x in res])
-        ok = all(x["status"] == 0 for x in res)
-        return dict(wall=wall, lat=lat, ok=ok, steps=sum(x["steps_done"] for x in res),
-                    unknowns=sum(x["unknowns"] for x in res), ticks=st["ticks"],
-                    iters=sum(x["inner_iters"] for x in res))
 
+def serve_stream(reqs, prec, budget, pc=0, tag=""):
+    """Submit the requests at their Poisson arrival times to a fresh server and
+    wait for all; wall time, per-request latency (submit to done), steps."""
+    import numpy as np
+
+    import se_amd
+    sv = se_amd.Server(memory_budget=budget, precision=prec, rtol=1e-10, preconditioner=pc)
+    t0 = time.perf_counter()
+    ids = []
+    for r in reqs:  # submit at the arrival times
+        wait = r["t_arrive"] - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        ids.append(sv.submit(r, *r["init"]))
+    res = []
+    for k, i in enumerate(ids):
+        res.append(sv.wait(i, fields=False)[0])
+        if (k + 1) % 8 == 0 or k + 1 == len(ids):
+            print("[qg2_stream %s] %d/%d requests done, %.1f s" % (tag, k + 1, len(ids), time.perf_counter() - t0),
+                  file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    st = sv.stats()
+    sv.close()
+    lat = np.array([x["t_done"] - x["t_submit"] for x in res])
+    ok = all(x["status"] == 0 for x in res)
+    return dict(wall=wall, lat=lat, ok=ok, steps=sum(x["steps_done"] for x in res),
+                unknowns=sum(x["unknowns"] for x in res), ticks=st["ticks"],
+                iters=sum(x["inner_iters"] for x in res))
+
+
+def stream_summary(r):
+    import numpy as np
+    return {"steps_per_s": r["steps"] / r["wall"], "latency_p50_s": float(np.percentile(r["lat"], 50)),
+            "latency_p99_s": float(np.percentile(r["lat"], 99)), "wall_s": r["wall"],
+            "value": r["unknowns"] / r["wall"] / 1e9, "steps_served": r["steps"], "all_converged": r["ok"],
+            "ticks": r["ticks"]}
+
+
+def run_stream(a, prec_name):
+    """Served two-layer QG step requests arriving as a Poisson process."""
+    import se_amd
+    prec = {"fp64": se_amd.PREC_FP64, "fp32_ir": se_amd.PREC_FP32_IR, "bf16_ir": se_amd.PREC_BF16_IR}[prec_name]
+    pc = pc_code(a, prec_name)
+    reqs = stream_requests(a.requests)
     for _ in range(a.warmup):
-        serve(prec)
-    runs = [serve(prec) for _ in range(a.steps)]
+        serve_stream(reqs, prec, a.budget, pc, prec_name)
+    runs = [serve_stream(reqs, prec, a.budget, pc, prec_name) for _ in range(a.steps)]
     r = min(runs, key=lambda x: x["wall"])
     value = r["unknowns"] / r["wall"] / 1e9
     config = {"workload": "qg2_stream_%d" % a.requests, "requests": a.requests, "rate_per_s": 2000.0,
               "mesh": "m^2 elements, m log-uniform in [16, 96], p = 6, x-periodic channel", "precision": prec_name,
-              "rtol": 1e-10, "memory_budget_bytes": a.budget, "steps_served": r["steps"], "ticks": r["ticks"],
-              "steps_per_s": r["steps"] / r["wall"], "latency_p50_s": float(np.percentile(r["lat"], 50)),
-              "latency_p99_s": float(np.percentile(r["lat"], 99)), "wall_s": r["wall"], "all_converged": r["ok"],
-              "inner_iterations": r["iters"]}
-    if not a.no_fp64 and prec != se_amd.PREC_FP64:
-        r64 = serve(se_amd.PREC_FP64)
+              "preconditioner": "fdm" if pc == se_amd.PC_FDM else "jacobi",
+              "rtol": 1e-10, "memory_budget_bytes": a.budget, "inner_iterations": r["iters"]}
+    config.update(stream_summary(r))
+    if not a.no_fp64 and not (prec == se_amd.PREC_FP64 and pc == se_amd.PC_JACOBI):
+        r64 = serve_stream(reqs, se_amd.PREC_FP64, a.budget, se_amd.PC_JACOBI, "fp64")
         config.update({"fp64_gpu_value": r64["unknowns"] / r64["wall"] / 1e9,
                        "fp64_steps_per_s": r64["steps"] / r64["wall"],
-                       "fp64_latency_p50_s": float(np.percentile(r64["lat"], 50)),
-                       "fp64_latency_p99_s": float(np.percentile(r64["lat"], 99)),
+                       "fp64_latency_p50_s": stream_summary(r64)["latency_p50_s"],
+                       "fp64_latency_p99_s": stream_summary(r64)["latency_p99_s"],
                        "speedup_vs_fp64_gpu": r64["wall"] / r["wall"]})
     return base_line(a, value, r["wall"] * 1e3, DTYPE[prec_name], config, None, None)
 

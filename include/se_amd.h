@@ -366,6 +366,8 @@ typedef struct {
     int precision;        /* SE_PREC_* of the inversions (default SE_PREC_FP32_IR) */
     double rtol;          /* default 1e-10 */
     int detj_mode;        /* SE_DETJ_* of the operators (default literal, as the reference) */
+    int preconditioner;   /* SE_PC_* of the inversions (default: FDM for FP32_IR / BF16_IR; SE_PREC_FP64
+                             with the default solves every inversion alone with the reference's Jacobi-PCG) */
 } se_server_opts;
 typedef struct {
     int nEls[2];

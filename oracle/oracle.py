@@ -23,6 +23,15 @@ def _d(x):
     return np.ascontiguousarray(x, dtype=np.float64)
 
 
+def set_threads(n):
+    """Threads of the oracle's parallel loops (results do not depend on it)."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return int(lib().orc_get_threads())
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -58,6 +67,8 @@ def lib():
             "orc_helm_lift": (None, [vp, _D]),
             "orc_helm_inv_diag": (None, [vp, _D]),
             "orc_helm_matvec": (None, [vp, _D, _D]),
+            "orc_set_threads": (None, [i]),
+            "orc_get_threads": (i, []),
             "orc_helm_rhs": (None, [vp, _D, _D]),
             "orc_helm_pcg": (i, [vp, _D, _D, d, i, i, C.POINTER(i), C.POINTER(d), vp]),
             "orc_helm_solve": (i, [vp, _D, _D, d, i, i, C.POINTER(i), C.POINTER(d)]),

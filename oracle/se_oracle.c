@@ -33,6 +33,9 @@
  *                                Ekman drag, viscosity, topography).
  */
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 #include "se_oracle.h"
@@ -922,8 +925,10 @@ void orc_helm_lift(const orc_helm* h, double* b) { memcpy(b, h->b, sizeof(double
 void orc_helm_inv_diag(const orc_helm* h, double* d) { memcpy(d, h->dinv, sizeof(double) * (size_t)h->m->n); }
 
 /* MatMult_SeqAIJ: y_i = sum over the sorted row of a_ij x_j */
+/* Rows are independent: the OpenMP split changes nothing in any row's sum. */
 void orc_helm_matvec(const orc_helm* h, const double* x, double* y) {
     int i, n = h->m->n;
+#pragma omp parallel for schedule(static)
     for (i = 0; i < n; i++) {
         double s = 0.0;
         long t;
@@ -954,10 +959,22 @@ void orc_helm_rhs(const orc_helm* h, const double* rhs_field, double* f) {
     free(re);
 }
 
+/* Sums over fixed blocks of DOT_BLOCK entries, the block sums added in block
+ * order: the same result for any number of threads. */
+#define DOT_BLOCK 4096
 static double dot(const double* a, const double* b, int n) {
+    int nb = (n + DOT_BLOCK - 1) / DOT_BLOCK, k;
     double s = 0.0;
-    int i;
-    for (i = 0; i < n; i++) s += a[i] * b[i];
+    double* part = (double*)malloc(sizeof(double) * (size_t)(nb > 0 ? nb : 1));
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < nb; k++) {
+        int i, i1 = (k + 1) * DOT_BLOCK < n ? (k + 1) * DOT_BLOCK : n;
+        double t = 0.0;
+        for (i = k * DOT_BLOCK; i < i1; i++) t += a[i] * b[i];
+        part[k] = t;
+    }
+    for (k = 0; k < nb; k++) s += part[k];
+    free(part);
     return s;
 }
 
@@ -967,6 +984,22 @@ static void remove_mean(double* v, int n) {
     for (i = 0; i < n; i++) s += v[i];
     s /= n;
     for (i = 0; i < n; i++) v[i] -= s;
+}
+
+/* number of threads of the oracle's parallel loops (0: the OpenMP default) */
+void orc_set_threads(int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+}
+int orc_get_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 /* Jacobi-preconditioned CG, the build's definition of the reference's
@@ -1001,10 +1034,12 @@ int orc_helm_pcg(const orc_helm* h, const double* f, double* x, double rtol, int
     for (k = 1; k <= maxit; k++) {
         double pq, alpha, rho_new, beta;
         orc_helm_matvec(h, p, q);
+#pragma omp parallel for schedule(static)
         for (i = 0; i < n; i++) q[i] *= sgn;
         pq = dot(p, q, n);
         if (!(pq > 0.0)) { status = isfinite(pq) ? 2 : 3; break; }
         alpha = rho / pq;
+#pragma omp parallel for schedule(static)
         for (i = 0; i < n; i++) { x[i] += alpha * p[i]; r[i] -= alpha * q[i]; }
         rnorm = sqrt(dot(r, r, n));
         if (hist) hist[k] = rnorm / bnorm;
@@ -1017,11 +1052,13 @@ int orc_helm_pcg(const orc_helm* h, const double* f, double* x, double rtol, int
             if (tnorm <= rtol * bnorm) { status = 0; break; }
             memcpy(r, q, sizeof(double) * (size_t)n);
         }
+#pragma omp parallel for schedule(static)
         for (i = 0; i < n; i++) z[i] = sgn * h->dinv[i] * r[i];
         if (null_space) remove_mean(z, n);
         rho_new = dot(r, z, n);
         beta = rho_new / rho;
         rho = rho_new;
+#pragma omp parallel for schedule(static)
         for (i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
     }
     if (k > maxit) k = maxit;

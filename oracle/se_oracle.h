@@ -111,6 +111,11 @@ int orc_qg2_step(const orc_mesh* m, double F1, double F2, double beta, double E2
                  double* omega2, const double* psi1p, const double* omega1p, const double* psi2p,
                  const double* omega2p, double rtol, int maxit, int* iters);
 
+/* threads of the oracle's parallel loops (CSR matvec, vector updates,
+ * fixed-block dot products: results do not depend on the count) */
+void orc_set_threads(int nthreads);
+int orc_get_threads(void);
+
 #ifdef __cplusplus
 }
 #endif

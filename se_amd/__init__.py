@@ -46,7 +46,7 @@ class SolveInfo(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("memory_budget", C.c_double), ("max_active", C.c_int), ("precision", C.c_int),
-                ("rtol", C.c_double), ("detj_mode", C.c_int)]
+                ("rtol", C.c_double), ("detj_mode", C.c_int), ("preconditioner", C.c_int)]
 
 
 class QG2Request(C.Structure):
@@ -704,11 +704,12 @@ class Server:
     """Request server for two-layer QG step requests (own context and thread)."""
 
     def __init__(self, device=0, memory_budget=200e9, max_active=0, precision=PREC_FP32_IR, rtol=1e-10,
-                 detj_mode=DETJ_LITERAL):
+                 detj_mode=DETJ_LITERAL, preconditioner=PC_DEFAULT):
         o = ServerOpts()
         lib().se_server_opts_default(C.byref(o))
         o.memory_budget, o.max_active, o.precision, o.rtol, o.detj_mode = memory_budget, max_active, precision, \
             rtol, detj_mode
+        o.preconditioner = preconditioner
         h = C.c_void_p()
         check(lib().se_server_create(device, C.byref(o), C.byref(h)), "se_server_create")
         self.h = h

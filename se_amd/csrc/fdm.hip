@@ -886,11 +886,12 @@ struct FdmPlan {
         DevBuf d_work;      // (local member, tile) items of the active members
         std::vector<int2> h_work;
     };
-    std::vector<Order> orders;
+    std::vector<Order> orders;   // one per order 2..8 (members may be empty)
     DevBuf d_vwork, d_gwork, d_list;  // active (member, chunk), (member, tile) items; member list
     std::vector<int2> h_vwork, h_gwork;
     std::vector<int> h_list;
-    CgState* h_states = nullptr;  // pinned
+    CgState* h_states = nullptr;  // pinned, h_cap entries
+    int h_cap = 0;
     DevBuf d_states;
     std::vector<char> lists_on, apply_on;  // member flags the device lists were built for
 
@@ -919,6 +920,7 @@ int plan_apply(FdmPlan* pl, bool from_x, const std::vector<char>& on, hipStream_
     const bool same = on == pl->apply_on;
     if (!same) pl->apply_on = on;
     for (auto& o : pl->orders) {
+        if (o.members.empty()) continue;
         if (same) {
             if (o.h_work.empty()) continue;
             const ApplyMember* A = from_x ? o.d_ax.as<ApplyMember>() : o.d_ap.as<ApplyMember>();
@@ -991,20 +993,45 @@ int plan_fdm(FdmPlan* pl, hipStream_t s) {
 
 }  // namespace
 
+// A plan's members can be replaced (the request server re-members one plan
+// for every inversion pass): buffers only grow, so a long-lived plan stops
+// allocating once it has seen its largest set.
+int fdm_plan_set_members(FdmPlan* pl, int count, se_op* const* ops) {
+    SE_HIP(hipStreamSynchronize(pl->ctx->stream));  // nothing of the previous set is in flight
+    pl->ops.assign(ops, ops + count);
+    if (pl->orders.empty()) {
+        pl->orders.resize(7);
+        for (int k = 0; k < 7; k++) pl->orders[k].N = k + 2;
+    }
+    for (auto& o : pl->orders) {
+        o.members.clear();
+        o.h_work.clear();
+    }
+    for (int i = 0; i < count; i++) {
+        const int N = ops[i]->mesh->N;
+        SE_CHECK(N >= 2 && N <= 8, SE_ERR_UNSUPPORTED, "FDM solve built for orders 2..8");
+        pl->orders[N - 2].members.push_back(i);
+    }
+    if (count > pl->h_cap) {
+        if (pl->h_states) (void)hipHostFree(pl->h_states);
+        pl->h_states = nullptr;
+        pl->h_cap = 0;
+        SE_HIP(hipHostMalloc((void**)&pl->h_states, sizeof(CgState) * count, hipHostMallocDefault));
+        pl->h_cap = count;
+    }
+    int rc = pl->d_members.alloc(sizeof(FbMember) * std::max(count, 1));
+    if (!rc) rc = pl->d_tasks.alloc(sizeof(GemmTask) * 4 * std::max(count, 1));
+    if (rc) return rc;
+    pl->prec = -1;
+    pl->lists_on.clear();
+    pl->apply_on.clear();
+    return SE_OK;
+}
+
 int fdm_plan_create(se_ctx* c, int count, se_op* const* ops, FdmPlan** out) {
     std::unique_ptr<FdmPlan> pl(new FdmPlan);
     pl->ctx = c;
-    pl->ops.assign(ops, ops + count);
-    for (int N = 2; N <= 8; N++) {
-        FdmPlan::Order o;
-        o.N = N;
-        for (int i = 0; i < count; i++)
-            if (ops[i]->mesh->N == N) o.members.push_back(i);
-        if (!o.members.empty()) pl->orders.push_back(std::move(o));
-    }
-    SE_HIP(hipHostMalloc((void**)&pl->h_states, sizeof(CgState) * std::max(count, 1), hipHostMallocDefault));
-    int rc = pl->d_members.alloc(sizeof(FbMember) * std::max(count, 1));
-    if (!rc) rc = pl->d_tasks.alloc(sizeof(GemmTask) * 4 * std::max(count, 1));
+    int rc = fdm_plan_set_members(pl.get(), count, ops);
     if (rc) return rc;
     *out = pl.release();
     return SE_OK;
@@ -1064,6 +1091,7 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
     pl->prec = prec;
     // FP64 apply descriptors
     for (auto& od : pl->orders) {
+        if (od.members.empty()) continue;
         std::vector<ApplyMember> ap, ax;
         for (int i : od.members) {
             se_op* op = pl->ops[i];

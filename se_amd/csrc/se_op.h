@@ -66,6 +66,7 @@ int pcg_fdm(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, s
 int solve_pc(const se_solve_opts& o);
 struct FdmPlan;
 int fdm_plan_create(se_ctx* c, int count, se_op* const* ops, FdmPlan** out);
+int fdm_plan_set_members(FdmPlan* pl, int count, se_op* const* ops);
 void fdm_plan_destroy(FdmPlan* pl);
 int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, const se_solve_opts& o,
                    se_solve_info* infos);

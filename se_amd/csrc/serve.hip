@@ -78,10 +78,16 @@ struct se_server {
     std::map<MeshKey, se_mesh*> meshes;
     std::multimap<PoolKey, se_qg2*> pool;
     size_t used = 0;  // bytes of admitted requests and pooled instances
+    // the inversions' solvers, kept for the server's lifetime: one FDM plan
+    // re-membered for every pass, or (Jacobi) one batch per member set
+    FdmPlan* plan = nullptr;
+    std::map<std::vector<se_op*>, se_batch*> batches;
     se_server_stats stats{};
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
     void run();
+    int batch_solve(const std::vector<se_op*>& ops, const se_solve_opts& o, std::vector<se_solve_info>& inf);
+    void drop_solvers(se_qg2* q);
     int tick(std::vector<std::shared_ptr<Request>>& active);
     int admit(std::vector<std::shared_ptr<Request>>& active);
     void retire(const std::shared_ptr<Request>& r, int status);
@@ -89,24 +95,39 @@ struct se_server {
 
 namespace {
 
-int batch_solve(se_ctx* ctx, const std::vector<se_op*>& ops, const se_solve_opts& o, std::vector<se_solve_info>& inf) {
+}  // namespace
+
+// one inversion pass of the admitted requests as one batched solve
+int se_server::batch_solve(const std::vector<se_op*>& ops, const se_solve_opts& o, std::vector<se_solve_info>& inf) {
     inf.assign(ops.size(), se_solve_info{});
     if (ops.empty()) return SE_OK;
-    se_batch* b = nullptr;
-    int rc = se_batch_create(ctx, (int)ops.size(), ops.data(), &b);
-    if (rc) return rc;
     std::vector<const double*> f(ops.size());
     std::vector<double*> x(ops.size());
     for (size_t i = 0; i < ops.size(); i++) {
         f[i] = ops[i]->f.as<double>();
         x[i] = ops[i]->x.as<double>();
     }
-    rc = se_batch_pcg_dev(b, f.data(), x.data(), &o, inf.data());
-    se_batch_destroy(b);
-    return rc;
+    if (solve_pc(o) == SE_PC_FDM) {
+        int rc = plan ? fdm_plan_set_members(plan, (int)ops.size(), ops.data())
+                      : fdm_plan_create(ctx, (int)ops.size(), ops.data(), &plan);
+        if (rc) return rc;
+        return fdm_plan_solve(plan, f.data(), x.data(), o, inf.data());
+    }
+    // Jacobi inner CG: a batch per member set, kept (graphs and launch
+    // shapes stay tuned while the admitted set does not change)
+    auto it = batches.find(ops);
+    if (it == batches.end()) {
+        if (batches.size() >= 8) {  // bound the cache: drop the oldest sets
+            for (auto& kv : batches) se_batch_destroy(kv.second);
+            batches.clear();
+        }
+        se_batch* b = nullptr;
+        int rc = se_batch_create(ctx, (int)ops.size(), ops.data(), &b);
+        if (rc) return rc;
+        it = batches.emplace(ops, b).first;
+    }
+    return se_batch_pcg_dev(it->second, f.data(), x.data(), &o, inf.data());
 }
-
-}  // namespace
 
 int se_server::admit(std::vector<std::shared_ptr<Request>>& active) {
     for (;;) {
@@ -143,6 +164,7 @@ int se_server::admit(std::vector<std::shared_ptr<Request>>& active) {
             while (used + need > (size_t)opts.memory_budget && !pool.empty()) {
                 auto victim = pool.begin();
                 used -= qg2_bytes(victim->second->mesh->n, victim->second->mesh->nV);
+                drop_solvers(victim->second);
                 se_qg2_destroy(victim->second);
                 pool.erase(victim);
             }
@@ -193,6 +215,24 @@ int se_server::admit(std::vector<std::shared_ptr<Request>>& active) {
     }
 }
 
+// forget every solver that holds an operator of q (before q is destroyed)
+void se_server::drop_solvers(se_qg2* q) {
+    if (plan) {
+        fdm_plan_destroy(plan);
+        plan = nullptr;
+    }
+    for (auto it = batches.begin(); it != batches.end();) {
+        bool uses = false;
+        for (se_op* op : it->first) uses = uses || op == qg2_pass_op(q, 0) || op == qg2_pass_op(q, 1);
+        if (uses) {
+            se_batch_destroy(it->second);
+            it = batches.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
 void se_server::retire(const std::shared_ptr<Request>& r, int status) {
     // server thread, with mu held
     r->res.status = status;
@@ -218,6 +258,8 @@ int se_server::tick(std::vector<std::shared_ptr<Request>>& active) {
     se_solve_opts_default(&o);
     o.precision = opts.precision;
     o.rtol = opts.rtol;
+    o.preconditioner = opts.preconditioner;
+    const bool one_by_one = opts.precision == SE_PREC_FP64 && solve_pc(o) == SE_PC_JACOBI;
     std::vector<int> failed(active.size(), 0);
     for (size_t i = 0; i < active.size(); i++) {
         int rc = qg2_phase_advect(active[i]->q, active[i]->dt);
@@ -236,8 +278,8 @@ int se_server::tick(std::vector<std::shared_ptr<Request>>& active) {
             }
         if (ops.empty()) break;
         std::vector<se_solve_info> inf;
-        int rc = opts.precision == SE_PREC_FP64 ? SE_OK : batch_solve(ctx, ops, o, inf);
-        if (opts.precision == SE_PREC_FP64) {  // the plain path: one solve after another
+        int rc = one_by_one ? SE_OK : batch_solve(ops, o, inf);
+        if (one_by_one) {  // the plain path: one solve after another
             inf.assign(ops.size(), se_solve_info{});
             for (size_t j = 0; j < ops.size() && !rc; j++)
                 rc = se_helm_pcg_dev(ops[j], ops[j]->f.as<double>(), ops[j]->x.as<double>(), &o, &inf[j]);
@@ -312,6 +354,7 @@ extern "C" void se_server_opts_default(se_server_opts* o) {
     o->precision = SE_PREC_FP32_IR;
     o->rtol = 1e-10;
     o->detj_mode = SE_DETJ_LITERAL;
+    o->preconditioner = SE_PC_DEFAULT;
 }
 
 extern "C" int se_server_create(int device, const se_server_opts* opts, se_server** out) {
@@ -400,6 +443,8 @@ extern "C" int se_server_destroy(se_server* sv) {
     sv->cv_work.notify_all();
     if (sv->th.joinable()) sv->th.join();
     (void)hipSetDevice(sv->ctx->device);
+    if (sv->plan) fdm_plan_destroy(sv->plan);
+    for (auto& kv : sv->batches) se_batch_destroy(kv.second);
     for (auto& p : sv->pool) se_qg2_destroy(p.second);
     for (auto& m : sv->meshes) se_mesh_destroy(m.second);
     se_ctx_destroy(sv->ctx);
