@@ -274,6 +274,21 @@ int se_sl_advect(se_mesh* mesh, const double* q, int nDofs, const double* vel, d
                  int32_t* dep_el);
 int se_sl_advect_dev(se_mesh* mesh, const double* d_q, int nDofs, const double* d_vel, double dt,
                      double* d_qSL, int32_t* d_dep_el);
+/* Interpolation of the advected field.  DEFAULT: the reference's choice,
+ * bicubic Lagrange on the 4 x 4 node stencil when N < 3, GLL cardinal
+ * functions otherwise (Advector's constructors set cubicInterp = N < 3,
+ * Advector.cpp:33,60); GLL / BICUBIC: as a caller that sets cubicInterp
+ * itself (QuasiGeostrophicSpectral.cpp:84).  The mesh keeps the mode for its
+ * SL steps (se_sl_advect*), velocities are always GLL-interpolated. */
+enum { SE_INTERP_DEFAULT = 0, SE_INTERP_GLL = 1, SE_INTERP_BICUBIC = 2 };
+int se_mesh_set_sl_interp(se_mesh* mesh, int mode);
+/* Values of a field (nV x nDofs, nDofs 1..3) at npts points (xy: npts x 2)
+ * [ref: Field::InterpGlobal Field.cpp:57-76 (GLL), Advector::Interpolate
+ *  Advector.cpp:278-337 (BICUBIC)]; out: npts x nDofs */
+int se_interp_points(se_mesh* mesh, const double* vals, int nDofs, int npts, const double* xy, int mode,
+                     double* out);
+int se_interp_points_dev(se_mesh* mesh, const double* d_vals, int nDofs, int npts, const double* d_xy, int mode,
+                         double* d_out);
 /* [ref: Advector::SecondOrder + ExtrapolateVelocities Advector.cpp:203-246,339-349] */
 int se_sl_advect2(se_mesh* mesh, const double* q, const double* qPrev, int nDofs, const double* vel,
                   const double* velPrev, double dt, double* qSL);

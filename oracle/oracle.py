@@ -77,6 +77,8 @@ def lib():
             "orc_field_periodic_update": (None, [vp, _D, i]),
             "orc_helm_apply_ideal": (None, [vp, d, d, i, _D, _D]),
             "orc_advect_first": (None, [vp, _D, i, _D, d, _D, vp, vp]),
+            "orc_advect_first_interp": (None, [vp, _D, i, _D, d, _D, i]),
+            "orc_interp_bicubic": (None, [vp, _D, i, _D, _D]),
             "orc_advect_second": (None, [vp, _D, _D, i, _D, _D, d, _D]),
             "orc_sl_points": (None, [vp, i, _D, _D, d, _D]),
             "orc_genvel": (None, [vp, _D, _D]),
@@ -272,6 +274,18 @@ class OracleMesh(_MeshBase):
         self.nEls, self.lo, self.hi, self.periodic, self.bc = tuple(nEls), tuple(lo), tuple(hi), tuple(periodic), tuple(bc)
         self._init_sizes()
 
+
+    def interp_bicubic(self, vals, nDofs, xy):
+        """Advector::Interpolate (bicubic on the 4 x 4 node stencil) at one point."""
+        out = np.zeros(3)
+        self.L.orc_interp_bicubic(self.h, _d(vals), nDofs, _d(xy), out)
+        return out[:nDofs]
+
+    def advect_first_interp(self, q, nDofs, vel, dt, interp):
+        """First-order advection with interp 0 (reference default), 1 GLL, 2 bicubic."""
+        qsl = np.zeros(self.nV * nDofs)
+        self.L.orc_advect_first_interp(self.h, _d(q), nDofs, _d(vel), dt, qsl, int(interp))
+        return qsl
     def _fn(self, name):
         return getattr(self.L, "orc_" + name)
 

@@ -1242,6 +1242,29 @@ static void integrate_rk2(const orc_mesh* m, double dt, const double* vel0, cons
 
 /* Advector::FirstOrder, Advector.cpp:181-201 (q: nV x nDofs, vel: nV x 2).
  * dep_el / dep_xy (nullable) receive the departure element and point. */
+/* Advector::Interpolate at one point (exported for the drop-in check) */
+void orc_interp_bicubic(const orc_mesh* m, const double* vals, int nDofs, const double* coord, double* psi) {
+    interp_bicubic(m, vals, nDofs, coord, psi);
+}
+
+/* Advector::FirstOrder with the interpolation of the advected field chosen
+ * by `interp`: 0 the reference's default (bicubic when N < 3, the
+ * constructors' cubicInterp), 1 GLL (Field::InterpGlobal), 2 bicubic
+ * (cubicInterp set true by the caller, QuasiGeostrophicSpectral.cpp:84). */
+void orc_advect_first_interp(const orc_mesh* m, const double* q, int nDofs, const double* vel,
+                             double dt, double* qSL, int interp) {
+    int v, d;
+    double c[2], val[3];
+    const int cubic = interp == 2 || (interp == 0 && m->N < 3);
+    for (v = 0; v < m->nV; v++) {
+        integrate_rk2(m, dt, vel, vel, &m->verts[2 * (size_t)v], c);
+        if (cubic) interp_bicubic(m, q, nDofs, c, val);
+        else orc_interp_global(m, q, nDofs, c, val);
+        for (d = 0; d < nDofs; d++) qSL[(size_t)v * nDofs + d] = val[d];
+    }
+    if (m->periodic[0] || m->periodic[1]) orc_field_periodic_update(m, qSL, nDofs);
+}
+
 void orc_advect_first(const orc_mesh* m, const double* q, int nDofs, const double* vel,
                       double dt, double* qSL, int* dep_el, double* dep_xy) {
     int v, d;

@@ -87,6 +87,9 @@ void orc_helm_apply_ideal(const orc_mesh* m, double c_mass, double c_lap, int de
                           const double* x, double* y);
 
 /* ---- a11-a16: semi-Lagrangian step (Advector.cpp, Field.cpp:57-118, Utils.cpp:23-40) ---- */
+void   orc_interp_bicubic(const orc_mesh* m, const double* vals, int nDofs, const double* coord, double* psi);
+void   orc_advect_first_interp(const orc_mesh* m, const double* q, int nDofs, const double* vel, double dt,
+                               double* qSL, int interp);
 void   orc_advect_first(const orc_mesh* m, const double* q, int nDofs, const double* vel,
                         double dt, double* qSL, int* dep_el, double* dep_xy);
 void   orc_advect_second(const orc_mesh* m, const double* q, const double* qPrev, int nDofs,

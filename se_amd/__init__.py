@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libse_amd.so")
 DETJ_LITERAL, DETJ_PHYSICAL = 0, 1
 PREC_FP64, PREC_FP32_IR, PREC_BF16_IR = 0, 1, 2
 PC_DEFAULT, PC_JACOBI, PC_FDM = 0, 1, 2
+INTERP_DEFAULT, INTERP_GLL, INTERP_BICUBIC = 0, 1, 2
 SOLVE_CONVERGED, SOLVE_MAXIT, SOLVE_BREAKDOWN, SOLVE_NONFINITE, SOLVE_STAGNATED = 0, 1, 2, 3, 4
 
 _D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -157,6 +158,9 @@ def lib():
             "se_sl_advect_dev": (i, [vp, vp, i, vp, d, vp, vp]),
             "se_sl_advect2": (i, [vp, _D, _D, i, _D, _D, d, _D]),
             "se_sl_advect2_dev": (i, [vp, vp, vp, i, vp, vp, d, vp]),
+            "se_mesh_set_sl_interp": (i, [vp, i]),
+            "se_interp_points": (i, [vp, _D, i, i, _D, i, _D]),
+            "se_interp_points_dev": (i, [vp, vp, i, i, vp, i, vp]),
             "se_genvel": (i, [vp, _D, _D]),
             "se_genvel_dev": (i, [vp, vp, vp]),
             "se_calc_timestep": (i, [vp, _D, d, C.POINTER(d)]),
@@ -389,6 +393,17 @@ class Mesh:
         v = np.zeros(2 * self.nV)
         check(lib().se_mesh_verts(self.h, v), "se_mesh_verts")
         return v.reshape(self.nV, 2)
+
+    def set_sl_interp(self, mode):
+        """Interpolation of the advected field in the SL steps (INTERP_*)."""
+        check(lib().se_mesh_set_sl_interp(self.h, mode), "se_mesh_set_sl_interp")
+
+    def interp_points(self, vals, xy, nDofs=1, mode=INTERP_GLL):
+        """Field values at points xy (k x 2): GLL (Field::InterpGlobal) or bicubic."""
+        xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+        out = np.zeros(len(xy) * nDofs)
+        check(lib().se_interp_points(self.h, _d(vals), nDofs, len(xy), xy.ravel(), mode, out), "se_interp_points")
+        return out.reshape(len(xy), nDofs)
 
     def advect(self, q, vel, dt, nDofs=1, want_dep=True):
         qsl = np.zeros(self.nV * nDofs)

@@ -90,6 +90,7 @@ struct se_mesh {
     int nV = 0, nElsTotal = 0, n = 0, size0 = 0;
     int periodic[2] = {0, 0};
     int bc[4] = {0, 0, 0, 0};      // effective bottom, top, left, right
+    int sl_interp = 0;             // SE_INTERP_* of the advected field in the SL steps
     int off[2] = {0, 0};           // first unknown column / row of the field grid
     int nU[2] = {0, 0};            // unknown grid extents
     double min[2] = {0, 0}, max[2] = {0, 0}, dx[2] = {0, 0};

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <string>
 
 #include "glibc_pow.h"
 #include "se_common.h"
@@ -22,6 +23,7 @@ namespace {
 
 struct SLGeom {
     int N, nEls0, nEls1, nv0, nv1, per0, per1;
+    int cubic;  // the advected field by bicubic Lagrange interpolation (Advector::Interpolate)
     double min0, min1, max0, max1, dx0, dx1;
     const double* xk;   // abscissae
     const double* D;    // derivative matrix
@@ -183,7 +185,7 @@ __device__ __forceinline__ void interp_bicubic(const SLGeom& g, const double* __
 template <int MAXN, int ND>
 __device__ __forceinline__ void interp_any(const SLGeom& g, const double* __restrict__ vals, int nd, double cx,
                                            double cy, double (&out)[ND]) {
-    if (g.N < 3) interp_bicubic<ND>(g, vals, nd, cx, cy, out);
+    if (g.cubic) interp_bicubic<ND>(g, vals, nd, cx, cy, out);
     else interp_global<MAXN, ND>(g, vals, nd, cx, cy, out);
 }
 
@@ -268,6 +270,20 @@ __global__ void k_periodic_copy(int nv0, int nv1, int per0, int per1, int nd, do
     for (int d = 0; d < nd; d++) f[(size_t)v * nd + d] = f[(size_t)src * nd + d];
 }
 
+// Field::InterpGlobal (mode GLL) or Advector::Interpolate (mode BICUBIC) at
+// arbitrary points, one thread per point; xy: npts x 2, out: npts x nd
+template <int MAXN>
+__global__ __launch_bounds__(256) void k_interp_points(SLGeom g, const double* __restrict__ vals, int nd, int npts,
+                                                       const double* __restrict__ xy, int cubic,
+                                                       double* __restrict__ out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npts) return;
+    double v[3];
+    if (cubic) interp_bicubic<3>(g, vals, nd, xy[2 * (size_t)p], xy[2 * (size_t)p + 1], v);
+    else interp_global<MAXN, 3>(g, vals, nd, xy[2 * (size_t)p], xy[2 * (size_t)p + 1], v);
+    for (int d = 0; d < nd; d++) out[(size_t)p * nd + d] = v[d];
+}
+
 // QuasiGeostrophicEqn::GenVel via Field::InterpDerivsGlobal and
 // Mesh::ShapeFuncDerivsAtCoord (Mesh.cpp:252-285).  At a vertex both local
 // coordinates are GLL roots, so the local derivative rows are columns of D
@@ -339,6 +355,8 @@ __global__ void k_maxvel2(int nV, const double* __restrict__ vel, unsigned long 
 SLGeom sl_geom(const se_mesh* m) {
     SLGeom g;
     g.N = m->N;
+    // Advector's cubicInterp: N < 3 unless the caller chose (Advector.cpp:33,60)
+    g.cubic = m->sl_interp == SE_INTERP_BICUBIC || (m->sl_interp == SE_INTERP_DEFAULT && m->N < 3);
     g.nEls0 = m->nEls[0];
     g.nEls1 = m->nEls[1];
     g.nv0 = m->nVerts[0];
@@ -395,19 +413,25 @@ extern "C" int se_sl_advect_dev(se_mesh* m, const double* d_q, int nd, const dou
     return SE_OK;
 }
 
-extern "C" int se_sl_advect2_dev(se_mesh* m, const double* d_q, const double* d_qp, int nd, const double* d_vel,
-                                 const double* d_velp, double dt, double* d_qsl) {
-    SE_CHECK(m && d_q && d_qp && d_vel && d_velp && d_qsl, SE_ERR_ARG, "se_sl_advect2_dev: NULL argument");
-    SE_CHECK(nd >= 1 && nd <= 3, SE_ERR_ARG, "se_sl_advect2_dev: nDofs must be 1..3");
-    SE_CHECK(m->N <= 12, SE_ERR_UNSUPPORTED, "se_sl_advect2_dev: order above 12");
+namespace {
+
+int sl_second_check(const se_mesh* m, const void* d_q, const void* d_qp, int nd, const void* d_vel,
+                    const void* d_velp, const void* d_qsl, const char* who) {
+    SE_CHECK(m && d_q && d_qp && d_vel && d_velp && d_qsl, SE_ERR_ARG, std::string(who) + ": NULL argument");
+    SE_CHECK(nd >= 1 && nd <= 3, SE_ERR_ARG, std::string(who) + ": nDofs must be 1..3");
+    SE_CHECK(m->N <= 12, SE_ERR_UNSUPPORTED, std::string(who) + ": order above 12");
     SE_CHECK(m->N >= 3 || (m->nVerts[0] >= 4 && m->nVerts[1] >= 4), SE_ERR_UNSUPPORTED,
              "bicubic advection (N < 3) needs at least 4 nodes per direction");
+    return SE_OK;
+}
+
+// the second-order step with the three extrapolated velocity fields in
+// vp1 | vph | vmh (2 nV doubles each)
+int sl_second(se_mesh* m, const double* d_q, const double* d_qp, int nd, const double* d_vel, const double* d_velp,
+              double dt, double* d_qsl, double* vp1) {
     SLGeom g = sl_geom(m);
     hipStream_t s = m->ctx->stream;
-    size_t n2 = 2 * (size_t)m->nV;
-    int rc = m->d_sl.alloc(3 * n2 * sizeof(double));
-    if (rc) return rc;
-    double* vp1 = m->d_sl.as<double>();
+    const size_t n2 = 2 * (size_t)m->nV;
     double* vph = vp1 + n2;
     double* vmh = vph + n2;
     hipLaunchKernelGGL(k_extrap, dim3(blocks_for((int)n2)), dim3(256), 0, s, (int)n2, d_vel, d_velp, vp1, vph, vmh);
@@ -418,6 +442,18 @@ extern "C" int se_sl_advect2_dev(se_mesh* m, const double* d_q, const double* d_
                            m->periodic[0], m->periodic[1], nd, d_qsl);
     SE_HIP(hipGetLastError());
     return SE_OK;
+}
+
+}  // namespace
+
+extern "C" int se_sl_advect2_dev(se_mesh* m, const double* d_q, const double* d_qp, int nd, const double* d_vel,
+                                 const double* d_velp, double dt, double* d_qsl) {
+    int rc = sl_second_check(m, d_q, d_qp, nd, d_vel, d_velp, d_qsl, "se_sl_advect2_dev");
+    if (rc) return rc;
+    size_t n2 = 2 * (size_t)m->nV;
+    rc = m->d_sl.alloc(3 * n2 * sizeof(double));
+    if (rc) return rc;
+    return sl_second(m, d_q, d_qp, nd, d_vel, d_velp, dt, d_qsl, m->d_sl.as<double>());
 }
 
 extern "C" int se_genvel_dev(se_mesh* m, const double* d_psi, double* d_vel) {
@@ -520,5 +556,51 @@ extern "C" int se_calc_timestep(se_mesh* m, const double* vel, double courant, d
     SLScratch b;
     int rc = up(s, b.a, vel, (size_t)m->nV * 2 * sizeof(double));
     if (!rc) rc = se_calc_timestep_dev(m, b.a.as<double>(), courant, dt);
+    return rc;
+}
+
+extern "C" int se_mesh_set_sl_interp(se_mesh* m, int mode) {
+    SE_CHECK(m, SE_ERR_ARG, "se_mesh_set_sl_interp: NULL argument");
+    SE_CHECK(mode == SE_INTERP_DEFAULT || mode == SE_INTERP_GLL || mode == SE_INTERP_BICUBIC, SE_ERR_ARG,
+             "se_mesh_set_sl_interp: mode must be SE_INTERP_DEFAULT, SE_INTERP_GLL or SE_INTERP_BICUBIC");
+    const bool cubic = mode == SE_INTERP_BICUBIC || (mode == SE_INTERP_DEFAULT && m->N < 3);
+    SE_CHECK(!cubic || (m->nVerts[0] >= 4 && m->nVerts[1] >= 4), SE_ERR_UNSUPPORTED,
+             "bicubic interpolation needs at least 4 nodes per direction");
+    m->sl_interp = mode;
+    return SE_OK;
+}
+
+extern "C" int se_interp_points_dev(se_mesh* m, const double* d_vals, int nd, int npts, const double* d_xy, int mode,
+                                    double* d_out) {
+    SE_CHECK(m && (npts == 0 || (d_vals && d_xy && d_out)), SE_ERR_ARG, "se_interp_points_dev: NULL argument");
+    SE_CHECK(nd >= 1 && nd <= 3 && npts >= 0, SE_ERR_ARG, "se_interp_points_dev: nDofs 1..3, npts >= 0");
+    SE_CHECK(mode == SE_INTERP_DEFAULT || mode == SE_INTERP_GLL || mode == SE_INTERP_BICUBIC, SE_ERR_ARG,
+             "se_interp_points_dev: bad mode");
+    SE_CHECK(m->N <= 12, SE_ERR_UNSUPPORTED, "se_interp_points_dev: order above 12");
+    const int cubic = mode == SE_INTERP_BICUBIC || (mode == SE_INTERP_DEFAULT && m->N < 3);
+    SE_CHECK(!cubic || (m->nVerts[0] >= 4 && m->nVerts[1] >= 4), SE_ERR_UNSUPPORTED,
+             "bicubic interpolation needs at least 4 nodes per direction");
+    if (npts == 0) return SE_OK;
+    SLGeom g = sl_geom(m);
+    hipLaunchKernelGGL(k_interp_points<12>, dim3(blocks_for(npts)), dim3(256), 0, m->ctx->stream, g, d_vals, nd, npts,
+                       d_xy, cubic, d_out);
+    SE_HIP(hipGetLastError());
+    return SE_OK;
+}
+
+extern "C" int se_interp_points(se_mesh* m, const double* vals, int nd, int npts, const double* xy, int mode,
+                                double* out) {
+    SE_CHECK(m && (npts == 0 || (vals && xy && out)), SE_ERR_ARG, "se_interp_points: NULL argument");
+    SE_CHECK(nd >= 1 && nd <= 3 && npts >= 0, SE_ERR_ARG, "se_interp_points: nDofs 1..3, npts >= 0");
+    if (npts == 0) return SE_OK;
+    SE_HIP(hipSetDevice(m->ctx->device));
+    hipStream_t s = m->ctx->stream;
+    SLScratch b;
+    const size_t nv = (size_t)m->nV * nd;
+    int rc = up(s, b.a, vals, nv * sizeof(double));
+    if (!rc) rc = up(s, b.b, xy, 2 * (size_t)npts * sizeof(double));
+    if (!rc) rc = b.c.alloc((size_t)npts * nd * sizeof(double));
+    if (!rc) rc = se_interp_points_dev(m, b.a.as<double>(), nd, npts, b.b.as<double>(), mode, b.c.as<double>());
+    if (!rc) rc = down(s, out, b.c, (size_t)npts * nd * sizeof(double));
     return rc;
 }

@@ -239,33 +239,67 @@ void PoissonEqn::Solve(bool removeNullSpace) {
 }
 
 // ---- Advector (src/base/Advector.cpp:26-349) ----------------------------------
+// names of the reference's fields: "<field>-SL", "<vel>_n+1", "<vel>_n+1/2", "<vel>_n-1/2"
 Advector::Advector(Field* _field, Field* _velocity)
-    : field(_field), velocity(_velocity), fieldMinusOne(nullptr), velMinusOne(nullptr), secondOrder(false) {
-    fieldSL = new Field(field->name + "SL", field->mesh, field->nDofs, field->bcs);
+    : field(_field), velocity(_velocity), fieldMinusOne(nullptr), velMinusOne(nullptr), velPlusOne(nullptr),
+      velPlusHalf(nullptr), velMinusHalf(nullptr), secondOrder(false), cubicInterp(_field->mesh->N < 3) {
+    fieldSL = new Field(field->name + "-SL", field->mesh, field->nDofs, field->bcs);
+    fieldSL->Copy(field);  // the boundary values, as the reference
 }
 
 Advector::Advector(Field* _field, Field* _velocity, Field* _fieldMinusOne, Field* _velMinusOne)
     : field(_field), velocity(_velocity), fieldMinusOne(_fieldMinusOne), velMinusOne(_velMinusOne),
-      secondOrder(true) {
-    fieldSL = new Field(field->name + "SL", field->mesh, field->nDofs, field->bcs);
+      secondOrder(true), cubicInterp(_field->mesh->N < 3) {
+    fieldSL = new Field(field->name + "-SL", field->mesh, field->nDofs, field->bcs);
+    fieldSL->Copy(field);
+    velPlusOne = new Field(velocity->name + "_n+1", velocity->mesh, velocity->nDofs, velocity->bcs);
+    velPlusHalf = new Field(velocity->name + "_n+1/2", velocity->mesh, velocity->nDofs, velocity->bcs);
+    velMinusHalf = new Field(velocity->name + "_n-1/2", velocity->mesh, velocity->nDofs, velocity->bcs);
 }
 
-Advector::~Advector() { delete fieldSL; }
+Advector::~Advector() {
+    delete fieldSL;
+    if (secondOrder) {
+        delete velPlusOne;
+        delete velPlusHalf;
+        delete velMinusHalf;
+    }
+}
 
 void Advector::Advect(double dt) {
     se_mesh* m = device_mesh(field);
+    int rc = se_mesh_set_sl_interp(m, cubicInterp ? SE_INTERP_BICUBIC : SE_INTERP_GLL);
+    if (rc) die("se_mesh_set_sl_interp", rc);
     std::vector<double> q, v, out((size_t)field->mesh->nVertsTotal * field->nDofs);
     gather(field, q);
     gather(velocity, v);
-    int rc;
     if (secondOrder) {
         std::vector<double> qp, vp;
         gather(fieldMinusOne, qp);
         gather(velMinusOne, vp);
+        // the extrapolated velocities the reference leaves in its fields
+        // (ExtrapolateVelocities, Advector.cpp:339-349); the device step
+        // forms the same values for itself
+        for (int i = 0; i < velocity->mesh->nVertsTotal; i++)
+            for (int d = 0; d < velocity->nDofs; d++) {
+                const double a = velocity->vals[i][d], b = velMinusOne->vals[i][d];
+                velPlusOne->vals[i][d] = 2.0 * a - 1.0 * b;
+                velPlusHalf->vals[i][d] = 1.5 * a - 0.5 * b;
+                velMinusHalf->vals[i][d] = 0.5 * a + 0.5 * b;
+            }
         rc = se_sl_advect2(m, q.data(), qp.data(), field->nDofs, v.data(), vp.data(), dt, out.data());
     } else {
         rc = se_sl_advect(m, q.data(), field->nDofs, v.data(), dt, out.data(), nullptr);
     }
     if (rc) die("se_sl_advect", rc);
     scatter(out, fieldSL);
+}
+
+void Advector::Interpolate(Field* f, double* coord, double* psi) {
+    se_mesh* m = device_mesh(f);
+    std::vector<double> vals;
+    gather(f, vals);
+    const double xy[2] = {coord[0], coord[1]};
+    int rc = se_interp_points(m, vals.data(), f->nDofs, 1, xy, SE_INTERP_BICUBIC, psi);
+    if (rc) die("se_interp_points", rc);
 }

@@ -104,6 +104,48 @@ int main() {
     std::vector<double> qp = values(qPrev), vp = values(velPrev);
     orc_advect_second(om, qv.data(), qp.data(), 1, vv.data(), vp.data(), 0.01, qs.data());
     compare("Advector::Advect (2nd)", values(a2->fieldSL), qs, 1e-12);
+    {
+        // the extrapolated velocities left in the public fields (Advector.cpp:339-349)
+        std::vector<double> e1(2 * (size_t)nV), eh(2 * (size_t)nV), em(2 * (size_t)nV);
+        for (size_t i = 0; i < e1.size(); i++) {
+            e1[i] = 2.0 * vv[i] - 1.0 * vp[i];
+            eh[i] = 1.5 * vv[i] - 0.5 * vp[i];
+            em[i] = 0.5 * vv[i] + 0.5 * vp[i];
+        }
+        compare("Advector::velPlusOne", values(a2->velPlusOne), e1, 0.0);
+        compare("Advector::velPlusHalf", values(a2->velPlusHalf), eh, 0.0);
+        compare("Advector::velMinusHalf", values(a2->velMinusHalf), em, 0.0);
+        if (a2->fieldSL->name != "q-SL") {
+            std::printf("FAIL  fieldSL name %s (expected q-SL)\n", a2->fieldSL->name.c_str());
+            failures++;
+        }
+    }
+
+    // cubicInterp: the reference's default (N < 3) is off here; a caller may
+    // set it (QuasiGeostrophicSpectral.cpp:84) -> bicubic field interpolation
+    {
+        Advector* a3 = new Advector(q, vel);
+        if (a3->cubicInterp) {
+            std::printf("FAIL  cubicInterp defaults to true at N = %d\n", N);
+            failures++;
+        }
+        a3->cubicInterp = true;
+        a3->Advect(0.01);
+        orc_advect_first_interp(om, qv.data(), 1, vv.data(), 0.01, qs.data(), 2);
+        compare("Advector::Advect (1st, cubicInterp)", values(a3->fieldSL), qs, 1e-12);
+        // Advector::Interpolate at scattered points, against the oracle's bicubic
+        std::vector<double> got, want;
+        for (int k = 0; k < 40; k++) {
+            double c[2] = {lo[0] + (hi[0] - lo[0]) * (0.013 + 0.0247 * k), lo[1] + (hi[1] - lo[1]) * (0.97 - 0.0233 * k)};
+            double p1 = 0.0, p0 = 0.0;
+            a3->Interpolate(q, c, &p1);
+            orc_interp_bicubic(om, qv.data(), 1, c, &p0);
+            got.push_back(p1);
+            want.push_back(p0);
+        }
+        compare("Advector::Interpolate", got, want, 1e-13);
+        delete a3;
+    }
 
     // Poisson (barotropic: c = -1, Dirichlet walls)
     Field* chi = new Field("chi", mesh, 1, bcs);

@@ -285,3 +285,41 @@ def test_qg_channel_advection_vs_oracle(gpu_ctx):
     qo, depo, xy = om.advect_first(q, 1, vel.ravel(), dt)
     assert np.array_equal(dep, depo), np.flatnonzero(dep != depo)[:20]
     report_sl("qg_channel", qg, qo)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_interp_points_bit_exact(gpu_ctx, name):
+    """Field::InterpGlobal (GLL) and Advector::Interpolate (bicubic) at scattered
+    points, against the oracle's restatements (pinned on the golden vectors
+    through the SL steps): bit for bit."""
+    d = load_mesh_case(name)
+    m = se_amd.Mesh(gpu_ctx, *mesh_args(d))
+    om = O.OracleMesh(*mesh_args(d))
+    rng = np.random.default_rng(13)
+    lo, hi = np.array(m.lo), np.array(m.hi)
+    xy = lo + (hi - lo) * rng.uniform(size=(64, 2))
+    xy[:4] = [lo, hi, [lo[0], hi[1]], [hi[0], lo[1]]]  # corners: the edge rules of GetEl
+    q = d["q"]
+    g = m.interp_points(q, xy, 1, se_amd.INTERP_GLL)[:, 0]
+    assert np.array_equal(g, [om.interp_global(q, 1, p)[0] for p in xy])
+    if min(m.nVerts) >= 4:
+        b = m.interp_points(q, xy, 1, se_amd.INTERP_BICUBIC)[:, 0]
+        assert np.array_equal(b, [om.interp_bicubic(q, 1, p)[0] for p in xy])
+    v = m.interp_points(d["vel"], xy, 2, se_amd.INTERP_GLL)
+    assert np.array_equal(v, [om.interp_global(d["vel"], 2, p) for p in xy])
+
+
+@pytest.mark.parametrize("mode", [se_amd.INTERP_GLL, se_amd.INTERP_BICUBIC], ids=["gll", "bicubic"])
+def test_advect_interpolation_modes(gpu_ctx, mode):
+    """A caller's cubicInterp choice (QuasiGeostrophicSpectral.cpp:84) on a p = 6
+    channel: the SL step with that interpolation equals the oracle's, bit for bit."""
+    d = load_mesh_case("chan_6x5_p6")
+    m = se_amd.Mesh(gpu_ctx, *mesh_args(d))
+    om = O.OracleMesh(*mesh_args(d))
+    m.set_sl_interp(mode)
+    q, _ = m.advect(d["q"], d["vel"], float(d["dt"]))
+    ref = om.advect_first_interp(d["q"], 1, d["vel"], float(d["dt"]), mode)
+    report_sl("chan_6x5_p6 interp %d" % mode, q, ref)
+    m.set_sl_interp(se_amd.INTERP_DEFAULT)
+    q2, _ = m.advect(d["q"], d["vel"], float(d["dt"]))
+    assert np.array_equal(q2, d["adv1_q"])
