@@ -294,6 +294,17 @@ int se_sl_advect2(se_mesh* mesh, const double* q, const double* qPrev, int nDofs
                   const double* velPrev, double dt, double* qSL);
 int se_sl_advect2_dev(se_mesh* mesh, const double* d_q, const double* d_qPrev, int nDofs, const double* d_vel,
                       const double* d_velPrev, double dt, double* d_qSL);
+/* se_sl_advect2_dev without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  The extrapolated velocities go to a fixed
+ * per-GPU scratch array instead of a buffer the library allocates, so meshes
+ * are limited to SE_SL_NOALLOC_MAX_NODES field nodes (nV; SE_ERR_ARG above
+ * it).  The result equals se_sl_advect2_dev's bit for bit.  Every call on one
+ * GPU shares that scratch: calls that overlap on one GPU (different streams
+ * or threads) race. */
+#define SE_SL_NOALLOC_MAX_NODES 524288
+int se_sl_advect2_dev_noalloc(se_mesh* mesh, const double* d_q, const double* d_qPrev, int nDofs,
+                              const double* d_vel, const double* d_velPrev, double dt, double* d_qSL);
 /* u = -dpsi/dy, v = dpsi/dx at every vertex  [ref: QuasiGeostrophicEqn::GenVel
  *  src/qg/QuasiGeostrophicEqn.cpp:207-223 via Field::InterpDerivsGlobal
  *  Field.cpp:96-118, Mesh::ShapeFuncDerivsAtCoord Mesh.cpp:252-285] */

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libse_amd.so")
 DETJ_LITERAL, DETJ_PHYSICAL = 0, 1
 PREC_FP64, PREC_FP32_IR, PREC_BF16_IR = 0, 1, 2
 PC_DEFAULT, PC_JACOBI, PC_FDM = 0, 1, 2
+SL_NOALLOC_MAX_NODES = 524288  # SE_SL_NOALLOC_MAX_NODES of include/se_amd.h
 INTERP_DEFAULT, INTERP_GLL, INTERP_BICUBIC = 0, 1, 2
 SOLVE_CONVERGED, SOLVE_MAXIT, SOLVE_BREAKDOWN, SOLVE_NONFINITE, SOLVE_STAGNATED = 0, 1, 2, 3, 4
 
@@ -158,6 +159,7 @@ def lib():
             "se_sl_advect_dev": (i, [vp, vp, i, vp, d, vp, vp]),
             "se_sl_advect2": (i, [vp, _D, _D, i, _D, _D, d, _D]),
             "se_sl_advect2_dev": (i, [vp, vp, vp, i, vp, vp, d, vp]),
+            "se_sl_advect2_dev_noalloc": (i, [vp, vp, vp, i, vp, vp, d, vp]),
             "se_mesh_set_sl_interp": (i, [vp, i]),
             "se_interp_points": (i, [vp, _D, i, i, _D, i, _D]),
             "se_interp_points_dev": (i, [vp, vp, i, i, vp, i, vp]),

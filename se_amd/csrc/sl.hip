@@ -12,6 +12,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 
 #include "glibc_pow.h"
@@ -444,7 +446,52 @@ int sl_second(se_mesh* m, const double* d_q, const double* d_qp, int nd, const d
     return SE_OK;
 }
 
+// Scratch of the allocation-free second-order step: its three extrapolated
+// velocity fields, 3 x 2 x SE_SL_NOALLOC_MAX_NODES doubles (24 MiB).  One
+// copy per GPU, held until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in for the device
+// symbol): it passes the per-device address that hipGetSymbolAddress gives.
+static __device__ __align__(256) double scratch[3 * 2 * (size_t)SE_SL_NOALLOC_MAX_NODES];
+
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_ptr;  // device -> address of that device's scratch
+
+// this device's scratch address, looked up once per device with that device
+// current (the caller's current device is restored afterwards)
+int scratch_of(int device, double** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    if (it == g_scratch_ptr.end()) {
+        double* ptr = nullptr;
+        int prev = 0;
+        SE_HIP(hipGetDevice(&prev));
+        SE_HIP(hipSetDevice(device));
+        hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+        SE_HIP(hipSetDevice(prev));
+        SE_CHECK(e == hipSuccess, SE_ERR_HIP,
+                 std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        SE_CHECK(ptr != nullptr, SE_ERR_HIP, "no device address for the SL scratch");
+        it = g_scratch_ptr.emplace(device, ptr).first;
+    }
+    *out = it->second;
+    return SE_OK;
+}
+
 }  // namespace
+
+extern "C" int se_sl_advect2_dev_noalloc(se_mesh* m, const double* d_q, const double* d_qp, int nd,
+                                         const double* d_vel, const double* d_velp, double dt, double* d_qsl) {
+    int rc = sl_second_check(m, d_q, d_qp, nd, d_vel, d_velp, d_qsl, "se_sl_advect2_dev_noalloc");
+    if (rc) return rc;
+    SE_CHECK((size_t)m->nV <= (size_t)SE_SL_NOALLOC_MAX_NODES, SE_ERR_ARG,
+             "se_sl_advect2_dev_noalloc: the mesh has " + std::to_string(m->nV) +
+                 " field nodes, more than the fixed scratch holds (SE_SL_NOALLOC_MAX_NODES = " +
+                 std::to_string(SE_SL_NOALLOC_MAX_NODES) + "); use se_sl_advect2_dev");
+    double* sc = nullptr;
+    rc = scratch_of(m->ctx->device, &sc);
+    if (rc) return rc;
+    return sl_second(m, d_q, d_qp, nd, d_vel, d_velp, dt, d_qsl, sc);
+}
 
 extern "C" int se_sl_advect2_dev(se_mesh* m, const double* d_q, const double* d_qp, int nd, const double* d_vel,
                                  const double* d_velp, double dt, double* d_qsl) {
