@@ -7,9 +7,14 @@ Workloads (--workload):
       GLL order p ~ U{4,6,8}, lambda = 10^U[0,3] (26.8 M unknowns in all),
       right-hand sides f = M g with g a seeded random field with an isotropic
       k^-3 spectrum.  One step = all 512 solves to a true relative residual of
-      1e-10, batched (FP32 or BF16 inner iterations, FP64 reliable updates).
-      The plain FP64 GPU path (each member alone, FP64 Jacobi-PCG) is timed on
-      the same members once and reported beside it.
+      1e-10, batched: FP64 CG preconditioned by fast diagonalisation with its
+      transforms on the matrix cores (--precision: FP64, FP32 or BF16
+      operands; --pc jacobi: the reference's Jacobi-PCG with FP32 or BF16
+      inner iterations and FP64 reliable updates).  The plain FP64 GPU path
+      (each member alone, FP64 Jacobi-PCG, the reference's algorithm) is timed
+      on the same members once and reported beside it, with the other
+      precisions / preconditioners batched, and a served two-layer QG stream of
+      16 requests for the metric's p50/p99 latency.
   helmholtz_128 -- one solve on 128x128 elements at p = 8 (1 048 576
       unknowns), lambda = 100, same kind of right-hand side.
   qg2_stream -- served two-layer QG step requests (Poisson arrivals at
@@ -22,7 +27,8 @@ value = unknowns solved / second (GDOF/s to solution), with every input
 resident in HBM before the timed region.  The JSON line carries the roofline
 of the dominant kernel, timed with HIP events on the stream it runs on, and a
 CPU baseline: the CPU oracle's assembled-CSR Jacobi-PCG (the reference's
-algorithm), single thread, on a bounded sample, scaled to the full workload.
+algorithm) on all host threads (and on one), on a bounded sample, scaled to
+the full workload.
 """
 import argparse
 import json

@@ -103,6 +103,11 @@ se_solve_opts solve_opts(bool null_space) {
         if (s == "fp32") o.precision = SE_PREC_FP32_IR;
         if (s == "bf16") o.precision = SE_PREC_BF16_IR;
     }
+    if (const char* p = getenv("SE_PRECONDITIONER")) {
+        const std::string s(p);
+        if (s == "jacobi") o.preconditioner = SE_PC_JACOBI;
+        if (s == "fdm") o.preconditioner = SE_PC_FDM;
+    }
     o.null_space = null_space ? 1 : 0;
     return o;
 }
