@@ -6,8 +6,11 @@
 // without atomics), but organised for LDS bandwidth:
 //   1. gather the node block through the numbering map into LDS;
 //   2. forward contractions u_xi, u_eta by element lines: one thread holds a
-//      line of N+1 values in registers and produces N+1 outputs, D is read
-//      with wave-uniform (broadcast) LDS addresses;
+//      line of N+1 values in registers and produces N+1 outputs; D comes in
+//      the kernel arguments (DMat), so with compile-time indices every D entry
+//      is a scalar load and an SGPR operand of the FMA, not an LDS read (as a
+//      wave-uniform LDS read it cost the full LDS return bandwidth: 2 (N+1)^3
+//      reads per element, most of the kernel's LDS traffic);
 //   3. per quadrature point: symmetric 2x2 metric of the reference's exact
 //      per-point Jacobian, assembled from per-element-column and per-element-row
 //      factor tables (see op_metric_tables), fluxes in place;
@@ -15,9 +18,37 @@
 //      with the needed columns of D held in registers, plus the GLL-diagonal
 //      mass term; the block's partial of p.Ap is stored for the CG update.
 #pragma once
+#include <vector>
+
 #include "helm_kernels.h"
 
 namespace se {
+
+// Global-memory view of a pointer: accesses through it are global_*
+// instructions.  Through a generic pointer (one read from a struct in memory)
+// they would be flat_*, which share the LDS counter, so waits for LDS reads
+// would also wait for them.
+#define SE_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const SE_GLOBAL T* gview(const T* p) {
+    return (const SE_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ SE_GLOBAL T* gview(T* p) {
+    return (SE_GLOBAL T*)p;
+}
+
+// The GLL derivative matrix D[q][k] = l_k'(x_q) of order N, passed by value.
+template <int N>
+struct DMat {
+    double d[(N + 1) * (N + 1)];
+};
+
+// element tile of the batched apply (work items are (member, tile))
+constexpr int kBatchApplyTile = 4;
+inline int batch_apply_tiles(int nEls0, int nEls1) {
+    return ((nEls0 + kBatchApplyTile - 1) / kBatchApplyTile) * ((nEls1 + kBatchApplyTile - 1) / kBatchApplyTile);
+}
 
 // block-sum K values (blockDim BS) and store them to partials[k*np + idx]
 template <int K, int BS>
@@ -77,8 +108,9 @@ __device__ __forceinline__ void reduce_partials(const double* __restrict__ parti
 // grid).  With DOT, the workgroup's partial of in.out goes to
 // partials[tile] (np = number of tiles).
 template <int N, int TX, int TY, bool DOT>
-__device__ __forceinline__ void apply_tile(const OpParams& P, int tile, const double* __restrict__ in,
-                                           double* __restrict__ out, double* __restrict__ partials, int np) {
+__device__ __forceinline__ void apply_tile(const OpParams& P, const DMat<N>& dm, int tile,
+                                           const double* __restrict__ in, double* __restrict__ out,
+                                           double* __restrict__ partials, int np) {
     constexpr int N1 = N + 1;
     constexpr int NP = N1 * N1;
     constexpr int BW = ((TX + 1) * N + 2) | 1;  // odd row pitch (in doubles): conflict-free column access
@@ -103,6 +135,9 @@ __device__ __forceinline__ void apply_tile(const OpParams& P, int tile, const do
     const int ex0 = (tile % tilesX) * TX;
     const int ey0 = (tile / tilesX) * TY;
     const int txo = min(TX, P.nEls0 - ex0), tyo = min(TY, P.nEls1 - ey0);
+    const SE_GLOBAL int32_t* f2v = gview(P.f2v);
+    const SE_GLOBAL double* gin = gview(in);
+    SE_GLOBAL double* gout = gview(out);
     auto exw = [&](int c) {  // wrapped element index of tile column c (-1: none)
         int ex = ex0 - 1 + c;
         if (ex < 0) ex = P.per0 ? ex + P.nEls0 : -1;
@@ -131,7 +166,7 @@ __device__ __forceinline__ void apply_tile(const OpParams& P, int tile, const do
             if (a <= N) { r = 0; iy = a; }
             else { int aa = a - N1; r = 1 + aa / N; iy = aa % N; if (iy == 0 && r >= 2) { r -= 1; iy = N; } }
             const int ex = exw(c), ey = eyw(r);
-            if (ex >= 0 && ey >= 0) um[g] = P.f2v[(ey * N + iy) * P.nv0 + ex * N + ix];
+            if (ex >= 0 && ey >= 0) um[g] = f2v[(ey * N + iy) * P.nv0 + ex * N + ix];
         }
     }
     int ou[NOWN];  // unknowns of this thread's owned nodes (phase 4)
@@ -141,23 +176,23 @@ __device__ __forceinline__ void apply_tile(const OpParams& P, int tile, const do
         ou[j] = -1;
         if (grp4 < NG && oe < txo * tyo) {
             const int ox = (oe % txo) * N + rx, oy = (oe / txo) * N + ry;
-            ou[j] = P.f2v[(ey0 * N + oy) * P.nv0 + ex0 * N + ox];
+            ou[j] = f2v[(ey0 * N + oy) * P.nv0 + ex0 * N + ox];
         }
     }
     if (tid <= TX) s_exw[tid] = exw(tid);
     if (tid >= 64 && tid <= 64 + TY) s_eyw[tid - 64] = eyw(tid - 64);
-    for (int i = tid; i < NP; i += 256) s_D[i] = P.D[i];
+    for (int i = tid; i < NP; i += 256) s_D[i] = dm.d[i];
     for (int i = tid; i < (TX + 1) * CS; i += 256) {
         const int e = exw(i / CS);
-        s_col[i] = e >= 0 ? P.colTab[(size_t)e * CS + i % CS] : 0.0;
+        s_col[i] = e >= 0 ? gview(P.colTab)[(size_t)e * CS + i % CS] : 0.0;
     }
     for (int i = tid; i < (TY + 1) * CS; i += 256) {
         const int e = eyw(i / CS);
-        s_row[i] = e >= 0 ? P.rowTab[(size_t)e * CS + i % CS] : 0.0;
+        s_row[i] = e >= 0 ? gview(P.rowTab)[(size_t)e * CS + i % CS] : 0.0;
     }
     double uv[NG1];
 #pragma unroll
-    for (int g = 0; g < NG1; g++) uv[g] = um[g] >= 0 ? in[um[g]] : 0.0;
+    for (int g = 0; g < NG1; g++) uv[g] = um[g] >= 0 ? gin[um[g]] : 0.0;
 #pragma unroll
     for (int g = 0; g < NG1; g++) {
         const int i = tid + g * 256;
@@ -184,7 +219,7 @@ __device__ __forceinline__ void apply_tile(const OpParams& P, int tile, const do
                     for (int q = 0; q < N1; q++) {
                         double s = 0.0;
 #pragma unroll
-                        for (int k = 0; k < N1; k++) s += s_D[q * N1 + k] * u[k];
+                        for (int k = 0; k < N1; k++) s += dm.d[q * N1 + k] * u[k];
                         s_a[e * NP + idx * N1 + q] = s;  // u_xi(qx = q, qy = idx)
                     }
                 } else {
@@ -195,7 +230,7 @@ __device__ __forceinline__ void apply_tile(const OpParams& P, int tile, const do
                     for (int q = 0; q < N1; q++) {
                         double s = 0.0;
 #pragma unroll
-                        for (int k = 0; k < N1; k++) s += s_D[q * N1 + k] * u[k];
+                        for (int k = 0; k < N1; k++) s += dm.d[q * N1 + k] * u[k];
                         s_b[e * NP + q * N1 + idx] = s;  // u_eta(qx = idx, qy = q)
                     }
                 }
@@ -210,7 +245,7 @@ __device__ __forceinline__ void apply_tile(const OpParams& P, int tile, const do
         const int q = tid % NP, qg = tid / NP;
         const int qx = q % N1, qy = q / N1;
         const double c = P.c_lap;
-        const double wq = P.w[qx] * P.w[qy];
+        const double wq = gview(P.w)[qx] * gview(P.w)[qy];
         if (qg < QG) {
             for (int e = qg; e < NE; e += QG) {
                 const int ce = e % (TX + 1), re = e / (TX + 1);
@@ -269,7 +304,7 @@ __device__ __forceinline__ void apply_tile(const OpParams& P, int tile, const do
             if (rx == 0 && s_exw[ce - 1] >= 0) sum += contrib(ce - 1, re, N, ry, DcN, Dcy);
             sum += contrib(ce, re, rx, ry, Dcx, Dcy);
             const double yv = P.scale * sum;
-            out[u] = yv;
+            gout[u] = yv;
             if (DOT) dot += s_u[slot<N>(re, ry) * BW + slot<N>(ce, rx)] * yv;
         }
     }
@@ -282,7 +317,7 @@ __device__ __forceinline__ void apply_tile(const OpParams& P, int tile, const do
             int ox, oy;
             if (i < nx) { ox = txo * N; oy = i; }
             else { ox = i - nx; oy = tyo * N; }
-            const int u = P.f2v[(ey0 * N + oy) * P.nv0 + ex0 * N + ox];
+            const int u = f2v[(ey0 * N + oy) * P.nv0 + ex0 * N + ox];
             if (u < 0) continue;
             int cs[2], xs[2], nc = 0, rs[2], ys[2], nr = 0;
             if (ox % N == 0) { cs[nc] = ox / N; xs[nc] = N; nc++; }
@@ -305,7 +340,7 @@ __device__ __forceinline__ void apply_tile(const OpParams& P, int tile, const do
                 }
             }
             const double yv = P.scale * sum;
-            out[u] = yv;
+            gout[u] = yv;
             if (DOT) dot += own * yv;
         }
     }
@@ -317,15 +352,16 @@ __device__ __forceinline__ void apply_tile(const OpParams& P, int tile, const do
 }
 
 template <int N, int TX, int TY, bool DOT>
-__global__ __launch_bounds__(256) void k_apply(OpParams P, const double* __restrict__ in, double* __restrict__ out,
-                                                double* __restrict__ partials, const CgState* __restrict__ st) {
+__global__ __launch_bounds__(256) void k_apply(OpParams P, DMat<N> dm, const double* __restrict__ in,
+                                                double* __restrict__ out, double* __restrict__ partials,
+                                                const CgState* __restrict__ st) {
     if (DOT && cg_stopped(st)) return;
-    apply_tile<N, TX, TY, DOT>(P, blockIdx.x, in, out, partials, gridDim.x);
+    apply_tile<N, TX, TY, DOT>(P, dm, blockIdx.x, in, out, partials, gridDim.x);
 }
 
-// Batched apply: work items (member, tile) over members of one order; the
-// grid loops over the items.  Each member brings its own operator, input and
-// output.
+// Batched apply: work items (member, tile) over members of one order (they
+// share D); the grid loops over the items.  Each member brings its own
+// operator, input and output.
 struct ApplyMember {
     OpParams P;
     const double* in;
@@ -333,14 +369,22 @@ struct ApplyMember {
 };
 
 template <int N, int TX, int TY>
-__global__ __launch_bounds__(256) void k_apply_batch(const ApplyMember* __restrict__ members,
+__global__ __launch_bounds__(256) void k_apply_batch(DMat<N> dm, const ApplyMember* __restrict__ members,
                                                       const int2* __restrict__ work, int nwork) {
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
         const int2 it = work[w];
         const ApplyMember& a = members[it.x];
         __syncthreads();  // the previous item's LDS reads are done
-        apply_tile<N, TX, TY, false>(a.P, it.y, a.in, a.out, nullptr, 0);
+        apply_tile<N, TX, TY, false>(a.P, dm, it.y, a.in, a.out, nullptr, 0);
     }
+}
+
+// D of a mesh's order as a kernel argument
+template <int N>
+inline DMat<N> dmat_of(const std::vector<double>& D) {
+    DMat<N> m;
+    for (int i = 0; i < (N + 1) * (N + 1); i++) m.d[i] = D[i];
+    return m;
 }
 
 }  // namespace se

@@ -186,12 +186,15 @@ int g_start(const Group& g, int nwork, const int2* work, hipStream_t s) {
     }
     return SE_OK;
 }
-int g_apply64(const Group& g, int nwork, const int2* work, hipStream_t s) {
+int g_apply64(const Group& g, const std::vector<double>& D, int nwork, const int2* work, hipStream_t s) {
     if (nwork == 0) return SE_OK;
     const ApplyMember* As = g.d_amembers.as<ApplyMember>();
     switch (g.N) {
-#define SE_CASE(NN) \
-    case NN: hipLaunchKernelGGL((k_apply_batch<NN, 2, 2>), dim3(nwork), dim3(256), 0, s, As, work, nwork); break;
+#define SE_CASE(NN)                                                                                                  \
+    case NN:                                                                                                         \
+        hipLaunchKernelGGL((k_apply_batch<NN, kBatchApplyTile, kBatchApplyTile>), dim3(nwork), dim3(256), 0, s, \
+                           dmat_of<NN>(D), As, work, nwork);                                                         \
+        break;
         SE_BATCH_ORDERS(SE_CASE)
 #undef SE_CASE
         default: return fail(SE_ERR_UNSUPPORTED, "batched solve built for orders 2..8");
@@ -199,7 +202,7 @@ int g_apply64(const Group& g, int nwork, const int2* work, hipStream_t s) {
     return SE_OK;
 }
 
-int apply_tiles22(const se_mesh* m) { return ((m->nEls[0] + 1) / 2) * ((m->nEls[1] + 1) / 2); }
+int apply_tiles_b(const se_mesh* m) { return batch_apply_tiles(m->nEls[0], m->nEls[1]); }
 
 // one graph: the k-iteration chains of every group on their own streams
 template <typename T>
@@ -424,7 +427,7 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
                 ul.push_back(j);
                 for (int t = 0; t < tcg_tiles(m); t++) ut.push_back(make_int2(j, t));
                 for (int t = 0; t < (m->n + TCG_FLUSH - 1) / TCG_FLUSH; t++) uf.push_back(make_int2(j, t));
-                for (int t = 0; t < apply_tiles22(m); t++) ua.push_back(make_int2(j, t));
+                for (int t = 0; t < apply_tiles_b(m); t++) ua.push_back(make_int2(j, t));
             }
             if (ul.empty()) continue;
             rc = upload(b, g.d_ut, ut, s);
@@ -435,7 +438,7 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
             const TcgMember* Ms = g.d_members.as<TcgMember>();
             hipLaunchKernelGGL(k_tcg_flush_batch, dim3((int)uf.size()), dim3(TCG_NT), 0, s, Ms, g.d_uf.as<int2>(),
                                (int)uf.size());
-            rc = g_apply64(g, (int)ua.size(), g.d_ua.as<int2>(), s);
+            rc = g_apply64(g, b->ops[g.members[0]]->mesh->gll.D, (int)ua.size(), g.d_ua.as<int2>(), s);
             if (!rc) rc = g_start<T, true>(g, (int)ut.size(), g.d_ut.as<int2>(), s);
             if (rc) return rc;
             hipLaunchKernelGGL(k_tcg_true_batch, dim3((int)ul.size()), dim3(TCG_NT), 0, s, Ms, g.d_ul.as<int>(),

@@ -21,7 +21,7 @@
 // preconditioner is the FDM in the solve's precision.  It runs members of a
 // batch in lockstep, one launch per phase over (member, item) work lists; a
 // single solve is a batch of one, so batched results are the single ones bit
-// for bit.  The exact FP64 operator apply is k_apply_batch (2 x 2 element
+// for bit.  The exact FP64 operator apply is k_apply_batch (4 x 4 element
 // tiles); every reduction is per item with consumer-side fixed-order sums.
 #include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
@@ -45,6 +45,11 @@ namespace se {
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+// global-memory views of the operands: loads and stores through them are
+// global_* instructions.  Through generic pointers they would be flat_*, which
+// share the LDS counter, so every wait for an LDS operand read would also wait
+// for the next chunk's prefetch (SE_GLOBAL, apply_kernels.h).
 
 enum { FDM_OUT_T = 0, FDM_OUT_SCALE = 1, FDM_OUT_FINAL = 2 };
 enum { FDM_F64 = 0, FDM_F32 = 1, FDM_BF16 = 2 };  // element types of the transforms
@@ -174,19 +179,20 @@ __device__ __forceinline__ void for_each_acc(const Acc<T>& acc, int wr, int wc, 
 
 // One operand chunk (64 rows x KC) in registers: 16-byte vectors, VPR per
 // row segment, NV per thread.  Loaded one chunk ahead of the MFMA work that
-// uses the LDS copy of the previous one.
+// uses the LDS copy of the previous one.  (Clang vector registers: HIP's
+// uint4 wrapper type kept the array in scratch memory.)
 template <typename T>
 struct OperandRegs {
     static constexpr int KC = GemmCfg<T>::KC, KP = GemmCfg<T>::KP;
     static constexpr int VEC = 16 / (int)sizeof(T), VPR = KC / VEC, NV = 64 * VPR / 256;
     static_assert(KC % VEC == 0 && (64 * VPR) % 256 == 0 && 256 % VPR == 0, "chunk / thread shape");
-    uint4 v[NV];
-    __device__ __forceinline__ void load(const T* __restrict__ src, int ld, int r0, int k0) {
+    u32x4 v[NV];
+    __device__ __forceinline__ void load(const T* src, int ld, int r0, int k0) {
         const int tid = threadIdx.x;
 #pragma unroll
         for (int s = 0; s < NV; s++) {
             const int row = tid / VPR + (256 / VPR) * s, cv = tid % VPR;
-            v[s] = *(const uint4*)(src + (size_t)(r0 + row) * ld + k0 + cv * VEC);
+            v[s] = *(const SE_GLOBAL u32x4*)(src + (size_t)(r0 + row) * ld + k0 + cv * VEC);
         }
     }
     __device__ __forceinline__ void store(T* S) const {
@@ -194,7 +200,7 @@ struct OperandRegs {
 #pragma unroll
         for (int s = 0; s < NV; s++) {
             const int row = tid / VPR + (256 / VPR) * s, cv = tid % VPR;
-            *(uint4*)(S + row * KP + cv * VEC) = v[s];
+            *(u32x4*)(S + row * KP + cv * VEC) = v[s];
         }
     }
 };
@@ -237,25 +243,29 @@ __device__ __forceinline__ void gemm_tile(const GemmTask& t, int tile, char* lds
         __syncthreads();  // done with the operand tiles
         for_each_acc<T>(acc, wr, wc, l, [&](int row, int col, double v) { Cs[col * 65 + row] = (S)v; });
         __syncthreads();
-        T* __restrict__ C = (T*)t.C;
+        SE_GLOBAL T* C = (SE_GLOBAL T*)t.C;
         for (int e = tid; e < 64 * 64; e += 256) {
             const int cr = e >> 6, cc = e & 63;
             C[(size_t)(n0 + cr) * t.ldc + m0 + cc] = to_t<T>((double)Cs[cr * 65 + cc]);
         }
     } else if (t.mode == FDM_OUT_SCALE) {
-        T* __restrict__ C = (T*)t.C;
+        SE_GLOBAL T* C = (SE_GLOBAL T*)t.C;
+        const SE_GLOBAL double* lx = (const SE_GLOBAL double*)t.lx;
+        const SE_GLOBAL double* ly = (const SE_GLOBAL double*)t.ly;
+        const SE_GLOBAL double* ex = (const SE_GLOBAL double*)t.ex;
+        const SE_GLOBAL double* ey = (const SE_GLOBAL double*)t.ey;
         for_each_acc<T>(acc, wr, wc, l, [&](int row, int col, double v) {
             const int gr = m0 + row, gc = n0 + col;
             double s = 0.0;
             if (gr < t.crows && gc < t.ccols) {
-                const double D = t.lx[gr] + t.ly[gc] + t.ex[gr] * t.ey[gc];
+                const double D = lx[gr] + ly[gc] + ex[gr] * ey[gc];
                 s = D > t.dtol ? 1.0 / D : 0.0;
             }
             C[(size_t)gr * t.ldc + gc] = to_t<T>(v * s);
         });
     } else {  // FDM_OUT_FINAL: FP64 z on the unknown grid, partials of r.z, sum z, sum r
-        double* __restrict__ Z = (double*)t.C;
-        const double* __restrict__ R = t.r;
+        SE_GLOBAL double* Z = (SE_GLOBAL double*)t.C;
+        const SE_GLOBAL double* R = (const SE_GLOBAL double*)t.r;
         double rz = 0.0, sz = 0.0, sr = 0.0;
         for_each_acc<T>(acc, wr, wc, l, [&](int row, int col, double v) {
             const int gr = m0 + row, gc = n0 + col;
@@ -320,6 +330,7 @@ struct FbMember {
     double* pa;  // [nch]      p.q
     double* pt;  // [2][nch]   sum t, sum t^2 of the true residual
     double* ps;  // [nch]      sum of sgn f
+    double* xo;  //            the caller's solution vector
     CgState* st;
     int n, nch, ntz, null_space;
     double sgn, rtol;
@@ -551,6 +562,18 @@ __global__ __launch_bounds__(FB_NT) void k_fb_true_fin(const FbMember* __restric
 }
 
 // continue from the true residual: r -= mean (null space); partials of r.r
+// x -> the caller's solution vector, all members in one launch
+__global__ __launch_bounds__(FB_NT) void k_fb_out(const FbMember* __restrict__ Ms, const int2* __restrict__ work,
+                                                  int nwork) {
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const int2 it = work[w];
+        const FbMember& M = Ms[it.x];
+        int i0, i1;
+        chunk_range(M, it.y, i0, i1);
+        for (int i = i0 + threadIdx.x; i < i1; i += FB_NT) M.xo[i] = M.x[i];
+    }
+}
+
 __global__ __launch_bounds__(FB_NT) void k_fb_restart(const FbMember* __restrict__ Ms, const int2* __restrict__ work,
                                                       int nwork) {
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
@@ -902,10 +925,14 @@ struct FdmPlan {
 
 namespace {
 
-int launch_apply_batch(int N, const ApplyMember* A, const int2* work, int nw, hipStream_t s) {
+int launch_apply_batch(int N, const std::vector<double>& D, const ApplyMember* A, const int2* work, int nw,
+                       hipStream_t s) {
     switch (N) {
-#define SE_CASE(NN) \
-    case NN: hipLaunchKernelGGL((k_apply_batch<NN, 2, 2>), dim3(nw), dim3(256), 0, s, A, work, nw); break;
+#define SE_CASE(NN)                                                                                               \
+    case NN:                                                                                                      \
+        hipLaunchKernelGGL((k_apply_batch<NN, kBatchApplyTile, kBatchApplyTile>), dim3(nw), dim3(256), 0, s, \
+                           dmat_of<NN>(D), A, work, nw);                                                          \
+        break;
         SE_CASE(2) SE_CASE(3) SE_CASE(4) SE_CASE(5) SE_CASE(6) SE_CASE(7) SE_CASE(8)
 #undef SE_CASE
         default: return fail(SE_ERR_UNSUPPORTED, "FDM solve built for orders 2..8");
@@ -925,7 +952,7 @@ int plan_apply(FdmPlan* pl, bool from_x, const std::vector<char>& on, hipStream_
             if (o.h_work.empty()) continue;
             const ApplyMember* A = from_x ? o.d_ax.as<ApplyMember>() : o.d_ap.as<ApplyMember>();
             const int nw = (int)o.h_work.size();
-            int rc = launch_apply_batch(o.N, A, o.d_work.as<int2>(), nw, s);
+            int rc = launch_apply_batch(o.N, pl->ops[o.members[0]]->mesh->gll.D, A, o.d_work.as<int2>(), nw, s);
             if (rc) return rc;
             continue;
         }
@@ -934,15 +961,15 @@ int plan_apply(FdmPlan* pl, bool from_x, const std::vector<char>& on, hipStream_
             const int i = o.members[j];
             if (!on[i]) continue;
             const se_mesh* m = pl->ops[i]->mesh;
-            const int t22 = ((m->nEls[0] + 1) / 2) * ((m->nEls[1] + 1) / 2);
-            for (int t = 0; t < t22; t++) o.h_work.push_back(make_int2(j, t));
+            const int tiles = batch_apply_tiles(m->nEls[0], m->nEls[1]);
+            for (int t = 0; t < tiles; t++) o.h_work.push_back(make_int2(j, t));
         }
         if (o.h_work.empty()) continue;
         int rc = o.d_work.alloc(sizeof(int2) * o.h_work.size());
         if (rc) return rc;
         SE_HIP(hipMemcpyAsync(o.d_work.ptr, o.h_work.data(), sizeof(int2) * o.h_work.size(), hipMemcpyHostToDevice, s));
         const ApplyMember* A = from_x ? o.d_ax.as<ApplyMember>() : o.d_ap.as<ApplyMember>();
-        rc = launch_apply_batch(o.N, A, o.d_work.as<int2>(), (int)o.h_work.size(), s);
+        rc = launch_apply_batch(o.N, pl->ops[o.members[0]]->mesh->gll.D, A, o.d_work.as<int2>(), (int)o.h_work.size(), s);
         if (rc) return rc;
     }
     return SE_OK;
@@ -1069,6 +1096,7 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
         M.pa = F->pa.as<double>();
         M.pt = F->pt.as<double>();
         M.ps = F->ps.as<double>();
+        M.xo = d_x[i];
         M.n = op->mesh->n;
         M.nch = (std::max(M.n, 1) + FB_CH - 1) / FB_CH;
         M.ntz = fdm_tiles(op);
@@ -1268,10 +1296,19 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
         rc = pull();
         if (rc) return rc;
     }
-    for (int i = 0; i < count; i++) {
-        se_op* op = pl->ops[i];
-        if (d_x[i] != op->x.as<double>())
-            SE_HIP(hipMemcpyAsync(d_x[i], op->x.ptr, sizeof(double) * op->mesh->n, hipMemcpyDeviceToDevice, s));
+    {
+        std::vector<char> out(count, 0);
+        bool any_out = false;
+        for (int i = 0; i < count; i++) {
+            out[i] = d_x[i] != pl->ops[i]->x.as<double>();
+            any_out = any_out || out[i];
+        }
+        if (any_out) {
+            rc = plan_lists(pl, out, s);
+            if (rc) return rc;
+            vlaunch(k_fb_out);
+            SE_HIP(hipGetLastError());
+        }
     }
     SE_HIP(hipEventRecord(c->ev1, s));
     SE_HIP(hipEventSynchronize(c->ev1));

@@ -282,19 +282,20 @@ void launch_tile_n(int tile, const OpParams& P, const double* in, double* out, c
 }
 
 template <int N, int TX, int TY, bool DOT>
-void launch_apply_t(const OpParams& P, const double* in, double* out, double* partials, const CgState* st,
-                    hipStream_t s) {
+void launch_apply_t(const OpParams& P, const std::vector<double>& D, const double* in, double* out, double* partials,
+                    const CgState* st, hipStream_t s) {
     int tiles = ((P.nEls0 + TX - 1) / TX) * ((P.nEls1 + TY - 1) / TY);
-    hipLaunchKernelGGL((k_apply<N, TX, TY, DOT>), dim3(tiles), dim3(kBlock), 0, s, P, in, out, partials, st);
+    hipLaunchKernelGGL((k_apply<N, TX, TY, DOT>), dim3(tiles), dim3(kBlock), 0, s, P, dmat_of<N>(D), in, out,
+                       partials, st);
 }
 
 template <bool DOT>
-int launch_apply(int N, int tile, const OpParams& P, const double* in, double* out, double* partials,
-                 const CgState* st, hipStream_t s) {
-#define SE_APPLY_CASE(NN)                                                      \
-    case NN:                                                                   \
-        if (tile == 4) launch_apply_t<NN, 4, 4, DOT>(P, in, out, partials, st, s); \
-        else launch_apply_t<NN, 2, 2, DOT>(P, in, out, partials, st, s);          \
+int launch_apply(int N, int tile, const OpParams& P, const std::vector<double>& D, const double* in, double* out,
+                 double* partials, const CgState* st, hipStream_t s) {
+#define SE_APPLY_CASE(NN)                                                             \
+    case NN:                                                                          \
+        if (tile == 4) launch_apply_t<NN, 4, 4, DOT>(P, D, in, out, partials, st, s); \
+        else launch_apply_t<NN, 2, 2, DOT>(P, D, in, out, partials, st, s);           \
         break;
     switch (N) {
         SE_APPLY_CASE(2)
@@ -458,15 +459,15 @@ int op_apply_dev(se_op* op, const double* d_x, double* d_y, double scale, bool d
     if (op->svv_mN > 0) {
         // mass + Laplacian, the SVV term, then the partials of x.y the CG
         // update reads (as many as the apply kernel would write)
-        int rc = launch_apply<false>(op->mesh->N, op->tile, P, d_x, d_y, pa, op->state.as<CgState>(), s);
+        int rc = launch_apply<false>(op->mesh->N, op->tile, P, op->mesh->gll.D, d_x, d_y, pa, op->state.as<CgState>(), s);
         if (rc) return rc;
         SE_HIP(hipGetLastError());
         rc = svv_apply(op, d_x, d_y, scale);
         if (!rc && dot) rc = svv_dot(op, d_x, d_y, pa, apply_tiles(op));
         return rc;
     }
-    int rc = dot ? launch_apply<true>(op->mesh->N, op->tile, P, d_x, d_y, pa, op->state.as<CgState>(), s)
-                 : launch_apply<false>(op->mesh->N, op->tile, P, d_x, d_y, pa, op->state.as<CgState>(), s);
+    int rc = dot ? launch_apply<true>(op->mesh->N, op->tile, P, op->mesh->gll.D, d_x, d_y, pa, op->state.as<CgState>(), s)
+                 : launch_apply<false>(op->mesh->N, op->tile, P, op->mesh->gll.D, d_x, d_y, pa, op->state.as<CgState>(), s);
     if (rc) return rc;
     SE_HIP(hipGetLastError());
     return SE_OK;
