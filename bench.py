@@ -74,9 +74,9 @@ def parse():
 
 
 def apply_bytes(m):
-    # algorithmic bytes of one FP64 operator apply: x read, y written (8 B each
-    # per unknown) and the int32 numbering map read once per field node
-    return 16.0 * m.n + 4.0 * m.nV
+    # algorithmic bytes of one FP64 operator apply: x read, y written, 8 B each
+    # per unknown (the kernel works on the unknown grid: no map is read)
+    return 16.0 * m.n
 
 
 def apply_flops(m):
@@ -257,6 +257,7 @@ def run_ensemble(a, prec_name):
         # dominant kernel: the FDM transforms on the matrix cores (four passes
         # over all members, timed back to back after the last solve)
         roof = fdm_roofline(b.fdm_probe(reps=a.probe_reps), prec_name, "k_fdm_gemm<%s>" % T)
+        roof["traffic"] = pmc_traffic(roof, "k_fdm_gemm__ensemble_512_%s_pmc.md" % prec_name.split("_")[0])
     else:
         # dominant kernel: the direction + separable operator kernel of the
         # largest order group (the probe continues the last solve's recurrences)
@@ -401,7 +402,7 @@ def run_single(a, prec_name):
         t_apply = ms_apply * 1e-3
         ach = apply_bytes(m) / t_apply / 1e9
         fl = apply_flops(m)
-        roof = {"kernel": "k_apply<%d, 4, 4, false> (FP64 operator apply)" % m.N, "bound": "hbm",
+        roof = {"kernel": "k_apply_lines<%d, false> (FP64 operator apply)" % m.N, "bound": "hbm",
                 "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                 "traffic": None, "us_per_launch": t_apply * 1e6, "bytes_per_launch": apply_bytes(m),
                 "fp64_tflops": fl / t_apply / 1e12, "fp64_frac": fl / t_apply / 1e12 / FP64_PEAK_TFLOPS}

@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "apply_lines.h"
 #include "se_common.h"
 #include "se_op.h"
 #include "tcg_host.h"
@@ -186,14 +187,13 @@ int g_start(const Group& g, int nwork, const int2* work, hipStream_t s) {
     }
     return SE_OK;
 }
-int g_apply64(const Group& g, const std::vector<double>& D, int nwork, const int2* work, hipStream_t s) {
+int g_apply64(const Group& g, const GLL& gl, int nwork, const int2* work, hipStream_t s) {
     if (nwork == 0) return SE_OK;
     const ApplyMember* As = g.d_amembers.as<ApplyMember>();
     switch (g.N) {
 #define SE_CASE(NN)                                                                                                  \
     case NN:                                                                                                         \
-        hipLaunchKernelGGL((k_apply_batch<NN, kBatchApplyTile, kBatchApplyTile>), dim3(nwork), dim3(256), 0, s, \
-                           dmat_of<NN>(D), As, work, nwork);                                                         \
+        hipLaunchKernelGGL((k_apply_lines_batch<NN>), dim3(nwork), dim3(256), 0, s, As, work, nwork);                                                                             \
         break;
         SE_BATCH_ORDERS(SE_CASE)
 #undef SE_CASE
@@ -202,7 +202,7 @@ int g_apply64(const Group& g, const std::vector<double>& D, int nwork, const int
     return SE_OK;
 }
 
-int apply_tiles_b(const se_mesh* m) { return batch_apply_tiles(m->nEls[0], m->nEls[1]); }
+int apply_tiles_b(const se_mesh* m) { return line_tiles(m->N, m->nEls[0], m->nEls[1]); }
 
 // one graph: the k-iteration chains of every group on their own streams
 template <typename T>
@@ -438,7 +438,7 @@ int batch_solve(se_batch* b, const double* const* d_f, double* const* d_x, const
             const TcgMember* Ms = g.d_members.as<TcgMember>();
             hipLaunchKernelGGL(k_tcg_flush_batch, dim3((int)uf.size()), dim3(TCG_NT), 0, s, Ms, g.d_uf.as<int2>(),
                                (int)uf.size());
-            rc = g_apply64(g, b->ops[g.members[0]]->mesh->gll.D, (int)ua.size(), g.d_ua.as<int2>(), s);
+            rc = g_apply64(g, b->ops[g.members[0]]->mesh->gll, (int)ua.size(), g.d_ua.as<int2>(), s);
             if (!rc) rc = g_start<T, true>(g, (int)ut.size(), g.d_ut.as<int2>(), s);
             if (rc) return rc;
             hipLaunchKernelGGL(k_tcg_true_batch, dim3((int)ul.size()), dim3(TCG_NT), 0, s, Ms, g.d_ul.as<int>(),

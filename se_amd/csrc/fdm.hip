@@ -21,8 +21,7 @@
 // preconditioner is the FDM in the solve's precision.  It runs members of a
 // batch in lockstep, one launch per phase over (member, item) work lists; a
 // single solve is a batch of one, so batched results are the single ones bit
-// for bit.  The exact FP64 operator apply is k_apply_batch (4 x 4 element
-// tiles); every reduction is per item with consumer-side fixed-order sums.
+// for bit.  The exact FP64 operator apply is k_apply_lines_batch; every reduction is per item with consumer-side fixed-order sums.
 #include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 
@@ -36,6 +35,7 @@
 #include <vector>
 
 #include "apply_kernels.h"
+#include "apply_lines.h"
 #include "fdm_host.h"
 #include "se_common.h"
 #include "se_op.h"
@@ -925,13 +925,11 @@ struct FdmPlan {
 
 namespace {
 
-int launch_apply_batch(int N, const std::vector<double>& D, const ApplyMember* A, const int2* work, int nw,
-                       hipStream_t s) {
+int launch_apply_batch(int N, const GLL& g, const ApplyMember* A, const int2* work, int nw, hipStream_t s) {
     switch (N) {
-#define SE_CASE(NN)                                                                                               \
-    case NN:                                                                                                      \
-        hipLaunchKernelGGL((k_apply_batch<NN, kBatchApplyTile, kBatchApplyTile>), dim3(nw), dim3(256), 0, s, \
-                           dmat_of<NN>(D), A, work, nw);                                                          \
+#define SE_CASE(NN)                                                                                             \
+    case NN:                                                                                                    \
+        hipLaunchKernelGGL((k_apply_lines_batch<NN>), dim3(nw), dim3(256), 0, s, A, work, nw);                                                                                 \
         break;
         SE_CASE(2) SE_CASE(3) SE_CASE(4) SE_CASE(5) SE_CASE(6) SE_CASE(7) SE_CASE(8)
 #undef SE_CASE
@@ -952,7 +950,7 @@ int plan_apply(FdmPlan* pl, bool from_x, const std::vector<char>& on, hipStream_
             if (o.h_work.empty()) continue;
             const ApplyMember* A = from_x ? o.d_ax.as<ApplyMember>() : o.d_ap.as<ApplyMember>();
             const int nw = (int)o.h_work.size();
-            int rc = launch_apply_batch(o.N, pl->ops[o.members[0]]->mesh->gll.D, A, o.d_work.as<int2>(), nw, s);
+            int rc = launch_apply_batch(o.N, pl->ops[o.members[0]]->mesh->gll, A, o.d_work.as<int2>(), nw, s);
             if (rc) return rc;
             continue;
         }
@@ -961,7 +959,7 @@ int plan_apply(FdmPlan* pl, bool from_x, const std::vector<char>& on, hipStream_
             const int i = o.members[j];
             if (!on[i]) continue;
             const se_mesh* m = pl->ops[i]->mesh;
-            const int tiles = batch_apply_tiles(m->nEls[0], m->nEls[1]);
+            const int tiles = line_tiles(m->N, m->nEls[0], m->nEls[1]);
             for (int t = 0; t < tiles; t++) o.h_work.push_back(make_int2(j, t));
         }
         if (o.h_work.empty()) continue;
@@ -969,7 +967,7 @@ int plan_apply(FdmPlan* pl, bool from_x, const std::vector<char>& on, hipStream_
         if (rc) return rc;
         SE_HIP(hipMemcpyAsync(o.d_work.ptr, o.h_work.data(), sizeof(int2) * o.h_work.size(), hipMemcpyHostToDevice, s));
         const ApplyMember* A = from_x ? o.d_ax.as<ApplyMember>() : o.d_ap.as<ApplyMember>();
-        rc = launch_apply_batch(o.N, pl->ops[o.members[0]]->mesh->gll.D, A, o.d_work.as<int2>(), (int)o.h_work.size(), s);
+        rc = launch_apply_batch(o.N, pl->ops[o.members[0]]->mesh->gll, A, o.d_work.as<int2>(), (int)o.h_work.size(), s);
         if (rc) return rc;
     }
     return SE_OK;
@@ -1333,7 +1331,7 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
 // and of the FP64 operator apply, with HIP events on the context stream.
 // Per pass: microseconds per launch and flops per launch (2 M N K summed over
 // the members' padded sizes); the apply: microseconds and algorithmic bytes
-// (16 B per unknown for x and y, 4 B per field node for the numbering map).
+// (16 B per unknown for x and y).
 int fdm_plan_probe(FdmPlan* pl, int reps, double* us_pass, double* flops_pass, double* us_apply, double* bytes_apply) {
     SE_CHECK(pl->prec >= 0, SE_ERR_STATE, "FDM probe: run an FDM solve first");
     se_ctx* c = pl->ctx;
@@ -1363,7 +1361,7 @@ int fdm_plan_probe(FdmPlan* pl, int reps, double* us_pass, double* flops_pass, d
         us_pass[pass] = 1e3 * ms / reps;
     }
     double by = 0.0;
-    for (se_op* op : pl->ops) by += 16.0 * op->mesh->n + 4.0 * op->mesh->nV;
+    for (se_op* op : pl->ops) by += 16.0 * op->mesh->n;
     *bytes_apply = by;
     rc = plan_apply(pl, false, all, s);
     if (rc) return rc;

@@ -6,6 +6,7 @@
 #include <cstring>
 
 #include "apply_kernels.h"
+#include "apply_lines.h"
 #include "helm_kernels.h"
 #include "se_common.h"
 #include "se_op.h"
@@ -281,22 +282,18 @@ void launch_tile_n(int tile, const OpParams& P, const double* in, double* out, c
     else launch_tile_t<N, 2, 2, MODE, DOT>(P, in, out, aux, partials, st, s);
 }
 
-template <int N, int TX, int TY, bool DOT>
-void launch_apply_t(const OpParams& P, const std::vector<double>& D, const double* in, double* out, double* partials,
+template <int N, bool DOT>
+void launch_apply_t(const OpParams& P, const GLL& g, const double* in, double* out, double* partials,
                     const CgState* st, hipStream_t s) {
-    int tiles = ((P.nEls0 + TX - 1) / TX) * ((P.nEls1 + TY - 1) / TY);
-    hipLaunchKernelGGL((k_apply<N, TX, TY, DOT>), dim3(tiles), dim3(kBlock), 0, s, P, dmat_of<N>(D), in, out,
-                       partials, st);
+    hipLaunchKernelGGL((k_apply_lines<N, DOT>), dim3(line_tiles(N, P.nEls0, P.nEls1)), dim3(kBlock), 0, s, P, in,
+                       out, partials, st);
 }
 
 template <bool DOT>
-int launch_apply(int N, int tile, const OpParams& P, const std::vector<double>& D, const double* in, double* out,
-                 double* partials, const CgState* st, hipStream_t s) {
-#define SE_APPLY_CASE(NN)                                                             \
-    case NN:                                                                          \
-        if (tile == 4) launch_apply_t<NN, 4, 4, DOT>(P, D, in, out, partials, st, s); \
-        else launch_apply_t<NN, 2, 2, DOT>(P, D, in, out, partials, st, s);           \
-        break;
+int launch_apply(int N, const OpParams& P, const GLL& g, const double* in, double* out, double* partials,
+                 const CgState* st, hipStream_t s) {
+#define SE_APPLY_CASE(NN)                                              \
+    case NN: launch_apply_t<NN, DOT>(P, g, in, out, partials, st, s); break;
     switch (N) {
         SE_APPLY_CASE(2)
         SE_APPLY_CASE(3)
@@ -334,8 +331,7 @@ int vec_blocks(int n) {
 
 int apply_tiles(const se_op* op) {
     const se_mesh* m = op->mesh;
-    int t = op->tile;
-    return ((m->nEls[0] + t - 1) / t) * ((m->nEls[1] + t - 1) / t);
+    return line_tiles(m->N, m->nEls[0], m->nEls[1]);
 }
 
 }  // namespace
@@ -367,8 +363,15 @@ OpParams op_params(const se_op* op, double scale) {
     const double* basis = m->d_basis.as<double>();
     P.w = basis + n1;
     P.D = basis + 2 * n1;
+    P.DT = basis + 4 * n1 + n1 * n1;
     P.colTab = op->tabs.as<double>();
     P.rowTab = op->tabs.as<double>() + (size_t)m->nEls[0] * (4 * n1 + n1 * n1);
+    P.colQ = op->qtabs.as<float>();
+    P.rowQ = op->qtabs.as<float>() + (size_t)m->nEls[0] * 2 * n1 * n1;
+    P.nU0 = m->nU[0];
+    P.nU1 = m->nU[1];
+    P.off0 = m->off[0];
+    P.off1 = m->off[1];
     return P;
 }
 
@@ -424,6 +427,23 @@ static int op_metric_tables(se_op* op) {
     int rc = op->tabs.alloc(sizeof(double) * t.size());
     if (rc) return rc;
     SE_HIP(hipMemcpy(op->tabs.ptr, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
+    // the line apply's FP32 copies of the round-off-sized cross-term tables,
+    // each followed by its transpose: QX [qx][qy] | [qy][qx] per column,
+    // QY [qy][qx] | [qx][qy] per row
+    const int np = n1 * n1;
+    std::vector<float> qt((size_t)(m->nEls[0] + m->nEls[1]) * 2 * np);
+    for (int e = 0; e < m->nEls[0] + m->nEls[1]; e++) {
+        const double* Q = &t[(size_t)e * CS + 4 * n1];
+        float* o = &qt[(size_t)e * 2 * np];
+        for (int a = 0; a < n1; a++)
+            for (int b = 0; b < n1; b++) {
+                o[a * n1 + b] = (float)Q[a * n1 + b];
+                o[np + b * n1 + a] = (float)Q[a * n1 + b];
+            }
+    }
+    rc = op->qtabs.alloc(sizeof(float) * qt.size());
+    if (rc) return rc;
+    SE_HIP(hipMemcpy(op->qtabs.ptr, qt.data(), sizeof(float) * qt.size(), hipMemcpyHostToDevice));
     op->tabs_host = std::move(t);
     return SE_OK;
 }
@@ -459,15 +479,15 @@ int op_apply_dev(se_op* op, const double* d_x, double* d_y, double scale, bool d
     if (op->svv_mN > 0) {
         // mass + Laplacian, the SVV term, then the partials of x.y the CG
         // update reads (as many as the apply kernel would write)
-        int rc = launch_apply<false>(op->mesh->N, op->tile, P, op->mesh->gll.D, d_x, d_y, pa, op->state.as<CgState>(), s);
+        int rc = launch_apply<false>(op->mesh->N, P, op->mesh->gll, d_x, d_y, pa, op->state.as<CgState>(), s);
         if (rc) return rc;
         SE_HIP(hipGetLastError());
         rc = svv_apply(op, d_x, d_y, scale);
         if (!rc && dot) rc = svv_dot(op, d_x, d_y, pa, apply_tiles(op));
         return rc;
     }
-    int rc = dot ? launch_apply<true>(op->mesh->N, op->tile, P, op->mesh->gll.D, d_x, d_y, pa, op->state.as<CgState>(), s)
-                 : launch_apply<false>(op->mesh->N, op->tile, P, op->mesh->gll.D, d_x, d_y, pa, op->state.as<CgState>(), s);
+    int rc = dot ? launch_apply<true>(op->mesh->N, P, op->mesh->gll, d_x, d_y, pa, op->state.as<CgState>(), s)
+                 : launch_apply<false>(op->mesh->N, P, op->mesh->gll, d_x, d_y, pa, op->state.as<CgState>(), s);
     if (rc) return rc;
     SE_HIP(hipGetLastError());
     return SE_OK;

@@ -1,7 +1,7 @@
 // helm_kernels.h -- device code of the matrix-free GLL Helmholtz operator.
 //
 // Setup-time modes (lifting, right-hand side, diagonal) and a reference form
-// of the apply; the hot apply is k_apply in apply_kernels.h.
+// of the apply; the hot apply is k_apply_lines in apply_lines.h.
 //
 // One workgroup owns a tile of TX x TY elements and produces the assembled
 // result on the tile's nodes (the half-open node block of its elements plus,
@@ -40,10 +40,17 @@ struct OpParams {
     const double* J01;  // [nEls1][N+1 (qy)][N+1 (qx)]
     const double* w;    // [N+1]
     const double* D;    // [(N+1)^2]
+    const double* DT;   // [(N+1)^2], D transposed
     // metric factor tables of the hot apply, per element column / row
     // (stride 4(N+1) + (N+1)^2, layout in op_metric_tables)
     const double* colTab;
     const double* rowTab;
+    // the same cross-term tables in FP32, each followed by its transpose
+    // (stride 2 (N+1)^2): QX [qx][qy] | [qy][qx], QY [qy][qx] | [qx][qy]
+    const float* colQ;
+    const float* rowQ;
+    // the unknown grid: nU0 x nU1, first unknown field column / row
+    int nU0, nU1, off0, off1;
 };
 
 // Scalars of the CG recurrence, in device memory.  Written by block 0 of the

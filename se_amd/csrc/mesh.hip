@@ -439,6 +439,8 @@ extern "C" int se_mesh_create(se_ctx* ctx, const int nEls[2], int N, const doubl
     basis.insert(basis.end(), m->gll.D.begin(), m->gll.D.end());
     basis.insert(basis.end(), m->gll.pnx.begin(), m->gll.pnx.end());
     basis.insert(basis.end(), m->gll.pn.begin(), m->gll.pn.end());
+    for (int k = 0; k <= m->N; k++)  // D transposed
+        for (int q = 0; q <= m->N; q++) basis.push_back(m->gll.D[(size_t)q * (m->N + 1) + k]);
     rc = SE_OK;
     if (!rc) rc = upload(m->d_X, m->X.data(), m->X.size() * sizeof(double));
     if (!rc) rc = upload(m->d_Y, m->Y.data(), m->Y.size() * sizeof(double));

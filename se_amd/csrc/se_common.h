@@ -100,6 +100,6 @@ struct se_mesh {
     std::vector<double> J00, J10, J11, J01;
     se::DevBuf d_f2v, d_v2f, d_verts, d_X, d_Y;
     se::DevBuf d_J00, d_J10, d_J11, d_J01;
-    se::DevBuf d_basis;            // x, w, D, pnx, pn  (N+1, N+1, (N+1)^2, N+1, N+1 doubles)
+    se::DevBuf d_basis;            // x, w, D, pnx, pn, D^T  (N+1, N+1, (N+1)^2, N+1, N+1, (N+1)^2 doubles)
     se::DevBuf d_sl, d_red;        // SL velocity extrapolations, reduction scratch
 };

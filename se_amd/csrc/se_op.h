@@ -21,6 +21,7 @@ struct se_op {
     se::DevBuf lift;       // Dirichlet lifting b
     se::DevBuf tabs;       // metric factor tables: element columns, then element rows
     std::vector<double> tabs_host;
+    se::DevBuf qtabs;      // FP32 cross-term tables (and transposes): columns, then rows
     // separable inner operator (FP32 band tables) and mixed-precision work
     se::DevBuf sep, sep_state, sep_part;
     int sep_ready = 0, sep_sign = 0;
@@ -44,7 +45,7 @@ struct se_op {
 
     void release_all() {
         se::fdm_release(this);
-        se::DevBuf* bufs[] = {&sep, &sep_state, &sep_part, &dinv, &dinv_s, &lift, &tabs, &state, &partials, &b, &x, &r, &p, &q, &f,
+        se::DevBuf* bufs[] = {&sep, &sep_state, &sep_part, &dinv, &dinv_s, &lift, &tabs, &qtabs, &state, &partials, &b, &x, &r, &p, &q, &f,
                               &sx, &sy, &sfield, &sfield2, &lo_r, &lo_p, &lo_q, &lo_z, &lo_x, &lo_dinv,
                               &lo_geo, &d64, &svv_mk};
         for (auto* b : bufs) b->release();
