@@ -280,7 +280,11 @@ __global__ __launch_bounds__(256) void k_apply_lines(OpParams P, const double* _
                                                       double* __restrict__ out, double* __restrict__ partials,
                                                       const CgState* __restrict__ st) {
     if (DOT && cg_stopped(st)) return;
-    apply_lines_tile<N, DOT>(P, blockIdx.x, in, out, partials, gridDim.x);
+    // workgroup b runs on XCD b mod 8: give each XCD a contiguous run of
+    // tiles, so that neighbouring tiles' shared halo lines meet in one L2
+    const int T = gridDim.x, b = blockIdx.x, q = T / 8, r = T % 8, x = b % 8;
+    const int tile = x * q + min(x, r) + b / 8;
+    apply_lines_tile<N, DOT>(P, tile, in, out, partials, T);
 }
 
 // Batched: work items (member, tile) over members of one order
@@ -289,6 +293,7 @@ __global__ __launch_bounds__(256) void k_apply_lines_batch(const ApplyMember* __
                                                             const int2* __restrict__ work, int nwork) {
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
         const int2 it = work[w];
+        if (it.x < 0) continue;  // a padding item of an XCD-ordered list
         const ApplyMember& a = members[it.x];
         __syncthreads();  // the previous item's LDS reads are done
         apply_lines_tile<N, false>(a.P, it.y, a.in, a.out, nullptr, 0);

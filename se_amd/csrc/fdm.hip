@@ -32,6 +32,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "apply_kernels.h"
@@ -70,6 +71,11 @@ struct GemmTask {
     int arows, acols;
     int crows, ccols;                      // valid extent (SCALE: modes; FINAL: the unknown grid)
     int mode, tilesN;
+    // the member's CG state and r.r partials (null outside a CG solve): an
+    // iteration's transforms skip a member that no longer needs z
+    const double* pv;
+    const CgState* st;
+    int nch;
 };
 
 // K chunk per LDS stage (KC) and padded LDS row length (KP, a multiple of 16
@@ -283,15 +289,31 @@ __device__ __forceinline__ void gemm_tile(const GemmTask& t, int tile, char* lds
     }
 }
 
+// Inside a CG iteration (skip != 0): a member whose recursive residual
+// already meets its target, or whose iteration has stopped, needs no new
+// z = FDM(r).  The direction kernel that follows decides the same from the
+// same partials summed in the same order (and leaves z unused), so every
+// workgroup of the member's transforms skips it without a host round trip.
+__device__ __forceinline__ bool fdm_skip(const GemmTask& t) {
+    if (!t.pv) return false;
+    if (cg_stopped(t.st)) return true;
+    double tr[1];
+    reduce_partials<1, 256>(t.pv, t.nch, tr);
+    const double rr = tr[0];
+    return !(rr - rr == 0.0) || rr <= t.st->tol2;
+}
+
 // pass `pass` of the members' transforms; work items (member, tile), or
 // tiles of member 0 when work is null
 template <typename T>
 __global__ __launch_bounds__(256) void k_fdm_gemm(const GemmTask* __restrict__ tasks, int pass,
-                                                  const int2* __restrict__ work, int nwork) {
+                                                  const int2* __restrict__ work, int nwork, int skip) {
     __shared__ __attribute__((aligned(16))) char lds[gemm_lds_bytes<T>()];
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
         const int2 it = work ? work[w] : make_int2(0, w);
+        if (it.x < 0) continue;  // a padding item of the XCD-ordered list
         const GemmTask t = tasks[it.x * 4 + pass];
+        if (skip && fdm_skip(t)) continue;
         gemm_tile<T>(t, it.y, lds);
     }
 }
@@ -300,10 +322,12 @@ __global__ __launch_bounds__(256) void k_fdm_gemm(const GemmTask* __restrict__ t
 // padded nP1 x nP0 array, zeros outside; one 64 x 64 block per item
 template <typename T>
 __global__ __launch_bounds__(256) void k_fdm_pack(const GemmTask* __restrict__ tasks, const int2* __restrict__ work,
-                                                  int nwork) {
+                                                  int nwork, int skip) {
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
         const int2 it = work ? work[w] : make_int2(0, w);
+        if (it.x < 0) continue;
         const GemmTask& t = tasks[it.x * 4];
+        if (skip && fdm_skip(t)) continue;
         const int tilesN = t.K / 64;  // pass 1: K = nP0 columns
         const int tm = it.y / tilesN, tn = it.y - tm * tilesN;
         T* __restrict__ P = (T*)t.A;
@@ -602,6 +626,15 @@ int pad64(int n) { return std::max(64, (n + 63) / 64 * 64); }
 struct FdmPlan;
 void fdm_plan_destroy(FdmPlan* pl);
 
+// A 1-D eigenvector matrix on the device in one transform precision, padded
+// to nP x nP, and its transpose.  It depends only on the line operator (the
+// mesh line and c_lap, not c_mass), so every operator of the same line on one
+// device shares one copy: an ensemble of 512 members has a dozen distinct
+// ones.
+struct VMats {
+    DevBuf v, vT;
+};
+
 struct FdmOp {
     FdmPlan* self = nullptr;  // the one-member plan of this operator's own solves
     int nU0 = 0, nU1 = 0, nP0 = 0, nP1 = 0;
@@ -610,11 +643,11 @@ struct FdmOp {
     DevBuf z, pv, pz, pa, pt, ps;
     DevBuf st;               // CgState
     struct Prec {
-        DevBuf vx, vxT, vy, vyT, t1, t2;
+        std::shared_ptr<VMats> mx, my;  // Vx, Vy (shared)
+        DevBuf t1, t2;
         bool ready = false;
     } prec[3];
     std::shared_ptr<const se_fdm::Fdm1D> ex1, ey1;
-    std::vector<double> hx_vx, hx_vy;  // FP64 padded Vx, Vy (host), for the other precisions' uploads
 };
 
 void fdm_release(se_op* op) {
@@ -689,6 +722,29 @@ int upload_matrix(DevBuf& d, const std::vector<double>& h, int rows, int cols, b
 
 size_t elem_size(int prec) { return prec == FDM_F64 ? 8 : (prec == FDM_F32 ? 4 : 2); }
 
+std::mutex g_vm_mu;
+std::map<std::tuple<const se_fdm::Fdm1D*, int, int>, std::weak_ptr<VMats>> g_vm_cache;
+
+// the shared device copy of e's eigenvectors in precision prec on device dev
+int vmats_of(const std::shared_ptr<const se_fdm::Fdm1D>& e, int nP, int prec, int dev, std::shared_ptr<VMats>& out) {
+    std::lock_guard<std::mutex> lk(g_vm_mu);
+    const auto key = std::make_tuple(e.get(), prec, dev);
+    auto it = g_vm_cache.find(key);
+    if (it != g_vm_cache.end() && (out = it->second.lock())) return SE_OK;
+    for (auto i = g_vm_cache.begin(); i != g_vm_cache.end();) i = i->second.expired() ? g_vm_cache.erase(i) : std::next(i);
+    const int nU = e->nU;
+    std::vector<double> h((size_t)nP * nP, 0.0);
+    for (int i = 0; i < nU; i++)
+        for (int k = 0; k < nU; k++) h[(size_t)i * nP + k] = e->V[(size_t)i * nU + k];
+    auto vm = std::make_shared<VMats>();
+    int rc = upload_matrix(vm->v, h, nP, nP, false, prec);
+    if (!rc) rc = upload_matrix(vm->vT, h, nP, nP, true, prec);
+    if (rc) return rc;
+    g_vm_cache[key] = vm;
+    out = vm;
+    return SE_OK;
+}
+
 }  // namespace
 
 // FDM tables of op (FP64 part: eigenvectors, eigenvalues, mass factors and
@@ -748,12 +804,6 @@ int fdm_prepare(se_op* op, int prec) {
         // a mode whose D is this small is the constant null mode of a
         // singular (periodic / Neumann Poisson) operator
         F->dtol = 1e-11 * (lmax + lmy + emx * emy);
-        F->hx_vx.assign((size_t)F->nP0 * F->nP0, 0.0);
-        F->hx_vy.assign((size_t)F->nP1 * F->nP1, 0.0);
-        for (int i = 0; i < F->nU0; i++)
-            for (int k = 0; k < F->nU0; k++) F->hx_vx[(size_t)i * F->nP0 + k] = Ex.V[(size_t)i * F->nU0 + k];
-        for (int j = 0; j < F->nU1; j++)
-            for (int k = 0; k < F->nU1; k++) F->hx_vy[(size_t)j * F->nP1 + k] = Ey.V[(size_t)j * F->nU1 + k];
         rc = F->lx.alloc(sizeof(double) * F->nP0);
         if (!rc) rc = F->ex.alloc(sizeof(double) * F->nP0);
         if (!rc) rc = F->ly.alloc(sizeof(double) * F->nP1);
@@ -779,10 +829,9 @@ int fdm_prepare(se_op* op, int prec) {
     FdmOp* F = op->fdm;
     FdmOp::Prec& P = F->prec[prec];
     if (!P.ready) {
-        rc = upload_matrix(P.vx, F->hx_vx, F->nP0, F->nP0, false, prec);
-        if (!rc) rc = upload_matrix(P.vxT, F->hx_vx, F->nP0, F->nP0, true, prec);
-        if (!rc) rc = upload_matrix(P.vy, F->hx_vy, F->nP1, F->nP1, false, prec);
-        if (!rc) rc = upload_matrix(P.vyT, F->hx_vy, F->nP1, F->nP1, true, prec);
+        const int dev = m->ctx->device;
+        rc = vmats_of(F->ex1, F->nP0, prec, dev, P.mx);
+        if (!rc) rc = vmats_of(F->ey1, F->nP1, prec, dev, P.my);
         if (!rc) rc = P.t1.alloc(elem_size(prec) * (size_t)F->nP0 * F->nP1);
         if (!rc) rc = P.t2.alloc(elem_size(prec) * (size_t)F->nP0 * F->nP1);
         if (rc) return rc;
@@ -807,7 +856,7 @@ void fdm_tasks(se_op* op, int prec, const double* r, double* z, double* part, Ge
     t[0].arows = F->nU1;
     t[0].acols = F->nU0;
     t[0].lda = nP0;
-    t[0].Bt = P.vxT.ptr;
+    t[0].Bt = P.mx->vT.ptr;
     t[0].ldb = nP0;
     t[0].C = P.t1.ptr;
     t[0].ldc = nP1;
@@ -818,7 +867,7 @@ void fdm_tasks(se_op* op, int prec, const double* r, double* z, double* part, Ge
     // pass 2: T2[k'][j'] = sum_J T1t[k'][J] Vy[J][j'] / D(k', j')   (Bt = Vy^T)
     t[1].A = P.t1.ptr;
     t[1].lda = nP1;
-    t[1].Bt = P.vyT.ptr;
+    t[1].Bt = P.my->vT.ptr;
     t[1].ldb = nP1;
     t[1].C = P.t2.ptr;
     t[1].ldc = nP1;
@@ -835,7 +884,7 @@ void fdm_tasks(se_op* op, int prec, const double* r, double* z, double* part, Ge
     // pass 3: T3t[J][k'] = sum_j' T2[k'][j'] Vy[J][j']   (Bt = Vy, store transposed)
     t[2].A = P.t2.ptr;
     t[2].lda = nP1;
-    t[2].Bt = P.vy.ptr;
+    t[2].Bt = P.my->v.ptr;
     t[2].ldb = nP1;
     t[2].C = P.t1.ptr;
     t[2].ldc = nP0;
@@ -846,7 +895,7 @@ void fdm_tasks(se_op* op, int prec, const double* r, double* z, double* part, Ge
     // pass 4: Z[J][I] = sum_k' T3t[J][k'] Vx[I][k']   (Bt = Vx), FP64 on the unknown grid
     t[3].A = P.t1.ptr;
     t[3].lda = nP0;
-    t[3].Bt = P.vx.ptr;
+    t[3].Bt = P.mx->v.ptr;
     t[3].ldb = nP0;
     t[3].C = z;
     t[3].ldc = F->nU0;
@@ -874,17 +923,17 @@ bool fdm_tasks_valid(const GemmTask (&t)[4], int prec) {
 
 int fdm_tiles(const se_op* op) { return (op->fdm->nP0 / 64) * (op->fdm->nP1 / 64); }
 
-int launch_gemm(int prec, const GemmTask* tasks, int pass, const int2* work, int nwork, hipStream_t s) {
+int launch_gemm(int prec, const GemmTask* tasks, int pass, const int2* work, int nwork, hipStream_t s, int skip = 0) {
     if (nwork <= 0) return SE_OK;
     const dim3 grid(nwork), block(256);
     if (pass == 0) {  // the residual, packed for pass 1
-        if (prec == FDM_F64) hipLaunchKernelGGL(k_fdm_pack<double>, grid, block, 0, s, tasks, work, nwork);
-        else if (prec == FDM_F32) hipLaunchKernelGGL(k_fdm_pack<float>, grid, block, 0, s, tasks, work, nwork);
-        else hipLaunchKernelGGL(k_fdm_pack<__bf16>, grid, block, 0, s, tasks, work, nwork);
+        if (prec == FDM_F64) hipLaunchKernelGGL(k_fdm_pack<double>, grid, block, 0, s, tasks, work, nwork, skip);
+        else if (prec == FDM_F32) hipLaunchKernelGGL(k_fdm_pack<float>, grid, block, 0, s, tasks, work, nwork, skip);
+        else hipLaunchKernelGGL(k_fdm_pack<__bf16>, grid, block, 0, s, tasks, work, nwork, skip);
     }
-    if (prec == FDM_F64) hipLaunchKernelGGL(k_fdm_gemm<double>, grid, block, 0, s, tasks, pass, work, nwork);
-    else if (prec == FDM_F32) hipLaunchKernelGGL(k_fdm_gemm<float>, grid, block, 0, s, tasks, pass, work, nwork);
-    else hipLaunchKernelGGL(k_fdm_gemm<__bf16>, grid, block, 0, s, tasks, pass, work, nwork);
+    if (prec == FDM_F64) hipLaunchKernelGGL(k_fdm_gemm<double>, grid, block, 0, s, tasks, pass, work, nwork, skip);
+    else if (prec == FDM_F32) hipLaunchKernelGGL(k_fdm_gemm<float>, grid, block, 0, s, tasks, pass, work, nwork, skip);
+    else hipLaunchKernelGGL(k_fdm_gemm<__bf16>, grid, block, 0, s, tasks, pass, work, nwork, skip);
     return SE_OK;
 }
 
@@ -938,6 +987,55 @@ int launch_apply_batch(int N, const GLL& g, const ApplyMember* A, const int2* wo
     return SE_OK;
 }
 
+// Work lists XCD by XCD.  The dispatcher hands workgroup b to XCD b mod 8
+// (one workgroup per item in these launches), so the item at position 8 j + x
+// runs on XCD x.  A member's tiles go to one XCD when it is small against
+// the whole list -- its operand panels or halo lines then stay in that XCD's
+// L2 -- and are cut into contiguous runs spread over the XCDs otherwise (a
+// single large member keeps all eight busy).  Runs are placed by longest
+// processing time on their cost; within an XCD, runs of the same class
+// (sharing eigenvector matrices) are adjacent.  Positions a shorter XCD list
+// leaves hold (-1, -1) items, which the kernels skip.
+constexpr int kXcds = 8;
+void xcd_interleave(const std::vector<int>& ids, const std::vector<int>& tiles, const std::vector<double>& cost,
+                    const std::vector<const void*>& cls, std::vector<int2>& out) {
+    struct Run {
+        int id, t0, t1;
+        double cost;
+        const void* cls;
+    };
+    long total = 0;
+    for (int t : tiles) total += t;
+    const int cap = (int)std::max<long>(1, (total + 2 * kXcds - 1) / (2 * kXcds));  // tiles per run at most
+    std::vector<Run> runs;
+    for (size_t u = 0; u < ids.size(); u++) {
+        const int n = tiles[u], parts = (n + cap - 1) / cap;
+        for (int k = 0; k < parts; k++) {
+            const int t0 = (int)((long)n * k / parts), t1 = (int)((long)n * (k + 1) / parts);
+            runs.push_back({ids[u], t0, t1, cost[u] * (t1 - t0) / std::max(n, 1), cls[u]});
+        }
+    }
+    std::stable_sort(runs.begin(), runs.end(), [](const Run& a, const Run& b) { return a.cost > b.cost; });
+    std::vector<std::vector<const Run*>> x(kXcds);
+    std::vector<double> load(kXcds, 0.0);
+    for (const Run& r : runs) {
+        const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        x[k].push_back(&r);
+        load[k] += r.cost;
+    }
+    std::vector<std::vector<int2>> items(kXcds);
+    size_t len = 0;
+    for (int k = 0; k < kXcds; k++) {
+        std::stable_sort(x[k].begin(), x[k].end(), [](const Run* a, const Run* b) { return a->cls < b->cls; });
+        for (const Run* r : x[k])
+            for (int t = r->t0; t < r->t1; t++) items[k].push_back(make_int2(r->id, t));
+        len = std::max(len, items[k].size());
+    }
+    out.clear();
+    for (size_t j = 0; j < len; j++)
+        for (int k = 0; k < kXcds; k++) out.push_back(j < items[k].size() ? items[k][j] : make_int2(-1, -1));
+}
+
 // q = sign A p (or sign A x) of the members flagged in `on`.  The host lists
 // are rebuilt only after the stream has been synchronised (every caller's
 // loop pulls the states in between), so no pending copy reads a changed list.
@@ -954,14 +1052,19 @@ int plan_apply(FdmPlan* pl, bool from_x, const std::vector<char>& on, hipStream_
             if (rc) return rc;
             continue;
         }
-        o.h_work.clear();
+        std::vector<int> ids, tiles;
+        std::vector<double> cost;
+        std::vector<const void*> cls;
         for (int j = 0; j < (int)o.members.size(); j++) {
             const int i = o.members[j];
             if (!on[i]) continue;
             const se_mesh* m = pl->ops[i]->mesh;
-            const int tiles = line_tiles(m->N, m->nEls[0], m->nEls[1]);
-            for (int t = 0; t < tiles; t++) o.h_work.push_back(make_int2(j, t));
+            ids.push_back(j);
+            tiles.push_back(line_tiles(m->N, m->nEls[0], m->nEls[1]));
+            cost.push_back((double)tiles.back());
+            cls.push_back(nullptr);
         }
+        xcd_interleave(ids, tiles, cost, cls, o.h_work);
         if (o.h_work.empty()) continue;
         int rc = o.d_work.alloc(sizeof(int2) * o.h_work.size());
         if (rc) return rc;
@@ -971,6 +1074,22 @@ int plan_apply(FdmPlan* pl, bool from_x, const std::vector<char>& on, hipStream_
         if (rc) return rc;
     }
     return SE_OK;
+}
+
+// the transforms' list: members' FLOPs as cost, grouped by x eigenvectors
+void xcd_order(const FdmPlan* pl, const std::vector<char>& on, std::vector<int2>& out) {
+    std::vector<int> ids, tiles;
+    std::vector<double> cost;
+    std::vector<const void*> cls;
+    for (int i = 0; i < (int)pl->ops.size(); i++) {
+        if (!on[i]) continue;
+        const FdmOp* F = pl->ops[i]->fdm;
+        ids.push_back(i);
+        tiles.push_back(pl->h_members[i].ntz);
+        cost.push_back((double)F->nP0 * F->nP1 * (F->nP0 + F->nP1));
+        cls.push_back(F->ex1.get());
+    }
+    xcd_interleave(ids, tiles, cost, cls, out);
 }
 
 // work lists of the members flagged in `on`
@@ -984,8 +1103,8 @@ int plan_lists(FdmPlan* pl, const std::vector<char>& on, hipStream_t s) {
         if (!on[i]) continue;
         pl->h_list.push_back(i);
         for (int c = 0; c < pl->h_members[i].nch; c++) pl->h_vwork.push_back(make_int2(i, c));
-        for (int t = 0; t < pl->h_members[i].ntz; t++) pl->h_gwork.push_back(make_int2(i, t));
     }
+    xcd_order(pl, on, pl->h_gwork);
     int rc = pl->d_vwork.alloc(sizeof(int2) * std::max<size_t>(pl->h_vwork.size(), 1));
     if (!rc) rc = pl->d_gwork.alloc(sizeof(int2) * std::max<size_t>(pl->h_gwork.size(), 1));
     if (!rc) rc = pl->d_list.alloc(sizeof(int) * std::max<size_t>(pl->h_list.size(), 1));
@@ -1007,10 +1126,10 @@ int grid_of(size_t items, const se_ctx* c) {
 }
 
 // z = FDM(r) of the listed members: four passes
-int plan_fdm(FdmPlan* pl, hipStream_t s) {
+int plan_fdm(FdmPlan* pl, hipStream_t s, int skip = 0) {
     const int nw = (int)pl->h_gwork.size();
     for (int pass = 0; pass < 4; pass++) {
-        int rc = launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, pl->d_gwork.as<int2>(), nw, s);
+        int rc = launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, pl->d_gwork.as<int2>(), nw, s, skip);
         if (rc) return rc;
     }
     return SE_OK;
@@ -1076,6 +1195,10 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
         if (rc) return rc;
     }
     // descriptors (f changes per call; the rest is fixed per plan and precision)
+    {
+        int rc = pl->d_states.alloc(sizeof(CgState) * std::max(count, 1));
+        if (rc) return rc;
+    }
     pl->h_members.assign(count, FbMember{});
     std::vector<GemmTask> tasks(4 * (size_t)count);
     for (int i = 0; i < count; i++) {
@@ -1101,14 +1224,18 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
         M.null_space = null;
         M.sgn = (double)op->sign;
         M.rtol = o.rtol;
+        M.st = pl->d_states.as<CgState>() + i;
         GemmTask t4[4];
         fdm_tasks(op, prec, M.r, M.z, M.pz, t4);
         SE_CHECK(fdm_tasks_valid(t4, prec), SE_ERR_STATE, "FDM: transform shapes break the kernels' assumptions");
-        for (int k = 0; k < 4; k++) tasks[4 * (size_t)i + k] = t4[k];
+        for (int k = 0; k < 4; k++) {
+            t4[k].pv = M.pv;
+            t4[k].st = M.st;
+            t4[k].nch = M.nch;
+            tasks[4 * (size_t)i + k] = t4[k];
+        }
     }
-    int rc = pl->d_states.alloc(sizeof(CgState) * std::max(count, 1));
-    if (rc) return rc;
-    for (int i = 0; i < count; i++) pl->h_members[i].st = pl->d_states.as<CgState>() + i;
+    int rc = SE_OK;
     if (count > 0) {
         SE_HIP(hipMemcpyAsync(pl->d_members.ptr, pl->h_members.data(), sizeof(FbMember) * count, hipMemcpyHostToDevice,
                               s));
@@ -1201,7 +1328,7 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
         if (rc) return rc;
         vlaunch(k_fb_dot);
         vlaunch(k_fb_update);
-        rc = plan_fdm(pl, s);
+        rc = plan_fdm(pl, s, 1);
         if (rc) return rc;
         dlaunch(FB_NORMAL);
         SE_HIP(hipGetLastError());
