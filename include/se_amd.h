@@ -140,8 +140,10 @@ int se_helm_destroy(se_op* op);
  * above `cutoff` (1 .. N-1) in both directions, Lyy built from the x terms as
  * the reference builds it [ref: HelmholtzEqn.cpp:71-76, SVV::AssembleElement
  * src/base/SVV.cpp:120-176].  Before set_dirichlet; once per operator.  Such
- * operators solve in FP64 (the mixed-precision and batched solvers refuse
- * them). */
+ * operators solve with FP64 Jacobi-PCG, or with SE_PC_FDM in any precision,
+ * alone or batched (the FDM is that of the operator without the SVV term; the
+ * CG applies the exact operator); the mixed-precision Jacobi-PCG refuses
+ * them. */
 int se_helm_set_svv(se_op* op, double dt, int cutoff);
 /* y = A x   [ref: MatMult on the matrix built by Matrix::Assemble
  *            src/base/Matrix.cpp:47-133] */

@@ -69,6 +69,9 @@ struct Group {
 struct se_batch {
     se_ctx* ctx = nullptr;
     std::vector<se_op*> ops;
+    // false when a member has an SVV term: the inner Jacobi-PCG's separable
+    // operator cannot carry it, so such a batch solves with FDM only
+    bool jacobi_ok = true;
     std::vector<Group> groups;
     DevBuf d_states;
     CgState* h_states = nullptr;
@@ -510,7 +513,6 @@ extern "C" int se_batch_create(se_ctx* ctx, int count, se_op* const* ops, se_bat
         SE_CHECK(ops[i] != nullptr, SE_ERR_ARG, "se_batch_create: NULL member");
         SE_CHECK(ops[i]->mesh->ctx == ctx, SE_ERR_ARG, "se_batch_create: member created on another context");
         SE_CHECK(ops[i]->definite, SE_ERR_UNSUPPORTED, "se_batch_create: member operator is not definite");
-        SE_CHECK(ops[i]->svv_mN == 0, SE_ERR_UNSUPPORTED, "se_batch_create: members with SVV solve in FP64 alone");
         SE_CHECK(ops[i]->mesh->N >= 2 && ops[i]->mesh->N <= 8, SE_ERR_UNSUPPORTED,
                  "se_batch_create: orders 2..8");
         for (int j = 0; j < i; j++)
@@ -519,8 +521,10 @@ extern "C" int se_batch_create(se_ctx* ctx, int count, se_op* const* ops, se_bat
     se_batch* b = new se_batch;
     b->ctx = ctx;
     b->ops.assign(ops, ops + count);
+    for (int i = 0; i < count; i++)
+        if (ops[i]->svv_mN > 0) b->jacobi_ok = false;
     int rc = SE_OK;
-    for (int i = 0; i < count && !rc; i++) rc = tcg_prepare(ops[i]);
+    for (int i = 0; i < count && !rc && b->jacobi_ok; i++) rc = tcg_prepare(ops[i]);
     if (!rc) rc = b->d_states.alloc(sizeof(CgState) * std::max(count, 1));
     if (!rc && hipHostMalloc((void**)&b->h_states, sizeof(CgState) * std::max(count, 1), hipHostMallocDefault) !=
                    hipSuccess)
@@ -528,7 +532,7 @@ extern "C" int se_batch_create(se_ctx* ctx, int count, se_op* const* ops, se_bat
     if (!rc && (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
                 hipEventCreate(&b->t0) != hipSuccess || hipEventCreate(&b->t1) != hipSuccess))
         rc = fail(SE_ERR_HIP, "se_batch_create: events");
-    for (int N = 2; N <= 8 && !rc; N++) {
+    for (int N = 2; N <= 8 && !rc && b->jacobi_ok; N++) {
         Group g;
         g.N = N;
         for (int i = 0; i < count; i++)
@@ -669,6 +673,8 @@ extern "C" int se_batch_pcg_dev(se_batch* b, const double* const* d_f, double* c
         }
         return fdm_plan_solve(b->fdm, d_f, d_x, o, infos);
     }
+    SE_CHECK(b->jacobi_ok, SE_ERR_UNSUPPORTED,
+             "se_batch_pcg_dev: a batch with SVV members solves with the FDM preconditioner");
     if (o.precision == SE_PREC_FP32_IR) return batch_solve<float>(b, d_f, d_x, o, infos);
     return batch_solve<__hip_bfloat16>(b, d_f, d_x, o, infos);
 }

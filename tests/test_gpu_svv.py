@@ -5,7 +5,9 @@ reference's 2-D construction literally (pinned in tests/test_oracle_svv.py).
 
 The device applies the separable 1-D factors with an owner-computes gather;
 apply, inverse diagonal and Dirichlet lifting agree with the CSR to 1e-12
-relative, solves to the solver tolerance.  A vector (2-dof) field is two
+relative, solves to the solver tolerance: FP64 Jacobi-PCG, and CG preconditioned
+by the fast diagonalisation of the operator without its SVV term (any
+transform precision, alone or batched).  A vector (2-dof) field is two
 such operators, one per dof with that dof's Dirichlet sides: the reference's
 2-dof system (BCs.cpp:176-322) is block diagonal, dof 0's unknowns first."""
 import numpy as np
@@ -61,12 +63,54 @@ def test_svv_solve_matches_oracle(gpu_ctx, case):
     assert np.max(np.abs(got - want)) <= 1e-9 * np.max(np.abs(want))
 
 
-def test_svv_refuses_mixed_precision_and_batching(gpu_ctx):
+# FDM of the operator without its SVV term preconditions the exact operator
+# (kappa(A0^-1 A) 1.02-2.45 on these cases, measured with the oracle's CSRs)
+FDM_PRECS = [(se_amd.PREC_FP64, 30), (se_amd.PREC_FP32_IR, 30), (se_amd.PREC_BF16_IR, 150)]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "N%d-mN%d-per%d%d" % (c[1], c[4], *c[2]))
+def test_svv_solves_with_fdm_in_every_precision(gpu_ctx, case):
+    m, om, oh, h, field, rng = build(gpu_ctx, *case)
+    f = rng.standard_normal(om.n)
+    for prec, itmax in FDM_PRECS:
+        x, info = h.pcg(f, rtol=1e-10, precision=prec, preconditioner=se_amd.PC_FDM)
+        print("prec %d: %d iterations, true rel residual %.2e" % (prec, info["iters"], info["true_rel_residual"]))
+        assert info["status"] == se_amd.SOLVE_CONVERGED, info
+        assert info["iters"] <= itmax, info
+        assert np.linalg.norm(f - oh.matvec(x)) <= 1.5e-10 * np.linalg.norm(f)
+
+
+def test_svv_members_batch_with_fdm(gpu_ctx):
+    """SVV and plain members in one FDM batch: each result bitwise its single solve."""
+    hs, ohs, fs = [], [], []
+    for k, case in enumerate(CASES):
+        m, om, oh, h, field, rng = build(gpu_ctx, *case, seed=10 + k)
+        hs.append(h)
+        ohs.append(oh)
+        fs.append(rng.standard_normal(om.n))
+    m = se_amd.Mesh(gpu_ctx, (6, 6), 5, (0.0, 0.0), (2.0, 1.0), (1, 0), (1, 1, 0, 0))
+    om = O.OracleMesh((6, 6), 5, (0.0, 0.0), (2.0, 1.0), (1, 0), (1, 1, 0, 0))
+    hs.append(se_amd.Helm(m, 3.0, 0.02, 1))
+    ohs.append(O.Helm(om, 3.0, 0.02, 1))
+    fs.append(np.random.default_rng(3).standard_normal(om.n))
+    b = se_amd.Batch(gpu_ctx, hs)
+    for prec in (se_amd.PREC_FP64, se_amd.PREC_FP32_IR):
+        xs, infos = b.pcg(fs, precision=prec, rtol=1e-10, preconditioner=se_amd.PC_FDM)
+        for h, oh, f, xb, ib in zip(hs, ohs, fs, xs, infos):
+            assert ib["status"] == se_amd.SOLVE_CONVERGED, ib
+            assert np.linalg.norm(f - oh.matvec(xb)) <= 1.5e-10 * np.linalg.norm(f)
+            x1, i1 = h.pcg(f, rtol=1e-10, precision=prec, preconditioner=se_amd.PC_FDM)
+            assert i1["iters"] == ib["iters"] and np.array_equal(x1, xb)
+    # the inner Jacobi-PCG's separable operator cannot carry the SVV term
+    with pytest.raises(se_amd.SEError):
+        b.pcg(fs, precision=se_amd.PREC_FP32_IR, rtol=1e-10, preconditioner=se_amd.PC_JACOBI)
+    b.close()
+
+
+def test_svv_refusals(gpu_ctx):
     m, om, oh, h, field, rng = build(gpu_ctx, *CASES[0])
-    with pytest.raises(se_amd.SEError):
-        h.pcg(rng.standard_normal(om.n), precision=se_amd.PREC_FP32_IR)
-    with pytest.raises(se_amd.SEError):
-        se_amd.Batch(gpu_ctx, [h])
+    with pytest.raises(se_amd.SEError):  # mixed-precision Jacobi-PCG
+        h.pcg(rng.standard_normal(om.n), precision=se_amd.PREC_FP32_IR, preconditioner=se_amd.PC_JACOBI)
     with pytest.raises(se_amd.SEError):
         h.set_svv(0.05, 2)  # once per operator
     with pytest.raises(se_amd.SEError):
