@@ -959,17 +959,34 @@ struct FdmPlan {
         int N;
         std::vector<int> members;
         DevBuf d_ap, d_ax;  // ApplyMember[]
-        DevBuf d_work;      // (local member, tile) items of the active members
-        std::vector<int2> h_work;
     };
     std::vector<Order> orders;   // one per order 2..8 (members may be empty)
-    DevBuf d_vwork, d_gwork, d_list;  // active (member, chunk), (member, tile) items; member list
-    std::vector<int2> h_vwork, h_gwork;
-    std::vector<int> h_list;
+    // Work lists of one member subset: (member, chunk) items of the vector
+    // kernels, (member, tile) items of the transforms, the member list, and per
+    // order the apply's (local member, tile) items.  Three independent sets --
+    // the running members' iteration, the true-residual checks, restarts and
+    // the end of a solve -- so that lists launched together in one round never
+    // share host vectors: a set's host vectors are rebuilt only after a
+    // stream synchronisation has completed the uploads from them.
+    struct ListSet {
+        std::vector<char> on, apply_on;  // member flags the device lists were built for
+        std::vector<int2> h_vwork, h_gwork;
+        std::vector<int> h_list;
+        DevBuf d_vwork, d_gwork, d_list;
+        std::vector<int2> h_apply[7];
+        DevBuf d_apply[7];
+    };
+    enum { SET_ITER = 0, SET_CHECK = 1, SET_OTHER = 2 };
+    ListSet sets[3];
     CgState* h_states = nullptr;  // pinned, h_cap entries
     int h_cap = 0;
     DevBuf d_states;
-    std::vector<char> lists_on, apply_on;  // member flags the device lists were built for
+    // the device descriptors (members, transforms, applies) were built for
+    // these members and this precision, null-space flag and tolerance; a
+    // later solve with the same ones only patches f and x
+    bool desc_ok = false;
+    int desc_null = -1;
+    double desc_rtol = 0.0;
 
     ~FdmPlan() {
         if (h_states) (void)hipHostFree(h_states);
@@ -1040,41 +1057,40 @@ void xcd_interleave(const std::vector<int>& ids, const std::vector<int>& tiles, 
         for (int k = 0; k < kXcds; k++) out.push_back(j < items[k].size() ? items[k][j] : make_int2(-1, -1));
 }
 
-// q = sign A p (or sign A x) of the members flagged in `on`.  The host lists
-// are rebuilt only after the stream has been synchronised (every caller's
-// loop pulls the states in between), so no pending copy reads a changed list.
-int plan_apply(FdmPlan* pl, bool from_x, const std::vector<char>& on, hipStream_t s) {
-    const bool same = on == pl->apply_on;
-    if (!same) pl->apply_on = on;
-    for (auto& o : pl->orders) {
+// q = sign A p (or sign A x) of the members flagged in `on`, with list set
+// `set` (its device lists rebuilt when the flags changed).
+int plan_apply(FdmPlan* pl, int set, bool from_x, const std::vector<char>& on, hipStream_t s) {
+    FdmPlan::ListSet& L = pl->sets[set];
+    const bool same = on == L.apply_on;
+    if (!same) L.apply_on = on;
+    for (int k = 0; k < (int)pl->orders.size(); k++) {
+        const FdmPlan::Order& o = pl->orders[k];
         if (o.members.empty()) continue;
-        if (same) {
-            if (o.h_work.empty()) continue;
-            const ApplyMember* A = from_x ? o.d_ax.as<ApplyMember>() : o.d_ap.as<ApplyMember>();
-            const int nw = (int)o.h_work.size();
-            int rc = launch_apply_batch(o.N, pl->ops[o.members[0]]->mesh->gll, A, o.d_work.as<int2>(), nw, s);
-            if (rc) return rc;
-            continue;
+        std::vector<int2>& hw = L.h_apply[k];
+        if (!same) {
+            std::vector<int> ids, tiles;
+            std::vector<double> cost;
+            std::vector<const void*> cls;
+            for (int j = 0; j < (int)o.members.size(); j++) {
+                const int i = o.members[j];
+                if (!on[i]) continue;
+                const se_mesh* m = pl->ops[i]->mesh;
+                ids.push_back(j);
+                tiles.push_back(line_tiles(m->N, m->nEls[0], m->nEls[1]));
+                cost.push_back((double)tiles.back());
+                cls.push_back(nullptr);
+            }
+            xcd_interleave(ids, tiles, cost, cls, hw);
+            if (!hw.empty()) {
+                int rc = L.d_apply[k].alloc(sizeof(int2) * hw.size());
+                if (rc) return rc;
+                SE_HIP(hipMemcpyAsync(L.d_apply[k].ptr, hw.data(), sizeof(int2) * hw.size(), hipMemcpyHostToDevice, s));
+            }
         }
-        std::vector<int> ids, tiles;
-        std::vector<double> cost;
-        std::vector<const void*> cls;
-        for (int j = 0; j < (int)o.members.size(); j++) {
-            const int i = o.members[j];
-            if (!on[i]) continue;
-            const se_mesh* m = pl->ops[i]->mesh;
-            ids.push_back(j);
-            tiles.push_back(line_tiles(m->N, m->nEls[0], m->nEls[1]));
-            cost.push_back((double)tiles.back());
-            cls.push_back(nullptr);
-        }
-        xcd_interleave(ids, tiles, cost, cls, o.h_work);
-        if (o.h_work.empty()) continue;
-        int rc = o.d_work.alloc(sizeof(int2) * o.h_work.size());
-        if (rc) return rc;
-        SE_HIP(hipMemcpyAsync(o.d_work.ptr, o.h_work.data(), sizeof(int2) * o.h_work.size(), hipMemcpyHostToDevice, s));
+        if (hw.empty()) continue;
         const ApplyMember* A = from_x ? o.d_ax.as<ApplyMember>() : o.d_ap.as<ApplyMember>();
-        rc = launch_apply_batch(o.N, pl->ops[o.members[0]]->mesh->gll, A, o.d_work.as<int2>(), (int)o.h_work.size(), s);
+        int rc = launch_apply_batch(o.N, pl->ops[o.members[0]]->mesh->gll, A, L.d_apply[k].as<int2>(), (int)hw.size(),
+                                    s);
         if (rc) return rc;
     }
     // members with an SVV term: add it to the batched apply's result
@@ -1104,32 +1120,31 @@ void xcd_order(const FdmPlan* pl, const std::vector<char>& on, std::vector<int2>
     xcd_interleave(ids, tiles, cost, cls, out);
 }
 
-// work lists of the members flagged in `on`
-int plan_lists(FdmPlan* pl, const std::vector<char>& on, hipStream_t s) {
-    if (on == pl->lists_on) return SE_OK;  // the device lists are current
-    pl->lists_on = on;
-    pl->h_vwork.clear();
-    pl->h_gwork.clear();
-    pl->h_list.clear();
+// work lists of the members flagged in `on`, in list set `set`
+int plan_lists(FdmPlan* pl, int set, const std::vector<char>& on, hipStream_t s) {
+    FdmPlan::ListSet& L = pl->sets[set];
+    if (on == L.on) return SE_OK;  // the device lists are current
+    L.on = on;
+    L.h_vwork.clear();
+    L.h_list.clear();
     for (int i = 0; i < (int)pl->ops.size(); i++) {
         if (!on[i]) continue;
-        pl->h_list.push_back(i);
-        for (int c = 0; c < pl->h_members[i].nch; c++) pl->h_vwork.push_back(make_int2(i, c));
+        L.h_list.push_back(i);
+        for (int c = 0; c < pl->h_members[i].nch; c++) L.h_vwork.push_back(make_int2(i, c));
     }
-    xcd_order(pl, on, pl->h_gwork);
-    int rc = pl->d_vwork.alloc(sizeof(int2) * std::max<size_t>(pl->h_vwork.size(), 1));
-    if (!rc) rc = pl->d_gwork.alloc(sizeof(int2) * std::max<size_t>(pl->h_gwork.size(), 1));
-    if (!rc) rc = pl->d_list.alloc(sizeof(int) * std::max<size_t>(pl->h_list.size(), 1));
+    xcd_order(pl, on, L.h_gwork);
+    int rc = L.d_vwork.alloc(sizeof(int2) * std::max<size_t>(L.h_vwork.size(), 1));
+    if (!rc) rc = L.d_gwork.alloc(sizeof(int2) * std::max<size_t>(L.h_gwork.size(), 1));
+    if (!rc) rc = L.d_list.alloc(sizeof(int) * std::max<size_t>(L.h_list.size(), 1));
     if (rc) return rc;
-    if (!pl->h_vwork.empty())
-        SE_HIP(hipMemcpyAsync(pl->d_vwork.ptr, pl->h_vwork.data(), sizeof(int2) * pl->h_vwork.size(),
-                              hipMemcpyHostToDevice, s));
-    if (!pl->h_gwork.empty())
-        SE_HIP(hipMemcpyAsync(pl->d_gwork.ptr, pl->h_gwork.data(), sizeof(int2) * pl->h_gwork.size(),
-                              hipMemcpyHostToDevice, s));
-    if (!pl->h_list.empty())
-        SE_HIP(hipMemcpyAsync(pl->d_list.ptr, pl->h_list.data(), sizeof(int) * pl->h_list.size(),
-                              hipMemcpyHostToDevice, s));
+    if (!L.h_vwork.empty())
+        SE_HIP(hipMemcpyAsync(L.d_vwork.ptr, L.h_vwork.data(), sizeof(int2) * L.h_vwork.size(), hipMemcpyHostToDevice,
+                              s));
+    if (!L.h_gwork.empty())
+        SE_HIP(hipMemcpyAsync(L.d_gwork.ptr, L.h_gwork.data(), sizeof(int2) * L.h_gwork.size(), hipMemcpyHostToDevice,
+                              s));
+    if (!L.h_list.empty())
+        SE_HIP(hipMemcpyAsync(L.d_list.ptr, L.h_list.data(), sizeof(int) * L.h_list.size(), hipMemcpyHostToDevice, s));
     return SE_OK;
 }
 
@@ -1138,10 +1153,11 @@ int grid_of(size_t items, const se_ctx* c) {
 }
 
 // z = FDM(r) of the listed members: four passes
-int plan_fdm(FdmPlan* pl, hipStream_t s, int skip = 0) {
-    const int nw = (int)pl->h_gwork.size();
+int plan_fdm(FdmPlan* pl, int set, hipStream_t s, int skip = 0) {
+    const FdmPlan::ListSet& L = pl->sets[set];
+    const int nw = (int)L.h_gwork.size();
     for (int pass = 0; pass < 4; pass++) {
-        int rc = launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, pl->d_gwork.as<int2>(), nw, s, skip);
+        int rc = launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, L.d_gwork.as<int2>(), nw, s, skip);
         if (rc) return rc;
     }
     return SE_OK;
@@ -1159,10 +1175,7 @@ int fdm_plan_set_members(FdmPlan* pl, int count, se_op* const* ops) {
         pl->orders.resize(7);
         for (int k = 0; k < 7; k++) pl->orders[k].N = k + 2;
     }
-    for (auto& o : pl->orders) {
-        o.members.clear();
-        o.h_work.clear();
-    }
+    for (auto& o : pl->orders) o.members.clear();
     for (int i = 0; i < count; i++) {
         const int N = ops[i]->mesh->N;
         SE_CHECK(N >= 2 && N <= 8, SE_ERR_UNSUPPORTED, "FDM solve built for orders 2..8");
@@ -1179,8 +1192,11 @@ int fdm_plan_set_members(FdmPlan* pl, int count, se_op* const* ops) {
     if (!rc) rc = pl->d_tasks.alloc(sizeof(GemmTask) * 4 * std::max(count, 1));
     if (rc) return rc;
     pl->prec = -1;
-    pl->lists_on.clear();
-    pl->apply_on.clear();
+    pl->desc_ok = false;
+    for (auto& L : pl->sets) {
+        L.on.clear();
+        L.apply_on.clear();
+    }
     return SE_OK;
 }
 
@@ -1206,74 +1222,96 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
         int rc = fdm_prepare(pl->ops[i], prec);
         if (rc) return rc;
     }
-    // descriptors (f changes per call; the rest is fixed per plan and precision)
-    {
-        int rc = pl->d_states.alloc(sizeof(CgState) * std::max(count, 1));
-        if (rc) return rc;
-    }
-    pl->h_members.assign(count, FbMember{});
-    std::vector<GemmTask> tasks(4 * (size_t)count);
-    for (int i = 0; i < count; i++) {
-        se_op* op = pl->ops[i];
-        FdmOp* F = op->fdm;
-        FbMember& M = pl->h_members[i];
-        M.f = d_f[i];
-        M.b = op->b.as<double>();
-        M.x = op->x.as<double>();
-        M.r = op->r.as<double>();
-        M.p = op->p.as<double>();
-        M.q = op->q.as<double>();
-        M.z = F->z.as<double>();
-        M.pv = F->pv.as<double>();
-        M.pz = F->pz.as<double>();
-        M.pa = F->pa.as<double>();
-        M.pt = F->pt.as<double>();
-        M.ps = F->ps.as<double>();
-        M.xo = d_x[i];
-        M.n = op->mesh->n;
-        M.nch = (std::max(M.n, 1) + FB_CH - 1) / FB_CH;
-        M.ntz = fdm_tiles(op);
-        M.null_space = null;
-        M.sgn = (double)op->sign;
-        M.rtol = o.rtol;
-        M.st = pl->d_states.as<CgState>() + i;
-        GemmTask t4[4];
-        fdm_tasks(op, prec, M.r, M.z, M.pz, t4);
-        SE_CHECK(fdm_tasks_valid(t4, prec), SE_ERR_STATE, "FDM: transform shapes break the kernels' assumptions");
-        for (int k = 0; k < 4; k++) {
-            t4[k].pv = M.pv;
-            t4[k].st = M.st;
-            t4[k].nch = M.nch;
-            tasks[4 * (size_t)i + k] = t4[k];
-        }
-    }
+    // descriptors: built once per members / precision / null flag / tolerance;
+    // a later solve patches only the right-hand sides and the results
     int rc = SE_OK;
-    if (count > 0) {
-        SE_HIP(hipMemcpyAsync(pl->d_members.ptr, pl->h_members.data(), sizeof(FbMember) * count, hipMemcpyHostToDevice,
-                              s));
-        SE_HIP(hipMemcpyAsync(pl->d_tasks.ptr, tasks.data(), sizeof(GemmTask) * 4 * count, hipMemcpyHostToDevice, s));
-    }
-    pl->prec = prec;
-    // FP64 apply descriptors
-    for (auto& od : pl->orders) {
-        if (od.members.empty()) continue;
-        std::vector<ApplyMember> ap, ax;
-        for (int i : od.members) {
-            se_op* op = pl->ops[i];
-            ApplyMember A;
-            A.P = op_params(op, (double)op->sign);
-            A.in = op->p.as<double>();
-            A.out = op->q.as<double>();
-            ap.push_back(A);
-            A.in = op->x.as<double>();
-            ax.push_back(A);
+    const bool reuse = pl->desc_ok && pl->prec == prec && pl->desc_null == null && pl->desc_rtol == o.rtol;
+    if (reuse) {
+        bool changed = false;
+        for (int i = 0; i < count; i++) {
+            FbMember& M = pl->h_members[i];
+            if (M.f != d_f[i] || M.xo != d_x[i]) {
+                M.f = d_f[i];
+                M.xo = d_x[i];
+                changed = true;
+            }
         }
-        rc = od.d_ap.alloc(sizeof(ApplyMember) * ap.size());
-        if (!rc) rc = od.d_ax.alloc(sizeof(ApplyMember) * ax.size());
-        if (rc) return rc;
-        SE_HIP(hipMemcpyAsync(od.d_ap.ptr, ap.data(), sizeof(ApplyMember) * ap.size(), hipMemcpyHostToDevice, s));
-        SE_HIP(hipMemcpyAsync(od.d_ax.ptr, ax.data(), sizeof(ApplyMember) * ax.size(), hipMemcpyHostToDevice, s));
-        SE_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
+        if (changed && count > 0)
+            SE_HIP(hipMemcpyAsync(pl->d_members.ptr, pl->h_members.data(), sizeof(FbMember) * count,
+                                  hipMemcpyHostToDevice, s));
+    } else {
+        {
+            int rc = pl->d_states.alloc(sizeof(CgState) * std::max(count, 1));
+            if (rc) return rc;
+        }
+        pl->h_members.assign(count, FbMember{});
+        std::vector<GemmTask> tasks(4 * (size_t)count);
+        for (int i = 0; i < count; i++) {
+            se_op* op = pl->ops[i];
+            FdmOp* F = op->fdm;
+            FbMember& M = pl->h_members[i];
+            M.f = d_f[i];
+            M.b = op->b.as<double>();
+            M.x = op->x.as<double>();
+            M.r = op->r.as<double>();
+            M.p = op->p.as<double>();
+            M.q = op->q.as<double>();
+            M.z = F->z.as<double>();
+            M.pv = F->pv.as<double>();
+            M.pz = F->pz.as<double>();
+            M.pa = F->pa.as<double>();
+            M.pt = F->pt.as<double>();
+            M.ps = F->ps.as<double>();
+            M.xo = d_x[i];
+            M.n = op->mesh->n;
+            M.nch = (std::max(M.n, 1) + FB_CH - 1) / FB_CH;
+            M.ntz = fdm_tiles(op);
+            M.null_space = null;
+            M.sgn = (double)op->sign;
+            M.rtol = o.rtol;
+            M.st = pl->d_states.as<CgState>() + i;
+            GemmTask t4[4];
+            fdm_tasks(op, prec, M.r, M.z, M.pz, t4);
+            SE_CHECK(fdm_tasks_valid(t4, prec), SE_ERR_STATE, "FDM: transform shapes break the kernels' assumptions");
+            for (int k = 0; k < 4; k++) {
+                t4[k].pv = M.pv;
+                t4[k].st = M.st;
+                t4[k].nch = M.nch;
+                tasks[4 * (size_t)i + k] = t4[k];
+            }
+        }
+        int rc = SE_OK;
+        if (count > 0) {
+            SE_HIP(hipMemcpyAsync(pl->d_members.ptr, pl->h_members.data(), sizeof(FbMember) * count, hipMemcpyHostToDevice,
+                                  s));
+            SE_HIP(hipMemcpyAsync(pl->d_tasks.ptr, tasks.data(), sizeof(GemmTask) * 4 * count, hipMemcpyHostToDevice, s));
+        }
+        pl->prec = prec;
+        // FP64 apply descriptors
+        for (auto& od : pl->orders) {
+            if (od.members.empty()) continue;
+            std::vector<ApplyMember> ap, ax;
+            for (int i : od.members) {
+                se_op* op = pl->ops[i];
+                ApplyMember A;
+                A.P = op_params(op, (double)op->sign);
+                A.in = op->p.as<double>();
+                A.out = op->q.as<double>();
+                ap.push_back(A);
+                A.in = op->x.as<double>();
+                ax.push_back(A);
+            }
+            rc = od.d_ap.alloc(sizeof(ApplyMember) * ap.size());
+            if (!rc) rc = od.d_ax.alloc(sizeof(ApplyMember) * ax.size());
+            if (rc) return rc;
+            SE_HIP(hipMemcpyAsync(od.d_ap.ptr, ap.data(), sizeof(ApplyMember) * ap.size(), hipMemcpyHostToDevice, s));
+            SE_HIP(hipMemcpyAsync(od.d_ax.ptr, ax.data(), sizeof(ApplyMember) * ax.size(), hipMemcpyHostToDevice, s));
+            SE_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
+        }
+        for (auto& L : pl->sets) L.apply_on.clear();  // the apply descriptors were re-uploaded
+        pl->desc_ok = true;
+        pl->desc_null = null;
+        pl->desc_rtol = o.rtol;
     }
     for (int i = 0; i < count; i++) {
         CgState h{};
@@ -1289,66 +1327,97 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
         SE_HIP(hipStreamSynchronize(s));
         return SE_OK;
     };
-    auto vlaunch = [&](void (*k)(const FbMember*, const int2*, int)) {
-        const int nw = (int)pl->h_vwork.size();
-        if (nw > 0) hipLaunchKernelGGL(k, dim3(grid_of(nw, c)), dim3(FB_NT), 0, s, Ms, pl->d_vwork.as<int2>(), nw);
+    auto vlaunch = [&](int set, void (*k)(const FbMember*, const int2*, int)) {
+        const FdmPlan::ListSet& L = pl->sets[set];
+        const int nw = (int)L.h_vwork.size();
+        if (nw > 0) hipLaunchKernelGGL(k, dim3(grid_of(nw, c)), dim3(FB_NT), 0, s, Ms, L.d_vwork.as<int2>(), nw);
     };
-    auto dlaunch = [&](int mode) {
-        const int nw = (int)pl->h_vwork.size();
+    auto dlaunch = [&](int set, int mode) {
+        const FdmPlan::ListSet& L = pl->sets[set];
+        const int nw = (int)L.h_vwork.size();
         if (nw > 0)
-            hipLaunchKernelGGL(k_fb_dir, dim3(grid_of(nw, c)), dim3(FB_NT), 0, s, Ms, pl->d_vwork.as<int2>(), nw, mode);
+            hipLaunchKernelGGL(k_fb_dir, dim3(grid_of(nw, c)), dim3(FB_NT), 0, s, Ms, L.d_vwork.as<int2>(), nw, mode);
     };
+    // the true residual of the flagged members: t = b - A x into r, ||t - mean t||^2 into the state
+    auto true_residual = [&](int set, const std::vector<char>& flags) -> int {
+        int e = plan_lists(pl, set, flags, s);
+        if (!e) e = plan_apply(pl, set, true, flags, s);
+        if (e) return e;
+        vlaunch(set, k_fb_true);
+        const FdmPlan::ListSet& L = pl->sets[set];
+        hipLaunchKernelGGL(k_fb_true_fin, dim3((int)L.h_list.size()), dim3(FB_NT), 0, s, Ms, L.d_list.as<int>(),
+                           (int)L.h_list.size());
+        return SE_OK;
+    };
+    const int IT = FdmPlan::SET_ITER, CK = FdmPlan::SET_CHECK, OT = FdmPlan::SET_OTHER;
 
     SE_HIP(hipEventRecord(c->ev0, s));
     SE_HIP(hipMemcpyAsync(pl->d_states.ptr, pl->h_states, sizeof(CgState) * count, hipMemcpyHostToDevice, s));
-    pl->apply_on.clear();  // the apply descriptors were re-uploaded: rebuild the lists too
-    std::vector<char> on(count, 1);
-    rc = plan_lists(pl, on, s);
+    // start: b, r = b, x = 0, z = FDM(r), p = z.  No host round trip: a member
+    // stopped here (zero or non-finite right-hand side) is skipped by every
+    // kernel of the first iteration and classified after it.
+    std::vector<char> on(count, 1), chk(count, 0);
+    rc = plan_lists(pl, IT, on, s);
     if (rc) return rc;
-    if (null) vlaunch(k_fb_sum);
-    vlaunch(k_fb_start);
-    rc = plan_fdm(pl, s);
+    if (null) vlaunch(IT, k_fb_sum);
+    vlaunch(IT, k_fb_start);
+    rc = plan_fdm(pl, IT, s);
     if (rc) return rc;
-    dlaunch(FB_FIRST);
-    SE_HIP(hipGetLastError());
-    rc = pull();
-    if (rc) return rc;
+    dlaunch(IT, FB_FIRST);
     std::vector<int> status(count, CG_RUNNING);
-    int running = 0;
-    for (int i = 0; i < count; i++) {
-        if (pl->h_states[i].stop) {
-            status[i] = pl->h_states[i].status;
-            on[i] = 0;
-        } else {
-            running++;
-        }
-    }
     // progress of each member's recursive residual: a member whose r.r has
     // not fallen 4x within FB_STALL iterations cannot be resolved by the
     // preconditioner's precision (a near-singular system with BF16
     // transforms) and stops with SE_SOLVE_STAGNATED
     constexpr int FB_STALL = 40;
-    std::vector<double> best(count, 0.0);
+    std::vector<double> best(count, HUGE_VAL);
     std::vector<int> best_it(count, 0);
-    for (int i = 0; i < count; i++) best[i] = pl->h_states[i].rr;
     int guard = 0;
-    while (running > 0) {
-        rc = plan_lists(pl, on, s);
-        if (rc) return rc;
-        // q = A p, alpha, x and r, z = FDM(r), beta and p
-        rc = plan_apply(pl, false, on, s);
-        if (rc) return rc;
-        vlaunch(k_fb_dot);
-        vlaunch(k_fb_update);
-        rc = plan_fdm(pl, s, 1);
-        if (rc) return rc;
-        dlaunch(FB_NORMAL);
+    // Rounds: the running members' next iteration and the true-residual check
+    // of the members whose recursive residual met the target in the previous
+    // round are launched together (they are disjoint), then one pull.
+    for (;;) {
+        bool any_on = false, any_chk = false;
+        for (int i = 0; i < count; i++) {
+            any_on = any_on || on[i];
+            any_chk = any_chk || chk[i];
+        }
+        if (!any_on && !any_chk) break;
+        if (any_on) {  // q = A p, alpha, x and r, z = FDM(r), beta and p
+            rc = plan_lists(pl, IT, on, s);
+            if (!rc) rc = plan_apply(pl, IT, false, on, s);
+            if (rc) return rc;
+            vlaunch(IT, k_fb_dot);
+            vlaunch(IT, k_fb_update);
+            rc = plan_fdm(pl, IT, s, 1);
+            if (rc) return rc;
+            dlaunch(IT, FB_NORMAL);
+        }
+        if (any_chk) {
+            rc = true_residual(CK, chk);
+            if (rc) return rc;
+        }
         SE_HIP(hipGetLastError());
         rc = pull();
         if (rc) return rc;
-        // members whose recursive residual reached the target: confirm with the true residual
-        std::vector<char> chk(count, 0);
-        bool any = false;
+        // checked members: converged, out of iterations, or restarted from the true residual
+        std::vector<char> again(count, 0), next_chk(count, 0);
+        bool restart = false;
+        for (int i = 0; i < count; i++) {
+            if (!chk[i]) continue;
+            CgState& h = pl->h_states[i];
+            if (h.tt <= h.tol2) {
+                status[i] = 0;
+            } else if (h.iter >= h.maxit || ++guard > 100 * count + 1000) {
+                status[i] = 1;
+            } else {
+                again[i] = 1;
+                restart = true;
+                h.stop = 0;
+                h.status = CG_RUNNING;
+            }
+        }
+        // running members: continuing, stagnated, stopped, or due for the check
         for (int i = 0; i < count; i++) {
             if (!on[i]) continue;
             const CgState& h = pl->h_states[i];
@@ -1359,61 +1428,26 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
                 } else if (h.iter - best_it[i] >= FB_STALL) {
                     status[i] = SE_SOLVE_STAGNATED;
                     on[i] = 0;
-                    running--;
                 }
                 continue;
             }
-            if (h.status == CG_PENDING) {
-                chk[i] = 1;
-                any = true;
-            } else {
-                status[i] = h.status;
-                on[i] = 0;
-                running--;
-            }
+            on[i] = 0;
+            if (h.status == CG_PENDING) next_chk[i] = 1;
+            else status[i] = h.status;
         }
-        if (!any) continue;
-        rc = plan_lists(pl, chk, s);
-        if (rc) return rc;
-        rc = plan_apply(pl, true, chk, s);
-        if (rc) return rc;
-        vlaunch(k_fb_true);
-        hipLaunchKernelGGL(k_fb_true_fin, dim3((int)pl->h_list.size()), dim3(FB_NT), 0, s, Ms, pl->d_list.as<int>(),
-                           (int)pl->h_list.size());
-        SE_HIP(hipGetLastError());
-        rc = pull();
-        if (rc) return rc;
-        std::vector<char> again(count, 0);
-        bool restart = false;
-        for (int i = 0; i < count; i++) {
-            if (!chk[i]) continue;
-            CgState& h = pl->h_states[i];
-            if (h.tt <= h.tol2) {
-                status[i] = 0;
-                on[i] = 0;
-                running--;
-            } else if (h.iter >= h.maxit || ++guard > 100 * count + 1000) {
-                status[i] = 1;
-                on[i] = 0;
-                running--;
-            } else {
-                again[i] = 1;
-                restart = true;
-                h.stop = 0;
-                h.status = CG_RUNNING;
-            }
-        }
+        chk = next_chk;
         if (!restart) continue;
         // continue from the true residual (now in r) with a fresh direction
         SE_HIP(hipMemcpyAsync(pl->d_states.ptr, pl->h_states, sizeof(CgState) * count, hipMemcpyHostToDevice, s));
-        rc = plan_lists(pl, again, s);
+        rc = plan_lists(pl, OT, again, s);
         if (rc) return rc;
-        vlaunch(k_fb_restart);
-        rc = plan_fdm(pl, s);
+        vlaunch(OT, k_fb_restart);
+        rc = plan_fdm(pl, OT, s);
         if (rc) return rc;
-        dlaunch(FB_RESTART);
+        dlaunch(OT, FB_RESTART);
         SE_HIP(hipGetLastError());
-        SE_HIP(hipStreamSynchronize(s));
+        for (int i = 0; i < count; i++)
+            if (again[i]) on[i] = 1;
     }
     // final true residuals of the members that did not converge
     std::vector<char> fin(count, 0);
@@ -1424,13 +1458,8 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
             anyfin = true;
         }
     if (anyfin) {
-        rc = plan_lists(pl, fin, s);
-        if (!rc) rc = plan_apply(pl, true, fin, s);
-        if (rc) return rc;
-        vlaunch(k_fb_true);
-        hipLaunchKernelGGL(k_fb_true_fin, dim3((int)pl->h_list.size()), dim3(FB_NT), 0, s, Ms, pl->d_list.as<int>(),
-                           (int)pl->h_list.size());
-        rc = pull();
+        rc = true_residual(OT, fin);
+        if (!rc) rc = pull();
         if (rc) return rc;
     }
     {
@@ -1441,9 +1470,9 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
             any_out = any_out || out[i];
         }
         if (any_out) {
-            rc = plan_lists(pl, out, s);
+            rc = plan_lists(pl, anyfin ? IT : OT, out, s);  // (a set whose uploads are complete)
             if (rc) return rc;
-            vlaunch(k_fb_out);
+            vlaunch(anyfin ? IT : OT, k_fb_out);
             SE_HIP(hipGetLastError());
         }
     }
@@ -1477,9 +1506,10 @@ int fdm_plan_probe(FdmPlan* pl, int reps, double* us_pass, double* flops_pass, d
     hipStream_t s = c->stream;
     const int count = (int)pl->ops.size();
     std::vector<char> all(count, 1);
-    int rc = plan_lists(pl, all, s);
+    int rc = plan_lists(pl, FdmPlan::SET_OTHER, all, s);
     if (rc) return rc;
-    const int nw = (int)pl->h_gwork.size();
+    const FdmPlan::ListSet& L = pl->sets[FdmPlan::SET_OTHER];
+    const int nw = (int)L.h_gwork.size();
     for (int pass = 0; pass < 4; pass++) {
         double fl = 0.0;
         for (se_op* op : pl->ops) {
@@ -1487,11 +1517,11 @@ int fdm_plan_probe(FdmPlan* pl, int reps, double* us_pass, double* flops_pass, d
             fl += 2.0 * nP0 * nP1 * ((pass == 0 || pass == 3) ? nP0 : nP1);
         }
         flops_pass[pass] = fl;
-        rc = launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, pl->d_gwork.as<int2>(), nw, s);
+        rc = launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, L.d_gwork.as<int2>(), nw, s);
         if (rc) return rc;
         SE_HIP(hipEventRecord(c->ev0, s));
         for (int r = 0; r < reps && !rc; r++)
-            rc = launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, pl->d_gwork.as<int2>(), nw, s);
+            rc = launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, L.d_gwork.as<int2>(), nw, s);
         SE_HIP(hipEventRecord(c->ev1, s));
         SE_HIP(hipEventSynchronize(c->ev1));
         if (rc) return rc;
@@ -1502,10 +1532,10 @@ int fdm_plan_probe(FdmPlan* pl, int reps, double* us_pass, double* flops_pass, d
     double by = 0.0;
     for (se_op* op : pl->ops) by += 16.0 * op->mesh->n;
     *bytes_apply = by;
-    rc = plan_apply(pl, false, all, s);
+    rc = plan_apply(pl, FdmPlan::SET_OTHER, false, all, s);
     if (rc) return rc;
     SE_HIP(hipEventRecord(c->ev0, s));
-    for (int r = 0; r < reps && !rc; r++) rc = plan_apply(pl, false, all, s);
+    for (int r = 0; r < reps && !rc; r++) rc = plan_apply(pl, FdmPlan::SET_OTHER, false, all, s);
     SE_HIP(hipEventRecord(c->ev1, s));
     SE_HIP(hipEventSynchronize(c->ev1));
     if (rc) return rc;
