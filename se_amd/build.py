@@ -22,6 +22,9 @@ HIPFLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions stay off (only ordinary scalar / float2 code)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # matrix-core accumulators in VGPRs: with the AGPR form the compiler copies
+    # every GEMM accumulator VGPR -> AGPR -> VGPR around each K chunk
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
     "-Wno-unused-command-line-argument",
 ]

@@ -12,10 +12,10 @@
 // (Laplacian + mass, src/base/Operator.cpp:63-84,157-177, with the exact
 // per-point Jacobian of Mesh::ShapeFuncDerivs, src/base/Mesh.cpp:217-250) is
 //   Y = Dx^T (g00 o Dx U + g01 o Dy U) + Dy^T (g01 o Dx U + g11 o Dy U) + M o U
-// with the metric factored per element column / row (op_metric_tables):
-//   g00 = c X0[qx] Y0[qy],  g11 = c X1[qx] Y1[qy],
-//   g01 = -c w[qx] w[qy] (X2[qx] QY[qy][qx] + QX[qx][qy] Y2[qy]),
-//   M   = c_mass MX[qx] MY[qy].
+// with the metric factored per element column / row and the operator's
+// coefficients folded in (op_metric_tables):
+//   g00 = A0[qx] Y0[qy],  g11 = A1[qx] Y1[qy],  M = AM[qx] MY[qy],
+//   g01 = A2[qx] QYw[qy][qx] + QXw[qx][qy] Y2w[qy].
 // Thread t of an element holds row t and column t of U in registers:
 //   1. row t: u_xi(q, t) = sum_k D[q][k] U[t][k];  column t: u_eta(t, q)
 //      likewise.  D is read through constant-address-space pointers with
@@ -93,14 +93,14 @@ __device__ __forceinline__ void apply_lines_tile(const OpParams& P, int tile,
     constexpr int BW = BWU | 1;                 // odd row pitch (doubles)
     constexpr int NE = (TX + 1) * (TY + 1);
     static_assert(NE * N1 <= 256, "one pass of line threads");
-    constexpr int CS = 4 * N1 + NP;             // stride of the FP64 metric tables
-    constexpr int F = 4 * N1;                   // line factors X0 X1 X2 M (Y0 Y1 Y2 M)
+    constexpr int F = 4 * N1;                   // line factors A0 A1 A2 AM (Y0 Y1 Y2w MY)
     constexpr int QS = 2 * NP;                  // FP32 cross tables: Q, Q transposed
 
     __shared__ double s_u[BH * BW];
     __shared__ double s_o[NE * NP];              // FP32 exchange, then the element outputs
     __shared__ double s_cf[(TX + 1) * F], s_rf[(TY + 1) * F];
     __shared__ float s_cq[(TX + 1) * QS], s_rq[(TY + 1) * QS];
+    __shared__ int s_I[BWU], s_J[BH];            // unknown column / row of each block column / row (-1: none)
     float* s_xf = (float*)s_o;                   // u_xi  [e][qx][qy]
     float* s_ef = s_xf + NE * NP;                // u_eta [e][qy][qx]
 
@@ -114,32 +114,22 @@ __device__ __forceinline__ void apply_lines_tile(const OpParams& P, int tile,
     SE_GLOBAL double* gout = gview(out);
     cdouble* Dc = (cdouble*)P.D;
     cdouble* DTc = (cdouble*)P.DT;
-    cdouble* Wc = (cdouble*)P.w;
 
-    // ---- 1. the tile's node block and its tables; one barrier -----------------
+    // ---- 1. the tile's node block and its tables; two barriers ----------------
+    for (int i = tid; i < BWU + BH; i += 256) {
+        if (i < BWU) s_I[i] = unk_line((ex0 - 1) * N + i, M0, P.per0, P.off0, P.nU0);
+        else s_J[i - BWU] = unk_line((ey0 - 1) * N + i - BWU, M1, P.per1, P.off1, P.nU1);
+    }
     {
-        constexpr int NL = (BH * BWU + 255) / 256;
-        double v[NL];
-#pragma unroll
-        for (int g = 0; g < NL; g++) {
-            const int i = tid + 256 * g;
-            v[g] = 0.0;
-            if (i < BH * BWU) {
-                const int a = i / BWU, b = i - a * BWU;
-                const int I = unk_line((ex0 - 1) * N + b, M0, P.per0, P.off0, P.nU0);
-                const int J = unk_line((ey0 - 1) * N + a, M1, P.per1, P.off1, P.nU1);
-                if (I >= 0 && J >= 0) v[g] = gin[(size_t)J * P.nU0 + I];
-            }
-        }
         for (int i = tid; i < (TX + 1) * F; i += 256) {
             const int c = i / F, ex = ex0 - 1 + c;
             const int e = ex < 0 ? ex + P.nEls0 : ex;
-            s_cf[i] = (e >= 0 && e < P.nEls0) ? gview(P.colTab)[(size_t)e * CS + i % F] : 0.0;
+            s_cf[i] = (e >= 0 && e < P.nEls0) ? gview(P.colTab)[(size_t)e * F + i % F] : 0.0;
         }
         for (int i = tid; i < (TY + 1) * F; i += 256) {
             const int r = i / F, ey = ey0 - 1 + r;
             const int e = ey < 0 ? ey + P.nEls1 : ey;
-            s_rf[i] = (e >= 0 && e < P.nEls1) ? gview(P.rowTab)[(size_t)e * CS + i % F] : 0.0;
+            s_rf[i] = (e >= 0 && e < P.nEls1) ? gview(P.rowTab)[(size_t)e * F + i % F] : 0.0;
         }
         for (int i = tid; i < (TX + 1) * QS; i += 256) {
             const int c = i / QS, ex = ex0 - 1 + c;
@@ -150,6 +140,21 @@ __device__ __forceinline__ void apply_lines_tile(const OpParams& P, int tile,
             const int r = i / QS, ey = ey0 - 1 + r;
             const int e = ey < 0 ? ey + P.nEls1 : ey;
             s_rq[i] = (e >= 0 && e < P.nEls1) ? gview(P.rowQ)[(size_t)e * QS + i % QS] : 0.0f;
+        }
+    }
+    __syncthreads();  // the index tables
+    {
+        constexpr int NL = (BH * BWU + 255) / 256;
+        double v[NL];
+#pragma unroll
+        for (int g = 0; g < NL; g++) {
+            const int i = tid + 256 * g;
+            v[g] = 0.0;
+            if (i < BH * BWU) {
+                const int a = i / BWU, b = i - a * BWU;
+                const int I = s_I[b], J = s_J[a];
+                if (I >= 0 && J >= 0) v[g] = gin[(size_t)J * P.nU0 + I];
+            }
         }
 #pragma unroll
         for (int g = 0; g < NL; g++) {
@@ -166,7 +171,6 @@ __device__ __forceinline__ void apply_lines_tile(const OpParams& P, int tile,
     const int e = tid / N1, t = tid - (tid / N1) * N1;
     const int c = e % (TX + 1), r = e / (TX + 1);
     const bool live = e < NE && (c > 0 || halo_c) && (r > 0 || halo_r) && c <= txo && r <= tyo;
-    const double cl = P.c_lap;
     double uxi[N1], uet[N1], ur[N1];
     if (live) {
         double uc[N1];
@@ -200,27 +204,24 @@ __device__ __forceinline__ void apply_lines_tile(const OpParams& P, int tile,
         const double* rf = &s_rf[r * F];
         const float* cq = &s_cq[c * QS];
         const float* rq = &s_rq[r * QS];
-        const double wt = gview(P.w)[t];
         double fx[N1], fy[N1];
-        // row t (qy = t, qx = q)
-        const double y0 = rf[t], y2 = rf[2 * N1 + t];
+        // row t (qy = t, qx = q): g01 = A2[q] QYw[t][q] + QXw[q][t] Y2w[t]
+        const double y0 = rf[t], y2w = rf[2 * N1 + t];
 #pragma unroll
         for (int q = 0; q < N1; q++) {
             const double ue = (double)s_ef[e * NP + t * N1 + q];  // u_eta at (q, t)
-            const double g01 = -cl * (Wc[q] * wt) *
-                               (cf[2 * N1 + q] * (double)rq[t * N1 + q] + (double)cq[NP + t * N1 + q] * y2);
-            fx[q] = cl * cf[q] * y0 * uxi[q] + g01 * ue;
+            const double g01 = cf[2 * N1 + q] * (double)rq[t * N1 + q] + (double)cq[NP + t * N1 + q] * y2w;
+            fx[q] = cf[q] * y0 * uxi[q] + g01 * ue;
         }
-        // column t (qx = t, qy = q)
-        const double x1 = cf[N1 + t], x2 = cf[2 * N1 + t];
+        // column t (qx = t, qy = q): g01 = A2[t] QYw[q][t] + QXw[t][q] Y2w[q]
+        const double a1 = cf[N1 + t], a2 = cf[2 * N1 + t];
 #pragma unroll
         for (int q = 0; q < N1; q++) {
             const double ux = (double)s_xf[e * NP + t * N1 + q];  // u_xi at (t, q)
-            const double g01 = -cl * (wt * Wc[q]) *
-                               (x2 * (double)rq[NP + t * N1 + q] + (double)cq[t * N1 + q] * rf[2 * N1 + q]);
-            fy[q] = g01 * ux + cl * x1 * rf[N1 + q] * uet[q];
+            const double g01 = a2 * (double)rq[NP + t * N1 + q] + (double)cq[t * N1 + q] * rf[2 * N1 + q];
+            fy[q] = g01 * ux + a1 * rf[N1 + q] * uet[q];
         }
-        const double my = P.c_mass * rf[3 * N1 + t];
+        const double my = rf[3 * N1 + t];
 #pragma unroll
         for (int l = 0; l < N1; l++) {
             double sx = 0.0, sy = 0.0;
@@ -252,8 +253,7 @@ __device__ __forceinline__ void apply_lines_tile(const OpParams& P, int tile,
     double dot = 0.0;
     for (int i = tid; i < NOX * NOY; i += 256) {
         const int oy = i / NOX, ox = i - oy * NOX;
-        const int I = unk_line(ex0 * N + ox, M0, P.per0, P.off0, P.nU0);
-        const int J = unk_line(ey0 * N + oy, M1, P.per1, P.off1, P.nU1);
+        const int I = s_I[N + ox], J = s_J[N + oy];
         if (I < 0 || J < 0) continue;
         const int cq = ox / N, lx = ox - cq * N, rq = oy / N, ly = oy - rq * N;
         // element columns / rows holding the node: (c, local index) pairs

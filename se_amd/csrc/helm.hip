@@ -365,7 +365,7 @@ OpParams op_params(const se_op* op, double scale) {
     P.D = basis + 2 * n1;
     P.DT = basis + 4 * n1 + n1 * n1;
     P.colTab = op->tabs.as<double>();
-    P.rowTab = op->tabs.as<double>() + (size_t)m->nEls[0] * (4 * n1 + n1 * n1);
+    P.rowTab = op->tabs.as<double>() + (size_t)m->nEls[0] * 4 * n1;
     P.colQ = op->qtabs.as<float>();
     P.rowQ = op->qtabs.as<float>() + (size_t)m->nEls[0] * 2 * n1 * n1;
     P.nU0 = m->nU[0];
@@ -424,25 +424,44 @@ static int op_metric_tables(se_op* op) {
             }
         }
     }
-    int rc = op->tabs.alloc(sizeof(double) * t.size());
-    if (rc) return rc;
-    SE_HIP(hipMemcpy(op->tabs.ptr, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
-    // the line apply's FP32 copies of the round-off-sized cross-term tables,
-    // each followed by its transpose: QX [qx][qy] | [qy][qx] per column,
-    // QY [qy][qx] | [qx][qy] per row
-    const int np = n1 * n1;
-    std::vector<float> qt((size_t)(m->nEls[0] + m->nEls[1]) * 2 * np);
-    for (int e = 0; e < m->nEls[0] + m->nEls[1]; e++) {
-        const double* Q = &t[(size_t)e * CS + 4 * n1];
+    // The line apply's tables, with the operator's coefficients folded in
+    // (c = c_lap):
+    //   per column  A0 = c X0, A1 = c X1, A2 = -c w X2, AM = c_mass M      (FP64)
+    //   per row     Y0,        Y1,       Y2w = w Y2,    MY = M             (FP64)
+    //   per column  QXw[qx][qy] = -c w[qx] QX[qx][qy], then its transpose   (FP32)
+    //   per row     QYw[qy][qx] =    w[qy] QY[qy][qx], then its transpose   (FP32)
+    // so that g00 = A0[qx] Y0[qy], g11 = A1[qx] Y1[qy] and
+    //   g01 = A2[qx] QYw[qy][qx] + QXw[qx][qy] Y2w[qy],  mass = AM[qx] MY[qy].
+    // The cross-term tables are round-off-sized, so FP32 holds them to ~1e-17
+    // of the operator.
+    const int np = n1 * n1, F = 4 * n1;
+    const int ncol = m->nEls[0], nrow = m->nEls[1];
+    std::vector<double> lf((size_t)(ncol + nrow) * F);
+    std::vector<float> qt((size_t)(ncol + nrow) * 2 * np);
+    const double c = op->c_lap;
+    for (int e = 0; e < ncol + nrow; e++) {
+        const double* T = &t[(size_t)e * CS];
+        const double* Q = T + 4 * n1;
+        double* L = &lf[(size_t)e * F];
         float* o = &qt[(size_t)e * 2 * np];
-        for (int a = 0; a < n1; a++)
+        const bool col = e < ncol;
+        for (int q = 0; q < n1; q++) {
+            L[q] = col ? c * T[q] : T[q];
+            L[n1 + q] = col ? c * T[n1 + q] : T[n1 + q];
+            L[2 * n1 + q] = col ? -c * w[q] * T[2 * n1 + q] : w[q] * T[2 * n1 + q];
+            L[3 * n1 + q] = col ? op->c_mass * T[3 * n1 + q] : T[3 * n1 + q];
+        }
+        for (int a = 0; a < n1; a++)  // Q[a][b]: column table [qx][qy], row table [qy][qx]; w of the first index
             for (int b = 0; b < n1; b++) {
-                o[a * n1 + b] = (float)Q[a * n1 + b];
-                o[np + b * n1 + a] = (float)Q[a * n1 + b];
+                const double v = col ? -c * w[a] * Q[a * n1 + b] : w[a] * Q[a * n1 + b];
+                o[a * n1 + b] = (float)v;
+                o[np + b * n1 + a] = (float)v;
             }
     }
-    rc = op->qtabs.alloc(sizeof(float) * qt.size());
+    int rc = op->tabs.alloc(sizeof(double) * lf.size());
+    if (!rc) rc = op->qtabs.alloc(sizeof(float) * qt.size());
     if (rc) return rc;
+    SE_HIP(hipMemcpy(op->tabs.ptr, lf.data(), sizeof(double) * lf.size(), hipMemcpyHostToDevice));
     SE_HIP(hipMemcpy(op->qtabs.ptr, qt.data(), sizeof(float) * qt.size(), hipMemcpyHostToDevice));
     op->tabs_host = std::move(t);
     return SE_OK;

@@ -41,12 +41,12 @@ struct OpParams {
     const double* w;    // [N+1]
     const double* D;    // [(N+1)^2]
     const double* DT;   // [(N+1)^2], D transposed
-    // metric factor tables of the hot apply, per element column / row
-    // (stride 4(N+1) + (N+1)^2, layout in op_metric_tables)
+    // the line apply's metric tables per element column / row, with the
+    // operator's coefficients folded in (op_metric_tables): FP64 line factors
+    // (stride 4(N+1)) and FP32 cross-term tables, each followed by its
+    // transpose (stride 2 (N+1)^2)
     const double* colTab;
     const double* rowTab;
-    // the same cross-term tables in FP32, each followed by its transpose
-    // (stride 2 (N+1)^2): QX [qx][qy] | [qy][qx], QY [qy][qx] | [qx][qy]
     const float* colQ;
     const float* rowQ;
     // the unknown grid: nU0 x nU1, first unknown field column / row
