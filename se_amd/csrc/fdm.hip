@@ -4,10 +4,10 @@
 // The operator in its separable form (mixed.hip, assemble_line) has the
 // near-exact inverse (Vy (x) Vx) D^-1 (Vy (x) Vx)^T (fdm_host.h).  On the
 // unknown grid R (rows J, columns I) applying it is four dense transforms:
-//   pass 1  T1t = (R Vx)^T        x-transform, stored transposed
-//   pass 2  T2  = (T1t Vy) ./ D   y-transform, scaled by 1/D
-//   pass 3  T3t = (T2 Vy^T)^T     y back-transform, stored transposed
-//   pass 4  Z   =  T3t Vx^T       x back-transform
+//   pass 1  T1t = Vx^T R^T  (= (R Vx)^T)      x-transform
+//   pass 2  T2  = (T1t Vy) ./ D               y-transform, scaled by 1/D
+//   pass 3  T3t = Vy T2^T   (= (T2 Vy^T)^T)   y back-transform
+//   pass 4  Z   = T3t Vx^T                    x back-transform
 // each an "NT" GEMM C = A Bt^T whose operands both have the summed index
 // contiguous, on 64 x 64 output tiles (four waves of 32 x 32) with MFMA:
 // v_mfma_f64_16x16x4f64 (FP64), v_mfma_f32_32x32x2f32 (FP32) or
@@ -33,6 +33,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "apply_kernels.h"
@@ -52,11 +53,11 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 // share the LDS counter, so every wait for an LDS operand read would also wait
 // for the next chunk's prefetch (SE_GLOBAL, apply_kernels.h).
 
-enum { FDM_OUT_T = 0, FDM_OUT_SCALE = 1, FDM_OUT_FINAL = 2 };
+enum { FDM_OUT_PLAIN = 0, FDM_OUT_SCALE = 1, FDM_OUT_FINAL = 2 };
 enum { FDM_F64 = 0, FDM_F32 = 1, FDM_BF16 = 2 };  // element types of the transforms
 
 // One GEMM of one member: C = A Bt^T.  A: M x K (lda), Bt: N x K (ldb), both
-// row-major, element type T.  Pass 1's A is the residual packed to T and
+// row-major, element type T.  Pass 1's Bt is the residual packed to T and
 // padded by k_fdm_pack from r (true extent arows x acols, leading dimension
 // acols).
 struct GemmTask {
@@ -78,87 +79,108 @@ struct GemmTask {
     int nch;
 };
 
-// K chunk per LDS stage (KC) and padded LDS row length (KP, a multiple of 16
-// bytes).  Every padded dimension is a multiple of 64, so KC must divide 64.
-template <typename T> struct GemmCfg;
-template <> struct GemmCfg<double> {
-    static constexpr int KC = 32, KP = 34;
-    using S = double;  // staging type of the transposed store
-};
-template <> struct GemmCfg<float> {
-    static constexpr int KC = 64, KP = 68;
-    using S = float;
-};
-template <> struct GemmCfg<__bf16> {
-    static constexpr int KC = 64, KP = 72;
-    using S = float;
-};
-static_assert(64 % GemmCfg<double>::KC == 0 && 64 % GemmCfg<float>::KC == 0 && 64 % GemmCfg<__bf16>::KC == 0,
-              "a K chunk must divide the padding granule");
+// Output tiles are 64 x 64, four waves of 32 x 32 each.
+constexpr int FDM_BT = 64;
+
+// K-chunk staging.  A chunk of one operand tile is 64 rows x 128 bytes in
+// LDS, copied straight from global memory by global_load_lds_dwordx4 (no
+// register staging).  Two stages alternate: the copy of the next chunk runs
+// during the current chunk's MFMAs, and one barrier per chunk separates them.
+// One such instruction fills 64 consecutive 16-byte slots (lane i: slot i);
+// the lanes choose which global vectors land there, which places vector c of
+// row r at slot r VR + (c ^ swz(r)).  An MFMA operand read takes one vector
+// from each of 16 consecutive rows; the swizzle puts them in 16 different
+// 16-byte bank groups (plain 128-byte rows would share banks).  Every padded
+// dimension is a multiple of 64, so KC divides 64.  (Rows of 256 bytes, 64 KB
+// per workgroup, measured slower: two workgroups per CU instead of four.)
 template <typename T>
-constexpr int gemm_lds_bytes() {
-    return (int)std::max(2 * 64 * GemmCfg<T>::KP * sizeof(T), 64 * 65 * sizeof(typename GemmCfg<T>::S));
+struct GemmCfg {
+    static constexpr int RB = 128;                         // bytes per row and chunk
+    static constexpr int VR = RB / 16;                     // 16-byte vectors per row
+    static constexpr int KC = RB / (int)sizeof(T);         // K per chunk
+    static constexpr int VEC = 16 / (int)sizeof(T);        // elements per vector
+    static constexpr int OPB = FDM_BT * RB;                // bytes of one operand's chunk
+    static constexpr int NI = FDM_BT * VR / 256;           // copies per thread and operand
+    static constexpr int STAGE = 2 * OPB;                  // both operands
+    static_assert(64 % KC == 0 && NI * 256 == FDM_BT * VR, "chunk shape");
+};
+template <typename T>
+__device__ __forceinline__ int swz(int r) {
+    static_assert(GemmCfg<T>::VR == 8, "two rows per 256 bytes of banks");
+    return (r >> 1) & 7;
 }
 
-template <typename T> struct Acc;
-template <> struct Acc<float> {
-    f32x16 v;
+// MFMA block edge per element type and the accumulators of one wave
+template <typename T> struct MfmaBlock { static constexpr int E = 32; };   // 32 x 32 (FP32 / BF16 operands)
+template <> struct MfmaBlock<double> { static constexpr int E = 16; };     // 16 x 16 (FP64)
+template <typename T>
+struct Acc {
+    static constexpr int NB = (FDM_BT / 2) / MfmaBlock<T>::E;  // blocks per wave and dimension
+    typename std::conditional<std::is_same<T, double>::value, f64x4, f32x16>::type c[NB][NB];
 };
-template <> struct Acc<__bf16> {
-    f32x16 v;
-};
-template <> struct Acc<double> {
-    f64x4 c[2][2];
-};
+// one 16-byte operand vector
+template <typename T> struct Vec16;
+template <> struct Vec16<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct Vec16<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct Vec16<__bf16> { typedef bf16x8 type; };
 
 template <typename T> __device__ __forceinline__ T to_t(double v);
 template <> __device__ __forceinline__ double to_t<double>(double v) { return v; }
 template <> __device__ __forceinline__ float to_t<float>(double v) { return (float)v; }
 template <> __device__ __forceinline__ __bf16 to_t<__bf16>(double v) { return (__bf16)(float)v; }
-template <typename T> __device__ __forceinline__ double from_t(T v) { return (double)v; }
-template <> __device__ __forceinline__ double from_t<__bf16>(__bf16 v) { return (double)(float)v; }
 
-// one K chunk of the wave's 32 x 32 block.  Lanes of half (l >> 5) (FP64:
-// quarter (l >> 4)) supply a contiguous run of the chunk's K values, so each
-// lane reads its operands from LDS with wide loads; both operands use the
-// same K assignment, so the sum covers the chunk once.
+// the MFMAs of one operand vector per block pair
+__device__ __forceinline__ void mfma_vec(const Vec16<double>::type& a, const Vec16<double>::type& b, f64x4& c) {
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[0], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], b[1], c, 0, 0, 0);
+}
+__device__ __forceinline__ void mfma_vec(const Vec16<float>::type& a, const Vec16<float>::type& b, f32x16& c) {
+#pragma unroll
+    for (int e = 0; e < 4; e++) c = __builtin_amdgcn_mfma_f32_32x32x2f32(a[e], b[e], c, 0, 0, 0);
+}
+__device__ __forceinline__ void mfma_vec(const bf16x8& a, const bf16x8& b, f32x16& c) {
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+// one K chunk of the wave's 32 x 32 block.  The lanes split into G = 64 / E
+// groups along K (FP64: quarters l >> 4; FP32 / BF16: halves l >> 5); group g
+// takes the same VR / G of the chunk's vectors from both operands, so the sum
+// covers the chunk once: FP32 / BF16 the run [g VR / G, (g + 1) VR / G), FP64
+// the vectors g + G v (its quarters read one row each and, on runs, meet in
+// the same banks: measured 4 conflict cycles per LDS cycle, none when
+// interleaved).  The wave's rows wr 32 + i E + (l mod E) differ by multiples
+// of 16, so they share one swizzle.  The operand reads
+// run RA vectors ahead of the MFMAs that use them (the scheduler would
+// otherwise issue each read just before its MFMAs and wait for it there).
 template <typename T>
-__device__ __forceinline__ void mma_chunk(const T* As, const T* Bs, int wr, int wc, int l, Acc<T>& acc);
-
-template <>
-__device__ __forceinline__ void mma_chunk<float>(const float* As, const float* Bs, int wr, int wc, int l,
-                                                 Acc<float>& acc) {
-    constexpr int KC = GemmCfg<float>::KC, KP = GemmCfg<float>::KP;
-    const float* a = As + (wr * 32 + (l & 31)) * KP + (l >> 5) * (KC / 2);
-    const float* b = Bs + (wc * 32 + (l & 31)) * KP + (l >> 5) * (KC / 2);
+__device__ __forceinline__ void mma_chunk(const char* As, const char* Bs, int wr, int wc, int l, Acc<T>& acc) {
+    using C = GemmCfg<T>;
+    using V = typename Vec16<T>::type;
+    constexpr int E = MfmaBlock<T>::E, G = 64 / E, NB = Acc<T>::NB, VPG = C::VR / G, RA = 2;
+    const int lr = l % E, g = l / E, sw = swz<T>(lr);
+    const char* a = As + (wr * 32 + lr) * C::RB;
+    const char* b = Bs + (wc * 32 + lr) * C::RB;
+    V av[VPG][NB], bv[VPG][NB];
+    auto read = [&](int v) {
+        const int c = std::is_same<T, double>::value ? g + G * v : g * VPG + v;
+        const int cs = (c ^ sw) * 16;
 #pragma unroll
-    for (int m = 0; m < KC / 2; m++) acc.v = __builtin_amdgcn_mfma_f32_32x32x2f32(a[m], b[m], acc.v, 0, 0, 0);
-}
-
-template <>
-__device__ __forceinline__ void mma_chunk<__bf16>(const __bf16* As, const __bf16* Bs, int wr, int wc, int l,
-                                                  Acc<__bf16>& acc) {
-    constexpr int KC = GemmCfg<__bf16>::KC, KP = GemmCfg<__bf16>::KP;
-    const bf16x8* a = (const bf16x8*)(As + (wr * 32 + (l & 31)) * KP + (l >> 5) * (KC / 2));
-    const bf16x8* b = (const bf16x8*)(Bs + (wc * 32 + (l & 31)) * KP + (l >> 5) * (KC / 2));
+        for (int i = 0; i < NB; i++) {
+            av[v][i] = *(const V*)(a + i * E * C::RB + cs);
+            bv[v][i] = *(const V*)(b + i * E * C::RB + cs);
+        }
+    };
 #pragma unroll
-    for (int m = 0; m < KC / 16; m++) acc.v = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[m], b[m], acc.v, 0, 0, 0);
-}
-
-template <>
-__device__ __forceinline__ void mma_chunk<double>(const double* As, const double* Bs, int wr, int wc, int l,
-                                                  Acc<double>& acc) {
-    constexpr int KC = GemmCfg<double>::KC, KP = GemmCfg<double>::KP;
-    const double* a0 = As + (wr * 32 + (l & 15)) * KP + (l >> 4) * (KC / 4);
-    const double* a1 = a0 + 16 * KP;
-    const double* b0 = Bs + (wc * 32 + (l & 15)) * KP + (l >> 4) * (KC / 4);
-    const double* b1 = b0 + 16 * KP;
+    for (int v = 0; v < RA; v++) read(v);
 #pragma unroll
-    for (int m = 0; m < KC / 4; m++) {
-        acc.c[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[m], b0[m], acc.c[0][0], 0, 0, 0);
-        acc.c[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[m], b1[m], acc.c[0][1], 0, 0, 0);
-        acc.c[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[m], b0[m], acc.c[1][0], 0, 0, 0);
-        acc.c[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[m], b1[m], acc.c[1][1], 0, 0, 0);
+    for (int v = 0; v < VPG; v++) {
+        if (v + RA < VPG) read(v + RA);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < NB; i++)
+#pragma unroll
+            for (int j = 0; j < NB; j++) mfma_vec(av[v][i], bv[v][j], acc.c[i][j]);
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -168,94 +190,100 @@ __device__ __forceinline__ void mma_chunk<double>(const double* As, const double
 // a one-wave product of random operands)
 template <typename T, typename F>
 __device__ __forceinline__ void for_each_acc(const Acc<T>& acc, int wr, int wc, int l, F&& f) {
-    if constexpr (std::is_same<T, double>::value) {
+    constexpr int NB = Acc<T>::NB;
 #pragma unroll
-        for (int bi = 0; bi < 2; bi++)
+    for (int bi = 0; bi < NB; bi++)
 #pragma unroll
-            for (int bj = 0; bj < 2; bj++)
+        for (int bj = 0; bj < NB; bj++) {
+            if constexpr (std::is_same<T, double>::value) {
 #pragma unroll
                 for (int v = 0; v < 4; v++)
                     f(wr * 32 + 16 * bi + (l >> 4) + 4 * v, wc * 32 + 16 * bj + (l & 15), acc.c[bi][bj][v]);
-    } else {
+            } else {
 #pragma unroll
-        for (int v = 0; v < 16; v++)
-            f(wr * 32 + (v & 3) + 4 * (l >> 5) + 8 * (v >> 2), wc * 32 + (l & 31), (double)acc.v[v]);
-    }
+                for (int v = 0; v < 16; v++)
+                    f(wr * 32 + 32 * bi + (v & 3) + 4 * (l >> 5) + 8 * (v >> 2), wc * 32 + 32 * bj + (l & 31),
+                      (double)acc.c[bi][bj][v]);
+            }
+        }
 }
 
-// One operand chunk (64 rows x KC) in registers: 16-byte vectors, VPR per
-// row segment, NV per thread.  Loaded one chunk ahead of the MFMA work that
-// uses the LDS copy of the previous one.  (Clang vector registers: HIP's
-// uint4 wrapper type kept the array in scratch memory.)
+// The copies of one chunk of one operand tile: thread-invariant element
+// offsets (row ld + column) of the NI vectors this lane copies, and the
+// wave-uniform LDS slot runs they fill.
 template <typename T>
-struct OperandRegs {
-    static constexpr int KC = GemmCfg<T>::KC, KP = GemmCfg<T>::KP;
-    static constexpr int VEC = 16 / (int)sizeof(T), VPR = KC / VEC, NV = 64 * VPR / 256;
-    static_assert(KC % VEC == 0 && (64 * VPR) % 256 == 0 && 256 % VPR == 0, "chunk / thread shape");
-    u32x4 v[NV];
-    __device__ __forceinline__ void load(const T* src, int ld, int r0, int k0) {
-        const int tid = threadIdx.x;
+struct ChunkCopy {
+    using C = GemmCfg<T>;
+    int off[C::NI];
+    __device__ __forceinline__ void init(int ld, int wv, int l) {
 #pragma unroll
-        for (int s = 0; s < NV; s++) {
-            const int row = tid / VPR + (256 / VPR) * s, cv = tid % VPR;
-            v[s] = *(const SE_GLOBAL u32x4*)(src + (size_t)(r0 + row) * ld + k0 + cv * VEC);
+        for (int j = 0; j < C::NI; j++) {
+            const int s = (wv * C::NI + j) * 64 + l, r = s / C::VR;
+            off[j] = r * ld + ((s % C::VR) ^ swz<T>(r)) * C::VEC;
         }
     }
-    __device__ __forceinline__ void store(T* S) const {
-        const int tid = threadIdx.x;
+    __device__ __forceinline__ void issue(const SE_GLOBAL T* src, int k0, char* S, int wv) const {
 #pragma unroll
-        for (int s = 0; s < NV; s++) {
-            const int row = tid / VPR + (256 / VPR) * s, cv = tid % VPR;
-            *(u32x4*)(S + row * KP + cv * VEC) = v[s];
-        }
+        for (int j = 0; j < C::NI; j++)
+            __builtin_amdgcn_global_load_lds((const SE_GLOBAL void*)(src + off[j] + k0),
+                                             (__attribute__((address_space(3))) void*)(S + (wv * C::NI + j) * 1024), 16,
+                                             0, 0);
     }
 };
+
+// the copies this thread issued have landed in LDS
+__device__ __forceinline__ void wait_copies() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 template <typename T>
-__device__ __forceinline__ void gemm_tile(const GemmTask& t, int tile, char* lds) {
-    constexpr int KC = GemmCfg<T>::KC, KP = GemmCfg<T>::KP;
-    using S = typename GemmCfg<T>::S;
-    T* As = (T*)lds;
-    T* Bs = As + 64 * KP;
+__device__ __forceinline__ void gemm_tile(const GemmTask& t, int tile, char* S0, char* S1) {
+    using C = GemmCfg<T>;
+    constexpr int NB = Acc<T>::NB;
     const int tid = threadIdx.x, l = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wr = wv >> 1, wc = wv & 1;
     const int tm = tile / t.tilesN, tn = tile - tm * t.tilesN;
-    const int m0 = tm * 64, n0 = tn * 64;
+    const int m0 = tm * FDM_BT, n0 = tn * FDM_BT;
     Acc<T> acc;
-    if constexpr (std::is_same<T, double>::value) {
-        for (int bi = 0; bi < 2; bi++)
-            for (int bj = 0; bj < 2; bj++) acc.c[bi][bj] = f64x4{0.0, 0.0, 0.0, 0.0};
-    } else {
-        for (int v = 0; v < 16; v++) acc.v[v] = 0.0f;
-    }
-    const T* __restrict__ A = (const T*)t.A;
-    const T* __restrict__ Bt = (const T*)t.Bt;
-    OperandRegs<T> ra, rb;
-    ra.load(A, t.lda, m0, 0);
-    rb.load(Bt, t.ldb, n0, 0);
-    for (int k0 = 0; k0 < t.K; k0 += KC) {
-        __syncthreads();  // the previous chunk (or item) is done with the tiles
-        ra.store(As);
-        rb.store(Bs);
-        __syncthreads();
-        if (k0 + KC < t.K) {  // next chunk's loads in flight during this chunk's MFMAs
-            ra.load(A, t.lda, m0, k0 + KC);
-            rb.load(Bt, t.ldb, n0, k0 + KC);
+#pragma unroll
+    for (int i = 0; i < NB; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            if constexpr (std::is_same<T, double>::value) acc.c[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+            else
+                for (int v = 0; v < 16; v++) acc.c[i][j][v] = 0.0f;
         }
-        mma_chunk<T>(As, Bs, wr, wc, l, acc);
-    }
-    if (t.mode == FDM_OUT_T) {
-        S* Cs = (S*)lds;
-        __syncthreads();  // done with the operand tiles
-        for_each_acc<T>(acc, wr, wc, l, [&](int row, int col, double v) { Cs[col * 65 + row] = (S)v; });
-        __syncthreads();
-        SE_GLOBAL T* C = (SE_GLOBAL T*)t.C;
-        for (int e = tid; e < 64 * 64; e += 256) {
-            const int cr = e >> 6, cc = e & 63;
-            C[(size_t)(n0 + cr) * t.ldc + m0 + cc] = to_t<T>((double)Cs[cr * 65 + cc]);
+    const SE_GLOBAL T* A = (const SE_GLOBAL T*)t.A + (size_t)m0 * t.lda;
+    const SE_GLOBAL T* B = (const SE_GLOBAL T*)t.Bt + (size_t)n0 * t.ldb;
+    ChunkCopy<T> ca, cb;
+    ca.init(t.lda, wv, l);
+    cb.init(t.ldb, wv, l);
+    const int nk = t.K / C::KC;
+    __syncthreads();  // the previous item is done reading both stages
+    ca.issue(A, 0, S0, wv);
+    cb.issue(B, 0, S0 + C::OPB, wv);
+    for (int kc = 0; kc < nk; kc += 2) {
+        wait_copies();
+        __syncthreads();  // chunk kc is in S0 for every wave; S1's last reads are done
+        if (kc + 1 < nk) {
+            ca.issue(A, (kc + 1) * C::KC, S1, wv);
+            cb.issue(B, (kc + 1) * C::KC, S1 + C::OPB, wv);
         }
+        mma_chunk<T>(S0, S0 + C::OPB, wr, wc, l, acc);
+        if (kc + 1 >= nk) break;
+        wait_copies();
+        __syncthreads();  // chunk kc + 1 is in S1; S0's reads are done
+        if (kc + 2 < nk) {
+            ca.issue(A, (kc + 2) * C::KC, S0, wv);
+            cb.issue(B, (kc + 2) * C::KC, S0 + C::OPB, wv);
+        }
+        mma_chunk<T>(S1, S1 + C::OPB, wr, wc, l, acc);
+    }
+    if (t.mode == FDM_OUT_PLAIN) {
+        SE_GLOBAL T* Cm = (SE_GLOBAL T*)t.C;
+        for_each_acc<T>(acc, wr, wc, l,
+                        [&](int row, int col, double v) { Cm[(size_t)(m0 + row) * t.ldc + n0 + col] = to_t<T>(v); });
     } else if (t.mode == FDM_OUT_SCALE) {
-        SE_GLOBAL T* C = (SE_GLOBAL T*)t.C;
+        SE_GLOBAL T* Cm = (SE_GLOBAL T*)t.C;
         const SE_GLOBAL double* lx = (const SE_GLOBAL double*)t.lx;
         const SE_GLOBAL double* ly = (const SE_GLOBAL double*)t.ly;
         const SE_GLOBAL double* ex = (const SE_GLOBAL double*)t.ex;
@@ -267,7 +295,7 @@ __device__ __forceinline__ void gemm_tile(const GemmTask& t, int tile, char* lds
                 const double D = lx[gr] + ly[gc] + ex[gr] * ey[gc];
                 s = D > t.dtol ? 1.0 / D : 0.0;
             }
-            C[(size_t)gr * t.ldc + gc] = to_t<T>(v * s);
+            Cm[(size_t)gr * t.ldc + gc] = to_t<T>(v * s);
         });
     } else {  // FDM_OUT_FINAL: FP64 z on the unknown grid, partials of r.z, sum z, sum r
         SE_GLOBAL double* Z = (SE_GLOBAL double*)t.C;
@@ -285,7 +313,7 @@ __device__ __forceinline__ void gemm_tile(const GemmTask& t, int tile, char* lds
             }
         });
         const double pv[3] = {rz, sz, sr};
-        block_store_partials_at<3, 256>(pv, t.part, tile, (t.M / 64) * t.tilesN);
+        block_store_partials_at<3, 256>(pv, t.part, tile, (t.M / FDM_BT) * t.tilesN);
     }
 }
 
@@ -303,22 +331,24 @@ __device__ __forceinline__ bool fdm_skip(const GemmTask& t) {
     return !(rr - rr == 0.0) || rr <= t.st->tol2;
 }
 
-// pass `pass` of the members' transforms; work items (member, tile), or
-// tiles of member 0 when work is null
+// pass `pass` of the listed members' transforms; work items (member, tile),
+// or tiles of member 0 when work is null.  Two stages of 16 KB each: four
+// workgroups per CU (one wave of each per SIMD, at most 128 VGPRs).
 template <typename T>
-__global__ __launch_bounds__(256) void k_fdm_gemm(const GemmTask* __restrict__ tasks, int pass,
-                                                  const int2* __restrict__ work, int nwork, int skip) {
-    __shared__ __attribute__((aligned(16))) char lds[gemm_lds_bytes<T>()];
+__global__ __launch_bounds__(256, 4) void k_fdm_gemm(const GemmTask* __restrict__ tasks, int pass,
+                                                    const int2* __restrict__ work, int nwork, int skip) {
+    __shared__ __attribute__((aligned(1024))) char S0[GemmCfg<T>::STAGE];
+    __shared__ __attribute__((aligned(1024))) char S1[GemmCfg<T>::STAGE];
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
         const int2 it = work ? work[w] : make_int2(0, w);
         if (it.x < 0) continue;  // a padding item of the XCD-ordered list
         const GemmTask t = tasks[it.x * 4 + pass];
         if (skip && fdm_skip(t)) continue;
-        gemm_tile<T>(t, it.y, lds);
+        gemm_tile<T>(t, it.y, S0, S1);
     }
 }
 
-// pass 1's A operand: the FP64 residual r (nU1 x nU0) rounded to T into the
+// pass 1's Bt operand: the FP64 residual r (nU1 x nU0) rounded to T into the
 // padded nP1 x nP0 array, zeros outside; one 64 x 64 block per item
 template <typename T>
 __global__ __launch_bounds__(256) void k_fdm_pack(const GemmTask* __restrict__ tasks, const int2* __restrict__ work,
@@ -328,14 +358,14 @@ __global__ __launch_bounds__(256) void k_fdm_pack(const GemmTask* __restrict__ t
         if (it.x < 0) continue;
         const GemmTask& t = tasks[it.x * 4];
         if (skip && fdm_skip(t)) continue;
-        const int tilesN = t.K / 64;  // pass 1: K = nP0 columns
+        const int tilesN = t.K / FDM_BT;  // pass 1: K = nP0 columns of the packed array
         const int tm = it.y / tilesN, tn = it.y - tm * tilesN;
-        T* __restrict__ P = (T*)t.A;
-        const double* __restrict__ r = t.r;
-        for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-            const int gr = tm * 64 + (e >> 6), gc = tn * 64 + (e & 63);
+        SE_GLOBAL T* P = (SE_GLOBAL T*)t.Bt;
+        const SE_GLOBAL double* r = (const SE_GLOBAL double*)t.r;
+        for (int e = threadIdx.x; e < FDM_BT * FDM_BT; e += 256) {
+            const int gr = tm * FDM_BT + e / FDM_BT, gc = tn * FDM_BT + e % FDM_BT;
             const double v = (gr < t.arows && gc < t.acols) ? r[(size_t)gr * t.acols + gc] : 0.0;
-            P[(size_t)gr * t.lda + gc] = to_t<T>(v);
+            P[(size_t)gr * t.ldb + gc] = to_t<T>(v);
         }
     }
 }
@@ -853,21 +883,20 @@ void fdm_tasks(se_op* op, int prec, const double* r, double* z, double* part, Ge
         t[k] = GemmTask{};
         t[k].dtol = F->dtol;
     }
-    // pass 1: T1t[k'][J] = sum_I R[J][I] Vx[I][k']   (A = R packed into t2, Bt = Vx^T,
-    // store transposed)
-    t[0].A = P.t2.ptr;
-    t[0].r = r;
-    t[0].arows = F->nU1;
-    t[0].acols = F->nU0;
+    // pass 1: T1t[k'][J] = sum_I Vx[I][k'] R[J][I]   (A = Vx^T, Bt = R packed into t2)
+    t[0].A = P.mx->vT.ptr;
     t[0].lda = nP0;
-    t[0].Bt = P.mx->vT.ptr;
+    t[0].Bt = P.t2.ptr;
     t[0].ldb = nP0;
+    t[0].r = r;
+    t[0].arows = F->nU1;  // the packed residual's true extent
+    t[0].acols = F->nU0;
     t[0].C = P.t1.ptr;
     t[0].ldc = nP1;
-    t[0].M = nP1;
-    t[0].N = nP0;
+    t[0].M = nP0;
+    t[0].N = nP1;
     t[0].K = nP0;
-    t[0].mode = FDM_OUT_T;
+    t[0].mode = FDM_OUT_PLAIN;
     // pass 2: T2[k'][j'] = sum_J T1t[k'][J] Vy[J][j'] / D(k', j')   (Bt = Vy^T)
     t[1].A = P.t1.ptr;
     t[1].lda = nP1;
@@ -885,17 +914,17 @@ void fdm_tasks(se_op* op, int prec, const double* r, double* z, double* part, Ge
     t[1].ey = F->ey.as<double>();
     t[1].crows = F->nU0;
     t[1].ccols = F->nU1;
-    // pass 3: T3t[J][k'] = sum_j' T2[k'][j'] Vy[J][j']   (Bt = Vy, store transposed)
-    t[2].A = P.t2.ptr;
+    // pass 3: T3t[J][k'] = sum_j' Vy[J][j'] T2[k'][j']   (A = Vy, Bt = T2)
+    t[2].A = P.my->v.ptr;
     t[2].lda = nP1;
-    t[2].Bt = P.my->v.ptr;
+    t[2].Bt = P.t2.ptr;
     t[2].ldb = nP1;
     t[2].C = P.t1.ptr;
     t[2].ldc = nP0;
-    t[2].M = nP0;
-    t[2].N = nP1;
+    t[2].M = nP1;
+    t[2].N = nP0;
     t[2].K = nP1;
-    t[2].mode = FDM_OUT_T;
+    t[2].mode = FDM_OUT_PLAIN;
     // pass 4: Z[J][I] = sum_k' T3t[J][k'] Vx[I][k']   (Bt = Vx), FP64 on the unknown grid
     t[3].A = P.t1.ptr;
     t[3].lda = nP0;
@@ -911,7 +940,7 @@ void fdm_tasks(se_op* op, int prec, const double* r, double* z, double* part, Ge
     t[3].mode = FDM_OUT_FINAL;
     t[3].crows = F->nU1;
     t[3].ccols = F->nU0;
-    for (int k = 0; k < 4; k++) t[k].tilesN = t[k].N / 64;
+    for (int k = 0; k < 4; k++) t[k].tilesN = t[k].N / FDM_BT;
 }
 
 // the kernels' shape assumptions, checked on the host before any launch
@@ -919,25 +948,28 @@ bool fdm_tasks_valid(const GemmTask (&t)[4], int prec) {
     const int kc = prec == FDM_F64 ? GemmCfg<double>::KC : (prec == FDM_F32 ? GemmCfg<float>::KC : GemmCfg<__bf16>::KC);
     for (int k = 0; k < 4; k++) {
         if (t[k].M <= 0 || t[k].N <= 0 || t[k].K <= 0) return false;
-        if (t[k].M % 64 || t[k].N % 64 || t[k].K % 64 || t[k].K % kc) return false;
+        if (t[k].M % FDM_BT || t[k].N % FDM_BT || t[k].K % 64 || t[k].K % kc) return false;
         if (t[k].lda < t[k].K || t[k].ldb < t[k].K || t[k].lda % 64 || t[k].ldb % 64) return false;
     }
-    return t[0].arows <= t[0].M && t[0].acols <= t[0].K && t[3].crows <= t[3].M && t[3].ccols <= t[3].N;
+    // the pack fills pass 1's Bt (N x K) with its 64 x 64 blocks
+    return t[0].arows <= t[0].N && t[0].acols <= t[0].K && t[3].crows <= t[3].M && t[3].ccols <= t[3].N;
 }
 
-int fdm_tiles(const se_op* op) { return (op->fdm->nP0 / 64) * (op->fdm->nP1 / 64); }
+int fdm_tiles(const se_op* op) { return (op->fdm->nP0 / FDM_BT) * (op->fdm->nP1 / FDM_BT); }
 
+template <typename T>
+void launch_gemm_t(const GemmTask* tasks, int pass, const int2* work, int nwork, hipStream_t s, int skip) {
+    const dim3 grid(nwork), block(256);
+    if (pass == 0) hipLaunchKernelGGL((k_fdm_pack<T>), grid, block, 0, s, tasks, work, nwork, skip);
+    hipLaunchKernelGGL((k_fdm_gemm<T>), grid, block, 0, s, tasks, pass, work, nwork, skip);
+}
+
+// pass `pass` (with pass 1's residual pack) of the listed items
 int launch_gemm(int prec, const GemmTask* tasks, int pass, const int2* work, int nwork, hipStream_t s, int skip = 0) {
     if (nwork <= 0) return SE_OK;
-    const dim3 grid(nwork), block(256);
-    if (pass == 0) {  // the residual, packed for pass 1
-        if (prec == FDM_F64) hipLaunchKernelGGL(k_fdm_pack<double>, grid, block, 0, s, tasks, work, nwork, skip);
-        else if (prec == FDM_F32) hipLaunchKernelGGL(k_fdm_pack<float>, grid, block, 0, s, tasks, work, nwork, skip);
-        else hipLaunchKernelGGL(k_fdm_pack<__bf16>, grid, block, 0, s, tasks, work, nwork, skip);
-    }
-    if (prec == FDM_F64) hipLaunchKernelGGL(k_fdm_gemm<double>, grid, block, 0, s, tasks, pass, work, nwork, skip);
-    else if (prec == FDM_F32) hipLaunchKernelGGL(k_fdm_gemm<float>, grid, block, 0, s, tasks, pass, work, nwork, skip);
-    else hipLaunchKernelGGL(k_fdm_gemm<__bf16>, grid, block, 0, s, tasks, pass, work, nwork, skip);
+    if (prec == FDM_F64) launch_gemm_t<double>(tasks, pass, work, nwork, s, skip);
+    else if (prec == FDM_F32) launch_gemm_t<float>(tasks, pass, work, nwork, s, skip);
+    else launch_gemm_t<__bf16>(tasks, pass, work, nwork, s, skip);
     return SE_OK;
 }
 
@@ -1014,8 +1046,9 @@ int launch_apply_batch(int N, const GLL& g, const ApplyMember* A, const int2* wo
 // the whole list -- its operand panels or halo lines then stay in that XCD's
 // L2 -- and are cut into contiguous runs spread over the XCDs otherwise (a
 // single large member keeps all eight busy).  Runs are placed by longest
-// processing time on their cost; within an XCD, runs of the same class
-// (sharing eigenvector matrices) are adjacent.  Positions a shorter XCD list
+// processing time on their cost; within an XCD, the runs with the costliest
+// tiles come first (a short tail), and runs of the same class (sharing
+// eigenvector matrices) and tile cost are adjacent.  Positions a shorter XCD list
 // leaves hold (-1, -1) items, which the kernels skip.
 constexpr int kXcds = 8;
 void xcd_interleave(const std::vector<int>& ids, const std::vector<int>& tiles, const std::vector<double>& cost,
@@ -1047,7 +1080,11 @@ void xcd_interleave(const std::vector<int>& ids, const std::vector<int>& tiles, 
     std::vector<std::vector<int2>> items(kXcds);
     size_t len = 0;
     for (int k = 0; k < kXcds; k++) {
-        std::stable_sort(x[k].begin(), x[k].end(), [](const Run* a, const Run* b) { return a->cls < b->cls; });
+        // the costliest tiles first (the shortest tail), a class's runs together
+        std::stable_sort(x[k].begin(), x[k].end(), [](const Run* a, const Run* b) {
+            const double ca = a->cost / (a->t1 - a->t0), cb = b->cost / (b->t1 - b->t0);
+            return ca != cb ? ca > cb : a->cls < b->cls;
+        });
         for (const Run* r : x[k])
             for (int t = r->t0; t < r->t1; t++) items[k].push_back(make_int2(r->id, t));
         len = std::max(len, items[k].size());
@@ -1152,12 +1189,18 @@ int grid_of(size_t items, const se_ctx* c) {
     return (int)std::max<size_t>(1, std::min<size_t>(items, (size_t)16 * c->cu_count));
 }
 
-// z = FDM(r) of the listed members: four passes
-int plan_fdm(FdmPlan* pl, int set, hipStream_t s, int skip = 0) {
+// one pass of the transforms of the listed members
+int plan_pass(FdmPlan* pl, int set, int pass, hipStream_t s, int skip = 0) {
     const FdmPlan::ListSet& L = pl->sets[set];
-    const int nw = (int)L.h_gwork.size();
+    return launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, L.d_gwork.as<int2>(), (int)L.h_gwork.size(), s,
+                       skip);
+}
+
+// z = FDM(r) of the listed members: four passes
+
+int plan_fdm(FdmPlan* pl, int set, hipStream_t s, int skip = 0) {
     for (int pass = 0; pass < 4; pass++) {
-        int rc = launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, L.d_gwork.as<int2>(), nw, s, skip);
+        int rc = plan_pass(pl, set, pass, s, skip);
         if (rc) return rc;
     }
     return SE_OK;
@@ -1508,8 +1551,6 @@ int fdm_plan_probe(FdmPlan* pl, int reps, double* us_pass, double* flops_pass, d
     std::vector<char> all(count, 1);
     int rc = plan_lists(pl, FdmPlan::SET_OTHER, all, s);
     if (rc) return rc;
-    const FdmPlan::ListSet& L = pl->sets[FdmPlan::SET_OTHER];
-    const int nw = (int)L.h_gwork.size();
     for (int pass = 0; pass < 4; pass++) {
         double fl = 0.0;
         for (se_op* op : pl->ops) {
@@ -1517,11 +1558,10 @@ int fdm_plan_probe(FdmPlan* pl, int reps, double* us_pass, double* flops_pass, d
             fl += 2.0 * nP0 * nP1 * ((pass == 0 || pass == 3) ? nP0 : nP1);
         }
         flops_pass[pass] = fl;
-        rc = launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, L.d_gwork.as<int2>(), nw, s);
+        rc = plan_pass(pl, FdmPlan::SET_OTHER, pass, s);
         if (rc) return rc;
         SE_HIP(hipEventRecord(c->ev0, s));
-        for (int r = 0; r < reps && !rc; r++)
-            rc = launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, L.d_gwork.as<int2>(), nw, s);
+        for (int r = 0; r < reps && !rc; r++) rc = plan_pass(pl, FdmPlan::SET_OTHER, pass, s);
         SE_HIP(hipEventRecord(c->ev1, s));
         SE_HIP(hipEventSynchronize(c->ev1));
         if (rc) return rc;

@@ -126,3 +126,44 @@ def test_fdm_refuses_what_it_cannot_invert(gpu_ctx):
         mass.pcg(np.ones(m.n), rtol=1e-10, preconditioner=7)
     mass.close()
     m.close()
+
+
+def test_fdm_large_tiles_invert_and_batch_with_small_ones(gpu_ctx):
+    """Operators whose padded extents are multiples of 128 take the 128 x 128
+    transform tiles (a periodic 32 x 16-element mesh at p = 8: 256 x 128
+    unknowns; a channel 8 x 16 at p = 8 with walls: 64 x 127).  The large-tile
+    layout is pinned by the inverse in every precision, and a batch mixing
+    both tile sizes gives each member its single solve bit for bit."""
+    cases = [((32, 16), 8, (1, 1), (0, 0, 0, 0), -2.0, -1.0, 1),
+             ((8, 16), 8, (1, 0), (1, 1, 0, 0), -30.0, -1.0, 0),
+             ((9, 7), 6, (1, 0), (1, 1, 0, 0), 5.0, 1.0, 1)]
+    mem = []
+    for nEls, N, per, bc, cm, cl, detj in cases:
+        om = O.OracleMesh(nEls, N, (0, 0), (1, 1), per, bc)
+        m = se_amd.Mesh(gpu_ctx, nEls, N, (0, 0), (1, 1), per, bc)
+        mem.append((O.Helm(om, cm, cl, detj), m, se_amd.Helm(m, cm, cl, detj)))
+    oh, m, h = mem[0]
+    r = np.random.default_rng(7).standard_normal(m.n)
+    sgn = 1.0 if oh.inv_diag()[0] > 0 else -1.0
+    z64 = h.apply_fdm(r, se_amd.PREC_FP64)
+    res = rel(sgn * oh.matvec(z64), r)
+    print("256 x 128 unknowns, FP64: ||sA z - r|| / ||r|| = %.2e" % res)
+    assert res <= INV_TOL[se_amd.PREC_FP64], res
+    # This operator is stiff (kappa ~ 5e5), so the residual of a rounded inverse
+    # is large; the reduced-precision transforms are instead held to the FP64
+    # one within their own rounding (a misplaced tile would differ by O(1)).
+    for prec, tol in ((se_amd.PREC_FP32_IR, 1e-4), (se_amd.PREC_BF16_IR, 5e-2)):
+        d = rel(h.apply_fdm(r, prec), z64)
+        print("256 x 128 unknowns, prec %d: ||z - z64|| / ||z64|| = %.2e" % (prec, d))
+        assert d <= tol, d
+    rng = np.random.default_rng(29)
+    fs = [rng.standard_normal(mm.n) for _, mm, _ in mem]
+    b = se_amd.Batch(gpu_ctx, [hh for _, _, hh in mem])
+    for prec, _ in PRECS:
+        xs, infos = b.pcg(fs, precision=prec, rtol=1e-10, preconditioner=se_amd.PC_FDM)
+        for (ohh, _, hh), f, xb, ib in zip(mem, fs, xs, infos):
+            assert ib["status"] == se_amd.SOLVE_CONVERGED, ib
+            assert rel(ohh.matvec(xb), f) <= 1.5e-10
+            x1, i1 = hh.pcg(f, rtol=1e-10, precision=prec, preconditioner=se_amd.PC_FDM)
+            assert i1["iters"] == ib["iters"] and np.array_equal(x1, xb)
+    b.close()
