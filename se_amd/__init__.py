@@ -46,6 +46,40 @@ class SolveInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class SolveInfos:
+    """Read-only sequence of per-member solve results (dicts with SolveInfo's
+    fields), converted on access: a batch call of hundreds of members then
+    costs no per-member Python work until the results are read."""
+
+    _names = [k for k, _ in SolveInfo._fields_]
+
+    def __init__(self, arr, n):
+        self._arr = arr
+        self._n = n
+
+    def __len__(self):
+        return self._n
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [self[j] for j in range(*i.indices(self._n))]
+        if i < 0:
+            i += self._n
+        if not 0 <= i < self._n:
+            raise IndexError(i)
+        r = self._arr[i]
+        return {k: getattr(r, k) for k in self._names}
+
+    def __iter__(self):
+        return (self[i] for i in range(self._n))
+
+    def __eq__(self, other):
+        return list(self) == list(other)
+
+    def __repr__(self):
+        return repr(list(self))
+
+
 class ServerOpts(C.Structure):
     _fields_ = [("memory_budget", C.c_double), ("max_active", C.c_int), ("precision", C.c_int),
                 ("rtol", C.c_double), ("detj_mode", C.c_int), ("preconditioner", C.c_int)]
@@ -577,13 +611,23 @@ class Batch:
         check(lib().se_batch_pcg(self.h, fp, xp, C.byref(o), infos), "se_batch_pcg")
         return xs, [infos[i].as_dict() for i in range(len(self.helms))]
 
+    def _ptrs(self, slot, ptrs):
+        """ctypes array of device pointers, reused while the caller passes the same ones."""
+        key = tuple(ptrs)
+        cache = self.__dict__.setdefault("_ptr_cache", {})
+        hit = cache.get(slot)
+        if hit is None or hit[0] != key:
+            hit = (key, (C.c_void_p * max(1, len(key)))(*key))
+            cache[slot] = hit
+        return hit[1]
+
     def pcg_dev(self, f_ptrs, x_ptrs, precision=PREC_FP32_IR, **kw):
+        """Device vectors in and out; returns a SolveInfos sequence (one dict per member)."""
         o = solve_opts(precision=precision, **kw)
-        fp = (C.c_void_p * max(1, len(f_ptrs)))(*f_ptrs)
-        xp = (C.c_void_p * max(1, len(x_ptrs)))(*x_ptrs)
         infos = self._infos()
-        check(lib().se_batch_pcg_dev(self.h, fp, xp, C.byref(o), infos), "se_batch_pcg_dev")
-        return [infos[i].as_dict() for i in range(len(self.helms))]
+        check(lib().se_batch_pcg_dev(self.h, self._ptrs("f", f_ptrs), self._ptrs("x", x_ptrs), C.byref(o), infos),
+              "se_batch_pcg_dev")
+        return SolveInfos(infos, len(self.helms))
 
     def fdm_probe(self, reps=20):
         """After an FDM solve of the batch: as Helm.fdm_probe over all members."""

@@ -994,19 +994,40 @@ struct FdmPlan {
     };
     std::vector<Order> orders;   // one per order 2..8 (members may be empty)
     // Work lists of one member subset: (member, chunk) items of the vector
-    // kernels, (member, tile) items of the transforms, the member list, and per
-    // order the apply's (local member, tile) items.  Three independent sets --
-    // the running members' iteration, the true-residual checks, restarts and
-    // the end of a solve -- so that lists launched together in one round never
-    // share host vectors: a set's host vectors are rebuilt only after a
-    // stream synchronisation has completed the uploads from them.
+    // kernels, (member, tile) items of the transforms, per order the apply's
+    // (local member, tile) items, and the member list, packed in one device
+    // buffer.  Three independent sets -- the running members' iteration, the
+    // true-residual checks, restarts and the end of a solve -- each keep the
+    // lists of the last kListCache subsets they were asked for (successive
+    // solves of a batch meet the same subsets: all members, then the same
+    // stragglers), so a repeated subset costs neither host work nor an upload.
+    // A new subset's lists go up in one copy from pinned memory (a DMA queued
+    // on the stream; from pageable memory the runtime stages the copy after
+    // waiting for the stream, and the GPU idles around it); the staging is
+    // rewritten only after a stream synchronisation, and lists launched
+    // together in one round come from different sets.
+    struct Lists {
+        std::vector<char> on;  // the member flags these lists were built for
+        DevBuf d;
+        const int2 *vwork = nullptr, *gwork = nullptr, *apply[7] = {};
+        const int* list = nullptr;
+        int n_vwork = 0, n_gwork = 0, n_apply[7] = {}, n_list = 0;
+        unsigned long long used = 0;
+    };
+    static constexpr int kListCache = 8;
     struct ListSet {
-        std::vector<char> on, apply_on;  // member flags the device lists were built for
-        std::vector<int2> h_vwork, h_gwork;
-        std::vector<int> h_list;
-        DevBuf d_vwork, d_gwork, d_list;
-        std::vector<int2> h_apply[7];
-        DevBuf d_apply[7];
+        std::vector<std::unique_ptr<Lists>> cache;
+        const Lists* cur = nullptr;  // the lists of the set's current subset
+        void* h = nullptr;           // pinned staging of an upload
+        size_t hcap = 0;
+        unsigned long long clock = 0;
+        void clear() {  // (no launch from these lists may be pending)
+            cache.clear();
+            cur = nullptr;
+        }
+        ~ListSet() {
+            if (h) (void)hipHostFree(h);
+        }
     };
     enum { SET_ITER = 0, SET_CHECK = 1, SET_OTHER = 2 };
     ListSet sets[3];
@@ -1094,46 +1115,22 @@ void xcd_interleave(const std::vector<int>& ids, const std::vector<int>& tiles, 
         for (int k = 0; k < kXcds; k++) out.push_back(j < items[k].size() ? items[k][j] : make_int2(-1, -1));
 }
 
-// q = sign A p (or sign A x) of the members flagged in `on`, with list set
-// `set` (its device lists rebuilt when the flags changed).
-int plan_apply(FdmPlan* pl, int set, bool from_x, const std::vector<char>& on, hipStream_t s) {
-    FdmPlan::ListSet& L = pl->sets[set];
-    const bool same = on == L.apply_on;
-    if (!same) L.apply_on = on;
+// q = sign A p (or sign A x) of the members of list set `set` (whose lists
+// plan_lists built)
+int plan_apply(FdmPlan* pl, int set, bool from_x, hipStream_t s) {
+    const FdmPlan::Lists& L = *pl->sets[set].cur;
     for (int k = 0; k < (int)pl->orders.size(); k++) {
         const FdmPlan::Order& o = pl->orders[k];
-        if (o.members.empty()) continue;
-        std::vector<int2>& hw = L.h_apply[k];
-        if (!same) {
-            std::vector<int> ids, tiles;
-            std::vector<double> cost;
-            std::vector<const void*> cls;
-            for (int j = 0; j < (int)o.members.size(); j++) {
-                const int i = o.members[j];
-                if (!on[i]) continue;
-                const se_mesh* m = pl->ops[i]->mesh;
-                ids.push_back(j);
-                tiles.push_back(line_tiles(m->N, m->nEls[0], m->nEls[1]));
-                cost.push_back((double)tiles.back());
-                cls.push_back(nullptr);
-            }
-            xcd_interleave(ids, tiles, cost, cls, hw);
-            if (!hw.empty()) {
-                int rc = L.d_apply[k].alloc(sizeof(int2) * hw.size());
-                if (rc) return rc;
-                SE_HIP(hipMemcpyAsync(L.d_apply[k].ptr, hw.data(), sizeof(int2) * hw.size(), hipMemcpyHostToDevice, s));
-            }
-        }
-        if (hw.empty()) continue;
+        const int nw = L.n_apply[k];
+        if (o.members.empty() || nw == 0) continue;
         const ApplyMember* A = from_x ? o.d_ax.as<ApplyMember>() : o.d_ap.as<ApplyMember>();
-        int rc = launch_apply_batch(o.N, pl->ops[o.members[0]]->mesh->gll, A, L.d_apply[k].as<int2>(), (int)hw.size(),
-                                    s);
+        int rc = launch_apply_batch(o.N, pl->ops[o.members[0]]->mesh->gll, A, L.apply[k], nw, s);
         if (rc) return rc;
     }
     // members with an SVV term: add it to the batched apply's result
     for (int i = 0; i < (int)pl->ops.size(); i++) {
         se_op* op = pl->ops[i];
-        if (!on[i] || op->svv_mN == 0) continue;
+        if (!L.on[i] || op->svv_mN == 0) continue;
         int rc = svv_apply(op, from_x ? op->x.as<double>() : op->p.as<double>(), op->q.as<double>(),
                            (double)op->sign);
         if (rc) return rc;
@@ -1157,31 +1154,89 @@ void xcd_order(const FdmPlan* pl, const std::vector<char>& on, std::vector<int2>
     xcd_interleave(ids, tiles, cost, cls, out);
 }
 
-// work lists of the members flagged in `on`, in list set `set`
+// list set `set` switched to the members flagged in `on`: their lists from
+// the set's cache, or built and uploaded (one copy) into it
 int plan_lists(FdmPlan* pl, int set, const std::vector<char>& on, hipStream_t s) {
-    FdmPlan::ListSet& L = pl->sets[set];
-    if (on == L.on) return SE_OK;  // the device lists are current
-    L.on = on;
-    L.h_vwork.clear();
-    L.h_list.clear();
+    FdmPlan::ListSet& S = pl->sets[set];
+    S.clock++;
+    for (auto& e : S.cache)
+        if (e->on == on) {
+            e->used = S.clock;
+            S.cur = e.get();
+            return SE_OK;
+        }
+    std::vector<int2> vwork, gwork, apply[7];
+    std::vector<int> list;
     for (int i = 0; i < (int)pl->ops.size(); i++) {
         if (!on[i]) continue;
-        L.h_list.push_back(i);
-        for (int c = 0; c < pl->h_members[i].nch; c++) L.h_vwork.push_back(make_int2(i, c));
+        list.push_back(i);
+        for (int c = 0; c < pl->h_members[i].nch; c++) vwork.push_back(make_int2(i, c));
     }
-    xcd_order(pl, on, L.h_gwork);
-    int rc = L.d_vwork.alloc(sizeof(int2) * std::max<size_t>(L.h_vwork.size(), 1));
-    if (!rc) rc = L.d_gwork.alloc(sizeof(int2) * std::max<size_t>(L.h_gwork.size(), 1));
-    if (!rc) rc = L.d_list.alloc(sizeof(int) * std::max<size_t>(L.h_list.size(), 1));
+    xcd_order(pl, on, gwork);
+    for (int k = 0; k < (int)pl->orders.size(); k++) {
+        const FdmPlan::Order& o = pl->orders[k];
+        std::vector<int> ids, tiles;
+        std::vector<double> cost;
+        std::vector<const void*> cls;
+        for (int j = 0; j < (int)o.members.size(); j++) {
+            const int i = o.members[j];
+            if (!on[i]) continue;
+            const se_mesh* m = pl->ops[i]->mesh;
+            ids.push_back(j);
+            tiles.push_back(line_tiles(m->N, m->nEls[0], m->nEls[1]));
+            cost.push_back((double)tiles.back());
+            cls.push_back(nullptr);
+        }
+        xcd_interleave(ids, tiles, cost, cls, apply[k]);
+    }
+    // packed: vector items, transform items, per-order apply items (int2), member list (int)
+    size_t n2 = vwork.size() + gwork.size();
+    for (const auto& a : apply) n2 += a.size();
+    const size_t bytes = sizeof(int2) * n2 + sizeof(int) * list.size();
+    if (bytes > S.hcap) {
+        if (S.h) {
+            SE_HIP(hipStreamSynchronize(s));  // no upload from the old staging is pending
+            (void)hipHostFree(S.h);
+            S.h = nullptr;
+            S.hcap = 0;
+        }
+        SE_HIP(hipHostMalloc(&S.h, bytes, hipHostMallocDefault));
+        S.hcap = bytes;
+    }
+    if ((int)S.cache.size() >= FdmPlan::kListCache) {  // the least recently used subset makes room
+        SE_HIP(hipStreamSynchronize(s));                // no launch reads its lists
+        auto lru = std::min_element(S.cache.begin(), S.cache.end(),
+                                    [](const std::unique_ptr<FdmPlan::Lists>& a,
+                                       const std::unique_ptr<FdmPlan::Lists>& b) { return a->used < b->used; });
+        S.cache.erase(lru);
+    }
+    std::unique_ptr<FdmPlan::Lists> L(new FdmPlan::Lists);
+    int rc = L->d.alloc(std::max<size_t>(bytes, sizeof(int2)));
     if (rc) return rc;
-    if (!L.h_vwork.empty())
-        SE_HIP(hipMemcpyAsync(L.d_vwork.ptr, L.h_vwork.data(), sizeof(int2) * L.h_vwork.size(), hipMemcpyHostToDevice,
-                              s));
-    if (!L.h_gwork.empty())
-        SE_HIP(hipMemcpyAsync(L.d_gwork.ptr, L.h_gwork.data(), sizeof(int2) * L.h_gwork.size(), hipMemcpyHostToDevice,
-                              s));
-    if (!L.h_list.empty())
-        SE_HIP(hipMemcpyAsync(L.d_list.ptr, L.h_list.data(), sizeof(int) * L.h_list.size(), hipMemcpyHostToDevice, s));
+    char* h = static_cast<char*>(S.h);
+    const char* d = L->d.as<char>();
+    size_t off = 0;
+    auto put = [&](const void* src, size_t nb) -> const char* {
+        if (nb) std::memcpy(h + off, src, nb);
+        const char* at = d + off;
+        off += nb;
+        return at;
+    };
+    L->vwork = reinterpret_cast<const int2*>(put(vwork.data(), sizeof(int2) * vwork.size()));
+    L->gwork = reinterpret_cast<const int2*>(put(gwork.data(), sizeof(int2) * gwork.size()));
+    for (int k = 0; k < 7; k++) {
+        L->apply[k] = reinterpret_cast<const int2*>(put(apply[k].data(), sizeof(int2) * apply[k].size()));
+        L->n_apply[k] = (int)apply[k].size();
+    }
+    L->list = reinterpret_cast<const int*>(put(list.data(), sizeof(int) * list.size()));
+    L->n_vwork = (int)vwork.size();
+    L->n_gwork = (int)gwork.size();
+    L->n_list = (int)list.size();
+    L->on = on;
+    L->used = S.clock;
+    if (bytes) SE_HIP(hipMemcpyAsync(L->d.ptr, S.h, bytes, hipMemcpyHostToDevice, s));
+    S.cur = L.get();
+    S.cache.push_back(std::move(L));
     return SE_OK;
 }
 
@@ -1191,9 +1246,8 @@ int grid_of(size_t items, const se_ctx* c) {
 
 // one pass of the transforms of the listed members
 int plan_pass(FdmPlan* pl, int set, int pass, hipStream_t s, int skip = 0) {
-    const FdmPlan::ListSet& L = pl->sets[set];
-    return launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, L.d_gwork.as<int2>(), (int)L.h_gwork.size(), s,
-                       skip);
+    const FdmPlan::Lists& L = *pl->sets[set].cur;
+    return launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, L.gwork, L.n_gwork, s, skip);
 }
 
 // z = FDM(r) of the listed members: four passes
@@ -1236,10 +1290,7 @@ int fdm_plan_set_members(FdmPlan* pl, int count, se_op* const* ops) {
     if (rc) return rc;
     pl->prec = -1;
     pl->desc_ok = false;
-    for (auto& L : pl->sets) {
-        L.on.clear();
-        L.apply_on.clear();
-    }
+    for (auto& L : pl->sets) L.clear();
     return SE_OK;
 }
 
@@ -1351,7 +1402,7 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
             SE_HIP(hipMemcpyAsync(od.d_ax.ptr, ax.data(), sizeof(ApplyMember) * ax.size(), hipMemcpyHostToDevice, s));
             SE_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
         }
-        for (auto& L : pl->sets) L.apply_on.clear();  // the apply descriptors were re-uploaded
+        for (auto& L : pl->sets) L.clear();  // new descriptors: new lists
         pl->desc_ok = true;
         pl->desc_null = null;
         pl->desc_rtol = o.rtol;
@@ -1371,25 +1422,24 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
         return SE_OK;
     };
     auto vlaunch = [&](int set, void (*k)(const FbMember*, const int2*, int)) {
-        const FdmPlan::ListSet& L = pl->sets[set];
-        const int nw = (int)L.h_vwork.size();
-        if (nw > 0) hipLaunchKernelGGL(k, dim3(grid_of(nw, c)), dim3(FB_NT), 0, s, Ms, L.d_vwork.as<int2>(), nw);
+        const FdmPlan::Lists& L = *pl->sets[set].cur;
+        const int nw = L.n_vwork;
+        if (nw > 0) hipLaunchKernelGGL(k, dim3(grid_of(nw, c)), dim3(FB_NT), 0, s, Ms, L.vwork, nw);
     };
     auto dlaunch = [&](int set, int mode) {
-        const FdmPlan::ListSet& L = pl->sets[set];
-        const int nw = (int)L.h_vwork.size();
+        const FdmPlan::Lists& L = *pl->sets[set].cur;
+        const int nw = L.n_vwork;
         if (nw > 0)
-            hipLaunchKernelGGL(k_fb_dir, dim3(grid_of(nw, c)), dim3(FB_NT), 0, s, Ms, L.d_vwork.as<int2>(), nw, mode);
+            hipLaunchKernelGGL(k_fb_dir, dim3(grid_of(nw, c)), dim3(FB_NT), 0, s, Ms, L.vwork, nw, mode);
     };
     // the true residual of the flagged members: t = b - A x into r, ||t - mean t||^2 into the state
     auto true_residual = [&](int set, const std::vector<char>& flags) -> int {
         int e = plan_lists(pl, set, flags, s);
-        if (!e) e = plan_apply(pl, set, true, flags, s);
+        if (!e) e = plan_apply(pl, set, true, s);
         if (e) return e;
         vlaunch(set, k_fb_true);
-        const FdmPlan::ListSet& L = pl->sets[set];
-        hipLaunchKernelGGL(k_fb_true_fin, dim3((int)L.h_list.size()), dim3(FB_NT), 0, s, Ms, L.d_list.as<int>(),
-                           (int)L.h_list.size());
+        const FdmPlan::Lists& L = *pl->sets[set].cur;
+        if (L.n_list > 0) hipLaunchKernelGGL(k_fb_true_fin, dim3(L.n_list), dim3(FB_NT), 0, s, Ms, L.list, L.n_list);
         return SE_OK;
     };
     const int IT = FdmPlan::SET_ITER, CK = FdmPlan::SET_CHECK, OT = FdmPlan::SET_OTHER;
@@ -1428,7 +1478,7 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
         if (!any_on && !any_chk) break;
         if (any_on) {  // q = A p, alpha, x and r, z = FDM(r), beta and p
             rc = plan_lists(pl, IT, on, s);
-            if (!rc) rc = plan_apply(pl, IT, false, on, s);
+            if (!rc) rc = plan_apply(pl, IT, false, s);
             if (rc) return rc;
             vlaunch(IT, k_fb_dot);
             vlaunch(IT, k_fb_update);
@@ -1572,10 +1622,10 @@ int fdm_plan_probe(FdmPlan* pl, int reps, double* us_pass, double* flops_pass, d
     double by = 0.0;
     for (se_op* op : pl->ops) by += 16.0 * op->mesh->n;
     *bytes_apply = by;
-    rc = plan_apply(pl, FdmPlan::SET_OTHER, false, all, s);
+    rc = plan_apply(pl, FdmPlan::SET_OTHER, false, s);
     if (rc) return rc;
     SE_HIP(hipEventRecord(c->ev0, s));
-    for (int r = 0; r < reps && !rc; r++) rc = plan_apply(pl, FdmPlan::SET_OTHER, false, all, s);
+    for (int r = 0; r < reps && !rc; r++) rc = plan_apply(pl, FdmPlan::SET_OTHER, false, s);
     SE_HIP(hipEventRecord(c->ev1, s));
     SE_HIP(hipEventSynchronize(c->ev1));
     if (rc) return rc;
