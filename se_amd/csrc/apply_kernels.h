@@ -80,6 +80,9 @@ struct ApplyMember {
     OpParams P;
     const double* in;
     double* out;
+    double* part = nullptr;      // AP_DOT / AP_TRUE partials, np of them per sum
+    const double* b = nullptr;   // AP_TRUE: the right-hand side
+    int np = 0;
 };
 
 }  // namespace se

@@ -34,12 +34,17 @@
 //      element node meet in one FP64 LDS array (Y_y first, then + Y_x).
 //   4. node phase: each owned node adds the contributions of the up to four
 //      elements containing it (bottom-left, bottom, left, own), scales, and
-//      stores; with DOT the workgroup's partial of x.y goes to partials[tile].
+//      stores y (AP_PLAIN); with AP_DOT also the workgroup's partial of x.y
+//      to partials[tile]; with AP_TRUE it stores t = b - y instead and the
+//      partials of sum t and sum t^2 (a CG's true residual).  A batch member
+//      without partials (or, for AP_TRUE, without b) gets plain y.
 // LDS stays under 40 KB at every order (four workgroups per CU).
 #pragma once
 #include "apply_kernels.h"
 
 namespace se {
+
+enum { AP_PLAIN = 0, AP_DOT = 1, AP_TRUE = 2 };  // epilogues of apply_lines_tile
 
 // owned elements per tile: (TX+1)(TY+1)(N+1) <= 256 line threads
 template <int N> struct LineShape;
@@ -83,10 +88,11 @@ __device__ __forceinline__ int unk_line(int X, int M, int per, int off, int nU) 
     return (I >= 0 && I < nU) ? I : -1;
 }
 
-template <int N, bool DOT>
+template <int N, int MODE>
 __device__ __forceinline__ void apply_lines_tile(const OpParams& P, int tile,
                                                  const double* __restrict__ in, double* __restrict__ out,
-                                                 double* __restrict__ partials, int np) {
+                                                 double* __restrict__ partials, int np,
+                                                 const double* __restrict__ b = nullptr) {
     constexpr int TX = LineShape<N>::TX, TY = LineShape<N>::TY;
     constexpr int N1 = N + 1, NP = N1 * N1;
     constexpr int BWU = (TX + 1) * N + 1, BH = (TY + 1) * N + 1;
@@ -250,7 +256,7 @@ __device__ __forceinline__ void apply_lines_tile(const OpParams& P, int tile,
     // ---- 4. owned nodes ----------------------------------------------------------
     const bool close_x = !P.per0 && ex0 + txo == P.nEls0, close_y = !P.per1 && ey0 + tyo == P.nEls1;
     const int NOX = txo * N + (close_x ? 1 : 0), NOY = tyo * N + (close_y ? 1 : 0);
-    double dot = 0.0;
+    double dot = 0.0, st = 0.0, st2 = 0.0;
     for (int i = tid; i < NOX * NOY; i += 256) {
         const int oy = i / NOX, ox = i - oy * NOX;
         const int I = s_I[N + ox], J = s_J[N + oy];
@@ -266,12 +272,24 @@ __device__ __forceinline__ void apply_lines_tile(const OpParams& P, int tile,
         for (int jr = 0; jr < nr; jr++)
             for (int jc = 0; jc < nc; jc++) sum += s_o[(rs[jr] * (TX + 1) + cs[jc]) * NP + ms[jr] * N1 + ls[jc]];
         const double yv = P.scale * sum;
-        gout[(size_t)J * P.nU0 + I] = yv;
-        if (DOT) dot += s_u[(N + oy) * BW + N + ox] * yv;
+        const size_t gi = (size_t)J * P.nU0 + I;
+        if (MODE == AP_TRUE && b) {
+            const double ti = b[gi] - yv;
+            gout[gi] = ti;
+            st += ti;
+            st2 += ti * ti;
+        } else {
+            gout[gi] = yv;
+            if (MODE == AP_DOT) dot += s_u[(N + oy) * BW + N + ox] * yv;
+        }
     }
-    if (DOT) {
+    if (!partials) return;
+    if (MODE == AP_DOT) {
         double v[1] = {dot};
         block_store_partials_at<1, 256>(v, partials, tile, np);
+    } else if (MODE == AP_TRUE) {
+        double v[2] = {st, st2};
+        block_store_partials_at<2, 256>(v, partials, tile, np);
     }
 }
 
@@ -284,11 +302,12 @@ __global__ __launch_bounds__(256) void k_apply_lines(OpParams P, const double* _
     // tiles, so that neighbouring tiles' shared halo lines meet in one L2
     const int T = gridDim.x, b = blockIdx.x, q = T / 8, r = T % 8, x = b % 8;
     const int tile = x * q + min(x, r) + b / 8;
-    apply_lines_tile<N, DOT>(P, tile, in, out, partials, T);
+    apply_lines_tile<N, DOT ? AP_DOT : AP_PLAIN>(P, tile, in, out, partials, T);
 }
 
-// Batched: work items (member, tile) over members of one order
-template <int N>
+// Batched: work items (member, tile) over members of one order; MODE's
+// partials go to the member's `part` (line_tiles of its mesh entries)
+template <int N, int MODE>
 __global__ __launch_bounds__(256) void k_apply_lines_batch(const ApplyMember* __restrict__ members,
                                                             const int2* __restrict__ work, int nwork) {
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
@@ -296,7 +315,7 @@ __global__ __launch_bounds__(256) void k_apply_lines_batch(const ApplyMember* __
         if (it.x < 0) continue;  // a padding item of an XCD-ordered list
         const ApplyMember& a = members[it.x];
         __syncthreads();  // the previous item's LDS reads are done
-        apply_lines_tile<N, false>(a.P, it.y, a.in, a.out, nullptr, 0);
+        apply_lines_tile<N, MODE>(a.P, it.y, a.in, a.out, a.part, a.np, a.b);
     }
 }
 

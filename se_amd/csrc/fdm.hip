@@ -381,14 +381,28 @@ struct FbMember {
     double *b, *x, *r, *p, *q, *z;
     double* pv;  // [nch]      r.r after start / update / restart
     double* pz;  // [3][ntz]   FDM partials r.z, sum z, sum r
-    double* pa;  // [nch]      p.q
-    double* pt;  // [2][nch]   sum t, sum t^2 of the true residual
+    double* pa;  // [nap]      p.q
+    double* pt;  // [2][nap]   sum t, sum t^2 of the true residual
     double* ps;  // [nch]      sum of sgn f
     double* xo;  //            the caller's solution vector
+    void* rp;    //            r in the transforms' type T, padded: pass 1's operand
     CgState* st;
     int n, nch, ntz, null_space;
+    int fused;  // p.q and the true residual from the apply's epilogue (no SVV term)
+    int nap;    // partials of p.q and of the true residual: apply tiles (fused), else nch
+    int nU0, nP0, rprec;  //   unknown-grid row length, padded row length, T (FDM_F64 / F32 / BF16)
     double sgn, rtol;
 };
+
+// r[i] into the packed residual (row i / nU0, column i mod nU0), rounded to T
+// as the pack kernel rounds it
+__device__ __forceinline__ void put_packed(const FbMember& M, int i, double v) {
+    const int J = i / M.nU0, I = i - J * M.nU0;
+    const size_t k = (size_t)J * M.nP0 + I;
+    if (M.rprec == FDM_F64) ((SE_GLOBAL double*)M.rp)[k] = v;
+    else if (M.rprec == FDM_F32) ((SE_GLOBAL float*)M.rp)[k] = to_t<float>(v);
+    else ((SE_GLOBAL __bf16*)M.rp)[k] = to_t<__bf16>(v);
+}
 
 __device__ __forceinline__ void chunk_range(const FbMember& M, int ch, int& i0, int& i1) {
     i0 = ch * FB_CH;
@@ -430,6 +444,7 @@ __global__ __launch_bounds__(FB_NT) void k_fb_start(const FbMember* __restrict__
             const double bi = M.sgn * M.f[i] - mu;
             M.b[i] = bi;
             M.r[i] = bi;
+            put_packed(M, i, bi);
             M.x[i] = 0.0;
             rr += bi * bi;
         }
@@ -518,13 +533,14 @@ __global__ __launch_bounds__(FB_NT) void k_fb_dir(const FbMember* __restrict__ M
     }
 }
 
-// partials of p.q
+// partials of p.q (members with an SVV term)
 __global__ __launch_bounds__(FB_NT) void k_fb_dot(const FbMember* __restrict__ Ms, const int2* __restrict__ work,
                                                   int nwork) {
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
         __syncthreads();  // the previous item's reduction is done with its LDS
         const int2 it = work[w];
         const FbMember& M = Ms[it.x];
+        if (M.fused) continue;  // the apply formed its partials
         int i0, i1;
         chunk_range(M, it.y, i0, i1);
         double s = 0.0;
@@ -545,7 +561,7 @@ __global__ __launch_bounds__(FB_NT) void k_fb_update(const FbMember* __restrict_
         __shared__ int s_stop;
         if (threadIdx.x == 0) s_stop = cg_stopped(st);
         double t[1];
-        reduce_partials<1, FB_NT>(M.pa, M.nch, t);
+        reduce_partials<1, FB_NT>(M.pa, M.nap, t);
         if (s_stop) continue;
         const double pq = t[0];
         const bool lead = it.y == 0 && threadIdx.x == 0;
@@ -571,6 +587,7 @@ __global__ __launch_bounds__(FB_NT) void k_fb_update(const FbMember* __restrict_
             M.x[i] += alpha * M.p[i];
             const double ri = M.r[i] - alpha * M.q[i];
             M.r[i] = ri;
+            put_packed(M, i, ri);
             rr += ri * ri;
         }
         const double v[1] = {rr};
@@ -579,12 +596,14 @@ __global__ __launch_bounds__(FB_NT) void k_fb_update(const FbMember* __restrict_
 }
 
 // true residual t = b - q (q = A x): r = t; partials of sum t, sum t^2
+// (members with an SVV term)
 __global__ __launch_bounds__(FB_NT) void k_fb_true(const FbMember* __restrict__ Ms, const int2* __restrict__ work,
                                                    int nwork) {
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
         __syncthreads();  // the previous item's reduction is done with its LDS
         const int2 it = work[w];
         const FbMember& M = Ms[it.x];
+        if (M.fused) continue;  // the apply formed r and its partials
         int i0, i1;
         chunk_range(M, it.y, i0, i1);
         double s = 0.0, s2 = 0.0;
@@ -606,7 +625,7 @@ __global__ __launch_bounds__(FB_NT) void k_fb_true_fin(const FbMember* __restric
         const FbMember& M = Ms[list[w]];
         __syncthreads();
         double t[2];
-        reduce_partials<2, FB_NT>(M.pt, M.nch, t);
+        reduce_partials<2, FB_NT>(M.pt, M.nap, t);
         if (threadIdx.x == 0) {
             const double mt = M.null_space ? t[0] / M.n : 0.0;
             M.st->mu_t = mt;
@@ -615,7 +634,6 @@ __global__ __launch_bounds__(FB_NT) void k_fb_true_fin(const FbMember* __restric
     }
 }
 
-// continue from the true residual: r -= mean (null space); partials of r.r
 // x -> the caller's solution vector, all members in one launch
 __global__ __launch_bounds__(FB_NT) void k_fb_out(const FbMember* __restrict__ Ms, const int2* __restrict__ work,
                                                   int nwork) {
@@ -628,6 +646,7 @@ __global__ __launch_bounds__(FB_NT) void k_fb_out(const FbMember* __restrict__ M
     }
 }
 
+// continue from the true residual: r -= mean (null space); partials of r.r
 __global__ __launch_bounds__(FB_NT) void k_fb_restart(const FbMember* __restrict__ Ms, const int2* __restrict__ work,
                                                       int nwork) {
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
@@ -641,6 +660,7 @@ __global__ __launch_bounds__(FB_NT) void k_fb_restart(const FbMember* __restrict
         for (int i = i0 + threadIdx.x; i < i1; i += FB_NT) {
             const double ri = M.r[i] - mt;
             M.r[i] = ri;
+            put_packed(M, i, ri);
             rr += ri * ri;
         }
         const double v[1] = {rr};
@@ -675,6 +695,7 @@ struct FdmOp {
     struct Prec {
         std::shared_ptr<VMats> mx, my;  // Vx, Vy (shared)
         DevBuf t1, t2;
+        DevBuf rp;  // the residual in T, padded nP1 x nP0 (zeros outside nU1 x nU0): pass 1's Bt
         bool ready = false;
     } prec[3];
     std::shared_ptr<const se_fdm::Fdm1D> ex1, ey1;
@@ -852,9 +873,12 @@ int fdm_prepare(se_op* op, int prec) {
         const int ntz = (F->nP1 / 64) * (F->nP0 / 64);
         rc = F->z.alloc(sizeof(double) * n);
         if (!rc) rc = F->pv.alloc(sizeof(double) * nch);
-        if (!rc) rc = F->pa.alloc(sizeof(double) * nch);
+        // p.q and true-residual partials: per chunk, or per apply tile when
+        // the applies form them
+        const int nap = std::max(nch, line_tiles(m->N, m->nEls[0], m->nEls[1]));
+        if (!rc) rc = F->pa.alloc(sizeof(double) * nap);
         if (!rc) rc = F->ps.alloc(sizeof(double) * nch);
-        if (!rc) rc = F->pt.alloc(sizeof(double) * 2 * nch);
+        if (!rc) rc = F->pt.alloc(sizeof(double) * 2 * nap);
         if (!rc) rc = F->pz.alloc(sizeof(double) * 3 * ntz);
         if (!rc) rc = F->st.alloc(sizeof(CgState));
         if (rc) return rc;
@@ -868,7 +892,11 @@ int fdm_prepare(se_op* op, int prec) {
         if (!rc) rc = vmats_of(F->ey1, F->nP1, prec, dev, P.my);
         if (!rc) rc = P.t1.alloc(elem_size(prec) * (size_t)F->nP0 * F->nP1);
         if (!rc) rc = P.t2.alloc(elem_size(prec) * (size_t)F->nP0 * F->nP1);
+        if (!rc) rc = P.rp.alloc(elem_size(prec) * (size_t)F->nP0 * F->nP1);
         if (rc) return rc;
+        // the padding is never written again: the CG kernels and the pack
+        // store the unknown grid's nU1 x nU0 values only
+        SE_HIP(hipMemset(P.rp.ptr, 0, elem_size(prec) * (size_t)F->nP0 * F->nP1));
         P.ready = true;
     }
     return SE_OK;
@@ -883,10 +911,10 @@ void fdm_tasks(se_op* op, int prec, const double* r, double* z, double* part, Ge
         t[k] = GemmTask{};
         t[k].dtol = F->dtol;
     }
-    // pass 1: T1t[k'][J] = sum_I Vx[I][k'] R[J][I]   (A = Vx^T, Bt = R packed into t2)
+    // pass 1: T1t[k'][J] = sum_I Vx[I][k'] R[J][I]   (A = Vx^T, Bt = R packed into rp)
     t[0].A = P.mx->vT.ptr;
     t[0].lda = nP0;
-    t[0].Bt = P.t2.ptr;
+    t[0].Bt = P.rp.ptr;
     t[0].ldb = nP0;
     t[0].r = r;
     t[0].arows = F->nU1;  // the packed residual's true extent
@@ -958,18 +986,20 @@ bool fdm_tasks_valid(const GemmTask (&t)[4], int prec) {
 int fdm_tiles(const se_op* op) { return (op->fdm->nP0 / FDM_BT) * (op->fdm->nP1 / FDM_BT); }
 
 template <typename T>
-void launch_gemm_t(const GemmTask* tasks, int pass, const int2* work, int nwork, hipStream_t s, int skip) {
+void launch_gemm_t(const GemmTask* tasks, int pass, const int2* work, int nwork, hipStream_t s, int skip, bool pack) {
     const dim3 grid(nwork), block(256);
-    if (pass == 0) hipLaunchKernelGGL((k_fdm_pack<T>), grid, block, 0, s, tasks, work, nwork, skip);
+    if (pass == 0 && pack) hipLaunchKernelGGL((k_fdm_pack<T>), grid, block, 0, s, tasks, work, nwork, skip);
     hipLaunchKernelGGL((k_fdm_gemm<T>), grid, block, 0, s, tasks, pass, work, nwork, skip);
 }
 
-// pass `pass` (with pass 1's residual pack) of the listed items
-int launch_gemm(int prec, const GemmTask* tasks, int pass, const int2* work, int nwork, hipStream_t s, int skip = 0) {
+// pass `pass` of the listed items; pass 1 first packs the residual when
+// `pack` (the CG kernels store it packed as they write r, so a solve does not)
+int launch_gemm(int prec, const GemmTask* tasks, int pass, const int2* work, int nwork, hipStream_t s, int skip,
+                bool pack) {
     if (nwork <= 0) return SE_OK;
-    if (prec == FDM_F64) launch_gemm_t<double>(tasks, pass, work, nwork, s, skip);
-    else if (prec == FDM_F32) launch_gemm_t<float>(tasks, pass, work, nwork, s, skip);
-    else launch_gemm_t<__bf16>(tasks, pass, work, nwork, s, skip);
+    if (prec == FDM_F64) launch_gemm_t<double>(tasks, pass, work, nwork, s, skip, pack);
+    else if (prec == FDM_F32) launch_gemm_t<float>(tasks, pass, work, nwork, s, skip, pack);
+    else launch_gemm_t<__bf16>(tasks, pass, work, nwork, s, skip, pack);
     return SE_OK;
 }
 
@@ -1038,6 +1068,11 @@ struct FdmPlan {
     // these members and this precision, null-space flag and tolerance; a
     // later solve with the same ones only patches f and x
     bool desc_ok = false;
+    // a member has an SVV term: it is added after the batched apply, so that
+    // member's p.q and true residual come from separate kernels (k_fb_dot,
+    // k_fb_true); every other member's come from the applies' epilogues
+    // (AP_DOT, AP_TRUE), a member's way the same alone or in any batch
+    bool any_svv = false;
     int desc_null = -1;
     double desc_rtol = 0.0;
 
@@ -1048,17 +1083,24 @@ struct FdmPlan {
 
 namespace {
 
-int launch_apply_batch(int N, const GLL& g, const ApplyMember* A, const int2* work, int nw, hipStream_t s) {
+template <int MODE>
+int launch_apply_batch_m(int N, const ApplyMember* A, const int2* work, int nw, hipStream_t s) {
     switch (N) {
-#define SE_CASE(NN)                                                                                             \
-    case NN:                                                                                                    \
-        hipLaunchKernelGGL((k_apply_lines_batch<NN>), dim3(nw), dim3(256), 0, s, A, work, nw);                                                                                 \
+#define SE_CASE(NN)                                                                               \
+    case NN:                                                                                      \
+        hipLaunchKernelGGL((k_apply_lines_batch<NN, MODE>), dim3(nw), dim3(256), 0, s, A, work, nw); \
         break;
         SE_CASE(2) SE_CASE(3) SE_CASE(4) SE_CASE(5) SE_CASE(6) SE_CASE(7) SE_CASE(8)
 #undef SE_CASE
         default: return fail(SE_ERR_UNSUPPORTED, "FDM solve built for orders 2..8");
     }
     return SE_OK;
+}
+
+int launch_apply_batch(int N, int mode, const ApplyMember* A, const int2* work, int nw, hipStream_t s) {
+    if (mode == AP_DOT) return launch_apply_batch_m<AP_DOT>(N, A, work, nw, s);
+    if (mode == AP_TRUE) return launch_apply_batch_m<AP_TRUE>(N, A, work, nw, s);
+    return launch_apply_batch_m<AP_PLAIN>(N, A, work, nw, s);
 }
 
 // Work lists XCD by XCD.  The dispatcher hands workgroup b to XCD b mod 8
@@ -1115,16 +1157,18 @@ void xcd_interleave(const std::vector<int>& ids, const std::vector<int>& tiles, 
         for (int k = 0; k < kXcds; k++) out.push_back(j < items[k].size() ? items[k][j] : make_int2(-1, -1));
 }
 
-// q = sign A p (or sign A x) of the members of list set `set` (whose lists
-// plan_lists built)
+// q = sign A p of the members of list set `set` (whose lists plan_lists
+// built), with the partials of p.q when fused; from x: q = sign A x, or when
+// fused the true residual r = b - sign A x with its partials
 int plan_apply(FdmPlan* pl, int set, bool from_x, hipStream_t s) {
     const FdmPlan::Lists& L = *pl->sets[set].cur;
+    const int mode = from_x ? AP_TRUE : AP_DOT;
     for (int k = 0; k < (int)pl->orders.size(); k++) {
         const FdmPlan::Order& o = pl->orders[k];
         const int nw = L.n_apply[k];
         if (o.members.empty() || nw == 0) continue;
         const ApplyMember* A = from_x ? o.d_ax.as<ApplyMember>() : o.d_ap.as<ApplyMember>();
-        int rc = launch_apply_batch(o.N, pl->ops[o.members[0]]->mesh->gll, A, L.apply[k], nw, s);
+        int rc = launch_apply_batch(o.N, mode, A, L.apply[k], nw, s);
         if (rc) return rc;
     }
     // members with an SVV term: add it to the batched apply's result
@@ -1247,7 +1291,7 @@ int grid_of(size_t items, const se_ctx* c) {
 // one pass of the transforms of the listed members
 int plan_pass(FdmPlan* pl, int set, int pass, hipStream_t s, int skip = 0) {
     const FdmPlan::Lists& L = *pl->sets[set].cur;
-    return launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, L.gwork, L.n_gwork, s, skip);
+    return launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, L.gwork, L.n_gwork, s, skip, false);
 }
 
 // z = FDM(r) of the listed members: four passes
@@ -1340,6 +1384,8 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
         }
         pl->h_members.assign(count, FbMember{});
         std::vector<GemmTask> tasks(4 * (size_t)count);
+        pl->any_svv = false;
+        for (se_op* op : pl->ops) pl->any_svv = pl->any_svv || op->svv_mN != 0;
         for (int i = 0; i < count; i++) {
             se_op* op = pl->ops[i];
             FdmOp* F = op->fdm;
@@ -1357,8 +1403,15 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
             M.pt = F->pt.as<double>();
             M.ps = F->ps.as<double>();
             M.xo = d_x[i];
+            M.rp = F->prec[prec].rp.ptr;
+            M.nU0 = F->nU0;
+            M.nP0 = F->nP0;
+            M.rprec = prec;
             M.n = op->mesh->n;
+            SE_CHECK(M.n == F->nU0 * F->nU1, SE_ERR_STATE, "FDM: the unknowns are not the nU1 x nU0 grid");
             M.nch = (std::max(M.n, 1) + FB_CH - 1) / FB_CH;
+            M.fused = op->svv_mN == 0;
+            M.nap = M.fused ? line_tiles(op->mesh->N, op->mesh->nEls[0], op->mesh->nEls[1]) : M.nch;
             M.ntz = fdm_tiles(op);
             M.null_space = null;
             M.sgn = (double)op->sign;
@@ -1387,12 +1440,22 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
             std::vector<ApplyMember> ap, ax;
             for (int i : od.members) {
                 se_op* op = pl->ops[i];
+                const FbMember& M = pl->h_members[i];
                 ApplyMember A;
                 A.P = op_params(op, (double)op->sign);
                 A.in = op->p.as<double>();
                 A.out = op->q.as<double>();
+                if (M.fused) {  // + partials of p.q
+                    A.part = M.pa;
+                    A.np = M.nap;
+                }
                 ap.push_back(A);
                 A.in = op->x.as<double>();
+                if (M.fused) {  // r = b - A x, partials of sum t, sum t^2
+                    A.out = op->r.as<double>();
+                    A.b = op->b.as<double>();
+                    A.part = M.pt;
+                }
                 ax.push_back(A);
             }
             rc = od.d_ap.alloc(sizeof(ApplyMember) * ap.size());
@@ -1437,7 +1500,7 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
         int e = plan_lists(pl, set, flags, s);
         if (!e) e = plan_apply(pl, set, true, s);
         if (e) return e;
-        vlaunch(set, k_fb_true);
+        if (pl->any_svv) vlaunch(set, k_fb_true);
         const FdmPlan::Lists& L = *pl->sets[set].cur;
         if (L.n_list > 0) hipLaunchKernelGGL(k_fb_true_fin, dim3(L.n_list), dim3(FB_NT), 0, s, Ms, L.list, L.n_list);
         return SE_OK;
@@ -1480,7 +1543,7 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
             rc = plan_lists(pl, IT, on, s);
             if (!rc) rc = plan_apply(pl, IT, false, s);
             if (rc) return rc;
-            vlaunch(IT, k_fb_dot);
+            if (pl->any_svv) vlaunch(IT, k_fb_dot);
             vlaunch(IT, k_fb_update);
             rc = plan_fdm(pl, IT, s, 1);
             if (rc) return rc;
@@ -1678,7 +1741,7 @@ extern "C" int se_helm_apply_fdm(se_op* op, int precision, const double* r, doub
     if (rc) return rc;
     SE_HIP(hipMemcpy(d.ptr, t, sizeof(t), hipMemcpyHostToDevice));
     for (int pass = 0; pass < 4; pass++) {
-        rc = launch_gemm(prec, d.as<GemmTask>(), pass, nullptr, fdm_tiles(op), s);
+        rc = launch_gemm(prec, d.as<GemmTask>(), pass, nullptr, fdm_tiles(op), s, 0, true);
         if (rc) return rc;
     }
     SE_HIP(hipGetLastError());
