@@ -78,9 +78,21 @@ struct se_server {
     std::map<MeshKey, se_mesh*> meshes;
     std::multimap<PoolKey, se_qg2*> pool;
     size_t used = 0;  // bytes of admitted requests and pooled instances
-    // the inversions' solvers, kept for the server's lifetime: one FDM plan
-    // re-membered for every pass, or (Jacobi) one batch per member set
-    FdmPlan* plan = nullptr;
+    // the inversions' solvers, kept across ticks: while the admitted set does
+    // not change, every tick's passes find their FDM plan (descriptors and
+    // work lists on the device) or Jacobi batch (graphs and launch shapes
+    // tuned) as the last tick left it.  A pass with a new member set takes
+    // over the least recently used of kPlans FDM plans (re-membered: its
+    // buffers are reused, nothing is allocated unless the set is larger than
+    // any before), or a new Jacobi batch (at most 8)
+    static constexpr int kPlans = 4;
+    struct PlanSlot {
+        std::vector<se_op*> ops;
+        FdmPlan* plan = nullptr;
+        unsigned long long used = 0;
+    };
+    PlanSlot plans[kPlans];
+    unsigned long long plan_clock = 0;
     std::map<std::vector<se_op*>, se_batch*> batches;
     se_server_stats stats{};
 
@@ -108,13 +120,30 @@ int se_server::batch_solve(const std::vector<se_op*>& ops, const se_solve_opts& 
         x[i] = ops[i]->x.as<double>();
     }
     if (solve_pc(o) == SE_PC_FDM) {
-        int rc = plan ? fdm_plan_set_members(plan, (int)ops.size(), ops.data())
-                      : fdm_plan_create(ctx, (int)ops.size(), ops.data(), &plan);
-        if (rc) return rc;
-        return fdm_plan_solve(plan, f.data(), x.data(), o, inf.data());
+        PlanSlot* slot = nullptr;
+        for (PlanSlot& p : plans)
+            if (p.plan && p.ops == ops) slot = &p;
+        if (!slot) {  // an empty slot, else the least recently used
+            for (PlanSlot& p : plans) {
+                if (!p.plan) {
+                    slot = &p;
+                    break;
+                }
+                if (!slot || p.used < slot->used) slot = &p;
+            }
+            int rc = slot->plan ? fdm_plan_set_members(slot->plan, (int)ops.size(), ops.data())
+                                : fdm_plan_create(ctx, (int)ops.size(), ops.data(), &slot->plan);
+            if (rc) {
+                if (slot->plan) fdm_plan_destroy(slot->plan);
+                *slot = PlanSlot{};
+                return rc;
+            }
+            slot->ops = ops;
+        }
+        slot->used = ++plan_clock;
+        return fdm_plan_solve(slot->plan, f.data(), x.data(), o, inf.data());
     }
-    // Jacobi inner CG: a batch per member set, kept (graphs and launch
-    // shapes stay tuned while the admitted set does not change)
+    // Jacobi inner CG
     auto it = batches.find(ops);
     if (it == batches.end()) {
         if (batches.size() >= 8) {  // bound the cache: drop the oldest sets
@@ -217,9 +246,13 @@ int se_server::admit(std::vector<std::shared_ptr<Request>>& active) {
 
 // forget every solver that holds an operator of q (before q is destroyed)
 void se_server::drop_solvers(se_qg2* q) {
-    if (plan) {
-        fdm_plan_destroy(plan);
-        plan = nullptr;
+    for (PlanSlot& p : plans) {
+        bool uses = false;
+        for (se_op* op : p.ops) uses = uses || op == qg2_pass_op(q, 0) || op == qg2_pass_op(q, 1);
+        if (uses && p.plan) {
+            fdm_plan_destroy(p.plan);
+            p = PlanSlot{};
+        }
     }
     for (auto it = batches.begin(); it != batches.end();) {
         bool uses = false;
@@ -443,7 +476,8 @@ extern "C" int se_server_destroy(se_server* sv) {
     sv->cv_work.notify_all();
     if (sv->th.joinable()) sv->th.join();
     (void)hipSetDevice(sv->ctx->device);
-    if (sv->plan) fdm_plan_destroy(sv->plan);
+    for (auto& p : sv->plans)
+        if (p.plan) fdm_plan_destroy(p.plan);
     for (auto& kv : sv->batches) se_batch_destroy(kv.second);
     for (auto& p : sv->pool) se_qg2_destroy(p.second);
     for (auto& m : sv->meshes) se_mesh_destroy(m.second);
