@@ -150,21 +150,31 @@ SE_POW_FN double exp_hi_lo(double x, double xtail, uint64_t sign_bias) {
     return fma(scale, tmp, scale);
 }
 
-// pow(x, k) for an integer 0 <= k < 2^53
-SE_POW_FN double pow_int(double x, int k) {
+// x's part of pow(x, k) for integers k -- the special cases, the sign and
+// log(x) = hi + lo -- so that several powers of one x share a single log;
+// pow_int_at(pow_base(x), k) performs exactly pow_int(x, k)'s operations
+struct PowBase {
+    double x, hi, lo;
+    uint64_t ix;
+    int special, neg;
+};
+
+SE_POW_FN PowBase pow_base(double x) {
 #pragma clang fp contract(off)
-    if (k == 0) return 1.0;
+    PowBase b;
+    b.x = x;
+    b.hi = b.lo = 0.0;
+    b.special = b.neg = 0;
     uint64_t ix = bits(x);
-    uint64_t sign_bias = 0;
     const uint32_t topx = (uint32_t)(ix >> 52);
     if (topx - 0x001u >= 0x7ffu - 0x001u) {
         if (2 * ix - 1 >= 2 * 0x7ff0000000000000ULL - 1) {  // x = +-0, +-inf or NaN
-            double x2 = x * x;
-            if ((ix >> 63) && (k & 1)) x2 = -x2;
-            return x2;
+            b.special = 1;
+            b.ix = ix;
+            return b;
         }
         if (ix >> 63) {  // finite x < 0 and an integer exponent
-            if (k & 1) sign_bias = 0x800ULL << 7;
+            b.neg = 1;
             ix &= 0x7fffffffffffffffULL;
         }
         if (((ix >> 52) & 0x7ff) == 0) {  // subnormal x: normalise
@@ -173,12 +183,30 @@ SE_POW_FN double pow_int(double x, int k) {
             ix -= 52ULL << 52;
         }
     }
-    double lo;
-    const double hi = log_hi_lo(ix, &lo);
+    b.ix = ix;
+    b.hi = log_hi_lo(ix, &b.lo);
+    return b;
+}
+
+SE_POW_FN double pow_int_at(const PowBase& b, int k) {
+#pragma clang fp contract(off)
+    if (k == 0) return 1.0;
+    if (b.special) {
+        double x2 = b.x * b.x;
+        if ((b.ix >> 63) && (k & 1)) x2 = -x2;
+        return x2;
+    }
+    const uint64_t sign_bias = (b.neg && (k & 1)) ? (0x800ULL << 7) : 0;
     const double y = (double)k;
-    const double ehi = y * hi;
-    const double elo = fma(y, lo, fma(y, hi, -ehi));
+    const double ehi = y * b.hi;
+    const double elo = fma(y, b.lo, fma(y, b.hi, -ehi));
     return exp_hi_lo(ehi, elo, sign_bias);
+}
+
+// pow(x, k) for an integer 0 <= k < 2^53
+SE_POW_FN double pow_int(double x, int k) {
+    if (k == 0) return 1.0;
+    return pow_int_at(pow_base(x), k);
 }
 
 }  // namespace se_pow

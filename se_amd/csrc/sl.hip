@@ -79,20 +79,35 @@ __device__ __forceinline__ void wrap(const SLGeom& g, double& cx, double& cy) {
     if (cy > g.max1) cy = g.per1 ? g.min1 - g.max1 + cy : g.max1;
 }
 
-// Jacobi::EvalDeriv (Jacobi.cpp:58-66) of P_N
+// The kernels take the order as a template argument NN (the launches
+// instantiate N = 2..8) or, with NN = 0, from the geometry (any N up to 12):
+// the same operations either way, with fixed trip counts and arrays of N + 1
+// when NN is known.
+template <int NN> struct Ord {
+    static constexpr int MAX = NN ? NN : 12;  // array bound
+};
+template <int NN>
+__device__ __forceinline__ int order(const SLGeom& g) {
+    return NN ? NN : g.N;
+}
+
+// Jacobi::EvalDeriv (Jacobi.cpp:58-66) of P_N; its powers of x share one
+// log (pow_int_at: each power the same bits as pow(x, i - 1))
+template <int NN>
 __device__ __forceinline__ double pn_deriv(const SLGeom& g, double x) {
 #pragma clang fp contract(off)
+    const se_pow::PowBase b = se_pow::pow_base(x);
     double dp = 0.0;
-    for (int i = g.N; i >= 1; i -= 2) dp += g.pn[i] * i * se_pow::pow_int(x, i - 1);
+    for (int i = order<NN>(g); i >= 1; i -= 2) dp += g.pn[i] * i * se_pow::pow_int_at(b, i - 1);
     return dp;
 }
 
 // 1-D GLL cardinal functions at local coordinate x (Legendre::IsRoot and the
 // per-dimension factor of ShapeFuncsAtCoord, Legendre.cpp:145-169,291-305)
-template <int MAXN>
-__device__ __forceinline__ void cardinal(const SLGeom& g, double x, double (&L)[MAXN + 1]) {
+template <int NN>
+__device__ __forceinline__ void cardinal(const SLGeom& g, double x, double (&L)[Ord<NN>::MAX + 1]) {
 #pragma clang fp contract(off)
-    const int N = g.N;
+    const int N = order<NN>(g);
     int root = -1;
     for (int k = 0; k <= N; k++)
         if (fabs(x - g.xk[k]) < 1.0e-8) {
@@ -103,18 +118,18 @@ __device__ __forceinline__ void cardinal(const SLGeom& g, double x, double (&L)[
         for (int j = 0; j <= N; j++) L[j] = (j == root) ? 1.0 : 0.0;
         return;
     }
-    const double dpx = pn_deriv(g, x);
+    const double dpx = pn_deriv<NN>(g, x);
     for (int j = 0; j <= N; j++)
         L[j] = -((1.0 - x * x) / (N * (N + 1) * g.pnx[j] * (x - g.xk[j]))) * dpx;
 }
 
 // Field::InterpGlobal (Field.cpp:57-76) through Mesh::GlobalToLocal
 // (Mesh.cpp:163-177); vals node-major with ND dofs
-template <int MAXN, int ND>
+template <int NN, int ND>
 __device__ __forceinline__ void interp_global(const SLGeom& g, const double* __restrict__ vals, int nd, double cx,
                                               double cy, double (&out)[ND]) {
 #pragma clang fp contract(off)
-    const int N = g.N;
+    const int N = order<NN>(g);
     int el = get_el(g, cx, cy);
     int ex = el % g.nEls0, ey = el / g.nEls0;
     double lx = (cx - g.min0 - ex * g.dx0) * (2.0 / g.dx0) - 1.0;
@@ -123,11 +138,12 @@ __device__ __forceinline__ void interp_global(const SLGeom& g, const double* __r
     if (lx > +1.0) lx = +1;
     if (ly < -1.0) ly = -1;
     if (ly > +1.0) ly = +1;
-    double Lx[MAXN + 1], Ly[MAXN + 1];
-    cardinal<MAXN>(g, lx, Lx);
-    cardinal<MAXN>(g, ly, Ly);
+    double Lx[Ord<NN>::MAX + 1], Ly[Ord<NN>::MAX + 1];
+    cardinal<NN>(g, lx, Lx);
+    cardinal<NN>(g, ly, Ly);
     for (int d = 0; d < ND; d++) out[d] = 0.0;
     const int n0 = N * ey * g.nv0 + N * ex;
+#pragma unroll 1
     for (int jj = 0; jj <= N; jj++)
         for (int ii = 0; ii <= N; ii++) {
             double Ni = Lx[ii] * Ly[jj];
@@ -184,15 +200,15 @@ __device__ __forceinline__ void interp_bicubic(const SLGeom& g, const double* __
     interp_lagrange<ND>(cy, py, rows, out);
 }
 
-template <int MAXN, int ND>
+template <int NN, int ND>
 __device__ __forceinline__ void interp_any(const SLGeom& g, const double* __restrict__ vals, int nd, double cx,
                                            double cy, double (&out)[ND]) {
     if (g.cubic) interp_bicubic<ND>(g, vals, nd, cx, cy, out);
-    else interp_global<MAXN, ND>(g, vals, nd, cx, cy, out);
+    else interp_global<NN, ND>(g, vals, nd, cx, cy, out);
 }
 
 // Advector::FirstOrder + IntegrateRK2 (Advector.cpp:136-148,181-201)
-template <int MAXN>
+template <int NN>
 __global__ __launch_bounds__(256) void k_sl_first(SLGeom g, const double* __restrict__ q, int nd,
                                                    const double* __restrict__ vel, double dt, double* __restrict__ qsl,
                                                    int32_t* __restrict__ dep) {
@@ -203,17 +219,17 @@ __global__ __launch_bounds__(256) void k_sl_first(SLGeom g, const double* __rest
     // velocity at the origin: the origin is a GLL node, so InterpGlobal
     // returns the nodal value exactly (IsRoot selects a Kronecker delta)
     double vel0[2];
-    interp_global<MAXN, 2>(g, vel, 2, ox, oy, vel0);
+    interp_global<NN, 2>(g, vel, 2, ox, oy, vel0);
     double hx = ox - 0.5 * dt * vel0[0];
     double hy = oy - 0.5 * dt * vel0[1];
     wrap(g, hx, hy);
     double vh[2];
-    interp_global<MAXN, 2>(g, vel, 2, hx, hy, vh);
+    interp_global<NN, 2>(g, vel, 2, hx, hy, vh);
     double cx = ox - dt * vh[0];
     double cy = oy - dt * vh[1];
     wrap(g, cx, cy);
     double val[3];
-    interp_any<MAXN, 3>(g, q, nd, cx, cy, val);
+    interp_any<NN, 3>(g, q, nd, cx, cy, val);
     for (int d = 0; d < nd; d++) qsl[(size_t)v * nd + d] = val[d];
     if (dep) dep[v] = get_el(g, cx, cy);
 }
@@ -231,7 +247,7 @@ __global__ void k_extrap(int n2, const double* __restrict__ vel, const double* _
 }
 
 // Advector::SecondOrder (Advector.cpp:203-246)
-template <int MAXN>
+template <int NN>
 __global__ __launch_bounds__(256) void k_sl_second(SLGeom g, const double* __restrict__ q, const double* __restrict__ qp,
                                                     int nd, const double* __restrict__ vel,
                                                     const double* __restrict__ vp1, const double* __restrict__ vph,
@@ -242,22 +258,22 @@ __global__ __launch_bounds__(256) void k_sl_second(SLGeom g, const double* __res
     if (v >= g.nv0 * g.nv1) return;
     const double ox = g.X[v % g.nv0], oy = g.Y[v / g.nv0];
     double w[2];
-    interp_global<MAXN, 2>(g, vp1, 2, ox, oy, w);
+    interp_global<NN, 2>(g, vp1, 2, ox, oy, w);
     double hx = ox - 0.5 * dt * w[0], hy = oy - 0.5 * dt * w[1];
     wrap(g, hx, hy);
-    interp_global<MAXN, 2>(g, vph, 2, hx, hy, w);
+    interp_global<NN, 2>(g, vph, 2, hx, hy, w);
     double x1 = ox - dt * w[0], y1 = oy - dt * w[1];
     wrap(g, x1, y1);
-    interp_global<MAXN, 2>(g, vel, 2, x1, y1, w);
+    interp_global<NN, 2>(g, vel, 2, x1, y1, w);
     hx = x1 - 0.5 * dt * w[0];
     hy = y1 - 0.5 * dt * w[1];
     wrap(g, hx, hy);
-    interp_global<MAXN, 2>(g, vmh, 2, hx, hy, w);
+    interp_global<NN, 2>(g, vmh, 2, hx, hy, w);
     double x2 = x1 - dt * w[0], y2 = y1 - dt * w[1];
     wrap(g, x2, y2);
     double p1[3], p2[3];
-    interp_any<MAXN, 3>(g, q, nd, x1, y1, p1);
-    interp_any<MAXN, 3>(g, qp, nd, x2, y2, p2);
+    interp_any<NN, 3>(g, q, nd, x1, y1, p1);
+    interp_any<NN, 3>(g, qp, nd, x2, y2, p2);
     for (int d = 0; d < nd; d++) qsl[(size_t)v * nd + d] = 2.0 * p1[d] - 0.5 * p2[d];
 }
 
@@ -274,7 +290,7 @@ __global__ void k_periodic_copy(int nv0, int nv1, int per0, int per1, int nd, do
 
 // Field::InterpGlobal (mode GLL) or Advector::Interpolate (mode BICUBIC) at
 // arbitrary points, one thread per point; xy: npts x 2, out: npts x nd
-template <int MAXN>
+template <int NN>
 __global__ __launch_bounds__(256) void k_interp_points(SLGeom g, const double* __restrict__ vals, int nd, int npts,
                                                        const double* __restrict__ xy, int cubic,
                                                        double* __restrict__ out) {
@@ -282,7 +298,7 @@ __global__ __launch_bounds__(256) void k_interp_points(SLGeom g, const double* _
     if (p >= npts) return;
     double v[3];
     if (cubic) interp_bicubic<3>(g, vals, nd, xy[2 * (size_t)p], xy[2 * (size_t)p + 1], v);
-    else interp_global<MAXN, 3>(g, vals, nd, xy[2 * (size_t)p], xy[2 * (size_t)p + 1], v);
+    else interp_global<NN, 3>(g, vals, nd, xy[2 * (size_t)p], xy[2 * (size_t)p + 1], v);
     for (int d = 0; d < nd; d++) out[(size_t)p * nd + d] = v[d];
 }
 
@@ -290,12 +306,12 @@ __global__ __launch_bounds__(256) void k_interp_points(SLGeom g, const double* _
 // Mesh::ShapeFuncDerivsAtCoord (Mesh.cpp:252-285).  At a vertex both local
 // coordinates are GLL roots, so the local derivative rows are columns of D
 // and the Jacobian is the tabulated one of that GLL point.
-template <int MAXN>
+template <int NN>
 __global__ __launch_bounds__(256) void k_genvel(SLGeom g, const double* __restrict__ psi, double* __restrict__ vel) {
 #pragma clang fp contract(off)
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= g.nv0 * g.nv1) return;
-    const int N = g.N, n1 = N + 1;
+    const int N = order<NN>(g), n1 = N + 1;
     const double ox = g.X[v % g.nv0], oy = g.Y[v / g.nv0];
     int el = get_el(g, ox, oy);
     int ex = el % g.nEls0, ey = el / g.nEls0;
@@ -320,21 +336,26 @@ __global__ __launch_bounds__(256) void k_genvel(SLGeom g, const double* __restri
     double i00 = j11 / det, i11 = j00 / det, i01 = -j01 / det, i10 = -j10 / det;
     double g0 = 0.0, g1 = 0.0;
     const int n0 = N * ey * g.nv0 + N * ex;
-    for (int jj = 0; jj <= N; jj++)
-        for (int ii = 0; ii <= N; ii++) {
-            double a0 = (jj == yi) ? g.D[xi * n1 + ii] : 0.0;  // GNx * Ny
-            double a1 = (ii == xi) ? g.D[yi * n1 + jj] : 0.0;  // Nx * GNy
-            if (a0 == 0.0 && a1 == 0.0) continue;
-            double t0 = 0.0;
-            t0 += a0 * i00;
-            t0 += a1 * i01;
-            double t1 = 0.0;
-            t1 += a0 * i10;
-            t1 += a1 * i11;
-            double pv = psi[n0 + jj * g.nv0 + ii];
-            g0 += t0 * pv;
-            g1 += t1 * pv;
-        }
+    // the element's nodes (jj, ii) in row-major order; only row yi and
+    // column xi can carry a non-zero shape-function derivative, so the walk
+    // visits those in the same order (the others were skipped anyway)
+    auto node = [&](int jj, int ii) {
+        double a0 = (jj == yi) ? g.D[xi * n1 + ii] : 0.0;  // GNx * Ny
+        double a1 = (ii == xi) ? g.D[yi * n1 + jj] : 0.0;  // Nx * GNy
+        if (a0 == 0.0 && a1 == 0.0) return;
+        double t0 = 0.0;
+        t0 += a0 * i00;
+        t0 += a1 * i01;
+        double t1 = 0.0;
+        t1 += a0 * i10;
+        t1 += a1 * i11;
+        double pv = psi[n0 + jj * g.nv0 + ii];
+        g0 += t0 * pv;
+        g1 += t1 * pv;
+    };
+    for (int jj = 0; jj < yi; jj++) node(jj, xi);
+    for (int ii = 0; ii <= N; ii++) node(yi, ii);
+    for (int jj = yi + 1; jj <= N; jj++) node(jj, xi);
     vel[2 * (size_t)v] = -1.0 * g1;
     vel[2 * (size_t)v + 1] = +1.0 * g0;
 }
@@ -388,6 +409,20 @@ SLGeom sl_geom(const se_mesh* m) {
 
 int blocks_for(int n) { return (n + 255) / 256; }
 
+// KERNEL<N> for the orders 2..8, KERNEL<0> (the order read from the
+// geometry) for the others; 256 threads per workgroup
+#define SE_SL_LAUNCH(KERNEL, N, GRID, ...)                                          \
+    switch (N) {                                                                    \
+        case 2: hipLaunchKernelGGL(KERNEL<2>, GRID, dim3(256), 0, __VA_ARGS__); break; \
+        case 3: hipLaunchKernelGGL(KERNEL<3>, GRID, dim3(256), 0, __VA_ARGS__); break; \
+        case 4: hipLaunchKernelGGL(KERNEL<4>, GRID, dim3(256), 0, __VA_ARGS__); break; \
+        case 5: hipLaunchKernelGGL(KERNEL<5>, GRID, dim3(256), 0, __VA_ARGS__); break; \
+        case 6: hipLaunchKernelGGL(KERNEL<6>, GRID, dim3(256), 0, __VA_ARGS__); break; \
+        case 7: hipLaunchKernelGGL(KERNEL<7>, GRID, dim3(256), 0, __VA_ARGS__); break; \
+        case 8: hipLaunchKernelGGL(KERNEL<8>, GRID, dim3(256), 0, __VA_ARGS__); break; \
+        default: hipLaunchKernelGGL(KERNEL<0>, GRID, dim3(256), 0, __VA_ARGS__); break; \
+    }
+
 struct SLScratch {
     DevBuf a, b, c, d, e, f, g, h;
     ~SLScratch() {
@@ -407,7 +442,7 @@ extern "C" int se_sl_advect_dev(se_mesh* m, const double* d_q, int nd, const dou
              "bicubic advection (N < 3) needs at least 4 nodes per direction");
     SLGeom g = sl_geom(m);
     hipStream_t s = m->ctx->stream;
-    hipLaunchKernelGGL(k_sl_first<12>, dim3(blocks_for(m->nV)), dim3(256), 0, s, g, d_q, nd, d_vel, dt, d_qsl, d_dep);
+    SE_SL_LAUNCH(k_sl_first, m->N, dim3(blocks_for(m->nV)), s, g, d_q, nd, d_vel, dt, d_qsl, d_dep);
     if (m->periodic[0] || m->periodic[1])
         hipLaunchKernelGGL(k_periodic_copy, dim3(blocks_for(m->nV)), dim3(256), 0, s, m->nVerts[0], m->nVerts[1],
                            m->periodic[0], m->periodic[1], nd, d_qsl);
@@ -437,8 +472,7 @@ int sl_second(se_mesh* m, const double* d_q, const double* d_qp, int nd, const d
     double* vph = vp1 + n2;
     double* vmh = vph + n2;
     hipLaunchKernelGGL(k_extrap, dim3(blocks_for((int)n2)), dim3(256), 0, s, (int)n2, d_vel, d_velp, vp1, vph, vmh);
-    hipLaunchKernelGGL(k_sl_second<12>, dim3(blocks_for(m->nV)), dim3(256), 0, s, g, d_q, d_qp, nd, d_vel, vp1, vph,
-                       vmh, dt, d_qsl);
+    SE_SL_LAUNCH(k_sl_second, m->N, dim3(blocks_for(m->nV)), s, g, d_q, d_qp, nd, d_vel, vp1, vph, vmh, dt, d_qsl);
     if (m->periodic[0] || m->periodic[1])
         hipLaunchKernelGGL(k_periodic_copy, dim3(blocks_for(m->nV)), dim3(256), 0, s, m->nVerts[0], m->nVerts[1],
                            m->periodic[0], m->periodic[1], nd, d_qsl);
@@ -506,7 +540,7 @@ extern "C" int se_sl_advect2_dev(se_mesh* m, const double* d_q, const double* d_
 extern "C" int se_genvel_dev(se_mesh* m, const double* d_psi, double* d_vel) {
     SE_CHECK(m && d_psi && d_vel, SE_ERR_ARG, "se_genvel_dev: NULL argument");
     SLGeom g = sl_geom(m);
-    hipLaunchKernelGGL(k_genvel<12>, dim3(blocks_for(m->nV)), dim3(256), 0, m->ctx->stream, g, d_psi, d_vel);
+    SE_SL_LAUNCH(k_genvel, m->N, dim3(blocks_for(m->nV)), m->ctx->stream, g, d_psi, d_vel);
     SE_HIP(hipGetLastError());
     return SE_OK;
 }
@@ -629,8 +663,8 @@ extern "C" int se_interp_points_dev(se_mesh* m, const double* d_vals, int nd, in
              "bicubic interpolation needs at least 4 nodes per direction");
     if (npts == 0) return SE_OK;
     SLGeom g = sl_geom(m);
-    hipLaunchKernelGGL(k_interp_points<12>, dim3(blocks_for(npts)), dim3(256), 0, m->ctx->stream, g, d_vals, nd, npts,
-                       d_xy, cubic, d_out);
+    SE_SL_LAUNCH(k_interp_points, m->N, dim3(blocks_for(npts)), m->ctx->stream, g, d_vals, nd, npts, d_xy, cubic,
+                 d_out);
     SE_HIP(hipGetLastError());
     return SE_OK;
 }
