@@ -153,6 +153,40 @@ __device__ __forceinline__ void interp_global(const SLGeom& g, const double* __r
         }
 }
 
+// interp_global of two node-major fields at one point (the weights computed
+// once): each output is summed exactly as interp_global sums it
+template <int NN, int NA, int NB>
+__device__ __forceinline__ void interp_global2(const SLGeom& g, const double* __restrict__ va, int na,
+                                               const double* __restrict__ vb, int nb, double cx, double cy,
+                                               double (&oa)[NA], double (&ob)[NB]) {
+#pragma clang fp contract(off)
+    const int N = order<NN>(g);
+    int el = get_el(g, cx, cy);
+    int ex = el % g.nEls0, ey = el / g.nEls0;
+    double lx = (cx - g.min0 - ex * g.dx0) * (2.0 / g.dx0) - 1.0;
+    double ly = (cy - g.min1 - ey * g.dx1) * (2.0 / g.dx1) - 1.0;
+    if (lx < -1.0) lx = -1;
+    if (lx > +1.0) lx = +1;
+    if (ly < -1.0) ly = -1;
+    if (ly > +1.0) ly = +1;
+    double Lx[Ord<NN>::MAX + 1], Ly[Ord<NN>::MAX + 1];
+    cardinal<NN>(g, lx, Lx);
+    cardinal<NN>(g, ly, Ly);
+    for (int d = 0; d < NA; d++) oa[d] = 0.0;
+    for (int d = 0; d < NB; d++) ob[d] = 0.0;
+    const int n0 = N * ey * g.nv0 + N * ex;
+#pragma unroll 1
+    for (int jj = 0; jj <= N; jj++)
+        for (int ii = 0; ii <= N; ii++) {
+            double Ni = Lx[ii] * Ly[jj];
+            const size_t node = (size_t)(n0 + jj * g.nv0 + ii);
+            for (int d = 0; d < NA; d++)
+                if (d < na) oa[d] += Ni * va[node * na + d];
+            for (int d = 0; d < NB; d++)
+                if (d < nb) ob[d] += Ni * vb[node * nb + d];
+        }
+}
+
 // Advector::InterpLagrange, Advector.cpp:248-275
 template <int ND>
 __device__ __forceinline__ void interp_lagrange(double x, const double* c, double (&vals)[4][ND], double (&psi)[ND]) {
@@ -266,15 +300,21 @@ __global__ __launch_bounds__(256, 4) void k_sl_second(SLGeom g, const double* __
     interp_global<NN, 2>(g, vph, 2, hx, hy, w);
     double x1 = ox - dt * w[0], y1 = oy - dt * w[1];
     wrap(g, x1, y1);
-    interp_global<NN, 2>(g, vel, 2, x1, y1, w);
+    // the velocity and (GLL interpolation) the advected field at (x1, y1)
+    // share one set of weights
+    double p1[3], p2[3];
+    if (g.cubic) {
+        interp_global<NN, 2>(g, vel, 2, x1, y1, w);
+        interp_bicubic<3>(g, q, nd, x1, y1, p1);
+    } else {
+        interp_global2<NN, 2, 3>(g, vel, 2, q, nd, x1, y1, w, p1);
+    }
     hx = x1 - 0.5 * dt * w[0];
     hy = y1 - 0.5 * dt * w[1];
     wrap(g, hx, hy);
     interp_global<NN, 2>(g, vmh, 2, hx, hy, w);
     double x2 = x1 - dt * w[0], y2 = y1 - dt * w[1];
     wrap(g, x2, y2);
-    double p1[3], p2[3];
-    interp_any<NN, 3>(g, q, nd, x1, y1, p1);
     interp_any<NN, 3>(g, qp, nd, x2, y2, p2);
     for (int d = 0; d < nd; d++) qsl[(size_t)v * nd + d] = 2.0 * p1[d] - 0.5 * p2[d];
 }
