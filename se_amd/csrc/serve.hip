@@ -94,10 +94,20 @@ struct se_server {
     PlanSlot plans[kPlans];
     unsigned long long plan_clock = 0;
     std::map<std::vector<se_op*>, se_batch*> batches;
+    // Lanes: the requests' own phases (advection, right-hand sides, updates)
+    // run on kLanes streams beside the context's stream, which runs the
+    // batched inversions; a small mesh fills a few CUs, so requests overlap.
+    // Requests of one mesh share its scratch (the SL velocity fields) and so
+    // one lane; every inversion waits for all lanes and they for it (events).
+    static constexpr int kLanes = 3;  // with the main stream: the process's four hardware queues
+    hipStream_t lanes[kLanes] = {};
+    hipEvent_t lane_ev[kLanes] = {}, main_ev = nullptr;
     se_server_stats stats{};
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
     void run();
+    int lanes_fork();
+    int lanes_join();
     int batch_solve(const std::vector<se_op*>& ops, const se_solve_opts& o, std::vector<se_solve_info>& inf);
     void drop_solvers(se_qg2* q);
     int tick(std::vector<std::shared_ptr<Request>>& active);
@@ -108,6 +118,31 @@ struct se_server {
 namespace {
 
 }  // namespace
+
+// the requests' phases launch on a lane: the context's stream is that lane
+// while this is in scope (the server thread is the context's only user)
+struct LaneScope {
+    se_ctx* c;
+    hipStream_t saved;
+    LaneScope(se_ctx* ctx, hipStream_t s) : c(ctx), saved(ctx->stream) { c->stream = s; }
+    ~LaneScope() { c->stream = saved; }
+};
+
+// the lanes start after everything queued on the main stream so far
+int se_server::lanes_fork() {
+    SE_HIP(hipEventRecord(main_ev, ctx->stream));
+    for (int l = 0; l < kLanes; l++) SE_HIP(hipStreamWaitEvent(lanes[l], main_ev, 0));
+    return SE_OK;
+}
+
+// the main stream continues after everything queued on the lanes so far
+int se_server::lanes_join() {
+    for (int l = 0; l < kLanes; l++) {
+        SE_HIP(hipEventRecord(lane_ev[l], lanes[l]));
+        SE_HIP(hipStreamWaitEvent(ctx->stream, lane_ev[l], 0));
+    }
+    return SE_OK;
+}
 
 // one inversion pass of the admitted requests as one batched solve
 int se_server::batch_solve(const std::vector<se_op*>& ops, const se_solve_opts& o, std::vector<se_solve_info>& inf) {
@@ -294,11 +329,25 @@ int se_server::tick(std::vector<std::shared_ptr<Request>>& active) {
     o.preconditioner = opts.preconditioner;
     const bool one_by_one = opts.precision == SE_PREC_FP64 && solve_pc(o) == SE_PC_JACOBI;
     std::vector<int> failed(active.size(), 0);
+    // lanes: distinct meshes round-robin, a mesh's requests together
+    std::vector<int> lane(active.size());
+    {
+        std::map<se_mesh*, int> of_mesh;
+        for (size_t i = 0; i < active.size(); i++) {
+            auto it = of_mesh.emplace(active[i]->q->mesh, (int)of_mesh.size() % kLanes).first;
+            lane[i] = it->second;
+        }
+    }
+    int rc0 = lanes_fork();
+    if (rc0) return rc0;
     for (size_t i = 0; i < active.size(); i++) {
+        LaneScope on(ctx, lanes[lane[i]]);
         int rc = qg2_phase_advect(active[i]->q, active[i]->dt);
         if (!rc) rc = qg2_phase_rhs(active[i]->q, 0);
         if (rc) failed[i] = rc;
     }
+    rc0 = lanes_join();
+    if (rc0) return rc0;
     // inversion k of every request that has one: 2 per step, 4 for the
     // dissipative ones (their first pair runs beside the others' only pair)
     for (int k = 0; k < 4; k++) {
@@ -318,7 +367,10 @@ int se_server::tick(std::vector<std::shared_ptr<Request>>& active) {
                 rc = se_helm_pcg_dev(ops[j], ops[j]->f.as<double>(), ops[j]->x.as<double>(), &o, &inf[j]);
         }
         if (rc) return rc;
+        rc = lanes_fork();
+        if (rc) return rc;
         for (size_t j = 0; j < idx.size(); j++) {
+            LaneScope on(ctx, lanes[lane[idx[j]]]);
             Request& r = *active[idx[j]];
             const int passes = qg2_passes(r.q);
             r.res.inner_iters += inf[j].iters;
@@ -334,6 +386,8 @@ int se_server::tick(std::vector<std::shared_ptr<Request>>& active) {
             if (!rc2) rc2 = k + 1 < passes ? qg2_phase_rhs(r.q, k + 1) : qg2_phase_post(r.q);
             if (rc2) failed[idx[j]] = rc2;
         }
+        rc = lanes_join();
+        if (rc) return rc;
     }
     SE_HIP(hipStreamSynchronize(ctx->stream));
     std::vector<std::shared_ptr<Request>> keep;
@@ -358,6 +412,19 @@ int se_server::tick(std::vector<std::shared_ptr<Request>>& active) {
 
 void se_server::run() {
     (void)hipSetDevice(ctx->device);
+    bool ok = hipEventCreateWithFlags(&main_ev, hipEventDisableTiming) == hipSuccess;
+    for (int l = 0; l < kLanes && ok; l++)
+        ok = hipStreamCreateWithFlags(&lanes[l], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&lane_ev[l], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {  // no request can run: refuse them all
+        std::lock_guard<std::mutex> lk(mu);
+        error = "request server: cannot create its streams";
+        stopping = true;
+        while (!queue.empty()) {
+            retire(queue.front(), SE_ERR_HIP);
+            queue.pop_front();
+        }
+    }
     std::vector<std::shared_ptr<Request>> active;
     for (;;) {
         {
@@ -378,6 +445,13 @@ void se_server::run() {
             }
         }
     }
+    (void)hipStreamSynchronize(ctx->stream);
+    for (int l = 0; l < kLanes; l++) {
+        if (lanes[l]) (void)hipStreamSynchronize(lanes[l]);
+        if (lanes[l]) (void)hipStreamDestroy(lanes[l]);
+        if (lane_ev[l]) (void)hipEventDestroy(lane_ev[l]);
+    }
+    if (main_ev) (void)hipEventDestroy(main_ev);
 }
 
 extern "C" void se_server_opts_default(se_server_opts* o) {
