@@ -12,6 +12,10 @@ struct se_qg2 {
     double E2 = 0.0, r = 0.0, nu = 0.0;  // Ekman drag (layer 2), thermal relaxation, viscosity
     bool has_tau_e = false;              // relaxation target set (otherwise zero)
     int nstep = 0;  // steps since the state was set (0: the next step is first order)
+    // vel1, vel2 still hold GenVel of the streamfunctions the last step began
+    // with (which its end copied into pp1, pp2): the next step takes them as
+    // the previous velocities instead of computing GenVel of pp1, pp2 again
+    bool vel_prev_ok = false;
     // the step in flight (set by the advection phase)
     int second = 0;
     double dt = 0.0;

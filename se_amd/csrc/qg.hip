@@ -325,6 +325,7 @@ int qg2_solveq(se_qg2* q) {
     const double* tau_e = diss && q->has_tau_e ? q->tau_e.as<double>() : nullptr;
     int rc = SE_OK;
     if (diss && q->nu != 0.0) {
+        q->vel_prev_ok = false;  // vel1, vel2 now hold the viscous term's velocities
         rc = se_genvel_dev(m, q->om1h.as<double>(), q->vel1.as<double>());
         if (!rc && q->second) rc = se_genvel_dev(m, q->om2h.as<double>(), q->vel2.as<double>());
         if (rc) return rc;
@@ -363,17 +364,28 @@ int qg2_phase_advect(se_qg2* q, double dt) {
     hipStream_t s = m->ctx->stream;
     q->second = q->nstep > 0;
     q->dt = dt;
+    // the previous velocities: last step's current ones when nothing has
+    // overwritten them since (bit for bit GenVel of pp1, pp2)
+    const bool reuse = q->second && q->vel_prev_ok;
+    if (reuse) {
+        std::swap(q->vp1, q->vel1);
+        std::swap(q->vp2, q->vel2);
+    }
+    q->vel_prev_ok = false;
     int rc = se_genvel_dev(m, q->psi1.as<double>(), q->vel1.as<double>());
     if (!rc) rc = se_genvel_dev(m, q->psi2.as<double>(), q->vel2.as<double>());
     if (rc) return rc;
+    q->vel_prev_ok = true;
     hipLaunchKernelGGL(k_qg2_pre, dim3(blocks(nV)), dim3(256), 0, s, nV, m->d_verts.as<double>(), q->F1, q->F2,
                        q->beta, q->psi1.as<double>(), q->om1.as<double>(), q->psi2.as<double>(), q->om2.as<double>(),
                        q->a1.as<double>(), q->a2.as<double>(), q->tau.as<double>(), q->chi.as<double>());
     SE_HIP(hipGetLastError());
     if (q->second) {
-        rc = se_genvel_dev(m, q->pp1.as<double>(), q->vp1.as<double>());
-        if (!rc) rc = se_genvel_dev(m, q->pp2.as<double>(), q->vp2.as<double>());
-        if (rc) return rc;
+        if (!reuse) {
+            rc = se_genvel_dev(m, q->pp1.as<double>(), q->vp1.as<double>());
+            if (!rc) rc = se_genvel_dev(m, q->pp2.as<double>(), q->vp2.as<double>());
+            if (rc) return rc;
+        }
         hipLaunchKernelGGL(k_qg2_pv, dim3(blocks(nV)), dim3(256), 0, s, nV, m->d_verts.as<double>(), q->F1, q->F2,
                            q->beta, q->pp1.as<double>(), q->po1.as<double>(), q->pp2.as<double>(),
                            q->po2.as<double>(), q->ap1.as<double>(), q->ap2.as<double>());
@@ -514,6 +526,7 @@ extern "C" int se_qg2_set_state(se_qg2* q, const double* psi1, const double* om1
     SE_HIP(hipMemcpy(q->psi2.ptr, psi2, b, hipMemcpyHostToDevice));
     SE_HIP(hipMemcpy(q->om2.ptr, om2, b, hipMemcpyHostToDevice));
     q->nstep = 0;  // the next step is first order
+    q->vel_prev_ok = false;
     return SE_OK;
 }
 
@@ -534,6 +547,7 @@ extern "C" int se_qg2_timestep(se_qg2* q, double courant, double* dt) {
     se_mesh* m = q->mesh;
     SE_HIP(hipSetDevice(m->ctx->device));
     double d1 = 0.0, d2 = 0.0;
+    q->vel_prev_ok = false;  // the velocities of the current state
     int rc = se_genvel_dev(m, q->psi1.as<double>(), q->vel1.as<double>());
     if (!rc) rc = se_genvel_dev(m, q->psi2.as<double>(), q->vel2.as<double>());
     if (!rc) rc = se_calc_timestep_dev(m, q->vel1.as<double>(), courant, &d1);
