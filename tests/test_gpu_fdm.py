@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 import se_amd
+from se_amd import workloads as W
 from oracle import oracle as O
 
 from test_gpu_parity import OP_CASES, rel
@@ -167,3 +168,43 @@ def test_fdm_large_tiles_invert_and_batch_with_small_ones(gpu_ctx):
             x1, i1 = hh.pcg(f, rtol=1e-10, precision=prec, preconditioner=se_amd.PC_FDM)
             assert i1["iters"] == ib["iters"] and np.array_equal(x1, xb)
     b.close()
+
+
+def test_fdm_full_size(gpu_ctx):
+    """helmholtz_128 (128 x 128 elements at p = 8: 1024 x 1024 unknowns, a
+    single member of 256 transform tiles): the inverse holds in every
+    precision (checked with the FP64 apply, itself pinned against the
+    oracle), FDM-preconditioned CG reaches 1e-10, and batched with a small
+    member each result equals its single solve bit for bit."""
+    cfg = W.helmholtz_128()
+    m = se_amd.Mesh(gpu_ctx, cfg["nEls"], cfg["N"], cfg["lo"], cfg["hi"], cfg["periodic"], cfg["bc"])
+    h = se_amd.Helm(m, cfg["c_mass"], cfg["c_lap"], cfg["detj_mode"])
+    rng = np.random.default_rng(41)
+    r = rng.standard_normal(m.n)
+    z64 = h.apply_fdm(r, se_amd.PREC_FP64)
+    az = h.apply(z64)
+    res = min(rel(az, r), rel(-az, r))
+    print("1024 x 1024, FP64: ||sA z - r|| / ||r|| = %.2e" % res)
+    assert res <= INV_TOL[se_amd.PREC_FP64], res
+    for prec, tol in ((se_amd.PREC_FP32_IR, 1e-4), (se_amd.PREC_BF16_IR, 5e-2)):
+        d = rel(h.apply_fdm(r, prec), z64)
+        print("1024 x 1024, prec %d: ||z - z64|| / ||z64|| = %.2e" % (prec, d))
+        assert d <= tol, d
+    ms = se_amd.Mesh(gpu_ctx, (9, 7), 6, (0, 0), (1, 1), (1, 0), (1, 1, 0, 0))
+    hs = se_amd.Helm(ms, 5.0, 1.0, 1)
+    f = rng.standard_normal(m.n)
+    fs = rng.standard_normal(ms.n)
+    b = se_amd.Batch(gpu_ctx, [h, hs])
+    for prec, _ in PRECS:
+        x1, i1 = h.pcg(f, rtol=1e-10, precision=prec, preconditioner=se_amd.PC_FDM)
+        assert i1["status"] == se_amd.SOLVE_CONVERGED, i1
+        ax = h.apply(x1)
+        assert min(rel(ax, f), rel(-ax, f)) <= 1.5e-10
+        xs1, _ = hs.pcg(fs, rtol=1e-10, precision=prec, preconditioner=se_amd.PC_FDM)
+        xb, ib = b.pcg([f, fs], precision=prec, rtol=1e-10, preconditioner=se_amd.PC_FDM)
+        assert all(i["status"] == se_amd.SOLVE_CONVERGED for i in ib), ib
+        assert ib[0]["iters"] == i1["iters"]
+        assert np.array_equal(xb[0], x1) and np.array_equal(xb[1], xs1)
+    b.close()
+    for o in (h, m, hs, ms):
+        o.close()
