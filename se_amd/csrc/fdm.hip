@@ -1175,7 +1175,7 @@ int plan_apply(FdmPlan* pl, int set, bool from_x, hipStream_t s) {
     for (int i = 0; i < (int)pl->ops.size(); i++) {
         se_op* op = pl->ops[i];
         if (!L.on[i] || op->svv_mN == 0) continue;
-        int rc = svv_apply(op, from_x ? op->x.as<double>() : op->p.as<double>(), op->q.as<double>(),
+        int rc = svv_apply(op, from_x ? pl->h_members[i].x : op->p.as<double>(), op->q.as<double>(),
                            (double)op->sign);
         if (rc) return rc;
     }
@@ -1349,6 +1349,48 @@ int fdm_plan_create(se_ctx* c, int count, se_op* const* ops, FdmPlan** out) {
 
 void fdm_plan_destroy(FdmPlan* pl) { delete pl; }
 
+// the FP64 apply descriptors of every order group: A p (+ p.q partials) and
+// A x (or, fused, the true residual), with each member's current iterate
+int upload_apply_descriptors(FdmPlan* pl, hipStream_t s) {
+    for (auto& od : pl->orders) {
+        if (od.members.empty()) continue;
+        std::vector<ApplyMember> ap, ax;
+        for (int i : od.members) {
+            se_op* op = pl->ops[i];
+            const FbMember& M = pl->h_members[i];
+            ApplyMember A;
+            A.P = op_params(op, (double)op->sign);
+            A.in = op->p.as<double>();
+            A.out = op->q.as<double>();
+            if (M.fused) {  // + partials of p.q
+                A.part = M.pa;
+                A.np = M.nap;
+            }
+            ap.push_back(A);
+            A.in = M.x;
+            if (M.fused) {  // r = b - A x, partials of sum t, sum t^2
+                A.out = op->r.as<double>();
+                A.b = op->b.as<double>();
+                A.part = M.pt;
+            }
+            ax.push_back(A);
+        }
+        int rc = od.d_ap.alloc(sizeof(ApplyMember) * ap.size());
+        if (!rc) rc = od.d_ax.alloc(sizeof(ApplyMember) * ax.size());
+        if (rc) return rc;
+        SE_HIP(hipMemcpyAsync(od.d_ap.ptr, ap.data(), sizeof(ApplyMember) * ap.size(), hipMemcpyHostToDevice, s));
+        SE_HIP(hipMemcpyAsync(od.d_ax.ptr, ax.data(), sizeof(ApplyMember) * ax.size(), hipMemcpyHostToDevice, s));
+        SE_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
+    }
+    return SE_OK;
+}
+
+// The iterate of member i: the caller's solution vector itself unless it is
+// also the right-hand side (then the operator's own x, copied out at the end)
+double* iterate_of(FdmPlan* pl, int i, const double* const* d_f, double* const* d_x) {
+    return d_x[i] != d_f[i] ? d_x[i] : pl->ops[i]->x.as<double>();
+}
+
 int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, const se_solve_opts& o,
                    se_solve_info* infos) {
     se_ctx* c = pl->ctx;
@@ -1365,18 +1407,25 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
     int rc = SE_OK;
     const bool reuse = pl->desc_ok && pl->prec == prec && pl->desc_null == null && pl->desc_rtol == o.rtol;
     if (reuse) {
-        bool changed = false;
+        bool changed = false, moved = false;
         for (int i = 0; i < count; i++) {
             FbMember& M = pl->h_members[i];
             if (M.f != d_f[i] || M.xo != d_x[i]) {
                 M.f = d_f[i];
                 M.xo = d_x[i];
+                double* x = iterate_of(pl, i, d_f, d_x);
+                moved = moved || x != M.x;
+                M.x = x;
                 changed = true;
             }
         }
         if (changed && count > 0)
             SE_HIP(hipMemcpyAsync(pl->d_members.ptr, pl->h_members.data(), sizeof(FbMember) * count,
                                   hipMemcpyHostToDevice, s));
+        if (moved) {  // the A x descriptors name the iterates
+            int e = upload_apply_descriptors(pl, s);
+            if (e) return e;
+        }
     } else {
         {
             int rc = pl->d_states.alloc(sizeof(CgState) * std::max(count, 1));
@@ -1392,7 +1441,7 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
             FbMember& M = pl->h_members[i];
             M.f = d_f[i];
             M.b = op->b.as<double>();
-            M.x = op->x.as<double>();
+            M.x = iterate_of(pl, i, d_f, d_x);
             M.r = op->r.as<double>();
             M.p = op->p.as<double>();
             M.q = op->q.as<double>();
@@ -1435,36 +1484,8 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
         }
         pl->prec = prec;
         // FP64 apply descriptors
-        for (auto& od : pl->orders) {
-            if (od.members.empty()) continue;
-            std::vector<ApplyMember> ap, ax;
-            for (int i : od.members) {
-                se_op* op = pl->ops[i];
-                const FbMember& M = pl->h_members[i];
-                ApplyMember A;
-                A.P = op_params(op, (double)op->sign);
-                A.in = op->p.as<double>();
-                A.out = op->q.as<double>();
-                if (M.fused) {  // + partials of p.q
-                    A.part = M.pa;
-                    A.np = M.nap;
-                }
-                ap.push_back(A);
-                A.in = op->x.as<double>();
-                if (M.fused) {  // r = b - A x, partials of sum t, sum t^2
-                    A.out = op->r.as<double>();
-                    A.b = op->b.as<double>();
-                    A.part = M.pt;
-                }
-                ax.push_back(A);
-            }
-            rc = od.d_ap.alloc(sizeof(ApplyMember) * ap.size());
-            if (!rc) rc = od.d_ax.alloc(sizeof(ApplyMember) * ax.size());
-            if (rc) return rc;
-            SE_HIP(hipMemcpyAsync(od.d_ap.ptr, ap.data(), sizeof(ApplyMember) * ap.size(), hipMemcpyHostToDevice, s));
-            SE_HIP(hipMemcpyAsync(od.d_ax.ptr, ax.data(), sizeof(ApplyMember) * ax.size(), hipMemcpyHostToDevice, s));
-            SE_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
-        }
+        rc = upload_apply_descriptors(pl, s);
+        if (rc) return rc;
         for (auto& L : pl->sets) L.clear();  // new descriptors: new lists
         pl->desc_ok = true;
         pl->desc_null = null;
@@ -1622,7 +1643,7 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
         std::vector<char> out(count, 0);
         bool any_out = false;
         for (int i = 0; i < count; i++) {
-            out[i] = d_x[i] != pl->ops[i]->x.as<double>();
+            out[i] = d_x[i] != pl->h_members[i].x;  // the iterate is not the caller's vector
             any_out = any_out || out[i];
         }
         if (any_out) {
