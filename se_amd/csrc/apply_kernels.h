@@ -82,6 +82,7 @@ struct ApplyMember {
     double* out;
     double* part = nullptr;      // AP_DOT / AP_TRUE partials, np of them per sum
     const double* b = nullptr;   // AP_TRUE: the right-hand side
+    const CgState* gate = nullptr;  // gated launches: only while this CG awaits its true residual
     int np = 0;
 };
 

@@ -306,14 +306,17 @@ __global__ __launch_bounds__(256) void k_apply_lines(OpParams P, const double* _
 }
 
 // Batched: work items (member, tile) over members of one order; MODE's
-// partials go to the member's `part` (line_tiles of its mesh entries)
+// partials go to the member's `part` (line_tiles of its mesh entries).
+// gated: a member whose `gate` CG is not awaiting its true-residual check
+// (the direction kernel before this launch marks it) has nothing to do.
 template <int N, int MODE>
 __global__ __launch_bounds__(256) void k_apply_lines_batch(const ApplyMember* __restrict__ members,
-                                                            const int2* __restrict__ work, int nwork) {
+                                                            const int2* __restrict__ work, int nwork, int gated) {
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
         const int2 it = work[w];
         if (it.x < 0) continue;  // a padding item of an XCD-ordered list
         const ApplyMember& a = members[it.x];
+        if (gated && a.gate && a.gate->status != CG_PENDING) continue;
         __syncthreads();  // the previous item's LDS reads are done
         apply_lines_tile<N, MODE>(a.P, it.y, a.in, a.out, a.part, a.np, a.b);
     }

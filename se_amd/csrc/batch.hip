@@ -196,7 +196,7 @@ int g_apply64(const Group& g, const GLL& gl, int nwork, const int2* work, hipStr
     switch (g.N) {
 #define SE_CASE(NN)                                                                                                  \
     case NN:                                                                                                         \
-        hipLaunchKernelGGL((k_apply_lines_batch<NN, AP_PLAIN>), dim3(nwork), dim3(256), 0, s, As, work, nwork);                                                                             \
+        hipLaunchKernelGGL((k_apply_lines_batch<NN, AP_PLAIN>), dim3(nwork), dim3(256), 0, s, As, work, nwork, 0);                                                                             \
         break;
         SE_BATCH_ORDERS(SE_CASE)
 #undef SE_CASE

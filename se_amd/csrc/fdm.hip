@@ -598,12 +598,13 @@ __global__ __launch_bounds__(FB_NT) void k_fb_update(const FbMember* __restrict_
 // true residual t = b - q (q = A x): r = t; partials of sum t, sum t^2
 // (members with an SVV term)
 __global__ __launch_bounds__(FB_NT) void k_fb_true(const FbMember* __restrict__ Ms, const int2* __restrict__ work,
-                                                   int nwork) {
+                                                   int nwork, int gated) {
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
         __syncthreads();  // the previous item's reduction is done with its LDS
         const int2 it = work[w];
         const FbMember& M = Ms[it.x];
         if (M.fused) continue;  // the apply formed r and its partials
+        if (gated && M.st->status != CG_PENDING) continue;
         int i0, i1;
         chunk_range(M, it.y, i0, i1);
         double s = 0.0, s2 = 0.0;
@@ -620,9 +621,10 @@ __global__ __launch_bounds__(FB_NT) void k_fb_true(const FbMember* __restrict__ 
 
 // per member (one workgroup each): ||t - mean t||^2 into the state
 __global__ __launch_bounds__(FB_NT) void k_fb_true_fin(const FbMember* __restrict__ Ms, const int* __restrict__ list,
-                                                       int count) {
+                                                       int count, int gated) {
     for (int w = blockIdx.x; w < count; w += gridDim.x) {
         const FbMember& M = Ms[list[w]];
+        if (gated && M.st->status != CG_PENDING) continue;
         __syncthreads();
         double t[2];
         reduce_partials<2, FB_NT>(M.pt, M.nap, t);
@@ -1084,11 +1086,11 @@ struct FdmPlan {
 namespace {
 
 template <int MODE>
-int launch_apply_batch_m(int N, const ApplyMember* A, const int2* work, int nw, hipStream_t s) {
+int launch_apply_batch_m(int N, const ApplyMember* A, const int2* work, int nw, hipStream_t s, int gated) {
     switch (N) {
-#define SE_CASE(NN)                                                                               \
-    case NN:                                                                                      \
-        hipLaunchKernelGGL((k_apply_lines_batch<NN, MODE>), dim3(nw), dim3(256), 0, s, A, work, nw); \
+#define SE_CASE(NN)                                                                                      \
+    case NN:                                                                                             \
+        hipLaunchKernelGGL((k_apply_lines_batch<NN, MODE>), dim3(nw), dim3(256), 0, s, A, work, nw, gated); \
         break;
         SE_CASE(2) SE_CASE(3) SE_CASE(4) SE_CASE(5) SE_CASE(6) SE_CASE(7) SE_CASE(8)
 #undef SE_CASE
@@ -1097,10 +1099,10 @@ int launch_apply_batch_m(int N, const ApplyMember* A, const int2* work, int nw, 
     return SE_OK;
 }
 
-int launch_apply_batch(int N, int mode, const ApplyMember* A, const int2* work, int nw, hipStream_t s) {
-    if (mode == AP_DOT) return launch_apply_batch_m<AP_DOT>(N, A, work, nw, s);
-    if (mode == AP_TRUE) return launch_apply_batch_m<AP_TRUE>(N, A, work, nw, s);
-    return launch_apply_batch_m<AP_PLAIN>(N, A, work, nw, s);
+int launch_apply_batch(int N, int mode, const ApplyMember* A, const int2* work, int nw, hipStream_t s, int gated) {
+    if (mode == AP_DOT) return launch_apply_batch_m<AP_DOT>(N, A, work, nw, s, gated);
+    if (mode == AP_TRUE) return launch_apply_batch_m<AP_TRUE>(N, A, work, nw, s, gated);
+    return launch_apply_batch_m<AP_PLAIN>(N, A, work, nw, s, gated);
 }
 
 // Work lists XCD by XCD.  The dispatcher hands workgroup b to XCD b mod 8
@@ -1159,8 +1161,9 @@ void xcd_interleave(const std::vector<int>& ids, const std::vector<int>& tiles, 
 
 // q = sign A p of the members of list set `set` (whose lists plan_lists
 // built), with the partials of p.q when fused; from x: q = sign A x, or when
-// fused the true residual r = b - sign A x with its partials
-int plan_apply(FdmPlan* pl, int set, bool from_x, hipStream_t s) {
+// fused the true residual r = b - sign A x with its partials (gated: only
+// the members whose CG awaits that check)
+int plan_apply(FdmPlan* pl, int set, bool from_x, hipStream_t s, bool gated = false) {
     const FdmPlan::Lists& L = *pl->sets[set].cur;
     const int mode = from_x ? AP_TRUE : AP_DOT;
     for (int k = 0; k < (int)pl->orders.size(); k++) {
@@ -1168,7 +1171,7 @@ int plan_apply(FdmPlan* pl, int set, bool from_x, hipStream_t s) {
         const int nw = L.n_apply[k];
         if (o.members.empty() || nw == 0) continue;
         const ApplyMember* A = from_x ? o.d_ax.as<ApplyMember>() : o.d_ap.as<ApplyMember>();
-        int rc = launch_apply_batch(o.N, mode, A, L.apply[k], nw, s);
+        int rc = launch_apply_batch(o.N, mode, A, L.apply[k], nw, s, gated ? 1 : 0);
         if (rc) return rc;
     }
     // members with an SVV term: add it to the batched apply's result
@@ -1176,7 +1179,7 @@ int plan_apply(FdmPlan* pl, int set, bool from_x, hipStream_t s) {
         se_op* op = pl->ops[i];
         if (!L.on[i] || op->svv_mN == 0) continue;
         int rc = svv_apply(op, from_x ? pl->h_members[i].x : op->p.as<double>(), op->q.as<double>(),
-                           (double)op->sign);
+                           (double)op->sign, gated ? pl->d_states.as<CgState>() + i : nullptr);
         if (rc) return rc;
     }
     return SE_OK;
@@ -1368,6 +1371,7 @@ int upload_apply_descriptors(FdmPlan* pl, hipStream_t s) {
             }
             ap.push_back(A);
             A.in = M.x;
+            A.gate = M.st;
             if (M.fused) {  // r = b - A x, partials of sum t, sum t^2
                 A.out = op->r.as<double>();
                 A.b = op->b.as<double>();
@@ -1517,23 +1521,28 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
             hipLaunchKernelGGL(k_fb_dir, dim3(grid_of(nw, c)), dim3(FB_NT), 0, s, Ms, L.vwork, nw, mode);
     };
     // the true residual of the flagged members: t = b - A x into r, ||t - mean t||^2 into the state
-    auto true_residual = [&](int set, const std::vector<char>& flags) -> int {
+    // (gated: of the flagged members only those whose CG has just marked
+    // itself as awaiting the check, decided on the device)
+    auto true_residual = [&](int set, const std::vector<char>& flags, bool gated) -> int {
         int e = plan_lists(pl, set, flags, s);
-        if (!e) e = plan_apply(pl, set, true, s);
+        if (!e) e = plan_apply(pl, set, true, s, gated);
         if (e) return e;
-        if (pl->any_svv) vlaunch(set, k_fb_true);
         const FdmPlan::Lists& L = *pl->sets[set].cur;
-        if (L.n_list > 0) hipLaunchKernelGGL(k_fb_true_fin, dim3(L.n_list), dim3(FB_NT), 0, s, Ms, L.list, L.n_list);
+        if (pl->any_svv && L.n_vwork > 0)
+            hipLaunchKernelGGL(k_fb_true, dim3(grid_of(L.n_vwork, c)), dim3(FB_NT), 0, s, Ms, L.vwork, L.n_vwork,
+                               gated ? 1 : 0);
+        if (L.n_list > 0)
+            hipLaunchKernelGGL(k_fb_true_fin, dim3(L.n_list), dim3(FB_NT), 0, s, Ms, L.list, L.n_list, gated ? 1 : 0);
         return SE_OK;
     };
-    const int IT = FdmPlan::SET_ITER, CK = FdmPlan::SET_CHECK, OT = FdmPlan::SET_OTHER;
+    const int IT = FdmPlan::SET_ITER, OT = FdmPlan::SET_OTHER;
 
     SE_HIP(hipEventRecord(c->ev0, s));
     SE_HIP(hipMemcpyAsync(pl->d_states.ptr, pl->h_states, sizeof(CgState) * count, hipMemcpyHostToDevice, s));
     // start: b, r = b, x = 0, z = FDM(r), p = z.  No host round trip: a member
     // stopped here (zero or non-finite right-hand side) is skipped by every
     // kernel of the first iteration and classified after it.
-    std::vector<char> on(count, 1), chk(count, 0);
+    std::vector<char> on(count, 1);
     rc = plan_lists(pl, IT, on, s);
     if (rc) return rc;
     if (null) vlaunch(IT, k_fb_sum);
@@ -1550,54 +1559,34 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
     std::vector<double> best(count, HUGE_VAL);
     std::vector<int> best_it(count, 0);
     int guard = 0;
-    // Rounds: the running members' next iteration and the true-residual check
-    // of the members whose recursive residual met the target in the previous
-    // round are launched together (they are disjoint), then one pull.
+    // Rounds: the running members' next iteration, then the true-residual
+    // check of those whose recursive residual has just met the target (gated
+    // on the device, so it needs no round trip of its own), then one pull.
     for (;;) {
-        bool any_on = false, any_chk = false;
-        for (int i = 0; i < count; i++) {
-            any_on = any_on || on[i];
-            any_chk = any_chk || chk[i];
-        }
-        if (!any_on && !any_chk) break;
-        if (any_on) {  // q = A p, alpha, x and r, z = FDM(r), beta and p
-            rc = plan_lists(pl, IT, on, s);
-            if (!rc) rc = plan_apply(pl, IT, false, s);
-            if (rc) return rc;
-            if (pl->any_svv) vlaunch(IT, k_fb_dot);
-            vlaunch(IT, k_fb_update);
-            rc = plan_fdm(pl, IT, s, 1);
-            if (rc) return rc;
-            dlaunch(IT, FB_NORMAL);
-        }
-        if (any_chk) {
-            rc = true_residual(CK, chk);
-            if (rc) return rc;
-        }
+        bool any_on = false;
+        for (int i = 0; i < count; i++) any_on = any_on || on[i];
+        if (!any_on) break;
+        // q = A p, alpha, x and r, z = FDM(r), beta and p
+        rc = plan_lists(pl, IT, on, s);
+        if (!rc) rc = plan_apply(pl, IT, false, s);
+        if (rc) return rc;
+        if (pl->any_svv) vlaunch(IT, k_fb_dot);
+        vlaunch(IT, k_fb_update);
+        rc = plan_fdm(pl, IT, s, 1);
+        if (rc) return rc;
+        dlaunch(IT, FB_NORMAL);
+        rc = true_residual(IT, on, true);
+        if (rc) return rc;
         SE_HIP(hipGetLastError());
         rc = pull();
         if (rc) return rc;
-        // checked members: converged, out of iterations, or restarted from the true residual
-        std::vector<char> again(count, 0), next_chk(count, 0);
+        // running members: continuing, stagnated, stopped, or checked in this
+        // round (converged, out of iterations, or restarted from the true residual)
+        std::vector<char> again(count, 0);
         bool restart = false;
         for (int i = 0; i < count; i++) {
-            if (!chk[i]) continue;
-            CgState& h = pl->h_states[i];
-            if (h.tt <= h.tol2) {
-                status[i] = 0;
-            } else if (h.iter >= h.maxit || ++guard > 100 * count + 1000) {
-                status[i] = 1;
-            } else {
-                again[i] = 1;
-                restart = true;
-                h.stop = 0;
-                h.status = CG_RUNNING;
-            }
-        }
-        // running members: continuing, stagnated, stopped, or due for the check
-        for (int i = 0; i < count; i++) {
             if (!on[i]) continue;
-            const CgState& h = pl->h_states[i];
+            CgState& h = pl->h_states[i];
             if (!h.stop) {
                 if (h.rr < 0.25 * best[i]) {
                     best[i] = h.rr;
@@ -1609,10 +1598,19 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
                 continue;
             }
             on[i] = 0;
-            if (h.status == CG_PENDING) next_chk[i] = 1;
-            else status[i] = h.status;
+            if (h.status != CG_PENDING) {
+                status[i] = h.status;
+            } else if (h.tt <= h.tol2) {
+                status[i] = 0;
+            } else if (h.iter >= h.maxit || ++guard > 100 * count + 1000) {
+                status[i] = 1;
+            } else {
+                again[i] = 1;
+                restart = true;
+                h.stop = 0;
+                h.status = CG_RUNNING;
+            }
         }
-        chk = next_chk;
         if (!restart) continue;
         // continue from the true residual (now in r) with a fresh direction
         SE_HIP(hipMemcpyAsync(pl->d_states.ptr, pl->h_states, sizeof(CgState) * count, hipMemcpyHostToDevice, s));
@@ -1635,7 +1633,7 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
             anyfin = true;
         }
     if (anyfin) {
-        rc = true_residual(OT, fin);
+        rc = true_residual(OT, fin, false);
         if (!rc) rc = pull();
         if (rc) return rc;
     }

@@ -73,8 +73,9 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
                    se_solve_info* infos);
 int fdm_plan_probe(FdmPlan* pl, int reps, double* us_pass, double* flops_pass, double* us_apply,
                    double* bytes_apply);
-// SVV term (svv.hip): y += scale S x; np partials of x.y; lift -= S x_D
-int svv_apply(se_op* op, const double* d_x, double* d_y, double scale);
+// SVV term (svv.hip): y += scale S x (gate: only while that CG awaits its
+// true-residual check); np partials of x.y; lift -= S x_D
+int svv_apply(se_op* op, const double* d_x, double* d_y, double scale, const CgState* gate = nullptr);
 int svv_dot(se_op* op, const double* d_x, const double* d_y, double* partials, int np);
 int svv_lift(se_op* op, const double* d_field, double* d_lift);
 }  // namespace se

@@ -52,10 +52,13 @@ __device__ __forceinline__ int svv_incidences(int c, int N, int nverts, int* e, 
 
 // mode APPLY: y[u] += coef (S x)[u]; LIFT: y[u] -= coef (S x_D)[u] with x_D the
 // field's Dirichlet values; DIAG: y[u] = coef S[u][u]
+// (gate: when given, nothing to do unless that CG awaits its true-residual check)
 __global__ __launch_bounds__(256) void k_svv(SvvGeom g, double coef, int mode, const double* __restrict__ x,
-                                             const double* __restrict__ field, double* __restrict__ y) {
+                                             const double* __restrict__ field, double* __restrict__ y,
+                                             const CgState* __restrict__ gate) {
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= g.n) return;
+    if (gate && gate->status != CG_PENDING) return;
     const int N = g.N, n1 = N + 1, nv0 = g.nv0;
     const int v = g.v2f[u];
     const int gx = v % nv0, gy = v / nv0;
@@ -232,10 +235,10 @@ int blocks_for_n(int n) { return std::max(1, (n + 255) / 256); }
 
 namespace se {
 
-int svv_apply(se_op* op, const double* d_x, double* d_y, double scale) {
+int svv_apply(se_op* op, const double* d_x, double* d_y, double scale, const CgState* gate) {
     if (op->mesh->n == 0) return SE_OK;
     hipLaunchKernelGGL(k_svv, dim3(blocks_for_n(op->mesh->n)), dim3(256), 0, op->mesh->ctx->stream, svv_geom(op),
-                       svv_coef(op, scale), (int)SVV_APPLY, d_x, (const double*)nullptr, d_y);
+                       svv_coef(op, scale), (int)SVV_APPLY, d_x, (const double*)nullptr, d_y, gate);
     SE_HIP(hipGetLastError());
     return SE_OK;
 }
@@ -250,7 +253,8 @@ int svv_dot(se_op* op, const double* d_x, const double* d_y, double* partials, i
 int svv_lift(se_op* op, const double* d_field, double* d_lift) {
     if (op->mesh->n == 0) return SE_OK;
     hipLaunchKernelGGL(k_svv, dim3(blocks_for_n(op->mesh->n)), dim3(256), 0, op->mesh->ctx->stream, svv_geom(op),
-                       svv_coef(op, 1.0), (int)SVV_LIFT, (const double*)nullptr, d_field, d_lift);
+                       svv_coef(op, 1.0), (int)SVV_LIFT, (const double*)nullptr, d_field, d_lift,
+                       (const CgState*)nullptr);
     SE_HIP(hipGetLastError());
     return SE_OK;
 }
@@ -281,7 +285,8 @@ extern "C" int se_helm_set_svv(se_op* op, double dt, int cutoff) {
         rc = d.alloc(sizeof(double) * n);
         if (rc) return rc;
         hipLaunchKernelGGL(k_svv, dim3(blocks_for_n(n)), dim3(256), 0, s, svv_geom(op), svv_coef(op, 1.0),
-                           (int)SVV_DIAG, (const double*)nullptr, (const double*)nullptr, d.as<double>());
+                           (int)SVV_DIAG, (const double*)nullptr, (const double*)nullptr, d.as<double>(),
+                           (const CgState*)nullptr);
         hipLaunchKernelGGL(k_svv_add_diag, dim3(blocks_for_n(n)), dim3(256), 0, s, n, d.as<double>(),
                            op->dinv.as<double>());
         SE_HIP(hipGetLastError());
