@@ -499,6 +499,14 @@ __global__ __launch_bounds__(FB_NT) void k_fb_dir(const FbMember* __restrict__ M
                 if (!finite) {
                     st->stop = 1;
                     st->status = 3;
+                } else if (conv && itn == 1) {
+                    // from x = 0 the first iterate is x = alpha p (one rounding)
+                    // and r = b - alpha (A p): r is already an evaluation of
+                    // b - A x, with the rounding a separate one would carry
+                    st->stop = 1;
+                    st->status = 0;
+                    st->tt = rr;
+                    st->mu_t = 0.0;
                 } else if (conv) {
                     st->stop = 1;
                     st->status = CG_PENDING;
