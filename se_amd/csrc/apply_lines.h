@@ -177,9 +177,9 @@ __device__ __forceinline__ void apply_lines_tile(const OpParams& P, int tile,
     const int e = tid / N1, t = tid - (tid / N1) * N1;
     const int c = e % (TX + 1), r = e / (TX + 1);
     const bool live = e < NE && (c > 0 || halo_c) && (r > 0 || halo_r) && c <= txo && r <= tyo;
-    double uxi[N1], uet[N1], ur[N1];
+    double uxi[N1], uet[N1];
     if (live) {
-        double uc[N1];
+        double ur[N1], uc[N1];
 #pragma unroll
         for (int k = 0; k < N1; k++) {
             ur[k] = s_u[(r * N + t) * BW + c * N + k];
@@ -236,7 +236,8 @@ __device__ __forceinline__ void apply_lines_tile(const OpParams& P, int tile,
                 sx += DTc[l * N1 + q] * fx[q];
                 sy += DTc[l * N1 + q] * fy[q];
             }
-            yx[l] = sx + my * cf[3 * N1 + l] * ur[l];  // node (l, t)
+            // node (l, t); the mass term re-reads u (fewer live registers)
+            yx[l] = sx + my * cf[3 * N1 + l] * s_u[(r * N + t) * BW + c * N + l];
             yy[l] = sy;                                // node (t, l)
             __builtin_amdgcn_sched_barrier(0);
         }

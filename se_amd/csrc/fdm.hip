@@ -77,6 +77,9 @@ struct GemmTask {
     const double* pv;
     const CgState* st;
     int nch;
+    // FINAL, launched for a CG's first or restarted direction: z goes straight
+    // to the CG's p (its direction, p = z - mean z), not to z
+    double* C2;
 };
 
 // Output tiles are 64 x 64, four waves of 32 x 32 each.
@@ -235,7 +238,7 @@ struct ChunkCopy {
 __device__ __forceinline__ void wait_copies() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 template <typename T>
-__device__ __forceinline__ void gemm_tile(const GemmTask& t, int tile, char* S0, char* S1) {
+__device__ __forceinline__ void gemm_tile(const GemmTask& t, int tile, char* S0, char* S1, bool to_p) {
     using C = GemmCfg<T>;
     constexpr int NB = Acc<T>::NB;
     const int tid = threadIdx.x, l = tid & 63;
@@ -298,7 +301,7 @@ __device__ __forceinline__ void gemm_tile(const GemmTask& t, int tile, char* S0,
             Cm[(size_t)gr * t.ldc + gc] = to_t<T>(v * s);
         });
     } else {  // FDM_OUT_FINAL: FP64 z on the unknown grid, partials of r.z, sum z, sum r
-        SE_GLOBAL double* Z = (SE_GLOBAL double*)t.C;
+        SE_GLOBAL double* Z = (SE_GLOBAL double*)(to_p ? t.C2 : t.C);
         const SE_GLOBAL double* R = (const SE_GLOBAL double*)t.r;
         double rz = 0.0, sz = 0.0, sr = 0.0;
         for_each_acc<T>(acc, wr, wc, l, [&](int row, int col, double v) {
@@ -336,7 +339,7 @@ __device__ __forceinline__ bool fdm_skip(const GemmTask& t) {
 // workgroups per CU (one wave of each per SIMD, at most 128 VGPRs).
 template <typename T>
 __global__ __launch_bounds__(256, 4) void k_fdm_gemm(const GemmTask* __restrict__ tasks, int pass,
-                                                    const int2* __restrict__ work, int nwork, int skip) {
+                                                    const int2* __restrict__ work, int nwork, int skip, int to_p) {
     __shared__ __attribute__((aligned(1024))) char S0[GemmCfg<T>::STAGE];
     __shared__ __attribute__((aligned(1024))) char S1[GemmCfg<T>::STAGE];
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
@@ -344,7 +347,7 @@ __global__ __launch_bounds__(256, 4) void k_fdm_gemm(const GemmTask* __restrict_
         if (it.x < 0) continue;  // a padding item of the XCD-ordered list
         const GemmTask t = tasks[it.x * 4 + pass];
         if (skip && fdm_skip(t)) continue;
-        gemm_tile<T>(t, it.y, S0, S1);
+        gemm_tile<T>(t, it.y, S0, S1, to_p != 0);
     }
 }
 
@@ -424,7 +427,9 @@ __global__ __launch_bounds__(FB_NT) void k_fb_sum(const FbMember* __restrict__ M
     }
 }
 
-// b = sgn f - mean (null space), x = 0, r = b; partials of r.r
+// b = sgn f - mean (null space), r = b; partials of r.r.  (x = 0 is implicit:
+// the first update writes x = alpha p, and a member stopped before it gets
+// x = 0 from the first direction kernel or the update)
 __global__ __launch_bounds__(FB_NT) void k_fb_start(const FbMember* __restrict__ Ms, const int2* __restrict__ work,
                                                     int nwork) {
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
@@ -445,7 +450,6 @@ __global__ __launch_bounds__(FB_NT) void k_fb_start(const FbMember* __restrict__
             M.b[i] = bi;
             M.r[i] = bi;
             put_packed(M, i, bi);
-            M.x[i] = 0.0;
             rr += bi * bi;
         }
         const double v[1] = {rr};
@@ -531,12 +535,18 @@ __global__ __launch_bounds__(FB_NT) void k_fb_dir(const FbMember* __restrict__ M
                 st->beta = 0.0;
             }
         }
-        if (!go) continue;
         int i0, i1;
         chunk_range(M, it.y, i0, i1);
-        for (int i = i0 + threadIdx.x; i < i1; i += FB_NT) {
-            const double zi = M.z[i] - mu;
-            M.p[i] = mode == FB_NORMAL ? zi + beta * M.p[i] : zi;
+        if (!go) {
+            // stopped before any update (b = 0, or not finite): x = 0
+            if (mode == FB_FIRST)
+                for (int i = i0 + threadIdx.x; i < i1; i += FB_NT) M.x[i] = 0.0;
+            continue;
+        }
+        if (mode == FB_NORMAL) {
+            for (int i = i0 + threadIdx.x; i < i1; i += FB_NT) M.p[i] = (M.z[i] - mu) + beta * M.p[i];
+        } else if (mu != 0.0) {  // the transforms wrote z into p
+            for (int i = i0 + threadIdx.x; i < i1; i += FB_NT) M.p[i] -= mu;
         }
     }
 }
@@ -573,12 +583,17 @@ __global__ __launch_bounds__(FB_NT) void k_fb_update(const FbMember* __restrict_
         if (s_stop) continue;
         const double pq = t[0];
         const bool lead = it.y == 0 && threadIdx.x == 0;
+        const bool first = st->iter == 0;  // x = 0 until this update
+        int i0, i1;
+        chunk_range(M, it.y, i0, i1);
         if (!(pq > 0.0)) {
             if (lead) {
                 st->pq = pq;
                 st->stop = 1;
                 st->status = (pq - pq == 0.0) ? 2 : 3;
             }
+            if (first)
+                for (int i = i0 + threadIdx.x; i < i1; i += FB_NT) M.x[i] = 0.0;
             continue;
         }
         const double rho = st->rho;
@@ -588,11 +603,9 @@ __global__ __launch_bounds__(FB_NT) void k_fb_update(const FbMember* __restrict_
             st->alpha = alpha;
             st->rho_prev = rho;
         }
-        int i0, i1;
-        chunk_range(M, it.y, i0, i1);
         double rr = 0.0;
         for (int i = i0 + threadIdx.x; i < i1; i += FB_NT) {
-            M.x[i] += alpha * M.p[i];
+            M.x[i] = first ? alpha * M.p[i] : M.x[i] + alpha * M.p[i];
             const double ri = M.r[i] - alpha * M.q[i];
             M.r[i] = ri;
             put_packed(M, i, ri);
@@ -996,20 +1009,22 @@ bool fdm_tasks_valid(const GemmTask (&t)[4], int prec) {
 int fdm_tiles(const se_op* op) { return (op->fdm->nP0 / FDM_BT) * (op->fdm->nP1 / FDM_BT); }
 
 template <typename T>
-void launch_gemm_t(const GemmTask* tasks, int pass, const int2* work, int nwork, hipStream_t s, int skip, bool pack) {
+void launch_gemm_t(const GemmTask* tasks, int pass, const int2* work, int nwork, hipStream_t s, int skip, bool pack,
+                   bool to_p) {
     const dim3 grid(nwork), block(256);
     if (pass == 0 && pack) hipLaunchKernelGGL((k_fdm_pack<T>), grid, block, 0, s, tasks, work, nwork, skip);
-    hipLaunchKernelGGL((k_fdm_gemm<T>), grid, block, 0, s, tasks, pass, work, nwork, skip);
+    hipLaunchKernelGGL((k_fdm_gemm<T>), grid, block, 0, s, tasks, pass, work, nwork, skip, to_p ? 1 : 0);
 }
 
 // pass `pass` of the listed items; pass 1 first packs the residual when
-// `pack` (the CG kernels store it packed as they write r, so a solve does not)
+// `pack` (the CG kernels store it packed as they write r, so a solve does not);
+// to_p: the last pass writes the tasks' C2
 int launch_gemm(int prec, const GemmTask* tasks, int pass, const int2* work, int nwork, hipStream_t s, int skip,
-                bool pack) {
+                bool pack, bool to_p = false) {
     if (nwork <= 0) return SE_OK;
-    if (prec == FDM_F64) launch_gemm_t<double>(tasks, pass, work, nwork, s, skip, pack);
-    else if (prec == FDM_F32) launch_gemm_t<float>(tasks, pass, work, nwork, s, skip, pack);
-    else launch_gemm_t<__bf16>(tasks, pass, work, nwork, s, skip, pack);
+    if (prec == FDM_F64) launch_gemm_t<double>(tasks, pass, work, nwork, s, skip, pack, to_p);
+    else if (prec == FDM_F32) launch_gemm_t<float>(tasks, pass, work, nwork, s, skip, pack, to_p);
+    else launch_gemm_t<__bf16>(tasks, pass, work, nwork, s, skip, pack, to_p);
     return SE_OK;
 }
 
@@ -1300,16 +1315,18 @@ int grid_of(size_t items, const se_ctx* c) {
 }
 
 // one pass of the transforms of the listed members
-int plan_pass(FdmPlan* pl, int set, int pass, hipStream_t s, int skip = 0) {
+int plan_pass(FdmPlan* pl, int set, int pass, hipStream_t s, int skip = 0, bool to_p = false) {
     const FdmPlan::Lists& L = *pl->sets[set].cur;
-    return launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, L.gwork, L.n_gwork, s, skip, false);
+    return launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, L.gwork, L.n_gwork, s, skip, false,
+                       to_p && pass == 3);
 }
 
 // z = FDM(r) of the listed members: four passes
 
-int plan_fdm(FdmPlan* pl, int set, hipStream_t s, int skip = 0) {
+// (to_p: into the CG's p, for a first or restarted direction)
+int plan_fdm(FdmPlan* pl, int set, hipStream_t s, int skip = 0, bool to_p = false) {
     for (int pass = 0; pass < 4; pass++) {
-        int rc = plan_pass(pl, set, pass, s, skip);
+        int rc = plan_pass(pl, set, pass, s, skip, to_p);
         if (rc) return rc;
     }
     return SE_OK;
@@ -1481,6 +1498,7 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
             GemmTask t4[4];
             fdm_tasks(op, prec, M.r, M.z, M.pz, t4);
             SE_CHECK(fdm_tasks_valid(t4, prec), SE_ERR_STATE, "FDM: transform shapes break the kernels' assumptions");
+            t4[3].C2 = M.p;
             for (int k = 0; k < 4; k++) {
                 t4[k].pv = M.pv;
                 t4[k].st = M.st;
@@ -1555,7 +1573,7 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
     if (rc) return rc;
     if (null) vlaunch(IT, k_fb_sum);
     vlaunch(IT, k_fb_start);
-    rc = plan_fdm(pl, IT, s);
+    rc = plan_fdm(pl, IT, s, 0, true);
     if (rc) return rc;
     dlaunch(IT, FB_FIRST);
     std::vector<int> status(count, CG_RUNNING);
@@ -1625,7 +1643,7 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
         rc = plan_lists(pl, OT, again, s);
         if (rc) return rc;
         vlaunch(OT, k_fb_restart);
-        rc = plan_fdm(pl, OT, s);
+        rc = plan_fdm(pl, OT, s, 0, true);
         if (rc) return rc;
         dlaunch(OT, FB_RESTART);
         SE_HIP(hipGetLastError());
