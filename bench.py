@@ -8,9 +8,11 @@ Workloads (--workload):
       right-hand sides f = M g with g a seeded random field with an isotropic
       k^-3 spectrum.  One step = all 512 solves to a true relative residual of
       1e-10, batched: FP64 CG preconditioned by fast diagonalisation with its
-      transforms on the matrix cores (--precision: FP64, FP32 or BF16
-      operands; --pc jacobi: the reference's Jacobi-PCG with FP32 or BF16
-      inner iterations and FP64 reliable updates).  The plain FP64 GPU path
+      transforms on the matrix cores (--precision fp64_fp32c, the default:
+      FP64 transforms for the first application, FP32 for the corrections;
+      fp64, fp32_ir, bf16_ir: FP64, FP32 or BF16 operands throughout;
+      --pc jacobi: the reference's Jacobi-PCG with FP32 or BF16 inner
+      iterations and FP64 reliable updates).  The plain FP64 GPU path
       (each member alone, FP64 Jacobi-PCG, the reference's algorithm) is timed
       on the same members once and reported beside it, with the other
       precisions / preconditioners batched, and a served two-layer QG stream of
@@ -43,8 +45,16 @@ METRIC = "Helmholtz CG GDOF/s at 1e-10 rel. residual vs FP64; p50/p99 latency un
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector
 # MI355X dense matrix-core peaks by operand type (TFLOP/s)
-MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32_ir": 157.3, "bf16_ir": 2500.0}
-DTYPE = {"fp64": "f64", "fp32_ir": "f32+f64", "bf16_ir": "bf16+f64"}
+MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp64_fp32c": 78.6, "fp32_ir": 157.3, "bf16_ir": 2500.0}
+DTYPE = {"fp64": "f64", "fp64_fp32c": "f64+f32", "fp32_ir": "f32+f64", "bf16_ir": "bf16+f64"}
+# the element type of the probed GEMM (for fp64_fp32c: the first application's, FP64)
+GEMM_T = {"fp64": "double", "fp64_fp32c": "double", "fp32_ir": "float", "bf16_ir": "bf16"}
+
+
+def prec_code(name):
+    import se_amd
+    return {"fp64": se_amd.PREC_FP64, "fp64_fp32c": se_amd.PREC_FP64_FP32C, "fp32_ir": se_amd.PREC_FP32_IR,
+            "bf16_ir": se_amd.PREC_BF16_IR}[name]
 
 
 def parse():
@@ -53,8 +63,9 @@ def parse():
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--workload", default="ensemble_512")
-    p.add_argument("--precision", default=None, choices=["fp64", "fp32_ir", "bf16_ir"],
-                   help="default: fp64 (FDM transforms on the FP64 matrix cores) for ensemble_512, fp32_ir for "
+    p.add_argument("--precision", default=None, choices=["fp64", "fp64_fp32c", "fp32_ir", "bf16_ir"],
+                   help="default: fp64_fp32c (FDM: first application on the FP64 matrix cores, corrections on "
+                        "the FP32 ones) for ensemble_512, fp32_ir for "
                         "qg2_stream, fp64 for helmholtz_*")
     p.add_argument("--pc", default="default", choices=["default", "jacobi", "fdm"],
                    help="preconditioner: default = FDM for ensemble_512 and the mixed precisions, Jacobi (the "
@@ -224,7 +235,7 @@ def run_ensemble(a, prec_name):
 
     import se_amd
     from se_amd import workloads as W
-    prec = {"fp64": se_amd.PREC_FP64, "fp32_ir": se_amd.PREC_FP32_IR, "bf16_ir": se_amd.PREC_BF16_IR}[prec_name]
+    prec = prec_code(prec_name)
     pc = pc_code(a, prec_name, fdm_default=True)
     cfgs = W.ensemble_512(count=a.members, detj_mode=1 if a.detj == "physical" else 0)
     ctx = se_amd.Context(0)
@@ -256,7 +267,7 @@ def run_ensemble(a, prec_name):
     ok = all(i["status"] == 0 and i["true_rel_residual"] <= 1e-10 for r in res for i in r)
     its = np.array([i["iters"] for i in infos])
     value = ntot / (ms * 1e-3) / 1e9
-    T = {se_amd.PREC_FP64: "double", se_amd.PREC_FP32_IR: "float", se_amd.PREC_BF16_IR: "bf16"}[prec]
+    T = GEMM_T[prec_name]
     if pc == se_amd.PC_FDM:
         # dominant kernel: the FDM transforms on the matrix cores (four passes
         # over all members, timed back to back after the last solve)
@@ -297,6 +308,7 @@ def run_ensemble(a, prec_name):
         # precision) and the preconditioner (the inner Jacobi-PCG mixed solve)
         legs = {}
         for lab, p2, c2 in (("fp64_fdm_batched", se_amd.PREC_FP64, se_amd.PC_FDM),
+                            ("fp64_fp32c_fdm_batched", se_amd.PREC_FP64_FP32C, se_amd.PC_FDM),
                             ("fp32_fdm_batched", se_amd.PREC_FP32_IR, se_amd.PC_FDM),
                             ("bf16_fdm_batched", se_amd.PREC_BF16_IR, se_amd.PC_FDM),
                             ("fp32_jacobi_batched", se_amd.PREC_FP32_IR, se_amd.PC_JACOBI)):
@@ -365,7 +377,7 @@ def run_single(a, prec_name):
     ne = int(parts[1]) if len(parts) > 1 else 128
     N = int(parts[2][1:]) if len(parts) > 2 else 8
     cfg = W.helmholtz_128(nEls=ne, N=N, detj_mode=1 if a.detj == "physical" else 0)
-    prec = {"fp64": se_amd.PREC_FP64, "fp32_ir": se_amd.PREC_FP32_IR, "bf16_ir": se_amd.PREC_BF16_IR}[prec_name]
+    prec = prec_code(prec_name)
     pc = pc_code(a, prec_name)
     ctx = se_amd.Context(0)
     m = se_amd.Mesh(ctx, cfg["nEls"], cfg["N"], cfg["lo"], cfg["hi"], cfg["periodic"], cfg["bc"])
@@ -396,7 +408,7 @@ def run_single(a, prec_name):
               "detj": a.detj, "rtol": 1e-10, "iters": info["iters"],
               "outer_iters": info["outer_iters"], "true_rel_residual": info["true_rel_residual"],
               "all_converged": ok, "n_iter_per_s": m.n * info["iters"] / (ms * 1e-3), "wall_s_timed": t_wall}
-    T = {se_amd.PREC_FP64: "double", se_amd.PREC_FP32_IR: "float", se_amd.PREC_BF16_IR: "bf16"}[prec]
+    T = GEMM_T[prec_name]
     if pc == se_amd.PC_FDM:
         roof = fdm_roofline(h.fdm_probe(reps=a.probe_reps), prec_name, "k_fdm_gemm<%s>" % T)
     elif prec == se_amd.PREC_FP64:
@@ -521,7 +533,7 @@ def stream_summary(r):
 def run_stream(a, prec_name):
     """Served two-layer QG step requests arriving as a Poisson process."""
     import se_amd
-    prec = {"fp64": se_amd.PREC_FP64, "fp32_ir": se_amd.PREC_FP32_IR, "bf16_ir": se_amd.PREC_BF16_IR}[prec_name]
+    prec = prec_code(prec_name)
     pc = pc_code(a, prec_name)
     reqs = stream_requests(a.requests)
     for _ in range(a.warmup):
@@ -551,8 +563,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if a.workload.startswith("ensemble"):
-        prec = a.precision or "fp64"
-        if prec == "fp64" and a.pc == "jacobi":
+        prec = a.precision or "fp64_fp32c"
+        if prec.startswith("fp64") and a.pc == "jacobi":
             raise SystemExit("batched FP64 solves take the FDM preconditioner (--pc fdm); the plain FP64 "
                              "Jacobi path is timed member by member inside every ensemble line")
         out = run_ensemble(a, prec)

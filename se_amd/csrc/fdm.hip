@@ -389,22 +389,30 @@ struct FbMember {
     double* ps;  // [nch]      sum of sgn f
     double* xo;  //            the caller's solution vector
     void* rp;    //            r in the transforms' type T, padded: pass 1's operand
+    void* rpc;   //            the same for the corrections' transforms (after an update)
     CgState* st;
     int n, nch, ntz, null_space;
     int fused;  // p.q and the true residual from the apply's epilogue (no SVV term)
     int nap;    // partials of p.q and of the true residual: apply tiles (fused), else nch
     int nU0, nP0, rprec;  //   unknown-grid row length, padded row length, T (FDM_F64 / F32 / BF16)
+    int rcprec;           //   T of the corrections' transforms
     double sgn, rtol;
 };
 
 // r[i] into the packed residual (row i / nU0, column i mod nU0), rounded to T
 // as the pack kernel rounds it
-__device__ __forceinline__ void put_packed(const FbMember& M, int i, double v) {
+__device__ __forceinline__ void put_packed_to(const FbMember& M, void* rp, int prec, int i, double v) {
     const int J = i / M.nU0, I = i - J * M.nU0;
     const size_t k = (size_t)J * M.nP0 + I;
-    if (M.rprec == FDM_F64) ((SE_GLOBAL double*)M.rp)[k] = v;
-    else if (M.rprec == FDM_F32) ((SE_GLOBAL float*)M.rp)[k] = to_t<float>(v);
-    else ((SE_GLOBAL __bf16*)M.rp)[k] = to_t<__bf16>(v);
+    if (prec == FDM_F64) ((SE_GLOBAL double*)rp)[k] = v;
+    else if (prec == FDM_F32) ((SE_GLOBAL float*)rp)[k] = to_t<float>(v);
+    else ((SE_GLOBAL __bf16*)rp)[k] = to_t<__bf16>(v);
+}
+// for the first (or a restarted) direction's transforms
+__device__ __forceinline__ void put_packed(const FbMember& M, int i, double v) { put_packed_to(M, M.rp, M.rprec, i, v); }
+// for a correction's transforms (the residual of an update)
+__device__ __forceinline__ void put_packed_c(const FbMember& M, int i, double v) {
+    put_packed_to(M, M.rpc, M.rcprec, i, v);
 }
 
 __device__ __forceinline__ void chunk_range(const FbMember& M, int ch, int& i0, int& i1) {
@@ -608,7 +616,7 @@ __global__ __launch_bounds__(FB_NT) void k_fb_update(const FbMember* __restrict_
             M.x[i] = first ? alpha * M.p[i] : M.x[i] + alpha * M.p[i];
             const double ri = M.r[i] - alpha * M.q[i];
             M.r[i] = ri;
-            put_packed(M, i, ri);
+            put_packed_c(M, i, ri);
             rr += ri * ri;
         }
         const double v[1] = {rr};
@@ -1029,8 +1037,12 @@ int launch_gemm(int prec, const GemmTask* tasks, int pass, const int2* work, int
 }
 
 int fdm_prec_of(int precision) {
-    return precision == SE_PREC_FP64 ? FDM_F64 : (precision == SE_PREC_FP32_IR ? FDM_F32 : FDM_BF16);
+    if (precision == SE_PREC_FP64 || precision == SE_PREC_FP64_FP32C) return FDM_F64;
+    return precision == SE_PREC_FP32_IR ? FDM_F32 : FDM_BF16;
 }
+// the transforms' precision for the corrections (the applications after the
+// first, whose input is a residual the first application left)
+int fdm_cprec_of(int precision) { return precision == SE_PREC_FP64_FP32C ? FDM_F32 : fdm_prec_of(precision); }
 
 // ---------------------------------------------------------------------------
 // host: the lockstep FDM-preconditioned CG over a set of operators
@@ -1038,8 +1050,9 @@ int fdm_prec_of(int precision) {
 struct FdmPlan {
     se_ctx* ctx = nullptr;
     std::vector<se_op*> ops;
-    int prec = -1;
+    int prec = -1, cprec = -1;  // the transforms' precision: first application, corrections
     DevBuf d_members, d_tasks;  // FbMember[count], GemmTask[4 count]
+    DevBuf d_tasks_c;           // GemmTask[4 count] of the corrections (cprec != prec)
     std::vector<FbMember> h_members;
     // FP64 apply descriptors per order (p -> q and x -> q)
     struct Order {
@@ -1315,18 +1328,20 @@ int grid_of(size_t items, const se_ctx* c) {
 }
 
 // one pass of the transforms of the listed members
-int plan_pass(FdmPlan* pl, int set, int pass, hipStream_t s, int skip = 0, bool to_p = false) {
+int plan_pass(FdmPlan* pl, int set, int pass, hipStream_t s, int skip = 0, bool to_p = false, bool corr = false) {
     const FdmPlan::Lists& L = *pl->sets[set].cur;
-    return launch_gemm(pl->prec, pl->d_tasks.as<GemmTask>(), pass, L.gwork, L.n_gwork, s, skip, false,
-                       to_p && pass == 3);
+    const bool c = corr && pl->cprec != pl->prec;
+    return launch_gemm(c ? pl->cprec : pl->prec, (c ? pl->d_tasks_c : pl->d_tasks).as<GemmTask>(), pass, L.gwork,
+                       L.n_gwork, s, skip, false, to_p && pass == 3);
 }
 
 // z = FDM(r) of the listed members: four passes
 
-// (to_p: into the CG's p, for a first or restarted direction)
-int plan_fdm(FdmPlan* pl, int set, hipStream_t s, int skip = 0, bool to_p = false) {
+// (to_p: into the CG's p, for a first or restarted direction; corr: in the
+// corrections' precision, for the residual of an update)
+int plan_fdm(FdmPlan* pl, int set, hipStream_t s, int skip = 0, bool to_p = false, bool corr = false) {
     for (int pass = 0; pass < 4; pass++) {
-        int rc = plan_pass(pl, set, pass, s, skip, to_p);
+        int rc = plan_pass(pl, set, pass, s, skip, to_p, corr);
         if (rc) return rc;
     }
     return SE_OK;
@@ -1361,6 +1376,7 @@ int fdm_plan_set_members(FdmPlan* pl, int count, se_op* const* ops) {
     if (!rc) rc = pl->d_tasks.alloc(sizeof(GemmTask) * 4 * std::max(count, 1));
     if (rc) return rc;
     pl->prec = -1;
+    pl->cprec = -1;
     pl->desc_ok = false;
     for (auto& L : pl->sets) L.clear();
     return SE_OK;
@@ -1425,16 +1441,18 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
     se_ctx* c = pl->ctx;
     hipStream_t s = c->stream;
     const int count = (int)pl->ops.size();
-    const int prec = fdm_prec_of(o.precision);
+    const int prec = fdm_prec_of(o.precision), cprec = fdm_cprec_of(o.precision);
     const int null = o.null_space ? 1 : 0;
     for (int i = 0; i < count; i++) {
         int rc = fdm_prepare(pl->ops[i], prec);
+        if (!rc && cprec != prec) rc = fdm_prepare(pl->ops[i], cprec);
         if (rc) return rc;
     }
     // descriptors: built once per members / precision / null flag / tolerance;
     // a later solve patches only the right-hand sides and the results
     int rc = SE_OK;
-    const bool reuse = pl->desc_ok && pl->prec == prec && pl->desc_null == null && pl->desc_rtol == o.rtol;
+    const bool reuse = pl->desc_ok && pl->prec == prec && pl->cprec == cprec && pl->desc_null == null &&
+                       pl->desc_rtol == o.rtol;
     if (reuse) {
         bool changed = false, moved = false;
         for (int i = 0; i < count; i++) {
@@ -1461,7 +1479,11 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
             if (rc) return rc;
         }
         pl->h_members.assign(count, FbMember{});
-        std::vector<GemmTask> tasks(4 * (size_t)count);
+        std::vector<GemmTask> tasks(4 * (size_t)count), tasks_c(cprec != prec ? 4 * (size_t)count : 0);
+        if (cprec != prec) {
+            int e = pl->d_tasks_c.alloc(sizeof(GemmTask) * 4 * std::max(count, 1));
+            if (e) return e;
+        }
         pl->any_svv = false;
         for (se_op* op : pl->ops) pl->any_svv = pl->any_svv || op->svv_mN != 0;
         for (int i = 0; i < count; i++) {
@@ -1482,9 +1504,11 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
             M.ps = F->ps.as<double>();
             M.xo = d_x[i];
             M.rp = F->prec[prec].rp.ptr;
+            M.rpc = F->prec[cprec].rp.ptr;
             M.nU0 = F->nU0;
             M.nP0 = F->nP0;
             M.rprec = prec;
+            M.rcprec = cprec;
             M.n = op->mesh->n;
             SE_CHECK(M.n == F->nU0 * F->nU1, SE_ERR_STATE, "FDM: the unknowns are not the nU1 x nU0 grid");
             M.nch = (std::max(M.n, 1) + FB_CH - 1) / FB_CH;
@@ -1505,14 +1529,30 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
                 t4[k].nch = M.nch;
                 tasks[4 * (size_t)i + k] = t4[k];
             }
+            if (cprec != prec) {
+                GemmTask tc[4];
+                fdm_tasks(op, cprec, M.r, M.z, M.pz, tc);
+                SE_CHECK(fdm_tasks_valid(tc, cprec), SE_ERR_STATE,
+                         "FDM: transform shapes break the kernels' assumptions");
+                for (int k = 0; k < 4; k++) {
+                    tc[k].pv = M.pv;
+                    tc[k].st = M.st;
+                    tc[k].nch = M.nch;
+                    tasks_c[4 * (size_t)i + k] = tc[k];
+                }
+            }
         }
         int rc = SE_OK;
         if (count > 0) {
             SE_HIP(hipMemcpyAsync(pl->d_members.ptr, pl->h_members.data(), sizeof(FbMember) * count, hipMemcpyHostToDevice,
                                   s));
             SE_HIP(hipMemcpyAsync(pl->d_tasks.ptr, tasks.data(), sizeof(GemmTask) * 4 * count, hipMemcpyHostToDevice, s));
+            if (cprec != prec)
+                SE_HIP(hipMemcpyAsync(pl->d_tasks_c.ptr, tasks_c.data(), sizeof(GemmTask) * 4 * count,
+                                      hipMemcpyHostToDevice, s));
         }
         pl->prec = prec;
+        pl->cprec = cprec;
         // FP64 apply descriptors
         rc = upload_apply_descriptors(pl, s);
         if (rc) return rc;
@@ -1598,7 +1638,7 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
         if (rc) return rc;
         if (pl->any_svv) vlaunch(IT, k_fb_dot);
         vlaunch(IT, k_fb_update);
-        rc = plan_fdm(pl, IT, s, 1);
+        rc = plan_fdm(pl, IT, s, 1, false, true);
         if (rc) return rc;
         dlaunch(IT, FB_NORMAL);
         rc = true_residual(IT, on, true);
@@ -1748,6 +1788,7 @@ int fdm_plan_probe(FdmPlan* pl, int reps, double* us_pass, double* flops_pass, d
 int pcg_fdm(se_op* op, const double* d_f, double* d_x, const se_solve_opts& o, se_solve_info& info) {
     SE_CHECK(op->definite, SE_ERR_UNSUPPORTED, "operator diagonal changes sign: not a definite system");
     int rc = fdm_prepare(op, fdm_prec_of(o.precision));
+    if (!rc && fdm_cprec_of(o.precision) != fdm_prec_of(o.precision)) rc = fdm_prepare(op, fdm_cprec_of(o.precision));
     if (rc) return rc;
     if (!op->fdm->self) {
         rc = fdm_plan_create(op->mesh->ctx, 1, &op, &op->fdm->self);

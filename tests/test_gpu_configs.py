@@ -1,8 +1,8 @@
 """The benchmark configurations at their own sizes on the GPU.
 
 * The 512-member Helmholtz ensemble (nEls ~ U{8,16,32,64}^2, p ~ U{4,6,8},
-  lambda log-uniform in [1, 1e3]) solved as one batch, with FP64 and with
-  FP32 preconditioner transforms: every member meets the
+  lambda log-uniform in [1, 1e3]) solved as one batch, with FP64, FP64 then
+  FP32 (corrections), and FP32 preconditioner transforms: every member meets the
   1e-10 true-residual target, measured with the exact FP64 operator apply
   (pinned at 1e-12 against the oracle's assembled CSR, a restatement of
   Matrix::Assemble, src/base/Matrix.cpp:47-133), and the small members also
@@ -26,7 +26,8 @@ def rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-@pytest.mark.parametrize("prec", [se_amd.PREC_FP64, se_amd.PREC_FP32_IR], ids=["fp64", "fp32_ir"])
+@pytest.mark.parametrize("prec", [se_amd.PREC_FP64, se_amd.PREC_FP64_FP32C, se_amd.PREC_FP32_IR],
+                         ids=["fp64", "fp64_fp32c", "fp32_ir"])
 def test_ensemble_512_full_batch(gpu_ctx, prec):
     cfgs = W.ensemble_512(count=512, detj_mode=1)
     ms, hs, fs = [], [], []

@@ -23,6 +23,9 @@ from test_gpu_parity import OP_CASES, rel
 pytestmark = pytest.mark.gpu
 
 PRECS = [(se_amd.PREC_FP64, "fp64"), (se_amd.PREC_FP32_IR, "fp32"), (se_amd.PREC_BF16_IR, "bf16")]
+# solves also in FP64_FP32C: FP64 transforms for the first application, FP32
+# ones for the corrections
+SOLVE_PRECS = PRECS + [(se_amd.PREC_FP64_FP32C, "fp64_fp32c")]
 # ||sign*A z - r|| / ||r|| of one FDM application: the separable form drops
 # round-off-sized Jacobian cross terms (FP64), then the transforms' rounding
 # (FP32: 2^-24 per operand and sum; BF16 operands: 2^-9)
@@ -30,7 +33,7 @@ PRECS = [(se_amd.PREC_FP64, "fp64"), (se_amd.PREC_FP32_IR, "fp32"), (se_amd.PREC
 # (BF16: up to ~0.3 measured)
 INV_TOL = {se_amd.PREC_FP64: 1e-9, se_amd.PREC_FP32_IR: 1e-4, se_amd.PREC_BF16_IR: 0.5}
 # FDM-preconditioned CG iterations to 1e-10
-ITER_MAX = {se_amd.PREC_FP64: 3, se_amd.PREC_FP32_IR: 6, se_amd.PREC_BF16_IR: 20}
+ITER_MAX = {se_amd.PREC_FP64: 3, se_amd.PREC_FP32_IR: 6, se_amd.PREC_BF16_IR: 20, se_amd.PREC_FP64_FP32C: 6}
 # Near-singular system: periodic literal-mode Laplacian with c_mass = 1 (the
 # constant mode's eigenvalue is ~1e-5 of the spectrum).  Its huge constant
 # component swamps the other modes in low precision: the FP32 transforms
@@ -68,7 +71,7 @@ def test_fdm_inverts_the_assembled_operator(case, prec):
     assert res <= tol, res
 
 
-@pytest.mark.parametrize("prec", [p for p, _ in PRECS], ids=[n for _, n in PRECS])
+@pytest.mark.parametrize("prec", [p for p, _ in SOLVE_PRECS], ids=[n for _, n in SOLVE_PRECS])
 def test_fdm_pcg_meets_fp64_target(case, prec):
     oh, h, n, null = case["oh"], case["h"], case["m"].n, case["null"]
     f = np.random.default_rng(19).standard_normal(n)
@@ -103,7 +106,7 @@ def test_fdm_batch_equals_single_solves(gpu_ctx):
     rng = np.random.default_rng(23)
     fs = [rng.standard_normal(m.n) for _, m, _ in mem]
     b = se_amd.Batch(gpu_ctx, [h for _, _, h in mem])
-    for prec, _ in PRECS:
+    for prec, _ in SOLVE_PRECS:
         single = [h.pcg(f, rtol=1e-10, precision=prec, preconditioner=se_amd.PC_FDM) for (_, _, h), f in zip(mem, fs)]
         xs, infos = b.pcg(fs, precision=prec, rtol=1e-10, preconditioner=se_amd.PC_FDM)
         for (lab, _, _), (x1, i1), xb, ib in zip(mem, single, xs, infos):
@@ -160,7 +163,7 @@ def test_fdm_large_tiles_invert_and_batch_with_small_ones(gpu_ctx):
     rng = np.random.default_rng(29)
     fs = [rng.standard_normal(mm.n) for _, mm, _ in mem]
     b = se_amd.Batch(gpu_ctx, [hh for _, _, hh in mem])
-    for prec, _ in PRECS:
+    for prec, _ in SOLVE_PRECS:
         xs, infos = b.pcg(fs, precision=prec, rtol=1e-10, preconditioner=se_amd.PC_FDM)
         for (ohh, _, hh), f, xb, ib in zip(mem, fs, xs, infos):
             assert ib["status"] == se_amd.SOLVE_CONVERGED, ib
@@ -195,7 +198,7 @@ def test_fdm_full_size(gpu_ctx):
     f = rng.standard_normal(m.n)
     fs = rng.standard_normal(ms.n)
     b = se_amd.Batch(gpu_ctx, [h, hs])
-    for prec, _ in PRECS:
+    for prec, _ in SOLVE_PRECS:
         x1, i1 = h.pcg(f, rtol=1e-10, precision=prec, preconditioner=se_amd.PC_FDM)
         assert i1["status"] == se_amd.SOLVE_CONVERGED, i1
         ax = h.apply(x1)

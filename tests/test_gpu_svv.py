@@ -65,7 +65,8 @@ def test_svv_solve_matches_oracle(gpu_ctx, case):
 
 # FDM of the operator without its SVV term preconditions the exact operator
 # (kappa(A0^-1 A) 1.02-2.45 on these cases, measured with the oracle's CSRs)
-FDM_PRECS = [(se_amd.PREC_FP64, 30), (se_amd.PREC_FP32_IR, 30), (se_amd.PREC_BF16_IR, 150)]
+FDM_PRECS = [(se_amd.PREC_FP64, 30), (se_amd.PREC_FP32_IR, 30), (se_amd.PREC_BF16_IR, 150),
+             (se_amd.PREC_FP64_FP32C, 30)]
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "N%d-mN%d-per%d%d" % (c[1], c[4], *c[2]))
