@@ -8,9 +8,10 @@ Workloads (--workload):
       right-hand sides f = M g with g a seeded random field with an isotropic
       k^-3 spectrum.  One step = all 512 solves to a true relative residual of
       1e-10, batched: FP64 CG preconditioned by fast diagonalisation with its
-      transforms on the matrix cores (--precision fp64_fp32c, the default:
-      FP64 transforms for the first application, FP32 for the corrections;
-      fp64, fp32_ir, bf16_ir: FP64, FP32 or BF16 operands throughout;
+      transforms on the matrix cores (--precision fp64_bf16c, the default:
+      FP64 transforms for the first application, BF16 operands for the
+      corrections; fp64_fp32c: FP32 corrections; fp64, fp32_ir, bf16_ir:
+      FP64, FP32 or BF16 operands throughout;
       --pc jacobi: the reference's Jacobi-PCG with FP32 or BF16 inner
       iterations and FP64 reliable updates).  The plain FP64 GPU path
       (each member alone, FP64 Jacobi-PCG, the reference's algorithm) is timed
@@ -45,16 +46,17 @@ METRIC = "Helmholtz CG GDOF/s at 1e-10 rel. residual vs FP64; p50/p99 latency un
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector
 # MI355X dense matrix-core peaks by operand type (TFLOP/s)
-MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp64_fp32c": 78.6, "fp32_ir": 157.3, "bf16_ir": 2500.0}
-DTYPE = {"fp64": "f64", "fp64_fp32c": "f64+f32", "fp32_ir": "f32+f64", "bf16_ir": "bf16+f64"}
-# the element type of the probed GEMM (for fp64_fp32c: the first application's, FP64)
-GEMM_T = {"fp64": "double", "fp64_fp32c": "double", "fp32_ir": "float", "bf16_ir": "bf16"}
+MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp64_fp32c": 78.6, "fp64_bf16c": 78.6, "fp32_ir": 157.3, "bf16_ir": 2500.0}
+DTYPE = {"fp64": "f64", "fp64_fp32c": "f64+f32", "fp64_bf16c": "f64+bf16", "fp32_ir": "f32+f64",
+         "bf16_ir": "bf16+f64"}
+# the element type of the probed GEMM (for fp64_*c: the first application's, FP64)
+GEMM_T = {"fp64": "double", "fp64_fp32c": "double", "fp64_bf16c": "double", "fp32_ir": "float", "bf16_ir": "bf16"}
 
 
 def prec_code(name):
     import se_amd
-    return {"fp64": se_amd.PREC_FP64, "fp64_fp32c": se_amd.PREC_FP64_FP32C, "fp32_ir": se_amd.PREC_FP32_IR,
-            "bf16_ir": se_amd.PREC_BF16_IR}[name]
+    return {"fp64": se_amd.PREC_FP64, "fp64_fp32c": se_amd.PREC_FP64_FP32C, "fp64_bf16c": se_amd.PREC_FP64_BF16C,
+            "fp32_ir": se_amd.PREC_FP32_IR, "bf16_ir": se_amd.PREC_BF16_IR}[name]
 
 
 def parse():
@@ -63,9 +65,9 @@ def parse():
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--workload", default="ensemble_512")
-    p.add_argument("--precision", default=None, choices=["fp64", "fp64_fp32c", "fp32_ir", "bf16_ir"],
-                   help="default: fp64_fp32c (FDM: first application on the FP64 matrix cores, corrections on "
-                        "the FP32 ones) for ensemble_512, fp32_ir for "
+    p.add_argument("--precision", default=None, choices=["fp64", "fp64_fp32c", "fp64_bf16c", "fp32_ir", "bf16_ir"],
+                   help="default: fp64_bf16c (FDM: first application on the FP64 matrix cores, corrections "
+                        "with BF16 operands) for ensemble_512, fp32_ir for "
                         "qg2_stream, fp64 for helmholtz_*")
     p.add_argument("--pc", default="default", choices=["default", "jacobi", "fdm"],
                    help="preconditioner: default = FDM for ensemble_512 and the mixed precisions, Jacobi (the "
@@ -309,6 +311,7 @@ def run_ensemble(a, prec_name):
         legs = {}
         for lab, p2, c2 in (("fp64_fdm_batched", se_amd.PREC_FP64, se_amd.PC_FDM),
                             ("fp64_fp32c_fdm_batched", se_amd.PREC_FP64_FP32C, se_amd.PC_FDM),
+                            ("fp64_bf16c_fdm_batched", se_amd.PREC_FP64_BF16C, se_amd.PC_FDM),
                             ("fp32_fdm_batched", se_amd.PREC_FP32_IR, se_amd.PC_FDM),
                             ("bf16_fdm_batched", se_amd.PREC_BF16_IR, se_amd.PC_FDM),
                             ("fp32_jacobi_batched", se_amd.PREC_FP32_IR, se_amd.PC_JACOBI)):
@@ -563,7 +566,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if a.workload.startswith("ensemble"):
-        prec = a.precision or "fp64_fp32c"
+        prec = a.precision or "fp64_bf16c"
         if prec.startswith("fp64") and a.pc == "jacobi":
             raise SystemExit("batched FP64 solves take the FDM preconditioner (--pc fdm); the plain FP64 "
                              "Jacobi path is timed member by member inside every ensemble line")

@@ -46,11 +46,12 @@ enum { SE_DETJ_LITERAL = 0, SE_DETJ_PHYSICAL = 1 };
  * fast-diagonalisation preconditioner's transforms) in FP32, resp. with BF16
  * operands (Jacobi: BF16 search direction, FP32 VALU arithmetic; FDM: BF16
  * inputs to the matrix cores with FP32 accumulation), inside an FP64 loop
- * whose residual, norms and update are FP64.  FP64_FP32C (FDM preconditioner
- * only): FP64 CG whose preconditioner's first application -- the one that
- * sets the residual's floor -- runs in FP64 and later ones, corrections of
- * the residual it left, in FP32. */
-enum { SE_PREC_FP64 = 0, SE_PREC_FP32_IR = 1, SE_PREC_BF16_IR = 2, SE_PREC_FP64_FP32C = 3 };
+ * whose residual, norms and update are FP64.  FP64_FP32C / FP64_BF16C (FDM
+ * preconditioner only): FP64 CG whose preconditioner's first application --
+ * the one that sets the residual's floor -- runs in FP64 and later ones,
+ * corrections of the residual it left, in FP32, resp. with BF16 operands
+ * (which, like BF16_IR, may stagnate on a near-singular system). */
+enum { SE_PREC_FP64 = 0, SE_PREC_FP32_IR = 1, SE_PREC_BF16_IR = 2, SE_PREC_FP64_FP32C = 3, SE_PREC_FP64_BF16C = 4 };
 
 /* Preconditioner.  JACOBI: the reference's (PETSc -pc_type jacobi; FP64) or
  * the inner Jacobi-PCG with FP64 reliable updates (FP32_IR / BF16_IR).

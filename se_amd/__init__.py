@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libse_amd.so")
 
 DETJ_LITERAL, DETJ_PHYSICAL = 0, 1
-PREC_FP64, PREC_FP32_IR, PREC_BF16_IR, PREC_FP64_FP32C = 0, 1, 2, 3
+PREC_FP64, PREC_FP32_IR, PREC_BF16_IR, PREC_FP64_FP32C, PREC_FP64_BF16C = 0, 1, 2, 3, 4
 PC_DEFAULT, PC_JACOBI, PC_FDM = 0, 1, 2
 SL_NOALLOC_MAX_NODES = 524288  # SE_SL_NOALLOC_MAX_NODES of include/se_amd.h
 INTERP_DEFAULT, INTERP_GLL, INTERP_BICUBIC = 0, 1, 2

@@ -659,7 +659,8 @@ extern "C" int se_batch_pcg_dev(se_batch* b, const double* const* d_f, double* c
              "se_batch_pcg_dev: bad preconditioner");
     const bool fdm = solve_pc(o) == SE_PC_FDM;
     SE_CHECK(o.precision == SE_PREC_FP32_IR || o.precision == SE_PREC_BF16_IR ||
-                 (fdm && (o.precision == SE_PREC_FP64 || o.precision == SE_PREC_FP64_FP32C)),
+                 (fdm && (o.precision == SE_PREC_FP64 || o.precision == SE_PREC_FP64_FP32C ||
+                          o.precision == SE_PREC_FP64_BF16C)),
              SE_ERR_UNSUPPORTED,
              "se_batch_pcg_dev: batched Jacobi solves are mixed precision (SE_PREC_FP32_IR or SE_PREC_BF16_IR); "
              "FP64 batches take the FDM preconditioner");

@@ -1037,12 +1037,17 @@ int launch_gemm(int prec, const GemmTask* tasks, int pass, const int2* work, int
 }
 
 int fdm_prec_of(int precision) {
-    if (precision == SE_PREC_FP64 || precision == SE_PREC_FP64_FP32C) return FDM_F64;
+    if (precision == SE_PREC_FP64 || precision == SE_PREC_FP64_FP32C || precision == SE_PREC_FP64_BF16C)
+        return FDM_F64;
     return precision == SE_PREC_FP32_IR ? FDM_F32 : FDM_BF16;
 }
 // the transforms' precision for the corrections (the applications after the
 // first, whose input is a residual the first application left)
-int fdm_cprec_of(int precision) { return precision == SE_PREC_FP64_FP32C ? FDM_F32 : fdm_prec_of(precision); }
+int fdm_cprec_of(int precision) {
+    if (precision == SE_PREC_FP64_FP32C) return FDM_F32;
+    if (precision == SE_PREC_FP64_BF16C) return FDM_BF16;
+    return fdm_prec_of(precision);
+}
 
 // ---------------------------------------------------------------------------
 // host: the lockstep FDM-preconditioned CG over a set of operators

@@ -883,12 +883,12 @@ extern "C" int se_helm_pcg_dev(se_op* op, const double* d_f, double* d_x, const 
     SE_HIP(hipSetDevice(op->mesh->ctx->device));
     int rc;
     SE_CHECK(o.precision == SE_PREC_FP64 || o.precision == SE_PREC_FP32_IR || o.precision == SE_PREC_BF16_IR ||
-                 o.precision == SE_PREC_FP64_FP32C,
+                 o.precision == SE_PREC_FP64_FP32C || o.precision == SE_PREC_FP64_BF16C,
              SE_ERR_ARG, "se_helm_pcg_dev: bad precision");
     SE_CHECK(o.preconditioner >= SE_PC_DEFAULT && o.preconditioner <= SE_PC_FDM, SE_ERR_ARG,
              "se_helm_pcg_dev: bad preconditioner");
-    SE_CHECK(o.precision != SE_PREC_FP64_FP32C || solve_pc(o) == SE_PC_FDM, SE_ERR_UNSUPPORTED,
-             "se_helm_pcg_dev: SE_PREC_FP64_FP32C takes the FDM preconditioner");
+    SE_CHECK((o.precision != SE_PREC_FP64_FP32C && o.precision != SE_PREC_FP64_BF16C) || solve_pc(o) == SE_PC_FDM,
+             SE_ERR_UNSUPPORTED, "se_helm_pcg_dev: SE_PREC_FP64_FP32C / FP64_BF16C take the FDM preconditioner");
     if (solve_pc(o) == SE_PC_FDM) rc = pcg_fdm(op, d_f, d_x, o, inf);
     else if (o.precision == SE_PREC_FP64) rc = pcg_fp64(op, d_f, d_x, o, inf);
     else rc = pcg_mixed(op, d_f, d_x, o, inf);
