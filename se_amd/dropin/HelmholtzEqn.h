@@ -7,8 +7,9 @@
 // and writes x back into solField (HelmholtzEqn.cpp:105-136).  Fields of 1 or
 // 2 dofs (a 2-dof field solves one operator per dof, with that dof's Dirichlet
 // sides).  SE_PRECISION=fp32 or bf16 selects the mixed-precision solver
-// (operators without SVV).  Include after the reference's Field.h, like the
-// original.
+// (operators without SVV); fp64_fp32c / fp64_bf16c the FDM-preconditioned CG
+// with an FP64 first application and FP32 / BF16 corrections.  Include after
+// the reference's Field.h, like the original.
 #ifndef SE_DROPIN_HELMHOLTZ_EQN_H
 #define SE_DROPIN_HELMHOLTZ_EQN_H
 #include <string>

@@ -102,6 +102,8 @@ se_solve_opts solve_opts(bool null_space) {
         const std::string s(p);
         if (s == "fp32") o.precision = SE_PREC_FP32_IR;
         if (s == "bf16") o.precision = SE_PREC_BF16_IR;
+        if (s == "fp64_fp32c") o.precision = SE_PREC_FP64_FP32C;
+        if (s == "fp64_bf16c") o.precision = SE_PREC_FP64_BF16C;
     }
     if (const char* p = getenv("SE_PRECONDITIONER")) {
         const std::string s(p);
