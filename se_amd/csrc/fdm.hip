@@ -78,9 +78,13 @@ struct GemmTask {
     const CgState* st;
     int nch;
     // FINAL, launched for a CG's first or restarted direction: z goes straight
-    // to the CG's p (its direction, p = z - mean z), not to z
+    // to the CG's p (its direction, p = z - mean z), not to z; for the first,
+    // the residual of the partials is b (r = b is not stored)
     double* C2;
+    const double* b;
 };
+// where pass 4 of a CG's transforms writes: z, or p (restart: from r; start: from b)
+enum { FDM_TO_Z = 0, FDM_TO_P = 1, FDM_TO_P_FROM_B = 2 };
 
 // Output tiles are 64 x 64, four waves of 32 x 32 each.
 constexpr int FDM_BT = 64;
@@ -238,7 +242,7 @@ struct ChunkCopy {
 __device__ __forceinline__ void wait_copies() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 template <typename T>
-__device__ __forceinline__ void gemm_tile(const GemmTask& t, int tile, char* S0, char* S1, bool to_p) {
+__device__ __forceinline__ void gemm_tile(const GemmTask& t, int tile, char* S0, char* S1, int to_p) {
     using C = GemmCfg<T>;
     constexpr int NB = Acc<T>::NB;
     const int tid = threadIdx.x, l = tid & 63;
@@ -301,8 +305,8 @@ __device__ __forceinline__ void gemm_tile(const GemmTask& t, int tile, char* S0,
             Cm[(size_t)gr * t.ldc + gc] = to_t<T>(v * s);
         });
     } else {  // FDM_OUT_FINAL: FP64 z on the unknown grid, partials of r.z, sum z, sum r
-        SE_GLOBAL double* Z = (SE_GLOBAL double*)(to_p ? t.C2 : t.C);
-        const SE_GLOBAL double* R = (const SE_GLOBAL double*)t.r;
+        SE_GLOBAL double* Z = (SE_GLOBAL double*)(to_p != FDM_TO_Z ? t.C2 : t.C);
+        const SE_GLOBAL double* R = (const SE_GLOBAL double*)(to_p == FDM_TO_P_FROM_B ? t.b : t.r);
         double rz = 0.0, sz = 0.0, sr = 0.0;
         for_each_acc<T>(acc, wr, wc, l, [&](int row, int col, double v) {
             const int gr = m0 + row, gc = n0 + col;
@@ -347,7 +351,7 @@ __global__ __launch_bounds__(256, 4) void k_fdm_gemm(const GemmTask* __restrict_
         if (it.x < 0) continue;  // a padding item of the XCD-ordered list
         const GemmTask t = tasks[it.x * 4 + pass];
         if (skip && fdm_skip(t)) continue;
-        gemm_tile<T>(t, it.y, S0, S1, to_p != 0);
+        gemm_tile<T>(t, it.y, S0, S1, to_p);
     }
 }
 
@@ -435,9 +439,10 @@ __global__ __launch_bounds__(FB_NT) void k_fb_sum(const FbMember* __restrict__ M
     }
 }
 
-// b = sgn f - mean (null space), r = b; partials of r.r.  (x = 0 is implicit:
-// the first update writes x = alpha p, and a member stopped before it gets
-// x = 0 from the first direction kernel or the update)
+// b = sgn f - mean (null space); partials of b.b.  (r = b and x = 0 are
+// implicit: the first update reads b for r and writes x = alpha p, and a
+// member stopped before it gets x = 0 from the first direction kernel or the
+// update)
 __global__ __launch_bounds__(FB_NT) void k_fb_start(const FbMember* __restrict__ Ms, const int2* __restrict__ work,
                                                     int nwork) {
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
@@ -456,7 +461,6 @@ __global__ __launch_bounds__(FB_NT) void k_fb_start(const FbMember* __restrict__
         for (int i = i0 + threadIdx.x; i < i1; i += FB_NT) {
             const double bi = M.sgn * M.f[i] - mu;
             M.b[i] = bi;
-            M.r[i] = bi;
             put_packed(M, i, bi);
             rr += bi * bi;
         }
@@ -614,7 +618,7 @@ __global__ __launch_bounds__(FB_NT) void k_fb_update(const FbMember* __restrict_
         double rr = 0.0;
         for (int i = i0 + threadIdx.x; i < i1; i += FB_NT) {
             M.x[i] = first ? alpha * M.p[i] : M.x[i] + alpha * M.p[i];
-            const double ri = M.r[i] - alpha * M.q[i];
+            const double ri = (first ? M.b[i] : M.r[i]) - alpha * M.q[i];
             M.r[i] = ri;
             put_packed_c(M, i, ri);
             rr += ri * ri;
@@ -1018,17 +1022,17 @@ int fdm_tiles(const se_op* op) { return (op->fdm->nP0 / FDM_BT) * (op->fdm->nP1 
 
 template <typename T>
 void launch_gemm_t(const GemmTask* tasks, int pass, const int2* work, int nwork, hipStream_t s, int skip, bool pack,
-                   bool to_p) {
+                   int to_p) {
     const dim3 grid(nwork), block(256);
     if (pass == 0 && pack) hipLaunchKernelGGL((k_fdm_pack<T>), grid, block, 0, s, tasks, work, nwork, skip);
-    hipLaunchKernelGGL((k_fdm_gemm<T>), grid, block, 0, s, tasks, pass, work, nwork, skip, to_p ? 1 : 0);
+    hipLaunchKernelGGL((k_fdm_gemm<T>), grid, block, 0, s, tasks, pass, work, nwork, skip, to_p);
 }
 
 // pass `pass` of the listed items; pass 1 first packs the residual when
 // `pack` (the CG kernels store it packed as they write r, so a solve does not);
-// to_p: the last pass writes the tasks' C2
+// to_p (FDM_TO_*): where the last pass writes
 int launch_gemm(int prec, const GemmTask* tasks, int pass, const int2* work, int nwork, hipStream_t s, int skip,
-                bool pack, bool to_p = false) {
+                bool pack, int to_p = FDM_TO_Z) {
     if (nwork <= 0) return SE_OK;
     if (prec == FDM_F64) launch_gemm_t<double>(tasks, pass, work, nwork, s, skip, pack, to_p);
     else if (prec == FDM_F32) launch_gemm_t<float>(tasks, pass, work, nwork, s, skip, pack, to_p);
@@ -1333,18 +1337,18 @@ int grid_of(size_t items, const se_ctx* c) {
 }
 
 // one pass of the transforms of the listed members
-int plan_pass(FdmPlan* pl, int set, int pass, hipStream_t s, int skip = 0, bool to_p = false, bool corr = false) {
+int plan_pass(FdmPlan* pl, int set, int pass, hipStream_t s, int skip = 0, int to_p = FDM_TO_Z, bool corr = false) {
     const FdmPlan::Lists& L = *pl->sets[set].cur;
     const bool c = corr && pl->cprec != pl->prec;
     return launch_gemm(c ? pl->cprec : pl->prec, (c ? pl->d_tasks_c : pl->d_tasks).as<GemmTask>(), pass, L.gwork,
-                       L.n_gwork, s, skip, false, to_p && pass == 3);
+                       L.n_gwork, s, skip, false, pass == 3 ? to_p : FDM_TO_Z);
 }
 
 // z = FDM(r) of the listed members: four passes
 
 // (to_p: into the CG's p, for a first or restarted direction; corr: in the
 // corrections' precision, for the residual of an update)
-int plan_fdm(FdmPlan* pl, int set, hipStream_t s, int skip = 0, bool to_p = false, bool corr = false) {
+int plan_fdm(FdmPlan* pl, int set, hipStream_t s, int skip = 0, int to_p = FDM_TO_Z, bool corr = false) {
     for (int pass = 0; pass < 4; pass++) {
         int rc = plan_pass(pl, set, pass, s, skip, to_p, corr);
         if (rc) return rc;
@@ -1528,6 +1532,7 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
             fdm_tasks(op, prec, M.r, M.z, M.pz, t4);
             SE_CHECK(fdm_tasks_valid(t4, prec), SE_ERR_STATE, "FDM: transform shapes break the kernels' assumptions");
             t4[3].C2 = M.p;
+            t4[3].b = M.b;
             for (int k = 0; k < 4; k++) {
                 t4[k].pv = M.pv;
                 t4[k].st = M.st;
@@ -1618,7 +1623,7 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
     if (rc) return rc;
     if (null) vlaunch(IT, k_fb_sum);
     vlaunch(IT, k_fb_start);
-    rc = plan_fdm(pl, IT, s, 0, true);
+    rc = plan_fdm(pl, IT, s, 0, FDM_TO_P_FROM_B);
     if (rc) return rc;
     dlaunch(IT, FB_FIRST);
     std::vector<int> status(count, CG_RUNNING);
@@ -1643,7 +1648,7 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
         if (rc) return rc;
         if (pl->any_svv) vlaunch(IT, k_fb_dot);
         vlaunch(IT, k_fb_update);
-        rc = plan_fdm(pl, IT, s, 1, false, true);
+        rc = plan_fdm(pl, IT, s, 1, FDM_TO_Z, true);
         if (rc) return rc;
         dlaunch(IT, FB_NORMAL);
         rc = true_residual(IT, on, true);
@@ -1688,7 +1693,7 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
         rc = plan_lists(pl, OT, again, s);
         if (rc) return rc;
         vlaunch(OT, k_fb_restart);
-        rc = plan_fdm(pl, OT, s, 0, true);
+        rc = plan_fdm(pl, OT, s, 0, FDM_TO_P);
         if (rc) return rc;
         dlaunch(OT, FB_RESTART);
         SE_HIP(hipGetLastError());
