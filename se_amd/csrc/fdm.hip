@@ -1131,11 +1131,11 @@ struct FdmPlan {
 namespace {
 
 template <int MODE>
-int launch_apply_batch_m(int N, const ApplyMember* A, const int2* work, int nw, hipStream_t s, int gated) {
+int launch_apply_batch_m(int N, const ApplyMember* A, const int2* work, int nw, int grid, hipStream_t s, int gated) {
     switch (N) {
-#define SE_CASE(NN)                                                                                      \
-    case NN:                                                                                             \
-        hipLaunchKernelGGL((k_apply_lines_batch<NN, MODE>), dim3(nw), dim3(256), 0, s, A, work, nw, gated); \
+#define SE_CASE(NN)                                                                                        \
+    case NN:                                                                                               \
+        hipLaunchKernelGGL((k_apply_lines_batch<NN, MODE>), dim3(grid), dim3(256), 0, s, A, work, nw, gated); \
         break;
         SE_CASE(2) SE_CASE(3) SE_CASE(4) SE_CASE(5) SE_CASE(6) SE_CASE(7) SE_CASE(8)
 #undef SE_CASE
@@ -1144,10 +1144,13 @@ int launch_apply_batch_m(int N, const ApplyMember* A, const int2* work, int nw, 
     return SE_OK;
 }
 
-int launch_apply_batch(int N, int mode, const ApplyMember* A, const int2* work, int nw, hipStream_t s, int gated) {
-    if (mode == AP_DOT) return launch_apply_batch_m<AP_DOT>(N, A, work, nw, s, gated);
-    if (mode == AP_TRUE) return launch_apply_batch_m<AP_TRUE>(N, A, work, nw, s, gated);
-    return launch_apply_batch_m<AP_PLAIN>(N, A, work, nw, s, gated);
+// (grid <= nw workgroups; a multiple of 8 below nw, so that a workgroup's
+// items keep its XCD's place in the XCD-ordered list)
+int launch_apply_batch(int N, int mode, const ApplyMember* A, const int2* work, int nw, int grid, hipStream_t s,
+                       int gated) {
+    if (mode == AP_DOT) return launch_apply_batch_m<AP_DOT>(N, A, work, nw, grid, s, gated);
+    if (mode == AP_TRUE) return launch_apply_batch_m<AP_TRUE>(N, A, work, nw, grid, s, gated);
+    return launch_apply_batch_m<AP_PLAIN>(N, A, work, nw, grid, s, gated);
 }
 
 // Work lists XCD by XCD.  The dispatcher hands workgroup b to XCD b mod 8
@@ -1216,7 +1219,12 @@ int plan_apply(FdmPlan* pl, int set, bool from_x, hipStream_t s, bool gated = fa
         const int nw = L.n_apply[k];
         if (o.members.empty() || nw == 0) continue;
         const ApplyMember* A = from_x ? o.d_ax.as<ApplyMember>() : o.d_ap.as<ApplyMember>();
-        int rc = launch_apply_batch(o.N, mode, A, L.apply[k], nw, s, gated ? 1 : 0);
+        // a gated check runs for few members, if any: one workgroup per resident
+        // slot (four per CU) walks the list, instead of one per tile whose
+        // dispatch alone costs ~20 us at the ensemble's 14 k tiles
+        const int slots = std::max(8, 4 * pl->ctx->cu_count / 8 * 8);
+        const int grid = gated ? std::min(nw, slots) : nw;
+        int rc = launch_apply_batch(o.N, mode, A, L.apply[k], nw, grid, s, gated ? 1 : 0);
         if (rc) return rc;
     }
     // members with an SVV term: add it to the batched apply's result
