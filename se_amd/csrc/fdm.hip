@@ -18,10 +18,14 @@
 //
 // The solver around it is FP64 CG (x0 = 0, true-residual stop, constant null
 // space by Euclidean-mean projection, as the Jacobi-PCG of helm.hip) whose
-// preconditioner is the FDM in the solve's precision.  It runs members of a
-// batch in lockstep, one launch per phase over (member, item) work lists; a
-// single solve is a batch of one, so batched results are the single ones bit
-// for bit.  The exact FP64 operator apply is k_apply_lines_batch; every reduction is per item with consumer-side fixed-order sums.
+// preconditioner is the FDM in the solve's precision (or, in the FP64_*C
+// precisions, FP64 for the first application and FP32 / BF16 operands for
+// the corrections after it).  It runs members of a batch in lockstep, one
+// launch per phase over (member, item) work lists, one host pull of the
+// members' states per round; a single solve is a batch of one, so batched
+// results are the single ones bit for bit.  The exact FP64 operator apply is
+// k_apply_lines_batch; every reduction is per item with consumer-side
+// fixed-order sums.
 #include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 
@@ -1073,8 +1077,8 @@ struct FdmPlan {
     // Work lists of one member subset: (member, chunk) items of the vector
     // kernels, (member, tile) items of the transforms, per order the apply's
     // (local member, tile) items, and the member list, packed in one device
-    // buffer.  Three independent sets -- the running members' iteration, the
-    // true-residual checks, restarts and the end of a solve -- each keep the
+    // buffer.  Two independent sets -- the running members' iteration with
+    // its true-residual checks, and restarts and the end of a solve -- each keep the
     // lists of the last kListCache subsets they were asked for (successive
     // solves of a batch meet the same subsets: all members, then the same
     // stragglers), so a repeated subset costs neither host work nor an upload.
@@ -1106,8 +1110,8 @@ struct FdmPlan {
             if (h) (void)hipHostFree(h);
         }
     };
-    enum { SET_ITER = 0, SET_CHECK = 1, SET_OTHER = 2 };
-    ListSet sets[3];
+    enum { SET_ITER = 0, SET_OTHER = 1 };
+    ListSet sets[2];
     CgState* h_states = nullptr;  // pinned, h_cap entries
     int h_cap = 0;
     DevBuf d_states;
