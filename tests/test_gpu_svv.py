@@ -66,7 +66,7 @@ def test_svv_solve_matches_oracle(gpu_ctx, case):
 # FDM of the operator without its SVV term preconditions the exact operator
 # (kappa(A0^-1 A) 1.02-2.45 on these cases, measured with the oracle's CSRs)
 FDM_PRECS = [(se_amd.PREC_FP64, 30), (se_amd.PREC_FP32_IR, 30), (se_amd.PREC_BF16_IR, 150),
-             (se_amd.PREC_FP64_FP32C, 30)]
+             (se_amd.PREC_FP64_FP32C, 30), (se_amd.PREC_FP64_BF16C, 150)]
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "N%d-mN%d-per%d%d" % (c[1], c[4], *c[2]))
@@ -95,7 +95,7 @@ def test_svv_members_batch_with_fdm(gpu_ctx):
     ohs.append(O.Helm(om, 3.0, 0.02, 1))
     fs.append(np.random.default_rng(3).standard_normal(om.n))
     b = se_amd.Batch(gpu_ctx, hs)
-    for prec in (se_amd.PREC_FP64, se_amd.PREC_FP32_IR):
+    for prec in (se_amd.PREC_FP64, se_amd.PREC_FP32_IR, se_amd.PREC_FP64_BF16C):
         xs, infos = b.pcg(fs, precision=prec, rtol=1e-10, preconditioner=se_amd.PC_FDM)
         for h, oh, f, xb, ib in zip(hs, ohs, fs, xs, infos):
             assert ib["status"] == se_amd.SOLVE_CONVERGED, ib
