@@ -62,8 +62,8 @@ def prec_code(name):
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=3)
-    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="ensemble_512")
     p.add_argument("--precision", default=None, choices=["fp64", "fp64_fp32c", "fp64_bf16c", "fp32_ir", "bf16_ir"],
                    help="default: fp64_bf16c (FDM: first application on the FP64 matrix cores, corrections "
