@@ -217,3 +217,36 @@ def test_fdm_full_size(gpu_ctx):
     b.close()
     for o in (h, m, hs, ms):
         o.close()
+
+
+def test_fdm_unreachable_target_restarts_and_stops(gpu_ctx):
+    """rtol = 1e-17 is below what FP64 can certify: every check of the true
+    residual fails, so the CG restarts from it (the restart's transforms write
+    the new direction) until the iteration limit.  The solve must say so, still
+    return its best iterate with an honest residual, and batch as it runs
+    alone, bit for bit, in every precision."""
+    cases = [c for c in OP_CASES if c[0] in ("helm_40_p8_physical", "mixed_9x6_p7", "dirichlet_7x5_p5")]
+    mem = []
+    for lab, nEls, N, lo, hi, per, bc, cm, cl, detj in cases:
+        om = O.OracleMesh(nEls, N, lo, hi, per, bc)
+        m = se_amd.Mesh(gpu_ctx, nEls, N, lo, hi, per, bc)
+        mem.append((lab, O.Helm(om, cm, cl, detj), m, se_amd.Helm(m, cm, cl, detj)))
+    rng = np.random.default_rng(31)
+    fs = [rng.standard_normal(m.n) for _, _, m, _ in mem]
+    b = se_amd.Batch(gpu_ctx, [h for _, _, _, h in mem])
+    for prec, _ in SOLVE_PRECS:
+        xs, infos = b.pcg(fs, precision=prec, rtol=1e-17, maxit=12, preconditioner=se_amd.PC_FDM)
+        for (lab, oh, _, h), f, xb, ib in zip(mem, fs, xs, infos):
+            assert ib["status"] in (se_amd.SOLVE_MAXIT, se_amd.SOLVE_STAGNATED), (lab, prec, ib)
+            assert ib["iters"] <= 12, (lab, prec, ib)
+            res = np.linalg.norm(f - oh.matvec(xb)) / np.linalg.norm(f)
+            assert res <= 1e-11, (lab, prec, res)
+            # the reported residual is the final true one (to FP64 rounding)
+            assert ib["true_rel_residual"] <= 1e-11 and res <= 10 * ib["true_rel_residual"] + 1e-15, (lab, ib, res)
+            x1, i1 = h.pcg(f, rtol=1e-17, maxit=12, precision=prec, preconditioner=se_amd.PC_FDM)
+            assert i1["status"] == ib["status"] and i1["iters"] == ib["iters"], (lab, i1, ib)
+            assert np.array_equal(x1, xb), lab
+    b.close()
+    for _, _, m, h in mem:
+        h.close()
+        m.close()
