@@ -220,11 +220,12 @@ def test_fdm_full_size(gpu_ctx):
 
 
 def test_fdm_unreachable_target_restarts_and_stops(gpu_ctx):
-    """rtol = 1e-17 is below what FP64 can certify: every check of the true
-    residual fails, so the CG restarts from it (the restart's transforms write
-    the new direction) until the iteration limit.  The solve must say so, still
-    return its best iterate with an honest residual, and batch as it runs
-    alone, bit for bit, in every precision."""
+    """rtol = 1e-17 is below what FP64 can certify: the solve runs to its
+    iteration limit (whenever the recursive residual falls below the target,
+    the true-residual check refutes it and the CG restarts from the true
+    residual).  The solve must say so, still return its best iterate with an
+    honest residual, and batch as it runs alone, bit for bit, in every
+    precision."""
     cases = [c for c in OP_CASES if c[0] in ("helm_40_p8_physical", "mixed_9x6_p7", "dirichlet_7x5_p5")]
     mem = []
     for lab, nEls, N, lo, hi, per, bc, cm, cl, detj in cases:
