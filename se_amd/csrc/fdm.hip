@@ -464,7 +464,7 @@ __global__ __launch_bounds__(FB_NT) void k_fb_start(const FbMember* __restrict__
         double rr = 0.0;
         for (int i = i0 + threadIdx.x; i < i1; i += FB_NT) {
             const double bi = M.sgn * M.f[i] - mu;
-            M.b[i] = bi;
+            if (M.b != M.f) M.b[i] = bi;  // (else b is f, and bi = f[i])
             put_packed(M, i, bi);
             rr += bi * bi;
         }
@@ -1124,6 +1124,11 @@ struct FdmPlan {
     // k_fb_true); every other member's come from the applies' epilogues
     // (AP_DOT, AP_TRUE), a member's way the same alone or in any batch
     bool any_svv = false;
+    // b is the caller's f itself where b = sgn f - mean is f (sgn = 1, no null
+    // space): the start then stores no b, and a new f moves b with it (the
+    // first pass-4 tasks and the A x descriptors name b)
+    std::vector<char> b_alias;
+    std::vector<GemmTask> h_tasks;  // the host copy of d_tasks
     int desc_null = -1;
     double desc_rtol = 0.0;
 
@@ -1436,7 +1441,7 @@ int upload_apply_descriptors(FdmPlan* pl, hipStream_t s) {
             A.gate = M.st;
             if (M.fused) {  // r = b - A x, partials of sum t, sum t^2
                 A.out = op->r.as<double>();
-                A.b = op->b.as<double>();
+                A.b = M.b;
                 A.part = M.pt;
             }
             ax.push_back(A);
@@ -1475,10 +1480,15 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
     const bool reuse = pl->desc_ok && pl->prec == prec && pl->cprec == cprec && pl->desc_null == null &&
                        pl->desc_rtol == o.rtol;
     if (reuse) {
-        bool changed = false, moved = false;
+        bool changed = false, moved = false, bmoved = false;
         for (int i = 0; i < count; i++) {
             FbMember& M = pl->h_members[i];
             if (M.f != d_f[i] || M.xo != d_x[i]) {
+                if (pl->b_alias[i] && M.f != d_f[i]) {
+                    M.b = const_cast<double*>(d_f[i]);  // read only: b is f
+                    pl->h_tasks[4 * (size_t)i + 3].b = M.b;
+                    bmoved = true;
+                }
                 M.f = d_f[i];
                 M.xo = d_x[i];
                 double* x = iterate_of(pl, i, d_f, d_x);
@@ -1490,7 +1500,10 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
         if (changed && count > 0)
             SE_HIP(hipMemcpyAsync(pl->d_members.ptr, pl->h_members.data(), sizeof(FbMember) * count,
                                   hipMemcpyHostToDevice, s));
-        if (moved) {  // the A x descriptors name the iterates
+        if (bmoved)
+            SE_HIP(hipMemcpyAsync(pl->d_tasks.ptr, pl->h_tasks.data(), sizeof(GemmTask) * 4 * count,
+                                  hipMemcpyHostToDevice, s));
+        if (moved || bmoved) {  // the A x descriptors name the iterates and b
             int e = upload_apply_descriptors(pl, s);
             if (e) return e;
         }
@@ -1507,12 +1520,14 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
         }
         pl->any_svv = false;
         for (se_op* op : pl->ops) pl->any_svv = pl->any_svv || op->svv_mN != 0;
+        pl->b_alias.assign(count, 0);
         for (int i = 0; i < count; i++) {
             se_op* op = pl->ops[i];
             FdmOp* F = op->fdm;
             FbMember& M = pl->h_members[i];
             M.f = d_f[i];
-            M.b = op->b.as<double>();
+            pl->b_alias[i] = op->sign == 1 && !null;
+            M.b = pl->b_alias[i] ? const_cast<double*>(d_f[i]) : op->b.as<double>();  // (b is read only when f)
             M.x = iterate_of(pl, i, d_f, d_x);
             M.r = op->r.as<double>();
             M.p = op->p.as<double>();
@@ -1569,6 +1584,7 @@ int fdm_plan_solve(FdmPlan* pl, const double* const* d_f, double* const* d_x, co
             SE_HIP(hipMemcpyAsync(pl->d_members.ptr, pl->h_members.data(), sizeof(FbMember) * count, hipMemcpyHostToDevice,
                                   s));
             SE_HIP(hipMemcpyAsync(pl->d_tasks.ptr, tasks.data(), sizeof(GemmTask) * 4 * count, hipMemcpyHostToDevice, s));
+            pl->h_tasks = tasks;
             if (cprec != prec)
                 SE_HIP(hipMemcpyAsync(pl->d_tasks_c.ptr, tasks_c.data(), sizeof(GemmTask) * 4 * count,
                                       hipMemcpyHostToDevice, s));
