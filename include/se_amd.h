@@ -7,8 +7,8 @@
  * pointers, sizes, int status codes.  A context owns one HIP stream and all
  * device memory it allocates; a context is used by one host thread at a time.
  * Drive one device from one host thread: FDM batch solves on two contexts of
- * one device from two threads at once ran slower than one batch, and in one
- * of two trials some members reported non-convergence (cause not found).
+ * one device from two threads at once ran slower than one batch, and in
+ * three of four trials some large members diverged (cause not found).
  *
  * Pointer conventions
  *   *_dev entry points take device pointers that are already resident in HBM
