@@ -189,7 +189,9 @@ typedef struct {
     int status;               /* SE_SOLVE_* */
     int iters;                /* CG iterations (inner iterations for IR) */
     int outer_iters;          /* IR outer steps (1 for FP64) */
-    double true_rel_residual; /* FP64 ||b - A x|| / ||b|| at exit */
+    double true_rel_residual; /* FP64 ||b - A x|| / ||b|| at exit (an FDM solve that
+                                 converges at its first iteration from x = 0: its
+                                 update's b - alpha A p, for x = alpha p) */
     double seconds;           /* device time of the solve */
 } se_solve_info;
 
